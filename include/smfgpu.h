@@ -1,0 +1,203 @@
+/*
+ * smfgpu: an MI355X (gfx950) numeric phase for the smf sparse symmetric
+ * multifrontal solver (danlil240/opt_solver, smf 0.2.3).
+ *
+ * The host keeps smf's symbolic analysis; this library takes its result
+ * (the AnalysisKeep fields a numeric phase reads) and runs the numeric
+ * phase on one GPU: value mapping, extend-add assembly, dense Cholesky /
+ * threshold LDL^T of every front, the Schur-complement update on MFMA,
+ * forward / diagonal / backward solves and FP64 iterative refinement.
+ *
+ * Plain C ABI: plain pointers and sizes, 0-based indices, column-major dense
+ * arrays, no exceptions cross it. Every call returns SMFGPU_OK (0) or a
+ * negative error code (smfgpu_error_string gives the message of the last
+ * failure on the calling thread).
+ *
+ * Entry points and the reference interface each one stands behind
+ * (reference paths relative to the reference repository root):
+ *
+ *   smfgpu_plan_create      replaces the per-analysis set-up of
+ *                           multifrontal_factor_serial
+ *                           (solver/src/multifrontal_serial.cpp:1105-1320),
+ *                           reading AnalysisKeep (solver/include/smf/analysis.hpp:25-144)
+ *   smfgpu_factor           replaces factor_posdef / factor_indef's numeric
+ *                           work: map_values_to_permuted
+ *                           (solver/src/workspace.cpp:129-271), eff_small
+ *                           (solver/src/factor_indef.cpp:633-636) and
+ *                           multifrontal_factor_serial
+ *                           (solver/include/smf/multifrontal_serial.hpp:98-105)
+ *   smfgpu_export_factor    the FactorKeep layout
+ *                           (solver/include/smf/factor_posdef.hpp:16-220)
+ *   smfgpu_solve            solve_core (solver/src/solve.cpp:858-1075)
+ *   smfgpu_refine           refine / compute_residual_and_backward_error
+ *                           (solver/src/refinement.cpp:39-130, 643-724)
+ */
+#pragma once
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(__GNUC__) || defined(__clang__)
+#define SMFGPU_API __attribute__((visibility("default")))
+#else
+#define SMFGPU_API
+#endif
+
+/* Return codes. */
+#define SMFGPU_OK 0
+#define SMFGPU_ERR_INVALID_ARG (-1)
+#define SMFGPU_ERR_ALLOC (-2)   /* device or host allocation failed (smf ErrorCode::AllocationError) */
+#define SMFGPU_ERR_VALUE (-3)   /* non-finite input value (smf ErrorCode::IllegalValue) */
+#define SMFGPU_ERR_DEVICE (-4)  /* HIP runtime error */
+#define SMFGPU_ERR_STATE (-5)   /* call not valid for this handle (e.g. solve of a failed factor) */
+#define SMFGPU_ERR_UNSUPPORTED (-6)
+
+/* Factor status, the values of smf::FactorStatus (solver/include/smf/types.hpp:230-238). */
+#define SMFGPU_STATUS_SUCCESS 0
+#define SMFGPU_STATUS_NOT_POSDEF 1
+#define SMFGPU_STATUS_SINGULAR 2
+#define SMFGPU_STATUS_MAX_PIVOT_DELAYS 3
+#define SMFGPU_STATUS_INERTIA_EXCEEDED 4
+
+/* Precision of the Schur-complement updates of large fronts. Pivot tests,
+ * D, inertia, residuals and the refined solution are always FP64. */
+#define SMFGPU_PREC_FP64 0
+#define SMFGPU_PREC_FP32 1 /* FP32 MFMA, FP32 accumulation */
+#define SMFGPU_PREC_BF16 2 /* BF16 MFMA, FP32 accumulation */
+#define SMFGPU_PREC_AUTO 3 /* per-supernode choice by the learned selector */
+
+/* Memory kind of value / right-hand-side pointers. */
+#define SMFGPU_MEM_HOST 0
+#define SMFGPU_MEM_DEVICE 1
+
+typedef struct smfgpu_plan_s *smfgpu_plan_t;
+typedef struct smfgpu_factor_s *smfgpu_factor_t;
+typedef struct smfgpu_analysis_s *smfgpu_analysis_t;
+
+/* The AnalysisKeep fields the numeric phase reads. Supernodes are numbered
+ * in postorder (children before parents); children of s are
+ * child_idx[child_ptr[s] .. child_ptr[s+1]) in descending order;
+ * front_row_idx[front_row_ptr[s] ..) are the rows of front s, sorted, the
+ * first sn_end[s] - sn_start[s] of them its pivot columns. pcol_ptr /
+ * prow_idx is the permuted lower pattern the values are scattered into,
+ * orig_to_slot maps each entry of the analysed matrix to its slot there
+ * (-1: dropped; several entries may share a slot and are summed). */
+typedef struct {
+    int32_t n;
+    int32_t nsup;
+    const int32_t *sn_start;
+    const int32_t *sn_end;
+    const int32_t *sn_parent;
+    const int32_t *child_ptr;
+    const int32_t *child_idx;
+    const int32_t *front_row_ptr;
+    const int32_t *front_row_idx;
+    const int32_t *pcol_ptr;
+    const int32_t *prow_idx;
+    int64_t n_orig;
+    const int32_t *orig_to_slot;
+    const int32_t *perm; /* perm[new] = old */
+} smfgpu_analysis_desc;
+
+/* Numeric options (the smf::Control fields the numeric phase reads,
+ * solver/include/smf/control.hpp:331-440). */
+typedef struct {
+    int32_t posdef;          /* 1: Cholesky (MatrixType 3), 0: LDL^T (MatrixType 4) */
+    double pivot_u;          /* threshold u (default 0.01) */
+    double small_pivot;      /* zero-pivot threshold (default 1e-20) */
+    int32_t panel;           /* indefinite panel width nb (default 16) */
+    int32_t precision;       /* SMFGPU_PREC_* for the Schur updates */
+    int32_t mixed_min_front; /* fronts with fewer rows stay FP64 (default 1024) */
+    int32_t max_negative_pivots; /* < 0: off */
+    int32_t value_mem;       /* SMFGPU_MEM_* of the values pointer */
+    int32_t reserved[8];
+} smfgpu_factor_options;
+
+/* Factor statistics (the smf::Info fields of a factorization,
+ * solver/include/smf/info.hpp:24-221). */
+typedef struct {
+    int32_t status;          /* SMFGPU_STATUS_* */
+    int32_t num_positive;
+    int32_t num_negative;
+    int32_t num_zero;
+    int32_t delayed_pivots;
+    int32_t two_by_two_pivots;
+    int32_t max_front;
+    int32_t fronts_reduced;  /* fronts whose Schur update ran below FP64 */
+    double actual_flops;     /* kernel flops (reference count) */
+    double front_work;       /* sum num_elig * f^2 */
+    double max_abs;          /* max |a_ij| of the input */
+    double factor_seconds;   /* device time of the numeric factorization */
+    int64_t factor_entries;  /* sum f * p */
+    double reserved[4];
+} smfgpu_factor_info;
+
+SMFGPU_API const char *smfgpu_error_string(void);
+SMFGPU_API int smfgpu_default_options(smfgpu_factor_options *opt);
+
+/* ---- symbolic analysis on the host (stands in for smf's Solver::analyse,
+ * solver/src/symbolic_analysis.cpp:434, when no smf analysis is at hand).
+ * perm (new -> old) may be null for the identity; nemin: relaxed
+ * amalgamation width (default 32 when <= 0). */
+SMFGPU_API int smfgpu_analyse(int32_t n, const int32_t *col_ptr, const int32_t *row_idx,
+                              const int32_t *perm, int32_t nemin, smfgpu_analysis_t *out);
+SMFGPU_API int smfgpu_analysis_describe(smfgpu_analysis_t an, smfgpu_analysis_desc *desc);
+/* out[0..5]: nsup, sum of front rows, factor entries (sum f p), max front,
+ * delay-free kernel flops, depth (levels). */
+SMFGPU_API int smfgpu_analysis_stats(smfgpu_analysis_t an, double *out);
+SMFGPU_API void smfgpu_analysis_free(smfgpu_analysis_t *an);
+
+/* ---- device plan (one per analysis; owns the device buffers) ---- */
+SMFGPU_API int smfgpu_plan_create(const smfgpu_analysis_desc *desc, int32_t posdef,
+                                  smfgpu_plan_t *out);
+SMFGPU_API void smfgpu_plan_free(smfgpu_plan_t *plan);
+/* out[0..7]: device bytes held, levels, kernel launches per factorization,
+ * delay-free flops, factor entries, contribution-block entries, assembly
+ * bytes (algorithmic), solve bytes per RHS (algorithmic). */
+SMFGPU_API int smfgpu_plan_stats(smfgpu_plan_t plan, double *out);
+
+/* ---- numeric factorization. values: one per entry of the analysed
+ * matrix (host or device memory per opt->value_mem). The returned factor
+ * stays on the device; a plan has one live factor (a new factorization
+ * invalidates the previous factor handle's contents). */
+SMFGPU_API int smfgpu_factor(smfgpu_plan_t plan, const double *values, int64_t nvalues,
+                             const smfgpu_factor_options *opt, smfgpu_factor_t *out,
+                             smfgpu_factor_info *info);
+SMFGPU_API void smfgpu_factor_free(smfgpu_factor_t *f);
+
+/* FactorKeep layout on the host: steps (4 int64 per supernode step:
+ * row_offset, factor_offset, p, f), factor values (sum f p doubles),
+ * row indices (sum f), pivot types (sum p int8; SPD: all 1). sizes[0..3]
+ * receive the four lengths when the pointers are null. */
+SMFGPU_API int smfgpu_export_factor(smfgpu_factor_t f, int64_t *sizes, int64_t *steps,
+                                    double *factor_values, int32_t *row_indices,
+                                    int8_t *pivot_types);
+
+/* ---- solves: b is n x nrhs, leading dimension ldb >= n, in place,
+ * original (unpermuted) ordering. job: 0 full. mem: SMFGPU_MEM_*. */
+SMFGPU_API int smfgpu_solve(smfgpu_factor_t f, double *b, int32_t nrhs, int32_t ldb,
+                            int32_t job, int32_t mem);
+
+/* ---- FP64 iterative refinement against the analysed matrix (values as
+ * given to smfgpu_factor, same memory kind): x (n x nrhs, device or host
+ * per mem) is refined in place until the scaled backward error
+ * eta_inf <= tol or max_iters steps or a stall. out[0] eta_inf,
+ * out[1] omega, out[2] steps taken. */
+SMFGPU_API int smfgpu_refine(smfgpu_factor_t f, const double *values, const double *b,
+                             double *x, int32_t nrhs, int32_t max_iters, double tol,
+                             int32_t mem, double *out);
+
+/* Device time (ms) of the last factorization's dominant kernel family:
+ * out[0] Schur-update kernel total ms, out[1] launches, out[2] flops in it,
+ * out[3] assembly ms, out[4] assembly bytes, out[5] solve ms (last solve),
+ * out[6] solve bytes (last solve). Measured with HIP events on the plan's
+ * stream when timing is enabled. */
+SMFGPU_API int smfgpu_set_timing(smfgpu_plan_t plan, int32_t enabled);
+SMFGPU_API int smfgpu_kernel_times(smfgpu_plan_t plan, double *out);
+
+#ifdef __cplusplus
+}
+#endif

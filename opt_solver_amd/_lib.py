@@ -1,0 +1,98 @@
+"""ctypes binding of libsmfgpu.so (include/smfgpu.h).
+
+The library is built in-tree (``opt_solver_amd/libsmfgpu.so``, see
+``__graft_entry__.build``). Loading fails loudly when it is missing: there is
+no CPU fallback for the numeric phase."""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsmfgpu.so")
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int32
+_L = ctypes.c_int64
+_D = ctypes.c_double
+
+SMFGPU_OK = 0
+PREC_FP64, PREC_FP32, PREC_BF16, PREC_AUTO = 0, 1, 2, 3
+MEM_HOST, MEM_DEVICE = 0, 1
+
+STATUS = {0: "Success", 1: "NotPositiveDefinite", 2: "Singular", 3: "MaxPivotDelays", 4: "InertiaExceeded"}
+
+
+class AnalysisDesc(ctypes.Structure):
+    _fields_ = [
+        ("n", _I), ("nsup", _I),
+        ("sn_start", _P), ("sn_end", _P), ("sn_parent", _P),
+        ("child_ptr", _P), ("child_idx", _P),
+        ("front_row_ptr", _P), ("front_row_idx", _P),
+        ("pcol_ptr", _P), ("prow_idx", _P),
+        ("n_orig", _L), ("orig_to_slot", _P), ("perm", _P),
+    ]
+
+
+class FactorOptions(ctypes.Structure):
+    _fields_ = [
+        ("posdef", _I), ("pivot_u", _D), ("small_pivot", _D), ("panel", _I), ("precision", _I),
+        ("mixed_min_front", _I), ("max_negative_pivots", _I), ("value_mem", _I),
+        ("reserved", _I * 8),
+    ]
+
+
+class FactorInfo(ctypes.Structure):
+    _fields_ = [
+        ("status", _I), ("num_positive", _I), ("num_negative", _I), ("num_zero", _I),
+        ("delayed_pivots", _I), ("two_by_two_pivots", _I), ("max_front", _I), ("fronts_reduced", _I),
+        ("actual_flops", _D), ("front_work", _D), ("max_abs", _D), ("factor_seconds", _D),
+        ("factor_entries", _L), ("reserved", _D * 4),
+    ]
+
+
+EXPORTED = [
+    "smfgpu_error_string", "smfgpu_default_options", "smfgpu_analyse", "smfgpu_analysis_describe",
+    "smfgpu_analysis_stats", "smfgpu_analysis_free", "smfgpu_plan_create", "smfgpu_plan_free",
+    "smfgpu_plan_stats", "smfgpu_factor", "smfgpu_factor_free", "smfgpu_export_factor", "smfgpu_solve",
+    "smfgpu_refine", "smfgpu_set_timing", "smfgpu_kernel_times",
+]
+
+_lib = None
+
+
+class SmfGpuError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise SmfGpuError(f"{LIB_PATH} is not built: run __graft_entry__.build() (no CPU fallback exists)")
+    L = ctypes.CDLL(LIB_PATH)
+    L.smfgpu_error_string.restype = ctypes.c_char_p
+    L.smfgpu_default_options.argtypes = [_P]
+    L.smfgpu_analyse.argtypes = [_I, _P, _P, _P, _I, _P]
+    L.smfgpu_analysis_describe.argtypes = [_P, _P]
+    L.smfgpu_analysis_stats.argtypes = [_P, _P]
+    L.smfgpu_analysis_free.argtypes = [_P]
+    L.smfgpu_plan_create.argtypes = [_P, _I, _P]
+    L.smfgpu_plan_free.argtypes = [_P]
+    L.smfgpu_plan_stats.argtypes = [_P, _P]
+    L.smfgpu_factor.argtypes = [_P, _P, _L, _P, _P, _P]
+    L.smfgpu_factor_free.argtypes = [_P]
+    L.smfgpu_export_factor.argtypes = [_P, _P, _P, _P, _P, _P]
+    L.smfgpu_solve.argtypes = [_P, _P, _I, _I, _I, _I]
+    L.smfgpu_refine.argtypes = [_P, _P, _P, _P, _I, _I, _D, _I, _P]
+    L.smfgpu_set_timing.argtypes = [_P, _I]
+    L.smfgpu_kernel_times.argtypes = [_P, _P]
+    _lib = L
+    return L
+
+
+def check(rc: int, what: str):
+    if rc != SMFGPU_OK:
+        msg = lib().smfgpu_error_string().decode(errors="replace")
+        raise SmfGpuError(f"{what}: error {rc}: {msg}")

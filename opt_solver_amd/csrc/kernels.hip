@@ -1,0 +1,616 @@
+// HIP kernels of the smfgpu numeric phase (gfx950 / CDNA4).
+//
+// Layout of a front (see FrontDev): pivot columns in the factor array L,
+// trailing contribution block in CB; col_base() gives, for local column c,
+// the pointer P with element (r, c) at P[r].
+#include "smfgpu_internal.hpp"
+
+#include <cfloat>
+
+namespace smfgpu {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+constexpr int kThreads = 256;
+
+__device__ __forceinline__ double *col_base(const DevPtrs &d, const FrontDev &F, int c) {
+    if (c < F.p) return d.L + F.lofs + static_cast<int64_t>(c) * F.f;
+    return d.CB + F.cbofs + static_cast<int64_t>(c - F.p) * F.q - F.p;
+}
+
+// ---------------------------------------------------------------------------
+// A1: permuted values. av[slot] = sum of the input entries mapped to slot,
+// in input order (map_values_to_permuted, workspace.cpp:129-271), plus
+// max |a| and a non-finite flag over the whole input (as a bit pattern max:
+// |v| >= 0 orders like its bits, and an Inf/NaN compares above every
+// finite value).
+__global__ void k_map_values(const double *__restrict__ vals, const int64_t *__restrict__ slot_ptr,
+                             const int64_t *__restrict__ slot_src, int64_t nslots,
+                             double *__restrict__ av) {
+    const int64_t s = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (s >= nslots) return;
+    double acc = 0.0;
+    for (int64_t k = slot_ptr[s]; k < slot_ptr[s + 1]; ++k) acc += vals[slot_src[k]];
+    av[s] = acc;
+}
+
+__global__ void k_absmax(const double *__restrict__ v, int64_t n, unsigned long long *out) {
+    __shared__ unsigned long long red[kThreads];
+    unsigned long long m = 0;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(fabs(v[i])));
+        m = b > m ? b : m;
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w && red[threadIdx.x + w] > red[threadIdx.x]) red[threadIdx.x] = red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicMax(out, red[0]);
+}
+
+__global__ void k_fill_u64(unsigned long long *p, int64_t n, unsigned long long v) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+void launch_map_values(hipStream_t st, const double *vals, int64_t nvals, const int64_t *slot_ptr,
+                       const int64_t *slot_src, int64_t nslots, double *av, double *redbuf) {
+    auto *rb = reinterpret_cast<unsigned long long *>(redbuf);
+    k_fill_u64<<<1, 64, 0, st>>>(rb, 1, 0ull);
+    if (nvals > 0) {
+        const int64_t g = std::min<int64_t>((nvals + kThreads - 1) / kThreads, 2048);
+        k_absmax<<<dim3(static_cast<unsigned>(g)), kThreads, 0, st>>>(vals, nvals, rb);
+    }
+    if (nslots > 0)
+        k_map_values<<<dim3(static_cast<unsigned>((nslots + kThreads - 1) / kThreads)), kThreads, 0, st>>>(
+            vals, slot_ptr, slot_src, nslots, av);
+}
+
+// ---------------------------------------------------------------------------
+// A3: assembly. One workgroup per front column; the column is built in LDS
+// in row chunks: zero, original entries (CSC order), then each child's CB
+// column in child order (the reference's summation order per entry,
+// multifrontal_serial.cpp:730-789), and written out once, coalesced. Rows
+// above the diagonal of a pivot column are written as zeros (the stored
+// factor's strict upper part).
+constexpr int kAsmChunk = 2048;
+
+__device__ __forceinline__ int lower_bound_i32(const int32_t *a, int lo, int hi, int32_t key) {
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(kThreads) void k_assemble(DevPtrs d, const AsmTask *__restrict__ tasks,
+                                                       const ChildCol *__restrict__ cc) {
+    __shared__ double buf[kAsmChunk];
+    const AsmTask t = tasks[blockIdx.x];
+    const FrontDev F = d.fronts[t.s];
+    const int c = t.c;
+    const int f = F.f;
+    double *colp = col_base(d, F, c);
+    const int tid = threadIdx.x;
+    if (c < F.p)
+        for (int r = tid; r < c; r += kThreads) colp[r] = 0.0;
+    for (int r0 = c; r0 < f; r0 += kAsmChunk) {
+        const int r1 = min(f, r0 + kAsmChunk);
+        for (int r = r0 + tid; r < r1; r += kThreads) buf[r - r0] = 0.0;
+        __syncthreads();
+        if (c < F.p) {
+            const int j = F.col0 + c;
+            const int k0 = d.pcol_ptr[j], k1 = d.pcol_ptr[j + 1];
+            for (int k = k0 + tid; k < k1; k += kThreads) {
+                const int lr = d.amap[k];
+                if (lr >= r0 && lr < r1) buf[lr - r0] += d.av[k];
+            }
+            __syncthreads();
+        }
+        for (int e = t.cc0; e < t.cc1; ++e) {
+            const ChildCol ch = cc[e];
+            const FrontDev C = d.fronts[ch.ch];
+            const int32_t *cm = d.cmap + d.cmofs[ch.ch];
+            const double *cbcol = d.CB + C.cbofs + static_cast<int64_t>(ch.jj) * C.q;
+            const int lo = lower_bound_i32(cm, ch.jj, C.q, r0);
+            const int hi = lower_bound_i32(cm, lo, C.q, r1);
+            for (int ii = lo + tid; ii < hi; ii += kThreads) buf[cm[ii] - r0] += cbcol[ii];
+            __syncthreads();
+        }
+        for (int r = r0 + tid; r < r1; r += kThreads) colp[r] = buf[r - r0];
+        __syncthreads();
+    }
+}
+
+void launch_assemble(hipStream_t st, const DevPtrs &d, const AsmTask *tasks, int64_t ntasks,
+                     const ChildCol *cc) {
+    if (ntasks <= 0) return;
+    k_assemble<<<dim3(static_cast<unsigned>(ntasks)), kThreads, 0, st>>>(d, tasks, cc);
+}
+
+// ---------------------------------------------------------------------------
+// A11 (SPD): Cholesky of a kb x kb diagonal block (kb <= 64) in LDS. Writes
+// L_kk (lower) and M = inv(L_kk)^T's strict upper part into the block's
+// strict upper triangle (M's diagonal is 1 / L_kk's), which the panel TRSM
+// and the solves read. A pivot that is not > 0 or not finite records the
+// front in flag[0] (smallest supernode index + 1), as dpotrf's info
+// (ldlt_kernel.cpp:1246-1289); the factorization then reports
+// NotPositiveDefinite.
+constexpr int kNb = 64;
+
+__global__ __launch_bounds__(kThreads) void k_potrf(DevPtrs d, const PotrfItem *__restrict__ items) {
+    __shared__ double A[kNb][kNb + 1];
+    __shared__ double Inv[kNb][kNb + 1];
+    const PotrfItem it = items[blockIdx.x];
+    const FrontDev F = d.fronts[it.s];
+    const int f = F.f, k = it.k, kb = it.kb;
+    double *base = d.L + F.lofs;
+    const int tid = threadIdx.x;
+    for (int idx = tid; idx < kb * kb; idx += kThreads) {
+        const int c = idx / kb, r = idx % kb;
+        A[r][c] = (r >= c) ? base[static_cast<int64_t>(k + c) * f + k + r] : 0.0;
+        Inv[r][c] = 0.0;
+    }
+    __syncthreads();
+    for (int j = 0; j < kb; ++j) {
+        if (tid == 0) {
+            const double dj = A[j][j];
+            if (!(dj > 0.0 && dj <= DBL_MAX)) atomicMin(d.flag, it.s + 1);
+            A[j][j] = sqrt(dj);
+        }
+        __syncthreads();
+        const double djj = A[j][j];
+        for (int r = j + 1 + tid; r < kb; r += kThreads) A[r][j] /= djj;
+        __syncthreads();
+        const int n2 = kb - j - 1;
+        for (int idx = tid; idx < n2 * n2; idx += kThreads) {
+            const int c = j + 1 + idx / n2, r = j + 1 + idx % n2;
+            if (r >= c) A[r][c] -= A[r][j] * A[c][j];
+        }
+        __syncthreads();
+    }
+    // inverse of L_kk, one column per thread
+    if (tid < kb) {
+        const int c = tid;
+        Inv[c][c] = 1.0 / A[c][c];
+        for (int i = c + 1; i < kb; ++i) {
+            double s = 0.0;
+            for (int t = c; t < i; ++t) s += A[i][t] * Inv[t][c];
+            Inv[i][c] = -s / A[i][i];
+        }
+    }
+    __syncthreads();
+    for (int idx = tid; idx < kb * kb; idx += kThreads) {
+        const int c = idx / kb, r = idx % kb;
+        // lower: L_kk; strict upper (r < c): M[r][c] = Inv[c][r]
+        base[static_cast<int64_t>(k + c) * f + k + r] = (r >= c) ? A[r][c] : Inv[c][r];
+    }
+}
+
+void launch_potrf(hipStream_t st, const DevPtrs &d, const PotrfItem *it, int64_t n) {
+    if (n <= 0) return;
+    k_potrf<<<dim3(static_cast<unsigned>(n)), kThreads, 0, st>>>(d, it);
+}
+
+// Panel TRSM: rows [r0, r0 + 64) of columns [k, k + kb): X = A M, with
+// M = inv(L_kk)^T read from the block's strict upper part and diagonal.
+__global__ __launch_bounds__(kThreads) void k_trsm(DevPtrs d, const TrsmItem *__restrict__ items) {
+    __shared__ double At[kNb][kNb + 1];
+    __shared__ double M[kNb][kNb + 1];
+    const TrsmItem it = items[blockIdx.x];
+    const FrontDev F = d.fronts[it.s];
+    const int f = F.f, k = it.k, kb = it.kb, r0 = it.r0;
+    double *base = d.L + F.lofs;
+    const int tid = threadIdx.x;
+    const int nr = min(kNb, f - r0);
+    for (int idx = tid; idx < kb * kNb; idx += kThreads) {
+        const int t = idx / kNb, r = idx % kNb;
+        At[r][t] = (r < nr) ? base[static_cast<int64_t>(k + t) * f + r0 + r] : 0.0;
+    }
+    for (int idx = tid; idx < kb * kb; idx += kThreads) {
+        const int c = idx / kb, t = idx % kb;
+        double v = 0.0;
+        if (t < c) v = base[static_cast<int64_t>(k + c) * f + k + t];
+        else if (t == c) v = 1.0 / base[static_cast<int64_t>(k + c) * f + k + c];
+        M[t][c] = v;
+    }
+    __syncthreads();
+    const int r = tid % kNb;
+    const int cg = tid / kNb;
+    if (r < nr) {
+        for (int c = cg; c < kb; c += kThreads / kNb) {
+            double s = 0.0;
+            for (int t = 0; t <= c; ++t) s += At[r][t] * M[t][c];
+            base[static_cast<int64_t>(k + c) * f + r0 + r] = s;
+        }
+    }
+}
+
+void launch_trsm(hipStream_t st, const DevPtrs &d, const TrsmItem *it, int64_t n) {
+    if (n <= 0) return;
+    k_trsm<<<dim3(static_cast<unsigned>(n)), kThreads, 0, st>>>(d, it);
+}
+
+// ---------------------------------------------------------------------------
+// A10/A11: Schur-complement update on MFMA. Tile 128 x 128 of
+// C[i, j] -= sum_{t in [t0, t1)} L[i, t] L[j, t], i >= j, four waves of
+// 64 x 64, each 4 x 4 MFMA 16x16x4 blocks. The MFMA computes the
+// transposed block (A operand = the j rows, B operand = the i rows) so that
+// the 16 lanes of a row group hold 16 consecutive rows of one column on the
+// store. prec 0: FP64 MFMA; 1: operands rounded to FP32, FP32 MFMA with FP32
+// accumulation over the panel, subtracted from the FP64 Schur complement.
+constexpr int kT = 128;
+constexpr int kKc = 16;
+
+__global__ __launch_bounds__(kThreads) void k_syrk(DevPtrs d, const SyrkItem *__restrict__ items) {
+    __shared__ double As[kKc][kT];
+    __shared__ double Bs[kKc][kT];
+    const SyrkItem it = items[blockIdx.x];
+    const FrontDev F = d.fronts[it.s];
+    const int f = F.f;
+    const int i0 = it.i0, j0 = it.j0;
+    const int j1 = min(it.j1, f);
+    const double *Lb = d.L + F.lofs;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int w = tid >> 6;
+    const int wr = w >> 1, wc = w & 1;
+    // waves whose sub-tile lies wholly above the diagonal do no MFMA work
+    const bool active = (i0 + wr * 64 + 63) >= (j0 + wc * 64);
+    v4d acc[4][4];
+    v4f accf[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
+            accf[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
+        }
+    const int prec = d.prec != nullptr ? d.prec[it.s] : 0;
+    for (int kc = it.t0; kc < it.t1; kc += kKc) {
+#pragma unroll
+        for (int e = 0; e < (kKc * kT) / kThreads; ++e) {
+            const int idx = tid + kThreads * e;
+            const int t = idx / kT, r = idx % kT;
+            const int tt = kc + t;
+            const bool tin = tt < it.t1;
+            As[t][r] = (tin && i0 + r < f) ? Lb[static_cast<int64_t>(tt) * f + i0 + r] : 0.0;
+            Bs[t][r] = (tin && j0 + r < j1) ? Lb[static_cast<int64_t>(tt) * f + j0 + r] : 0.0;
+        }
+        __syncthreads();
+        if (active) {
+            if (prec == 0) {
+#pragma unroll
+                for (int kk = 0; kk < kKc; kk += 4) {
+                    double a[4], b[4];
+                    const int kr = kk + (lane >> 4);
+#pragma unroll
+                    for (int m = 0; m < 4; ++m) a[m] = Bs[kr][wc * 64 + m * 16 + (lane & 15)];
+#pragma unroll
+                    for (int nn = 0; nn < 4; ++nn) b[nn] = As[kr][wr * 64 + nn * 16 + (lane & 15)];
+#pragma unroll
+                    for (int m = 0; m < 4; ++m)
+#pragma unroll
+                        for (int nn = 0; nn < 4; ++nn)
+                            acc[m][nn] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[m], b[nn], acc[m][nn], 0, 0, 0);
+                }
+            } else {
+#pragma unroll
+                for (int kk = 0; kk < kKc; kk += 4) {
+                    float a[4], b[4];
+                    const int kr = kk + (lane >> 4);
+#pragma unroll
+                    for (int m = 0; m < 4; ++m) a[m] = static_cast<float>(Bs[kr][wc * 64 + m * 16 + (lane & 15)]);
+#pragma unroll
+                    for (int nn = 0; nn < 4; ++nn) b[nn] = static_cast<float>(As[kr][wr * 64 + nn * 16 + (lane & 15)]);
+#pragma unroll
+                    for (int m = 0; m < 4; ++m)
+#pragma unroll
+                        for (int nn = 0; nn < 4; ++nn)
+                            accf[m][nn] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[m], b[nn], accf[m][nn], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (!active) return;
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int nn = 0; nn < 4; ++nn)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                // D row of value v in lane: FP64 16x16x4 holds rows 4 v + lane / 16,
+                // FP32 16x16x4 rows 4 (lane / 16) + v (both: column lane % 16).
+                const int mrow = prec == 0 ? 4 * v + (lane >> 4) : 4 * (lane >> 4) + v;
+                const int j = j0 + wc * 64 + m * 16 + mrow;
+                const int i = i0 + wr * 64 + nn * 16 + (lane & 15);
+                if (i < f && j < j1 && i >= j) {
+                    const double u = prec == 0 ? acc[m][nn][v] : static_cast<double>(accf[m][nn][v]);
+                    col_base(d, F, j)[i] -= u;
+                }
+            }
+}
+
+void launch_syrk(hipStream_t st, const DevPtrs &d, const SyrkItem *it, int64_t n) {
+    if (n <= 0) return;
+    k_syrk<<<dim3(static_cast<unsigned>(n)), kThreads, 0, st>>>(d, it);
+}
+
+void launch_zero_upper(hipStream_t, const DevPtrs &, const PotrfItem *, int64_t) {}
+
+// ---------------------------------------------------------------------------
+// A15: solves. y is the permuted right-hand-side block (n x nrhs, ld n).
+constexpr int kMaxRhs = 16;
+
+__global__ void k_gather(const int32_t *__restrict__ perm, const double *__restrict__ b, int64_t ldb,
+                         double *__restrict__ y, int32_t n, int32_t nrhs) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t o = perm[i];
+    for (int r = 0; r < nrhs; ++r) y[i + static_cast<int64_t>(r) * n] = b[o + r * ldb];
+}
+
+__global__ void k_scatter(const int32_t *__restrict__ perm, const double *__restrict__ y, double *__restrict__ b,
+                          int64_t ldb, int32_t n, int32_t nrhs) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t o = perm[i];
+    for (int r = 0; r < nrhs; ++r) b[o + r * ldb] = y[i + static_cast<int64_t>(r) * n];
+}
+
+void launch_gather_rhs(hipStream_t st, const int32_t *perm, const double *b, int64_t ldb, double *y,
+                       int32_t n, int32_t nrhs) {
+    if (n <= 0) return;
+    k_gather<<<dim3((n + kThreads - 1) / kThreads), kThreads, 0, st>>>(perm, b, ldb, y, n, nrhs);
+}
+
+void launch_scatter_rhs(hipStream_t st, const int32_t *perm, const double *y, double *b, int64_t ldb,
+                        int32_t n, int32_t nrhs) {
+    if (n <= 0) return;
+    k_scatter<<<dim3((n + kThreads - 1) / kThreads), kThreads, 0, st>>>(perm, y, b, ldb, n, nrhs);
+}
+
+// Forward: per front, blocks of 64 pivots: z = L_kk^{-1} y_blk via the
+// stored inverse, then the rows below: own pivot rows in place, ancestor
+// rows by atomic add (sibling fronts of one level share ancestors).
+__global__ __launch_bounds__(kThreads) void k_fwd(DevPtrs d, const int32_t *__restrict__ flist,
+                                                 double *__restrict__ y, int32_t n, int32_t nrhs) {
+    __shared__ double yb[kNb][kMaxRhs];
+    __shared__ double z[kNb][kMaxRhs];
+    const int s = flist[blockIdx.x];
+    const FrontDev F = d.fronts[s];
+    const int f = F.f, p = F.p;
+    const double *Lb = d.L + F.lofs;
+    const int32_t *frows = d.rows + F.rofs;
+    const int tid = threadIdx.x;
+    for (int k = 0; k < p; k += kNb) {
+        const int kb = min(kNb, p - k);
+        for (int idx = tid; idx < kb * nrhs; idx += kThreads) {
+            const int t = idx % kb, r = idx / kb;
+            yb[t][r] = y[F.col0 + k + t + static_cast<int64_t>(r) * n];
+        }
+        __syncthreads();
+        {
+            const int c = tid % kNb;
+            if (c < kb) {
+                const double *Mc = Lb + static_cast<int64_t>(k + c) * f + k;  // Mc[t] = M[t][c], t < c
+                const double dinv = 1.0 / Mc[c];
+                for (int r = tid / kNb; r < nrhs; r += kThreads / kNb) {
+                    double s2 = yb[c][r] * dinv;
+                    for (int t = 0; t < c; ++t) s2 += Mc[t] * yb[t][r];
+                    z[c][r] = s2;
+                }
+            }
+        }
+        __syncthreads();
+        for (int idx = tid; idx < kb * nrhs; idx += kThreads) {
+            const int t = idx % kb, r = idx / kb;
+            y[F.col0 + k + t + static_cast<int64_t>(r) * n] = z[t][r];
+        }
+        for (int i = k + kb + tid; i < f; i += kThreads) {
+            double acc[kMaxRhs];
+#pragma unroll
+            for (int r = 0; r < kMaxRhs; ++r) acc[r] = 0.0;
+            for (int t = 0; t < kb; ++t) {
+                const double l = Lb[static_cast<int64_t>(k + t) * f + i];
+#pragma unroll
+                for (int r = 0; r < kMaxRhs; ++r)
+                    if (r < nrhs) acc[r] += l * z[t][r];
+            }
+            if (i < p) {
+                for (int r = 0; r < nrhs; ++r) y[F.col0 + i + static_cast<int64_t>(r) * n] -= acc[r];
+            } else {
+                const int64_t gi = frows[i];
+                for (int r = 0; r < nrhs; ++r) atomicAdd(&y[gi + static_cast<int64_t>(r) * n], -acc[r]);
+            }
+        }
+        __threadfence_block();
+        __syncthreads();
+    }
+}
+
+// Backward: blocks from the last to the first: S = L[below, blk]^T x[below]
+// (one wave per column, lanes over rows, wave reduction), then
+// x_blk = L_kk^{-T} (x_blk - S) with L_kk^{-T} = M (stored strict upper).
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+__global__ __launch_bounds__(kThreads) void k_bwd(DevPtrs d, const int32_t *__restrict__ flist,
+                                                 double *__restrict__ y, int32_t n, int32_t nrhs) {
+    __shared__ double S[kNb][kMaxRhs];
+    __shared__ double v[kNb][kMaxRhs];
+    const int s = flist[blockIdx.x];
+    const FrontDev F = d.fronts[s];
+    const int f = F.f, p = F.p;
+    const double *Lb = d.L + F.lofs;
+    const int32_t *frows = d.rows + F.rofs;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, w = tid >> 6;
+    const int nblk = (p + kNb - 1) / kNb;
+    for (int bi = nblk - 1; bi >= 0; --bi) {
+        const int k = bi * kNb;
+        const int kb = min(kNb, p - k);
+        for (int c = w; c < kb; c += kThreads / 64) {
+            double acc[kMaxRhs];
+#pragma unroll
+            for (int r = 0; r < kMaxRhs; ++r) acc[r] = 0.0;
+            const double *Lc = Lb + static_cast<int64_t>(k + c) * f;
+            for (int i = k + kb + lane; i < f; i += 64) {
+                const double l = Lc[i];
+                const int64_t gi = (i < p) ? static_cast<int64_t>(F.col0 + i) : static_cast<int64_t>(frows[i]);
+#pragma unroll
+                for (int r = 0; r < kMaxRhs; ++r)
+                    if (r < nrhs) acc[r] += l * y[gi + static_cast<int64_t>(r) * n];
+            }
+            for (int r = 0; r < nrhs; ++r) {
+                const double sr = wave_sum(acc[r]);
+                if (lane == 0) S[c][r] = sr;
+            }
+        }
+        __syncthreads();
+        for (int idx = tid; idx < kb * nrhs; idx += kThreads) {
+            const int t = idx % kb, r = idx / kb;
+            v[t][r] = y[F.col0 + k + t + static_cast<int64_t>(r) * n] - S[t][r];
+        }
+        __syncthreads();
+        {
+            const int c = tid % kNb;
+            if (c < kb) {
+                const double dinv = 1.0 / Lb[static_cast<int64_t>(k + c) * f + k + c];
+                for (int r = tid / kNb; r < nrhs; r += kThreads / kNb) {
+                    double s2 = v[c][r] * dinv;
+                    for (int t = c + 1; t < kb; ++t) s2 += Lb[static_cast<int64_t>(k + t) * f + k + c] * v[t][r];
+                    y[F.col0 + k + c + static_cast<int64_t>(r) * n] = s2;
+                }
+            }
+        }
+        __threadfence_block();
+        __syncthreads();
+    }
+}
+
+void launch_fwd(hipStream_t st, const DevPtrs &d, const int32_t *fronts, int32_t nf, double *y,
+                int32_t n, int32_t nrhs) {
+    if (nf <= 0) return;
+    k_fwd<<<dim3(nf), kThreads, 0, st>>>(d, fronts, y, n, nrhs);
+}
+
+void launch_bwd(hipStream_t st, const DevPtrs &d, const int32_t *fronts, int32_t nf, double *y,
+                int32_t n, int32_t nrhs) {
+    if (nf <= 0) return;
+    k_bwd<<<dim3(nf), kThreads, 0, st>>>(d, fronts, y, n, nrhs);
+}
+
+// ---------------------------------------------------------------------------
+// A16: residual r = b - A x in permuted numbering over the full symmetric
+// CSR, products and sums in double-double (TwoProd by fma, TwoSum), and the
+// per-row terms of the backward errors (refinement.cpp:39-130):
+// rowstat[4i] = |r_i|, [4i+1] = (|A||x|)_i + |b_i|, [4i+2] = |x_i|,
+// [4i+3] = |b_i|.
+__device__ __forceinline__ void two_sum(double a, double b, double &s, double &e) {
+    s = a + b;
+    const double bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+
+__global__ void k_residual(int32_t n, const int64_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                           const int64_t *__restrict__ slot, const double *__restrict__ av,
+                           const double *__restrict__ x, const double *__restrict__ b,
+                           double *__restrict__ r, double *__restrict__ rowstat, int32_t nrhs) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int c = 0; c < nrhs; ++c) {
+        const double *xc = x + static_cast<int64_t>(c) * n;
+        const double bi = b[i + static_cast<int64_t>(c) * n];
+        double sh = bi, sl = 0.0, absax = 0.0;
+        for (int64_t k = rp[i]; k < rp[i + 1]; ++k) {
+            const double a = av[slot[k]];
+            const double xv = xc[ci[k]];
+            const double ph = -a * xv;
+            const double pl = fma(-a, xv, -ph);
+            double s, e;
+            two_sum(sh, ph, s, e);
+            sh = s;
+            sl += e + pl;
+            absax += fabs(a) * fabs(xv);
+        }
+        const double res = sh + sl;
+        r[i + static_cast<int64_t>(c) * n] = res;
+        double *rs = rowstat + 4 * (i + static_cast<int64_t>(c) * n);
+        rs[0] = fabs(res);
+        rs[1] = absax + fabs(bi);
+        rs[2] = fabs(xc[i]);
+        rs[3] = fabs(bi);
+    }
+}
+
+void launch_residual(hipStream_t st, int32_t n, const int64_t *rp, const int32_t *ci, const int64_t *slot,
+                     const double *av, const double *x, const double *b, double *r, double *rowstat,
+                     int32_t nrhs) {
+    if (n <= 0) return;
+    k_residual<<<dim3((n + kThreads - 1) / kThreads), kThreads, 0, st>>>(n, rp, ci, slot, av, x, b, r, rowstat,
+                                                                         nrhs);
+}
+
+// row sums of |A| (for ||A||_inf)
+__global__ void k_rowabs(int32_t n, const int64_t *__restrict__ rp, const int64_t *__restrict__ slot,
+                         const double *__restrict__ av, double *__restrict__ out) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int64_t k = rp[i]; k < rp[i + 1]; ++k) s += fabs(av[slot[k]]);
+    out[i] = s;
+}
+
+void launch_rowabs(hipStream_t st, int32_t n, const int64_t *rp, const int64_t *slot, const double *av,
+                   double *out) {
+    if (n <= 0) return;
+    k_rowabs<<<dim3((n + kThreads - 1) / kThreads), kThreads, 0, st>>>(n, rp, slot, av, out);
+}
+
+__global__ void k_axpy(int64_t n, const double *__restrict__ dlt, double *__restrict__ x) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) x[i] += dlt[i];
+}
+
+void launch_axpy(hipStream_t st, int64_t n, const double *dlt, double *x) {
+    if (n <= 0) return;
+    k_axpy<<<dim3(static_cast<unsigned>((n + kThreads - 1) / kThreads)), kThreads, 0, st>>>(n, dlt, x);
+}
+
+// max over i of v[i] (v >= 0), into *out (bit-pattern max, out zeroed first)
+__global__ void k_maxbits(const double *__restrict__ v, int64_t n, int64_t stride, unsigned long long *out) {
+    __shared__ unsigned long long red[kThreads];
+    unsigned long long m = 0;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const unsigned long long bb = static_cast<unsigned long long>(__double_as_longlong(v[i * stride]));
+        m = bb > m ? bb : m;
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w && red[threadIdx.x + w] > red[threadIdx.x]) red[threadIdx.x] = red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicMax(out, red[0]);
+}
+
+void launch_reduce_max(hipStream_t st, const double *v, int64_t n, double *out) {
+    auto *o = reinterpret_cast<unsigned long long *>(out);
+    k_fill_u64<<<1, 64, 0, st>>>(o, 1, 0ull);
+    if (n <= 0) return;
+    const int64_t g = std::min<int64_t>((n + kThreads - 1) / kThreads, 1024);
+    k_maxbits<<<dim3(static_cast<unsigned>(g)), kThreads, 0, st>>>(v, n, 1, o);
+}
+
+}  // namespace smfgpu

@@ -1,0 +1,869 @@
+// Host orchestration of the smfgpu numeric phase: plan creation (the
+// per-analysis set-up of multifrontal_factor_serial,
+// solver/src/multifrontal_serial.cpp:1105-1320 of the reference), the
+// level-batched factorization, solves, refinement and the FactorKeep export.
+#include "smfgpu_internal.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <numeric>
+
+namespace smfgpu {
+
+namespace {
+thread_local std::string g_err;
+}
+
+int set_error(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_OK(expr)                                                                       \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return set_error(SMFGPU_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// Device buffer with RAII.
+template <class T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    int alloc(size_t count) {
+        release();
+        if (count == 0) return SMFGPU_OK;
+        if (hipMalloc(reinterpret_cast<void **>(&p), count * sizeof(T)) != hipSuccess) {
+            p = nullptr;
+            return set_error(SMFGPU_ERR_ALLOC, "device allocation of " + std::to_string(count * sizeof(T)) + " bytes failed");
+        }
+        n = count;
+        return SMFGPU_OK;
+    }
+    int upload(const std::vector<T> &v) {
+        int rc = alloc(v.size());
+        if (rc != SMFGPU_OK) return rc;
+        if (!v.empty() && hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+            return set_error(SMFGPU_ERR_DEVICE, "upload failed");
+        return SMFGPU_OK;
+    }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+constexpr int kPanelOuter = 256;
+constexpr int kPanelInner = 64;
+constexpr int kTile = 128;
+
+}  // namespace smfgpu
+
+using namespace smfgpu;
+
+struct smfgpu_analysis_s {
+    Analysis an;
+};
+
+struct smfgpu_plan_s {
+    int32_t n = 0, nsup = 0;
+    int32_t posdef = 1;
+    hipStream_t stream = nullptr;
+    std::vector<FrontDev> fronts;
+    std::vector<int32_t> level;                    // per supernode
+    std::vector<std::vector<int32_t>> level_fronts;
+    std::vector<int32_t> perm;
+    int64_t nnz_perm = 0, n_orig = 0;
+    int64_t factor_entries = 0, cb_entries = 0, rows_total = 0;
+    double flops = 0.0;
+    std::vector<double> front_flops;
+    std::vector<Launch> launches;
+    // device
+    DBuf<FrontDev> d_fronts;
+    DBuf<int32_t> d_rows, d_cmap, d_pcol_ptr, d_amap, d_perm, d_level_fronts, d_flag;
+    DBuf<int64_t> d_cmofs, d_slot_ptr, d_slot_src;
+    DBuf<double> d_av, d_L, d_CB, d_red, d_vals;
+    DBuf<AsmTask> d_asm;
+    DBuf<ChildCol> d_cc;
+    DBuf<PotrfItem> d_potrf;
+    DBuf<TrsmItem> d_trsm;
+    DBuf<SyrkItem> d_syrk;
+    DBuf<int8_t> d_prec;
+    std::vector<int64_t> level_front_ofs;          // offsets into d_level_fronts
+    // refinement CSR (full symmetric, permuted numbering)
+    DBuf<int64_t> d_fr_ptr, d_fr_slot;
+    DBuf<int32_t> d_fr_col;
+    // solve / refine work
+    DBuf<double> d_y, d_b, d_r, d_rowstat, d_x, d_bb;
+    std::vector<int64_t> syrk_flops_of_launch;
+    // timing
+    bool timing = false;
+    double t_syrk_ms = 0, t_asm_ms = 0, t_solve_ms = 0;
+    double syrk_launches = 0, syrk_flops = 0, asm_bytes = 0, solve_bytes = 0;
+    std::vector<hipEvent_t> ev;
+    int8_t last_prec_any = 0;
+    smfgpu_factor_s *live = nullptr;
+    double max_abs = 0.0;
+};
+
+struct smfgpu_factor_s {
+    smfgpu_plan_s *plan = nullptr;
+    smfgpu_factor_info info{};
+    bool valid = false;
+    std::vector<int8_t> prec;
+};
+
+namespace {
+
+DevPtrs dev_ptrs(smfgpu_plan_s *P) {
+    DevPtrs d{};
+    d.fronts = P->d_fronts.p;
+    d.rows = P->d_rows.p;
+    d.cmap = P->d_cmap.p;
+    d.cmofs = P->d_cmofs.p;
+    d.pcol_ptr = P->d_pcol_ptr.p;
+    d.amap = P->d_amap.p;
+    d.av = P->d_av.p;
+    d.L = P->d_L.p;
+    d.CB = P->d_CB.p;
+    d.flag = P->d_flag.p;
+    d.prec = P->d_prec.p;
+    return d;
+}
+
+int plan_build(smfgpu_plan_s *P, const smfgpu_analysis_desc *D) {
+    const int32_t n = D->n, ns = D->nsup;
+    P->n = n;
+    P->nsup = ns;
+    P->perm.assign(D->perm, D->perm + n);
+    P->nnz_perm = D->pcol_ptr[n];
+    P->n_orig = D->n_orig;
+    std::vector<int32_t> col2sn(n, -1);
+    P->fronts.resize(ns);
+    P->front_flops.assign(ns, 0.0);
+    int64_t lofs = 0, cbofs = 0;
+    for (int32_t s = 0; s < ns; ++s) {
+        FrontDev &F = P->fronts[s];
+        F.f = D->front_row_ptr[s + 1] - D->front_row_ptr[s];
+        F.p = D->sn_end[s] - D->sn_start[s];
+        F.q = F.f - F.p;
+        F.col0 = D->sn_start[s];
+        F.lofs = lofs;
+        F.cbofs = cbofs;
+        F.rofs = D->front_row_ptr[s];
+        if (F.p <= 0 || F.q < 0) return set_error(SMFGPU_ERR_INVALID_ARG, "plan: bad supernode sizes");
+        for (int32_t j = D->sn_start[s]; j < D->sn_end[s]; ++j) col2sn[j] = s;
+        lofs += static_cast<int64_t>(F.f) * F.p;
+        cbofs += static_cast<int64_t>(F.q) * F.q;
+        double fl = 0.0;
+        for (int c = 0; c < F.p; ++c) {
+            const double rem = static_cast<double>(F.f - c - 1);
+            fl += rem + rem * (rem + 1.0);
+        }
+        P->front_flops[s] = fl;
+        P->flops += fl;
+    }
+    P->factor_entries = lofs;
+    P->cb_entries = cbofs;
+    P->rows_total = D->front_row_ptr[ns];
+    // levels (children before parents; supernodes are in postorder)
+    P->level.assign(ns, 0);
+    int32_t nlev = 0;
+    for (int32_t s = 0; s < ns; ++s) {
+        const int32_t par = D->sn_parent[s];
+        if (par >= 0) {
+            if (par <= s) return set_error(SMFGPU_ERR_INVALID_ARG, "plan: supernodes not in postorder");
+            P->level[par] = std::max(P->level[par], P->level[s] + 1);
+        }
+        nlev = std::max(nlev, P->level[s] + 1);
+    }
+    P->level_fronts.assign(nlev, {});
+    for (int32_t s = 0; s < ns; ++s) P->level_fronts[P->level[s]].push_back(s);
+    // local positions: amap for the permuted entries, cmap for each child
+    std::vector<int32_t> pos(n, -1);
+    std::vector<int32_t> amap(P->nnz_perm, 0);
+    std::vector<int64_t> cmofs(ns + 1, 0);
+    for (int32_t s = 0; s < ns; ++s) cmofs[s + 1] = cmofs[s] + P->fronts[s].q;
+    std::vector<int32_t> cmap(cmofs[ns]);
+    for (int32_t s = 0; s < ns; ++s) {
+        const FrontDev &F = P->fronts[s];
+        const int32_t *fr = D->front_row_idx + F.rofs;
+        for (int t = 0; t < F.f; ++t) pos[fr[t]] = t;
+        for (int t = 0; t < F.p; ++t)
+            if (fr[t] != F.col0 + t) return set_error(SMFGPU_ERR_INVALID_ARG, "plan: front rows do not start with the pivots");
+        for (int32_t j = D->sn_start[s]; j < D->sn_end[s]; ++j)
+            for (int32_t k = D->pcol_ptr[j]; k < D->pcol_ptr[j + 1]; ++k) {
+                const int32_t r = pos[D->prow_idx[k]];
+                if (r < 0) return set_error(SMFGPU_ERR_INVALID_ARG, "plan: entry outside its front");
+                amap[k] = r;
+            }
+        for (int32_t c = D->child_ptr[s]; c < D->child_ptr[s + 1]; ++c) {
+            const int32_t ch = D->child_idx[c];
+            const FrontDev &C = P->fronts[ch];
+            const int32_t *cr = D->front_row_idx + C.rofs + C.p;
+            int32_t prev = -1;
+            for (int ii = 0; ii < C.q; ++ii) {
+                const int32_t r = pos[cr[ii]];
+                if (r < 0 || r <= prev) return set_error(SMFGPU_ERR_INVALID_ARG, "plan: child row outside parent front");
+                cmap[cmofs[ch] + ii] = r;
+                prev = r;
+            }
+        }
+        for (int t = 0; t < F.f; ++t) pos[fr[t]] = -1;
+    }
+    // assembly tasks per level, child contributions per parent column
+    std::vector<AsmTask> tasks;
+    std::vector<ChildCol> cc;
+    std::vector<int64_t> asm_first(nlev + 1, 0);
+    std::vector<std::vector<ChildCol>> bucket;
+    for (int32_t lv = 0; lv < nlev; ++lv) {
+        asm_first[lv] = static_cast<int64_t>(tasks.size());
+        for (int32_t s : P->level_fronts[lv]) {
+            const FrontDev &F = P->fronts[s];
+            bucket.assign(F.f, {});
+            for (int32_t c = D->child_ptr[s]; c < D->child_ptr[s + 1]; ++c) {
+                const int32_t ch = D->child_idx[c];
+                const FrontDev &C = P->fronts[ch];
+                for (int jj = 0; jj < C.q; ++jj) bucket[cmap[cmofs[ch] + jj]].push_back(ChildCol{ch, jj});
+            }
+            for (int c = 0; c < F.f; ++c) {
+                AsmTask t{s, c, static_cast<int32_t>(cc.size()), 0};
+                cc.insert(cc.end(), bucket[c].begin(), bucket[c].end());
+                t.cc1 = static_cast<int32_t>(cc.size());
+                tasks.push_back(t);
+            }
+        }
+    }
+    asm_first[nlev] = static_cast<int64_t>(tasks.size());
+    if (cc.size() > static_cast<size_t>(INT32_MAX)) return set_error(SMFGPU_ERR_ALLOC, "plan: too many child columns");
+    // dense-kernel items and the launch sequence
+    std::vector<PotrfItem> potrf;
+    std::vector<TrsmItem> trsm;
+    std::vector<SyrkItem> syrk;
+    auto add_syrk = [&](int32_t s, int32_t jlo, int32_t jhi, int32_t t0, int32_t t1) {
+        const FrontDev &F = P->fronts[s];
+        for (int32_t jt = jlo; jt < jhi; jt += kTile)
+            for (int32_t it = jt; it < F.f; it += kTile) syrk.push_back(SyrkItem{s, it, jt, jhi, t0, t1, 0, 0});
+    };
+    for (int32_t lv = 0; lv < nlev; ++lv) {
+        const auto &fl = P->level_fronts[lv];
+        P->launches.push_back(Launch{K_ASSEMBLE, lv, asm_first[lv], asm_first[lv + 1] - asm_first[lv]});
+        int32_t maxp = 0;
+        for (int32_t s : fl) maxp = std::max(maxp, P->fronts[s].p);
+        for (int32_t K = 0; K < maxp; K += kPanelOuter) {
+            for (int32_t k = K; k < std::min(K + kPanelOuter, maxp); k += kPanelInner) {
+                const int64_t p0 = static_cast<int64_t>(potrf.size());
+                const int64_t t0 = static_cast<int64_t>(trsm.size());
+                for (int32_t s : fl) {
+                    const FrontDev &F = P->fronts[s];
+                    if (F.p <= k) continue;
+                    const int32_t kb = std::min(kPanelInner, F.p - k);
+                    potrf.push_back(PotrfItem{s, k, kb, 0});
+                    for (int32_t r0 = k + kb; r0 < F.f; r0 += kPanelInner) trsm.push_back(TrsmItem{s, k, kb, r0});
+                }
+                P->launches.push_back(Launch{K_POTRF, lv, p0, static_cast<int64_t>(potrf.size()) - p0});
+                P->launches.push_back(Launch{K_TRSM, lv, t0, static_cast<int64_t>(trsm.size()) - t0});
+                const int64_t y0 = static_cast<int64_t>(syrk.size());
+                for (int32_t s : fl) {
+                    const FrontDev &F = P->fronts[s];
+                    if (F.p <= k) continue;
+                    const int32_t kb = std::min(kPanelInner, F.p - k);
+                    const int32_t kend = std::min(K + kPanelOuter, F.p);
+                    if (k + kb < kend) add_syrk(s, k + kb, kend, k, k + kb);
+                }
+                P->launches.push_back(Launch{K_SYRK, lv, y0, static_cast<int64_t>(syrk.size()) - y0});
+            }
+            const int64_t y0 = static_cast<int64_t>(syrk.size());
+            for (int32_t s : fl) {
+                const FrontDev &F = P->fronts[s];
+                if (F.p <= K) continue;
+                const int32_t kend = std::min(K + kPanelOuter, F.p);
+                if (kend < F.f) add_syrk(s, kend, F.f, K, kend);
+            }
+            P->launches.push_back(Launch{K_SYRK, lv, y0, static_cast<int64_t>(syrk.size()) - y0});
+        }
+    }
+    P->launches.erase(std::remove_if(P->launches.begin(), P->launches.end(),
+                                     [](const Launch &l) { return l.count <= 0; }),
+                      P->launches.end());
+    // syrk flops per launch (algorithmic: 2 * entries updated * panel width)
+    for (const Launch &l : P->launches) {
+        if (l.kind != K_SYRK) {
+            P->syrk_flops_of_launch.push_back(0);
+            continue;
+        }
+        int64_t fl = 0;
+        for (int64_t i = l.first; i < l.first + l.count; ++i) {
+            const SyrkItem &y = syrk[i];
+            const FrontDev &F = P->fronts[y.s];
+            const int64_t jh = std::min<int64_t>(y.j0 + kTile, std::min(y.j1, F.f));
+            const int64_t ih = std::min<int64_t>(y.i0 + kTile, F.f);
+            int64_t cnt = 0;
+            for (int64_t j = y.j0; j < jh; ++j) cnt += std::max<int64_t>(0, ih - std::max<int64_t>(j, y.i0));
+            fl += 2 * cnt * (y.t1 - y.t0);
+        }
+        P->syrk_flops_of_launch.push_back(fl);
+    }
+    // value map: slot -> input entries in input order
+    const int64_t nslots = P->nnz_perm;
+    std::vector<int64_t> slot_ptr(nslots + 1, 0), slot_src;
+    for (int64_t e = 0; e < D->n_orig; ++e) {
+        const int32_t sl = D->orig_to_slot[e];
+        if (sl >= 0) {
+            if (sl >= nslots) return set_error(SMFGPU_ERR_INVALID_ARG, "plan: orig_to_slot out of range");
+            ++slot_ptr[sl + 1];
+        }
+    }
+    for (int64_t s = 0; s < nslots; ++s) slot_ptr[s + 1] += slot_ptr[s];
+    slot_src.resize(slot_ptr[nslots]);
+    {
+        std::vector<int64_t> fill(slot_ptr.begin(), slot_ptr.end() - 1);
+        for (int64_t e = 0; e < D->n_orig; ++e) {
+            const int32_t sl = D->orig_to_slot[e];
+            if (sl >= 0) slot_src[fill[sl]++] = e;
+        }
+    }
+    // full symmetric CSR for the residual (permuted numbering)
+    std::vector<int64_t> fr_ptr(n + 1, 0);
+    for (int32_t j = 0; j < n; ++j)
+        for (int32_t k = D->pcol_ptr[j]; k < D->pcol_ptr[j + 1]; ++k) {
+            const int32_t i = D->prow_idx[k];
+            ++fr_ptr[i + 1];
+            if (i != j) ++fr_ptr[j + 1];
+        }
+    for (int32_t i = 0; i < n; ++i) fr_ptr[i + 1] += fr_ptr[i];
+    std::vector<int32_t> fr_col(fr_ptr[n]);
+    std::vector<int64_t> fr_slot(fr_ptr[n]);
+    {
+        std::vector<int64_t> fill(fr_ptr.begin(), fr_ptr.end() - 1);
+        for (int32_t j = 0; j < n; ++j)
+            for (int32_t k = D->pcol_ptr[j]; k < D->pcol_ptr[j + 1]; ++k) {
+                const int32_t i = D->prow_idx[k];
+                fr_col[fill[i]] = j;
+                fr_slot[fill[i]++] = k;
+                if (i != j) {
+                    fr_col[fill[j]] = i;
+                    fr_slot[fill[j]++] = k;
+                }
+            }
+    }
+    // level front lists for the solves
+    std::vector<int32_t> lfl;
+    P->level_front_ofs.assign(nlev + 1, 0);
+    for (int32_t lv = 0; lv < nlev; ++lv) {
+        P->level_front_ofs[lv] = static_cast<int64_t>(lfl.size());
+        lfl.insert(lfl.end(), P->level_fronts[lv].begin(), P->level_fronts[lv].end());
+    }
+    P->level_front_ofs[nlev] = static_cast<int64_t>(lfl.size());
+    // uploads
+    int rc;
+    std::vector<int32_t> rows(D->front_row_idx, D->front_row_idx + P->rows_total);
+    std::vector<int32_t> pcp(D->pcol_ptr, D->pcol_ptr + n + 1);
+    if ((rc = P->d_fronts.upload(P->fronts)) || (rc = P->d_rows.upload(rows)) || (rc = P->d_cmap.upload(cmap)) ||
+        (rc = P->d_cmofs.upload(cmofs)) || (rc = P->d_pcol_ptr.upload(pcp)) || (rc = P->d_amap.upload(amap)) ||
+        (rc = P->d_perm.upload(P->perm)) || (rc = P->d_level_fronts.upload(lfl)) ||
+        (rc = P->d_slot_ptr.upload(slot_ptr)) || (rc = P->d_slot_src.upload(slot_src)) ||
+        (rc = P->d_asm.upload(tasks)) || (rc = P->d_cc.upload(cc)) || (rc = P->d_potrf.upload(potrf)) ||
+        (rc = P->d_trsm.upload(trsm)) || (rc = P->d_syrk.upload(syrk)) || (rc = P->d_fr_ptr.upload(fr_ptr)) ||
+        (rc = P->d_fr_col.upload(fr_col)) || (rc = P->d_fr_slot.upload(fr_slot)))
+        return rc;
+    if ((rc = P->d_av.alloc(std::max<int64_t>(nslots, 1))) || (rc = P->d_L.alloc(std::max<int64_t>(P->factor_entries, 1))) ||
+        (rc = P->d_CB.alloc(std::max<int64_t>(P->cb_entries, 1))) || (rc = P->d_red.alloc(8)) ||
+        (rc = P->d_flag.alloc(4)) || (rc = P->d_prec.alloc(std::max(ns, 1))))
+        return rc;
+    // algorithmic bytes (assembly, one solve sweep pair per RHS)
+    double asm_b = 12.0 * static_cast<double>(P->nnz_perm);
+    for (int32_t s = 0; s < ns; ++s) {
+        const double f = P->fronts[s].f, q = P->fronts[s].q;
+        asm_b += 8.0 * f * (f + 1) / 2.0;
+        if (D->sn_parent[s] >= 0) asm_b += 8.0 * q * (q + 1) / 2.0 + 4.0 * q;
+    }
+    P->asm_bytes = asm_b;
+    return SMFGPU_OK;
+}
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+extern "C" {
+
+SMFGPU_API const char *smfgpu_error_string(void) { return g_err.c_str(); }
+
+SMFGPU_API int smfgpu_default_options(smfgpu_factor_options *opt) {
+    if (!opt) return set_error(SMFGPU_ERR_INVALID_ARG, "null options");
+    std::memset(opt, 0, sizeof(*opt));
+    opt->posdef = 1;
+    opt->pivot_u = 0.01;
+    opt->small_pivot = 1e-20;
+    opt->panel = 16;
+    opt->precision = SMFGPU_PREC_FP64;
+    opt->mixed_min_front = 1024;
+    opt->max_negative_pivots = -1;
+    opt->value_mem = SMFGPU_MEM_HOST;
+    return SMFGPU_OK;
+}
+
+SMFGPU_API int smfgpu_analyse(int32_t n, const int32_t *col_ptr, const int32_t *row_idx, const int32_t *perm,
+                              int32_t nemin, smfgpu_analysis_t *out) {
+    if (!out) return set_error(SMFGPU_ERR_INVALID_ARG, "null out");
+    *out = nullptr;
+    try {
+        auto h = std::make_unique<smfgpu_analysis_s>();
+        const int rc = analyse(n, col_ptr, row_idx, perm, nemin, h->an);
+        if (rc != SMFGPU_OK) return rc;
+        *out = h.release();
+        return SMFGPU_OK;
+    } catch (const std::bad_alloc &) {
+        return set_error(SMFGPU_ERR_ALLOC, "analyse: host allocation failed");
+    } catch (const std::exception &e) {
+        return set_error(SMFGPU_ERR_INVALID_ARG, e.what());
+    }
+}
+
+SMFGPU_API int smfgpu_analysis_describe(smfgpu_analysis_t an, smfgpu_analysis_desc *desc) {
+    if (!an || !desc) return set_error(SMFGPU_ERR_INVALID_ARG, "null handle");
+    an->an.describe(*desc);
+    return SMFGPU_OK;
+}
+
+SMFGPU_API int smfgpu_analysis_stats(smfgpu_analysis_t an, double *out) {
+    if (!an || !out) return set_error(SMFGPU_ERR_INVALID_ARG, "null handle");
+    const Analysis &a = an->an;
+    double rows = static_cast<double>(a.front_row_ptr.empty() ? 0 : a.front_row_ptr.back());
+    double fp = 0, fl = 0;
+    int maxf = 0;
+    std::vector<int> lev(a.nsup, 0);
+    int depth = 0;
+    for (int s = 0; s < a.nsup; ++s) {
+        const int f = a.front_row_ptr[s + 1] - a.front_row_ptr[s];
+        const int p = a.sn_end[s] - a.sn_start[s];
+        fp += static_cast<double>(f) * p;
+        maxf = std::max(maxf, f);
+        for (int c = 0; c < p; ++c) {
+            const double rem = f - c - 1.0;
+            fl += rem + rem * (rem + 1.0);
+        }
+        if (a.sn_parent[s] >= 0) lev[a.sn_parent[s]] = std::max(lev[a.sn_parent[s]], lev[s] + 1);
+        depth = std::max(depth, lev[s] + 1);
+    }
+    out[0] = a.nsup;
+    out[1] = rows;
+    out[2] = fp;
+    out[3] = maxf;
+    out[4] = fl;
+    out[5] = depth;
+    return SMFGPU_OK;
+}
+
+SMFGPU_API void smfgpu_analysis_free(smfgpu_analysis_t *an) {
+    if (an && *an) {
+        delete *an;
+        *an = nullptr;
+    }
+}
+
+SMFGPU_API int smfgpu_plan_create(const smfgpu_analysis_desc *desc, int32_t posdef, smfgpu_plan_t *out) {
+    if (!desc || !out) return set_error(SMFGPU_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    if (!posdef) return set_error(SMFGPU_ERR_UNSUPPORTED, "plan: indefinite LDL^T plans are not built by this entry point yet");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return set_error(SMFGPU_ERR_DEVICE, "no HIP device");
+    try {
+        auto P = std::make_unique<smfgpu_plan_s>();
+        P->posdef = posdef;
+        HIP_OK(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking));
+        const int rc = plan_build(P.get(), desc);
+        if (rc != SMFGPU_OK) {
+            (void)hipStreamDestroy(P->stream);
+            return rc;
+        }
+        *out = P.release();
+        return SMFGPU_OK;
+    } catch (const std::bad_alloc &) {
+        return set_error(SMFGPU_ERR_ALLOC, "plan: host allocation failed");
+    }
+}
+
+SMFGPU_API void smfgpu_plan_free(smfgpu_plan_t *plan) {
+    if (!plan || !*plan) return;
+    smfgpu_plan_s *P = *plan;
+    if (P->stream) {
+        (void)hipStreamSynchronize(P->stream);
+        (void)hipStreamDestroy(P->stream);
+    }
+    for (hipEvent_t e : P->ev) (void)hipEventDestroy(e);
+    if (P->live) P->live->plan = nullptr;
+    delete P;
+    *plan = nullptr;
+}
+
+SMFGPU_API int smfgpu_plan_stats(smfgpu_plan_t P, double *out) {
+    if (!P || !out) return set_error(SMFGPU_ERR_INVALID_ARG, "null handle");
+    double bytes = 0;
+    bytes += P->d_L.bytes() + P->d_CB.bytes() + P->d_av.bytes() + P->d_rows.bytes() + P->d_cmap.bytes() +
+             P->d_asm.bytes() + P->d_cc.bytes() + P->d_potrf.bytes() + P->d_trsm.bytes() + P->d_syrk.bytes() +
+             P->d_amap.bytes() + P->d_slot_src.bytes() + P->d_fr_col.bytes() + P->d_fr_slot.bytes();
+    out[0] = bytes;
+    out[1] = static_cast<double>(P->level_fronts.size());
+    out[2] = static_cast<double>(P->launches.size());
+    out[3] = P->flops;
+    out[4] = static_cast<double>(P->factor_entries);
+    out[5] = static_cast<double>(P->cb_entries);
+    out[6] = P->asm_bytes;
+    double sb = 0;
+    for (const FrontDev &F : P->fronts) sb += 2.0 * (8.0 * F.f * static_cast<double>(F.p) + 4.0 * F.f) + 16.0 * F.f;
+    out[7] = sb;
+    return SMFGPU_OK;
+}
+
+SMFGPU_API int smfgpu_set_timing(smfgpu_plan_t P, int32_t enabled) {
+    if (!P) return set_error(SMFGPU_ERR_INVALID_ARG, "null handle");
+    P->timing = enabled != 0;
+    return SMFGPU_OK;
+}
+
+SMFGPU_API int smfgpu_kernel_times(smfgpu_plan_t P, double *out) {
+    if (!P || !out) return set_error(SMFGPU_ERR_INVALID_ARG, "null handle");
+    out[0] = P->t_syrk_ms;
+    out[1] = P->syrk_launches;
+    out[2] = P->syrk_flops;
+    out[3] = P->t_asm_ms;
+    out[4] = P->asm_bytes;
+    out[5] = P->t_solve_ms;
+    out[6] = P->solve_bytes;
+    return SMFGPU_OK;
+}
+
+SMFGPU_API int smfgpu_factor(smfgpu_plan_t P, const double *values, int64_t nvalues,
+                             const smfgpu_factor_options *opt_in, smfgpu_factor_t *out, smfgpu_factor_info *info) {
+    if (!P || !out) return set_error(SMFGPU_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    smfgpu_factor_options opt;
+    if (opt_in) opt = *opt_in;
+    else smfgpu_default_options(&opt);
+    if (!opt.posdef || !P->posdef) return set_error(SMFGPU_ERR_UNSUPPORTED, "factor: LDL^T not available in this plan");
+    if (nvalues != P->n_orig) return set_error(SMFGPU_ERR_INVALID_ARG, "factor: wrong number of values");
+    if (nvalues > 0 && !values) return set_error(SMFGPU_ERR_INVALID_ARG, "factor: null values");
+    hipStream_t st = P->stream;
+    const double *dv = values;
+    if (opt.value_mem == SMFGPU_MEM_HOST && nvalues > 0) {
+        if (P->d_vals.n < static_cast<size_t>(nvalues)) {
+            const int rc = P->d_vals.alloc(nvalues);
+            if (rc) return rc;
+        }
+        HIP_OK(hipMemcpyAsync(P->d_vals.p, values, nvalues * sizeof(double), hipMemcpyHostToDevice, st));
+        dv = P->d_vals.p;
+    }
+    // per-front Schur precision
+    auto F = std::make_unique<smfgpu_factor_s>();
+    F->plan = P;
+    F->prec.assign(P->nsup, 0);
+    int reduced = 0;
+    if (opt.precision == SMFGPU_PREC_FP32 || opt.precision == SMFGPU_PREC_BF16) {
+        for (int32_t s = 0; s < P->nsup; ++s)
+            if (P->fronts[s].f >= opt.mixed_min_front) {
+                F->prec[s] = 1;  // BF16 requests run as FP32 until the BF16 tile kernel exists
+                ++reduced;
+            }
+    }
+    HIP_OK(hipMemcpyAsync(P->d_prec.p, F->prec.data(), P->nsup, hipMemcpyHostToDevice, st));
+    const int32_t big = INT32_MAX;
+    HIP_OK(hipMemcpyAsync(P->d_flag.p, &big, sizeof(int32_t), hipMemcpyHostToDevice, st));
+    DevPtrs d = dev_ptrs(P);
+    // timing events
+    const bool timing = P->timing;
+    const size_t nl = P->launches.size();
+    if (timing && P->ev.size() < 2 * nl + 2) {
+        for (size_t i = P->ev.size(); i < 2 * nl + 2; ++i) {
+            hipEvent_t e;
+            HIP_OK(hipEventCreate(&e));
+            P->ev.push_back(e);
+        }
+    }
+    const double t0 = now_s();
+    if (timing) HIP_OK(hipEventRecord(P->ev[2 * nl], st));
+    launch_map_values(st, dv, nvalues, P->d_slot_ptr.p, P->d_slot_src.p, P->nnz_perm, P->d_av.p, P->d_red.p);
+    for (size_t li = 0; li < nl; ++li) {
+        const Launch &l = P->launches[li];
+        const bool rec = timing && (l.kind == K_SYRK || l.kind == K_ASSEMBLE);
+        if (rec) HIP_OK(hipEventRecord(P->ev[2 * li], st));
+        switch (l.kind) {
+        case K_ASSEMBLE: launch_assemble(st, d, P->d_asm.p + l.first, l.count, P->d_cc.p); break;
+        case K_POTRF: launch_potrf(st, d, P->d_potrf.p + l.first, l.count); break;
+        case K_TRSM: launch_trsm(st, d, P->d_trsm.p + l.first, l.count); break;
+        case K_SYRK: launch_syrk(st, d, P->d_syrk.p + l.first, l.count); break;
+        default: break;
+        }
+        if (rec) HIP_OK(hipEventRecord(P->ev[2 * li + 1], st));
+    }
+    if (timing) HIP_OK(hipEventRecord(P->ev[2 * nl + 1], st));
+    HIP_OK(hipGetLastError());
+    int32_t flag = 0;
+    unsigned long long maxbits = 0;
+    HIP_OK(hipMemcpyAsync(&flag, P->d_flag.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipMemcpyAsync(&maxbits, P->d_red.p, sizeof(maxbits), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    const double t1 = now_s();
+    double max_abs;
+    std::memcpy(&max_abs, &maxbits, sizeof(double));
+    smfgpu_factor_info fi{};
+    fi.max_abs = max_abs;
+    fi.max_front = 0;
+    for (const FrontDev &Fd : P->fronts) fi.max_front = std::max(fi.max_front, Fd.f);
+    fi.fronts_reduced = reduced;
+    fi.factor_entries = P->factor_entries;
+    fi.factor_seconds = t1 - t0;
+    if (timing) {
+        float ms = 0;
+        HIP_OK(hipEventElapsedTime(&ms, P->ev[2 * nl], P->ev[2 * nl + 1]));
+        fi.factor_seconds = ms * 1e-3;
+        P->t_syrk_ms = P->t_asm_ms = 0;
+        P->syrk_launches = 0;
+        P->syrk_flops = 0;
+        for (size_t li = 0; li < nl; ++li) {
+            const Launch &l = P->launches[li];
+            if (l.kind != K_SYRK && l.kind != K_ASSEMBLE) continue;
+            HIP_OK(hipEventElapsedTime(&ms, P->ev[2 * li], P->ev[2 * li + 1]));
+            if (l.kind == K_SYRK) {
+                P->t_syrk_ms += ms;
+                P->syrk_launches += 1;
+                P->syrk_flops += static_cast<double>(P->syrk_flops_of_launch[li]);
+            } else {
+                P->t_asm_ms += ms;
+            }
+        }
+    }
+    if (!(maxbits < 0x7ff0000000000000ull)) {
+        fi.status = SMFGPU_STATUS_SINGULAR;
+        if (info) *info = fi;
+        return set_error(SMFGPU_ERR_VALUE, "factor: non-finite input value");
+    }
+    int32_t npos = P->n;
+    if (flag != INT32_MAX) {
+        fi.status = SMFGPU_STATUS_NOT_POSDEF;
+        // pivots accepted before the failing front (postorder): those fronts' widths
+        npos = 0;
+        for (int32_t s = 0; s < flag - 1; ++s) npos += P->fronts[s].p;
+    } else {
+        fi.status = SMFGPU_STATUS_SUCCESS;
+    }
+    fi.num_positive = npos;
+    fi.actual_flops = P->flops;
+    double fw = 0;
+    for (const FrontDev &Fd : P->fronts) fw += static_cast<double>(Fd.p) * Fd.f * static_cast<double>(Fd.f);
+    fi.front_work = fw;
+    F->info = fi;
+    F->valid = fi.status == SMFGPU_STATUS_SUCCESS;
+    if (P->live) P->live->valid = false;
+    P->live = F.get();
+    P->max_abs = max_abs;
+    if (info) *info = fi;
+    *out = F.release();
+    return SMFGPU_OK;
+}
+
+SMFGPU_API void smfgpu_factor_free(smfgpu_factor_t *f) {
+    if (!f || !*f) return;
+    if ((*f)->plan && (*f)->plan->live == *f) (*f)->plan->live = nullptr;
+    delete *f;
+    *f = nullptr;
+}
+
+SMFGPU_API int smfgpu_export_factor(smfgpu_factor_t f, int64_t *sizes, int64_t *steps, double *factor_values,
+                                    int32_t *row_indices, int8_t *pivot_types) {
+    if (!f || !f->plan) return set_error(SMFGPU_ERR_INVALID_ARG, "null factor");
+    smfgpu_plan_s *P = f->plan;
+    if (P->live != f) return set_error(SMFGPU_ERR_STATE, "export: factor was superseded by a newer factorization");
+    if (sizes) {
+        sizes[0] = P->nsup;
+        sizes[1] = P->factor_entries;
+        sizes[2] = P->rows_total;
+        sizes[3] = P->n;
+    }
+    if (steps)
+        for (int32_t s = 0; s < P->nsup; ++s) {
+            const FrontDev &F = P->fronts[s];
+            steps[4 * s + 0] = F.rofs;
+            steps[4 * s + 1] = F.lofs;
+            steps[4 * s + 2] = F.p;
+            steps[4 * s + 3] = F.f;
+        }
+    if (factor_values) {
+        HIP_OK(hipMemcpy(factor_values, P->d_L.p, P->factor_entries * sizeof(double), hipMemcpyDeviceToHost));
+        // the device keeps inv(L_kk)^T in the strict upper part of each
+        // pivot block; the stored factor has zeros there
+        for (int32_t s = 0; s < P->nsup; ++s) {
+            const FrontDev &F = P->fronts[s];
+            double *Lb = factor_values + F.lofs;
+            for (int c = 0; c < F.p; ++c) std::fill_n(Lb + static_cast<int64_t>(c) * F.f, c, 0.0);
+        }
+    }
+    if (row_indices)
+        HIP_OK(hipMemcpy(row_indices, P->d_rows.p, P->rows_total * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (pivot_types) std::fill_n(pivot_types, P->n, static_cast<int8_t>(1));
+    return SMFGPU_OK;
+}
+
+static int solve_device(smfgpu_plan_s *P, const double *b_dev, int64_t ldb, double *out_dev, int64_t ldo,
+                        int32_t nrhs) {
+    hipStream_t st = P->stream;
+    const int32_t n = P->n;
+    DevPtrs d = dev_ptrs(P);
+    const int32_t nl = static_cast<int32_t>(P->level_fronts.size());
+    for (int32_t r0 = 0; r0 < nrhs; r0 += 16) {
+        const int32_t nr = std::min(16, nrhs - r0);
+        if (P->d_y.n < static_cast<size_t>(n) * nr) {
+            const int rc = P->d_y.alloc(static_cast<size_t>(n) * 16);
+            if (rc) return rc;
+        }
+        launch_gather_rhs(st, P->d_perm.p, b_dev + r0 * ldb, ldb, P->d_y.p, n, nr);
+        for (int32_t lv = 0; lv < nl; ++lv)
+            launch_fwd(st, d, P->d_level_fronts.p + P->level_front_ofs[lv],
+                       static_cast<int32_t>(P->level_front_ofs[lv + 1] - P->level_front_ofs[lv]), P->d_y.p, n, nr);
+        for (int32_t lv = nl - 1; lv >= 0; --lv)
+            launch_bwd(st, d, P->d_level_fronts.p + P->level_front_ofs[lv],
+                       static_cast<int32_t>(P->level_front_ofs[lv + 1] - P->level_front_ofs[lv]), P->d_y.p, n, nr);
+        launch_scatter_rhs(st, P->d_perm.p, P->d_y.p, out_dev + r0 * ldo, ldo, n, nr);
+    }
+    HIP_OK(hipGetLastError());
+    return SMFGPU_OK;
+}
+
+SMFGPU_API int smfgpu_solve(smfgpu_factor_t f, double *b, int32_t nrhs, int32_t ldb, int32_t job, int32_t mem) {
+    if (!f || !f->plan || !b) return set_error(SMFGPU_ERR_INVALID_ARG, "null argument");
+    smfgpu_plan_s *P = f->plan;
+    if (P->live != f || !f->valid) return set_error(SMFGPU_ERR_STATE, "solve: factor not usable");
+    if (job != 0) return set_error(SMFGPU_ERR_UNSUPPORTED, "solve: only the full solve (job 0)");
+    if (nrhs <= 0 || ldb < P->n) return set_error(SMFGPU_ERR_INVALID_ARG, "solve: bad nrhs/ldb");
+    hipStream_t st = P->stream;
+    const int32_t n = P->n;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (P->timing) {
+        HIP_OK(hipEventCreate(&e0));
+        HIP_OK(hipEventCreate(&e1));
+    }
+    int rc;
+    if (mem == SMFGPU_MEM_DEVICE) {
+        if (P->timing) HIP_OK(hipEventRecord(e0, st));
+        rc = solve_device(P, b, ldb, b, ldb, nrhs);
+        if (P->timing) HIP_OK(hipEventRecord(e1, st));
+    } else {
+        if (P->d_b.n < static_cast<size_t>(n) * nrhs && (rc = P->d_b.alloc(static_cast<size_t>(n) * nrhs))) return rc;
+        HIP_OK(hipMemcpy2DAsync(P->d_b.p, n * sizeof(double), b, ldb * sizeof(double), n * sizeof(double), nrhs,
+                                hipMemcpyHostToDevice, st));
+        if (P->timing) HIP_OK(hipEventRecord(e0, st));
+        rc = solve_device(P, P->d_b.p, n, P->d_b.p, n, nrhs);
+        if (P->timing) HIP_OK(hipEventRecord(e1, st));
+        HIP_OK(hipMemcpy2DAsync(b, ldb * sizeof(double), P->d_b.p, n * sizeof(double), n * sizeof(double), nrhs,
+                                hipMemcpyDeviceToHost, st));
+    }
+    HIP_OK(hipStreamSynchronize(st));
+    if (P->timing) {
+        float ms = 0;
+        HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+        P->t_solve_ms = ms;
+        double sb = 0;
+        for (const FrontDev &F : P->fronts) sb += 2.0 * (8.0 * F.f * static_cast<double>(F.p) + 4.0 * F.f) + 16.0 * F.f * nrhs;
+        P->solve_bytes = sb;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+    return rc;
+}
+
+SMFGPU_API int smfgpu_refine(smfgpu_factor_t f, const double *values, const double *b, double *x, int32_t nrhs,
+                             int32_t max_iters, double tol, int32_t mem, double *out) {
+    if (!f || !f->plan || !b || !x) return set_error(SMFGPU_ERR_INVALID_ARG, "null argument");
+    smfgpu_plan_s *P = f->plan;
+    if (P->live != f || !f->valid) return set_error(SMFGPU_ERR_STATE, "refine: factor not usable");
+    if (nrhs <= 0) return set_error(SMFGPU_ERR_INVALID_ARG, "refine: bad nrhs");
+    (void)values;  // the factorization's permuted values are already on the device
+    hipStream_t st = P->stream;
+    const int32_t n = P->n;
+    const size_t nn = static_cast<size_t>(n) * nrhs;
+    int rc;
+    if (P->d_x.n < nn && (rc = P->d_x.alloc(nn))) return rc;
+    if (P->d_bb.n < nn && (rc = P->d_bb.alloc(nn))) return rc;
+    if (P->d_r.n < nn && (rc = P->d_r.alloc(nn))) return rc;
+    if (P->d_b.n < nn && (rc = P->d_b.alloc(nn))) return rc;
+    if (P->d_rowstat.n < 4 * nn && (rc = P->d_rowstat.alloc(4 * nn))) return rc;
+    const hipMemcpyKind kin = mem == SMFGPU_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    const hipMemcpyKind kout = mem == SMFGPU_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    // permuted copies of b and x (the residual runs in the permuted numbering)
+    HIP_OK(hipMemcpyAsync(P->d_b.p, b, nn * sizeof(double), kin, st));
+    launch_gather_rhs(st, P->d_perm.p, P->d_b.p, n, P->d_bb.p, n, nrhs);
+    HIP_OK(hipMemcpyAsync(P->d_b.p, x, nn * sizeof(double), kin, st));
+    launch_gather_rhs(st, P->d_perm.p, P->d_b.p, n, P->d_x.p, n, nrhs);
+    // ||A||_inf = max row sum of |A|
+    launch_rowabs(st, n, P->d_fr_ptr.p, P->d_fr_slot.p, P->d_av.p, P->d_r.p);
+    launch_reduce_max(st, P->d_r.p, n, P->d_red.p + 1);
+    double normA = 0.0;
+    HIP_OK(hipMemcpyAsync(&normA, P->d_red.p + 1, sizeof(double), hipMemcpyDeviceToHost, st));
+    std::vector<double> rs(4 * nn);
+    auto eval = [&](double &eta, double &omega) -> int {
+        launch_residual(st, n, P->d_fr_ptr.p, P->d_fr_col.p, P->d_fr_slot.p, P->d_av.p, P->d_x.p, P->d_bb.p,
+                        P->d_r.p, P->d_rowstat.p, nrhs);
+        HIP_OK(hipMemcpyAsync(rs.data(), P->d_rowstat.p, rs.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIP_OK(hipStreamSynchronize(st));
+        eta = 0;
+        omega = 0;
+        for (int32_t c = 0; c < nrhs; ++c) {
+            double rmax = 0, xmax = 0, bmax = 0, om = 0;
+            const double *r = rs.data() + 4 * static_cast<size_t>(c) * n;
+            for (int32_t i = 0; i < n; ++i) {
+                rmax = std::max(rmax, r[4 * i]);
+                xmax = std::max(xmax, r[4 * i + 2]);
+                bmax = std::max(bmax, r[4 * i + 3]);
+                const double den = r[4 * i + 1];
+                if (den > 0) om = std::max(om, r[4 * i] / den);
+                else if (r[4 * i] != 0) om = INFINITY;
+            }
+            const double den = normA * xmax + bmax;
+            eta = std::max(eta, den > 0 ? rmax / den : (rmax == 0 ? 0.0 : INFINITY));
+            omega = std::max(omega, om);
+        }
+        return SMFGPU_OK;
+    };
+    double eta = 0, omega = 0;
+    int steps = 0;
+    if ((rc = eval(eta, omega))) return rc;
+    while (steps < max_iters && eta > tol) {
+        // correction d = A^{-1} r: r is in the permuted numbering, the solve
+        // takes the original one
+        launch_scatter_rhs(st, P->d_perm.p, P->d_r.p, P->d_b.p, n, n, nrhs);
+        if ((rc = solve_device(P, P->d_b.p, n, P->d_b.p, n, nrhs))) return rc;
+        launch_gather_rhs(st, P->d_perm.p, P->d_b.p, n, P->d_r.p, n, nrhs);
+        launch_axpy(st, static_cast<int64_t>(nn), P->d_r.p, P->d_x.p);
+        ++steps;
+        const double eta_prev = eta;
+        if ((rc = eval(eta, omega))) return rc;
+        if (!(eta < 0.5 * eta_prev)) break;  // stalled
+    }
+    launch_scatter_rhs(st, P->d_perm.p, P->d_x.p, P->d_b.p, n, n, nrhs);
+    HIP_OK(hipMemcpyAsync(x, P->d_b.p, nn * sizeof(double), kout, st));
+    HIP_OK(hipStreamSynchronize(st));
+    if (out) {
+        out[0] = eta;
+        out[1] = omega;
+        out[2] = steps;
+    }
+    return SMFGPU_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,129 @@
+// Internal declarations shared by the host orchestration and the HIP
+// kernels of the smfgpu numeric phase.
+#pragma once
+
+#include "../../include/smfgpu.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace smfgpu {
+
+int set_error(int code, const std::string &msg);
+
+// ---------------------------------------------------------------------
+// Host analysis result (the AnalysisKeep fields of the numeric phase).
+struct Analysis {
+    int32_t n = 0, nsup = 0;
+    std::vector<int32_t> sn_start, sn_end, sn_parent;
+    std::vector<int32_t> child_ptr, child_idx;
+    std::vector<int32_t> front_row_ptr, front_row_idx;
+    std::vector<int32_t> pcol_ptr, prow_idx, orig_to_slot, perm;
+    void describe(smfgpu_analysis_desc &d) const;
+};
+
+int analyse(int32_t n, const int32_t *col_ptr, const int32_t *row_idx, const int32_t *perm,
+            int32_t nemin, Analysis &out);
+
+// ---------------------------------------------------------------------
+// Device data layout.
+//
+// Front s (supernode index = postorder position) has f rows, p pivot
+// columns, q = f - p. Its columns [0, p) live in the factor array at lofs
+// (f x p, column-major, leading dimension f: the FactorKeep layout of
+// SolveStep::factor_offset), its trailing q x q contribution block in the
+// CB array at cbofs (column-major, leading dimension q, lower triangle).
+// Element (r, c), r >= c, of the front is therefore
+//   c <  p : L[lofs + c * f + r]
+//   c >= p : CB[cbofs + (c - p) * q + (r - p)].
+struct FrontDev {
+    int32_t f, p, q, col0;  // col0: first pivot column (permuted numbering)
+    int64_t lofs;           // factor offset
+    int64_t cbofs;          // contribution-block offset
+    int64_t rofs;           // row-index offset (rows[rofs .. rofs + f))
+};
+
+// One column of one front to assemble (a workgroup per task).
+struct AsmTask {
+    int32_t s, c;       // front, local column
+    int32_t cc0, cc1;   // range in the child-contribution list
+};
+// A child's CB column that lands in the task's column: child ch's column jj.
+struct ChildCol {
+    int32_t ch, jj;
+};
+
+// Dense-kernel work items (expanded on the host at plan time).
+struct PotrfItem {
+    int32_t s, k, kb, pad;
+};
+struct TrsmItem {
+    int32_t s, k, kb, r0;  // rows [r0, min(r0 + 64, f)) of columns [k, k + kb)
+};
+struct SyrkItem {
+    int32_t s, i0, j0, j1;  // tile rows [i0, i0 + T), cols [j0, min(j0 + T, j1)), i >= j
+    int32_t t0, t1, prec, pad;  // panel columns [t0, t1); prec: 0 fp64, 1 fp32, 2 bf16
+};
+
+enum LaunchKind : int32_t {
+    K_ASSEMBLE = 0,
+    K_POTRF = 1,
+    K_TRSM = 2,
+    K_SYRK = 3,
+};
+
+struct Launch {
+    int32_t kind;
+    int32_t level;
+    int64_t first;  // first item in the kind's item array
+    int64_t count;  // number of items (= workgroups)
+};
+
+// ---------------------------------------------------------------------
+// Kernel launchers (kernels.hip). All asynchronous on `st`.
+struct DevPtrs {
+    const FrontDev *fronts;
+    const int32_t *rows;    // front rows (global permuted numbering)
+    const int32_t *cmap;    // per child: positions of its CB rows in the parent
+    const int64_t *cmofs;   // per supernode: offset into cmap
+    const int32_t *pcol_ptr;
+    const int32_t *amap;    // per permuted entry: local row in its column's front
+    const double *av;       // permuted values
+    double *L;
+    double *CB;
+    int32_t *flag;          // [0]: smallest failing supernode + 1 (SPD); INT32_MAX when none
+    const int8_t *prec;     // per supernode: Schur-update precision (null: all FP64)
+};
+
+void launch_map_values(hipStream_t st, const double *vals, int64_t nvals, const int64_t *slot_ptr,
+                       const int64_t *slot_src, int64_t nslots, double *av, double *redbuf);
+void launch_assemble(hipStream_t st, const DevPtrs &d, const AsmTask *tasks, int64_t ntasks,
+                     const ChildCol *cc);
+void launch_potrf(hipStream_t st, const DevPtrs &d, const PotrfItem *it, int64_t n);
+void launch_trsm(hipStream_t st, const DevPtrs &d, const TrsmItem *it, int64_t n);
+void launch_syrk(hipStream_t st, const DevPtrs &d, const SyrkItem *it, int64_t n);
+void launch_zero_upper(hipStream_t st, const DevPtrs &d, const PotrfItem *it, int64_t n);
+
+// Solves: y is the permuted right-hand side block, n x nrhs (ld n).
+void launch_gather_rhs(hipStream_t st, const int32_t *perm, const double *b, int64_t ldb, double *y,
+                       int32_t n, int32_t nrhs);
+void launch_scatter_rhs(hipStream_t st, const int32_t *perm, const double *y, double *b, int64_t ldb,
+                        int32_t n, int32_t nrhs);
+void launch_fwd(hipStream_t st, const DevPtrs &d, const int32_t *fronts, int32_t nf, double *y,
+                int32_t n, int32_t nrhs);
+void launch_bwd(hipStream_t st, const DevPtrs &d, const int32_t *fronts, int32_t nf, double *y,
+                int32_t n, int32_t nrhs);
+
+// Refinement: full symmetric CSR in permuted numbering (slot into av).
+void launch_residual(hipStream_t st, int32_t n, const int64_t *rp, const int32_t *ci,
+                     const int64_t *slot, const double *av, const double *x, const double *b,
+                     double *r, double *rowstat, int32_t nrhs);
+void launch_rowabs(hipStream_t st, int32_t n, const int64_t *rp, const int64_t *slot, const double *av,
+                   double *out);
+void launch_axpy(hipStream_t st, int64_t n, const double *d, double *x);
+void launch_reduce_max(hipStream_t st, const double *v, int64_t n, double *out);
+
+}  // namespace smfgpu

@@ -1,0 +1,204 @@
+"""Host-side mirror of smf's analyse / factor / solve split over the smfgpu
+C ABI (include/smfgpu.h).
+
+Mirrors ``smf::Solver`` (solver/include/smf/solver.hpp:60-127 of the
+reference): ``analyse`` gives an analysis (the AnalysisKeep fields the
+numeric phase reads), ``Plan`` is the per-analysis device set-up, ``factor``
+runs the numeric factorization on the GPU and returns a ``Factor`` whose
+``solve`` / ``refine`` run on the GPU as well. Status values are
+smf::FactorStatus's; failures raise ``SmfGpuError``."""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import MEM_DEVICE, MEM_HOST, PREC_BF16, PREC_FP32, PREC_FP64, SmfGpuError, check, lib
+
+__all__ = ["analyse", "Analysis", "Plan", "Factor", "SmfGpuError", "PREC_FP64", "PREC_FP32", "PREC_BF16"]
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+class Analysis:
+    """Result of the host symbolic analysis (``smfgpu_analyse``)."""
+
+    def __init__(self, A, perm=None, nemin=32):
+        n, cp, ri, _ = A
+        self.n = int(n)
+        cp = _i32(cp)
+        ri = _i32(ri)
+        pm = None if perm is None else _i32(perm)
+        h = ctypes.c_void_p()
+        check(lib().smfgpu_analyse(self.n, _p(cp), _p(ri), _p(pm), nemin, ctypes.byref(h)), "analyse")
+        self.h = h
+        st = np.zeros(6)
+        check(lib().smfgpu_analysis_stats(self.h, _p(st)), "analysis_stats")
+        self.nsup, self.front_rows, self.factor_entries, self.max_front, self.flops, self.depth = (
+            int(st[0]), int(st[1]), int(st[2]), int(st[3]), float(st[4]), int(st[5]))
+        self.desc = _lib.AnalysisDesc()
+        check(lib().smfgpu_analysis_describe(self.h, ctypes.byref(self.desc)), "analysis_describe")
+
+    def arrays(self):
+        """Copies of the AnalysisKeep-style arrays."""
+        d = self.desc
+        ns, n = d.nsup, d.n
+
+        def arr(ptr, count, dt=np.int32):
+            if count == 0:
+                return np.zeros(0, dt)
+            return np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctypes.c_int32)), shape=(count,)).copy()
+
+        cp = arr(d.child_ptr, ns + 1)
+        fp = arr(d.front_row_ptr, ns + 1)
+        pc = arr(d.pcol_ptr, n + 1)
+        return dict(n=n, sn_start=arr(d.sn_start, ns), sn_end=arr(d.sn_end, ns), sn_parent=arr(d.sn_parent, ns),
+                    child_ptr=cp, child_idx=arr(d.child_idx, int(cp[-1])), front_row_ptr=fp,
+                    front_row_idx=arr(d.front_row_idx, int(fp[-1])), pcol_ptr=pc,
+                    prow_idx=arr(d.prow_idx, int(pc[-1])), orig_to_slot=arr(d.orig_to_slot, int(d.n_orig)),
+                    perm=arr(d.perm, n), postorder=np.arange(ns, dtype=np.int32))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().smfgpu_analysis_free(ctypes.byref(self.h))
+            self.h = None
+
+
+def analyse(A, perm=None, nemin=32) -> Analysis:
+    return Analysis(A, perm=perm, nemin=nemin)
+
+
+class Plan:
+    """Device plan of one analysis (``smfgpu_plan_create``)."""
+
+    def __init__(self, analysis, posdef=True):
+        if isinstance(analysis, Analysis):
+            self._owner = analysis
+            desc = analysis.desc
+            self.n = analysis.n
+        else:  # dict of AnalysisKeep arrays (e.g. exported from smf's own analysis)
+            a = {k: (np.ascontiguousarray(v, np.int32) if isinstance(v, np.ndarray) else v)
+                 for k, v in analysis.items()}
+            self._owner = a
+            desc = _lib.AnalysisDesc()
+            desc.n = int(a["n"])
+            desc.nsup = int(a["sn_start"].size)
+            for k in ("sn_start", "sn_end", "sn_parent", "child_ptr", "child_idx", "front_row_ptr",
+                      "front_row_idx", "pcol_ptr", "prow_idx", "orig_to_slot", "perm"):
+                setattr(desc, k, _p(a[k]) if a[k].size else None)
+            desc.n_orig = int(a["orig_to_slot"].size)
+            self.n = desc.n
+        self._desc = desc
+        self.posdef = posdef
+        h = ctypes.c_void_p()
+        check(lib().smfgpu_plan_create(ctypes.byref(desc), 1 if posdef else 0, ctypes.byref(h)), "plan_create")
+        self.h = h
+        self.n_orig = int(desc.n_orig)
+
+    def stats(self):
+        st = np.zeros(8)
+        check(lib().smfgpu_plan_stats(self.h, _p(st)), "plan_stats")
+        keys = ("device_bytes", "levels", "launches", "flops", "factor_entries", "cb_entries", "asm_bytes",
+                "solve_bytes_per_rhs")
+        return dict(zip(keys, st.tolist()))
+
+    def set_timing(self, on=True):
+        check(lib().smfgpu_set_timing(self.h, 1 if on else 0), "set_timing")
+
+    def kernel_times(self):
+        st = np.zeros(8)
+        check(lib().smfgpu_kernel_times(self.h, _p(st)), "kernel_times")
+        keys = ("syrk_ms", "syrk_launches", "syrk_flops", "asm_ms", "asm_bytes", "solve_ms", "solve_bytes")
+        return dict(zip(keys, st[:7].tolist()))
+
+    def factor(self, values=None, values_ptr=None, precision=PREC_FP64, mixed_min_front=1024, u=0.01,
+               small_pivot=1e-20, panel=16, max_negative_pivots=-1) -> "Factor":
+        opt = _lib.FactorOptions()
+        lib().smfgpu_default_options(ctypes.byref(opt))
+        opt.posdef = 1 if self.posdef else 0
+        opt.pivot_u = u
+        opt.small_pivot = small_pivot
+        opt.panel = panel
+        opt.precision = precision
+        opt.mixed_min_front = mixed_min_front
+        opt.max_negative_pivots = max_negative_pivots
+        if values_ptr is not None:
+            opt.value_mem = MEM_DEVICE
+            vp = ctypes.c_void_p(values_ptr)
+            keep = None
+        else:
+            keep = np.ascontiguousarray(values, np.float64)
+            if keep.size != self.n_orig:
+                raise ValueError(f"expected {self.n_orig} values, got {keep.size}")
+            opt.value_mem = MEM_HOST
+            vp = _p(keep)
+        info = _lib.FactorInfo()
+        h = ctypes.c_void_p()
+        check(lib().smfgpu_factor(self.h, vp, self.n_orig, ctypes.byref(opt), ctypes.byref(h), ctypes.byref(info)),
+              "factor")
+        return Factor(self, h, info)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().smfgpu_plan_free(ctypes.byref(self.h))
+            self.h = None
+
+
+class Factor:
+    def __init__(self, plan: Plan, h, info):
+        self.plan = plan
+        self.h = h
+        self.info = info
+        self.status = int(info.status)
+        self.inertia = (int(info.num_positive), int(info.num_negative), int(info.num_zero))
+        self.flops = float(info.actual_flops)
+        self.seconds = float(info.factor_seconds)
+
+    def solve(self, b):
+        b = np.array(b, dtype=np.float64, order="F", copy=True)
+        nrhs = 1 if b.ndim == 1 else b.shape[1]
+        check(lib().smfgpu_solve(self.h, _p(b), nrhs, self.plan.n, 0, MEM_HOST), "solve")
+        return b
+
+    def solve_device(self, ptr, nrhs, ldb=None):
+        check(lib().smfgpu_solve(self.h, ctypes.c_void_p(ptr), nrhs, ldb or self.plan.n, 0, MEM_DEVICE),
+              "solve")
+
+    def refine(self, b, x, max_iters=10, tol=1e-14):
+        b = np.array(b, dtype=np.float64, order="F", copy=True)
+        x = np.array(x, dtype=np.float64, order="F", copy=True)
+        nrhs = 1 if b.ndim == 1 else b.shape[1]
+        out = np.zeros(3)
+        check(lib().smfgpu_refine(self.h, None, _p(b), _p(x), nrhs, max_iters, tol, MEM_HOST, _p(out)), "refine")
+        return x, dict(eta=float(out[0]), omega=float(out[1]), steps=int(out[2]))
+
+    def refine_device(self, b_ptr, x_ptr, nrhs, max_iters=10, tol=1e-14):
+        out = np.zeros(3)
+        check(lib().smfgpu_refine(self.h, None, ctypes.c_void_p(b_ptr), ctypes.c_void_p(x_ptr), nrhs, max_iters,
+                                  tol, MEM_DEVICE, _p(out)), "refine")
+        return dict(eta=float(out[0]), omega=float(out[1]), steps=int(out[2]))
+
+    def export(self):
+        """The FactorKeep layout: steps (row_offset, factor_offset, p, f per
+        supernode), factor values, row indices, pivot types."""
+        sz = np.zeros(4, np.int64)
+        check(lib().smfgpu_export_factor(self.h, _p(sz), None, None, None, None), "export sizes")
+        ns, nf, nr, n = (int(v) for v in sz)
+        steps = np.zeros((ns, 4), np.int64)
+        fv = np.zeros(nf)
+        rows = np.zeros(nr, np.int32)
+        pt = np.zeros(n, np.int8)
+        check(lib().smfgpu_export_factor(self.h, _p(sz), _p(steps), _p(fv), _p(rows), _p(pt)), "export")
+        return dict(steps=steps, factor_values=fv, row_indices=rows, pivot_types=pt)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().smfgpu_factor_free(ctypes.byref(self.h))
+            self.h = None
