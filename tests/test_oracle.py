@@ -1,0 +1,203 @@
+"""The oracle (oracle/oracle.c) pinned against the reference's known-answer
+tests, closed forms and, when it is built, the reference itself."""
+import numpy as np
+import pytest
+
+from tests.helpers import reconstruct_front
+
+
+def _lower(M):
+    return np.tril(np.asarray(M, dtype=np.float64))
+
+
+# --- known answers of solver/tests/test_inertia_dense.cpp:6-60 and
+# test_ldlt_dense.cpp:18-48 (restated on the front kernel) ---------------
+@pytest.mark.parametrize("A,inertia", [
+    ([[4, 2], [2, 3]], (2, 0, 0)),                       # LDLT.SPD2x2
+    ([[-4, -2], [-2, -3]], (0, 2, 0)),                    # LDLT.AllNegative2x2
+    ([[0, 1], [1, 0]], (1, 1, 0)),                        # AddInertia2x2_Saddle
+    ([[2, 0], [0, 3]], (2, 0, 0)),                        # BothPos
+    ([[-2, 0], [0, -3]], (0, 2, 0)),                      # BothNeg
+    ([[4, 2, 1], [2, 5, 2], [1, 2, 6]], (3, 0, 0)),       # SPD3x3_Inertia
+    ([[2, 1, 0], [1, 0, 1], [0, 1, 2]], (2, 1, 0)),       # LDLT.Indefinite3x3 (det -4)
+])
+def test_inertia_known_answers(oracle_lib, A, inertia):
+    A = np.array(A, dtype=float)
+    _, _, tags, st = oracle_lib.ldlt_front(_lower(A), A.shape[0], u=0.01)
+    assert st["p"] == A.shape[0]
+    assert (st["pos"], st["neg"], st["zero"]) == inertia
+    # Sylvester: the same counts as the eigenvalues
+    ev = np.linalg.eigvalsh(A)
+    assert (int((ev > 0).sum()), int((ev < 0).sum())) == inertia[:2]
+
+
+@pytest.mark.parametrize("scale", [1e-100, 1e100])
+def test_inertia_extreme_scaling(oracle_lib, scale):
+    A = np.array([[0.0, scale], [scale, 0.0]])
+    # eff_small as factor_indef sets it (factor_indef.cpp:633-636): small *
+    # max|A| when max|A| < small
+    eff = 1e-20 * scale if scale < 1e-20 else 1e-20
+    _, _, tags, st = oracle_lib.ldlt_front(_lower(A), 2, u=0.01, eff_small=eff)
+    assert (st["pos"], st["neg"], st["zero"]) == (1, 1, 0)
+    assert list(tags) == [2, -1]
+
+
+def test_tiny_u_prefers_2x2_over_near_null_partner(oracle_lib):
+    # test_inertia_dense.cpp: TinyUPrefers2x2OverNearNullPartner
+    A = np.array([[0.0, 1.0, 0.0], [1.0, 2e-6, 100.0], [0.0, 100.0, 1.0]])
+    _, _, tags, st = oracle_lib.ldlt_front(_lower(A), 3, u=1e-8)
+    assert tags[0] == 2
+    assert (st["pos"], st["neg"], st["zero"]) == (2, 1, 0)
+
+
+def _weak_front(d00, a21):
+    A = np.zeros((4, 4))
+    A[0, 0] = d00
+    A[1, 0] = 1.0
+    A[2, 0] = 0.1
+    A[3, 0] = 0.05
+    A[1, 1] = 1e-3
+    A[2, 1] = a21
+    A[3, 1] = 0.1
+    A[2, 2] = 4.0
+    A[3, 2] = 0.3
+    A[3, 3] = -5.0
+    return A
+
+
+@pytest.mark.parametrize("d00,a21,u,first", [
+    (5e-8, 0.2, 1e-8, 2),    # weak 1x1 gives way to the stable 2x2
+    (5e-8, 3e7, 1e-8, 1),    # 2x2 growth larger: the weak 1x1 stays
+    (0.05, 0.2, 0.01, 1),    # not weak
+    (0.05, 0.2, 1e-8, 1),
+    (5e-8, 0.2, 0.01, 2),    # u = 0.01: the weak diagonal fails outright
+])
+def test_weak_one_by_one_rule(oracle_lib, d00, a21, u, first):
+    # test_ldlt_dense.cpp:232-272
+    _, _, tags, _ = oracle_lib.ldlt_front(_weak_front(d00, a21), 4, u=u)
+    assert tags[0] == first
+
+
+def _random_front(rng, f, diag_scale=0.3):
+    M = rng.standard_normal((f, f))
+    M = M + M.T
+    M[np.arange(f), np.arange(f)] *= rng.uniform(0, diag_scale, f)
+    return M
+
+
+@pytest.mark.parametrize("f,ne,nb", [(12, 12, 16), (30, 20, 16), (80, 50, 16), (130, 90, 16), (160, 160, 8)])
+def test_front_reconstruction_and_schur(oracle_lib, f, ne, nb):
+    """P^T A P = L D L^T on the eliminated block and the Schur complement of
+    the rest equals A22 - A21 A11^-1 A12 (dense LAPACK): closed forms that a
+    dropped term, wrong sign or transposed operand fails."""
+    rng = np.random.default_rng(f * 7 + ne)
+    A = _random_front(rng, f)
+    Fo, rows, tags, st = oracle_lib.ldlt_front(_lower(A), ne, nb=nb)
+    p = st["p"]
+    P = A[np.ix_(rows, rows)]
+    L, D = reconstruct_front(Fo, tags, p)
+    # eliminated columns
+    R = L @ D @ L.T
+    scale = np.abs(A).max()
+    assert np.abs((P - R)[:, :p]).max() / scale < 1e-11
+    # Schur complement
+    if p < f:
+        S = P[p:, p:] - P[p:, :p] @ np.linalg.solve(P[:p, :p], P[:p, p:])
+        Sk = np.tril(Fo[p:, p:])
+        assert np.abs(np.tril(S) - Sk).max() / max(np.abs(S).max(), 1.0) < 1e-9
+    # inertia of the eliminated block (Sylvester)
+    ev = np.linalg.eigvalsh(P[:p, :p])
+    assert (st["pos"], st["neg"]) == (int((ev > 0).sum()), int((ev < 0).sum()))
+
+
+def test_stalled_panel_pulls_partners(oracle_lib):
+    """test_ldlt_dense.cpp:274-414: constraint columns with zero diagonal
+    coupled only to far variable columns; every eligible column is accepted
+    with the block's inertia."""
+    m, nv, q, kfar = 40, 120, 40, 100
+    e = m + nv
+    f = e + q
+    rng = np.random.default_rng(5)
+    A = np.zeros((f, f))
+    B = rng.standard_normal((nv, nv))
+    A[m:e, m:e] = B @ B.T + nv * np.eye(nv)
+    for c in range(m):
+        v = m + (c + kfar - m) % nv if c + kfar >= m else m + c
+        v = m + ((c + kfar) % nv)
+        A[v, c] = A[c, v] = 1.0 + rng.uniform()
+    A[e:, :e] = 0.01 * rng.standard_normal((q, e))
+    A[:e, e:] = A[e:, :e].T
+    A[e:, e:] = np.eye(q)
+    _, _, tags, st = oracle_lib.ldlt_front(_lower(A), e, nb=16)
+    assert st["p"] == e
+    ev = np.linalg.eigvalsh(A[:e, :e])
+    assert (st["pos"], st["neg"], st["zero"]) == (int((ev > 0).sum()), int((ev < 0).sum()), 0)
+
+
+def test_backward_error_closed_form(oracle_lib):
+    # A = [[2, 1], [1, 3]] (lower CSC), x = [1, 1], b = A x + [1e-3, 0]
+    A = (2, np.array([0, 2, 3], np.int32), np.array([0, 1, 1], np.int32), np.array([2.0, 1.0, 3.0]))
+    x = np.array([1.0, 1.0])
+    b = np.array([3.0 + 1e-3, 4.0])
+    eta, om = oracle_lib.backward_error(A, b, x)
+    # ||r|| = 1e-3, ||A||_inf = 4, ||x|| = 1, ||b|| = max(3.001, 4) = 4
+    assert abs(eta - 1e-3 / (4 * 1 + 4)) < 1e-15
+    # omega = max_i |r_i| / ((|A||x|)_i + |b_i|) = 1e-3 / (3 + 3.001)
+    assert abs(om - 1e-3 / (3 + 3.001)) < 1e-15
+
+
+# --- the oracle against the compiled reference -------------------------
+@pytest.mark.parametrize("f,ne", [(20, 12), (60, 60), (90, 70), (140, 100), (200, 150)])
+def test_front_kernel_matches_reference(oracle_lib, ref_lib, f, ne):
+    rng = np.random.default_rng(f + ne)
+    for _ in range(4):
+        A = _random_front(rng, f)
+        Fr, rr, tr, sr = ref_lib.ldlt_front(_lower(A), ne, nb=16)
+        Fo, ro, to, so = oracle_lib.ldlt_front(_lower(A), ne, nb=16)
+        assert np.array_equal(rr, ro) and np.array_equal(tr, to) and sr["p"] == so["p"]
+        assert (sr["pos"], sr["neg"], sr["zero"]) == (so["pos"], so["neg"], so["zero"])
+        assert sr["flops"] == so["flops"]
+        d = np.abs(np.tril(Fr) - np.tril(Fo)).max() / np.abs(Fr).max()
+        if f < 96:
+            assert d == 0.0  # the unblocked path is bitwise the reference's
+        else:
+            assert d < 1e-10
+
+
+def _cases(g, R):
+    return [
+        ("laplace7_12", g.laplace7_3d(12), True, R.ORDER_METIS),
+        ("ocp_T10", g.ocp_kkt(10, seed=0), False, R.ORDER_AMD),
+        ("ocp_T40", g.ocp_kkt(40, seed=3), False, R.ORDER_AMD),
+        ("kkt_20", g.kkt_grid(20, 60, seed=1)[0], False, R.ORDER_METIS),
+    ]
+
+
+def test_multifrontal_matches_reference(oracle_lib, ref_lib):
+    from opt_solver_amd import generators as g
+    for name, A, posdef, order in _cases(g, ref_lib):
+        an = ref_lib.RefAnalysis(A, ordering=order, posdef=posdef)
+        rf = ref_lib.RefFactor(an, A[3])
+        of = oracle_lib.Factor(an.plan_arrays(), A[3], posdef=posdef)
+        ex, eo = rf.export(), of.export()
+        assert rf.status == of.status == 0, name
+        assert np.array_equal(ex["steps"], eo["steps"]), name
+        assert np.array_equal(ex["row_indices"], eo["row_indices"]), name
+        if not posdef:
+            assert np.array_equal(ex["pivot_types"], eo["pivot_types"]), name
+            assert rf.inertia == of.inertia and rf.delayed == of.delayed and rf.two_by_two == of.two_by_two
+        assert rf.flops == of.flops
+        d = np.abs(ex["factor_values"] - eo["factor_values"]).max() / np.abs(ex["factor_values"]).max()
+        assert d < 1e-12, (name, d)
+
+
+def test_kkt_inertia_sylvester(oracle_lib):
+    """[[H, A^T], [A, -dI]] with H SPD and A of full row rank has inertia
+    (n_x, m, 0); the OCP KKT (16T+12, 12T, 0)."""
+    from opt_solver_amd import generators as g
+    from opt_solver_amd.solver import analyse  # host analysis only: no GPU needed
+    cases = [(g.kkt_grid(16, 40, seed=2)[0], (256, 40, 0)), (g.ocp_kkt(12, seed=1), (16 * 12 + 12, 12 * 12, 0))]
+    for A, inertia in cases:
+        an = analyse(A, perm=None, nemin=8)
+        of = oracle_lib.Factor(an.arrays(), A[3], posdef=False)
+        assert of.inertia == inertia
