@@ -55,8 +55,9 @@ EXPORTED = [
     "smfgpu_error_string", "smfgpu_default_options", "smfgpu_analyse", "smfgpu_analysis_describe",
     "smfgpu_analysis_stats", "smfgpu_analysis_free", "smfgpu_plan_create", "smfgpu_plan_free",
     "smfgpu_plan_stats", "smfgpu_factor", "smfgpu_factor_free", "smfgpu_export_factor", "smfgpu_solve",
-    "smfgpu_refine", "smfgpu_set_timing", "smfgpu_kernel_times",
+    "smfgpu_refine", "smfgpu_set_timing", "smfgpu_kernel_times", "smfgpu_plan_tune",
 ]
+TUNE_SOLVE_BIG_MIN = 1
 
 _lib = None
 
@@ -88,6 +89,7 @@ def lib():
     L.smfgpu_refine.argtypes = [_P, _P, _P, _P, _I, _I, _D, _I, _P]
     L.smfgpu_set_timing.argtypes = [_P, _I]
     L.smfgpu_kernel_times.argtypes = [_P, _P]
+    L.smfgpu_plan_tune.argtypes = [_P, _I, _L]
     _lib = L
     return L
 

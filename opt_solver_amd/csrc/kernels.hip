@@ -151,44 +151,43 @@ __global__ __launch_bounds__(kThreads) void k_potrf(DevPtrs d, const PotrfItem *
     const int f = F.f, k = it.k, kb = it.kb;
     double *base = d.L + F.lofs;
     const int tid = threadIdx.x;
-    for (int idx = tid; idx < kb * kb; idx += kThreads) {
-        const int c = idx / kb, r = idx % kb;
-        A[r][c] = (r >= c) ? base[static_cast<int64_t>(k + c) * f + k + r] : 0.0;
-        Inv[r][c] = 0.0;
+    for (int idx = tid; idx < kb * kNb; idx += kThreads) {
+        const int c = idx / kNb, r = idx % kNb;
+        A[r][c] = (r < kb && r >= c) ? base[static_cast<int64_t>(k + c) * f + k + r] : 0.0;
+        Inv[r][c] = (r == c) ? 1.0 : 0.0;
     }
     __syncthreads();
+    // Column j: scale it by 1/sqrt(a_jj) and scale row j of Inv (the
+    // row operations of the elimination applied to the identity give
+    // inv(L_kk)); then the rank-1 update of the trailing triangle and of
+    // the rows of Inv below j.
+    // A[j][j] keeps d_jj until the end (no thread writes it while others
+    // read it); its square root is stored on the way out.
     for (int j = 0; j < kb; ++j) {
-        if (tid == 0) {
-            const double dj = A[j][j];
-            if (!(dj > 0.0 && dj <= DBL_MAX)) atomicMin(d.flag, it.s + 1);
-            A[j][j] = sqrt(dj);
-        }
         __syncthreads();
         const double djj = A[j][j];
-        for (int r = j + 1 + tid; r < kb; r += kThreads) A[r][j] /= djj;
+        const double rs = 1.0 / sqrt(djj);
+        if (tid == 0 && !(djj > 0.0 && djj <= DBL_MAX)) atomicMin(d.flag, it.s + 1);
+        for (int r = j + 1 + tid; r < kb; r += kThreads) A[r][j] *= rs;
+        for (int c = tid; c <= j; c += kThreads) Inv[j][c] *= rs;
         __syncthreads();
-        const int n2 = kb - j - 1;
-        for (int idx = tid; idx < n2 * n2; idx += kThreads) {
-            const int c = j + 1 + idx / n2, r = j + 1 + idx % n2;
+        const int m = kb - j - 1;
+        for (int idx = tid; idx < m * m; idx += kThreads) {
+            const int c = j + 1 + idx / m, r = j + 1 + idx % m;
             if (r >= c) A[r][c] -= A[r][j] * A[c][j];
         }
-        __syncthreads();
-    }
-    // inverse of L_kk, one column per thread
-    if (tid < kb) {
-        const int c = tid;
-        Inv[c][c] = 1.0 / A[c][c];
-        for (int i = c + 1; i < kb; ++i) {
-            double s = 0.0;
-            for (int t = c; t < i; ++t) s += A[i][t] * Inv[t][c];
-            Inv[i][c] = -s / A[i][i];
+        for (int idx = tid; idx < m * (j + 1); idx += kThreads) {
+            const int r = j + 1 + idx / (j + 1), c = idx % (j + 1);
+            Inv[r][c] -= A[r][j] * Inv[j][c];
         }
     }
     __syncthreads();
+    // lower: L_kk (diagonal sqrt(d_jj)); strict upper (r < c):
+    // M[r][c] = inv(L_kk)[c][r]
     for (int idx = tid; idx < kb * kb; idx += kThreads) {
         const int c = idx / kb, r = idx % kb;
-        // lower: L_kk; strict upper (r < c): M[r][c] = Inv[c][r]
-        base[static_cast<int64_t>(k + c) * f + k + r] = (r >= c) ? A[r][c] : Inv[c][r];
+        const double v = (r > c) ? A[r][c] : (r == c ? sqrt(A[r][r]) : Inv[c][r]);
+        base[static_cast<int64_t>(k + c) * f + k + r] = v;
     }
 }
 
@@ -246,11 +245,103 @@ void launch_trsm(hipStream_t st, const DevPtrs &d, const TrsmItem *it, int64_t n
 // accumulation over the panel, subtracted from the FP64 Schur complement.
 constexpr int kT = 128;
 constexpr int kKc = 16;
+constexpr int kLoadPer = (kKc * kT) / kThreads;  // elements of A (and of B) per thread per chunk
 
-__global__ __launch_bounds__(kThreads) void k_syrk(DevPtrs d, const SyrkItem *__restrict__ items) {
-    __shared__ double As[kKc][kT];
-    __shared__ double Bs[kKc][kT];
-    const SyrkItem it = items[blockIdx.x];
+template <typename E>
+struct SyrkMfma;
+template <>
+struct SyrkMfma<double> {
+    typedef v4d acc_t;
+    static __device__ __forceinline__ acc_t zero() { return v4d{0.0, 0.0, 0.0, 0.0}; }
+    static __device__ __forceinline__ acc_t mfma(double a, double b, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    // row (within the 16 x 16 block) of accumulator value v in this lane
+    static __device__ __forceinline__ int row(int v, int lane) { return 4 * v + (lane >> 4); }
+};
+template <>
+struct SyrkMfma<float> {
+    typedef v4f acc_t;
+    static __device__ __forceinline__ acc_t zero() { return v4f{0.f, 0.f, 0.f, 0.f}; }
+    static __device__ __forceinline__ acc_t mfma(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int v, int lane) { return 4 * (lane >> 4) + v; }
+};
+
+// One 128 x 128 tile, operands of type E staged in a double-buffered LDS
+// ring (global loads of chunk k+1 in flight during the MFMAs of chunk k).
+// Loads are branch-free (indices clamped into the front, out-of-range
+// values selected to zero); waves wholly above the diagonal run the same
+// loop without MFMAs (ACTIVE = false), so the accumulators stay in AGPRs.
+template <typename E, bool ACTIVE>
+__device__ __forceinline__ void syrk_loop(const SyrkItem &it, const double *Lb, int f, int j1, unsigned char *smem,
+                                          typename SyrkMfma<E>::acc_t (&acc)[4][4]) {
+    typedef SyrkMfma<E> M;
+    E (*As)[kKc][kT] = reinterpret_cast<E (*)[kKc][kT]>(smem);                            // [2][kKc][kT]
+    E (*Bs)[kKc][kT] = reinterpret_cast<E (*)[kKc][kT]>(smem + 2 * kKc * kT * sizeof(E));  // [2][kKc][kT]
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int w = tid >> 6;
+    const int wr = w >> 1, wc = w & 1;
+    const int i0 = it.i0, j0 = it.j0;
+    double ra[kLoadPer], rb[kLoadPer];
+    auto gload = [&](int kc) {
+#pragma unroll
+        for (int e = 0; e < kLoadPer; ++e) {
+            const int idx = tid + kThreads * e;
+            const int t = idx / kT, r = idx % kT;
+            const int tt = kc + t;
+            const bool tin = tt < it.t1;
+            const int ttc = tin ? tt : it.t1 - 1;
+            const int ia = min(i0 + r, f - 1);
+            const int jb = min(j0 + r, j1 - 1);
+            const double va = Lb[static_cast<int64_t>(ttc) * f + ia];
+            const double vb = Lb[static_cast<int64_t>(ttc) * f + jb];
+            ra[e] = (tin && i0 + r < f) ? va : 0.0;
+            rb[e] = (tin && j0 + r < j1) ? vb : 0.0;
+        }
+    };
+    auto sstore = [&](int stg) {
+#pragma unroll
+        for (int e = 0; e < kLoadPer; ++e) {
+            const int idx = tid + kThreads * e;
+            const int t = idx / kT, r = idx % kT;
+            As[stg][t][r] = static_cast<E>(ra[e]);
+            Bs[stg][t][r] = static_cast<E>(rb[e]);
+        }
+    };
+    gload(it.t0);
+    sstore(0);
+    __syncthreads();
+    int stg = 0;
+    for (int kc = it.t0; kc < it.t1; kc += kKc) {
+        const bool more = kc + kKc < it.t1;
+        if (more) gload(kc + kKc);
+        if (ACTIVE) {
+#pragma unroll
+            for (int kk = 0; kk < kKc; kk += 4) {
+                E a[4], b[4];
+                const int kr = kk + (lane >> 4);
+#pragma unroll
+                for (int m = 0; m < 4; ++m) a[m] = Bs[stg][kr][wc * 64 + m * 16 + (lane & 15)];
+#pragma unroll
+                for (int nn = 0; nn < 4; ++nn) b[nn] = As[stg][kr][wr * 64 + nn * 16 + (lane & 15)];
+#pragma unroll
+                for (int m = 0; m < 4; ++m)
+#pragma unroll
+                    for (int nn = 0; nn < 4; ++nn) acc[m][nn] = M::mfma(a[m], b[nn], acc[m][nn]);
+            }
+        }
+        if (more) sstore(stg ^ 1);
+        __syncthreads();
+        stg ^= 1;
+    }
+}
+
+template <typename E>
+__device__ __forceinline__ void syrk_tile(const DevPtrs &d, const SyrkItem &it, unsigned char *smem) {
+    typedef SyrkMfma<E> M;
     const FrontDev F = d.fronts[it.s];
     const int f = F.f;
     const int i0 = it.i0, j0 = it.j0;
@@ -260,64 +351,14 @@ __global__ __launch_bounds__(kThreads) void k_syrk(DevPtrs d, const SyrkItem *__
     const int lane = tid & 63;
     const int w = tid >> 6;
     const int wr = w >> 1, wc = w & 1;
-    // waves whose sub-tile lies wholly above the diagonal do no MFMA work
     const bool active = (i0 + wr * 64 + 63) >= (j0 + wc * 64);
-    v4d acc[4][4];
-    v4f accf[4][4];
+    typename M::acc_t acc[4][4];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
-            accf[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
-        }
-    const int prec = d.prec != nullptr ? d.prec[it.s] : 0;
-    for (int kc = it.t0; kc < it.t1; kc += kKc) {
-#pragma unroll
-        for (int e = 0; e < (kKc * kT) / kThreads; ++e) {
-            const int idx = tid + kThreads * e;
-            const int t = idx / kT, r = idx % kT;
-            const int tt = kc + t;
-            const bool tin = tt < it.t1;
-            As[t][r] = (tin && i0 + r < f) ? Lb[static_cast<int64_t>(tt) * f + i0 + r] : 0.0;
-            Bs[t][r] = (tin && j0 + r < j1) ? Lb[static_cast<int64_t>(tt) * f + j0 + r] : 0.0;
-        }
-        __syncthreads();
-        if (active) {
-            if (prec == 0) {
-#pragma unroll
-                for (int kk = 0; kk < kKc; kk += 4) {
-                    double a[4], b[4];
-                    const int kr = kk + (lane >> 4);
-#pragma unroll
-                    for (int m = 0; m < 4; ++m) a[m] = Bs[kr][wc * 64 + m * 16 + (lane & 15)];
-#pragma unroll
-                    for (int nn = 0; nn < 4; ++nn) b[nn] = As[kr][wr * 64 + nn * 16 + (lane & 15)];
-#pragma unroll
-                    for (int m = 0; m < 4; ++m)
-#pragma unroll
-                        for (int nn = 0; nn < 4; ++nn)
-                            acc[m][nn] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[m], b[nn], acc[m][nn], 0, 0, 0);
-                }
-            } else {
-#pragma unroll
-                for (int kk = 0; kk < kKc; kk += 4) {
-                    float a[4], b[4];
-                    const int kr = kk + (lane >> 4);
-#pragma unroll
-                    for (int m = 0; m < 4; ++m) a[m] = static_cast<float>(Bs[kr][wc * 64 + m * 16 + (lane & 15)]);
-#pragma unroll
-                    for (int nn = 0; nn < 4; ++nn) b[nn] = static_cast<float>(As[kr][wr * 64 + nn * 16 + (lane & 15)]);
-#pragma unroll
-                    for (int m = 0; m < 4; ++m)
-#pragma unroll
-                        for (int nn = 0; nn < 4; ++nn)
-                            accf[m][nn] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[m], b[nn], accf[m][nn], 0, 0, 0);
-                }
-            }
-        }
-        __syncthreads();
-    }
+        for (int b = 0; b < 4; ++b) acc[a][b] = M::zero();
+    if (active) syrk_loop<E, true>(it, Lb, f, j1, smem, acc);
+    else syrk_loop<E, false>(it, Lb, f, j1, smem, acc);
     if (!active) return;
 #pragma unroll
     for (int m = 0; m < 4; ++m)
@@ -325,16 +366,26 @@ __global__ __launch_bounds__(kThreads) void k_syrk(DevPtrs d, const SyrkItem *__
         for (int nn = 0; nn < 4; ++nn)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                // D row of value v in lane: FP64 16x16x4 holds rows 4 v + lane / 16,
-                // FP32 16x16x4 rows 4 (lane / 16) + v (both: column lane % 16).
-                const int mrow = prec == 0 ? 4 * v + (lane >> 4) : 4 * (lane >> 4) + v;
-                const int j = j0 + wc * 64 + m * 16 + mrow;
+                const int j = j0 + wc * 64 + m * 16 + M::row(v, lane);
                 const int i = i0 + wr * 64 + nn * 16 + (lane & 15);
-                if (i < f && j < j1 && i >= j) {
-                    const double u = prec == 0 ? acc[m][nn][v] : static_cast<double>(accf[m][nn][v]);
-                    col_base(d, F, j)[i] -= u;
-                }
+                if (i < f && j < j1 && i >= j) col_base(d, F, j)[i] -= static_cast<double>(acc[m][nn][v]);
             }
+}
+
+// A10/A11: Schur-complement update on MFMA. Tile 128 x 128 of
+// C[i, j] -= sum_{t in [t0, t1)} L[i, t] L[j, t], i >= j, four waves of
+// 64 x 64, each 4 x 4 MFMA 16x16x4 blocks. The MFMA computes the
+// transposed block (A operand = the j rows, B operand = the i rows) so that
+// the 16 lanes of a row group hold 16 consecutive rows of one column on the
+// store. Per-front precision: FP64 MFMA, or operands rounded to FP32 with
+// FP32 MFMA accumulation over the panel, subtracted from the FP64 Schur
+// complement.
+__global__ __launch_bounds__(kThreads) void k_syrk(DevPtrs d, const SyrkItem *__restrict__ items) {
+    __shared__ __attribute__((aligned(16))) unsigned char smem[4 * kKc * kT * sizeof(double)];
+    const SyrkItem it = items[blockIdx.x];
+    const int prec = d.prec != nullptr ? d.prec[it.s] : 0;
+    if (prec == 0) syrk_tile<double>(d, it, smem);
+    else syrk_tile<float>(d, it, smem);
 }
 
 void launch_syrk(hipStream_t st, const DevPtrs &d, const SyrkItem *it, int64_t n) {
@@ -376,11 +427,50 @@ void launch_scatter_rhs(hipStream_t st, const int32_t *perm, const double *y, do
     k_scatter<<<dim3((n + kThreads - 1) / kThreads), kThreads, 0, st>>>(perm, y, b, ldb, n, nrhs);
 }
 
-// Forward: per front, blocks of 64 pivots: z = L_kk^{-1} y_blk via the
-// stored inverse, then the rows below: own pivot rows in place, ancestor
-// rows by atomic add (sibling fronts of one level share ancestors).
+// The 64 x 64 diagonal block of pivot block k into LDS: Dg[t][c] for
+// t <= c holds M[t][c] = inv(L_kk)[c][t] above the diagonal and L_kk's
+// diagonal on it (the potrf kernel's layout of the block).
+__device__ __forceinline__ void stage_diag(double (*Dg)[kNb + 1], const double *Lb, int f, int k, int kb) {
+    for (int idx = threadIdx.x; idx < kb * kNb; idx += blockDim.x) {
+        const int t = idx % kNb, c = idx / kNb;
+        Dg[t][c] = (t <= c && t < kb) ? Lb[static_cast<int64_t>(k + c) * f + k + t] : 0.0;
+    }
+}
+
+// z = inv(L_kk) y (forward) with z[c] = y[c] / L_cc + sum_{t<c} M[t][c] y[t]
+__device__ __forceinline__ void diag_fwd(const double (*Dg)[kNb + 1], const double (*yb)[kMaxRhs],
+                                         double (*z)[kMaxRhs], int kb, int nrhs) {
+    const int c = threadIdx.x % kNb;
+    if (c < kb) {
+        const double dinv = 1.0 / Dg[c][c];
+        for (int r = threadIdx.x / kNb; r < nrhs; r += blockDim.x / kNb) {
+            double s2 = yb[c][r] * dinv;
+            for (int t = 0; t < c; ++t) s2 += Dg[t][c] * yb[t][r];
+            z[c][r] = s2;
+        }
+    }
+}
+
+// u = inv(L_kk)^T v (backward): u[c] = v[c] / L_cc + sum_{t>c} M[c][t] v[t]
+__device__ __forceinline__ void diag_bwd(const double (*Dg)[kNb + 1], const double (*v)[kMaxRhs],
+                                         double (*u)[kMaxRhs], int kb, int nrhs) {
+    const int c = threadIdx.x % kNb;
+    if (c < kb) {
+        const double dinv = 1.0 / Dg[c][c];
+        for (int r = threadIdx.x / kNb; r < nrhs; r += blockDim.x / kNb) {
+            double s2 = v[c][r] * dinv;
+            for (int t = c + 1; t < kb; ++t) s2 += Dg[c][t] * v[t][r];
+            u[c][r] = s2;
+        }
+    }
+}
+
+// Forward, one workgroup per front: blocks of 64 pivots, z = inv(L_kk) y_blk
+// through the stored inverse, then the rows below (own pivot rows in place,
+// ancestor rows by atomic add: sibling fronts of a level share ancestors).
 __global__ __launch_bounds__(kThreads) void k_fwd(DevPtrs d, const int32_t *__restrict__ flist,
                                                  double *__restrict__ y, int32_t n, int32_t nrhs) {
+    __shared__ double Dg[kNb][kNb + 1];
     __shared__ double yb[kNb][kMaxRhs];
     __shared__ double z[kNb][kMaxRhs];
     const int s = flist[blockIdx.x];
@@ -391,23 +481,13 @@ __global__ __launch_bounds__(kThreads) void k_fwd(DevPtrs d, const int32_t *__re
     const int tid = threadIdx.x;
     for (int k = 0; k < p; k += kNb) {
         const int kb = min(kNb, p - k);
+        stage_diag(Dg, Lb, f, k, kb);
         for (int idx = tid; idx < kb * nrhs; idx += kThreads) {
             const int t = idx % kb, r = idx / kb;
             yb[t][r] = y[F.col0 + k + t + static_cast<int64_t>(r) * n];
         }
         __syncthreads();
-        {
-            const int c = tid % kNb;
-            if (c < kb) {
-                const double *Mc = Lb + static_cast<int64_t>(k + c) * f + k;  // Mc[t] = M[t][c], t < c
-                const double dinv = 1.0 / Mc[c];
-                for (int r = tid / kNb; r < nrhs; r += kThreads / kNb) {
-                    double s2 = yb[c][r] * dinv;
-                    for (int t = 0; t < c; ++t) s2 += Mc[t] * yb[t][r];
-                    z[c][r] = s2;
-                }
-            }
-        }
+        diag_fwd(Dg, yb, z, kb, nrhs);
         __syncthreads();
         for (int idx = tid; idx < kb * nrhs; idx += kThreads) {
             const int t = idx % kb, r = idx / kb;
@@ -435,79 +515,341 @@ __global__ __launch_bounds__(kThreads) void k_fwd(DevPtrs d, const int32_t *__re
     }
 }
 
-// Backward: blocks from the last to the first: S = L[below, blk]^T x[below]
-// (one wave per column, lanes over rows, wave reduction), then
-// x_blk = L_kk^{-T} (x_blk - S) with L_kk^{-T} = M (stored strict upper).
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    return v;
-}
-
+// Backward, one workgroup per front, blocks from the last to the first:
+// S = L[below, blk]^T x[below] over 64-row tiles staged in LDS (coalesced
+// column reads), then x_blk = inv(L_kk)^T (x_blk - S).
 __global__ __launch_bounds__(kThreads) void k_bwd(DevPtrs d, const int32_t *__restrict__ flist,
                                                  double *__restrict__ y, int32_t n, int32_t nrhs) {
-    __shared__ double S[kNb][kMaxRhs];
+    __shared__ double tile[kNb][kNb + 1];
+    __shared__ double xs[kNb][kMaxRhs];
     __shared__ double v[kNb][kMaxRhs];
+    __shared__ double u[kNb][kMaxRhs];
+    __shared__ double part[4][kNb][kMaxRhs];
     const int s = flist[blockIdx.x];
     const FrontDev F = d.fronts[s];
     const int f = F.f, p = F.p;
     const double *Lb = d.L + F.lofs;
     const int32_t *frows = d.rows + F.rofs;
     const int tid = threadIdx.x;
-    const int lane = tid & 63, w = tid >> 6;
+    const int cl = tid % kNb, rg = tid / kNb;
     const int nblk = (p + kNb - 1) / kNb;
     for (int bi = nblk - 1; bi >= 0; --bi) {
         const int k = bi * kNb;
         const int kb = min(kNb, p - k);
-        for (int c = w; c < kb; c += kThreads / 64) {
-            double acc[kMaxRhs];
+        double acc[kMaxRhs];
 #pragma unroll
-            for (int r = 0; r < kMaxRhs; ++r) acc[r] = 0.0;
-            const double *Lc = Lb + static_cast<int64_t>(k + c) * f;
-            for (int i = k + kb + lane; i < f; i += 64) {
-                const double l = Lc[i];
-                const int64_t gi = (i < p) ? static_cast<int64_t>(F.col0 + i) : static_cast<int64_t>(frows[i]);
-#pragma unroll
-                for (int r = 0; r < kMaxRhs; ++r)
-                    if (r < nrhs) acc[r] += l * y[gi + static_cast<int64_t>(r) * n];
+        for (int q = 0; q < kMaxRhs; ++q) acc[q] = 0.0;
+        for (int r0 = k + kb; r0 < f; r0 += kNb) {
+            const int nr = min(kNb, f - r0);
+            __syncthreads();
+            for (int idx = tid; idx < kNb * kNb; idx += kThreads) {
+                const int c = idx / kNb, r = idx % kNb;
+                tile[r][c] = (c < kb && r < nr) ? Lb[static_cast<int64_t>(k + c) * f + r0 + r] : 0.0;
             }
-            for (int r = 0; r < nrhs; ++r) {
-                const double sr = wave_sum(acc[r]);
-                if (lane == 0) S[c][r] = sr;
+            for (int idx = tid; idx < kNb * nrhs; idx += kThreads) {
+                const int r = idx % kNb, q = idx / kNb;
+                const int i = r0 + r;
+                const int64_t gi = i < p ? static_cast<int64_t>(F.col0 + i) : static_cast<int64_t>(frows[min(i, f - 1)]);
+                xs[r][q] = r < nr ? y[gi + static_cast<int64_t>(q) * n] : 0.0;
+            }
+            __syncthreads();
+            for (int r = rg * 16; r < rg * 16 + 16; ++r) {
+                const double l = tile[r][cl];
+#pragma unroll
+                for (int q = 0; q < kMaxRhs; ++q)
+                    if (q < nrhs) acc[q] += l * xs[r][q];
             }
         }
+#pragma unroll
+        for (int q = 0; q < kMaxRhs; ++q)
+            if (q < nrhs) part[rg][cl][q] = acc[q];
+        __syncthreads();
+        // the diagonal block into `tile` (Dg layout) and v = x_blk - S
+        stage_diag(tile, Lb, f, k, kb);
+        for (int idx = tid; idx < kb * nrhs; idx += kThreads) {
+            const int t = idx % kb, r = idx / kb;
+            v[t][r] = y[F.col0 + k + t + static_cast<int64_t>(r) * n] -
+                      (part[0][t][r] + part[1][t][r] + part[2][t][r] + part[3][t][r]);
+        }
+        __syncthreads();
+        diag_bwd(tile, v, u, kb, nrhs);
         __syncthreads();
         for (int idx = tid; idx < kb * nrhs; idx += kThreads) {
             const int t = idx % kb, r = idx / kb;
-            v[t][r] = y[F.col0 + k + t + static_cast<int64_t>(r) * n] - S[t][r];
-        }
-        __syncthreads();
-        {
-            const int c = tid % kNb;
-            if (c < kb) {
-                const double dinv = 1.0 / Lb[static_cast<int64_t>(k + c) * f + k + c];
-                for (int r = tid / kNb; r < nrhs; r += kThreads / kNb) {
-                    double s2 = v[c][r] * dinv;
-                    for (int t = c + 1; t < kb; ++t) s2 += Lb[static_cast<int64_t>(k + t) * f + k + c] * v[t][r];
-                    y[F.col0 + k + c + static_cast<int64_t>(r) * n] = s2;
-                }
-            }
+            y[F.col0 + k + t + static_cast<int64_t>(r) * n] = u[t][r];
         }
         __threadfence_block();
         __syncthreads();
     }
 }
 
-void launch_fwd(hipStream_t st, const DevPtrs &d, const int32_t *fronts, int32_t nf, double *y,
-                int32_t n, int32_t nrhs) {
+void launch_fwd(hipStream_t st, const DevPtrs &d, const int32_t *fronts, int32_t nf, double *y, int32_t n,
+                int32_t nrhs) {
     if (nf <= 0) return;
     k_fwd<<<dim3(nf), kThreads, 0, st>>>(d, fronts, y, n, nrhs);
 }
 
-void launch_bwd(hipStream_t st, const DevPtrs &d, const int32_t *fronts, int32_t nf, double *y,
-                int32_t n, int32_t nrhs) {
+void launch_bwd(hipStream_t st, const DevPtrs &d, const int32_t *fronts, int32_t nf, double *y, int32_t n,
+                int32_t nrhs) {
     if (nf <= 0) return;
     k_bwd<<<dim3(nf), kThreads, 0, st>>>(d, fronts, y, n, nrhs);
+}
+
+// ---------------------------------------------------------------------------
+// Large fronts: the sweep is split into blocks of kBig pivots; each block is
+// one single-workgroup kernel for its triangle (four 64-pivot sub-blocks
+// through the stored inverses) and one many-workgroup kernel for the rest
+// of the front (forward: the rows below; backward: the accumulators S of
+// the columns before). Kernel boundaries order the steps: no workgroup
+// waits on another.
+constexpr int kBig = 256;
+
+// Forward triangle of block [k0, k0 + kb): y_blk <- inv(L_blk) y_blk.
+__global__ __launch_bounds__(kThreads) void k_fwd_blk_diag(DevPtrs d, int32_t s, int32_t k0, int32_t kb,
+                                                          double *__restrict__ y, int32_t n, int32_t nrhs) {
+    __shared__ double Dg[kNb][kNb + 1];
+    __shared__ double yb[kBig][kMaxRhs];
+    __shared__ double z[kNb][kMaxRhs];
+    const FrontDev F = d.fronts[s];
+    const int f = F.f;
+    const double *Lb = d.L + F.lofs;
+    const int tid = threadIdx.x;
+    for (int idx = tid; idx < kb * nrhs; idx += kThreads) {
+        const int t = idx % kb, r = idx / kb;
+        yb[t][r] = y[F.col0 + k0 + t + static_cast<int64_t>(r) * n];
+    }
+    for (int sb = 0; sb < kb; sb += kNb) {
+        const int sbn = min(kNb, kb - sb);
+        stage_diag(Dg, Lb, f, k0 + sb, sbn);
+        __syncthreads();
+        diag_fwd(Dg, yb + sb, z, sbn, nrhs);
+        __syncthreads();
+        for (int idx = tid; idx < sbn * nrhs; idx += kThreads) {
+            const int t = idx % sbn, r = idx / sbn;
+            yb[sb + t][r] = z[t][r];
+        }
+        // rows of the block below this sub-block
+        for (int i = sb + sbn + tid; i < kb; i += kThreads) {
+            for (int r = 0; r < nrhs; ++r) {
+                double acc = 0.0;
+                for (int t = 0; t < sbn; ++t) acc += Lb[static_cast<int64_t>(k0 + sb + t) * f + k0 + i] * z[t][r];
+                yb[i][r] -= acc;
+            }
+        }
+        __syncthreads();
+    }
+    for (int idx = tid; idx < kb * nrhs; idx += kThreads) {
+        const int t = idx % kb, r = idx / kb;
+        y[F.col0 + k0 + t + static_cast<int64_t>(r) * n] = yb[t][r];
+    }
+}
+
+// Forward update of rows [k0 + kb, f): one workgroup per 64-row chunk.
+__global__ __launch_bounds__(kThreads) void k_fwd_blk_upd(DevPtrs d, int32_t s, int32_t k0, int32_t kb,
+                                                         double *__restrict__ y, int32_t n, int32_t nrhs) {
+    __shared__ double z[kBig][kMaxRhs];
+    __shared__ double part[4][kNb][kMaxRhs];
+    const FrontDev F = d.fronts[s];
+    const int f = F.f, p = F.p;
+    const double *Lb = d.L + F.lofs;
+    const int32_t *frows = d.rows + F.rofs;
+    const int tid = threadIdx.x;
+    for (int idx = tid; idx < kb * nrhs; idx += kThreads) {
+        const int t = idx % kb, r = idx / kb;
+        z[t][r] = y[F.col0 + k0 + t + static_cast<int64_t>(r) * n];
+    }
+    __syncthreads();
+    const int rl = tid % kNb, cg = tid / kNb;
+    const int row = k0 + kb + blockIdx.x * kNb + rl;
+    double acc[kMaxRhs];
+#pragma unroll
+    for (int r = 0; r < kMaxRhs; ++r) acc[r] = 0.0;
+    if (row < f) {
+        const int t0 = cg * (kBig / 4), t1 = min(kb, t0 + kBig / 4);
+        for (int t = t0; t < t1; ++t) {
+            const double l = Lb[static_cast<int64_t>(k0 + t) * f + row];
+#pragma unroll
+            for (int r = 0; r < kMaxRhs; ++r)
+                if (r < nrhs) acc[r] += l * z[t][r];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < kMaxRhs; ++r)
+        if (r < nrhs) part[cg][rl][r] = acc[r];
+    __syncthreads();
+    if (tid < kNb) {
+        const int rr = k0 + kb + blockIdx.x * kNb + tid;
+        if (rr < f) {
+            const int64_t gi = rr < p ? static_cast<int64_t>(F.col0 + rr) : static_cast<int64_t>(frows[rr]);
+            for (int r = 0; r < nrhs; ++r)
+                y[gi + static_cast<int64_t>(r) * n] -= part[0][tid][r] + part[1][tid][r] + part[2][tid][r] + part[3][tid][r];
+        }
+    }
+}
+
+// Backward, first: S[c] = sum_{i >= p} L[i][c] x[rows[i]] for every pivot
+// column c; one workgroup per 64-column chunk.
+__global__ __launch_bounds__(kThreads) void k_bwd_blk_q(DevPtrs d, int32_t s, const double *__restrict__ y, int32_t n,
+                                                       int32_t nrhs, double *__restrict__ sbuf) {
+    __shared__ double tile[kNb][kNb + 1];
+    __shared__ double xs[kNb][kMaxRhs];
+    __shared__ double part[4][kNb][kMaxRhs];
+    const FrontDev F = d.fronts[s];
+    const int f = F.f, p = F.p;
+    const double *Lb = d.L + F.lofs;
+    const int32_t *frows = d.rows + F.rofs;
+    const int tid = threadIdx.x;
+    const int cl = tid % kNb, rg = tid / kNb;
+    const int c0 = blockIdx.x * kNb;
+    const int cn = min(kNb, p - c0);
+    double acc[kMaxRhs];
+#pragma unroll
+    for (int q = 0; q < kMaxRhs; ++q) acc[q] = 0.0;
+    for (int r0 = p; r0 < f; r0 += kNb) {
+        const int nr = min(kNb, f - r0);
+        __syncthreads();
+        for (int idx = tid; idx < kNb * kNb; idx += kThreads) {
+            const int c = idx / kNb, r = idx % kNb;
+            tile[r][c] = (c < cn && r < nr) ? Lb[static_cast<int64_t>(c0 + c) * f + r0 + r] : 0.0;
+        }
+        for (int idx = tid; idx < kNb * nrhs; idx += kThreads) {
+            const int r = idx % kNb, q = idx / kNb;
+            xs[r][q] = r < nr ? y[static_cast<int64_t>(frows[r0 + r]) + static_cast<int64_t>(q) * n] : 0.0;
+        }
+        __syncthreads();
+        for (int r = rg * 16; r < rg * 16 + 16; ++r) {
+            const double l = tile[r][cl];
+#pragma unroll
+            for (int q = 0; q < kMaxRhs; ++q)
+                if (q < nrhs) acc[q] += l * xs[r][q];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < kMaxRhs; ++q)
+        if (q < nrhs) part[rg][cl][q] = acc[q];
+    __syncthreads();
+    if (tid < cn)
+        for (int q = 0; q < nrhs; ++q)
+            sbuf[static_cast<int64_t>(c0 + tid) * kMaxRhs + q] =
+                part[0][tid][q] + part[1][tid][q] + part[2][tid][q] + part[3][tid][q];
+}
+
+// Backward triangle of block [k0, k0 + kb): sub-blocks from the last,
+// x_sub = inv(L_sub)^T (x_sub - S_sub), then S of the earlier sub-blocks of
+// this block.
+__global__ __launch_bounds__(kThreads) void k_bwd_blk_diag(DevPtrs d, int32_t s, int32_t k0, int32_t kb,
+                                                          double *__restrict__ y, int32_t n, int32_t nrhs,
+                                                          double *__restrict__ sbuf) {
+    __shared__ double Dg[kNb][kNb + 1];
+    __shared__ double v[kNb][kMaxRhs];
+    __shared__ double u[kNb][kMaxRhs];
+    __shared__ double S[kBig][kMaxRhs];
+    const FrontDev F = d.fronts[s];
+    const int f = F.f;
+    const double *Lb = d.L + F.lofs;
+    const int tid = threadIdx.x;
+    for (int idx = tid; idx < kb * nrhs; idx += kThreads) {
+        const int t = idx % kb, r = idx / kb;
+        S[t][r] = sbuf[static_cast<int64_t>(k0 + t) * kMaxRhs + r];
+    }
+    const int nsb = (kb + kNb - 1) / kNb;
+    for (int si = nsb - 1; si >= 0; --si) {
+        const int sb = si * kNb;
+        const int sbn = min(kNb, kb - sb);
+        __syncthreads();
+        stage_diag(Dg, Lb, f, k0 + sb, sbn);
+        for (int idx = tid; idx < sbn * nrhs; idx += kThreads) {
+            const int t = idx % sbn, r = idx / sbn;
+            v[t][r] = y[F.col0 + k0 + sb + t + static_cast<int64_t>(r) * n] - S[sb + t][r];
+        }
+        __syncthreads();
+        diag_bwd(Dg, v, u, sbn, nrhs);
+        __syncthreads();
+        for (int idx = tid; idx < sbn * nrhs; idx += kThreads) {
+            const int t = idx % sbn, r = idx / sbn;
+            y[F.col0 + k0 + sb + t + static_cast<int64_t>(r) * n] = u[t][r];
+        }
+        // S of the earlier columns of this block: S[c] += L[sub rows][c] u
+        for (int c = tid; c < sb; c += kThreads) {
+            for (int r = 0; r < nrhs; ++r) {
+                double acc = 0.0;
+                for (int t = 0; t < sbn; ++t) acc += Lb[static_cast<int64_t>(k0 + c) * f + k0 + sb + t] * u[t][r];
+                S[c][r] += acc;
+            }
+        }
+    }
+}
+
+// Backward update: S[c] += sum_{r in block} L[r][c] x[r] for c in [0, k0),
+// one workgroup per 64-column chunk (the block's rows in 64-row tiles).
+__global__ __launch_bounds__(kThreads) void k_bwd_blk_upd(DevPtrs d, int32_t s, int32_t k0, int32_t kb,
+                                                         const double *__restrict__ y, int32_t n, int32_t nrhs,
+                                                         double *__restrict__ sbuf) {
+    __shared__ double tile[kNb][kNb + 1];
+    __shared__ double xs[kNb][kMaxRhs];
+    __shared__ double part[4][kNb][kMaxRhs];
+    const FrontDev F = d.fronts[s];
+    const int f = F.f;
+    const double *Lb = d.L + F.lofs;
+    const int tid = threadIdx.x;
+    const int cl = tid % kNb, rg = tid / kNb;
+    const int c0 = blockIdx.x * kNb;
+    const int cn = min(kNb, k0 - c0);
+    double acc[kMaxRhs];
+#pragma unroll
+    for (int q = 0; q < kMaxRhs; ++q) acc[q] = 0.0;
+    for (int r0 = 0; r0 < kb; r0 += kNb) {
+        const int nr = min(kNb, kb - r0);
+        __syncthreads();
+        for (int idx = tid; idx < kNb * kNb; idx += kThreads) {
+            const int c = idx / kNb, r = idx % kNb;
+            tile[r][c] = (c < cn && r < nr) ? Lb[static_cast<int64_t>(c0 + c) * f + k0 + r0 + r] : 0.0;
+        }
+        for (int idx = tid; idx < kNb * nrhs; idx += kThreads) {
+            const int r = idx % kNb, q = idx / kNb;
+            xs[r][q] = r < nr ? y[F.col0 + k0 + r0 + r + static_cast<int64_t>(q) * n] : 0.0;
+        }
+        __syncthreads();
+        for (int r = rg * 16; r < rg * 16 + 16; ++r) {
+            const double l = tile[r][cl];
+#pragma unroll
+            for (int q = 0; q < kMaxRhs; ++q)
+                if (q < nrhs) acc[q] += l * xs[r][q];
+        }
+        (void)f;
+    }
+#pragma unroll
+    for (int q = 0; q < kMaxRhs; ++q)
+        if (q < nrhs) part[rg][cl][q] = acc[q];
+    __syncthreads();
+    if (tid < cn)
+        for (int q = 0; q < nrhs; ++q)
+            sbuf[static_cast<int64_t>(c0 + tid) * kMaxRhs + q] +=
+                part[0][tid][q] + part[1][tid][q] + part[2][tid][q] + part[3][tid][q];
+}
+
+void launch_fwd_big(hipStream_t st, const DevPtrs &d, int32_t s, int32_t f, int32_t p, double *y, int32_t n,
+                    int32_t nrhs) {
+    for (int32_t k0 = 0; k0 < p; k0 += kBig) {
+        const int32_t kb = std::min(kBig, p - k0);
+        k_fwd_blk_diag<<<1, kThreads, 0, st>>>(d, s, k0, kb, y, n, nrhs);
+        const int32_t rows = f - (k0 + kb);
+        if (rows > 0)
+            k_fwd_blk_upd<<<dim3((rows + kNb - 1) / kNb), kThreads, 0, st>>>(d, s, k0, kb, y, n, nrhs);
+    }
+}
+
+void launch_bwd_big(hipStream_t st, const DevPtrs &d, int32_t s, int32_t f, int32_t p, double *y, int32_t n,
+                    int32_t nrhs, double *sbuf) {
+    k_bwd_blk_q<<<dim3((p + kNb - 1) / kNb), kThreads, 0, st>>>(d, s, y, n, nrhs, sbuf);
+    const int32_t nb = (p + kBig - 1) / kBig;
+    for (int32_t bi = nb - 1; bi >= 0; --bi) {
+        const int32_t k0 = bi * kBig;
+        const int32_t kb = std::min(kBig, p - k0);
+        k_bwd_blk_diag<<<1, kThreads, 0, st>>>(d, s, k0, kb, y, n, nrhs, sbuf);
+        if (k0 > 0) k_bwd_blk_upd<<<dim3((k0 + kNb - 1) / kNb), kThreads, 0, st>>>(d, s, k0, kb, y, n, nrhs, sbuf);
+    }
+    (void)f;
 }
 
 // ---------------------------------------------------------------------------

@@ -1,0 +1,862 @@
+// HIP kernels of the indefinite (LDL^T, threshold 1x1 / 2x2 pivoting)
+// numeric phase. Compiled with -ffp-contract=off: every multiply and add
+// below rounds separately, as the reference's ldlt_kernel.cpp
+// (solver/CMakeLists.txt:632-642), so the unblocked front kernel
+// reproduces ldlt_factor_front's pivot sequence and values bit for bit.
+#include "smfgpu_ldlt.hpp"
+
+#include <cfloat>
+
+namespace smfgpu {
+
+constexpr int kLT = 256;            // threads per workgroup
+constexpr double kWeakPivotRatio = 1e-6;  // ldlt_kernel.hpp:100
+
+// ---------------------------------------------------------------------------
+// Workgroup reductions (exact: max and first argmax).
+struct RedScratch {
+    double v[kLT];
+    int i[kLT];
+};
+
+// (max, first index of the max) over the thread's candidates, reduced
+// across the workgroup: ties keep the smaller index, as a sequential scan
+// with a strict '>' does.
+__device__ __forceinline__ void red_maxfirst(RedScratch &rs, double v, int idx, double &outv, int &outi) {
+    const int tid = threadIdx.x;
+    rs.v[tid] = v;
+    rs.i[tid] = idx;
+    __syncthreads();
+    for (int w = kLT / 2; w > 0; w >>= 1) {
+        if (tid < w) {
+            const double a = rs.v[tid], b = rs.v[tid + w];
+            const int ia = rs.i[tid], ib = rs.i[tid + w];
+            if (b > a || (b == a && ib >= 0 && (ia < 0 || ib < ia))) {
+                rs.v[tid] = b;
+                rs.i[tid] = ib;
+            }
+        }
+        __syncthreads();
+    }
+    outv = rs.v[0];
+    outi = rs.i[0];
+    __syncthreads();
+}
+
+__device__ __forceinline__ double red_max(RedScratch &rs, double v) {
+    const int tid = threadIdx.x;
+    rs.v[tid] = v;
+    __syncthreads();
+    for (int w = kLT / 2; w > 0; w >>= 1) {
+        if (tid < w && rs.v[tid + w] > rs.v[tid]) rs.v[tid] = rs.v[tid + w];
+        __syncthreads();
+    }
+    const double out = rs.v[0];
+    __syncthreads();
+    return out;
+}
+
+// View of one front: element (i, j), i >= j, at F[j * ld + i].
+struct FV {
+    double *F;
+    int ld, f;
+    __device__ __forceinline__ double &at(int i, int j) const { return F[static_cast<int64_t>(j) * ld + i]; }
+    __device__ __forceinline__ double sym(int i, int j) const { return i >= j ? at(i, j) : at(j, i); }
+};
+
+struct Prm {
+    double u, eff_small, small_pivot;
+};
+
+// select_pivot (ldlt_kernel.cpp:127-275) with workgroup reductions. Every
+// thread returns the same choice.
+__device__ void select_pivot_wg(const FV &A, int k, int ne, int limit, const Prm &prm, RedScratch &rs, int &kind,
+                                int &pa, int &pb) {
+    const int tid = threadIdx.x;
+    const int f = A.f;
+    kind = 0;
+    pa = pb = -1;
+    if (k < limit) {
+        const double a0 = fabs(A.at(k, k));
+        if (a0 >= prm.eff_small) {
+            double m = 0.0;
+            for (int i = k + 1 + tid; i < f; i += kLT) m = fmax(m, fabs(A.at(i, k)));
+            const double colmax = red_max(rs, m);
+            if (!(prm.u * colmax > a0) && !(a0 < kWeakPivotRatio * colmax)) {
+                kind = 1;
+                pa = k;
+                return;
+            }
+        }
+    }
+    for (int cand = k; cand < limit; ++cand) {
+        const double a_cand = fabs(A.at(cand, cand));
+        // largest entry over the eligible rows [k, ne) other than cand (first
+        // index on ties), then the largest of the rest
+        double bv = -1.0;
+        int bi = -1;
+        for (int i = k + tid; i < ne; i += kLT) {
+            if (i == cand) continue;
+            const double v = fabs(A.sym(i, cand));
+            if (v > bv) {
+                bv = v;
+                bi = i;
+            }
+        }
+        double m1;
+        int r_best;
+        red_maxfirst(rs, bv, bi, m1, r_best);
+        if (!(m1 > 0.0)) {
+            m1 = 0.0;
+            r_best = -1;
+        }
+        double m2l = 0.0;
+        for (int i = k + tid; i < ne; i += kLT)
+            if (i != cand && i != r_best) m2l = fmax(m2l, fabs(A.sym(i, cand)));
+        const double m2 = red_max(rs, m2l);
+        const int ext0 = max(ne, cand + 1);
+        double mel = 0.0;
+        for (int i = ext0 + tid; i < f; i += kLT) mel = fmax(mel, fabs(A.at(i, cand)));
+        const double m_ext = red_max(rs, mel);
+        bool weak = false;
+        double weak_growth = 0.0;
+        if (a_cand >= prm.eff_small && !(prm.u * m1 > a_cand)) {
+            if (!(prm.u * m_ext > a_cand)) {
+                const double colmax = fmax(m1, m_ext);
+                if (!(a_cand < kWeakPivotRatio * colmax)) {
+                    kind = 1;
+                    pa = cand;
+                    return;
+                }
+                weak = true;
+                weak_growth = colmax / a_cand;
+            }
+        }
+        if (r_best < k || r_best >= limit || m1 <= 0.0) {
+            if (weak) {
+                kind = 1;
+                pa = cand;
+                return;
+            }
+            continue;
+        }
+        // partner column: max excluding r_best, and excluding cand too
+        double o2 = 0.0, o1 = 0.0;
+        for (int i = k + tid; i < f; i += kLT) {
+            if (i == r_best) continue;
+            const double v = fabs(A.sym(i, r_best));
+            o2 = fmax(o2, v);
+            if (i != cand) o1 = fmax(o1, v);
+        }
+        const double om2 = red_max(rs, o2);
+        const double off1 = red_max(rs, o1);
+        // two_by_two_ok (ldlt_kernel.cpp:97-122)
+        bool ok2 = false;
+        double growth2 = 0.0;
+        if (k + 1 < limit) {
+            const double d00 = A.at(cand, cand);
+            const double d10 = A.sym(r_best, cand);
+            const double d11 = A.at(r_best, r_best);
+            double s2 = fabs(d00);
+            if (s2 < fabs(d10)) s2 = fabs(d10);
+            if (s2 < fabs(d11)) s2 = fabs(d11);
+            if (s2 >= prm.eff_small && isfinite(s2)) {
+                const double a = d00 / s2, b = d10 / s2, c = d11 / s2;
+                const double det_s = __dsub_rn(__dmul_rn(a, c), __dmul_rn(b, b));
+                if (fabs(det_s) > prm.small_pivot) {
+                    const double off0 = fmax(m2, m_ext);
+                    ok2 = true;
+                    if (off0 > 0.0 || off1 > 0.0) {
+                        const double bound0 = __dadd_rn(__dmul_rn(fabs(c), off0), __dmul_rn(fabs(b), off1)) / s2;
+                        const double bound1 = __dadd_rn(__dmul_rn(fabs(b), off0), __dmul_rn(fabs(a), off1)) / s2;
+                        const double bm = bound0 > bound1 ? bound0 : bound1;
+                        if (prm.u * bm > fabs(det_s)) ok2 = false;
+                        else growth2 = bm / fabs(det_s);
+                    }
+                }
+            }
+        }
+        if (ok2 && (!weak || growth2 < weak_growth)) {
+            kind = 2;
+            pa = cand;
+            pb = r_best;
+            return;
+        }
+        if (weak) {
+            kind = 1;
+            pa = cand;
+            return;
+        }
+        const double a_r = fabs(A.at(r_best, r_best));
+        if (a_r >= prm.eff_small && (om2 == 0.0 || a_r >= prm.u * om2)) {
+            kind = 1;
+            pa = r_best;
+            return;
+        }
+    }
+}
+
+// sym_swap (ldlt_kernel.cpp:333-345): positions p < q. Columns [c_lo, p)
+// swap rows p and q (c_lo = 0: all earlier columns).
+__device__ void sym_swap_wg(const FV &A, int p, int q, int32_t *rows, double *W, int wld, int w_cols, int c_lo) {
+    if (p == q) return;
+    if (p > q) {
+        const int t = p;
+        p = q;
+        q = t;
+    }
+    const int tid = threadIdx.x;
+    const int f = A.f;
+    __syncthreads();
+    if (tid == 0) {
+        const double t = A.at(p, p);
+        A.at(p, p) = A.at(q, q);
+        A.at(q, q) = t;
+        const int32_t tr = rows[p];
+        rows[p] = rows[q];
+        rows[q] = tr;
+    }
+    for (int m = c_lo + tid; m < p; m += kLT) {
+        const double t = A.at(p, m);
+        A.at(p, m) = A.at(q, m);
+        A.at(q, m) = t;
+    }
+    for (int m = p + 1 + tid; m < q; m += kLT) {
+        const double t = A.at(m, p);
+        A.at(m, p) = A.at(q, m);
+        A.at(q, m) = t;
+    }
+    for (int m = q + 1 + tid; m < f; m += kLT) {
+        const double t = A.at(m, p);
+        A.at(m, p) = A.at(m, q);
+        A.at(m, q) = t;
+    }
+    for (int idx = tid; idx < w_cols; idx += kLT) {
+        const double t = W[static_cast<int64_t>(idx) * wld + p];
+        W[static_cast<int64_t>(idx) * wld + p] = W[static_cast<int64_t>(idx) * wld + q];
+        W[static_cast<int64_t>(idx) * wld + q] = t;
+    }
+    __syncthreads();
+}
+
+// eliminate_1x1 (ldlt_kernel.cpp:350-432): columns (k, ucol_end) get the
+// rank-1 update c_j[i] -= c_k[i] * (c_k[j] / d) (multiply, then subtract),
+// the unscaled column is saved in W column w_col, then c_k is scaled.
+__device__ void elim1_wg(const FV &A, int k, int ucol_end, double *W, int wld, int w_col) {
+    const int tid = threadIdx.x;
+    const int f = A.f;
+    const double inv_d = 1.0 / A.at(k, k);
+    if (W)
+        for (int i = k + 1 + tid; i < f; i += kLT) W[static_cast<int64_t>(w_col) * wld + i] = A.at(i, k);
+    const int nc = ucol_end - (k + 1);
+    if (nc > 0) {
+        const int64_t tot = static_cast<int64_t>(nc) * f;
+        for (int64_t idx = tid; idx < tot; idx += kLT) {
+            const int j = k + 1 + static_cast<int>(idx / f);
+            const int i = static_cast<int>(idx % f);
+            if (i < j) continue;
+            const double lj = __dmul_rn(A.at(j, k), inv_d);
+            if (lj == 0.0) continue;
+            A.at(i, j) = __dsub_rn(A.at(i, j), __dmul_rn(A.at(i, k), lj));
+        }
+    }
+    __syncthreads();
+    for (int i = k + 1 + tid; i < f; i += kLT) A.at(i, k) = __dmul_rn(A.at(i, k), inv_d);
+    __syncthreads();
+}
+
+// eliminate_2x2 (ldlt_kernel.cpp:436-545)
+__device__ void elim2_wg(const FV &A, int k, int ucol_end, double *W, int wld, int w_col) {
+    const int tid = threadIdx.x;
+    const int f = A.f;
+    const double d00 = A.at(k, k), d10 = A.at(k + 1, k), d11 = A.at(k + 1, k + 1);
+    double s = fabs(d00);
+    if (s < fabs(d10)) s = fabs(d10);
+    if (s < fabs(d11)) s = fabs(d11);
+    const double a = d00 / s, b = d10 / s, c = d11 / s;
+    const double inv = 1.0 / __dmul_rn(__dsub_rn(__dmul_rn(a, c), __dmul_rn(b, b)), s);
+    const double i00 = __dmul_rn(c, inv), i10 = __dmul_rn(-b, inv), i11 = __dmul_rn(a, inv);
+    if (W)
+        for (int i = k + 2 + tid; i < f; i += kLT) {
+            W[static_cast<int64_t>(w_col) * wld + i] = A.at(i, k);
+            W[static_cast<int64_t>(w_col + 1) * wld + i] = A.at(i, k + 1);
+        }
+    const int nc = ucol_end - (k + 2);
+    if (nc > 0) {
+        const int64_t tot = static_cast<int64_t>(nc) * f;
+        for (int64_t idx = tid; idx < tot; idx += kLT) {
+            const int j = k + 2 + static_cast<int>(idx / f);
+            const int i = static_cast<int>(idx % f);
+            if (i < j) continue;
+            const double a0 = A.at(j, k), a1 = A.at(j, k + 1);
+            const double l0 = __dadd_rn(__dmul_rn(i00, a0), __dmul_rn(i10, a1));
+            const double l1 = __dadd_rn(__dmul_rn(i10, a0), __dmul_rn(i11, a1));
+            if (l0 == 0.0 && l1 == 0.0) continue;
+            const double t = __dadd_rn(__dmul_rn(A.at(i, k), l0), __dmul_rn(A.at(i, k + 1), l1));
+            A.at(i, j) = __dsub_rn(A.at(i, j), t);
+        }
+    }
+    __syncthreads();
+    for (int i = k + 2 + tid; i < f; i += kLT) {
+        const double a0 = A.at(i, k), a1 = A.at(i, k + 1);
+        A.at(i, k) = __dadd_rn(__dmul_rn(i00, a0), __dmul_rn(i10, a1));
+        A.at(i, k + 1) = __dadd_rn(__dmul_rn(i10, a0), __dmul_rn(i11, a1));
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ void inertia_1x1(double d, double small, int *cnt) {
+    if (!isfinite(d) || fabs(d) < small) ++cnt[2];
+    else if (d > 0.0) ++cnt[0];
+    else ++cnt[1];
+}
+
+// add_inertia_2x2 (dense_indef.cpp:52-88) with scaled_det_2x2 (pivoting.cpp:18-28)
+__device__ void inertia_2x2(double d00, double d10, double d11, double small, int *cnt) {
+    if (d10 == 0.0) {
+        inertia_1x1(d00, small, cnt);
+        inertia_1x1(d11, small, cnt);
+        return;
+    }
+    double s = fabs(d00);
+    if (s < fabs(d10)) s = fabs(d10);
+    if (s < fabs(d11)) s = fabs(d11);
+    double det_s = 0.0;
+    if (isfinite(s) && s != 0.0) {
+        const double a = d00 / s, b = d10 / s, c = d11 / s;
+        det_s = __dsub_rn(__dmul_rn(a, c), __dmul_rn(b, b));
+    }
+    if (s == 0.0 || !isfinite(det_s)) {
+        cnt[2] += 2;
+        return;
+    }
+    const double tr_s = __dadd_rn(d00 / s, d11 / s);
+    if (det_s < -small) {
+        ++cnt[0];
+        ++cnt[1];
+    } else if (fabs(det_s) <= small) {
+        ++cnt[2];
+        if (tr_s > small) ++cnt[0];
+        else if (tr_s < -small) ++cnt[1];
+        else ++cnt[2];
+    } else {
+        if (tr_s > 0.0) cnt[0] += 2;
+        else if (tr_s < 0.0) cnt[1] += 2;
+        else cnt[2] += 2;
+    }
+}
+
+// apply_pivot (ldlt_kernel.cpp:665-698). Thread 0 keeps the counters.
+__device__ void apply_pivot_wg(const FV &A, int kind, int pa, int pb, int &k, int32_t *rows, int8_t *tags, double *W,
+                               int wld, int w_cols, int ucol_end, const Prm &prm, int *cnt, double &flops, int c_lo) {
+    const int f = A.f;
+    if (kind == 1) {
+        sym_swap_wg(A, k, pa, rows, W, wld, w_cols, c_lo);
+        if (threadIdx.x == 0) {
+            inertia_1x1(A.at(k, k), prm.eff_small, cnt);
+            tags[k] = 1;
+        }
+        elim1_wg(A, k, ucol_end, W, wld, w_cols);
+        const double rem = static_cast<double>(f - (k + 1));
+        flops += rem + rem * (rem + 1.0);
+        k += 1;
+        return;
+    }
+    int partner = pb;
+    if (pa != k) {
+        sym_swap_wg(A, k, pa, rows, W, wld, w_cols, c_lo);
+        if (partner == k) partner = pa;
+    }
+    if (partner != k + 1) sym_swap_wg(A, k + 1, partner, rows, W, wld, w_cols, c_lo);
+    if (threadIdx.x == 0) {
+        inertia_2x2(A.at(k, k), A.at(k + 1, k), A.at(k + 1, k + 1), prm.small_pivot, cnt);
+        tags[k] = 2;
+        tags[k + 1] = -1;
+    }
+    __syncthreads();
+    elim2_wg(A, k, ucol_end, W, wld, w_cols);
+    const double rem = static_cast<double>(f - (k + 2));
+    flops += 6.0 * rem + 2.0 * rem * (rem + 1.0);
+    k += 2;
+}
+
+// ---------------------------------------------------------------------------
+// Unblocked fronts (f < blocked_min_front): the whole front in LDS, one
+// workgroup, ldlt_factor_front's unblocked loop (ldlt_kernel.cpp:977-993).
+constexpr int kSmallMax = 95;
+
+__global__ __launch_bounds__(kLT) void k_ldlt_small(LdltDev d, const int32_t *__restrict__ flist, Prm prm) {
+    __shared__ double Fs[kSmallMax * kSmallMax];
+    __shared__ RedScratch rs;
+    __shared__ int cnt[3];
+    const int s = flist[blockIdx.x];
+    FrontLd &Fd = d.fronts[s];
+    const int f = Fd.f, ne = Fd.ne;
+    double *G = d.arena + Fd.fofs;
+    int32_t *rows = d.rows + Fd.rofs;
+    int8_t *tags = d.tags + Fd.rofs;
+    const int tid = threadIdx.x;
+    for (int idx = tid; idx < f * f; idx += kLT) Fs[idx] = G[idx];
+    if (tid < 3) cnt[tid] = 0;
+    __syncthreads();
+    FV A{Fs, f, f};
+    int k = 0;
+    double flops = 0.0;
+    while (k < ne) {
+        int kind, pa, pb;
+        select_pivot_wg(A, k, ne, ne, prm, rs, kind, pa, pb);
+        if (kind == 0) break;
+        apply_pivot_wg(A, kind, pa, pb, k, rows, tags, nullptr, 0, 0, f, prm, cnt, flops, 0);
+    }
+    __syncthreads();
+    for (int idx = tid; idx < f * f; idx += kLT) G[idx] = Fs[idx];
+    if (tid == 0) {
+        Fd.p = k;
+        Fd.npos = cnt[0];
+        Fd.nneg = cnt[1];
+        Fd.nzero = cnt[2];
+        Fd.flops = flops;
+    }
+}
+
+void launch_ldlt_small(hipStream_t st, const LdltDev &d, const int32_t *flist, int32_t nf, double u, double eff_small,
+                       double small_pivot) {
+    if (nf <= 0) return;
+    Prm prm{u, eff_small, small_pivot};
+    k_ldlt_small<<<dim3(nf), kLT, 0, st>>>(d, flist, prm);
+}
+
+// ---------------------------------------------------------------------------
+// Blocked fronts (f >= blocked_min_front), one workgroup per front in global
+// memory: ldlt_factor_front's blocked path (ldlt_kernel.cpp:1028-1139):
+// first pass over nb-wide panels with partner pulls and rotation of failed
+// columns, second pass with a full search on a stall, deferred panel
+// updates, final Schur update. W (f x nb) in the front's workspace.
+__device__ void deferred_update_wg(const FV &A, int p0, int p1, int c0, int e, const double *W, int wld) {
+    const int jb = p1 - p0;
+    const int m = e - c0;
+    if (jb <= 0 || m <= 0) return;
+    const int f = A.f;
+    const int rowsN = f - c0;
+    const int64_t tot = static_cast<int64_t>(m) * rowsN;
+    for (int64_t idx = threadIdx.x; idx < tot; idx += kLT) {
+        const int j = c0 + static_cast<int>(idx / rowsN);
+        const int i = c0 + static_cast<int>(idx % rowsN);
+        if (i < j) continue;
+        double sacc = 0.0;
+        for (int t = 0; t < jb; ++t) sacc += A.at(i, p0 + t) * W[static_cast<int64_t>(t) * wld + j];
+        A.at(i, j) -= sacc;
+    }
+    __syncthreads();
+}
+
+__device__ void schur_update_wg(const FV &A, int p, int e, const int8_t *tags) {
+    const int f = A.f;
+    const int q = f - e;
+    if (p <= 0 || q <= 0) return;
+    const int64_t tot = static_cast<int64_t>(q) * q;
+    for (int64_t idx = threadIdx.x; idx < tot; idx += kLT) {
+        const int jj = static_cast<int>(idx / q), ii = static_cast<int>(idx % q);
+        if (ii < jj) continue;
+        const int i = e + ii, j = e + jj;
+        double sacc = 0.0;
+        for (int c = 0; c < p; ++c) {
+            const int8_t tg = tags[c];
+            double ld;
+            if (tg == 1) ld = A.at(j, c) * A.at(c, c);
+            else if (tg == 2) ld = A.at(j, c) * A.at(c, c) + A.at(j, c + 1) * A.at(c + 1, c);
+            else if (tg == -1) ld = A.at(j, c - 1) * A.at(c, c - 1) + A.at(j, c) * A.at(c, c);
+            else ld = 0.0;
+            sacc += A.at(i, c) * ld;
+        }
+        A.at(i, j) -= sacc;
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kLT) void k_ldlt_blocked(LdltDev d, const int32_t *__restrict__ flist, Prm prm,
+                                                     int32_t nb) {
+    __shared__ RedScratch rs;
+    __shared__ int cnt[3];
+    __shared__ int32_t order[128];
+    __shared__ int np_s;
+    const int s = flist[blockIdx.x];
+    FrontLd &Fd = d.fronts[s];
+    const int f = Fd.f, ne = Fd.ne;
+    int32_t *rows = d.rows + Fd.rofs;
+    int8_t *tags = d.tags + Fd.rofs;
+    double *W = d.work + Fd.wofs;
+    const int wld = f;
+    const int tid = threadIdx.x;
+    FV A{d.arena + Fd.fofs, f, f};
+    if (tid < 3) cnt[tid] = 0;
+    __syncthreads();
+    int k = 0;
+    double flops = 0.0;
+    {
+        int live = ne, k_paired = -1;
+        while (k < live) {
+            const int ps = k;
+            const int pl = min(live, ps + nb);
+            bool stalled = false;
+            while (k < pl) {
+                int kind, pa, pb;
+                select_pivot_wg(A, k, ne, pl, prm, rs, kind, pa, pb);
+                if (kind == 0) {
+                    stalled = true;
+                    break;
+                }
+                apply_pivot_wg(A, kind, pa, pb, k, rows, tags, W, wld, k - ps, pl, prm, cnt, flops, 0);
+            }
+            deferred_update_wg(A, ps, k, pl, ne, W, wld);
+            if (stalled && k != k_paired) {
+                // (failed column, partner) row-id pairs, distinct partners
+                if (tid == 0) {
+                    int np = 0;
+                    const int max_pairs = min(nb / 2, 64);
+                    for (int pos = k; pos < pl && np / 2 < max_pairs; ++pos) {
+                        double in_max = 0.0;
+                        for (int i = k; i < pos; ++i) in_max = fmax(in_max, fabs(A.at(pos, i)));
+                        for (int i = pos + 1; i < pl; ++i) in_max = fmax(in_max, fabs(A.at(i, pos)));
+                        double out_max = 0.0;
+                        int r = -1;
+                        for (int i = pl; i < ne; ++i) {
+                            const double v = fabs(A.at(i, pos));
+                            if (v > out_max) {
+                                out_max = v;
+                                r = i;
+                            }
+                        }
+                        if (r < 0 || !(out_max > in_max)) continue;
+                        bool taken = false;
+                        for (int t = 1; t < np; t += 2) taken = taken || order[t] == rows[r];
+                        if (taken) continue;
+                        order[np++] = rows[pos];
+                        order[np++] = rows[r];
+                    }
+                    np_s = np;
+                }
+                __syncthreads();
+                const int np = np_s;
+                if (np > 0) {
+                    for (int t = 0; t < np; ++t) {
+                        int q = k + t;
+                        while (rows[q] != order[t]) ++q;
+                        sym_swap_wg(A, k + t, q, rows, nullptr, 0, 0, 0);
+                    }
+                    live = max(live, k + np);
+                    k_paired = k;
+                    continue;
+                }
+            }
+            if (stalled) {
+                const int m = pl - k;
+                const int mv = min(m, live - pl);
+                for (int t = 0; t < mv; ++t) sym_swap_wg(A, k + t, live - mv + t, rows, nullptr, 0, 0, 0);
+                live -= m;
+            }
+        }
+    }
+    while (k < ne) {
+        const int ps = k;
+        const int pl = min(ne, ps + nb);
+        bool stalled = false;
+        while (k < pl) {
+            int kind, pa, pb;
+            select_pivot_wg(A, k, ne, pl, prm, rs, kind, pa, pb);
+            if (kind == 0) {
+                stalled = true;
+                break;
+            }
+            apply_pivot_wg(A, kind, pa, pb, k, rows, tags, W, wld, k - ps, pl, prm, cnt, flops, 0);
+        }
+        deferred_update_wg(A, ps, k, pl, ne, W, wld);
+        if (stalled && k < ne) {
+            const int ss = k;
+            int kind, pa, pb;
+            select_pivot_wg(A, k, ne, ne, prm, rs, kind, pa, pb);
+            if (kind == 0) break;
+            apply_pivot_wg(A, kind, pa, pb, k, rows, tags, W, wld, 0, ss, prm, cnt, flops, 0);
+            deferred_update_wg(A, ss, k, k, ne, W, wld);
+        }
+    }
+    schur_update_wg(A, k, ne, tags);
+    if (tid == 0) {
+        Fd.p = k;
+        Fd.npos = cnt[0];
+        Fd.nneg = cnt[1];
+        Fd.nzero = cnt[2];
+        Fd.flops = flops;
+    }
+}
+
+void launch_ldlt_blocked(hipStream_t st, const LdltDev &d, const int32_t *flist, int32_t nf, double u,
+                         double eff_small, double small_pivot, int32_t nb) {
+    if (nf <= 0) return;
+    Prm prm{u, eff_small, small_pivot};
+    k_ldlt_blocked<<<dim3(nf), kLT, 0, st>>>(d, flist, prm, nb);
+}
+
+// ---------------------------------------------------------------------------
+// Assembly of the level's fronts (full f x f squares in the arena). One
+// workgroup per front column: zero, original entries of the front's own
+// pivot columns, then each child's CB column in child order; the CB of a
+// child is the trailing (f_c - p_c) square of its own front.
+constexpr int kAsmChunkL = 2048;
+
+__device__ __forceinline__ int lbound(const int32_t *a, int lo, int hi, int32_t key) {
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(kLT) void k_ldlt_assemble(LdltDev d, const AsmTask *__restrict__ tasks,
+                                                      const ChildCol *__restrict__ cc) {
+    __shared__ double buf[kAsmChunkL];
+    const AsmTask t = tasks[blockIdx.x];
+    const FrontLd F = d.fronts[t.s];
+    const int c = t.c;
+    const int f = F.f;
+    double *colp = d.arena + F.fofs + static_cast<int64_t>(c) * f;
+    const int tid = threadIdx.x;
+    for (int r = tid; r < c; r += kLT) colp[r] = 0.0;
+    const int oc = c - F.nd;  // own pivot column index, when in [0, psym)
+    for (int r0 = c; r0 < f; r0 += kAsmChunkL) {
+        const int r1 = min(f, r0 + kAsmChunkL);
+        for (int r = r0 + tid; r < r1; r += kLT) buf[r - r0] = 0.0;
+        __syncthreads();
+        if (oc >= 0 && oc < F.psym) {
+            const int j = F.col0 + oc;
+            const int k0 = d.pcol_ptr[j], k1 = d.pcol_ptr[j + 1];
+            for (int k = k0 + tid; k < k1; k += kLT) {
+                const int lr = F.nd + d.amap[k];
+                if (lr >= r0 && lr < r1) buf[lr - r0] += d.av[k];
+            }
+            __syncthreads();
+        }
+        for (int e = t.cc0; e < t.cc1; ++e) {
+            const ChildCol ch = cc[e];
+            const FrontLd C = d.fronts[ch.ch];
+            const int q = C.f - C.p;
+            const int32_t *cm = d.cmap + C.cmofs;
+            const double *cbcol = d.arena + C.fofs + static_cast<int64_t>(C.p + ch.jj) * C.f + C.p;
+            const int lo = lbound(cm, ch.jj, q, r0);
+            const int hi = lbound(cm, lo, q, r1);
+            for (int ii = lo + tid; ii < hi; ii += kLT) buf[cm[ii] - r0] += cbcol[ii];
+            __syncthreads();
+        }
+        for (int r = r0 + tid; r < r1; r += kLT) colp[r] = buf[r - r0];
+        __syncthreads();
+    }
+}
+
+void launch_ldlt_assemble(hipStream_t st, const LdltDev &d, const AsmTask *tasks, int64_t ntasks, const ChildCol *cc) {
+    if (ntasks <= 0) return;
+    k_ldlt_assemble<<<dim3(static_cast<unsigned>(ntasks)), kLT, 0, st>>>(d, tasks, cc);
+}
+
+// ---------------------------------------------------------------------------
+// Front epilogue (multifrontal_serial.cpp:892-945): a root's unaccepted
+// eligible columns become zero pivots (tag 0); the stored factor keeps D
+// on the diagonal, a 2x2 block's off-diagonal moved to the strict upper
+// slot (c, c+1), zeros elsewhere above the diagonal; D^-1 per pivot as
+// build_inline_dinv (solve_diag.cpp:49-123): kinds 1 / 2 / 0.
+__global__ __launch_bounds__(kLT) void k_ldlt_epilogue(LdltDev d, const int32_t *__restrict__ flist) {
+    const int s = flist[blockIdx.x];
+    FrontLd &F = d.fronts[s];
+    const int f = F.f;
+    int p = F.p;
+    int8_t *tags = d.tags + F.rofs;
+    double *G = d.arena + F.fofs;
+    const int tid = threadIdx.x;
+    const bool root = F.root != 0;
+    int ps = p;
+    if (root && p < F.ne) {
+        for (int c = p + tid; c < F.ne; c += kLT) tags[c] = 0;
+        ps = F.ne;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        F.pstore = ps;
+        if (root && p < F.ne) F.nzero += F.ne - p;
+        int n2 = 0;
+        for (int c = 0; c < ps; ++c) n2 += tags[c] == 2;
+        F.n2x2 = n2;
+        F.delayed = root ? 0 : F.ne - p;
+    }
+    for (int c = tid; c < ps; c += kLT) {
+        double *Lc = G + static_cast<int64_t>(c) * f;
+        const int8_t tg = tags[c];
+        const double d00 = Lc[c];
+        // D^-1 entries
+        double *dv = d.dinv + 3 * (F.dofs + c);
+        int8_t *dk = d.dkind + F.dofs + c;
+        if (tg == 2) {
+            const double d10 = Lc[c + 1];
+            const double d11 = G[static_cast<int64_t>(c + 1) * f + c + 1];
+            double sc = fabs(d00);
+            if (sc < fabs(d10)) sc = fabs(d10);
+            if (sc < fabs(d11)) sc = fabs(d11);
+            double det_s = 0.0;
+            if (isfinite(sc) && sc != 0.0) det_s = __dsub_rn(__dmul_rn(d00 / sc, d11 / sc), __dmul_rn(d10 / sc, d10 / sc));
+            if (sc == 0.0 || !isfinite(det_s) || det_s == 0.0) {
+                dv[0] = dv[1] = dv[2] = 0.0;
+            } else {
+                dv[0] = (d11 / sc) / det_s / sc;
+                dv[1] = -(d10 / sc) / det_s / sc;
+                dv[2] = (d00 / sc) / det_s / sc;
+            }
+            *dk = 2;
+        } else if (tg == -1) {
+            dv[0] = dv[1] = dv[2] = 0.0;
+            *dk = 0;
+        } else if (tg == 1) {
+            dv[0] = (d00 == 0.0 || !isfinite(d00)) ? 0.0 : 1.0 / d00;
+            dv[1] = dv[2] = 0.0;
+            *dk = 1;
+        } else {
+            dv[0] = dv[1] = dv[2] = 0.0;
+            *dk = 1;
+        }
+    }
+    __syncthreads();
+    // stored layout (the 2x2 off-diagonal read at its first column, before
+    // that slot is cleared, and written at the second)
+    double d10 = 0.0;
+    for (int c = 0; c < ps; ++c) {
+        double *Lc = G + static_cast<int64_t>(c) * f;
+        const int8_t tg = tags[c];
+        if (tg == 2) d10 = Lc[c + 1];
+        __syncthreads();
+        if (tg == 0)
+            for (int i = c + tid; i < f; i += kLT) Lc[i] = 0.0;
+        if (tg == 2 && tid == 0) Lc[c + 1] = 0.0;
+        for (int i = tid; i < c; i += kLT) Lc[i] = 0.0;
+        __syncthreads();
+        if (tg == -1 && tid == 0) Lc[c - 1] = d10;
+    }
+}
+
+__global__ void k_front_patch(FrontLd *fronts, const FrontLd *patch, const int32_t *ids, int32_t np) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < np) fronts[ids[i]] = patch[i];
+}
+
+__global__ void k_front_gather(const FrontLd *fronts, FrontLd *out, const int32_t *ids, int32_t np) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < np) out[i] = fronts[ids[i]];
+}
+
+void launch_front_gather(hipStream_t st, const FrontLd *fronts, FrontLd *out, const int32_t *ids, int32_t np) {
+    if (np <= 0) return;
+    k_front_gather<<<dim3((np + 127) / 128), 128, 0, st>>>(fronts, out, ids, np);
+}
+
+void launch_front_patch(hipStream_t st, FrontLd *fronts, const FrontLd *patch, const int32_t *ids, int32_t np) {
+    if (np <= 0) return;
+    k_front_patch<<<dim3((np + 127) / 128), 128, 0, st>>>(fronts, patch, ids, np);
+}
+
+void launch_ldlt_epilogue(hipStream_t st, const LdltDev &d, const int32_t *flist, int32_t nf) {
+    if (nf <= 0) return;
+    k_ldlt_epilogue<<<dim3(nf), kLT, 0, st>>>(d, flist);
+}
+
+// ---------------------------------------------------------------------------
+// Solves of an LDL^T factor, one workgroup per front, level by level.
+// Forward: unit L over the stored pivots (rows through the front's row
+// ids), column by column in 64-wide blocks; ancestor rows by atomic add.
+__global__ __launch_bounds__(kLT) void k_ldlt_fwd(LdltDev d, const int32_t *__restrict__ flist, double *__restrict__ y,
+                                                 int32_t n, int32_t nrhs) {
+    const int s = flist[blockIdx.x];
+    const FrontLd F = d.fronts[s];
+    const int f = F.f, p = F.pstore;
+    const double *G = d.arena + F.fofs;
+    const int32_t *rows = d.rows + F.rofs;
+    const int tid = threadIdx.x;
+    // pivot rows are own rows: sequential column sweep within the pivot
+    // block, the rows below the pivots updated once per column block
+    for (int r = 0; r < nrhs; ++r) {
+        double *yr = y + static_cast<int64_t>(r) * n;
+        for (int c = 0; c < p; ++c) {
+            __syncthreads();
+            const double xc = yr[rows[c]];
+            for (int i = c + 1 + tid; i < f; i += kLT) {
+                const double l = G[static_cast<int64_t>(c) * f + i];
+                if (l == 0.0) continue;
+                if (i < p) yr[rows[i]] -= l * xc;
+                else atomicAdd(&yr[rows[i]], -l * xc);
+            }
+            __threadfence_block();
+        }
+    }
+}
+
+__global__ void k_ldlt_diag(LdltDev d, const int32_t *__restrict__ flist, int32_t nf, double *__restrict__ y,
+                            int32_t n, int32_t nrhs) {
+    const int fi = blockIdx.x;
+    if (fi >= nf) return;
+    const FrontLd F = d.fronts[flist[fi]];
+    const int32_t *rows = d.rows + F.rofs;
+    for (int c = threadIdx.x; c < F.pstore; c += blockDim.x) {
+        const int8_t kd = d.dkind[F.dofs + c];
+        const double *dv = d.dinv + 3 * (F.dofs + c);
+        for (int r = 0; r < nrhs; ++r) {
+            double *yr = y + static_cast<int64_t>(r) * n;
+            if (kd == 1) {
+                yr[rows[c]] *= dv[0];
+            } else if (kd == 2) {
+                const double r0 = yr[rows[c]], r1 = yr[rows[c + 1]];
+                yr[rows[c]] = dv[0] * r0 + dv[1] * r1;
+                yr[rows[c + 1]] = dv[1] * r0 + dv[2] * r1;
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ double wsum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+__global__ __launch_bounds__(kLT) void k_ldlt_bwd(LdltDev d, const int32_t *__restrict__ flist, double *__restrict__ y,
+                                                 int32_t n, int32_t nrhs) {
+    __shared__ double part[kLT / 64];
+    const int s = flist[blockIdx.x];
+    const FrontLd F = d.fronts[s];
+    const int f = F.f, p = F.pstore;
+    const double *G = d.arena + F.fofs;
+    const int32_t *rows = d.rows + F.rofs;
+    const int tid = threadIdx.x;
+    for (int r = 0; r < nrhs; ++r) {
+        double *yr = y + static_cast<int64_t>(r) * n;
+        for (int c = p - 1; c >= 0; --c) {
+            double acc = 0.0;
+            for (int i = c + 1 + tid; i < f; i += kLT) acc += G[static_cast<int64_t>(c) * f + i] * yr[rows[i]];
+            acc = wsum(acc);
+            if ((tid & 63) == 0) part[tid >> 6] = acc;
+            __syncthreads();
+            if (tid == 0) {
+                double t = 0.0;
+                for (int w = 0; w < kLT / 64; ++w) t += part[w];
+                yr[rows[c]] -= t;
+            }
+            __threadfence_block();
+            __syncthreads();
+        }
+    }
+}
+
+void launch_ldlt_solve_level(hipStream_t st, const LdltDev &d, const int32_t *flist, int32_t nf, double *y, int32_t n,
+                             int32_t nrhs, int32_t phase) {
+    if (nf <= 0) return;
+    if (phase == 0) k_ldlt_fwd<<<dim3(nf), kLT, 0, st>>>(d, flist, y, n, nrhs);
+    else if (phase == 1) k_ldlt_diag<<<dim3(nf), 64, 0, st>>>(d, flist, nf, y, n, nrhs);
+    else k_ldlt_bwd<<<dim3(nf), kLT, 0, st>>>(d, flist, y, n, nrhs);
+}
+
+}  // namespace smfgpu

@@ -2,7 +2,7 @@
 // per-analysis set-up of multifrontal_factor_serial,
 // solver/src/multifrontal_serial.cpp:1105-1320 of the reference), the
 // level-batched factorization, solves, refinement and the FactorKeep export.
-#include "smfgpu_internal.hpp"
+#include "smfgpu_plan.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -24,106 +24,11 @@ int set_error(int code, const std::string &msg) {
     return code;
 }
 
-#define HIP_OK(expr)                                                                       \
-    do {                                                                                   \
-        hipError_t e_ = (expr);                                                            \
-        if (e_ != hipSuccess)                                                              \
-            return set_error(SMFGPU_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-    } while (0)
-
-// Device buffer with RAII.
-template <class T>
-struct DBuf {
-    T *p = nullptr;
-    size_t n = 0;
-    DBuf() = default;
-    DBuf(const DBuf &) = delete;
-    DBuf &operator=(const DBuf &) = delete;
-    ~DBuf() { release(); }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-    int alloc(size_t count) {
-        release();
-        if (count == 0) return SMFGPU_OK;
-        if (hipMalloc(reinterpret_cast<void **>(&p), count * sizeof(T)) != hipSuccess) {
-            p = nullptr;
-            return set_error(SMFGPU_ERR_ALLOC, "device allocation of " + std::to_string(count * sizeof(T)) + " bytes failed");
-        }
-        n = count;
-        return SMFGPU_OK;
-    }
-    int upload(const std::vector<T> &v) {
-        int rc = alloc(v.size());
-        if (rc != SMFGPU_OK) return rc;
-        if (!v.empty() && hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
-            return set_error(SMFGPU_ERR_DEVICE, "upload failed");
-        return SMFGPU_OK;
-    }
-    size_t bytes() const { return n * sizeof(T); }
-};
-
-constexpr int kPanelOuter = 256;
-constexpr int kPanelInner = 64;
-constexpr int kTile = 128;
+const char *last_error() { return g_err.c_str(); }
 
 }  // namespace smfgpu
 
 using namespace smfgpu;
-
-struct smfgpu_analysis_s {
-    Analysis an;
-};
-
-struct smfgpu_plan_s {
-    int32_t n = 0, nsup = 0;
-    int32_t posdef = 1;
-    hipStream_t stream = nullptr;
-    std::vector<FrontDev> fronts;
-    std::vector<int32_t> level;                    // per supernode
-    std::vector<std::vector<int32_t>> level_fronts;
-    std::vector<int32_t> perm;
-    int64_t nnz_perm = 0, n_orig = 0;
-    int64_t factor_entries = 0, cb_entries = 0, rows_total = 0;
-    double flops = 0.0;
-    std::vector<double> front_flops;
-    std::vector<Launch> launches;
-    // device
-    DBuf<FrontDev> d_fronts;
-    DBuf<int32_t> d_rows, d_cmap, d_pcol_ptr, d_amap, d_perm, d_level_fronts, d_flag;
-    DBuf<int64_t> d_cmofs, d_slot_ptr, d_slot_src;
-    DBuf<double> d_av, d_L, d_CB, d_red, d_vals;
-    DBuf<AsmTask> d_asm;
-    DBuf<ChildCol> d_cc;
-    DBuf<PotrfItem> d_potrf;
-    DBuf<TrsmItem> d_trsm;
-    DBuf<SyrkItem> d_syrk;
-    DBuf<int8_t> d_prec;
-    std::vector<int64_t> level_front_ofs;          // offsets into d_level_fronts
-    // refinement CSR (full symmetric, permuted numbering)
-    DBuf<int64_t> d_fr_ptr, d_fr_slot;
-    DBuf<int32_t> d_fr_col;
-    // solve / refine work
-    DBuf<double> d_y, d_b, d_r, d_rowstat, d_x, d_bb;
-    std::vector<int64_t> syrk_flops_of_launch;
-    // timing
-    bool timing = false;
-    double t_syrk_ms = 0, t_asm_ms = 0, t_solve_ms = 0;
-    double syrk_launches = 0, syrk_flops = 0, asm_bytes = 0, solve_bytes = 0;
-    std::vector<hipEvent_t> ev;
-    int8_t last_prec_any = 0;
-    smfgpu_factor_s *live = nullptr;
-    double max_abs = 0.0;
-};
-
-struct smfgpu_factor_s {
-    smfgpu_plan_s *plan = nullptr;
-    smfgpu_factor_info info{};
-    bool valid = false;
-    std::vector<int8_t> prec;
-};
 
 namespace {
 
@@ -141,6 +46,42 @@ DevPtrs dev_ptrs(smfgpu_plan_s *P) {
     d.flag = P->d_flag.p;
     d.prec = P->d_prec.p;
     return d;
+}
+
+// Level front lists of the solves: small fronts one workgroup each, large
+// ones (f p >= solve_big_min) pipelined over a cooperative grid.
+int solve_lists(smfgpu_plan_s *P) {
+    const int32_t nlev = static_cast<int32_t>(P->level_fronts.size());
+    std::vector<int32_t> lfl;
+    P->level_front_ofs.assign(nlev + 1, 0);
+    P->level_big.assign(nlev, {});
+    int32_t maxb = 0;
+    for (int32_t lv = 0; lv < nlev; ++lv) {
+        P->level_front_ofs[lv] = static_cast<int64_t>(lfl.size());
+        for (int32_t s : P->level_fronts[lv]) {
+            const FrontDev &F = P->fronts[s];
+            if (static_cast<int64_t>(F.f) * F.p >= P->solve_big_min) {
+                P->level_big[lv].push_back(s);
+                maxb = std::max(maxb, (F.p + 63) / 64);
+            } else {
+                lfl.push_back(s);
+            }
+        }
+    }
+    P->level_front_ofs[nlev] = static_cast<int64_t>(lfl.size());
+    int rc;
+    if ((rc = P->d_level_fronts.upload(lfl))) return rc;
+    if (maxb > P->max_blocks || P->d_sflags.p == nullptr) {
+        P->max_blocks = std::max(maxb, 1);
+        if ((rc = P->d_sflags.alloc(2 * P->max_blocks)) ||
+            (rc = P->d_zbuf.alloc(static_cast<size_t>(P->max_blocks) * 64 * 16)) ||
+            (rc = P->d_sbuf.alloc(static_cast<size_t>(P->max_blocks) * 64 * 16)))
+            return rc;
+        if (hipMemset(P->d_sflags.p, 0, P->d_sflags.bytes()) != hipSuccess)
+            return set_error(SMFGPU_ERR_DEVICE, "plan: flag initialisation failed");
+        P->epoch = 0;
+    }
+    return SMFGPU_OK;
 }
 
 int plan_build(smfgpu_plan_s *P, const smfgpu_analysis_desc *D) {
@@ -224,10 +165,15 @@ int plan_build(smfgpu_plan_s *P, const smfgpu_analysis_desc *D) {
         for (int t = 0; t < F.f; ++t) pos[fr[t]] = -1;
     }
     // assembly tasks per level, child contributions per parent column
+    // (SPD: sizes are static, the whole launch sequence is planned here)
     std::vector<AsmTask> tasks;
     std::vector<ChildCol> cc;
     std::vector<int64_t> asm_first(nlev + 1, 0);
     std::vector<std::vector<ChildCol>> bucket;
+    std::vector<PotrfItem> potrf;
+    std::vector<TrsmItem> trsm;
+    std::vector<SyrkItem> syrk;
+    if (P->posdef) {
     for (int32_t lv = 0; lv < nlev; ++lv) {
         asm_first[lv] = static_cast<int64_t>(tasks.size());
         for (int32_t s : P->level_fronts[lv]) {
@@ -249,9 +195,6 @@ int plan_build(smfgpu_plan_s *P, const smfgpu_analysis_desc *D) {
     asm_first[nlev] = static_cast<int64_t>(tasks.size());
     if (cc.size() > static_cast<size_t>(INT32_MAX)) return set_error(SMFGPU_ERR_ALLOC, "plan: too many child columns");
     // dense-kernel items and the launch sequence
-    std::vector<PotrfItem> potrf;
-    std::vector<TrsmItem> trsm;
-    std::vector<SyrkItem> syrk;
     auto add_syrk = [&](int32_t s, int32_t jlo, int32_t jhi, int32_t t0, int32_t t1) {
         const FrontDev &F = P->fronts[s];
         for (int32_t jt = jlo; jt < jhi; jt += kTile)
@@ -316,6 +259,7 @@ int plan_build(smfgpu_plan_s *P, const smfgpu_analysis_desc *D) {
         }
         P->syrk_flops_of_launch.push_back(fl);
     }
+    }  // posdef
     // value map: slot -> input entries in input order
     const int64_t nslots = P->nnz_perm;
     std::vector<int64_t> slot_ptr(nslots + 1, 0), slot_src;
@@ -359,30 +303,28 @@ int plan_build(smfgpu_plan_s *P, const smfgpu_analysis_desc *D) {
                 }
             }
     }
-    // level front lists for the solves
-    std::vector<int32_t> lfl;
-    P->level_front_ofs.assign(nlev + 1, 0);
-    for (int32_t lv = 0; lv < nlev; ++lv) {
-        P->level_front_ofs[lv] = static_cast<int64_t>(lfl.size());
-        lfl.insert(lfl.end(), P->level_fronts[lv].begin(), P->level_fronts[lv].end());
-    }
-    P->level_front_ofs[nlev] = static_cast<int64_t>(lfl.size());
     // uploads
     int rc;
     std::vector<int32_t> rows(D->front_row_idx, D->front_row_idx + P->rows_total);
     std::vector<int32_t> pcp(D->pcol_ptr, D->pcol_ptr + n + 1);
     if ((rc = P->d_fronts.upload(P->fronts)) || (rc = P->d_rows.upload(rows)) || (rc = P->d_cmap.upload(cmap)) ||
         (rc = P->d_cmofs.upload(cmofs)) || (rc = P->d_pcol_ptr.upload(pcp)) || (rc = P->d_amap.upload(amap)) ||
-        (rc = P->d_perm.upload(P->perm)) || (rc = P->d_level_fronts.upload(lfl)) ||
+        (rc = P->d_perm.upload(P->perm)) ||
         (rc = P->d_slot_ptr.upload(slot_ptr)) || (rc = P->d_slot_src.upload(slot_src)) ||
         (rc = P->d_asm.upload(tasks)) || (rc = P->d_cc.upload(cc)) || (rc = P->d_potrf.upload(potrf)) ||
         (rc = P->d_trsm.upload(trsm)) || (rc = P->d_syrk.upload(syrk)) || (rc = P->d_fr_ptr.upload(fr_ptr)) ||
         (rc = P->d_fr_col.upload(fr_col)) || (rc = P->d_fr_slot.upload(fr_slot)))
         return rc;
-    if ((rc = P->d_av.alloc(std::max<int64_t>(nslots, 1))) || (rc = P->d_L.alloc(std::max<int64_t>(P->factor_entries, 1))) ||
-        (rc = P->d_CB.alloc(std::max<int64_t>(P->cb_entries, 1))) || (rc = P->d_red.alloc(8)) ||
-        (rc = P->d_flag.alloc(4)) || (rc = P->d_prec.alloc(std::max(ns, 1))))
+    const bool spd = P->posdef != 0;
+    if ((rc = P->d_av.alloc(std::max<int64_t>(nslots, 1))) ||
+        (rc = P->d_L.alloc(spd ? std::max<int64_t>(P->factor_entries, 1) : 1)) ||
+        (rc = P->d_CB.alloc(spd ? std::max<int64_t>(P->cb_entries, 1) : 1)) || (rc = P->d_red.alloc(8)) ||
+        (rc = P->d_flag.alloc(4)) || (rc = P->d_prec.alloc(std::max(ns, 1))) || (rc = P->d_serr.alloc(4)))
         return rc;
+    if (hipMemset(P->d_serr.p, 0, P->d_serr.bytes()) != hipSuccess)
+        return set_error(SMFGPU_ERR_DEVICE, "plan: flag initialisation failed");
+    if (!spd) return ldlt_plan_build(P, D);
+    if ((rc = solve_lists(P))) return rc;
     // algorithmic bytes (assembly, one solve sweep pair per RHS)
     double asm_b = 12.0 * static_cast<double>(P->nnz_perm);
     for (int32_t s = 0; s < ns; ++s) {
@@ -402,7 +344,7 @@ double now_s() {
 
 extern "C" {
 
-SMFGPU_API const char *smfgpu_error_string(void) { return g_err.c_str(); }
+SMFGPU_API const char *smfgpu_error_string(void) { return last_error(); }
 
 SMFGPU_API int smfgpu_default_options(smfgpu_factor_options *opt) {
     if (!opt) return set_error(SMFGPU_ERR_INVALID_ARG, "null options");
@@ -480,7 +422,6 @@ SMFGPU_API void smfgpu_analysis_free(smfgpu_analysis_t *an) {
 SMFGPU_API int smfgpu_plan_create(const smfgpu_analysis_desc *desc, int32_t posdef, smfgpu_plan_t *out) {
     if (!desc || !out) return set_error(SMFGPU_ERR_INVALID_ARG, "null argument");
     *out = nullptr;
-    if (!posdef) return set_error(SMFGPU_ERR_UNSUPPORTED, "plan: indefinite LDL^T plans are not built by this entry point yet");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
         return set_error(SMFGPU_ERR_DEVICE, "no HIP device");
@@ -488,6 +429,12 @@ SMFGPU_API int smfgpu_plan_create(const smfgpu_analysis_desc *desc, int32_t posd
         auto P = std::make_unique<smfgpu_plan_s>();
         P->posdef = posdef;
         HIP_OK(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking));
+        {
+            int dev = 0, ncu = 0;
+            HIP_OK(hipGetDevice(&dev));
+            HIP_OK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+            P->num_cu = std::max(1, ncu);
+        }
         const int rc = plan_build(P.get(), desc);
         if (rc != SMFGPU_OK) {
             (void)hipStreamDestroy(P->stream);
@@ -508,6 +455,7 @@ SMFGPU_API void smfgpu_plan_free(smfgpu_plan_t *plan) {
         (void)hipStreamDestroy(P->stream);
     }
     for (hipEvent_t e : P->ev) (void)hipEventDestroy(e);
+    for (hipEvent_t e : P->sev) (void)hipEventDestroy(e);
     if (P->live) P->live->plan = nullptr;
     delete P;
     *plan = nullptr;
@@ -532,6 +480,18 @@ SMFGPU_API int smfgpu_plan_stats(smfgpu_plan_t P, double *out) {
     return SMFGPU_OK;
 }
 
+SMFGPU_API int smfgpu_plan_tune(smfgpu_plan_t P, int32_t key, int64_t value) {
+    if (!P) return set_error(SMFGPU_ERR_INVALID_ARG, "null handle");
+    switch (key) {
+    case SMFGPU_TUNE_SOLVE_BIG_MIN:
+        if (value < 1) return set_error(SMFGPU_ERR_INVALID_ARG, "tune: bad value");
+        P->solve_big_min = value;
+        return solve_lists(P);
+    default:
+        return set_error(SMFGPU_ERR_INVALID_ARG, "tune: unknown key");
+    }
+}
+
 SMFGPU_API int smfgpu_set_timing(smfgpu_plan_t P, int32_t enabled) {
     if (!P) return set_error(SMFGPU_ERR_INVALID_ARG, "null handle");
     P->timing = enabled != 0;
@@ -547,6 +507,7 @@ SMFGPU_API int smfgpu_kernel_times(smfgpu_plan_t P, double *out) {
     out[4] = P->asm_bytes;
     out[5] = P->t_solve_ms;
     out[6] = P->solve_bytes;
+    for (int k = 0; k < 4; ++k) out[7 + k] = P->t_solve_phase[k];
     return SMFGPU_OK;
 }
 
@@ -557,7 +518,8 @@ SMFGPU_API int smfgpu_factor(smfgpu_plan_t P, const double *values, int64_t nval
     smfgpu_factor_options opt;
     if (opt_in) opt = *opt_in;
     else smfgpu_default_options(&opt);
-    if (!opt.posdef || !P->posdef) return set_error(SMFGPU_ERR_UNSUPPORTED, "factor: LDL^T not available in this plan");
+    if ((opt.posdef != 0) != (P->posdef != 0))
+        return set_error(SMFGPU_ERR_INVALID_ARG, "factor: matrix type differs from the plan's");
     if (nvalues != P->n_orig) return set_error(SMFGPU_ERR_INVALID_ARG, "factor: wrong number of values");
     if (nvalues > 0 && !values) return set_error(SMFGPU_ERR_INVALID_ARG, "factor: null values");
     hipStream_t st = P->stream;
@@ -569,6 +531,25 @@ SMFGPU_API int smfgpu_factor(smfgpu_plan_t P, const double *values, int64_t nval
         }
         HIP_OK(hipMemcpyAsync(P->d_vals.p, values, nvalues * sizeof(double), hipMemcpyHostToDevice, st));
         dv = P->d_vals.p;
+    }
+    if (!P->posdef) {
+        auto F = std::make_unique<smfgpu_factor_s>();
+        F->plan = P;
+        smfgpu_factor_info fi{};
+        if (P->live) P->live->valid = false;
+        P->live = nullptr;
+        const double t0 = now_s();
+        const int rc = ldlt_factor(P, dv, nvalues, opt, fi);
+        fi.factor_seconds = now_s() - t0;
+        if (info) *info = fi;
+        if (rc != SMFGPU_OK) return rc;
+        F->info = fi;
+        // solve_core refuses a singular factor (solve.cpp:888-903)
+        F->valid = fi.status == SMFGPU_STATUS_SUCCESS;
+        P->live = F.get();
+        P->max_abs = fi.max_abs;
+        *out = F.release();
+        return SMFGPU_OK;
     }
     // per-front Schur precision
     auto F = std::make_unique<smfgpu_factor_s>();
@@ -690,6 +671,7 @@ SMFGPU_API int smfgpu_export_factor(smfgpu_factor_t f, int64_t *sizes, int64_t *
     if (!f || !f->plan) return set_error(SMFGPU_ERR_INVALID_ARG, "null factor");
     smfgpu_plan_s *P = f->plan;
     if (P->live != f) return set_error(SMFGPU_ERR_STATE, "export: factor was superseded by a newer factorization");
+    if (!P->posdef) return ldlt_export(P, sizes, steps, factor_values, row_indices, pivot_types);
     if (sizes) {
         sizes[0] = P->nsup;
         sizes[1] = P->factor_entries;
@@ -722,6 +704,7 @@ SMFGPU_API int smfgpu_export_factor(smfgpu_factor_t f, int64_t *sizes, int64_t *
 
 static int solve_device(smfgpu_plan_s *P, const double *b_dev, int64_t ldb, double *out_dev, int64_t ldo,
                         int32_t nrhs) {
+    if (!P->posdef) return ldlt_solve(P, b_dev, ldb, out_dev, ldo, nrhs);
     hipStream_t st = P->stream;
     const int32_t n = P->n;
     DevPtrs d = dev_ptrs(P);
@@ -733,12 +716,55 @@ static int solve_device(smfgpu_plan_s *P, const double *b_dev, int64_t ldb, doub
             if (rc) return rc;
         }
         launch_gather_rhs(st, P->d_perm.p, b_dev + r0 * ldb, ldb, P->d_y.p, n, nr);
-        for (int32_t lv = 0; lv < nl; ++lv)
+        const bool tm = P->timing;
+        if (tm && P->sev.size() < static_cast<size_t>(8 * nl)) {
+            for (size_t i = P->sev.size(); i < static_cast<size_t>(8 * nl); ++i) {
+                hipEvent_t e;
+                HIP_OK(hipEventCreate(&e));
+                P->sev.push_back(e);
+            }
+        }
+        auto mark = [&](int lv, int k) -> int {
+            if (tm) HIP_OK(hipEventRecord(P->sev[8 * lv + k], st));
+            return SMFGPU_OK;
+        };
+        for (int32_t lv = 0; lv < nl; ++lv) {
+            if (mark(lv, 0)) return SMFGPU_ERR_DEVICE;
             launch_fwd(st, d, P->d_level_fronts.p + P->level_front_ofs[lv],
                        static_cast<int32_t>(P->level_front_ofs[lv + 1] - P->level_front_ofs[lv]), P->d_y.p, n, nr);
-        for (int32_t lv = nl - 1; lv >= 0; --lv)
+            if (mark(lv, 1)) return SMFGPU_ERR_DEVICE;
+            for (int32_t s : P->level_big[lv]) {
+                const FrontDev &F = P->fronts[s];
+                launch_fwd_big(st, d, s, F.f, F.p, P->d_y.p, n, nr);
+            }
+            if (mark(lv, 2)) return SMFGPU_ERR_DEVICE;
+        }
+        for (int32_t lv = nl - 1; lv >= 0; --lv) {
+            if (mark(lv, 3)) return SMFGPU_ERR_DEVICE;
+            for (int32_t s : P->level_big[lv]) {
+                const FrontDev &F = P->fronts[s];
+                launch_bwd_big(st, d, s, F.f, F.p, P->d_y.p, n, nr, P->d_sbuf.p);
+            }
+            if (mark(lv, 4)) return SMFGPU_ERR_DEVICE;
             launch_bwd(st, d, P->d_level_fronts.p + P->level_front_ofs[lv],
                        static_cast<int32_t>(P->level_front_ofs[lv + 1] - P->level_front_ofs[lv]), P->d_y.p, n, nr);
+            if (mark(lv, 5)) return SMFGPU_ERR_DEVICE;
+        }
+        if (tm) {
+            HIP_OK(hipStreamSynchronize(st));
+            for (int k = 0; k < 4; ++k) P->t_solve_phase[k] = 0;
+            for (int32_t lv = 0; lv < nl; ++lv) {
+                float ms;
+                HIP_OK(hipEventElapsedTime(&ms, P->sev[8 * lv + 0], P->sev[8 * lv + 1]));
+                P->t_solve_phase[0] += ms;
+                HIP_OK(hipEventElapsedTime(&ms, P->sev[8 * lv + 1], P->sev[8 * lv + 2]));
+                P->t_solve_phase[1] += ms;
+                HIP_OK(hipEventElapsedTime(&ms, P->sev[8 * lv + 3], P->sev[8 * lv + 4]));
+                P->t_solve_phase[2] += ms;
+                HIP_OK(hipEventElapsedTime(&ms, P->sev[8 * lv + 4], P->sev[8 * lv + 5]));
+                P->t_solve_phase[3] += ms;
+            }
+        }
         launch_scatter_rhs(st, P->d_perm.p, P->d_y.p, out_dev + r0 * ldo, ldo, n, nr);
     }
     HIP_OK(hipGetLastError());

@@ -1,0 +1,360 @@
+// Host driver of the indefinite (LDL^T) numeric phase: level by level, the
+// host lays out the level's fronts from its children's results (delayed
+// columns first, as process_front, solver/src/multifrontal_serial.cpp:510-545
+// of the reference), uploads the row lists, child maps and assembly tasks,
+// launches assembly, the front kernels and the epilogue, and reads back the
+// pivot counts and final row orders the next level needs.
+#include "smfgpu_plan.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace smfgpu {
+
+void launch_front_gather(hipStream_t st, const FrontLd *fronts, FrontLd *out, const int32_t *ids, int32_t np);
+
+constexpr int kBlockedMinFront = 96;  // ldlt_kernel.hpp:15 (blocked_min_front)
+
+namespace {
+
+template <class T>
+int ensure(DBuf<T> &b, size_t need, bool keep) {
+    if (b.n >= need) return SMFGPU_OK;
+    size_t cap = std::max(need, b.n + b.n / 2);
+    if (!keep) return b.alloc(cap);
+    DBuf<T> nb;
+    int rc = nb.alloc(cap);
+    if (rc) return rc;
+    if (b.n && hipMemcpy(nb.p, b.p, b.n * sizeof(T), hipMemcpyDeviceToDevice) != hipSuccess)
+        return set_error(SMFGPU_ERR_DEVICE, "arena growth copy failed");
+    std::swap(b.p, nb.p);
+    std::swap(b.n, nb.n);
+    return SMFGPU_OK;
+}
+
+LdltDev ldlt_dev(smfgpu_plan_s *P) {
+    LdltDev d{};
+    d.fronts = P->d_fl.p;
+    d.arena = P->d_arena.p;
+    d.rows = P->d_lrows.p;
+    d.tags = P->d_ltags.p;
+    d.work = P->d_work.p;
+    d.dinv = P->d_dinv.p;
+    d.dkind = P->d_dkind.p;
+    d.cmap = P->d_lcmap.p;
+    d.pcol_ptr = P->d_pcol_ptr.p;
+    d.amap = P->d_amap.p;
+    d.av = P->d_av.p;
+    return d;
+}
+
+// staging layout helper: appends a vector's bytes (8-aligned), returns offset
+template <class T>
+size_t stage(std::vector<unsigned char> &buf, const std::vector<T> &v) {
+    size_t off = (buf.size() + 7) & ~size_t{7};
+    buf.resize(off + v.size() * sizeof(T));
+    if (!v.empty()) std::memcpy(buf.data() + off, v.data(), v.size() * sizeof(T));
+    return off;
+}
+
+}  // namespace
+
+int ldlt_plan_build(smfgpu_plan_s *P, const smfgpu_analysis_desc *D) {
+    const int32_t ns = D->nsup;
+    P->sn_parent.assign(D->sn_parent, D->sn_parent + ns);
+    P->child_ptr.assign(D->child_ptr, D->child_ptr + ns + 1);
+    P->child_idx.assign(D->child_idx, D->child_idx + P->child_ptr[ns]);
+    P->front_row_ptr.assign(D->front_row_ptr, D->front_row_ptr + ns + 1);
+    P->front_row_idx.assign(D->front_row_idx, D->front_row_idx + P->front_row_ptr[ns]);
+    P->fl.assign(ns, FrontLd{});
+    // level front lists (all fronts) for the solves
+    std::vector<int32_t> lf;
+    const int32_t nlev = static_cast<int32_t>(P->level_fronts.size());
+    P->lv_flist_ofs.assign(nlev + 1, 0);
+    for (int32_t lv = 0; lv < nlev; ++lv) {
+        P->lv_flist_ofs[lv] = static_cast<int64_t>(lf.size());
+        lf.insert(lf.end(), P->level_fronts[lv].begin(), P->level_fronts[lv].end());
+    }
+    P->lv_flist_ofs[nlev] = static_cast<int64_t>(lf.size());
+    int rc;
+    if ((rc = P->d_flist.upload(lf)) || (rc = P->d_fl.alloc(std::max(ns, 1)))) return rc;
+    // initial arena: the symbolic fronts' squares plus a quarter for delays
+    int64_t sq = 0, rw = 0;
+    for (int32_t s = 0; s < ns; ++s) {
+        const int64_t f = P->fronts[s].f;
+        sq += f * f;
+        rw += f;
+    }
+    if ((rc = P->d_arena.alloc(static_cast<size_t>(sq + sq / 4 + 1024))) ||
+        (rc = P->d_lrows.alloc(static_cast<size_t>(rw + rw / 4 + 1024))) ||
+        (rc = P->d_ltags.alloc(static_cast<size_t>(rw + rw / 4 + 1024))) ||
+        (rc = P->d_dinv.alloc(static_cast<size_t>(3 * (P->n + 1024)))) ||
+        (rc = P->d_dkind.alloc(static_cast<size_t>(P->n + 1024))))
+        return rc;
+    return SMFGPU_OK;
+}
+
+int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgpu_factor_options &opt,
+                smfgpu_factor_info &fi) {
+    hipStream_t st = P->stream;
+    const int32_t ns = P->nsup;
+    const int32_t n = P->n;
+    P->ldlt_ready = false;
+    launch_map_values(st, dv, nvalues, P->d_slot_ptr.p, P->d_slot_src.p, P->nnz_perm, P->d_av.p, P->d_red.p);
+    unsigned long long maxbits = 0;
+    HIP_OK(hipMemcpyAsync(&maxbits, P->d_red.p, sizeof(maxbits), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    double max_abs;
+    std::memcpy(&max_abs, &maxbits, sizeof(double));
+    fi.max_abs = max_abs;
+    if (!(maxbits < 0x7ff0000000000000ull)) {
+        fi.status = SMFGPU_STATUS_SINGULAR;
+        return set_error(SMFGPU_ERR_VALUE, "factor: non-finite input value");
+    }
+    // eff_small (factor_indef.cpp:633-636)
+    const double small = opt.small_pivot;
+    const double eff_small = (max_abs > 0.0 && max_abs < small) ? small * max_abs : small;
+    const int32_t nb = std::max(2, opt.panel);
+    P->arena_used = P->rows_used = P->dinv_used = 0;
+    std::vector<int32_t> pos(n, -1);
+    std::vector<int32_t> cmap;
+    std::vector<AsmTask> tasks;
+    std::vector<ChildCol> cc;
+    std::vector<std::vector<ChildCol>> bucket;
+    std::vector<int32_t> ids, small_list, blocked_list;
+    std::vector<FrontLd> patch;
+    std::vector<int32_t> lrows;
+    const int32_t nlev = static_cast<int32_t>(P->level_fronts.size());
+    int rc;
+    P->rows_h.resize(0);
+    for (int32_t lv = 0; lv < nlev; ++lv) {
+        const auto &fr = P->level_fronts[lv];
+        cmap.clear();
+        tasks.clear();
+        cc.clear();
+        ids.clear();
+        patch.clear();
+        lrows.clear();
+        small_list.clear();
+        blocked_list.clear();
+        const int64_t rows_start = P->rows_used;
+        int64_t work_used = 0;
+        for (int32_t s : fr) {
+            const FrontDev &Sy = P->fronts[s];
+            FrontLd F{};
+            int32_t nd = 0;
+            for (int32_t c = P->child_ptr[s]; c < P->child_ptr[s + 1]; ++c) nd += P->fl[P->child_idx[c]].delayed;
+            F.nd = nd;
+            F.psym = Sy.p;
+            F.col0 = Sy.col0;
+            F.f = nd + Sy.f;
+            F.ne = nd + Sy.p;
+            F.root = P->sn_parent[s] < 0 ? 1 : 0;
+            F.fofs = P->arena_used;
+            F.rofs = P->rows_used;
+            F.dofs = P->dinv_used;
+            F.wofs = 0;
+            if (F.f >= kBlockedMinFront) {
+                F.wofs = work_used;
+                work_used += static_cast<int64_t>(F.f) * nb;
+                blocked_list.push_back(s);
+            } else {
+                small_list.push_back(s);
+            }
+            P->arena_used += static_cast<int64_t>(F.f) * F.f;
+            P->rows_used += F.f;
+            P->dinv_used += F.ne;
+            // rows: children's delayed columns in child order, then the
+            // symbolic rows
+            int32_t at = 0;
+            const size_t r0 = lrows.size();
+            for (int32_t c = P->child_ptr[s]; c < P->child_ptr[s + 1]; ++c) {
+                const FrontLd &C = P->fl[P->child_idx[c]];
+                const int32_t *cr = P->rows_h.data() + C.rofs;
+                for (int32_t t = 0; t < C.delayed; ++t) {
+                    lrows.push_back(cr[C.p + t]);
+                    pos[cr[C.p + t]] = at++;
+                }
+            }
+            for (int32_t t = P->front_row_ptr[s]; t < P->front_row_ptr[s + 1]; ++t) {
+                lrows.push_back(P->front_row_idx[t]);
+                pos[P->front_row_idx[t]] = at++;
+            }
+            // child maps (increasing: delayed rows come first in both) and
+            // the contributions per parent column
+            bucket.assign(F.f, {});
+            for (int32_t c = P->child_ptr[s]; c < P->child_ptr[s + 1]; ++c) {
+                const int32_t ch = P->child_idx[c];
+                FrontLd &C = P->fl[ch];
+                const int32_t q = C.f - C.p;
+                C.cmofs = static_cast<int64_t>(cmap.size());
+                const int32_t *cr = P->rows_h.data() + C.rofs + C.p;
+                int32_t prev = -1;
+                for (int32_t ii = 0; ii < q; ++ii) {
+                    const int32_t l = pos[cr[ii]];
+                    if (l < 0 || l <= prev)
+                        return set_error(SMFGPU_ERR_INVALID_ARG, "factor: child row outside its parent front");
+                    cmap.push_back(l);
+                    prev = l;
+                }
+                for (int32_t jj = 0; jj < q; ++jj) bucket[cmap[C.cmofs + jj]].push_back(ChildCol{ch, jj});
+                ids.push_back(ch);
+                patch.push_back(C);
+            }
+            for (int32_t c = 0; c < F.f; ++c) {
+                AsmTask t{s, c, static_cast<int32_t>(cc.size()), 0};
+                cc.insert(cc.end(), bucket[c].begin(), bucket[c].end());
+                t.cc1 = static_cast<int32_t>(cc.size());
+                tasks.push_back(t);
+            }
+            for (size_t t = r0; t < lrows.size(); ++t) pos[lrows[t]] = -1;
+            P->fl[s] = F;
+            ids.push_back(s);
+            patch.push_back(F);
+        }
+        if (cc.size() > static_cast<size_t>(INT32_MAX)) return set_error(SMFGPU_ERR_ALLOC, "factor: level too large");
+        // capacities (arena contents stay: offsets are relative)
+        if ((rc = ensure(P->d_arena, static_cast<size_t>(P->arena_used), true)) ||
+            (rc = ensure(P->d_lrows, static_cast<size_t>(P->rows_used), true)) ||
+            (rc = ensure(P->d_ltags, static_cast<size_t>(P->rows_used), true)) ||
+            (rc = ensure(P->d_dinv, static_cast<size_t>(3 * P->dinv_used), true)) ||
+            (rc = ensure(P->d_dkind, static_cast<size_t>(P->dinv_used), true)) ||
+            (rc = ensure(P->d_work, static_cast<size_t>(std::max<int64_t>(work_used, 1)), false)) ||
+            (rc = ensure(P->d_lcmap, std::max<size_t>(cmap.size(), 1), false)))
+            return rc;
+        // one staged upload: patch, ids, rows, cmap, tasks, contributions,
+        // front lists
+        P->h_stage.clear();
+        const size_t o_patch = stage(P->h_stage, patch);
+        const size_t o_ids = stage(P->h_stage, ids);
+        const size_t o_rows = stage(P->h_stage, lrows);
+        const size_t o_cmap = stage(P->h_stage, cmap);
+        const size_t o_tasks = stage(P->h_stage, tasks);
+        const size_t o_cc = stage(P->h_stage, cc);
+        const size_t o_small = stage(P->h_stage, small_list);
+        const size_t o_blk = stage(P->h_stage, blocked_list);
+        const size_t o_fr = stage(P->h_stage, fr);
+        const size_t o_back = (P->h_stage.size() + 7) & ~size_t{7};
+        P->h_stage.resize(o_back + fr.size() * sizeof(FrontLd));
+        if ((rc = ensure(P->d_stage, P->h_stage.size(), false))) return rc;
+        unsigned char *S = P->d_stage.p;
+        HIP_OK(hipMemcpyAsync(S, P->h_stage.data(), o_back, hipMemcpyHostToDevice, st));
+        HIP_OK(hipMemcpyAsync(P->d_lrows.p + rows_start, S + o_rows, lrows.size() * sizeof(int32_t),
+                              hipMemcpyDeviceToDevice, st));
+        HIP_OK(hipMemsetAsync(P->d_ltags.p + rows_start, 0, lrows.size(), st));
+        if (!cmap.empty())
+            HIP_OK(hipMemcpyAsync(P->d_lcmap.p, S + o_cmap, cmap.size() * sizeof(int32_t), hipMemcpyDeviceToDevice, st));
+        launch_front_patch(st, P->d_fl.p, reinterpret_cast<const FrontLd *>(S + o_patch),
+                           reinterpret_cast<const int32_t *>(S + o_ids), static_cast<int32_t>(ids.size()));
+        LdltDev d = ldlt_dev(P);
+        launch_ldlt_assemble(st, d, reinterpret_cast<const AsmTask *>(S + o_tasks), static_cast<int64_t>(tasks.size()),
+                             reinterpret_cast<const ChildCol *>(S + o_cc));
+        launch_ldlt_small(st, d, reinterpret_cast<const int32_t *>(S + o_small), static_cast<int32_t>(small_list.size()),
+                          opt.pivot_u, eff_small, small);
+        launch_ldlt_blocked(st, d, reinterpret_cast<const int32_t *>(S + o_blk),
+                            static_cast<int32_t>(blocked_list.size()), opt.pivot_u, eff_small, small, nb);
+        launch_ldlt_epilogue(st, d, reinterpret_cast<const int32_t *>(S + o_fr), static_cast<int32_t>(fr.size()));
+        launch_front_gather(st, P->d_fl.p, reinterpret_cast<FrontLd *>(S + o_back),
+                            reinterpret_cast<const int32_t *>(S + o_fr), static_cast<int32_t>(fr.size()));
+        HIP_OK(hipGetLastError());
+        // read back the level's fronts and final row orders
+        std::vector<FrontLd> back(fr.size());
+        HIP_OK(hipMemcpyAsync(back.data(), S + o_back, back.size() * sizeof(FrontLd), hipMemcpyDeviceToHost, st));
+        P->rows_h.resize(P->rows_used);
+        HIP_OK(hipMemcpyAsync(P->rows_h.data() + rows_start, P->d_lrows.p + rows_start, lrows.size() * sizeof(int32_t),
+                              hipMemcpyDeviceToHost, st));
+        HIP_OK(hipStreamSynchronize(st));
+        for (size_t i = 0; i < fr.size(); ++i) P->fl[fr[i]] = back[i];
+    }
+    // statistics (Info fields, factor_indef.cpp:725-770)
+    fi.num_positive = fi.num_negative = fi.num_zero = 0;
+    fi.delayed_pivots = fi.two_by_two_pivots = 0;
+    fi.actual_flops = 0;
+    fi.front_work = 0;
+    fi.max_front = 0;
+    int64_t fent = 0;
+    for (int32_t s = 0; s < ns; ++s) {
+        const FrontLd &F = P->fl[s];
+        fi.num_positive += F.npos;
+        fi.num_negative += F.nneg;
+        fi.num_zero += F.nzero;
+        fi.delayed_pivots += F.delayed;
+        fi.two_by_two_pivots += F.n2x2;
+        fi.actual_flops += F.flops;
+        fi.front_work += static_cast<double>(F.ne) * F.f * static_cast<double>(F.f);
+        fi.max_front = std::max(fi.max_front, F.f);
+        fent += static_cast<int64_t>(F.f) * F.pstore;
+    }
+    fi.factor_entries = fent;
+    fi.status = fi.num_zero > 0 ? SMFGPU_STATUS_SINGULAR : SMFGPU_STATUS_SUCCESS;
+    P->ldlt_ready = true;
+    return SMFGPU_OK;
+}
+
+int ldlt_solve(smfgpu_plan_s *P, const double *b_dev, int64_t ldb, double *out_dev, int64_t ldo, int32_t nrhs) {
+    hipStream_t st = P->stream;
+    const int32_t n = P->n;
+    LdltDev d = ldlt_dev(P);
+    const int32_t nl = static_cast<int32_t>(P->level_fronts.size());
+    for (int32_t r0 = 0; r0 < nrhs; r0 += 16) {
+        const int32_t nr = std::min(16, nrhs - r0);
+        if (P->d_y.n < static_cast<size_t>(n) * nr) {
+            const int rc = P->d_y.alloc(static_cast<size_t>(n) * 16);
+            if (rc) return rc;
+        }
+        launch_gather_rhs(st, P->d_perm.p, b_dev + r0 * ldb, ldb, P->d_y.p, n, nr);
+        for (int32_t lv = 0; lv < nl; ++lv)
+            launch_ldlt_solve_level(st, d, P->d_flist.p + P->lv_flist_ofs[lv],
+                                    static_cast<int32_t>(P->lv_flist_ofs[lv + 1] - P->lv_flist_ofs[lv]), P->d_y.p, n,
+                                    nr, 0);
+        launch_ldlt_solve_level(st, d, P->d_flist.p, static_cast<int32_t>(P->lv_flist_ofs[nl]), P->d_y.p, n, nr, 1);
+        for (int32_t lv = nl - 1; lv >= 0; --lv)
+            launch_ldlt_solve_level(st, d, P->d_flist.p + P->lv_flist_ofs[lv],
+                                    static_cast<int32_t>(P->lv_flist_ofs[lv + 1] - P->lv_flist_ofs[lv]), P->d_y.p, n,
+                                    nr, 2);
+        launch_scatter_rhs(st, P->d_perm.p, P->d_y.p, out_dev + r0 * ldo, ldo, n, nr);
+    }
+    HIP_OK(hipGetLastError());
+    return SMFGPU_OK;
+}
+
+// FactorKeep layout: steps in supernode order with the offsets of a
+// serial factorization (row_offset += f, factor_offset += f * pstore).
+int ldlt_export(smfgpu_plan_s *P, int64_t *sizes, int64_t *steps, double *fv, int32_t *rows, int8_t *tags) {
+    const int32_t ns = P->nsup;
+    int64_t nf = 0, nr = 0, nt = 0;
+    for (int32_t s = 0; s < ns; ++s) {
+        nf += static_cast<int64_t>(P->fl[s].f) * P->fl[s].pstore;
+        nr += P->fl[s].f;
+        nt += P->fl[s].pstore;
+    }
+    if (sizes) {
+        sizes[0] = ns;
+        sizes[1] = nf;
+        sizes[2] = nr;
+        sizes[3] = nt;
+    }
+    int64_t fo = 0, ro = 0, to = 0;
+    std::vector<int8_t> tg;
+    for (int32_t s = 0; s < ns; ++s) {
+        const FrontLd &F = P->fl[s];
+        if (steps) {
+            steps[4 * s + 0] = ro;
+            steps[4 * s + 1] = fo;
+            steps[4 * s + 2] = F.pstore;
+            steps[4 * s + 3] = F.f;
+        }
+        if (fv && F.pstore > 0)
+            HIP_OK(hipMemcpy(fv + fo, P->d_arena.p + F.fofs, static_cast<size_t>(F.f) * F.pstore * sizeof(double),
+                             hipMemcpyDeviceToHost));
+        if (rows) std::memcpy(rows + ro, P->rows_h.data() + F.rofs, static_cast<size_t>(F.f) * sizeof(int32_t));
+        if (tags && F.pstore > 0) HIP_OK(hipMemcpy(tags + to, P->d_ltags.p + F.rofs, F.pstore, hipMemcpyDeviceToHost));
+        fo += static_cast<int64_t>(F.f) * F.pstore;
+        ro += F.f;
+        to += F.pstore;
+    }
+    return SMFGPU_OK;
+}
+
+}  // namespace smfgpu

@@ -117,6 +117,13 @@ void launch_fwd(hipStream_t st, const DevPtrs &d, const int32_t *fronts, int32_t
 void launch_bwd(hipStream_t st, const DevPtrs &d, const int32_t *fronts, int32_t nf, double *y,
                 int32_t n, int32_t nrhs);
 
+// Large fronts: sweeps in blocks of 256 pivots, two launches per block
+// (the block's triangle, then the rest of the front). sbuf: p x 16 doubles.
+void launch_fwd_big(hipStream_t st, const DevPtrs &d, int32_t s, int32_t f, int32_t p, double *y, int32_t n,
+                    int32_t nrhs);
+void launch_bwd_big(hipStream_t st, const DevPtrs &d, int32_t s, int32_t f, int32_t p, double *y, int32_t n,
+                    int32_t nrhs, double *sbuf);
+
 // Refinement: full symmetric CSR in permuted numbering (slot into av).
 void launch_residual(hipStream_t st, int32_t n, const int64_t *rp, const int32_t *ci,
                      const int64_t *slot, const double *av, const double *x, const double *b,

@@ -1,0 +1,144 @@
+// Host-side state of a device plan and of a factorization (shared by the
+// SPD and LDL^T drivers).
+#pragma once
+
+#include "smfgpu_internal.hpp"
+#include "smfgpu_ldlt.hpp"
+
+#include <string>
+#include <vector>
+
+namespace smfgpu {
+
+#define HIP_OK(expr)                                                                       \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return set_error(SMFGPU_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// Device buffer with RAII.
+template <class T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    int alloc(size_t count) {
+        release();
+        if (count == 0) return SMFGPU_OK;
+        if (hipMalloc(reinterpret_cast<void **>(&p), count * sizeof(T)) != hipSuccess) {
+            p = nullptr;
+            return set_error(SMFGPU_ERR_ALLOC, "device allocation of " + std::to_string(count * sizeof(T)) + " bytes failed");
+        }
+        n = count;
+        return SMFGPU_OK;
+    }
+    int upload(const std::vector<T> &v) {
+        int rc = alloc(v.size());
+        if (rc != SMFGPU_OK) return rc;
+        if (!v.empty() && hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+            return set_error(SMFGPU_ERR_DEVICE, "upload failed");
+        return SMFGPU_OK;
+    }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+constexpr int kPanelOuter = 256;
+constexpr int kPanelInner = 64;
+constexpr int kTile = 128;
+
+}  // namespace smfgpu
+
+using namespace smfgpu;
+
+struct smfgpu_analysis_s {
+    Analysis an;
+};
+
+struct smfgpu_plan_s {
+    int32_t n = 0, nsup = 0;
+    int32_t posdef = 1;
+    hipStream_t stream = nullptr;
+    std::vector<FrontDev> fronts;
+    std::vector<int32_t> level;                    // per supernode
+    std::vector<std::vector<int32_t>> level_fronts;
+    std::vector<int32_t> perm;
+    int64_t nnz_perm = 0, n_orig = 0;
+    int64_t factor_entries = 0, cb_entries = 0, rows_total = 0;
+    double flops = 0.0;
+    std::vector<double> front_flops;
+    std::vector<Launch> launches;
+    // device
+    DBuf<FrontDev> d_fronts;
+    DBuf<int32_t> d_rows, d_cmap, d_pcol_ptr, d_amap, d_perm, d_level_fronts, d_flag;
+    DBuf<int64_t> d_cmofs, d_slot_ptr, d_slot_src;
+    DBuf<double> d_av, d_L, d_CB, d_red, d_vals;
+    DBuf<AsmTask> d_asm;
+    DBuf<ChildCol> d_cc;
+    DBuf<PotrfItem> d_potrf;
+    DBuf<TrsmItem> d_trsm;
+    DBuf<SyrkItem> d_syrk;
+    DBuf<int8_t> d_prec;
+    std::vector<int64_t> level_front_ofs;          // offsets into d_level_fronts (small fronts per level)
+    std::vector<std::vector<int32_t>> level_big;   // per level: large fronts (pipelined cooperative solve)
+    DBuf<int32_t> d_sflags, d_serr;                // solve pipeline flags (2 x max blocks) and error word
+    DBuf<double> d_zbuf, d_sbuf;                   // published block values, backward accumulators
+    int32_t epoch = 0;
+    int32_t max_blocks = 0;
+    int32_t num_cu = 256;                          // cooperative grid bound
+    int64_t solve_big_min = int64_t{1} << 21;      // f p from which a front's solve is pipelined
+    double t_solve_phase[4] = {0, 0, 0, 0};        // fwd small, fwd big, bwd big, bwd small (ms)
+    // refinement CSR (full symmetric, permuted numbering)
+    DBuf<int64_t> d_fr_ptr, d_fr_slot;
+    DBuf<int32_t> d_fr_col;
+    // solve / refine work
+    DBuf<double> d_y, d_b, d_r, d_rowstat, d_x, d_bb;
+    std::vector<int64_t> syrk_flops_of_launch;
+    // timing
+    bool timing = false;
+    double t_syrk_ms = 0, t_asm_ms = 0, t_solve_ms = 0;
+    double syrk_launches = 0, syrk_flops = 0, asm_bytes = 0, solve_bytes = 0;
+    std::vector<hipEvent_t> ev, sev;
+    int8_t last_prec_any = 0;
+    smfgpu_factor_s *live = nullptr;
+    double max_abs = 0.0;
+    // ---- LDL^T (posdef = 0) ----
+    std::vector<int32_t> sn_parent, child_ptr, child_idx, front_row_ptr, front_row_idx;
+    std::vector<int32_t> amap_h;                    // host copy of the symbolic-position map
+    std::vector<FrontLd> fl;                        // host mirror of the fronts of the last factorization
+    std::vector<int32_t> rows_h;                    // host mirror of the final row ids (rofs offsets)
+    std::vector<int8_t> tags_h;
+    DBuf<FrontLd> d_fl;
+    DBuf<double> d_arena, d_work, d_dinv;
+    DBuf<int32_t> d_lrows, d_lcmap, d_flist;
+    DBuf<int8_t> d_ltags, d_dkind;
+    DBuf<unsigned char> d_stage;                    // per-level upload staging
+    std::vector<unsigned char> h_stage;
+    int64_t arena_used = 0, rows_used = 0, dinv_used = 0;
+    std::vector<int64_t> lv_flist_ofs;              // per level: offset of its fronts in d_flist (solve)
+    bool ldlt_ready = false;                        // a factorization's fronts are valid
+};
+
+struct smfgpu_factor_s {
+    smfgpu_plan_s *plan = nullptr;
+    smfgpu_factor_info info{};
+    bool valid = false;
+    std::vector<int8_t> prec;
+};
+
+
+namespace smfgpu {
+int ldlt_plan_build(smfgpu_plan_s *P, const smfgpu_analysis_desc *D);
+int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgpu_factor_options &opt,
+                smfgpu_factor_info &fi);
+int ldlt_solve(smfgpu_plan_s *P, const double *b_dev, int64_t ldb, double *out_dev, int64_t ldo, int32_t nrhs);
+int ldlt_export(smfgpu_plan_s *P, int64_t *sizes, int64_t *steps, double *fv, int32_t *rows, int8_t *tags);
+}  // namespace smfgpu

@@ -113,10 +113,14 @@ class Plan:
         check(lib().smfgpu_set_timing(self.h, 1 if on else 0), "set_timing")
 
     def kernel_times(self):
-        st = np.zeros(8)
+        st = np.zeros(16)
         check(lib().smfgpu_kernel_times(self.h, _p(st)), "kernel_times")
-        keys = ("syrk_ms", "syrk_launches", "syrk_flops", "asm_ms", "asm_bytes", "solve_ms", "solve_bytes")
-        return dict(zip(keys, st[:7].tolist()))
+        keys = ("syrk_ms", "syrk_launches", "syrk_flops", "asm_ms", "asm_bytes", "solve_ms", "solve_bytes",
+                "fwd_small_ms", "fwd_big_ms", "bwd_big_ms", "bwd_small_ms")
+        return dict(zip(keys, st[:11].tolist()))
+
+    def tune(self, key, value):
+        check(lib().smfgpu_plan_tune(self.h, key, int(value)), "plan_tune")
 
     def factor(self, values=None, values_ptr=None, precision=PREC_FP64, mixed_min_front=1024, u=0.01,
                small_pivot=1e-20, panel=16, max_negative_pivots=-1) -> "Factor":

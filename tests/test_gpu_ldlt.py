@@ -1,0 +1,119 @@
+"""GPU parity of the indefinite (LDL^T, threshold 1x1/2x2 pivoting with
+delayed pivots) numeric phase through the C ABI."""
+import numpy as np
+import pytest
+
+from tests.conftest import gpu_available
+from tests.helpers import matvec
+from tests.test_golden import load_mf
+
+pytestmark = pytest.mark.gpu
+
+if not gpu_available():
+    pytest.skip("no GPU", allow_module_level=True)
+
+from opt_solver_amd import generators as g  # noqa: E402
+from opt_solver_amd.solver import Plan, SmfGpuError, analyse  # noqa: E402
+
+
+def compare_factor(ex, eo, tol):
+    assert np.array_equal(ex["steps"], eo["steps"])
+    assert np.array_equal(ex["row_indices"], eo["row_indices"])
+    assert np.array_equal(ex["pivot_types"], eo["pivot_types"])
+    err = 0.0
+    for s in range(ex["steps"].shape[0]):
+        ro, fo, p, f = ex["steps"][s]
+        a = ex["factor_values"][fo:fo + f * p]
+        b = eo["factor_values"][fo:fo + f * p]
+        if a.size:
+            err = max(err, np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+    assert err <= tol, err
+    return err
+
+
+@pytest.mark.parametrize("name", ["ocp_T10", "kkt_12"])
+def test_golden_indefinite(name):
+    """The reference's own analysis and factor (tests/golden): identical
+    steps, row order, pivot types, inertia, delays and 2x2 count; every
+    front is below the blocked size, so L and D are bitwise the
+    reference's."""
+    z, a, A = load_mf(name)
+    print("case golden", name, flush=True)
+    plan = Plan(a, posdef=False)
+    fac = plan.factor(A[3], u=0.01)
+    st = z["stats"]
+    assert fac.status == int(st[0])
+    assert fac.inertia == (int(st[1]), int(st[2]), int(st[3]))
+    assert fac.info.delayed_pivots == int(st[4]) and fac.info.two_by_two_pivots == int(st[5])
+    assert fac.flops == st[6]
+    ex = fac.export()
+    gold = dict(steps=z["steps"], row_indices=z["row_indices"], pivot_types=z["pivot_types"],
+                factor_values=z["factor_values"])
+    err = compare_factor(ex, gold, 0.0)
+    x = fac.solve(z["b"])
+    assert np.abs(x - z["x_ref"]).max() < 1e-10
+    print("  bitwise, err", err)
+
+
+def _arrays_for(A, ref_order):
+    from oracle import refshim
+    if refshim.available():
+        return refshim.RefAnalysis(A, ordering=ref_order, posdef=False).plan_arrays()
+    return analyse(A, nemin=16).arrays()
+
+
+@pytest.mark.parametrize("case", ["ocp_T60", "ocp_T200", "kkt_30", "kkt_48"])
+def test_indefinite_against_oracle(oracle_lib, case):
+    """OCP KKTs (many delays, 2x2 pivots) and grid KKTs with fronts past the
+    blocked size, on the reference's analysis when it is built: the GPU
+    factor against the oracle's, inertia against Sylvester's law."""
+    from oracle import refshim
+    print("case", case, flush=True)
+    if case.startswith("ocp"):
+        T = int(case[5:])
+        A = g.ocp_kkt(T, seed=T)
+        inertia = (16 * T + 12, 12 * T, 0)
+        order = refshim.ORDER_AMD
+    else:
+        gsz = int(case[4:])
+        m = gsz * gsz // 5
+        A, _ = g.kkt_grid(gsz, m, seed=gsz)
+        inertia = (gsz * gsz, m, 0)
+        order = refshim.ORDER_METIS
+    arrays = _arrays_for(A, order)
+    plan = Plan(arrays, posdef=False)
+    fac = plan.factor(A[3], u=0.01)
+    orc = oracle_lib.Factor(arrays, A[3], posdef=False)
+    assert fac.status == orc.status == 0
+    assert fac.inertia == orc.inertia == inertia
+    assert fac.info.delayed_pivots == orc.delayed and fac.info.two_by_two_pivots == orc.two_by_two
+    assert fac.flops == orc.flops
+    err = compare_factor(fac.export(), orc.export(), 1e-11)
+    x = np.random.default_rng(1).uniform(-1, 1, A[0])
+    b = matvec(A, x)
+    xs = fac.solve(b)
+    eta, om = oracle_lib.backward_error(A, b, xs)
+    xr, st = fac.refine(b, xs, max_iters=5, tol=1e-15)
+    eta2, _ = oracle_lib.backward_error(A, b, xr)
+    print(f"  max front {fac.info.max_front} delays {fac.info.delayed_pivots} 2x2 {fac.info.two_by_two_pivots} "
+          f"L err {err:.1e} eta {eta:.1e} refined {eta2:.1e}")
+    assert eta2 <= 1e-14
+
+
+def test_singular_reported(oracle_lib):
+    """A saddle point with a rank-deficient constraint block: zero pivots at
+    the root, status Singular, as the oracle."""
+    print("case singular", flush=True)
+    n = 6
+    rows = [0, 1, 2, 3, 4, 5, 3, 4, 5, 5]
+    cols = [0, 1, 2, 3, 4, 5, 0, 0, 1, 2]
+    vals = [2.0, 2.0, 2.0, 0.0, 0.0, 0.0, 1.0, 1.0, 0.0, 0.0]
+    A = g.coo_lower_to_csc(n, rows, cols, vals)
+    an = analyse(A, nemin=8)
+    plan = Plan(an, posdef=False)
+    fac = plan.factor(A[3])
+    orc = oracle_lib.Factor(an.arrays(), A[3], posdef=False)
+    assert fac.status == orc.status == 2
+    assert fac.inertia == orc.inertia
+    with pytest.raises(SmfGpuError):
+        fac.solve(np.ones(n))

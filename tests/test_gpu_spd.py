@@ -92,6 +92,27 @@ def test_spd27_own_analysis_multi_rhs(oracle_lib):
         assert np.abs(fac.solve(B[:, r]) - XS[:, r]).max() < 1e-13
 
 
+def test_large_front_pipelined_solve(oracle_lib):
+    """spd27 40^3 has fronts past the pipelined-solve size (f p >= 2^21):
+    1 and 16 right-hand sides against the known solution and the
+    reference backward-error formula."""
+    m = 40
+    print("case spd27_40 pipelined solve", flush=True)
+    A = g.spd27_3d(m, seed=4)
+    an = analyse(A, perm=o.grid_nd((m, m, m), leaf=64), nemin=32)
+    assert an.max_front >= 2000
+    plan = Plan(an, posdef=True)
+    fac = plan.factor(A[3])
+    X = np.random.default_rng(4).uniform(-1, 1, (A[0], 16))
+    B = matvec(A, X)
+    for nr in (1, 16):
+        XS = fac.solve(B[:, :nr] if nr > 1 else B[:, 0])
+        XS = XS.reshape(A[0], nr)
+        eta, om = oracle_lib.backward_error(A, B[:, :nr], XS)
+        assert eta <= 1e-14, (nr, eta)
+        assert np.abs(XS - X[:, :nr]).max() < 1e-10
+
+
 def test_mixed_precision_refinement(oracle_lib):
     """FP32 Schur updates on every front, FP64 refinement against A: scaled
     backward error <= 1e-12 (the mixed-precision gate), inertia unchanged."""
