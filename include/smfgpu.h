@@ -191,17 +191,18 @@ SMFGPU_API int smfgpu_refine(smfgpu_factor_t f, const double *values, const doub
                              int32_t mem, double *out);
 
 /* Per-plan tuning knobs (the autotuner's parameters). Keys:
- * SMFGPU_TUNE_SOLVE_BIG_MIN: fronts with f * p at or above this value are
- * solved by the pipelined multi-workgroup sweep (default 2^21). */
-#define SMFGPU_TUNE_SOLVE_BIG_MIN 1
+ * SMFGPU_TUNE_SOLVE_FUSE_ROWS: an SPD front whose last pivot block has at
+ * most this many rows below it has them updated by the workgroup that solves
+ * the block, saving a step of the sweep (default 512). */
+#define SMFGPU_TUNE_SOLVE_FUSE_ROWS 1
 SMFGPU_API int smfgpu_plan_tune(smfgpu_plan_t plan, int32_t key, int64_t value);
 
 /* Device time (ms) of the last factorization's dominant kernel family:
  * out[0] Schur-update kernel total ms, out[1] launches, out[2] flops in it,
  * out[3] assembly ms, out[4] assembly bytes, out[5] solve ms (last solve),
- * out[6] solve bytes (last solve), out[7..10] last solve's phases: forward
- * small fronts, forward large fronts, backward large, backward small (ms,
- * over all levels). Measured with HIP events on the plan's
+ * out[6] solve bytes (last solve), out[7] forward sweep ms, out[8] backward
+ * sweep ms (SPD, last right-hand-side group), out[9] forward launches,
+ * out[10] backward launches. Measured with HIP events on the plan's
  * stream when timing is enabled. */
 SMFGPU_API int smfgpu_set_timing(smfgpu_plan_t plan, int32_t enabled);
 SMFGPU_API int smfgpu_kernel_times(smfgpu_plan_t plan, double *out);

@@ -57,7 +57,7 @@ EXPORTED = [
     "smfgpu_plan_stats", "smfgpu_factor", "smfgpu_factor_free", "smfgpu_export_factor", "smfgpu_solve",
     "smfgpu_refine", "smfgpu_set_timing", "smfgpu_kernel_times", "smfgpu_plan_tune",
 ]
-TUNE_SOLVE_BIG_MIN = 1
+TUNE_SOLVE_FUSE_ROWS = 1
 
 _lib = None
 

@@ -397,7 +397,6 @@ void launch_zero_upper(hipStream_t, const DevPtrs &, const PotrfItem *, int64_t)
 
 // ---------------------------------------------------------------------------
 // A15: solves. y is the permuted right-hand-side block (n x nrhs, ld n).
-constexpr int kMaxRhs = 16;
 
 __global__ void k_gather(const int32_t *__restrict__ perm, const double *__restrict__ b, int64_t ldb,
                          double *__restrict__ y, int32_t n, int32_t nrhs) {
@@ -425,431 +424,6 @@ void launch_scatter_rhs(hipStream_t st, const int32_t *perm, const double *y, do
                         int32_t n, int32_t nrhs) {
     if (n <= 0) return;
     k_scatter<<<dim3((n + kThreads - 1) / kThreads), kThreads, 0, st>>>(perm, y, b, ldb, n, nrhs);
-}
-
-// The 64 x 64 diagonal block of pivot block k into LDS: Dg[t][c] for
-// t <= c holds M[t][c] = inv(L_kk)[c][t] above the diagonal and L_kk's
-// diagonal on it (the potrf kernel's layout of the block).
-__device__ __forceinline__ void stage_diag(double (*Dg)[kNb + 1], const double *Lb, int f, int k, int kb) {
-    for (int idx = threadIdx.x; idx < kb * kNb; idx += blockDim.x) {
-        const int t = idx % kNb, c = idx / kNb;
-        Dg[t][c] = (t <= c && t < kb) ? Lb[static_cast<int64_t>(k + c) * f + k + t] : 0.0;
-    }
-}
-
-// z = inv(L_kk) y (forward) with z[c] = y[c] / L_cc + sum_{t<c} M[t][c] y[t]
-__device__ __forceinline__ void diag_fwd(const double (*Dg)[kNb + 1], const double (*yb)[kMaxRhs],
-                                         double (*z)[kMaxRhs], int kb, int nrhs) {
-    const int c = threadIdx.x % kNb;
-    if (c < kb) {
-        const double dinv = 1.0 / Dg[c][c];
-        for (int r = threadIdx.x / kNb; r < nrhs; r += blockDim.x / kNb) {
-            double s2 = yb[c][r] * dinv;
-            for (int t = 0; t < c; ++t) s2 += Dg[t][c] * yb[t][r];
-            z[c][r] = s2;
-        }
-    }
-}
-
-// u = inv(L_kk)^T v (backward): u[c] = v[c] / L_cc + sum_{t>c} M[c][t] v[t]
-__device__ __forceinline__ void diag_bwd(const double (*Dg)[kNb + 1], const double (*v)[kMaxRhs],
-                                         double (*u)[kMaxRhs], int kb, int nrhs) {
-    const int c = threadIdx.x % kNb;
-    if (c < kb) {
-        const double dinv = 1.0 / Dg[c][c];
-        for (int r = threadIdx.x / kNb; r < nrhs; r += blockDim.x / kNb) {
-            double s2 = v[c][r] * dinv;
-            for (int t = c + 1; t < kb; ++t) s2 += Dg[c][t] * v[t][r];
-            u[c][r] = s2;
-        }
-    }
-}
-
-// Forward, one workgroup per front: blocks of 64 pivots, z = inv(L_kk) y_blk
-// through the stored inverse, then the rows below (own pivot rows in place,
-// ancestor rows by atomic add: sibling fronts of a level share ancestors).
-__global__ __launch_bounds__(kThreads) void k_fwd(DevPtrs d, const int32_t *__restrict__ flist,
-                                                 double *__restrict__ y, int32_t n, int32_t nrhs) {
-    __shared__ double Dg[kNb][kNb + 1];
-    __shared__ double yb[kNb][kMaxRhs];
-    __shared__ double z[kNb][kMaxRhs];
-    const int s = flist[blockIdx.x];
-    const FrontDev F = d.fronts[s];
-    const int f = F.f, p = F.p;
-    const double *Lb = d.L + F.lofs;
-    const int32_t *frows = d.rows + F.rofs;
-    const int tid = threadIdx.x;
-    for (int k = 0; k < p; k += kNb) {
-        const int kb = min(kNb, p - k);
-        stage_diag(Dg, Lb, f, k, kb);
-        for (int idx = tid; idx < kb * nrhs; idx += kThreads) {
-            const int t = idx % kb, r = idx / kb;
-            yb[t][r] = y[F.col0 + k + t + static_cast<int64_t>(r) * n];
-        }
-        __syncthreads();
-        diag_fwd(Dg, yb, z, kb, nrhs);
-        __syncthreads();
-        for (int idx = tid; idx < kb * nrhs; idx += kThreads) {
-            const int t = idx % kb, r = idx / kb;
-            y[F.col0 + k + t + static_cast<int64_t>(r) * n] = z[t][r];
-        }
-        for (int i = k + kb + tid; i < f; i += kThreads) {
-            double acc[kMaxRhs];
-#pragma unroll
-            for (int r = 0; r < kMaxRhs; ++r) acc[r] = 0.0;
-            for (int t = 0; t < kb; ++t) {
-                const double l = Lb[static_cast<int64_t>(k + t) * f + i];
-#pragma unroll
-                for (int r = 0; r < kMaxRhs; ++r)
-                    if (r < nrhs) acc[r] += l * z[t][r];
-            }
-            if (i < p) {
-                for (int r = 0; r < nrhs; ++r) y[F.col0 + i + static_cast<int64_t>(r) * n] -= acc[r];
-            } else {
-                const int64_t gi = frows[i];
-                for (int r = 0; r < nrhs; ++r) atomicAdd(&y[gi + static_cast<int64_t>(r) * n], -acc[r]);
-            }
-        }
-        __threadfence_block();
-        __syncthreads();
-    }
-}
-
-// Backward, one workgroup per front, blocks from the last to the first:
-// S = L[below, blk]^T x[below] over 64-row tiles staged in LDS (coalesced
-// column reads), then x_blk = inv(L_kk)^T (x_blk - S).
-__global__ __launch_bounds__(kThreads) void k_bwd(DevPtrs d, const int32_t *__restrict__ flist,
-                                                 double *__restrict__ y, int32_t n, int32_t nrhs) {
-    __shared__ double tile[kNb][kNb + 1];
-    __shared__ double xs[kNb][kMaxRhs];
-    __shared__ double v[kNb][kMaxRhs];
-    __shared__ double u[kNb][kMaxRhs];
-    __shared__ double part[4][kNb][kMaxRhs];
-    const int s = flist[blockIdx.x];
-    const FrontDev F = d.fronts[s];
-    const int f = F.f, p = F.p;
-    const double *Lb = d.L + F.lofs;
-    const int32_t *frows = d.rows + F.rofs;
-    const int tid = threadIdx.x;
-    const int cl = tid % kNb, rg = tid / kNb;
-    const int nblk = (p + kNb - 1) / kNb;
-    for (int bi = nblk - 1; bi >= 0; --bi) {
-        const int k = bi * kNb;
-        const int kb = min(kNb, p - k);
-        double acc[kMaxRhs];
-#pragma unroll
-        for (int q = 0; q < kMaxRhs; ++q) acc[q] = 0.0;
-        for (int r0 = k + kb; r0 < f; r0 += kNb) {
-            const int nr = min(kNb, f - r0);
-            __syncthreads();
-            for (int idx = tid; idx < kNb * kNb; idx += kThreads) {
-                const int c = idx / kNb, r = idx % kNb;
-                tile[r][c] = (c < kb && r < nr) ? Lb[static_cast<int64_t>(k + c) * f + r0 + r] : 0.0;
-            }
-            for (int idx = tid; idx < kNb * nrhs; idx += kThreads) {
-                const int r = idx % kNb, q = idx / kNb;
-                const int i = r0 + r;
-                const int64_t gi = i < p ? static_cast<int64_t>(F.col0 + i) : static_cast<int64_t>(frows[min(i, f - 1)]);
-                xs[r][q] = r < nr ? y[gi + static_cast<int64_t>(q) * n] : 0.0;
-            }
-            __syncthreads();
-            for (int r = rg * 16; r < rg * 16 + 16; ++r) {
-                const double l = tile[r][cl];
-#pragma unroll
-                for (int q = 0; q < kMaxRhs; ++q)
-                    if (q < nrhs) acc[q] += l * xs[r][q];
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < kMaxRhs; ++q)
-            if (q < nrhs) part[rg][cl][q] = acc[q];
-        __syncthreads();
-        // the diagonal block into `tile` (Dg layout) and v = x_blk - S
-        stage_diag(tile, Lb, f, k, kb);
-        for (int idx = tid; idx < kb * nrhs; idx += kThreads) {
-            const int t = idx % kb, r = idx / kb;
-            v[t][r] = y[F.col0 + k + t + static_cast<int64_t>(r) * n] -
-                      (part[0][t][r] + part[1][t][r] + part[2][t][r] + part[3][t][r]);
-        }
-        __syncthreads();
-        diag_bwd(tile, v, u, kb, nrhs);
-        __syncthreads();
-        for (int idx = tid; idx < kb * nrhs; idx += kThreads) {
-            const int t = idx % kb, r = idx / kb;
-            y[F.col0 + k + t + static_cast<int64_t>(r) * n] = u[t][r];
-        }
-        __threadfence_block();
-        __syncthreads();
-    }
-}
-
-void launch_fwd(hipStream_t st, const DevPtrs &d, const int32_t *fronts, int32_t nf, double *y, int32_t n,
-                int32_t nrhs) {
-    if (nf <= 0) return;
-    k_fwd<<<dim3(nf), kThreads, 0, st>>>(d, fronts, y, n, nrhs);
-}
-
-void launch_bwd(hipStream_t st, const DevPtrs &d, const int32_t *fronts, int32_t nf, double *y, int32_t n,
-                int32_t nrhs) {
-    if (nf <= 0) return;
-    k_bwd<<<dim3(nf), kThreads, 0, st>>>(d, fronts, y, n, nrhs);
-}
-
-// ---------------------------------------------------------------------------
-// Large fronts: the sweep is split into blocks of kBig pivots; each block is
-// one single-workgroup kernel for its triangle (four 64-pivot sub-blocks
-// through the stored inverses) and one many-workgroup kernel for the rest
-// of the front (forward: the rows below; backward: the accumulators S of
-// the columns before). Kernel boundaries order the steps: no workgroup
-// waits on another.
-constexpr int kBig = 256;
-
-// Forward triangle of block [k0, k0 + kb): y_blk <- inv(L_blk) y_blk.
-__global__ __launch_bounds__(kThreads) void k_fwd_blk_diag(DevPtrs d, int32_t s, int32_t k0, int32_t kb,
-                                                          double *__restrict__ y, int32_t n, int32_t nrhs) {
-    __shared__ double Dg[kNb][kNb + 1];
-    __shared__ double yb[kBig][kMaxRhs];
-    __shared__ double z[kNb][kMaxRhs];
-    const FrontDev F = d.fronts[s];
-    const int f = F.f;
-    const double *Lb = d.L + F.lofs;
-    const int tid = threadIdx.x;
-    for (int idx = tid; idx < kb * nrhs; idx += kThreads) {
-        const int t = idx % kb, r = idx / kb;
-        yb[t][r] = y[F.col0 + k0 + t + static_cast<int64_t>(r) * n];
-    }
-    for (int sb = 0; sb < kb; sb += kNb) {
-        const int sbn = min(kNb, kb - sb);
-        stage_diag(Dg, Lb, f, k0 + sb, sbn);
-        __syncthreads();
-        diag_fwd(Dg, yb + sb, z, sbn, nrhs);
-        __syncthreads();
-        for (int idx = tid; idx < sbn * nrhs; idx += kThreads) {
-            const int t = idx % sbn, r = idx / sbn;
-            yb[sb + t][r] = z[t][r];
-        }
-        // rows of the block below this sub-block
-        for (int i = sb + sbn + tid; i < kb; i += kThreads) {
-            for (int r = 0; r < nrhs; ++r) {
-                double acc = 0.0;
-                for (int t = 0; t < sbn; ++t) acc += Lb[static_cast<int64_t>(k0 + sb + t) * f + k0 + i] * z[t][r];
-                yb[i][r] -= acc;
-            }
-        }
-        __syncthreads();
-    }
-    for (int idx = tid; idx < kb * nrhs; idx += kThreads) {
-        const int t = idx % kb, r = idx / kb;
-        y[F.col0 + k0 + t + static_cast<int64_t>(r) * n] = yb[t][r];
-    }
-}
-
-// Forward update of rows [k0 + kb, f): one workgroup per 64-row chunk.
-__global__ __launch_bounds__(kThreads) void k_fwd_blk_upd(DevPtrs d, int32_t s, int32_t k0, int32_t kb,
-                                                         double *__restrict__ y, int32_t n, int32_t nrhs) {
-    __shared__ double z[kBig][kMaxRhs];
-    __shared__ double part[4][kNb][kMaxRhs];
-    const FrontDev F = d.fronts[s];
-    const int f = F.f, p = F.p;
-    const double *Lb = d.L + F.lofs;
-    const int32_t *frows = d.rows + F.rofs;
-    const int tid = threadIdx.x;
-    for (int idx = tid; idx < kb * nrhs; idx += kThreads) {
-        const int t = idx % kb, r = idx / kb;
-        z[t][r] = y[F.col0 + k0 + t + static_cast<int64_t>(r) * n];
-    }
-    __syncthreads();
-    const int rl = tid % kNb, cg = tid / kNb;
-    const int row = k0 + kb + blockIdx.x * kNb + rl;
-    double acc[kMaxRhs];
-#pragma unroll
-    for (int r = 0; r < kMaxRhs; ++r) acc[r] = 0.0;
-    if (row < f) {
-        const int t0 = cg * (kBig / 4), t1 = min(kb, t0 + kBig / 4);
-        for (int t = t0; t < t1; ++t) {
-            const double l = Lb[static_cast<int64_t>(k0 + t) * f + row];
-#pragma unroll
-            for (int r = 0; r < kMaxRhs; ++r)
-                if (r < nrhs) acc[r] += l * z[t][r];
-        }
-    }
-#pragma unroll
-    for (int r = 0; r < kMaxRhs; ++r)
-        if (r < nrhs) part[cg][rl][r] = acc[r];
-    __syncthreads();
-    if (tid < kNb) {
-        const int rr = k0 + kb + blockIdx.x * kNb + tid;
-        if (rr < f) {
-            const int64_t gi = rr < p ? static_cast<int64_t>(F.col0 + rr) : static_cast<int64_t>(frows[rr]);
-            for (int r = 0; r < nrhs; ++r)
-                y[gi + static_cast<int64_t>(r) * n] -= part[0][tid][r] + part[1][tid][r] + part[2][tid][r] + part[3][tid][r];
-        }
-    }
-}
-
-// Backward, first: S[c] = sum_{i >= p} L[i][c] x[rows[i]] for every pivot
-// column c; one workgroup per 64-column chunk.
-__global__ __launch_bounds__(kThreads) void k_bwd_blk_q(DevPtrs d, int32_t s, const double *__restrict__ y, int32_t n,
-                                                       int32_t nrhs, double *__restrict__ sbuf) {
-    __shared__ double tile[kNb][kNb + 1];
-    __shared__ double xs[kNb][kMaxRhs];
-    __shared__ double part[4][kNb][kMaxRhs];
-    const FrontDev F = d.fronts[s];
-    const int f = F.f, p = F.p;
-    const double *Lb = d.L + F.lofs;
-    const int32_t *frows = d.rows + F.rofs;
-    const int tid = threadIdx.x;
-    const int cl = tid % kNb, rg = tid / kNb;
-    const int c0 = blockIdx.x * kNb;
-    const int cn = min(kNb, p - c0);
-    double acc[kMaxRhs];
-#pragma unroll
-    for (int q = 0; q < kMaxRhs; ++q) acc[q] = 0.0;
-    for (int r0 = p; r0 < f; r0 += kNb) {
-        const int nr = min(kNb, f - r0);
-        __syncthreads();
-        for (int idx = tid; idx < kNb * kNb; idx += kThreads) {
-            const int c = idx / kNb, r = idx % kNb;
-            tile[r][c] = (c < cn && r < nr) ? Lb[static_cast<int64_t>(c0 + c) * f + r0 + r] : 0.0;
-        }
-        for (int idx = tid; idx < kNb * nrhs; idx += kThreads) {
-            const int r = idx % kNb, q = idx / kNb;
-            xs[r][q] = r < nr ? y[static_cast<int64_t>(frows[r0 + r]) + static_cast<int64_t>(q) * n] : 0.0;
-        }
-        __syncthreads();
-        for (int r = rg * 16; r < rg * 16 + 16; ++r) {
-            const double l = tile[r][cl];
-#pragma unroll
-            for (int q = 0; q < kMaxRhs; ++q)
-                if (q < nrhs) acc[q] += l * xs[r][q];
-        }
-    }
-#pragma unroll
-    for (int q = 0; q < kMaxRhs; ++q)
-        if (q < nrhs) part[rg][cl][q] = acc[q];
-    __syncthreads();
-    if (tid < cn)
-        for (int q = 0; q < nrhs; ++q)
-            sbuf[static_cast<int64_t>(c0 + tid) * kMaxRhs + q] =
-                part[0][tid][q] + part[1][tid][q] + part[2][tid][q] + part[3][tid][q];
-}
-
-// Backward triangle of block [k0, k0 + kb): sub-blocks from the last,
-// x_sub = inv(L_sub)^T (x_sub - S_sub), then S of the earlier sub-blocks of
-// this block.
-__global__ __launch_bounds__(kThreads) void k_bwd_blk_diag(DevPtrs d, int32_t s, int32_t k0, int32_t kb,
-                                                          double *__restrict__ y, int32_t n, int32_t nrhs,
-                                                          double *__restrict__ sbuf) {
-    __shared__ double Dg[kNb][kNb + 1];
-    __shared__ double v[kNb][kMaxRhs];
-    __shared__ double u[kNb][kMaxRhs];
-    __shared__ double S[kBig][kMaxRhs];
-    const FrontDev F = d.fronts[s];
-    const int f = F.f;
-    const double *Lb = d.L + F.lofs;
-    const int tid = threadIdx.x;
-    for (int idx = tid; idx < kb * nrhs; idx += kThreads) {
-        const int t = idx % kb, r = idx / kb;
-        S[t][r] = sbuf[static_cast<int64_t>(k0 + t) * kMaxRhs + r];
-    }
-    const int nsb = (kb + kNb - 1) / kNb;
-    for (int si = nsb - 1; si >= 0; --si) {
-        const int sb = si * kNb;
-        const int sbn = min(kNb, kb - sb);
-        __syncthreads();
-        stage_diag(Dg, Lb, f, k0 + sb, sbn);
-        for (int idx = tid; idx < sbn * nrhs; idx += kThreads) {
-            const int t = idx % sbn, r = idx / sbn;
-            v[t][r] = y[F.col0 + k0 + sb + t + static_cast<int64_t>(r) * n] - S[sb + t][r];
-        }
-        __syncthreads();
-        diag_bwd(Dg, v, u, sbn, nrhs);
-        __syncthreads();
-        for (int idx = tid; idx < sbn * nrhs; idx += kThreads) {
-            const int t = idx % sbn, r = idx / sbn;
-            y[F.col0 + k0 + sb + t + static_cast<int64_t>(r) * n] = u[t][r];
-        }
-        // S of the earlier columns of this block: S[c] += L[sub rows][c] u
-        for (int c = tid; c < sb; c += kThreads) {
-            for (int r = 0; r < nrhs; ++r) {
-                double acc = 0.0;
-                for (int t = 0; t < sbn; ++t) acc += Lb[static_cast<int64_t>(k0 + c) * f + k0 + sb + t] * u[t][r];
-                S[c][r] += acc;
-            }
-        }
-    }
-}
-
-// Backward update: S[c] += sum_{r in block} L[r][c] x[r] for c in [0, k0),
-// one workgroup per 64-column chunk (the block's rows in 64-row tiles).
-__global__ __launch_bounds__(kThreads) void k_bwd_blk_upd(DevPtrs d, int32_t s, int32_t k0, int32_t kb,
-                                                         const double *__restrict__ y, int32_t n, int32_t nrhs,
-                                                         double *__restrict__ sbuf) {
-    __shared__ double tile[kNb][kNb + 1];
-    __shared__ double xs[kNb][kMaxRhs];
-    __shared__ double part[4][kNb][kMaxRhs];
-    const FrontDev F = d.fronts[s];
-    const int f = F.f;
-    const double *Lb = d.L + F.lofs;
-    const int tid = threadIdx.x;
-    const int cl = tid % kNb, rg = tid / kNb;
-    const int c0 = blockIdx.x * kNb;
-    const int cn = min(kNb, k0 - c0);
-    double acc[kMaxRhs];
-#pragma unroll
-    for (int q = 0; q < kMaxRhs; ++q) acc[q] = 0.0;
-    for (int r0 = 0; r0 < kb; r0 += kNb) {
-        const int nr = min(kNb, kb - r0);
-        __syncthreads();
-        for (int idx = tid; idx < kNb * kNb; idx += kThreads) {
-            const int c = idx / kNb, r = idx % kNb;
-            tile[r][c] = (c < cn && r < nr) ? Lb[static_cast<int64_t>(c0 + c) * f + k0 + r0 + r] : 0.0;
-        }
-        for (int idx = tid; idx < kNb * nrhs; idx += kThreads) {
-            const int r = idx % kNb, q = idx / kNb;
-            xs[r][q] = r < nr ? y[F.col0 + k0 + r0 + r + static_cast<int64_t>(q) * n] : 0.0;
-        }
-        __syncthreads();
-        for (int r = rg * 16; r < rg * 16 + 16; ++r) {
-            const double l = tile[r][cl];
-#pragma unroll
-            for (int q = 0; q < kMaxRhs; ++q)
-                if (q < nrhs) acc[q] += l * xs[r][q];
-        }
-        (void)f;
-    }
-#pragma unroll
-    for (int q = 0; q < kMaxRhs; ++q)
-        if (q < nrhs) part[rg][cl][q] = acc[q];
-    __syncthreads();
-    if (tid < cn)
-        for (int q = 0; q < nrhs; ++q)
-            sbuf[static_cast<int64_t>(c0 + tid) * kMaxRhs + q] +=
-                part[0][tid][q] + part[1][tid][q] + part[2][tid][q] + part[3][tid][q];
-}
-
-void launch_fwd_big(hipStream_t st, const DevPtrs &d, int32_t s, int32_t f, int32_t p, double *y, int32_t n,
-                    int32_t nrhs) {
-    for (int32_t k0 = 0; k0 < p; k0 += kBig) {
-        const int32_t kb = std::min(kBig, p - k0);
-        k_fwd_blk_diag<<<1, kThreads, 0, st>>>(d, s, k0, kb, y, n, nrhs);
-        const int32_t rows = f - (k0 + kb);
-        if (rows > 0)
-            k_fwd_blk_upd<<<dim3((rows + kNb - 1) / kNb), kThreads, 0, st>>>(d, s, k0, kb, y, n, nrhs);
-    }
-}
-
-void launch_bwd_big(hipStream_t st, const DevPtrs &d, int32_t s, int32_t f, int32_t p, double *y, int32_t n,
-                    int32_t nrhs, double *sbuf) {
-    k_bwd_blk_q<<<dim3((p + kNb - 1) / kNb), kThreads, 0, st>>>(d, s, y, n, nrhs, sbuf);
-    const int32_t nb = (p + kBig - 1) / kBig;
-    for (int32_t bi = nb - 1; bi >= 0; --bi) {
-        const int32_t k0 = bi * kBig;
-        const int32_t kb = std::min(kBig, p - k0);
-        k_bwd_blk_diag<<<1, kThreads, 0, st>>>(d, s, k0, kb, y, n, nrhs, sbuf);
-        if (k0 > 0) k_bwd_blk_upd<<<dim3((k0 + kNb - 1) / kNb), kThreads, 0, st>>>(d, s, k0, kb, y, n, nrhs, sbuf);
-    }
-    (void)f;
 }
 
 // ---------------------------------------------------------------------------

@@ -1,0 +1,334 @@
+// A15: forward and backward sweeps of the SPD factor, level by level.
+//
+// A front's pivot columns are solved in blocks of kSB = 128 (two 64-pivot
+// diagonal blocks whose inverses the Cholesky kernel stored in their strict
+// upper triangles). One launch per step covers every front of a level:
+// in step j one workgroup per front (the "triangle" item) applies block
+// j-1's update to block j's rows and solves block j, while the other items
+// apply block j-1's update to the rest of the front (forward: 64-row chunks;
+// backward: 64-column chunks). Kernel boundaries order the steps, no
+// workgroup waits for another. Reference: solve.cpp:858-1075 (the sweeps),
+// solve_forward.cpp / solve_backward.cpp (per-front dense steps).
+//
+// y is the permuted right-hand side block, n x nrhs (ld n). Rows of a front
+// below its pivots belong to ancestors that sibling fronts of the same level
+// also update: those forward updates are atomic adds.
+#include <hip/hip_runtime.h>
+
+#include "smfgpu_internal.hpp"
+
+namespace smfgpu {
+namespace {
+
+constexpr int kSB = 128;        // pivot columns per step
+constexpr int kSD = 64;         // diagonal block (stored inverse)
+constexpr int kSThreads = 512;  // eight waves
+constexpr int kSeg = 512;       // rows per backward update item
+
+struct TriRegs {
+    double a[8], b[8], c[8];  // Dg_a, L_ba, Dg_b of the triangle, element (rl, g + 8 e)
+};
+
+// Element (t = rl, c = g + 8 e) of the triangle's three 64 x 64 blocks:
+// a: upper part of the first diagonal block (t <= c: inverse rows and the
+// diagonal), b: rows 64.. of the block below it, c: second diagonal block.
+__device__ __forceinline__ void tri_prefetch(TriRegs &R, const double *Lb, int f, int tk, int tkb, int rl, int g) {
+    const int na = min(kSD, tkb), nb = tkb - kSD;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const int c = g + 8 * e;
+        R.a[e] = (rl <= c && c < na) ? Lb[static_cast<int64_t>(tk + c) * f + tk + rl] : 0.0;
+        R.b[e] = (rl < nb && c < na) ? Lb[static_cast<int64_t>(tk + c) * f + tk + kSD + rl] : 0.0;
+        R.c[e] = (rl <= c && c < nb) ? Lb[static_cast<int64_t>(tk + kSD + c) * f + tk + kSD + rl] : 0.0;
+    }
+}
+
+__device__ __forceinline__ void put_block(double (*Dg)[kSD + 1], const double (&v)[8], int rl, int g) {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) Dg[rl][g + 8 * e] = v[e];
+}
+
+// z[c] = y[c] / L_cc + sum_{t<c} M[t][c] y[t]  (M[t][c] = inv(L)[c][t])
+template <int NR>
+__device__ __forceinline__ void tri_fwd(const double (*Dg)[kSD + 1], const double (*yb)[NR], double (*z)[NR], int nb,
+                                        int nrhs) {
+    const int c = threadIdx.x & 63, r = threadIdx.x >> 6;
+    if (r < NR && r < nrhs && c < nb) {
+        double s = yb[c][r] / Dg[c][c];
+        for (int t = 0; t < c; ++t) s += Dg[t][c] * yb[t][r];
+        z[c][r] = s;
+    }
+}
+
+// x[c] = v[c] / L_cc + sum_{t>c} M[c][t] v[t]
+template <int NR>
+__device__ __forceinline__ void tri_bwd(const double (*Dg)[kSD + 1], const double (*v)[NR], double (*x)[NR], int nb,
+                                        int nrhs) {
+    const int c = threadIdx.x & 63, r = threadIdx.x >> 6;
+    if (r < NR && r < nrhs && c < nb) {
+        double s = v[c][r] / Dg[c][c];
+        for (int t = c + 1; t < nb; ++t) s += Dg[c][t] * v[t][r];
+        x[c][r] = s;
+    }
+}
+
+// Forward update of rows [lo, hi): y[row] -= sum_{t < kb} L[row][k + t] zs[t].
+// Rows in the triangle [tk, tk + tkb) go to yb (LDS), other own pivot rows
+// in place, ancestor rows by atomic add.
+template <int NR>
+__device__ void fwd_update(const double *Lb, const int32_t *frows, int f, int p, int col0, int k, int kb, int lo,
+                           int hi, const double (*zs)[NR], double (*part)[64][NR], double (*yb)[NR], int tk, int tkb,
+                           double *y, int n, int nrhs) {
+    const int tid = threadIdx.x, rl = tid & 63, g = tid >> 6;
+    for (int r0 = lo; r0 < hi; r0 += 64) {
+        const int row = r0 + rl;
+        double acc[NR];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) acc[r] = 0.0;
+        if (row < hi) {
+            const double *col = Lb + static_cast<int64_t>(k + g * 16) * f + row;
+            double l[16];
+#pragma unroll
+            for (int tt = 0; tt < 16; ++tt) l[tt] = (g * 16 + tt < kb) ? col[static_cast<int64_t>(tt) * f] : 0.0;
+#pragma unroll
+            for (int tt = 0; tt < 16; ++tt)
+#pragma unroll
+                for (int r = 0; r < NR; ++r) acc[r] += l[tt] * zs[g * 16 + tt][r];
+        }
+#pragma unroll
+        for (int r = 0; r < NR; ++r) part[g][rl][r] = acc[r];
+        __syncthreads();
+        if (tid < 64 * NR) {
+            const int rr = tid & 63, r = tid >> 6;
+            const int i = r0 + rr;
+            if (i < hi && r < nrhs) {
+                double s = 0.0;
+#pragma unroll
+                for (int g2 = 0; g2 < 8; ++g2) s += part[g2][rr][r];
+                if (i >= tk && i < tk + tkb) {
+                    yb[i - tk][r] = y[col0 + i + static_cast<int64_t>(r) * n] - s;
+                } else if (i < p) {
+                    y[col0 + i + static_cast<int64_t>(r) * n] -= s;
+                } else {
+                    unsafeAtomicAdd(&y[static_cast<int64_t>(frows[i]) + static_cast<int64_t>(r) * n], -s);
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <int NR>
+__global__ __launch_bounds__(kSThreads) void k_fwd_step(DevPtrs d, const SolveItem *__restrict__ items,
+                                                       double *__restrict__ y, int32_t n, int32_t nrhs) {
+    __shared__ double Dg[kSD][kSD + 1];
+    __shared__ double zs[kSB][NR];
+    __shared__ double yb[kSB][NR];
+    __shared__ double part[8][64][NR];
+    const SolveItem it = items[blockIdx.x];
+    const FrontDev F = d.fronts[it.s];
+    const int f = F.f, p = F.p;
+    const double *Lb = d.L + F.lofs;
+    const int32_t *frows = d.rows + F.rofs;
+    const int tid = threadIdx.x, rl = tid & 63, g = tid >> 6;
+    TriRegs R;
+    if (it.tkb > 0) tri_prefetch(R, Lb, f, it.tk, it.tkb, rl, g);
+    for (int idx = tid; idx < kSB * NR; idx += kSThreads) {
+        const int t = idx % kSB, r = idx / kSB;
+        zs[t][r] = (t < it.kb && r < nrhs) ? y[F.col0 + it.k + t + static_cast<int64_t>(r) * n] : 0.0;
+        if (it.tkb > 0 && it.kb == 0) yb[t][r] = (t < it.tkb && r < nrhs) ? y[F.col0 + it.tk + t + static_cast<int64_t>(r) * n] : 0.0;
+    }
+    __syncthreads();
+    if (it.kb > 0)
+        fwd_update<NR>(Lb, frows, f, p, F.col0, it.k, it.kb, it.a0, it.a1, zs, part, yb, it.tk, it.tkb, y, n, nrhs);
+    if (it.tkb <= 0) return;
+    // the triangle: z_a = inv(L_a) y_a; y_b -= L_ba z_a; z_b = inv(L_b) y_b
+    const int na = min(kSD, it.tkb), nb = it.tkb - kSD;
+    put_block(Dg, R.a, rl, g);
+    __syncthreads();
+    tri_fwd<NR>(Dg, yb, zs, na, nrhs);
+    __syncthreads();
+    if (nb > 0) {
+        double acc[NR];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            acc[r] = 0.0;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) acc[r] += R.b[e] * zs[g + 8 * e][r];
+            part[g][rl][r] = acc[r];
+        }
+        put_block(Dg, R.c, rl, g);
+        __syncthreads();
+        if (tid < 64 * NR) {
+            const int rr = tid & 63, r = tid >> 6;
+            if (rr < nb && r < nrhs) {
+                double s = 0.0;
+#pragma unroll
+                for (int g2 = 0; g2 < 8; ++g2) s += part[g2][rr][r];
+                yb[kSD + rr][r] -= s;
+            }
+        }
+        __syncthreads();
+        tri_fwd<NR>(Dg, yb + kSD, zs + kSD, nb, nrhs);
+        __syncthreads();
+    }
+    // zs now holds z of the whole block (entries past tkb are still zero)
+    for (int idx = tid; idx < it.tkb * nrhs; idx += kSThreads) {
+        const int t = idx % it.tkb, r = idx / it.tkb;
+        y[F.col0 + it.tk + t + static_cast<int64_t>(r) * n] = zs[t][r];
+    }
+    if (it.flags & 2)  // the rows below the last block, by this workgroup
+        fwd_update<NR>(Lb, frows, f, p, F.col0, it.tk, it.tkb, it.tk + it.tkb, f, zs, part, yb, -1, 0, y, n, nrhs);
+}
+
+// Backward update of columns [c0, c1): y[col0 + c] -= sum_{i in [r0, r1)}
+// L[i][c] x_i, x_i = y[col0 + i] for own pivot rows, y[rows[i]] else.
+// Lanes: 16 rows x 4 columns; wave g owns columns 8 g + {cq, cq + 4} of
+// each 64-column tile. Columns of the triangle go to v (LDS).
+template <int NR>
+__device__ void bwd_update(const double *Lb, const int32_t *frows, int f, int p, int col0, int r0, int r1, int c0,
+                           int c1, bool atomic, double (*xs)[NR], double (*v)[NR], int tk, int tkb,
+                           double *y, int n, int nrhs) {
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, g = tid >> 6, rq = lane & 15, cq = lane >> 4;
+    const int nr = r1 - r0;
+    for (int idx = tid; idx < kSeg * NR; idx += kSThreads) {
+        const int i = idx % kSeg, r = idx / kSeg;
+        double xv = 0.0;
+        if (i < nr && r < nrhs) {
+            const int row = r0 + i;
+            const int64_t gi = row < p ? static_cast<int64_t>(col0 + row) : static_cast<int64_t>(frows[row]);
+            xv = y[gi + static_cast<int64_t>(r) * n];
+        }
+        xs[i][r] = xv;
+    }
+    __syncthreads();
+    for (int cb = c0; cb < c1; cb += 64) {
+        const int ca = cb + 8 * g + cq, cbb = ca + 4;
+        const bool va = ca < c1, vb = cbb < c1;
+        const double *La = Lb + static_cast<int64_t>(va ? ca : c0) * f + r0;
+        const double *Lbb = Lb + static_cast<int64_t>(vb ? cbb : c0) * f + r0;
+        double acc0[NR], acc1[NR];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) acc0[r] = acc1[r] = 0.0;
+        for (int i0 = 0; i0 < nr; i0 += 128) {
+            double la[8], lb[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int i = i0 + 16 * u + rq;
+                la[u] = (va && i < nr) ? La[i] : 0.0;
+                lb[u] = (vb && i < nr) ? Lbb[i] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int i = min(i0 + 16 * u + rq, kSeg - 1);
+#pragma unroll
+                for (int r = 0; r < NR; ++r) {
+                    acc0[r] += la[u] * xs[i][r];
+                    acc1[r] += lb[u] * xs[i][r];
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+#pragma unroll
+            for (int o = 8; o >= 1; o >>= 1) {
+                acc0[r] += __shfl_xor(acc0[r], o);
+                acc1[r] += __shfl_xor(acc1[r], o);
+            }
+        }
+        if (rq == 0) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int c = h ? cbb : ca;
+                if (!(h ? vb : va)) continue;
+#pragma unroll
+                for (int r = 0; r < NR; ++r) {
+                    if (r >= nrhs) break;
+                    const double s = h ? acc1[r] : acc0[r];
+                    double *dst = &y[col0 + c + static_cast<int64_t>(r) * n];
+                    if (c >= tk && c < tk + tkb) v[c - tk][r] = *dst - s;
+                    else if (atomic) unsafeAtomicAdd(dst, -s);
+                    else *dst -= s;
+                }
+            }
+        }
+    }
+    __syncthreads();
+}
+
+template <int NR>
+__global__ __launch_bounds__(kSThreads) void k_bwd_step(DevPtrs d, const SolveItem *__restrict__ items,
+                                                       double *__restrict__ y, int32_t n, int32_t nrhs) {
+    __shared__ double Dg[kSD][kSD + 1];
+    __shared__ double xs[kSeg][NR];
+    __shared__ double v[kSB][NR];
+    __shared__ double xo[kSB][NR];
+    const SolveItem it = items[blockIdx.x];
+    const FrontDev F = d.fronts[it.s];
+    const int f = F.f, p = F.p;
+    const double *Lb = d.L + F.lofs;
+    const int32_t *frows = d.rows + F.rofs;
+    const int tid = threadIdx.x, rl = tid & 63, g = tid >> 6;
+    TriRegs R;
+    if (it.tkb > 0) {
+        tri_prefetch(R, Lb, f, it.tk, it.tkb, rl, g);
+        if (it.kb == 0)
+            for (int idx = tid; idx < kSB * NR; idx += kSThreads) {
+                const int t = idx % kSB, r = idx / kSB;
+                v[t][r] = (t < it.tkb && r < nrhs) ? y[F.col0 + it.tk + t + static_cast<int64_t>(r) * n] : 0.0;
+            }
+    }
+    if (it.kb > 0)
+        bwd_update<NR>(Lb, frows, f, p, F.col0, it.k, it.k + it.kb, it.a0, it.a1, (it.flags & 1) != 0, xs, v, it.tk,
+                       it.tkb, y, n, nrhs);
+    if (it.tkb <= 0) return;
+    // the triangle, second diagonal block first:
+    // x_b = inv(L_b)^T v_b; v_a -= L_ba^T x_b; x_a = inv(L_a)^T v_a
+    const int na = min(kSD, it.tkb), nb = it.tkb - kSD;
+    if (nb > 0) {
+        put_block(Dg, R.c, rl, g);
+        __syncthreads();
+        tri_bwd<NR>(Dg, v + kSD, xo + kSD, nb, nrhs);
+        __syncthreads();
+        put_block(Dg, R.b, rl, g);  // Dg[t][c] = L_ba[t][c]
+        __syncthreads();
+        {
+            const int c = tid & 63, r = tid >> 6;
+            if (r < NR && r < nrhs && c < na) {
+                double s = 0.0;
+                for (int t = 0; t < nb; ++t) s += Dg[t][c] * xo[kSD + t][r];
+                v[c][r] -= s;
+            }
+        }
+        __syncthreads();
+    }
+    put_block(Dg, R.a, rl, g);
+    __syncthreads();
+    tri_bwd<NR>(Dg, v, xo, na, nrhs);
+    __syncthreads();
+    for (int idx = tid; idx < it.tkb * nrhs; idx += kSThreads) {
+        const int t = idx % it.tkb, r = idx / it.tkb;
+        y[F.col0 + it.tk + t + static_cast<int64_t>(r) * n] = xo[t][r];
+    }
+}
+
+}  // namespace
+
+void launch_fwd_step(hipStream_t st, const DevPtrs &d, const SolveItem *items, int64_t count, double *y, int32_t n,
+                     int32_t nrhs) {
+    if (count <= 0) return;
+    const dim3 grid(static_cast<unsigned>(count));
+    if (nrhs == 1) k_fwd_step<1><<<grid, kSThreads, 0, st>>>(d, items, y, n, nrhs);
+    else k_fwd_step<4><<<grid, kSThreads, 0, st>>>(d, items, y, n, nrhs);
+}
+
+void launch_bwd_step(hipStream_t st, const DevPtrs &d, const SolveItem *items, int64_t count, double *y, int32_t n,
+                     int32_t nrhs) {
+    if (count <= 0) return;
+    const dim3 grid(static_cast<unsigned>(count));
+    if (nrhs == 1) k_bwd_step<1><<<grid, kSThreads, 0, st>>>(d, items, y, n, nrhs);
+    else k_bwd_step<4><<<grid, kSThreads, 0, st>>>(d, items, y, n, nrhs);
+}
+
+}  // namespace smfgpu

@@ -48,39 +48,106 @@ DevPtrs dev_ptrs(smfgpu_plan_s *P) {
     return d;
 }
 
-// Level front lists of the solves: small fronts one workgroup each, large
-// ones (f p >= solve_big_min) pipelined over a cooperative grid.
+// Solve schedule (static for SPD). Per front, pivot blocks of
+// kSolveBlock columns; step j of the forward sweep solves block j's triangle
+// (one workgroup, after applying block j-1 to its rows) while 64-row chunks
+// apply block j-1 to the rows below. The backward sweep runs the blocks in
+// reverse: the rows below the pivots first (segments of kSolveSeg rows),
+// then per block the triangle and 64-column chunks of the columns before it.
+// A front whose last block has at most solve_fuse_rows rows below it lets
+// the triangle's workgroup apply that block itself. Steps of all fronts of
+// a level are merged (triangle items first: they are each step's critical
+// path).
 int solve_lists(smfgpu_plan_s *P) {
     const int32_t nlev = static_cast<int32_t>(P->level_fronts.size());
-    std::vector<int32_t> lfl;
-    P->level_front_ofs.assign(nlev + 1, 0);
-    P->level_big.assign(nlev, {});
-    int32_t maxb = 0;
+    const int32_t B = kSolveBlock, fuse = P->solve_fuse_rows;
+    std::vector<SolveItem> fwd_items, bwd_items;
+    P->fwd_steps.clear();
+    P->bwd_steps.clear();
+    P->fwd_lv.assign(nlev + 1, 0);
+    P->bwd_lv.assign(nlev + 1, 0);
+    std::vector<std::vector<SolveItem>> ftri, fchunk, btri, bchunk;
+    auto put = [](std::vector<std::vector<SolveItem>> &v, size_t t, const SolveItem &it) {
+        if (v.size() <= t) v.resize(t + 1);
+        v[t].push_back(it);
+    };
+    auto flush = [](std::vector<std::vector<SolveItem>> &tri, std::vector<std::vector<SolveItem>> &chunk,
+                    std::vector<SolveItem> &items, std::vector<std::pair<int64_t, int64_t>> &steps) {
+        const size_t nt = std::max(tri.size(), chunk.size());
+        tri.resize(nt);
+        chunk.resize(nt);
+        for (size_t t = 0; t < nt; ++t) {
+            const int64_t first = static_cast<int64_t>(items.size());
+            items.insert(items.end(), tri[t].begin(), tri[t].end());
+            items.insert(items.end(), chunk[t].begin(), chunk[t].end());
+            const int64_t cnt = static_cast<int64_t>(items.size()) - first;
+            if (cnt > 0) steps.emplace_back(first, cnt);
+        }
+        tri.clear();
+        chunk.clear();
+    };
     for (int32_t lv = 0; lv < nlev; ++lv) {
-        P->level_front_ofs[lv] = static_cast<int64_t>(lfl.size());
+        P->fwd_lv[lv] = static_cast<int32_t>(P->fwd_steps.size());
+        P->bwd_lv[lv] = static_cast<int32_t>(P->bwd_steps.size());
         for (int32_t s : P->level_fronts[lv]) {
             const FrontDev &F = P->fronts[s];
-            if (static_cast<int64_t>(F.f) * F.p >= P->solve_big_min) {
-                P->level_big[lv].push_back(s);
-                maxb = std::max(maxb, (F.p + 63) / 64);
+            const int32_t f = F.f, p = F.p, q = f - p;
+            if (p <= 0) continue;
+            const int32_t nb = (p + B - 1) / B;
+            // forward
+            size_t t = 0;
+            bool tail_done = false;
+            for (int32_t j = 0; j < nb; ++j, ++t) {
+                const int32_t tk = j * B, tkb = std::min(B, p - tk);
+                const int32_t k = j > 0 ? tk - B : 0, kb = j > 0 ? B : 0;
+                const bool tail = j == nb - 1 && q > 0 && q <= fuse;
+                tail_done = tail_done || tail;
+                put(ftri, t, SolveItem{s, k, kb, tk, tk + tkb, tk, tkb, tail ? 2 : 0});
+                if (j > 0)
+                    for (int32_t r = tk + tkb; r < f; r += 64)
+                        put(fchunk, t, SolveItem{s, k, kb, r, std::min(r + 64, f), 0, 0, 0});
+            }
+            if (q > 0 && !tail_done) {
+                const int32_t k = (nb - 1) * B;
+                for (int32_t r = p; r < f; r += 64)
+                    put(fchunk, t, SolveItem{s, k, p - k, r, std::min(r + 64, f), 0, 0, 0});
+            }
+            // backward
+            const int32_t kL = (nb - 1) * B, tkbL = p - kL;
+            t = 0;
+            if (q > 0 && q <= kSolveSeg) {
+                for (int32_t c = 0; c < kL; c += 64)
+                    put(bchunk, 0, SolveItem{s, p, q, c, std::min(c + 64, kL), 0, 0, 0});
+                put(btri, 0, SolveItem{s, p, q, kL, p, kL, tkbL, 0});
+                t = 1;
+            } else if (q > 0) {
+                for (int32_t r = p; r < f; r += kSolveSeg)
+                    for (int32_t c = 0; c < p; c += 64)
+                        put(bchunk, 0, SolveItem{s, r, std::min(kSolveSeg, f - r), c, std::min(c + 64, p), 0, 0, 1});
+                put(btri, 1, SolveItem{s, 0, 0, 0, 0, kL, tkbL, 0});
+                t = 2;
             } else {
-                lfl.push_back(s);
+                put(btri, 0, SolveItem{s, 0, 0, 0, 0, kL, tkbL, 0});
+                t = 1;
+            }
+            for (int32_t j = nb - 2; j >= 0; --j, ++t) {
+                const int32_t tk = j * B, k = tk + B, kb = std::min(B, p - k);
+                put(btri, t, SolveItem{s, k, kb, tk, tk + B, tk, B, 0});
+                for (int32_t c = 0; c < tk; c += 64)
+                    put(bchunk, t, SolveItem{s, k, kb, c, std::min(c + 64, tk), 0, 0, 0});
             }
         }
+        flush(ftri, fchunk, fwd_items, P->fwd_steps);
+        flush(btri, bchunk, bwd_items, P->bwd_steps);
     }
-    P->level_front_ofs[nlev] = static_cast<int64_t>(lfl.size());
+    P->fwd_lv[nlev] = static_cast<int32_t>(P->fwd_steps.size());
+    P->bwd_lv[nlev] = static_cast<int32_t>(P->bwd_steps.size());
+    for (const auto &st : P->fwd_steps)
+        if (st.second > INT32_MAX) return set_error(SMFGPU_ERR_ALLOC, "plan: solve step too large");
+    for (const auto &st : P->bwd_steps)
+        if (st.second > INT32_MAX) return set_error(SMFGPU_ERR_ALLOC, "plan: solve step too large");
     int rc;
-    if ((rc = P->d_level_fronts.upload(lfl))) return rc;
-    if (maxb > P->max_blocks || P->d_sflags.p == nullptr) {
-        P->max_blocks = std::max(maxb, 1);
-        if ((rc = P->d_sflags.alloc(2 * P->max_blocks)) ||
-            (rc = P->d_zbuf.alloc(static_cast<size_t>(P->max_blocks) * 64 * 16)) ||
-            (rc = P->d_sbuf.alloc(static_cast<size_t>(P->max_blocks) * 64 * 16)))
-            return rc;
-        if (hipMemset(P->d_sflags.p, 0, P->d_sflags.bytes()) != hipSuccess)
-            return set_error(SMFGPU_ERR_DEVICE, "plan: flag initialisation failed");
-        P->epoch = 0;
-    }
+    if ((rc = P->d_fwd_items.upload(fwd_items)) || (rc = P->d_bwd_items.upload(bwd_items))) return rc;
     return SMFGPU_OK;
 }
 
@@ -319,10 +386,8 @@ int plan_build(smfgpu_plan_s *P, const smfgpu_analysis_desc *D) {
     if ((rc = P->d_av.alloc(std::max<int64_t>(nslots, 1))) ||
         (rc = P->d_L.alloc(spd ? std::max<int64_t>(P->factor_entries, 1) : 1)) ||
         (rc = P->d_CB.alloc(spd ? std::max<int64_t>(P->cb_entries, 1) : 1)) || (rc = P->d_red.alloc(8)) ||
-        (rc = P->d_flag.alloc(4)) || (rc = P->d_prec.alloc(std::max(ns, 1))) || (rc = P->d_serr.alloc(4)))
+        (rc = P->d_flag.alloc(4)) || (rc = P->d_prec.alloc(std::max(ns, 1))))
         return rc;
-    if (hipMemset(P->d_serr.p, 0, P->d_serr.bytes()) != hipSuccess)
-        return set_error(SMFGPU_ERR_DEVICE, "plan: flag initialisation failed");
     if (!spd) return ldlt_plan_build(P, D);
     if ((rc = solve_lists(P))) return rc;
     // algorithmic bytes (assembly, one solve sweep pair per RHS)
@@ -429,12 +494,6 @@ SMFGPU_API int smfgpu_plan_create(const smfgpu_analysis_desc *desc, int32_t posd
         auto P = std::make_unique<smfgpu_plan_s>();
         P->posdef = posdef;
         HIP_OK(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking));
-        {
-            int dev = 0, ncu = 0;
-            HIP_OK(hipGetDevice(&dev));
-            HIP_OK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-            P->num_cu = std::max(1, ncu);
-        }
         const int rc = plan_build(P.get(), desc);
         if (rc != SMFGPU_OK) {
             (void)hipStreamDestroy(P->stream);
@@ -483,10 +542,10 @@ SMFGPU_API int smfgpu_plan_stats(smfgpu_plan_t P, double *out) {
 SMFGPU_API int smfgpu_plan_tune(smfgpu_plan_t P, int32_t key, int64_t value) {
     if (!P) return set_error(SMFGPU_ERR_INVALID_ARG, "null handle");
     switch (key) {
-    case SMFGPU_TUNE_SOLVE_BIG_MIN:
-        if (value < 1) return set_error(SMFGPU_ERR_INVALID_ARG, "tune: bad value");
-        P->solve_big_min = value;
-        return solve_lists(P);
+    case SMFGPU_TUNE_SOLVE_FUSE_ROWS:
+        if (value < 0 || value > (int64_t{1} << 30)) return set_error(SMFGPU_ERR_INVALID_ARG, "tune: bad value");
+        P->solve_fuse_rows = static_cast<int32_t>(value);
+        return P->posdef ? solve_lists(P) : SMFGPU_OK;
     default:
         return set_error(SMFGPU_ERR_INVALID_ARG, "tune: unknown key");
     }
@@ -709,61 +768,41 @@ static int solve_device(smfgpu_plan_s *P, const double *b_dev, int64_t ldb, doub
     const int32_t n = P->n;
     DevPtrs d = dev_ptrs(P);
     const int32_t nl = static_cast<int32_t>(P->level_fronts.size());
-    for (int32_t r0 = 0; r0 < nrhs; r0 += 16) {
-        const int32_t nr = std::min(16, nrhs - r0);
+    for (int32_t r0 = 0; r0 < nrhs; r0 += 4) {
+        const int32_t nr = std::min(4, nrhs - r0);
         if (P->d_y.n < static_cast<size_t>(n) * nr) {
-            const int rc = P->d_y.alloc(static_cast<size_t>(n) * 16);
+            const int rc = P->d_y.alloc(static_cast<size_t>(n) * 4);
             if (rc) return rc;
         }
         launch_gather_rhs(st, P->d_perm.p, b_dev + r0 * ldb, ldb, P->d_y.p, n, nr);
         const bool tm = P->timing;
-        if (tm && P->sev.size() < static_cast<size_t>(8 * nl)) {
-            for (size_t i = P->sev.size(); i < static_cast<size_t>(8 * nl); ++i) {
+        if (tm && P->sev.size() < 3) {
+            for (size_t i = P->sev.size(); i < 3; ++i) {
                 hipEvent_t e;
                 HIP_OK(hipEventCreate(&e));
                 P->sev.push_back(e);
             }
         }
-        auto mark = [&](int lv, int k) -> int {
-            if (tm) HIP_OK(hipEventRecord(P->sev[8 * lv + k], st));
-            return SMFGPU_OK;
-        };
-        for (int32_t lv = 0; lv < nl; ++lv) {
-            if (mark(lv, 0)) return SMFGPU_ERR_DEVICE;
-            launch_fwd(st, d, P->d_level_fronts.p + P->level_front_ofs[lv],
-                       static_cast<int32_t>(P->level_front_ofs[lv + 1] - P->level_front_ofs[lv]), P->d_y.p, n, nr);
-            if (mark(lv, 1)) return SMFGPU_ERR_DEVICE;
-            for (int32_t s : P->level_big[lv]) {
-                const FrontDev &F = P->fronts[s];
-                launch_fwd_big(st, d, s, F.f, F.p, P->d_y.p, n, nr);
-            }
-            if (mark(lv, 2)) return SMFGPU_ERR_DEVICE;
-        }
-        for (int32_t lv = nl - 1; lv >= 0; --lv) {
-            if (mark(lv, 3)) return SMFGPU_ERR_DEVICE;
-            for (int32_t s : P->level_big[lv]) {
-                const FrontDev &F = P->fronts[s];
-                launch_bwd_big(st, d, s, F.f, F.p, P->d_y.p, n, nr, P->d_sbuf.p);
-            }
-            if (mark(lv, 4)) return SMFGPU_ERR_DEVICE;
-            launch_bwd(st, d, P->d_level_fronts.p + P->level_front_ofs[lv],
-                       static_cast<int32_t>(P->level_front_ofs[lv + 1] - P->level_front_ofs[lv]), P->d_y.p, n, nr);
-            if (mark(lv, 5)) return SMFGPU_ERR_DEVICE;
-        }
+        if (tm) HIP_OK(hipEventRecord(P->sev[0], st));
+        for (int32_t lv = 0; lv < nl; ++lv)
+            for (int32_t i = P->fwd_lv[lv]; i < P->fwd_lv[lv + 1]; ++i)
+                launch_fwd_step(st, d, P->d_fwd_items.p + P->fwd_steps[i].first, P->fwd_steps[i].second, P->d_y.p, n,
+                                nr);
+        if (tm) HIP_OK(hipEventRecord(P->sev[1], st));
+        for (int32_t lv = nl - 1; lv >= 0; --lv)
+            for (int32_t i = P->bwd_lv[lv]; i < P->bwd_lv[lv + 1]; ++i)
+                launch_bwd_step(st, d, P->d_bwd_items.p + P->bwd_steps[i].first, P->bwd_steps[i].second, P->d_y.p, n,
+                                nr);
         if (tm) {
+            HIP_OK(hipEventRecord(P->sev[2], st));
             HIP_OK(hipStreamSynchronize(st));
-            for (int k = 0; k < 4; ++k) P->t_solve_phase[k] = 0;
-            for (int32_t lv = 0; lv < nl; ++lv) {
-                float ms;
-                HIP_OK(hipEventElapsedTime(&ms, P->sev[8 * lv + 0], P->sev[8 * lv + 1]));
-                P->t_solve_phase[0] += ms;
-                HIP_OK(hipEventElapsedTime(&ms, P->sev[8 * lv + 1], P->sev[8 * lv + 2]));
-                P->t_solve_phase[1] += ms;
-                HIP_OK(hipEventElapsedTime(&ms, P->sev[8 * lv + 3], P->sev[8 * lv + 4]));
-                P->t_solve_phase[2] += ms;
-                HIP_OK(hipEventElapsedTime(&ms, P->sev[8 * lv + 4], P->sev[8 * lv + 5]));
-                P->t_solve_phase[3] += ms;
-            }
+            float ms;
+            HIP_OK(hipEventElapsedTime(&ms, P->sev[0], P->sev[1]));
+            P->t_solve_phase[0] = ms;
+            HIP_OK(hipEventElapsedTime(&ms, P->sev[1], P->sev[2]));
+            P->t_solve_phase[1] = ms;
+            P->t_solve_phase[2] = static_cast<double>(P->fwd_steps.size());
+            P->t_solve_phase[3] = static_cast<double>(P->bwd_steps.size());
         }
         launch_scatter_rhs(st, P->d_perm.p, P->d_y.p, out_dev + r0 * ldo, ldo, n, nr);
     }

@@ -68,6 +68,16 @@ struct SyrkItem {
     int32_t t0, t1, prec, pad;  // panel columns [t0, t1); prec: 0 fp64, 1 fp32, 2 bf16
 };
 
+// One workgroup of a solve step on front s.
+// Forward: update rows [a0, a1) by pivot block [k, k + kb) (kb = 0: none),
+//   then solve the triangle of block [tk, tk + tkb) (tkb = 0: none; its rows
+//   are then [a0, a1)); flags & 2: also update the rows below the block.
+// Backward: update columns [a0, a1) by rows [k, k + kb) (own pivot rows or
+//   rows below the pivots), flags & 1: by atomic adds; then the triangle.
+struct SolveItem {
+    int32_t s, k, kb, a0, a1, tk, tkb, flags;
+};
+
 enum LaunchKind : int32_t {
     K_ASSEMBLE = 0,
     K_POTRF = 1,
@@ -112,17 +122,13 @@ void launch_gather_rhs(hipStream_t st, const int32_t *perm, const double *b, int
                        int32_t n, int32_t nrhs);
 void launch_scatter_rhs(hipStream_t st, const int32_t *perm, const double *y, double *b, int64_t ldb,
                         int32_t n, int32_t nrhs);
-void launch_fwd(hipStream_t st, const DevPtrs &d, const int32_t *fronts, int32_t nf, double *y,
-                int32_t n, int32_t nrhs);
-void launch_bwd(hipStream_t st, const DevPtrs &d, const int32_t *fronts, int32_t nf, double *y,
-                int32_t n, int32_t nrhs);
-
-// Large fronts: sweeps in blocks of 256 pivots, two launches per block
-// (the block's triangle, then the rest of the front). sbuf: p x 16 doubles.
-void launch_fwd_big(hipStream_t st, const DevPtrs &d, int32_t s, int32_t f, int32_t p, double *y, int32_t n,
-                    int32_t nrhs);
-void launch_bwd_big(hipStream_t st, const DevPtrs &d, int32_t s, int32_t f, int32_t p, double *y, int32_t n,
-                    int32_t nrhs, double *sbuf);
+// One step of a sweep (nrhs <= 4): a workgroup per item.
+constexpr int32_t kSolveBlock = 128;  // pivot columns per step
+constexpr int32_t kSolveSeg = 512;    // rows per backward update item
+void launch_fwd_step(hipStream_t st, const DevPtrs &d, const SolveItem *items, int64_t count, double *y,
+                     int32_t n, int32_t nrhs);
+void launch_bwd_step(hipStream_t st, const DevPtrs &d, const SolveItem *items, int64_t count, double *y,
+                     int32_t n, int32_t nrhs);
 
 // Refinement: full symmetric CSR in permuted numbering (slot into av).
 void launch_residual(hipStream_t st, int32_t n, const int64_t *rp, const int32_t *ci,

@@ -87,15 +87,14 @@ struct smfgpu_plan_s {
     DBuf<TrsmItem> d_trsm;
     DBuf<SyrkItem> d_syrk;
     DBuf<int8_t> d_prec;
-    std::vector<int64_t> level_front_ofs;          // offsets into d_level_fronts (small fronts per level)
-    std::vector<std::vector<int32_t>> level_big;   // per level: large fronts (pipelined cooperative solve)
-    DBuf<int32_t> d_sflags, d_serr;                // solve pipeline flags (2 x max blocks) and error word
-    DBuf<double> d_zbuf, d_sbuf;                   // published block values, backward accumulators
-    int32_t epoch = 0;
-    int32_t max_blocks = 0;
-    int32_t num_cu = 256;                          // cooperative grid bound
-    int64_t solve_big_min = int64_t{1} << 21;      // f p from which a front's solve is pipelined
-    double t_solve_phase[4] = {0, 0, 0, 0};        // fwd small, fwd big, bwd big, bwd small (ms)
+    // solve schedule: items of every step of every level; step i of the
+    // forward sweep is fwd_steps[i] = {first item, count}, level lv owns
+    // steps [fwd_lv[lv], fwd_lv[lv + 1]) (backward likewise)
+    DBuf<SolveItem> d_fwd_items, d_bwd_items;
+    std::vector<std::pair<int64_t, int64_t>> fwd_steps, bwd_steps;
+    std::vector<int32_t> fwd_lv, bwd_lv;
+    int32_t solve_fuse_rows = 512;                 // rows below a last block its own workgroup updates
+    double t_solve_phase[4] = {0, 0, 0, 0};        // fwd ms, bwd ms, fwd launches, bwd launches
     // refinement CSR (full symmetric, permuted numbering)
     DBuf<int64_t> d_fr_ptr, d_fr_slot;
     DBuf<int32_t> d_fr_col;

@@ -116,7 +116,7 @@ class Plan:
         st = np.zeros(16)
         check(lib().smfgpu_kernel_times(self.h, _p(st)), "kernel_times")
         keys = ("syrk_ms", "syrk_launches", "syrk_flops", "asm_ms", "asm_bytes", "solve_ms", "solve_bytes",
-                "fwd_small_ms", "fwd_big_ms", "bwd_big_ms", "bwd_small_ms")
+                "fwd_ms", "bwd_ms", "fwd_launches", "bwd_launches")
         return dict(zip(keys, st[:11].tolist()))
 
     def tune(self, key, value):

@@ -15,6 +15,7 @@ if not gpu_available():
 
 from opt_solver_amd import generators as g  # noqa: E402
 from opt_solver_amd import ordering as o  # noqa: E402
+from opt_solver_amd._lib import TUNE_SOLVE_FUSE_ROWS  # noqa: E402
 from opt_solver_amd.solver import PREC_FP32, Plan, SmfGpuError, analyse  # noqa: E402
 
 
@@ -92,12 +93,13 @@ def test_spd27_own_analysis_multi_rhs(oracle_lib):
         assert np.abs(fac.solve(B[:, r]) - XS[:, r]).max() < 1e-13
 
 
-def test_large_front_pipelined_solve(oracle_lib):
-    """spd27 40^3 has fronts past the pipelined-solve size (f p >= 2^21):
-    1 and 16 right-hand sides against the known solution and the
-    reference backward-error formula."""
+def test_large_front_stepped_solve(oracle_lib):
+    """spd27 40^3 has fronts of several pivot blocks (2000+ rows, the rows
+    below the pivots past the fused-update limit): 1, 3, 4, 5 and 16
+    right-hand sides, and the fused-update limit at 0 / default / all,
+    against the known solution and the reference backward-error formula."""
     m = 40
-    print("case spd27_40 pipelined solve", flush=True)
+    print("case spd27_40 stepped solve", flush=True)
     A = g.spd27_3d(m, seed=4)
     an = analyse(A, perm=o.grid_nd((m, m, m), leaf=64), nemin=32)
     assert an.max_front >= 2000
@@ -105,12 +107,15 @@ def test_large_front_pipelined_solve(oracle_lib):
     fac = plan.factor(A[3])
     X = np.random.default_rng(4).uniform(-1, 1, (A[0], 16))
     B = matvec(A, X)
-    for nr in (1, 16):
-        XS = fac.solve(B[:, :nr] if nr > 1 else B[:, 0])
-        XS = XS.reshape(A[0], nr)
-        eta, om = oracle_lib.backward_error(A, B[:, :nr], XS)
-        assert eta <= 1e-14, (nr, eta)
-        assert np.abs(XS - X[:, :nr]).max() < 1e-10
+    for fuse in (512, 0, 1 << 20):
+        plan.tune(TUNE_SOLVE_FUSE_ROWS, fuse)
+        for nr in (1, 3, 4, 5, 16):
+            print(f"  fuse {fuse} nrhs {nr}", flush=True)
+            XS = fac.solve(B[:, :nr] if nr > 1 else B[:, 0])
+            XS = XS.reshape(A[0], nr)
+            eta, om = oracle_lib.backward_error(A, B[:, :nr], XS)
+            assert eta <= 1e-14, (fuse, nr, eta)
+            assert np.abs(XS - X[:, :nr]).max() < 1e-10
 
 
 def test_mixed_precision_refinement(oracle_lib):
