@@ -144,50 +144,75 @@ void launch_assemble(hipStream_t st, const DevPtrs &d, const AsmTask *tasks, int
 constexpr int kNb = 64;
 
 __global__ __launch_bounds__(kThreads) void k_potrf(DevPtrs d, const PotrfItem *__restrict__ items) {
-    __shared__ double A[kNb][kNb + 1];
-    __shared__ double Inv[kNb][kNb + 1];
+    __shared__ double colb[2][kNb];
+    __shared__ double rowb[2][kNb];
+    __shared__ double T[kNb][kNb + 1];
     const PotrfItem it = items[blockIdx.x];
     const FrontDev F = d.fronts[it.s];
     const int f = F.f, k = it.k, kb = it.kb;
     double *base = d.L + F.lofs;
     const int tid = threadIdx.x;
-    for (int idx = tid; idx < kb * kNb; idx += kThreads) {
-        const int c = idx / kNb, r = idx % kNb;
-        A[r][c] = (r < kb && r >= c) ? base[static_cast<int64_t>(k + c) * f + k + r] : 0.0;
-        Inv[r][c] = (r == c) ? 1.0 : 0.0;
+    const int r = tid & 63, cg = tid >> 6;  // row r, columns 16 cg + i
+    // a: the block's lower triangle (padded with the identity past kb);
+    // w: the identity, turned into inv(L_kk) by the same row operations.
+    double a[16], w[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int c = 16 * cg + i;
+        const bool in = r < kb && c < kb && r >= c;
+        a[i] = in ? base[static_cast<int64_t>(k + c) * f + k + r] : (r == c ? 1.0 : 0.0);
+        w[i] = (r == c) ? 1.0 : 0.0;
     }
-    __syncthreads();
-    // Column j: scale it by 1/sqrt(a_jj) and scale row j of Inv (the
-    // row operations of the elimination applied to the identity give
-    // inv(L_kk)); then the rank-1 update of the trailing triangle and of
-    // the rows of Inv below j.
-    // A[j][j] keeps d_jj until the end (no thread writes it while others
-    // read it); its square root is stored on the way out.
-    for (int j = 0; j < kb; ++j) {
-        __syncthreads();
-        const double djj = A[j][j];
-        const double rs = 1.0 / sqrt(djj);
-        if (tid == 0 && !(djj > 0.0 && djj <= DBL_MAX)) atomicMin(d.flag, it.s + 1);
-        for (int r = j + 1 + tid; r < kb; r += kThreads) A[r][j] *= rs;
-        for (int c = tid; c <= j; c += kThreads) Inv[j][c] *= rs;
-        __syncthreads();
-        const int m = kb - j - 1;
-        for (int idx = tid; idx < m * m; idx += kThreads) {
-            const int c = j + 1 + idx / m, r = j + 1 + idx % m;
-            if (r >= c) A[r][c] -= A[r][j] * A[c][j];
-        }
-        for (int idx = tid; idx < m * (j + 1); idx += kThreads) {
-            const int r = j + 1 + idx / (j + 1), c = idx % (j + 1);
-            Inv[r][c] -= A[r][j] * Inv[j][c];
+    bool bad = false;
+    // Step j: column j (unscaled) and row j of W through LDS (alternating
+    // buffers: one barrier per step), then every row r > j applies
+    // A[r][c] -= L[r][j] L[c][j] (j < c <= r) and W[r][c] -= L[r][j] X[j][c]
+    // (c <= j) with X[j] = W[j] / L_jj.
+    for (int jb = 0; jb < 4; ++jb) {
+#pragma unroll
+        for (int jj = 0; jj < 16; ++jj) {
+            const int j = 16 * jb + jj;
+            const int bs = jj & 1;
+            if (cg == jb) colb[bs][r] = a[jj];
+            if (r == j) {
+#pragma unroll
+                for (int i = 0; i < 16; ++i) rowb[bs][16 * cg + i] = w[i];
+            }
+            __syncthreads();
+            const double djj = colb[bs][j];
+            const double rs = 1.0 / sqrt(djj);
+            const double lrj = colb[bs][r] * rs;
+            if (r > j) {
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
+                    const int c = 16 * cg + i;
+                    if (c > j) {
+                        if (c <= r) a[i] -= lrj * (colb[bs][c] * rs);
+                    } else {
+                        w[i] -= lrj * (rowb[bs][c] * rs);
+                    }
+                }
+            } else if (r == j) {
+#pragma unroll
+                for (int i = 0; i < 16; ++i) w[i] *= rs;
+            }
+            if (cg == jb) {
+                if (r > j) a[jj] = lrj;
+                else if (r == j) a[jj] = sqrt(djj);
+            }
+            bad = bad || (j < kb && !(djj > 0.0 && djj <= DBL_MAX));
         }
     }
-    __syncthreads();
+    if (tid == 0 && bad) atomicMin(d.flag, it.s + 1);
     // lower: L_kk (diagonal sqrt(d_jj)); strict upper (r < c):
-    // M[r][c] = inv(L_kk)[c][r]
-    for (int idx = tid; idx < kb * kb; idx += kThreads) {
-        const int c = idx / kb, r = idx % kb;
-        const double v = (r > c) ? A[r][c] : (r == c ? sqrt(A[r][r]) : Inv[c][r]);
-        base[static_cast<int64_t>(k + c) * f + k + r] = v;
+    // M[r][c] = inv(L_kk)[c][r], through a transpose in LDS
+#pragma unroll
+    for (int i = 0; i < 16; ++i) T[r][16 * cg + i] = w[i];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int c = 16 * cg + i;
+        if (r < kb && c < kb) base[static_cast<int64_t>(k + c) * f + k + r] = (r >= c) ? a[i] : T[c][r];
     }
 }
 
@@ -197,36 +222,58 @@ void launch_potrf(hipStream_t st, const DevPtrs &d, const PotrfItem *it, int64_t
 }
 
 // Panel TRSM: rows [r0, r0 + 64) of columns [k, k + kb): X = A M, with
-// M = inv(L_kk)^T read from the block's strict upper part and diagonal.
+// M = inv(L_kk)^T read from the block's strict upper part and diagonal, on
+// FP64 MFMA. Computed transposed (X^T = M^T A^T: MFMA A operand = M^T,
+// B operand = A^T) so that 16 lanes store 16 consecutive rows of a column.
+// Wave w: rows 16 w .. 16 w + 15, four 16-column blocks; block cb needs
+// k-steps t < 16 cb + 16 only (M is upper triangular).
 __global__ __launch_bounds__(kThreads) void k_trsm(DevPtrs d, const TrsmItem *__restrict__ items) {
-    __shared__ double At[kNb][kNb + 1];
-    __shared__ double M[kNb][kNb + 1];
+    __shared__ double As[kNb][kNb + 2];
+    __shared__ double Ms[kNb][kNb + 2];
     const TrsmItem it = items[blockIdx.x];
     const FrontDev F = d.fronts[it.s];
     const int f = F.f, k = it.k, kb = it.kb, r0 = it.r0;
     double *base = d.L + F.lofs;
     const int tid = threadIdx.x;
     const int nr = min(kNb, f - r0);
-    for (int idx = tid; idx < kb * kNb; idx += kThreads) {
-        const int t = idx / kNb, r = idx % kNb;
-        At[r][t] = (r < nr) ? base[static_cast<int64_t>(k + t) * f + r0 + r] : 0.0;
-    }
-    for (int idx = tid; idx < kb * kb; idx += kThreads) {
-        const int c = idx / kb, t = idx % kb;
-        double v = 0.0;
-        if (t < c) v = base[static_cast<int64_t>(k + c) * f + k + t];
-        else if (t == c) v = 1.0 / base[static_cast<int64_t>(k + c) * f + k + c];
-        M[t][c] = v;
+    {
+        const int rr = tid & 63, cg = tid >> 6;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int t = cg + 4 * i;
+            As[rr][t] = (rr < nr && t < kb) ? base[static_cast<int64_t>(k + t) * f + r0 + rr] : 0.0;
+            // Ms[t = rr][c]
+            double mv = 0.0;
+            if (rr < kb && t < kb) {
+                const double g = base[static_cast<int64_t>(k + t) * f + k + rr];
+                mv = rr < t ? g : (rr == t ? 1.0 / g : 0.0);
+            }
+            Ms[rr][t] = mv;
+        }
     }
     __syncthreads();
-    const int r = tid % kNb;
-    const int cg = tid / kNb;
-    if (r < nr) {
-        for (int c = cg; c < kb; c += kThreads / kNb) {
-            double s = 0.0;
-            for (int t = 0; t <= c; ++t) s += At[r][t] * M[t][c];
-            base[static_cast<int64_t>(k + c) * f + r0 + r] = s;
-        }
+    const int lane = tid & 63, w = tid >> 6;
+    const int lr = lane & 15, lk = lane >> 4;
+    v4d acc[4];
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) acc[cb] = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk) {
+        const int t = 4 * kk + lk;
+        const double bop = As[16 * w + lr][t];
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb)
+            if (kk < 4 * cb + 4) acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(Ms[t][16 * cb + lr], bop, acc[cb], 0, 0, 0);
+    }
+    const int rr = 16 * w + lr;
+    if (rr < nr) {
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int c = 16 * cb + 4 * v + lk;
+                if (c < kb) base[static_cast<int64_t>(k + c) * f + r0 + rr] = acc[cb][v];
+            }
     }
 }
 
