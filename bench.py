@@ -50,7 +50,6 @@ def cpu_baseline(m_cpu, threads):
     an = R.RefAnalysis(A, ordering=R.ORDER_METIS, posdef=True)
     x = np.random.default_rng(0).uniform(-1, 1, A[0])
     b = _matvec(A, x)
-    R.RefFactor(an, A[3], threads=threads)  # warm-up
     t = time.perf_counter()
     rf = R.RefFactor(an, A[3], threads=threads)
     xs = rf.solve(b)
@@ -78,8 +77,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--m", type=int, default=100, help="grid side (n = m^3)")
     ap.add_argument("--nrhs", type=int, default=1)
-    ap.add_argument("--cpu-m", type=int, default=40)
+    ap.add_argument("--cpu-m", type=int, default=76)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="run one plain 1 GiB device copy first (scale for the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU is a design in DESIGN.md",
@@ -93,6 +94,12 @@ def main():
         print("no GPU", file=sys.stderr)
         sys.exit(1)
     dev = torch.device("cuda:0")
+    if args.copy_calib:
+        src = torch.ones(1 << 27, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     m = args.m
     t0 = time.perf_counter()
     A, perm = build_problem(m)

@@ -176,6 +176,14 @@ SMFGPU_API int smfgpu_export_factor(smfgpu_factor_t f, int64_t *sizes, int64_t *
                                     double *factor_values, int32_t *row_indices,
                                     int8_t *pivot_types);
 
+/* Per-supernode results of the last factorization, 8 int32 per supernode:
+ * front rows f (with delays), accepted pivots p, positive / negative / zero
+ * pivots, 2x2 pivots, columns delayed to the parent, eligible columns
+ * (process_front's per-front Info terms, multifrontal_serial.cpp:864-1031).
+ * Lets a caller that factors many independent systems as one forest report
+ * each system's inertia and delays. */
+SMFGPU_API int smfgpu_factor_front_stats(smfgpu_factor_t f, int32_t *out);
+
 /* ---- solves: b is n x nrhs, leading dimension ldb >= n, in place,
  * original (unpermuted) ordering. job: 0 full. mem: SMFGPU_MEM_*. */
 SMFGPU_API int smfgpu_solve(smfgpu_factor_t f, double *b, int32_t nrhs, int32_t ldb,
@@ -195,6 +203,9 @@ SMFGPU_API int smfgpu_refine(smfgpu_factor_t f, const double *values, const doub
  * most this many rows below it has them updated by the workgroup that solves
  * the block, saving a step of the sweep (default 512). */
 #define SMFGPU_TUNE_SOLVE_FUSE_ROWS 1
+/* SMFGPU_TUNE_SYRK_KC: panel columns per LDS stage of the SPD Schur-update
+ * kernel, 16 or 32 (default 16). */
+#define SMFGPU_TUNE_SYRK_KC 2
 SMFGPU_API int smfgpu_plan_tune(smfgpu_plan_t plan, int32_t key, int64_t value);
 
 /* Device time (ms) of the last factorization's dominant kernel family:

@@ -146,15 +146,18 @@ constexpr int kNb = 64;
 __global__ __launch_bounds__(kThreads) void k_potrf(DevPtrs d, const PotrfItem *__restrict__ items) {
     __shared__ double colb[2][kNb];
     __shared__ double rowb[2][kNb];
+    __shared__ double rsv[kNb];
     __shared__ double T[kNb][kNb + 1];
     const PotrfItem it = items[blockIdx.x];
     const FrontDev F = d.fronts[it.s];
     const int f = F.f, k = it.k, kb = it.kb;
     double *base = d.L + F.lofs;
     const int tid = threadIdx.x;
-    const int r = tid & 63, cg = tid >> 6;  // row r, columns 16 cg + i
+    const int r = tid & 63;
+    const int cg = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave cg owns columns 16 cg .. 16 cg + 15
     // a: the block's lower triangle (padded with the identity past kb);
-    // w: the identity, turned into inv(L_kk) by the same row operations.
+    // w: the identity, turned into inv(L_kk) (rows unscaled) by the same
+    // row operations. Upper entries of a are never read.
     double a[16], w[16];
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
@@ -164,10 +167,11 @@ __global__ __launch_bounds__(kThreads) void k_potrf(DevPtrs d, const PotrfItem *
         w[i] = (r == c) ? 1.0 : 0.0;
     }
     bool bad = false;
-    // Step j: column j (unscaled) and row j of W through LDS (alternating
-    // buffers: one barrier per step), then every row r > j applies
-    // A[r][c] -= L[r][j] L[c][j] (j < c <= r) and W[r][c] -= L[r][j] X[j][c]
-    // (c <= j) with X[j] = W[j] / L_jj.
+    // Step j: column j of A and row j of W through LDS (alternating
+    // buffers: one barrier per step). With g = A[r][j] / d_jj for rows
+    // r > j (0 above): A[r][c] -= g A[c][j] (c > j), W[r][c] -= g W[j][c]
+    // (c <= j), i.e. L[r][j] L[c][j] and L[r][j] X[j][c] with
+    // X[j] = W[j] / L_jj. Rows of W are scaled by 1 / L_rr at the end.
     for (int jb = 0; jb < 4; ++jb) {
 #pragma unroll
         for (int jj = 0; jj < 16; ++jj) {
@@ -181,33 +185,32 @@ __global__ __launch_bounds__(kThreads) void k_potrf(DevPtrs d, const PotrfItem *
             __syncthreads();
             const double djj = colb[bs][j];
             const double rs = 1.0 / sqrt(djj);
-            const double lrj = colb[bs][r] * rs;
-            if (r > j) {
+            const double gr = r > j ? colb[bs][r] * (rs * rs) : 0.0;
+            if (cg > jb) {
+#pragma unroll
+                for (int i = 0; i < 16; ++i) a[i] -= gr * colb[bs][16 * cg + i];
+            } else if (cg < jb) {
+#pragma unroll
+                for (int i = 0; i < 16; ++i) w[i] -= gr * rowb[bs][16 * cg + i];
+            } else {
 #pragma unroll
                 for (int i = 0; i < 16; ++i) {
-                    const int c = 16 * cg + i;
-                    if (c > j) {
-                        if (c <= r) a[i] -= lrj * (colb[bs][c] * rs);
-                    } else {
-                        w[i] -= lrj * (rowb[bs][c] * rs);
-                    }
+                    if (i > jj) a[i] -= gr * colb[bs][16 * cg + i];
+                    else w[i] -= gr * rowb[bs][16 * cg + i];
                 }
-            } else if (r == j) {
-#pragma unroll
-                for (int i = 0; i < 16; ++i) w[i] *= rs;
+                a[jj] = r > j ? colb[bs][r] * rs : (r == j ? djj * rs : a[jj]);
             }
-            if (cg == jb) {
-                if (r > j) a[jj] = lrj;
-                else if (r == j) a[jj] = sqrt(djj);
-            }
+            if (tid == 0) rsv[j] = rs;
             bad = bad || (j < kb && !(djj > 0.0 && djj <= DBL_MAX));
         }
     }
     if (tid == 0 && bad) atomicMin(d.flag, it.s + 1);
+    __syncthreads();
     // lower: L_kk (diagonal sqrt(d_jj)); strict upper (r < c):
-    // M[r][c] = inv(L_kk)[c][r], through a transpose in LDS
+    // M[r][c] = inv(L_kk)[c][r] = W[c][r] / L_cc, through a transpose in LDS
+    const double rr = rsv[r];
 #pragma unroll
-    for (int i = 0; i < 16; ++i) T[r][16 * cg + i] = w[i];
+    for (int i = 0; i < 16; ++i) T[r][16 * cg + i] = w[i] * rr;
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
@@ -291,8 +294,6 @@ void launch_trsm(hipStream_t st, const DevPtrs &d, const TrsmItem *it, int64_t n
 // store. prec 0: FP64 MFMA; 1: operands rounded to FP32, FP32 MFMA with FP32
 // accumulation over the panel, subtracted from the FP64 Schur complement.
 constexpr int kT = 128;
-constexpr int kKc = 16;
-constexpr int kLoadPer = (kKc * kT) / kThreads;  // elements of A (and of B) per thread per chunk
 
 template <typename E>
 struct SyrkMfma;
@@ -321,10 +322,11 @@ struct SyrkMfma<float> {
 // Loads are branch-free (indices clamped into the front, out-of-range
 // values selected to zero); waves wholly above the diagonal run the same
 // loop without MFMAs (ACTIVE = false), so the accumulators stay in AGPRs.
-template <typename E, bool ACTIVE>
+template <typename E, bool ACTIVE, int kKc>
 __device__ __forceinline__ void syrk_loop(const SyrkItem &it, const double *Lb, int f, int j1, unsigned char *smem,
                                           typename SyrkMfma<E>::acc_t (&acc)[4][4]) {
     typedef SyrkMfma<E> M;
+    constexpr int kLoadPer = (kKc * kT) / kThreads;  // elements of A (and of B) per thread per chunk
     E (*As)[kKc][kT] = reinterpret_cast<E (*)[kKc][kT]>(smem);                            // [2][kKc][kT]
     E (*Bs)[kKc][kT] = reinterpret_cast<E (*)[kKc][kT]>(smem + 2 * kKc * kT * sizeof(E));  // [2][kKc][kT]
     const int tid = threadIdx.x;
@@ -386,7 +388,7 @@ __device__ __forceinline__ void syrk_loop(const SyrkItem &it, const double *Lb, 
     }
 }
 
-template <typename E>
+template <typename E, int kKc>
 __device__ __forceinline__ void syrk_tile(const DevPtrs &d, const SyrkItem &it, unsigned char *smem) {
     typedef SyrkMfma<E> M;
     const FrontDev F = d.fronts[it.s];
@@ -404,8 +406,8 @@ __device__ __forceinline__ void syrk_tile(const DevPtrs &d, const SyrkItem &it, 
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = M::zero();
-    if (active) syrk_loop<E, true>(it, Lb, f, j1, smem, acc);
-    else syrk_loop<E, false>(it, Lb, f, j1, smem, acc);
+    if (active) syrk_loop<E, true, kKc>(it, Lb, f, j1, smem, acc);
+    else syrk_loop<E, false, kKc>(it, Lb, f, j1, smem, acc);
     if (!active) return;
 #pragma unroll
     for (int m = 0; m < 4; ++m)
@@ -427,17 +429,21 @@ __device__ __forceinline__ void syrk_tile(const DevPtrs &d, const SyrkItem &it, 
 // store. Per-front precision: FP64 MFMA, or operands rounded to FP32 with
 // FP32 MFMA accumulation over the panel, subtracted from the FP64 Schur
 // complement.
+// kKc: panel columns per LDS stage (16: 64 KB of LDS, 32: 128 KB; two
+// stages, the global loads of the next one in flight during the MFMAs).
+template <int kKc>
 __global__ __launch_bounds__(kThreads) void k_syrk(DevPtrs d, const SyrkItem *__restrict__ items) {
     __shared__ __attribute__((aligned(16))) unsigned char smem[4 * kKc * kT * sizeof(double)];
     const SyrkItem it = items[blockIdx.x];
     const int prec = d.prec != nullptr ? d.prec[it.s] : 0;
-    if (prec == 0) syrk_tile<double>(d, it, smem);
-    else syrk_tile<float>(d, it, smem);
+    if (prec == 0) syrk_tile<double, kKc>(d, it, smem);
+    else syrk_tile<float, kKc>(d, it, smem);
 }
 
-void launch_syrk(hipStream_t st, const DevPtrs &d, const SyrkItem *it, int64_t n) {
+void launch_syrk(hipStream_t st, const DevPtrs &d, const SyrkItem *it, int64_t n, int kc) {
     if (n <= 0) return;
-    k_syrk<<<dim3(static_cast<unsigned>(n)), kThreads, 0, st>>>(d, it);
+    if (kc == 16) k_syrk<16><<<dim3(static_cast<unsigned>(n)), kThreads, 0, st>>>(d, it);
+    else k_syrk<32><<<dim3(static_cast<unsigned>(n)), kThreads, 0, st>>>(d, it);
 }
 
 void launch_zero_upper(hipStream_t, const DevPtrs &, const PotrfItem *, int64_t) {}

@@ -48,6 +48,13 @@ DevPtrs dev_ptrs(smfgpu_plan_s *P) {
     return d;
 }
 
+static void release_graph(smfgpu_plan_s *P) {
+    if (P->fexec) (void)hipGraphExecDestroy(P->fexec);
+    if (P->fgraph) (void)hipGraphDestroy(P->fgraph);
+    P->fexec = nullptr;
+    P->fgraph = nullptr;
+}
+
 // Solve schedule (static for SPD). Per front, pivot blocks of
 // kSolveBlock columns; step j of the forward sweep solves block j's triangle
 // (one workgroup, after applying block j-1 to its rows) while 64-row chunks
@@ -515,6 +522,8 @@ SMFGPU_API void smfgpu_plan_free(smfgpu_plan_t *plan) {
     }
     for (hipEvent_t e : P->ev) (void)hipEventDestroy(e);
     for (hipEvent_t e : P->sev) (void)hipEventDestroy(e);
+    if (P->fexec) (void)hipGraphExecDestroy(P->fexec);
+    if (P->fgraph) (void)hipGraphDestroy(P->fgraph);
     if (P->live) P->live->plan = nullptr;
     delete P;
     *plan = nullptr;
@@ -546,6 +555,11 @@ SMFGPU_API int smfgpu_plan_tune(smfgpu_plan_t P, int32_t key, int64_t value) {
         if (value < 0 || value > (int64_t{1} << 30)) return set_error(SMFGPU_ERR_INVALID_ARG, "tune: bad value");
         P->solve_fuse_rows = static_cast<int32_t>(value);
         return P->posdef ? solve_lists(P) : SMFGPU_OK;
+    case SMFGPU_TUNE_SYRK_KC:
+        if (value != 16 && value != 32) return set_error(SMFGPU_ERR_INVALID_ARG, "tune: K chunk must be 16 or 32");
+        P->syrk_kc = static_cast<int32_t>(value);
+        release_graph(P);
+        return SMFGPU_OK;
     default:
         return set_error(SMFGPU_ERR_INVALID_ARG, "tune: unknown key");
     }
@@ -583,12 +597,18 @@ SMFGPU_API int smfgpu_factor(smfgpu_plan_t P, const double *values, int64_t nval
     if (nvalues > 0 && !values) return set_error(SMFGPU_ERR_INVALID_ARG, "factor: null values");
     hipStream_t st = P->stream;
     const double *dv = values;
-    if (opt.value_mem == SMFGPU_MEM_HOST && nvalues > 0) {
+    // values go through d_vals when they are on the host, and for the SPD
+    // replayed sequence (its nodes read a fixed buffer)
+    const bool via_buf = opt.value_mem == SMFGPU_MEM_HOST || (P->posdef && !P->timing);
+    if (via_buf && nvalues > 0) {
         if (P->d_vals.n < static_cast<size_t>(nvalues)) {
             const int rc = P->d_vals.alloc(nvalues);
             if (rc) return rc;
+            release_graph(P);
         }
-        HIP_OK(hipMemcpyAsync(P->d_vals.p, values, nvalues * sizeof(double), hipMemcpyHostToDevice, st));
+        HIP_OK(hipMemcpyAsync(P->d_vals.p, values, nvalues * sizeof(double),
+                              opt.value_mem == SMFGPU_MEM_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice,
+                              st));
         dv = P->d_vals.p;
     }
     if (!P->posdef) {
@@ -637,23 +657,50 @@ SMFGPU_API int smfgpu_factor(smfgpu_plan_t P, const double *values, int64_t nval
         }
     }
     const double t0 = now_s();
-    if (timing) HIP_OK(hipEventRecord(P->ev[2 * nl], st));
-    launch_map_values(st, dv, nvalues, P->d_slot_ptr.p, P->d_slot_src.p, P->nnz_perm, P->d_av.p, P->d_red.p);
-    for (size_t li = 0; li < nl; ++li) {
-        const Launch &l = P->launches[li];
-        const bool rec = timing && (l.kind == K_SYRK || l.kind == K_ASSEMBLE);
-        if (rec) HIP_OK(hipEventRecord(P->ev[2 * li], st));
-        switch (l.kind) {
-        case K_ASSEMBLE: launch_assemble(st, d, P->d_asm.p + l.first, l.count, P->d_cc.p); break;
-        case K_POTRF: launch_potrf(st, d, P->d_potrf.p + l.first, l.count); break;
-        case K_TRSM: launch_trsm(st, d, P->d_trsm.p + l.first, l.count); break;
-        case K_SYRK: launch_syrk(st, d, P->d_syrk.p + l.first, l.count); break;
-        default: break;
+    auto enqueue = [&](bool rec_all) -> hipError_t {
+        hipError_t e = hipSuccess;
+        auto mark = [&](size_t i) {
+            if (e == hipSuccess) e = hipEventRecord(P->ev[i], st);
+        };
+        if (rec_all) mark(2 * nl);
+        launch_map_values(st, dv, nvalues, P->d_slot_ptr.p, P->d_slot_src.p, P->nnz_perm, P->d_av.p, P->d_red.p);
+        for (size_t li = 0; li < nl; ++li) {
+            const Launch &l = P->launches[li];
+            const bool rec = rec_all && (l.kind == K_SYRK || l.kind == K_ASSEMBLE);
+            if (rec) mark(2 * li);
+            switch (l.kind) {
+            case K_ASSEMBLE: launch_assemble(st, d, P->d_asm.p + l.first, l.count, P->d_cc.p); break;
+            case K_POTRF: launch_potrf(st, d, P->d_potrf.p + l.first, l.count); break;
+            case K_TRSM: launch_trsm(st, d, P->d_trsm.p + l.first, l.count); break;
+            case K_SYRK: launch_syrk(st, d, P->d_syrk.p + l.first, l.count, P->syrk_kc); break;
+            default: break;
+            }
+            if (rec) mark(2 * li + 1);
         }
-        if (rec) HIP_OK(hipEventRecord(P->ev[2 * li + 1], st));
+        if (rec_all) mark(2 * nl + 1);
+        const hipError_t e2 = hipGetLastError();
+        return e != hipSuccess ? e : e2;
+    };
+    if (timing || nvalues == 0) {
+        HIP_OK(enqueue(timing));
+    } else {
+        if (!P->fexec) {
+            HIP_OK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            const hipError_t e1 = enqueue(false);
+            hipGraph_t g = nullptr;
+            const hipError_t e2 = hipStreamEndCapture(st, &g);
+            if (e1 != hipSuccess || e2 != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return set_error(SMFGPU_ERR_DEVICE, "factor: capture of the launch sequence failed");
+            }
+            P->fgraph = g;
+            if (hipGraphInstantiate(&P->fexec, g, nullptr, nullptr, 0) != hipSuccess) {
+                release_graph(P);
+                return set_error(SMFGPU_ERR_DEVICE, "factor: graph instantiation failed");
+            }
+        }
+        HIP_OK(hipGraphLaunch(P->fexec, st));
     }
-    if (timing) HIP_OK(hipEventRecord(P->ev[2 * nl + 1], st));
-    HIP_OK(hipGetLastError());
     int32_t flag = 0;
     unsigned long long maxbits = 0;
     HIP_OK(hipMemcpyAsync(&flag, P->d_flag.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
@@ -723,6 +770,24 @@ SMFGPU_API void smfgpu_factor_free(smfgpu_factor_t *f) {
     if ((*f)->plan && (*f)->plan->live == *f) (*f)->plan->live = nullptr;
     delete *f;
     *f = nullptr;
+}
+
+SMFGPU_API int smfgpu_factor_front_stats(smfgpu_factor_t f, int32_t *out) {
+    if (!f || !f->plan || !out) return set_error(SMFGPU_ERR_INVALID_ARG, "null argument");
+    smfgpu_plan_s *P = f->plan;
+    if (P->live != f) return set_error(SMFGPU_ERR_STATE, "front stats: factor was superseded by a newer factorization");
+    for (int32_t s = 0; s < P->nsup; ++s) {
+        int32_t *o = out + 8 * static_cast<int64_t>(s);
+        if (P->posdef) {
+            const FrontDev &F = P->fronts[s];
+            o[0] = F.f; o[1] = F.p; o[2] = F.p; o[3] = 0; o[4] = 0; o[5] = 0; o[6] = 0; o[7] = F.p;
+        } else {
+            const FrontLd &F = P->fl[s];
+            o[0] = F.f; o[1] = F.p; o[2] = F.npos; o[3] = F.nneg; o[4] = F.nzero; o[5] = F.n2x2;
+            o[6] = F.delayed; o[7] = F.ne;
+        }
+    }
+    return SMFGPU_OK;
 }
 
 SMFGPU_API int smfgpu_export_factor(smfgpu_factor_t f, int64_t *sizes, int64_t *steps, double *factor_values,

@@ -67,6 +67,10 @@ struct smfgpu_plan_s {
     int32_t n = 0, nsup = 0;
     int32_t posdef = 1;
     hipStream_t stream = nullptr;
+    // SPD: the factorization's launch sequence, captured once and replayed
+    // (it reads the values from d_vals)
+    hipGraph_t fgraph = nullptr;
+    hipGraphExec_t fexec = nullptr;
     std::vector<FrontDev> fronts;
     std::vector<int32_t> level;                    // per supernode
     std::vector<std::vector<int32_t>> level_fronts;
@@ -93,6 +97,7 @@ struct smfgpu_plan_s {
     DBuf<SolveItem> d_fwd_items, d_bwd_items;
     std::vector<std::pair<int64_t, int64_t>> fwd_steps, bwd_steps;
     std::vector<int32_t> fwd_lv, bwd_lv;
+    int32_t syrk_kc = 16;                          // Schur-update panel columns per LDS stage (16 or 32)
     int32_t solve_fuse_rows = 512;                 // rows below a last block its own workgroup updates
     double t_solve_phase[4] = {0, 0, 0, 0};        // fwd ms, bwd ms, fwd launches, bwd launches
     // refinement CSR (full symmetric, permuted numbering)

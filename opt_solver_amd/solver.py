@@ -101,6 +101,7 @@ class Plan:
         check(lib().smfgpu_plan_create(ctypes.byref(desc), 1 if posdef else 0, ctypes.byref(h)), "plan_create")
         self.h = h
         self.n_orig = int(desc.n_orig)
+        self.nsup = int(desc.nsup)
 
     def stats(self):
         st = np.zeros(8)
@@ -170,6 +171,12 @@ class Factor:
         nrhs = 1 if b.ndim == 1 else b.shape[1]
         check(lib().smfgpu_solve(self.h, _p(b), nrhs, self.plan.n, 0, MEM_HOST), "solve")
         return b
+
+    def front_stats(self):
+        """Per supernode: f, p, positive, negative, zero, 2x2, delayed, eligible."""
+        out = np.zeros((self.plan.nsup, 8), np.int32)
+        check(lib().smfgpu_factor_front_stats(self.h, _p(out)), "front_stats")
+        return out
 
     def solve_device(self, ptr, nrhs, ldb=None):
         check(lib().smfgpu_solve(self.h, ctypes.c_void_p(ptr), nrhs, ldb or self.plan.n, 0, MEM_DEVICE),
