@@ -203,9 +203,9 @@ SMFGPU_API int smfgpu_refine(smfgpu_factor_t f, const double *values, const doub
  * most this many rows below it has them updated by the workgroup that solves
  * the block, saving a step of the sweep (default 512). */
 #define SMFGPU_TUNE_SOLVE_FUSE_ROWS 1
-/* SMFGPU_TUNE_SYRK_KC: panel columns per LDS stage of the SPD Schur-update
- * kernel, 16 or 32 (default 16). */
-#define SMFGPU_TUNE_SYRK_KC 2
+/* SMFGPU_TUNE_SYRK_AHEAD: panel chunks (16 columns each) whose global loads
+ * the SPD Schur-update kernel keeps in flight, 1 or 2 (default 1). */
+#define SMFGPU_TUNE_SYRK_AHEAD 2
 SMFGPU_API int smfgpu_plan_tune(smfgpu_plan_t plan, int32_t key, int64_t value);
 
 /* Device time (ms) of the last factorization's dominant kernel family:

@@ -59,7 +59,7 @@ EXPORTED = [
     "smfgpu_factor_front_stats",
 ]
 TUNE_SOLVE_FUSE_ROWS = 1
-TUNE_SYRK_KC = 2
+TUNE_SYRK_AHEAD = 2
 
 _lib = None
 

@@ -19,6 +19,12 @@ __device__ __forceinline__ double *col_base(const DevPtrs &d, const FrontDev &F,
     return d.CB + F.cbofs + static_cast<int64_t>(c - F.p) * F.q - F.p;
 }
 
+// Loads guarded by a condition are written as an unconditional load from a
+// valid (clamped) address times a 0 / 1 mask: a select around the load lets
+// the compiler move the load into a branch, which then waits for every load
+// in flight (one memory latency per element). Front data are finite.
+__device__ __forceinline__ double masked(double v, bool keep) { return v * (keep ? 1.0 : 0.0); }
+
 // ---------------------------------------------------------------------------
 // A1: permuted values. av[slot] = sum of the input entries mapped to slot,
 // in input order (map_values_to_permuted, workspace.cpp:129-271), plus
@@ -90,7 +96,7 @@ __device__ __forceinline__ int lower_bound_i32(const int32_t *a, int lo, int hi,
 
 __global__ __launch_bounds__(kThreads) void k_assemble(DevPtrs d, const AsmTask *__restrict__ tasks,
                                                        const ChildCol *__restrict__ cc) {
-    __shared__ double buf[kAsmChunk];
+    __shared__ double buf[kAsmChunk + 1];  // [kAsmChunk]: target of masked-out adds
     const AsmTask t = tasks[blockIdx.x];
     const FrontDev F = d.fronts[t.s];
     const int c = t.c;
@@ -108,7 +114,8 @@ __global__ __launch_bounds__(kThreads) void k_assemble(DevPtrs d, const AsmTask 
             const int k0 = d.pcol_ptr[j], k1 = d.pcol_ptr[j + 1];
             for (int k = k0 + tid; k < k1; k += kThreads) {
                 const int lr = d.amap[k];
-                if (lr >= r0 && lr < r1) buf[lr - r0] += d.av[k];
+                const double v = d.av[k];
+                buf[(lr >= r0 && lr < r1) ? lr - r0 : kAsmChunk] += v;
             }
             __syncthreads();
         }
@@ -119,7 +126,21 @@ __global__ __launch_bounds__(kThreads) void k_assemble(DevPtrs d, const AsmTask 
             const double *cbcol = d.CB + C.cbofs + static_cast<int64_t>(ch.jj) * C.q;
             const int lo = lower_bound_i32(cm, ch.jj, C.q, r0);
             const int hi = lower_bound_i32(cm, lo, C.q, r1);
-            for (int ii = lo + tid; ii < hi; ii += kThreads) buf[cm[ii] - r0] += cbcol[ii];
+            // four loads in flight per thread; the adds are unconditional
+            // (past hi: the masked slot), so the loads cannot be moved
+            // behind a branch
+            for (int ii0 = lo + tid; ii0 < hi; ii0 += 4 * kThreads) {
+                int x[4];
+                double v[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int ii = min(ii0 + u * kThreads, hi - 1);
+                    x[u] = cm[ii];
+                    v[u] = cbcol[ii];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) buf[ii0 + u * kThreads < hi ? x[u] - r0 : kAsmChunk] += v[u];
+            }
             __syncthreads();
         }
         for (int r = r0 + tid; r < r1; r += kThreads) colp[r] = buf[r - r0];
@@ -162,8 +183,13 @@ __global__ __launch_bounds__(kThreads) void k_potrf(DevPtrs d, const PotrfItem *
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
         const int c = 16 * cg + i;
+        a[i] = base[static_cast<int64_t>(k + min(c, kb - 1)) * f + k + min(r, kb - 1)];
+    }
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int c = 16 * cg + i;
         const bool in = r < kb && c < kb && r >= c;
-        a[i] = in ? base[static_cast<int64_t>(k + c) * f + k + r] : (r == c ? 1.0 : 0.0);
+        a[i] = masked(a[i], in) + ((!in && r == c) ? 1.0 : 0.0);
         w[i] = (r == c) ? 1.0 : 0.0;
     }
     bool bad = false;
@@ -241,17 +267,21 @@ __global__ __launch_bounds__(kThreads) void k_trsm(DevPtrs d, const TrsmItem *__
     const int nr = min(kNb, f - r0);
     {
         const int rr = tid & 63, cg = tid >> 6;
+        double av[16], gv[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int t = min(cg + 4 * i, kb - 1);
+            av[i] = base[static_cast<int64_t>(k + t) * f + r0 + min(rr, nr - 1)];
+            gv[i] = base[static_cast<int64_t>(k + t) * f + k + min(rr, kb - 1)];
+        }
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
             const int t = cg + 4 * i;
-            As[rr][t] = (rr < nr && t < kb) ? base[static_cast<int64_t>(k + t) * f + r0 + rr] : 0.0;
-            // Ms[t = rr][c]
-            double mv = 0.0;
-            if (rr < kb && t < kb) {
-                const double g = base[static_cast<int64_t>(k + t) * f + k + rr];
-                mv = rr < t ? g : (rr == t ? 1.0 / g : 0.0);
-            }
-            Ms[rr][t] = mv;
+            As[rr][t] = masked(av[i], rr < nr && t < kb);
+            // Ms[t = rr][c]: strict upper M, diagonal 1 / L_cc
+            const double g = gv[i];
+            const bool in = rr < kb && t < kb;
+            Ms[rr][t] = (in && rr < t) ? g : ((in && rr == t) ? 1.0 / g : 0.0);
         }
     }
     __syncthreads();
@@ -318,11 +348,13 @@ struct SyrkMfma<float> {
 };
 
 // One 128 x 128 tile, operands of type E staged in a double-buffered LDS
-// ring (global loads of chunk k+1 in flight during the MFMAs of chunk k).
-// Loads are branch-free (indices clamped into the front, out-of-range
-// values selected to zero); waves wholly above the diagonal run the same
-// loop without MFMAs (ACTIVE = false), so the accumulators stay in AGPRs.
-template <typename E, bool ACTIVE, int kKc>
+// ring. kAhead = 1: the global loads of chunk k+1 are in flight during the
+// MFMAs of chunk k; kAhead = 2: those of chunk k+2 (two register sets, the
+// loads get two MFMA phases to arrive). Loads are branch-free (indices
+// clamped into the front, out-of-range values selected to zero); waves
+// wholly above the diagonal run the same loop without MFMAs (ACTIVE =
+// false).
+template <typename E, bool ACTIVE, int kKc, int kAhead>
 __device__ __forceinline__ void syrk_loop(const SyrkItem &it, const double *Lb, int f, int j1, unsigned char *smem,
                                           typename SyrkMfma<E>::acc_t (&acc)[4][4]) {
     typedef SyrkMfma<E> M;
@@ -334,24 +366,28 @@ __device__ __forceinline__ void syrk_loop(const SyrkItem &it, const double *Lb, 
     const int w = tid >> 6;
     const int wr = w >> 1, wc = w & 1;
     const int i0 = it.i0, j0 = it.j0;
-    double ra[kLoadPer], rb[kLoadPer];
-    auto gload = [&](int kc) {
+    const int t1 = it.t1;
+    typedef double Regs[kLoadPer];
+    auto gload = [&](Regs &ra, Regs &rb, int kc) {
 #pragma unroll
         for (int e = 0; e < kLoadPer; ++e) {
             const int idx = tid + kThreads * e;
             const int t = idx / kT, r = idx % kT;
             const int tt = kc + t;
-            const bool tin = tt < it.t1;
-            const int ttc = tin ? tt : it.t1 - 1;
+            const bool tin = tt < t1;
+            const int ttc = tin ? tt : t1 - 1;
             const int ia = min(i0 + r, f - 1);
             const int jb = min(j0 + r, j1 - 1);
+            // unconditional loads (clamped addresses hold finite front
+            // data), masked by 0 / 1: a select would let the compiler sink
+            // the load into a branch that waits for every load in flight
             const double va = Lb[static_cast<int64_t>(ttc) * f + ia];
             const double vb = Lb[static_cast<int64_t>(ttc) * f + jb];
-            ra[e] = (tin && i0 + r < f) ? va : 0.0;
-            rb[e] = (tin && j0 + r < j1) ? vb : 0.0;
+            ra[e] = va * ((tin && i0 + r < f) ? 1.0 : 0.0);
+            rb[e] = vb * ((tin && j0 + r < j1) ? 1.0 : 0.0);
         }
     };
-    auto sstore = [&](int stg) {
+    auto sstore = [&](const Regs &ra, const Regs &rb, int stg) {
 #pragma unroll
         for (int e = 0; e < kLoadPer; ++e) {
             const int idx = tid + kThreads * e;
@@ -360,35 +396,59 @@ __device__ __forceinline__ void syrk_loop(const SyrkItem &it, const double *Lb, 
             Bs[stg][t][r] = static_cast<E>(rb[e]);
         }
     };
-    gload(it.t0);
-    sstore(0);
-    __syncthreads();
-    int stg = 0;
-    for (int kc = it.t0; kc < it.t1; kc += kKc) {
-        const bool more = kc + kKc < it.t1;
-        if (more) gload(kc + kKc);
-        if (ACTIVE) {
+    auto mfma_stage = [&](int stg) {
+        if (!ACTIVE) return;
 #pragma unroll
-            for (int kk = 0; kk < kKc; kk += 4) {
-                E a[4], b[4];
-                const int kr = kk + (lane >> 4);
+        for (int kk = 0; kk < kKc; kk += 4) {
+            E a[4], b[4];
+            const int kr = kk + (lane >> 4);
 #pragma unroll
-                for (int m = 0; m < 4; ++m) a[m] = Bs[stg][kr][wc * 64 + m * 16 + (lane & 15)];
+            for (int m = 0; m < 4; ++m) a[m] = Bs[stg][kr][wc * 64 + m * 16 + (lane & 15)];
 #pragma unroll
-                for (int nn = 0; nn < 4; ++nn) b[nn] = As[stg][kr][wr * 64 + nn * 16 + (lane & 15)];
+            for (int nn = 0; nn < 4; ++nn) b[nn] = As[stg][kr][wr * 64 + nn * 16 + (lane & 15)];
 #pragma unroll
-                for (int m = 0; m < 4; ++m)
+            for (int m = 0; m < 4; ++m)
 #pragma unroll
-                    for (int nn = 0; nn < 4; ++nn) acc[m][nn] = M::mfma(a[m], b[nn], acc[m][nn]);
-            }
+                for (int nn = 0; nn < 4; ++nn) acc[m][nn] = M::mfma(a[m], b[nn], acc[m][nn]);
         }
-        if (more) sstore(stg ^ 1);
+    };
+    int stg = 0;
+    if (kAhead == 1) {
+        Regs ra, rb;
+        gload(ra, rb, it.t0);
+        sstore(ra, rb, 0);
         __syncthreads();
-        stg ^= 1;
+        for (int kc = it.t0; kc < t1; kc += kKc) {
+            const bool more = kc + kKc < t1;
+            if (more) gload(ra, rb, kc + kKc);
+            mfma_stage(stg);
+            if (more) sstore(ra, rb, stg ^ 1);
+            __syncthreads();
+            stg ^= 1;
+        }
+    } else {
+        // chunk c in LDS stage stg, chunk c+1 in one register set; the
+        // step loads chunk c+2 into the other set (the one just stored)
+        Regs ra0, rb0, ra1, rb1;
+        gload(ra0, rb0, it.t0);
+        if (it.t0 + kKc < t1) gload(ra1, rb1, it.t0 + kKc);
+        sstore(ra0, rb0, 0);
+        __syncthreads();
+        auto step = [&](Regs &la, Regs &lb, const Regs &sa, const Regs &sb, int kc) {
+            if (kc + 2 * kKc < t1) gload(la, lb, kc + 2 * kKc);
+            mfma_stage(stg);
+            if (kc + kKc < t1) sstore(sa, sb, stg ^ 1);
+            __syncthreads();
+            stg ^= 1;
+        };
+        for (int kc = it.t0; kc < t1; kc += 2 * kKc) {
+            step(ra0, rb0, ra1, rb1, kc);
+            if (kc + kKc < t1) step(ra1, rb1, ra0, rb0, kc + kKc);
+        }
     }
 }
 
-template <typename E, int kKc>
+template <typename E, int kKc, int kAhead>
 __device__ __forceinline__ void syrk_tile(const DevPtrs &d, const SyrkItem &it, unsigned char *smem) {
     typedef SyrkMfma<E> M;
     const FrontDev F = d.fronts[it.s];
@@ -406,19 +466,36 @@ __device__ __forceinline__ void syrk_tile(const DevPtrs &d, const SyrkItem &it, 
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = M::zero();
-    if (active) syrk_loop<E, true, kKc>(it, Lb, f, j1, smem, acc);
-    else syrk_loop<E, false, kKc>(it, Lb, f, j1, smem, acc);
+    if (active) syrk_loop<E, true, kKc, kAhead>(it, Lb, f, j1, smem, acc);
+    else syrk_loop<E, false, kKc, kAhead>(it, Lb, f, j1, smem, acc);
     if (!active) return;
+    // C -= acc, 16 elements at a time: all 16 loads issued before the
+    // stores; elements outside the tile's lower part read and write a junk
+    // slot instead, so loads and stores are unconditional
+    double *junk = d.junk + threadIdx.x;
 #pragma unroll
-    for (int m = 0; m < 4; ++m)
+    for (int m = 0; m < 4; ++m) {
+        double *dst[4][4];
+        double old[4][4];
 #pragma unroll
-        for (int nn = 0; nn < 4; ++nn)
+        for (int v = 0; v < 4; ++v) {
+            const int j = j0 + wc * 64 + m * 16 + M::row(v, lane);
+            double *cb = col_base(d, F, min(j, j1 - 1));
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int j = j0 + wc * 64 + m * 16 + M::row(v, lane);
+            for (int nn = 0; nn < 4; ++nn) {
                 const int i = i0 + wr * 64 + nn * 16 + (lane & 15);
-                if (i < f && j < j1 && i >= j) col_base(d, F, j)[i] -= static_cast<double>(acc[m][nn][v]);
+                dst[nn][v] = (i < f && j < j1 && i >= j) ? cb + i : junk;
             }
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+            for (int nn = 0; nn < 4; ++nn) old[nn][v] = *dst[nn][v];
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+            for (int nn = 0; nn < 4; ++nn) *dst[nn][v] = old[nn][v] - static_cast<double>(acc[m][nn][v]);
+    }
 }
 
 // A10/A11: Schur-complement update on MFMA. Tile 128 x 128 of
@@ -429,21 +506,23 @@ __device__ __forceinline__ void syrk_tile(const DevPtrs &d, const SyrkItem &it, 
 // store. Per-front precision: FP64 MFMA, or operands rounded to FP32 with
 // FP32 MFMA accumulation over the panel, subtracted from the FP64 Schur
 // complement.
-// kKc: panel columns per LDS stage (16: 64 KB of LDS, 32: 128 KB; two
-// stages, the global loads of the next one in flight during the MFMAs).
-template <int kKc>
+// 16 panel columns per LDS stage (two stages, 64 KB: two workgroups per
+// CU); kAhead: chunks whose global loads are in flight (1 or 2).
+constexpr int kSyrkKc = 16;
+
+template <int kAhead>
 __global__ __launch_bounds__(kThreads) void k_syrk(DevPtrs d, const SyrkItem *__restrict__ items) {
-    __shared__ __attribute__((aligned(16))) unsigned char smem[4 * kKc * kT * sizeof(double)];
+    __shared__ __attribute__((aligned(16))) unsigned char smem[4 * kSyrkKc * kT * sizeof(double)];
     const SyrkItem it = items[blockIdx.x];
     const int prec = d.prec != nullptr ? d.prec[it.s] : 0;
-    if (prec == 0) syrk_tile<double, kKc>(d, it, smem);
-    else syrk_tile<float, kKc>(d, it, smem);
+    if (prec == 0) syrk_tile<double, kSyrkKc, kAhead>(d, it, smem);
+    else syrk_tile<float, kSyrkKc, kAhead>(d, it, smem);
 }
 
-void launch_syrk(hipStream_t st, const DevPtrs &d, const SyrkItem *it, int64_t n, int kc) {
+void launch_syrk(hipStream_t st, const DevPtrs &d, const SyrkItem *it, int64_t n, int ahead) {
     if (n <= 0) return;
-    if (kc == 16) k_syrk<16><<<dim3(static_cast<unsigned>(n)), kThreads, 0, st>>>(d, it);
-    else k_syrk<32><<<dim3(static_cast<unsigned>(n)), kThreads, 0, st>>>(d, it);
+    if (ahead == 1) k_syrk<1><<<dim3(static_cast<unsigned>(n)), kThreads, 0, st>>>(d, it);
+    else k_syrk<2><<<dim3(static_cast<unsigned>(n)), kThreads, 0, st>>>(d, it);
 }
 
 void launch_zero_upper(hipStream_t, const DevPtrs &, const PotrfItem *, int64_t) {}

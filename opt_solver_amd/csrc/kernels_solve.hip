@@ -32,14 +32,23 @@ struct TriRegs {
 // Element (t = rl, c = g + 8 e) of the triangle's three 64 x 64 blocks:
 // a: upper part of the first diagonal block (t <= c: inverse rows and the
 // diagonal), b: rows 64.. of the block below it, c: second diagonal block.
+// Guarded loads are unconditional loads from clamped (in-front) addresses
+// times a 0 / 1 mask: a select around a load lets the compiler move the
+// load into a branch that waits for every load in flight.
+__device__ __forceinline__ double masked(double v, bool keep) { return v * (keep ? 1.0 : 0.0); }
+
 __device__ __forceinline__ void tri_prefetch(TriRegs &R, const double *Lb, int f, int tk, int tkb, int rl, int g) {
     const int na = min(kSD, tkb), nb = tkb - kSD;
+    const int ra = tk + min(rl, na - 1);                     // rows of block a
+    const int rb = nb > 0 ? tk + kSD + min(rl, nb - 1) : ra;  // rows of block b (a's when there is none)
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
         const int c = g + 8 * e;
-        R.a[e] = (rl <= c && c < na) ? Lb[static_cast<int64_t>(tk + c) * f + tk + rl] : 0.0;
-        R.b[e] = (rl < nb && c < na) ? Lb[static_cast<int64_t>(tk + c) * f + tk + kSD + rl] : 0.0;
-        R.c[e] = (rl <= c && c < nb) ? Lb[static_cast<int64_t>(tk + kSD + c) * f + tk + kSD + rl] : 0.0;
+        const int ca = tk + min(c, na - 1);
+        const int cb = nb > 0 ? tk + kSD + min(c, nb - 1) : ca;
+        R.a[e] = masked(Lb[static_cast<int64_t>(ca) * f + ra], rl <= c && c < na);
+        R.b[e] = masked(Lb[static_cast<int64_t>(ca) * f + rb], rl < nb && c < na);
+        R.c[e] = masked(Lb[static_cast<int64_t>(cb) * f + rb], rl <= c && c < nb);
     }
 }
 
@@ -82,18 +91,19 @@ __device__ void fwd_update(const double *Lb, const int32_t *frows, int f, int p,
     const int tid = threadIdx.x, rl = tid & 63, g = tid >> 6;
     for (int r0 = lo; r0 < hi; r0 += 64) {
         const int row = r0 + rl;
+        const int rowc = min(row, hi - 1);
         double acc[NR];
 #pragma unroll
         for (int r = 0; r < NR; ++r) acc[r] = 0.0;
-        if (row < hi) {
-            const double *col = Lb + static_cast<int64_t>(k + g * 16) * f + row;
-            double l[16];
+        double l[16];
 #pragma unroll
-            for (int tt = 0; tt < 16; ++tt) l[tt] = (g * 16 + tt < kb) ? col[static_cast<int64_t>(tt) * f] : 0.0;
+        for (int tt = 0; tt < 16; ++tt)
+            l[tt] = Lb[static_cast<int64_t>(k + min(g * 16 + tt, kb - 1)) * f + rowc];
 #pragma unroll
-            for (int tt = 0; tt < 16; ++tt)
+        for (int tt = 0; tt < 16; ++tt) {
+            const double lv = masked(l[tt], row < hi && g * 16 + tt < kb);
 #pragma unroll
-                for (int r = 0; r < NR; ++r) acc[r] += l[tt] * zs[g * 16 + tt][r];
+            for (int r = 0; r < NR; ++r) acc[r] += lv * zs[g * 16 + tt][r];
         }
 #pragma unroll
         for (int r = 0; r < NR; ++r) part[g][rl][r] = acc[r];
@@ -215,9 +225,15 @@ __device__ void bwd_update(const double *Lb, const int32_t *frows, int f, int p,
             double la[8], lb[8];
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
-                const int i = i0 + 16 * u + rq;
-                la[u] = (va && i < nr) ? La[i] : 0.0;
-                lb[u] = (vb && i < nr) ? Lbb[i] : 0.0;
+                const int i = min(i0 + 16 * u + rq, nr - 1);
+                la[u] = La[i];
+                lb[u] = Lbb[i];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const bool in = i0 + 16 * u + rq < nr;
+                la[u] = masked(la[u], va && in);
+                lb[u] = masked(lb[u], vb && in);
             }
 #pragma unroll
             for (int u = 0; u < 8; ++u) {

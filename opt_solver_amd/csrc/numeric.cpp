@@ -45,6 +45,7 @@ DevPtrs dev_ptrs(smfgpu_plan_s *P) {
     d.CB = P->d_CB.p;
     d.flag = P->d_flag.p;
     d.prec = P->d_prec.p;
+    d.junk = P->d_junk.p;
     return d;
 }
 
@@ -393,7 +394,7 @@ int plan_build(smfgpu_plan_s *P, const smfgpu_analysis_desc *D) {
     if ((rc = P->d_av.alloc(std::max<int64_t>(nslots, 1))) ||
         (rc = P->d_L.alloc(spd ? std::max<int64_t>(P->factor_entries, 1) : 1)) ||
         (rc = P->d_CB.alloc(spd ? std::max<int64_t>(P->cb_entries, 1) : 1)) || (rc = P->d_red.alloc(8)) ||
-        (rc = P->d_flag.alloc(4)) || (rc = P->d_prec.alloc(std::max(ns, 1))))
+        (rc = P->d_flag.alloc(4)) || (rc = P->d_prec.alloc(std::max(ns, 1))) || (rc = P->d_junk.alloc(1024)))
         return rc;
     if (!spd) return ldlt_plan_build(P, D);
     if ((rc = solve_lists(P))) return rc;
@@ -555,9 +556,9 @@ SMFGPU_API int smfgpu_plan_tune(smfgpu_plan_t P, int32_t key, int64_t value) {
         if (value < 0 || value > (int64_t{1} << 30)) return set_error(SMFGPU_ERR_INVALID_ARG, "tune: bad value");
         P->solve_fuse_rows = static_cast<int32_t>(value);
         return P->posdef ? solve_lists(P) : SMFGPU_OK;
-    case SMFGPU_TUNE_SYRK_KC:
-        if (value != 16 && value != 32) return set_error(SMFGPU_ERR_INVALID_ARG, "tune: K chunk must be 16 or 32");
-        P->syrk_kc = static_cast<int32_t>(value);
+    case SMFGPU_TUNE_SYRK_AHEAD:
+        if (value != 1 && value != 2) return set_error(SMFGPU_ERR_INVALID_ARG, "tune: chunks ahead must be 1 or 2");
+        P->syrk_ahead = static_cast<int32_t>(value);
         release_graph(P);
         return SMFGPU_OK;
     default:
@@ -672,7 +673,7 @@ SMFGPU_API int smfgpu_factor(smfgpu_plan_t P, const double *values, int64_t nval
             case K_ASSEMBLE: launch_assemble(st, d, P->d_asm.p + l.first, l.count, P->d_cc.p); break;
             case K_POTRF: launch_potrf(st, d, P->d_potrf.p + l.first, l.count); break;
             case K_TRSM: launch_trsm(st, d, P->d_trsm.p + l.first, l.count); break;
-            case K_SYRK: launch_syrk(st, d, P->d_syrk.p + l.first, l.count, P->syrk_kc); break;
+            case K_SYRK: launch_syrk(st, d, P->d_syrk.p + l.first, l.count, P->syrk_ahead); break;
             default: break;
             }
             if (rec) mark(2 * li + 1);

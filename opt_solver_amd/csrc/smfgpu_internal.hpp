@@ -106,6 +106,7 @@ struct DevPtrs {
     double *CB;
     int32_t *flag;          // [0]: smallest failing supernode + 1 (SPD); INT32_MAX when none
     const int8_t *prec;     // per supernode: Schur-update precision (null: all FP64)
+    double *junk;           // kThreads doubles: target of masked-out stores (contents meaningless)
 };
 
 void launch_map_values(hipStream_t st, const double *vals, int64_t nvals, const int64_t *slot_ptr,

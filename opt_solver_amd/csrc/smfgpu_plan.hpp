@@ -91,13 +91,14 @@ struct smfgpu_plan_s {
     DBuf<TrsmItem> d_trsm;
     DBuf<SyrkItem> d_syrk;
     DBuf<int8_t> d_prec;
+    DBuf<double> d_junk;
     // solve schedule: items of every step of every level; step i of the
     // forward sweep is fwd_steps[i] = {first item, count}, level lv owns
     // steps [fwd_lv[lv], fwd_lv[lv + 1]) (backward likewise)
     DBuf<SolveItem> d_fwd_items, d_bwd_items;
     std::vector<std::pair<int64_t, int64_t>> fwd_steps, bwd_steps;
     std::vector<int32_t> fwd_lv, bwd_lv;
-    int32_t syrk_kc = 16;                          // Schur-update panel columns per LDS stage (16 or 32)
+    int32_t syrk_ahead = 1;                        // Schur update: chunks with global loads in flight (1 or 2)
     int32_t solve_fuse_rows = 512;                 // rows below a last block its own workgroup updates
     double t_solve_phase[4] = {0, 0, 0, 0};        // fwd ms, bwd ms, fwd launches, bwd launches
     // refinement CSR (full symmetric, permuted numbering)
