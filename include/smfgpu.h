@@ -206,6 +206,13 @@ SMFGPU_API int smfgpu_refine(smfgpu_factor_t f, const double *values, const doub
 /* SMFGPU_TUNE_SYRK_AHEAD: panel chunks (16 columns each) whose global loads
  * the SPD Schur-update kernel keeps in flight, 1 or 2 (default 1). */
 #define SMFGPU_TUNE_SYRK_AHEAD 2
+/* SMFGPU_TUNE_PANEL_OUTER: SPD pivot columns per trailing Schur update
+ * (a multiple of 64, default 512); rebuilds the launch sequence. */
+#define SMFGPU_TUNE_PANEL_OUTER 3
+/* SMFGPU_TUNE_CB_SEG: SPD pivot columns per update of a contribution block,
+ * which is updated once its front's pivots are all factored (default 2048;
+ * FP32 Schur updates accumulate in FP32 over one segment). */
+#define SMFGPU_TUNE_CB_SEG 4
 SMFGPU_API int smfgpu_plan_tune(smfgpu_plan_t plan, int32_t key, int64_t value);
 
 /* Device time (ms) of the last factorization's dominant kernel family:

@@ -60,6 +60,8 @@ EXPORTED = [
 ]
 TUNE_SOLVE_FUSE_ROWS = 1
 TUNE_SYRK_AHEAD = 2
+TUNE_PANEL_OUTER = 3
+TUNE_CB_SEG = 4
 
 _lib = None
 

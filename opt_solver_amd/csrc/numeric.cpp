@@ -159,6 +159,106 @@ int solve_lists(smfgpu_plan_s *P) {
     return SMFGPU_OK;
 }
 
+// SPD launch sequence (static): per level, the assembly, then per outer
+// panel of panel_outer pivot columns and per 64-column block of it: the
+// diagonal Cholesky, the panel TRSM and the Schur update of the rest of the
+// outer panel (K = 64); after the outer panel, the Schur update of the
+// remaining pivot columns with the whole outer panel (K = panel_outer);
+// after the last pivot, the contribution block's update in K segments of
+// cb_seg columns. Items of all fronts of the level share each launch.
+int dense_schedule(smfgpu_plan_s *P) {
+    const int32_t nlev = static_cast<int32_t>(P->level_fronts.size());
+    const int32_t outer = P->panel_outer;
+    std::vector<PotrfItem> potrf;
+    std::vector<TrsmItem> trsm;
+    std::vector<SyrkItem> syrk;
+    P->launches.clear();
+    P->syrk_flops_of_launch.clear();
+    // dense-kernel items and the launch sequence
+    auto add_syrk = [&](int32_t s, int32_t jlo, int32_t jhi, int32_t t0, int32_t t1) {
+        const FrontDev &F = P->fronts[s];
+        for (int32_t jt = jlo; jt < jhi; jt += kTile)
+            for (int32_t it = jt; it < F.f; it += kTile) syrk.push_back(SyrkItem{s, it, jt, jhi, t0, t1, 0, 0});
+    };
+    for (int32_t lv = 0; lv < nlev; ++lv) {
+        const auto &fl = P->level_fronts[lv];
+        P->launches.push_back(Launch{K_ASSEMBLE, lv, P->asm_first[lv], P->asm_first[lv + 1] - P->asm_first[lv]});
+        int32_t maxp = 0;
+        for (int32_t s : fl) maxp = std::max(maxp, P->fronts[s].p);
+        for (int32_t K = 0; K < maxp; K += outer) {
+            for (int32_t k = K; k < std::min(K + outer, maxp); k += kPanelInner) {
+                const int64_t p0 = static_cast<int64_t>(potrf.size());
+                const int64_t t0 = static_cast<int64_t>(trsm.size());
+                for (int32_t s : fl) {
+                    const FrontDev &F = P->fronts[s];
+                    if (F.p <= k) continue;
+                    const int32_t kb = std::min(kPanelInner, F.p - k);
+                    potrf.push_back(PotrfItem{s, k, kb, 0});
+                    for (int32_t r0 = k + kb; r0 < F.f; r0 += kPanelInner) trsm.push_back(TrsmItem{s, k, kb, r0});
+                }
+                P->launches.push_back(Launch{K_POTRF, lv, p0, static_cast<int64_t>(potrf.size()) - p0});
+                P->launches.push_back(Launch{K_TRSM, lv, t0, static_cast<int64_t>(trsm.size()) - t0});
+                const int64_t y0 = static_cast<int64_t>(syrk.size());
+                for (int32_t s : fl) {
+                    const FrontDev &F = P->fronts[s];
+                    if (F.p <= k) continue;
+                    const int32_t kb = std::min(kPanelInner, F.p - k);
+                    const int32_t kend = std::min(K + outer, F.p);
+                    if (k + kb < kend) add_syrk(s, k + kb, kend, k, k + kb);
+                }
+                P->launches.push_back(Launch{K_SYRK, lv, y0, static_cast<int64_t>(syrk.size()) - y0});
+            }
+            // trailing update of the remaining pivot columns only (all
+            // their rows, the ones below the pivots included)
+            const int64_t y0 = static_cast<int64_t>(syrk.size());
+            for (int32_t s : fl) {
+                const FrontDev &F = P->fronts[s];
+                if (F.p <= K) continue;
+                const int32_t kend = std::min(K + outer, F.p);
+                if (kend < F.p) add_syrk(s, kend, F.p, K, kend);
+            }
+            P->launches.push_back(Launch{K_SYRK, lv, y0, static_cast<int64_t>(syrk.size()) - y0});
+        }
+        // the contribution blocks, once per cb_seg pivot columns (the
+        // reference's final schur_update, ldlt_kernel.cpp:1135): far less
+        // read-modify-write traffic than one update per outer panel
+        for (int32_t t0 = 0; t0 < maxp; t0 += P->cb_seg) {
+            const int64_t y0 = static_cast<int64_t>(syrk.size());
+            for (int32_t s : fl) {
+                const FrontDev &F = P->fronts[s];
+                if (F.p <= t0 || F.q <= 0) continue;
+                add_syrk(s, F.p, F.f, t0, std::min(t0 + P->cb_seg, F.p));
+            }
+            P->launches.push_back(Launch{K_SYRK, lv, y0, static_cast<int64_t>(syrk.size()) - y0});
+        }
+    }
+    P->launches.erase(std::remove_if(P->launches.begin(), P->launches.end(),
+                                     [](const Launch &l) { return l.count <= 0; }),
+                      P->launches.end());
+    // syrk flops per launch (algorithmic: 2 * entries updated * panel width)
+    for (const Launch &l : P->launches) {
+        if (l.kind != K_SYRK) {
+            P->syrk_flops_of_launch.push_back(0);
+            continue;
+        }
+        int64_t fl = 0;
+        for (int64_t i = l.first; i < l.first + l.count; ++i) {
+            const SyrkItem &y = syrk[i];
+            const FrontDev &F = P->fronts[y.s];
+            const int64_t jh = std::min<int64_t>(y.j0 + kTile, std::min(y.j1, F.f));
+            const int64_t ih = std::min<int64_t>(y.i0 + kTile, F.f);
+            int64_t cnt = 0;
+            for (int64_t j = y.j0; j < jh; ++j) cnt += std::max<int64_t>(0, ih - std::max<int64_t>(j, y.i0));
+            fl += 2 * cnt * (y.t1 - y.t0);
+        }
+        P->syrk_flops_of_launch.push_back(fl);
+    }
+    int rc;
+    if ((rc = P->d_potrf.upload(potrf)) || (rc = P->d_trsm.upload(trsm)) || (rc = P->d_syrk.upload(syrk))) return rc;
+    release_graph(P);
+    return SMFGPU_OK;
+}
+
 int plan_build(smfgpu_plan_s *P, const smfgpu_analysis_desc *D) {
     const int32_t n = D->n, ns = D->nsup;
     P->n = n;
@@ -245,9 +345,6 @@ int plan_build(smfgpu_plan_s *P, const smfgpu_analysis_desc *D) {
     std::vector<ChildCol> cc;
     std::vector<int64_t> asm_first(nlev + 1, 0);
     std::vector<std::vector<ChildCol>> bucket;
-    std::vector<PotrfItem> potrf;
-    std::vector<TrsmItem> trsm;
-    std::vector<SyrkItem> syrk;
     if (P->posdef) {
     for (int32_t lv = 0; lv < nlev; ++lv) {
         asm_first[lv] = static_cast<int64_t>(tasks.size());
@@ -269,71 +366,7 @@ int plan_build(smfgpu_plan_s *P, const smfgpu_analysis_desc *D) {
     }
     asm_first[nlev] = static_cast<int64_t>(tasks.size());
     if (cc.size() > static_cast<size_t>(INT32_MAX)) return set_error(SMFGPU_ERR_ALLOC, "plan: too many child columns");
-    // dense-kernel items and the launch sequence
-    auto add_syrk = [&](int32_t s, int32_t jlo, int32_t jhi, int32_t t0, int32_t t1) {
-        const FrontDev &F = P->fronts[s];
-        for (int32_t jt = jlo; jt < jhi; jt += kTile)
-            for (int32_t it = jt; it < F.f; it += kTile) syrk.push_back(SyrkItem{s, it, jt, jhi, t0, t1, 0, 0});
-    };
-    for (int32_t lv = 0; lv < nlev; ++lv) {
-        const auto &fl = P->level_fronts[lv];
-        P->launches.push_back(Launch{K_ASSEMBLE, lv, asm_first[lv], asm_first[lv + 1] - asm_first[lv]});
-        int32_t maxp = 0;
-        for (int32_t s : fl) maxp = std::max(maxp, P->fronts[s].p);
-        for (int32_t K = 0; K < maxp; K += kPanelOuter) {
-            for (int32_t k = K; k < std::min(K + kPanelOuter, maxp); k += kPanelInner) {
-                const int64_t p0 = static_cast<int64_t>(potrf.size());
-                const int64_t t0 = static_cast<int64_t>(trsm.size());
-                for (int32_t s : fl) {
-                    const FrontDev &F = P->fronts[s];
-                    if (F.p <= k) continue;
-                    const int32_t kb = std::min(kPanelInner, F.p - k);
-                    potrf.push_back(PotrfItem{s, k, kb, 0});
-                    for (int32_t r0 = k + kb; r0 < F.f; r0 += kPanelInner) trsm.push_back(TrsmItem{s, k, kb, r0});
-                }
-                P->launches.push_back(Launch{K_POTRF, lv, p0, static_cast<int64_t>(potrf.size()) - p0});
-                P->launches.push_back(Launch{K_TRSM, lv, t0, static_cast<int64_t>(trsm.size()) - t0});
-                const int64_t y0 = static_cast<int64_t>(syrk.size());
-                for (int32_t s : fl) {
-                    const FrontDev &F = P->fronts[s];
-                    if (F.p <= k) continue;
-                    const int32_t kb = std::min(kPanelInner, F.p - k);
-                    const int32_t kend = std::min(K + kPanelOuter, F.p);
-                    if (k + kb < kend) add_syrk(s, k + kb, kend, k, k + kb);
-                }
-                P->launches.push_back(Launch{K_SYRK, lv, y0, static_cast<int64_t>(syrk.size()) - y0});
-            }
-            const int64_t y0 = static_cast<int64_t>(syrk.size());
-            for (int32_t s : fl) {
-                const FrontDev &F = P->fronts[s];
-                if (F.p <= K) continue;
-                const int32_t kend = std::min(K + kPanelOuter, F.p);
-                if (kend < F.f) add_syrk(s, kend, F.f, K, kend);
-            }
-            P->launches.push_back(Launch{K_SYRK, lv, y0, static_cast<int64_t>(syrk.size()) - y0});
-        }
-    }
-    P->launches.erase(std::remove_if(P->launches.begin(), P->launches.end(),
-                                     [](const Launch &l) { return l.count <= 0; }),
-                      P->launches.end());
-    // syrk flops per launch (algorithmic: 2 * entries updated * panel width)
-    for (const Launch &l : P->launches) {
-        if (l.kind != K_SYRK) {
-            P->syrk_flops_of_launch.push_back(0);
-            continue;
-        }
-        int64_t fl = 0;
-        for (int64_t i = l.first; i < l.first + l.count; ++i) {
-            const SyrkItem &y = syrk[i];
-            const FrontDev &F = P->fronts[y.s];
-            const int64_t jh = std::min<int64_t>(y.j0 + kTile, std::min(y.j1, F.f));
-            const int64_t ih = std::min<int64_t>(y.i0 + kTile, F.f);
-            int64_t cnt = 0;
-            for (int64_t j = y.j0; j < jh; ++j) cnt += std::max<int64_t>(0, ih - std::max<int64_t>(j, y.i0));
-            fl += 2 * cnt * (y.t1 - y.t0);
-        }
-        P->syrk_flops_of_launch.push_back(fl);
-    }
+    P->asm_first = asm_first;
     }  // posdef
     // value map: slot -> input entries in input order
     const int64_t nslots = P->nnz_perm;
@@ -386,8 +419,7 @@ int plan_build(smfgpu_plan_s *P, const smfgpu_analysis_desc *D) {
         (rc = P->d_cmofs.upload(cmofs)) || (rc = P->d_pcol_ptr.upload(pcp)) || (rc = P->d_amap.upload(amap)) ||
         (rc = P->d_perm.upload(P->perm)) ||
         (rc = P->d_slot_ptr.upload(slot_ptr)) || (rc = P->d_slot_src.upload(slot_src)) ||
-        (rc = P->d_asm.upload(tasks)) || (rc = P->d_cc.upload(cc)) || (rc = P->d_potrf.upload(potrf)) ||
-        (rc = P->d_trsm.upload(trsm)) || (rc = P->d_syrk.upload(syrk)) || (rc = P->d_fr_ptr.upload(fr_ptr)) ||
+        (rc = P->d_asm.upload(tasks)) || (rc = P->d_cc.upload(cc)) || (rc = P->d_fr_ptr.upload(fr_ptr)) ||
         (rc = P->d_fr_col.upload(fr_col)) || (rc = P->d_fr_slot.upload(fr_slot)))
         return rc;
     const bool spd = P->posdef != 0;
@@ -397,6 +429,7 @@ int plan_build(smfgpu_plan_s *P, const smfgpu_analysis_desc *D) {
         (rc = P->d_flag.alloc(4)) || (rc = P->d_prec.alloc(std::max(ns, 1))) || (rc = P->d_junk.alloc(1024)))
         return rc;
     if (!spd) return ldlt_plan_build(P, D);
+    if ((rc = dense_schedule(P))) return rc;
     if ((rc = solve_lists(P))) return rc;
     // algorithmic bytes (assembly, one solve sweep pair per RHS)
     double asm_b = 12.0 * static_cast<double>(P->nnz_perm);
@@ -556,6 +589,15 @@ SMFGPU_API int smfgpu_plan_tune(smfgpu_plan_t P, int32_t key, int64_t value) {
         if (value < 0 || value > (int64_t{1} << 30)) return set_error(SMFGPU_ERR_INVALID_ARG, "tune: bad value");
         P->solve_fuse_rows = static_cast<int32_t>(value);
         return P->posdef ? solve_lists(P) : SMFGPU_OK;
+    case SMFGPU_TUNE_PANEL_OUTER:
+        if (value < kPanelInner || value % kPanelInner != 0 || value > 4096)
+            return set_error(SMFGPU_ERR_INVALID_ARG, "tune: outer panel must be a multiple of 64 in [64, 4096]");
+        P->panel_outer = static_cast<int32_t>(value);
+        return P->posdef ? dense_schedule(P) : SMFGPU_OK;
+    case SMFGPU_TUNE_CB_SEG:
+        if (value < 64 || value > (int64_t{1} << 30)) return set_error(SMFGPU_ERR_INVALID_ARG, "tune: bad segment");
+        P->cb_seg = static_cast<int32_t>(value);
+        return P->posdef ? dense_schedule(P) : SMFGPU_OK;
     case SMFGPU_TUNE_SYRK_AHEAD:
         if (value != 1 && value != 2) return set_error(SMFGPU_ERR_INVALID_ARG, "tune: chunks ahead must be 1 or 2");
         P->syrk_ahead = static_cast<int32_t>(value);

@@ -51,7 +51,7 @@ struct DBuf {
     size_t bytes() const { return n * sizeof(T); }
 };
 
-constexpr int kPanelOuter = 256;
+constexpr int kPanelOuter = 512;  // default outer panel (tunable per plan)
 constexpr int kPanelInner = 64;
 constexpr int kTile = 128;
 
@@ -98,6 +98,9 @@ struct smfgpu_plan_s {
     DBuf<SolveItem> d_fwd_items, d_bwd_items;
     std::vector<std::pair<int64_t, int64_t>> fwd_steps, bwd_steps;
     std::vector<int32_t> fwd_lv, bwd_lv;
+    int32_t panel_outer = kPanelOuter;             // SPD: pivot columns per trailing Schur update
+    std::vector<int64_t> asm_first;                // SPD: first assembly task of each level
+    int32_t cb_seg = 2048;                         // SPD: pivot columns per contribution-block update
     int32_t syrk_ahead = 1;                        // Schur update: chunks with global loads in flight (1 or 2)
     int32_t solve_fuse_rows = 512;                 // rows below a last block its own workgroup updates
     double t_solve_phase[4] = {0, 0, 0, 0};        // fwd ms, bwd ms, fwd launches, bwd launches

@@ -15,7 +15,7 @@ if not gpu_available():
 
 from opt_solver_amd import generators as g  # noqa: E402
 from opt_solver_amd import ordering as o  # noqa: E402
-from opt_solver_amd._lib import TUNE_SOLVE_FUSE_ROWS  # noqa: E402
+from opt_solver_amd._lib import TUNE_CB_SEG, TUNE_PANEL_OUTER, TUNE_SOLVE_FUSE_ROWS  # noqa: E402
 from opt_solver_amd.solver import PREC_FP32, Plan, SmfGpuError, analyse  # noqa: E402
 
 
@@ -174,6 +174,31 @@ def test_refactor_same_plan():
     assert np.abs(f2.solve(matvec(A2, x)) - x).max() < 1e-12
     with pytest.raises(SmfGpuError):
         f1.solve(x)
+
+
+def test_outer_panel_width_invariance():
+    """The trailing-update width (64, 256, the default 512, 1024) and the
+    contribution-block segment (64, 100, unbounded, the default 2048)
+    change only the order of the Schur sums: factors agree to 1e-13
+    relative."""
+    print("case outer panel widths", flush=True)
+    m = 24
+    A = g.spd27_3d(m, seed=8)
+    plan = Plan(analyse(A, perm=o.grid_nd((m, m, m), leaf=32), nemin=16), posdef=True)
+    ref = plan.factor(A[3]).export()
+    x = np.random.default_rng(8).uniform(-1, 1, A[0])
+    for w in (64, 512, 1024, 256):
+        plan.tune(TUNE_PANEL_OUTER, w)
+        fac = plan.factor(A[3])
+        assert block_rel_err(fac.export(), ref) < 1e-13, w
+        assert np.abs(fac.solve(matvec(A, x)) - x).max() < 1e-12, w
+    for seg in (64, 100, 1 << 20, 2048):
+        plan.tune(TUNE_CB_SEG, seg)
+        fac = plan.factor(A[3])
+        assert block_rel_err(fac.export(), ref) < 1e-13, seg
+        assert np.abs(fac.solve(matvec(A, x)) - x).max() < 1e-12, seg
+    with pytest.raises(SmfGpuError):
+        plan.tune(TUNE_PANEL_OUTER, 100)
 
 
 def test_replayed_factor_matches_eager():
