@@ -94,9 +94,15 @@ __device__ __forceinline__ int lower_bound_i32(const int32_t *a, int lo, int hi,
     return lo;
 }
 
+// Child columns per task whose scatter positions are tracked in LDS
+// cursors; further ones (rare) find their chunk start by binary search.
+constexpr int kAsmCursors = 128;
+
 __global__ __launch_bounds__(kThreads) void k_assemble(DevPtrs d, const AsmTask *__restrict__ tasks,
                                                        const ChildCol *__restrict__ cc) {
     __shared__ double buf[kAsmChunk + 1];  // [kAsmChunk]: target of masked-out adds
+    __shared__ int cursor[kAsmCursors];
+    __shared__ int first_out;
     const AsmTask t = tasks[blockIdx.x];
     const FrontDev F = d.fronts[t.s];
     const int c = t.c;
@@ -105,6 +111,10 @@ __global__ __launch_bounds__(kThreads) void k_assemble(DevPtrs d, const AsmTask 
     const int tid = threadIdx.x;
     if (c < F.p)
         for (int r = tid; r < c; r += kThreads) colp[r] = 0.0;
+    // child column jj lands in parent column cm[jj]: its rows ii >= jj go to
+    // the increasing rows cm[ii], so the first chunk starts at ii = jj and
+    // each chunk continues where the previous one stopped
+    for (int e = t.cc0 + tid; e < min(t.cc1, t.cc0 + kAsmCursors); e += kThreads) cursor[e - t.cc0] = cc[e].jj;
     for (int r0 = c; r0 < f; r0 += kAsmChunk) {
         const int r1 = min(f, r0 + kAsmChunk);
         for (int r = r0 + tid; r < r1; r += kThreads) buf[r - r0] = 0.0;
@@ -122,27 +132,41 @@ __global__ __launch_bounds__(kThreads) void k_assemble(DevPtrs d, const AsmTask 
         for (int e = t.cc0; e < t.cc1; ++e) {
             const ChildCol ch = cc[e];
             const FrontDev C = d.fronts[ch.ch];
+            const int q = C.q;
             const int32_t *cm = d.cmap + d.cmofs[ch.ch];
-            const double *cbcol = d.CB + C.cbofs + static_cast<int64_t>(ch.jj) * C.q;
-            const int lo = lower_bound_i32(cm, ch.jj, C.q, r0);
-            const int hi = lower_bound_i32(cm, lo, C.q, r1);
-            // four loads in flight per thread; the adds are unconditional
-            // (past hi: the masked slot), so the loads cannot be moved
-            // behind a branch
-            for (int ii0 = lo + tid; ii0 < hi; ii0 += 4 * kThreads) {
+            const double *cbcol = d.CB + C.cbofs + static_cast<int64_t>(ch.jj) * q;
+            const bool tracked = e - t.cc0 < kAsmCursors;
+            int cur = tracked ? cursor[e - t.cc0] : lower_bound_i32(cm, ch.jj, q, r0);
+            // rounds of 4 x kThreads entries; the entries of this chunk are
+            // a prefix (cm increases), its end found by a minimum in LDS
+            while (cur < q) {
+                if (tid == 0) first_out = 4 * kThreads;
                 int x[4];
                 double v[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    const int ii = min(ii0 + u * kThreads, hi - 1);
+                    const int ii = min(cur + u * kThreads + tid, q - 1);
                     x[u] = cm[ii];
                     v[u] = cbcol[ii];
                 }
+                __syncthreads();
+                int mine = 4 * kThreads;
 #pragma unroll
-                for (int u = 0; u < 4; ++u) buf[ii0 + u * kThreads < hi ? x[u] - r0 : kAsmChunk] += v[u];
+                for (int u = 0; u < 4; ++u) {
+                    const bool take = cur + u * kThreads + tid < q && x[u] < r1;
+                    buf[take ? x[u] - r0 : kAsmChunk] += v[u];
+                    if (!take) mine = min(mine, u * kThreads + tid);
+                }
+                if (mine < 4 * kThreads) atomicMin(&first_out, mine);
+                __syncthreads();
+                const int taken = first_out;
+                __syncthreads();
+                cur += taken;
+                if (taken < 4 * kThreads) break;
             }
-            __syncthreads();
+            if (tracked && tid == 0) cursor[e - t.cc0] = cur;
         }
+        __syncthreads();
         for (int r = r0 + tid; r < r1; r += kThreads) colp[r] = buf[r - r0];
         __syncthreads();
     }
