@@ -224,6 +224,11 @@ SMFGPU_API int smfgpu_plan_tune(smfgpu_plan_t plan, int32_t key, int64_t value);
  * stream when timing is enabled. */
 SMFGPU_API int smfgpu_set_timing(smfgpu_plan_t plan, int32_t enabled);
 SMFGPU_API int smfgpu_kernel_times(smfgpu_plan_t plan, double *out);
+/* SPD launch sequence of the last timed factorization: 5 doubles per launch
+ * (kind 0 assembly / 1 Cholesky / 2 TRSM / 3 Schur, level, items, Schur
+ * flops, ms or -1 when not timed), at most cap launches; returns the
+ * number of launches (out may be null). */
+SMFGPU_API int64_t smfgpu_launch_profile(smfgpu_plan_t plan, double *out, int64_t cap);
 
 #ifdef __cplusplus
 }

@@ -56,7 +56,7 @@ EXPORTED = [
     "smfgpu_analysis_stats", "smfgpu_analysis_free", "smfgpu_plan_create", "smfgpu_plan_free",
     "smfgpu_plan_stats", "smfgpu_factor", "smfgpu_factor_free", "smfgpu_export_factor", "smfgpu_solve",
     "smfgpu_refine", "smfgpu_set_timing", "smfgpu_kernel_times", "smfgpu_plan_tune",
-    "smfgpu_factor_front_stats",
+    "smfgpu_factor_front_stats", "smfgpu_launch_profile",
 ]
 TUNE_SOLVE_FUSE_ROWS = 1
 TUNE_SYRK_AHEAD = 2
@@ -95,6 +95,8 @@ def lib():
     L.smfgpu_kernel_times.argtypes = [_P, _P]
     L.smfgpu_plan_tune.argtypes = [_P, _I, _L]
     L.smfgpu_factor_front_stats.argtypes = [_P, _P]
+    L.smfgpu_launch_profile.argtypes = [_P, _P, _L]
+    L.smfgpu_launch_profile.restype = _L
     _lib = L
     return L
 

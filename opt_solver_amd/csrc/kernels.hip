@@ -6,6 +6,7 @@
 #include "smfgpu_internal.hpp"
 
 #include <cfloat>
+#include <type_traits>
 
 namespace smfgpu {
 
@@ -24,6 +25,16 @@ __device__ __forceinline__ double *col_base(const DevPtrs &d, const FrontDev &F,
 // the compiler move the load into a branch, which then waits for every load
 // in flight (one memory latency per element). Front data are finite.
 __device__ __forceinline__ double masked(double v, bool keep) { return v * (keep ? 1.0 : 0.0); }
+
+// Workgroups are dealt to the 8 XCDs round-robin by index; this bijection
+// gives XCD x the contiguous item range [start_x, start_x + count_x), so
+// neighbouring items (which share operands) meet in the same L2.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_contiguous(int b, int n) {
+    const int per = n / kXcds, rem = n % kXcds;
+    const int x = b % kXcds, y = b / kXcds;
+    return x < rem ? x * (per + 1) + y : rem * (per + 1) + (x - rem) * per + y;
+}
 
 // ---------------------------------------------------------------------------
 // A1: permuted values. av[slot] = sum of the input entries mapped to slot,
@@ -323,13 +334,20 @@ __global__ __launch_bounds__(kThreads) void k_trsm(DevPtrs d, const TrsmItem *__
             if (kk < 4 * cb + 4) acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(Ms[t][16 * cb + lr], bop, acc[cb], 0, 0, 0);
     }
     const int rr = 16 * w + lr;
+    // fronts with FP32 Schur updates also get the panel in FP32 (the Schur
+    // kernel's operands: half the bytes)
+    float *b32 = (d.L32 != nullptr && d.prec != nullptr && d.prec[it.s] == 1) ? d.L32 + F.lofs : nullptr;
     if (rr < nr) {
 #pragma unroll
         for (int cb = 0; cb < 4; ++cb)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int c = 16 * cb + 4 * v + lk;
-                if (c < kb) base[static_cast<int64_t>(k + c) * f + r0 + rr] = acc[cb][v];
+                if (c < kb) {
+                    const int64_t o = static_cast<int64_t>(k + c) * f + r0 + rr;
+                    base[o] = acc[cb][v];
+                    if (b32) b32[o] = static_cast<float>(acc[cb][v]);
+                }
             }
     }
 }
@@ -379,7 +397,7 @@ struct SyrkMfma<float> {
 // wholly above the diagonal run the same loop without MFMAs (ACTIVE =
 // false).
 template <typename E, bool ACTIVE, int kKc, int kAhead>
-__device__ __forceinline__ void syrk_loop(const SyrkItem &it, const double *Lb, int f, int j1, unsigned char *smem,
+__device__ __forceinline__ void syrk_loop(const SyrkItem &it, const E *Lb, int f, int j1, unsigned char *smem,
                                           typename SyrkMfma<E>::acc_t (&acc)[4][4]) {
     typedef SyrkMfma<E> M;
     constexpr int kLoadPer = (kKc * kT) / kThreads;  // elements of A (and of B) per thread per chunk
@@ -391,33 +409,34 @@ __device__ __forceinline__ void syrk_loop(const SyrkItem &it, const double *Lb, 
     const int wr = w >> 1, wc = w & 1;
     const int i0 = it.i0, j0 = it.j0;
     const int t1 = it.t1;
-    typedef double Regs[kLoadPer];
+    typedef E Regs[kLoadPer];
+    // Unconditional loads from clamped addresses (finite front data); the
+    // 0 / 1 masks for rows past the front, columns past j1 and panel
+    // columns past t1 are applied when the chunk is written to LDS, after
+    // the MFMAs of the previous chunk, so nothing waits for the loads early.
+    // wave-uniform chunk base + 32-bit per-lane offsets (scalar-base
+    // global loads: one offset VGPR per load instead of a 64-bit address)
+    const int ia = min(i0 + (tid % kT), f - 1);
+    const int jb = min(j0 + (tid % kT), j1 - 1);
     auto gload = [&](Regs &ra, Regs &rb, int kc) {
+        const E *cb = Lb + static_cast<int64_t>(kc) * f;
+        const int tlim = t1 - 1 - kc;
 #pragma unroll
         for (int e = 0; e < kLoadPer; ++e) {
-            const int idx = tid + kThreads * e;
-            const int t = idx / kT, r = idx % kT;
-            const int tt = kc + t;
-            const bool tin = tt < t1;
-            const int ttc = tin ? tt : t1 - 1;
-            const int ia = min(i0 + r, f - 1);
-            const int jb = min(j0 + r, j1 - 1);
-            // unconditional loads (clamped addresses hold finite front
-            // data), masked by 0 / 1: a select would let the compiler sink
-            // the load into a branch that waits for every load in flight
-            const double va = Lb[static_cast<int64_t>(ttc) * f + ia];
-            const double vb = Lb[static_cast<int64_t>(ttc) * f + jb];
-            ra[e] = va * ((tin && i0 + r < f) ? 1.0 : 0.0);
-            rb[e] = vb * ((tin && j0 + r < j1) ? 1.0 : 0.0);
+            const int t = min((tid + kThreads * e) / kT, tlim);
+            const uint32_t col = static_cast<uint32_t>(t) * static_cast<uint32_t>(f);
+            ra[e] = cb[col + static_cast<uint32_t>(ia)];
+            rb[e] = cb[col + static_cast<uint32_t>(jb)];
         }
     };
-    auto sstore = [&](const Regs &ra, const Regs &rb, int stg) {
+    auto sstore = [&](const Regs &ra, const Regs &rb, int stg, int kc) {
 #pragma unroll
         for (int e = 0; e < kLoadPer; ++e) {
             const int idx = tid + kThreads * e;
             const int t = idx / kT, r = idx % kT;
-            As[stg][t][r] = static_cast<E>(ra[e]);
-            Bs[stg][t][r] = static_cast<E>(rb[e]);
+            const bool tin = kc + t < t1;
+            As[stg][t][r] = ra[e] * static_cast<E>((tin && i0 + r < f) ? 1 : 0);
+            Bs[stg][t][r] = rb[e] * static_cast<E>((tin && j0 + r < j1) ? 1 : 0);
         }
     };
     auto mfma_stage = [&](int stg) {
@@ -440,13 +459,13 @@ __device__ __forceinline__ void syrk_loop(const SyrkItem &it, const double *Lb, 
     if (kAhead == 1) {
         Regs ra, rb;
         gload(ra, rb, it.t0);
-        sstore(ra, rb, 0);
+        sstore(ra, rb, 0, it.t0);
         __syncthreads();
         for (int kc = it.t0; kc < t1; kc += kKc) {
             const bool more = kc + kKc < t1;
             if (more) gload(ra, rb, kc + kKc);
             mfma_stage(stg);
-            if (more) sstore(ra, rb, stg ^ 1);
+            if (more) sstore(ra, rb, stg ^ 1, kc + kKc);
             __syncthreads();
             stg ^= 1;
         }
@@ -456,12 +475,12 @@ __device__ __forceinline__ void syrk_loop(const SyrkItem &it, const double *Lb, 
         Regs ra0, rb0, ra1, rb1;
         gload(ra0, rb0, it.t0);
         if (it.t0 + kKc < t1) gload(ra1, rb1, it.t0 + kKc);
-        sstore(ra0, rb0, 0);
+        sstore(ra0, rb0, 0, it.t0);
         __syncthreads();
         auto step = [&](Regs &la, Regs &lb, const Regs &sa, const Regs &sb, int kc) {
             if (kc + 2 * kKc < t1) gload(la, lb, kc + 2 * kKc);
             mfma_stage(stg);
-            if (kc + kKc < t1) sstore(sa, sb, stg ^ 1);
+            if (kc + kKc < t1) sstore(sa, sb, stg ^ 1, kc + kKc);
             __syncthreads();
             stg ^= 1;
         };
@@ -479,7 +498,9 @@ __device__ __forceinline__ void syrk_tile(const DevPtrs &d, const SyrkItem &it, 
     const int f = F.f;
     const int i0 = it.i0, j0 = it.j0;
     const int j1 = min(it.j1, f);
-    const double *Lb = d.L + F.lofs;
+    // FP64 tiles read the factor, FP32 tiles its FP32 panel copy
+    const E *Lb = std::is_same<E, double>::value ? reinterpret_cast<const E *>(d.L + F.lofs)
+                                                 : reinterpret_cast<const E *>(d.L32 + F.lofs);
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int w = tid >> 6;
@@ -530,17 +551,22 @@ __device__ __forceinline__ void syrk_tile(const DevPtrs &d, const SyrkItem &it, 
 // store. Per-front precision: FP64 MFMA, or operands rounded to FP32 with
 // FP32 MFMA accumulation over the panel, subtracted from the FP64 Schur
 // complement.
-// 16 panel columns per LDS stage (two stages, 64 KB: two workgroups per
-// CU); kAhead: chunks whose global loads are in flight (1 or 2).
-constexpr int kSyrkKc = 16;
+// Panel columns per LDS stage: 16 for FP64 operands, 32 for FP32 (the
+// same 64 KB for the two stages of both operands, so two workgroups share a
+// CU; FP32 MFMAs take half the time, the longer chunk keeps as much MFMA
+// work in front of each chunk's loads); kAhead: chunks whose global loads
+// are in flight (1 or 2).
+constexpr int kSyrkKc64 = 16;
+constexpr int kSyrkKc32 = 32;
 
 template <int kAhead>
 __global__ __launch_bounds__(kThreads) void k_syrk(DevPtrs d, const SyrkItem *__restrict__ items) {
-    __shared__ __attribute__((aligned(16))) unsigned char smem[4 * kSyrkKc * kT * sizeof(double)];
-    const SyrkItem it = items[blockIdx.x];
+    __shared__ __attribute__((aligned(16))) unsigned char smem[4 * kSyrkKc64 * kT * sizeof(double)];
+    static_assert(kSyrkKc32 * sizeof(float) == kSyrkKc64 * sizeof(double), "LDS stages of equal size");
+    const SyrkItem it = items[xcd_contiguous(blockIdx.x, gridDim.x)];
     const int prec = d.prec != nullptr ? d.prec[it.s] : 0;
-    if (prec == 0) syrk_tile<double, kSyrkKc, kAhead>(d, it, smem);
-    else syrk_tile<float, kSyrkKc, kAhead>(d, it, smem);
+    if (prec == 0) syrk_tile<double, kSyrkKc64, kAhead>(d, it, smem);
+    else syrk_tile<float, kSyrkKc32, kAhead>(d, it, smem);
 }
 
 void launch_syrk(hipStream_t st, const DevPtrs &d, const SyrkItem *it, int64_t n, int ahead) {

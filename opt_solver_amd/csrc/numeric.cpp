@@ -42,6 +42,7 @@ DevPtrs dev_ptrs(smfgpu_plan_s *P) {
     d.amap = P->d_amap.p;
     d.av = P->d_av.p;
     d.L = P->d_L.p;
+    d.L32 = P->d_L32.p;
     d.CB = P->d_CB.p;
     d.flag = P->d_flag.p;
     d.prec = P->d_prec.p;
@@ -175,10 +176,18 @@ int dense_schedule(smfgpu_plan_s *P) {
     P->launches.clear();
     P->syrk_flops_of_launch.clear();
     // dense-kernel items and the launch sequence
+    // tiles (it, jt) of the lower part of rows [jlo, f) x columns [jlo, jhi),
+    // in super-blocks of kSyrkGroup x kSyrkGroup tiles: the kernel gives each
+    // XCD a contiguous range of items, so the tiles resident on one XCD
+    // share their row and column panels in its L2
     auto add_syrk = [&](int32_t s, int32_t jlo, int32_t jhi, int32_t t0, int32_t t1) {
         const FrontDev &F = P->fronts[s];
-        for (int32_t jt = jlo; jt < jhi; jt += kTile)
-            for (int32_t it = jt; it < F.f; it += kTile) syrk.push_back(SyrkItem{s, it, jt, jhi, t0, t1, 0, 0});
+        const int32_t nc = (jhi - jlo + kTile - 1) / kTile, nr = (F.f - jlo + kTile - 1) / kTile;
+        for (int32_t C = 0; C < nc; C += kSyrkGroup)
+            for (int32_t R = C; R < nr; R += kSyrkGroup)
+                for (int32_t cb = C; cb < std::min(C + kSyrkGroup, nc); ++cb)
+                    for (int32_t rb = std::max(R, cb); rb < std::min(R + kSyrkGroup, nr); ++rb)
+                        syrk.push_back(SyrkItem{s, jlo + rb * kTile, jlo + cb * kTile, jhi, t0, t1, 0, 0});
     };
     for (int32_t lv = 0; lv < nlev; ++lv) {
         const auto &fl = P->level_fronts[lv];
@@ -582,6 +591,26 @@ SMFGPU_API int smfgpu_plan_stats(smfgpu_plan_t P, double *out) {
     return SMFGPU_OK;
 }
 
+SMFGPU_API int64_t smfgpu_launch_profile(smfgpu_plan_t P, double *out, int64_t cap) {
+    if (!P) return set_error(SMFGPU_ERR_INVALID_ARG, "null handle");
+    const int64_t nl = static_cast<int64_t>(P->launches.size());
+    if (!out) return nl;
+    const bool have = P->timing && P->ev.size() >= static_cast<size_t>(2 * nl + 2);
+    for (int64_t i = 0; i < std::min(nl, cap); ++i) {
+        const Launch &l = P->launches[i];
+        float ms = -1.0f;
+        if (have && (l.kind == K_SYRK || l.kind == K_ASSEMBLE) &&
+            hipEventElapsedTime(&ms, P->ev[2 * i], P->ev[2 * i + 1]) != hipSuccess)
+            ms = -1.0f;
+        out[5 * i + 0] = l.kind;
+        out[5 * i + 1] = l.level;
+        out[5 * i + 2] = static_cast<double>(l.count);
+        out[5 * i + 3] = static_cast<double>(P->syrk_flops_of_launch[i]);
+        out[5 * i + 4] = ms;
+    }
+    return nl;
+}
+
 SMFGPU_API int smfgpu_plan_tune(smfgpu_plan_t P, int32_t key, int64_t value) {
     if (!P) return set_error(SMFGPU_ERR_INVALID_ARG, "null handle");
     switch (key) {
@@ -684,6 +713,12 @@ SMFGPU_API int smfgpu_factor(smfgpu_plan_t P, const double *values, int64_t nval
                 F->prec[s] = 1;  // BF16 requests run as FP32 until the BF16 tile kernel exists
                 ++reduced;
             }
+    }
+    if (reduced > 0 && P->d_L32.p == nullptr) {
+        // FP32 panel copies (the reduced fronts' Schur operands), once per plan
+        int rc = P->d_L32.alloc(std::max<int64_t>(P->factor_entries, 1));
+        if (rc) return rc;
+        release_graph(P);
     }
     HIP_OK(hipMemcpyAsync(P->d_prec.p, F->prec.data(), P->nsup, hipMemcpyHostToDevice, st));
     const int32_t big = INT32_MAX;

@@ -98,7 +98,6 @@ int ldlt_plan_build(smfgpu_plan_s *P, const smfgpu_analysis_desc *D) {
 int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgpu_factor_options &opt,
                 smfgpu_factor_info &fi) {
     hipStream_t st = P->stream;
-    const int32_t ns = P->nsup;
     const int32_t n = P->n;
     P->ldlt_ready = false;
     launch_map_values(st, dv, nvalues, P->d_slot_ptr.p, P->d_slot_src.p, P->nnz_perm, P->d_av.p, P->d_red.p);
@@ -127,6 +126,9 @@ int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgp
     std::vector<int32_t> lrows;
     const int32_t nlev = static_cast<int32_t>(P->level_fronts.size());
     int rc;
+    int64_t negatives = 0;
+    int32_t levels_done = 0;
+    bool inertia_exceeded = false;
     P->rows_h.resize(0);
     for (int32_t lv = 0; lv < nlev; ++lv) {
         const auto &fr = P->level_fronts[lv];
@@ -265,16 +267,33 @@ int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgp
         HIP_OK(hipMemcpyAsync(P->rows_h.data() + rows_start, P->d_lrows.p + rows_start, lrows.size() * sizeof(int32_t),
                               hipMemcpyDeviceToHost, st));
         HIP_OK(hipStreamSynchronize(st));
-        for (size_t i = 0; i < fr.size(); ++i) P->fl[fr[i]] = back[i];
+        for (size_t i = 0; i < fr.size(); ++i) {
+            P->fl[fr[i]] = back[i];
+            negatives += back[i].nneg;
+        }
+        levels_done = lv + 1;
+        // inertia limit (multifrontal_serial.cpp:1619-1627): the reference
+        // stops after the front that takes the negative pivots past the
+        // limit; here the check runs at level boundaries, so the status is
+        // the same (counts only grow) and the partial counts cover whole
+        // levels
+        if (opt.max_negative_pivots >= 0 && negatives > opt.max_negative_pivots) {
+            inertia_exceeded = true;
+            break;
+        }
     }
-    // statistics (Info fields, factor_indef.cpp:725-770)
+    // statistics (Info fields, factor_indef.cpp:725-770) over the fronts
+    // factored
     fi.num_positive = fi.num_negative = fi.num_zero = 0;
     fi.delayed_pivots = fi.two_by_two_pivots = 0;
     fi.actual_flops = 0;
     fi.front_work = 0;
     fi.max_front = 0;
     int64_t fent = 0;
-    for (int32_t s = 0; s < ns; ++s) {
+    std::vector<int32_t> done_fronts;
+    for (int32_t lv = 0; lv < levels_done; ++lv)
+        done_fronts.insert(done_fronts.end(), P->level_fronts[lv].begin(), P->level_fronts[lv].end());
+    for (int32_t s : done_fronts) {
         const FrontLd &F = P->fl[s];
         fi.num_positive += F.npos;
         fi.num_negative += F.nneg;
@@ -287,6 +306,10 @@ int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgp
         fent += static_cast<int64_t>(F.f) * F.pstore;
     }
     fi.factor_entries = fent;
+    if (inertia_exceeded) {
+        fi.status = SMFGPU_STATUS_INERTIA_EXCEEDED;  // factor_indef.cpp:682-715: unusable
+        return SMFGPU_OK;
+    }
     fi.status = fi.num_zero > 0 ? SMFGPU_STATUS_SINGULAR : SMFGPU_STATUS_SUCCESS;
     P->ldlt_ready = true;
     return SMFGPU_OK;

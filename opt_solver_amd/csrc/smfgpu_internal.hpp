@@ -103,6 +103,7 @@ struct DevPtrs {
     const int32_t *amap;    // per permuted entry: local row in its column's front
     const double *av;       // permuted values
     double *L;
+    float *L32;             // FP32 copy of the panel columns of FP32-Schur fronts (null: none)
     double *CB;
     int32_t *flag;          // [0]: smallest failing supernode + 1 (SPD); INT32_MAX when none
     const int8_t *prec;     // per supernode: Schur-update precision (null: all FP64)

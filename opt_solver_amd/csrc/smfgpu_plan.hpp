@@ -54,6 +54,7 @@ struct DBuf {
 constexpr int kPanelOuter = 512;  // default outer panel (tunable per plan)
 constexpr int kPanelInner = 64;
 constexpr int kTile = 128;
+constexpr int kSyrkGroup = 8;  // Schur tiles per super-block side (one XCD's resident tiles)
 
 }  // namespace smfgpu
 
@@ -92,6 +93,7 @@ struct smfgpu_plan_s {
     DBuf<SyrkItem> d_syrk;
     DBuf<int8_t> d_prec;
     DBuf<double> d_junk;
+    DBuf<float> d_L32;  // allocated at the first factorization with FP32 Schur updates
     // solve schedule: items of every step of every level; step i of the
     // forward sweep is fwd_steps[i] = {first item, count}, level lv owns
     // steps [fwd_lv[lv], fwd_lv[lv + 1]) (backward likewise)

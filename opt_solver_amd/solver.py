@@ -120,6 +120,14 @@ class Plan:
                 "fwd_ms", "bwd_ms", "fwd_launches", "bwd_launches")
         return dict(zip(keys, st[:11].tolist()))
 
+    def launch_profile(self):
+        """Per launch of the last timed SPD factorization: kind, level, items,
+        Schur flops, ms (-1: not timed)."""
+        n = int(lib().smfgpu_launch_profile(self.h, None, 0))
+        out = np.zeros((max(n, 1), 5))
+        lib().smfgpu_launch_profile(self.h, _p(out), n)
+        return out[:n]
+
     def tune(self, key, value):
         check(lib().smfgpu_plan_tune(self.h, key, int(value)), "plan_tune")
 

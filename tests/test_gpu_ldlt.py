@@ -117,3 +117,23 @@ def test_singular_reported(oracle_lib):
     assert fac.inertia == orc.inertia
     with pytest.raises(SmfGpuError):
         fac.solve(np.ones(n))
+
+
+def test_inertia_limit():
+    """max_negative_pivots (multifrontal_serial.cpp:1619-1627): a KKT with
+    12 T negative eigenvalues factors with the limit at 12 T and stops with
+    InertiaExceeded below it (the factor is then unusable, as the
+    reference's, factor_indef.cpp:682-715)."""
+    print("case inertia limit", flush=True)
+    T = 20
+    A = g.ocp_kkt(T, seed=3)
+    plan = Plan(analyse(A, nemin=16), posdef=False)
+    ok = plan.factor(A[3], max_negative_pivots=12 * T)
+    assert ok.status == 0 and ok.inertia == (16 * T + 12, 12 * T, 0)
+    stop = plan.factor(A[3], max_negative_pivots=12 * T - 1)
+    assert stop.status == 4
+    assert stop.inertia[1] > 12 * T - 1
+    with pytest.raises(SmfGpuError):
+        stop.solve(np.ones(A[0]))
+    again = plan.factor(A[3], max_negative_pivots=-1)
+    assert again.status == 0 and again.inertia == (16 * T + 12, 12 * T, 0)
