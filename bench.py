@@ -124,7 +124,7 @@ def main():
         fac.solve_device(x_d.data_ptr(), args.nrhs)
         info = {"eta": None, "steps": 0}
         if prec != PREC_FP64:
-            info = fac.refine_device(b_d.data_ptr(), x_d.data_ptr(), args.nrhs, max_iters=10, tol=1e-14)
+            info = fac.refine_device(b_d.data_ptr(), x_d.data_ptr(), args.nrhs, max_iters=10, tol=1e-12)
         return fac, info
 
     results = {}

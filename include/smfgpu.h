@@ -204,7 +204,7 @@ SMFGPU_API int smfgpu_refine(smfgpu_factor_t f, const double *values, const doub
  * the block, saving a step of the sweep (default 512). */
 #define SMFGPU_TUNE_SOLVE_FUSE_ROWS 1
 /* SMFGPU_TUNE_SYRK_AHEAD: panel chunks (16 columns each) whose global loads
- * the SPD Schur-update kernel keeps in flight, 1 or 2 (default 1). */
+ * the SPD Schur-update kernel keeps in flight, 1 or 2 (default 2). */
 #define SMFGPU_TUNE_SYRK_AHEAD 2
 /* SMFGPU_TUNE_PANEL_OUTER: SPD pivot columns per trailing Schur update
  * (a multiple of 64, default 512); rebuilds the launch sequence. */
