@@ -1,0 +1,154 @@
+// MFMA tile of a symmetric rank-K update, shared by the SPD Schur update and
+// the indefinite large-front updates (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace smfgpu {
+
+typedef double v4d_t __attribute__((ext_vector_type(4)));
+typedef float v4f_t __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// A10/A11: Schur-complement update on MFMA. Tile 128 x 128 of
+// C[i, j] -= sum_{t in [t0, t1)} L[i, t] L[j, t], i >= j, four waves of
+// 64 x 64, each 4 x 4 MFMA 16x16x4 blocks. The MFMA computes the
+// transposed block (A operand = the j rows, B operand = the i rows) so that
+// the 16 lanes of a row group hold 16 consecutive rows of one column on the
+// store. prec 0: FP64 MFMA; 1: operands rounded to FP32, FP32 MFMA with FP32
+// accumulation over the panel, subtracted from the FP64 Schur complement.
+constexpr int kT = 128;
+constexpr int kTileThreads = 256;
+
+template <typename E>
+struct SyrkMfma;
+template <>
+struct SyrkMfma<double> {
+    typedef v4d_t acc_t;
+    static __device__ __forceinline__ acc_t zero() { return v4d_t{0.0, 0.0, 0.0, 0.0}; }
+    static __device__ __forceinline__ acc_t mfma(double a, double b, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    // row (within the 16 x 16 block) of accumulator value v in this lane
+    static __device__ __forceinline__ int row(int v, int lane) { return 4 * v + (lane >> 4); }
+};
+template <>
+struct SyrkMfma<float> {
+    typedef v4f_t acc_t;
+    static __device__ __forceinline__ acc_t zero() { return v4f_t{0.f, 0.f, 0.f, 0.f}; }
+    static __device__ __forceinline__ acc_t mfma(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int v, int lane) { return 4 * (lane >> 4) + v; }
+};
+
+// One 128 x 128 tile, operands of type E staged in a double-buffered LDS
+// ring. kAhead = 1: the global loads of chunk k+1 are in flight during the
+// MFMAs of chunk k; kAhead = 2: those of chunk k+2 (two register sets, the
+// loads get two MFMA phases to arrive). Loads are branch-free (indices
+// clamped into the front, out-of-range values selected to zero); waves
+// wholly above the diagonal run the same loop without MFMAs (ACTIVE =
+// false).
+// A operand element (row i, panel column t) = Ab[t * ld + i], B operand
+// element (row j, t) = Bb[(t - boff) * ld + j]; rows i < f, j < j1 are
+// valid, panel columns [t0, t1).
+template <typename E, bool ACTIVE, int kKc, int kAhead>
+__device__ __forceinline__ void tile_loop(const E *Ab, const E *Bb, int ld, int boff, int i0, int j0, int f, int j1,
+                                          int t0, int t1, unsigned char *smem,
+                                          typename SyrkMfma<E>::acc_t (&acc)[4][4]) {
+    typedef SyrkMfma<E> M;
+    constexpr int kThreads = kTileThreads;
+    constexpr int kLoadPer = (kKc * kT) / kThreads;  // elements of A (and of B) per thread per chunk
+    E (*As)[kKc][kT] = reinterpret_cast<E (*)[kKc][kT]>(smem);                            // [2][kKc][kT]
+    E (*Bs)[kKc][kT] = reinterpret_cast<E (*)[kKc][kT]>(smem + 2 * kKc * kT * sizeof(E));  // [2][kKc][kT]
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int w = tid >> 6;
+    const int wr = w >> 1, wc = w & 1;
+    typedef E Regs[kLoadPer];
+    // Unconditional loads from clamped addresses (finite front data); the
+    // 0 / 1 masks for rows past the front, columns past j1 and panel
+    // columns past t1 are applied when the chunk is written to LDS, after
+    // the MFMAs of the previous chunk, so nothing waits for the loads early.
+    // wave-uniform chunk base + 32-bit per-lane offsets (scalar-base
+    // global loads: one offset VGPR per load instead of a 64-bit address)
+    const int ia = min(i0 + (tid % kT), f - 1);
+    const int jb = min(j0 + (tid % kT), j1 - 1);
+    auto gload = [&](Regs &ra, Regs &rb, int kc) {
+        const E *ca = Ab + static_cast<int64_t>(kc) * ld;
+        const E *cb = Bb + static_cast<int64_t>(kc - boff) * ld;
+        const int tlim = t1 - 1 - kc;
+#pragma unroll
+        for (int e = 0; e < kLoadPer; ++e) {
+            const int t = min((tid + kThreads * e) / kT, tlim);
+            const uint32_t col = static_cast<uint32_t>(t) * static_cast<uint32_t>(ld);
+            ra[e] = ca[col + static_cast<uint32_t>(ia)];
+            rb[e] = cb[col + static_cast<uint32_t>(jb)];
+        }
+    };
+    auto sstore = [&](const Regs &ra, const Regs &rb, int stg, int kc) {
+#pragma unroll
+        for (int e = 0; e < kLoadPer; ++e) {
+            const int idx = tid + kThreads * e;
+            const int t = idx / kT, r = idx % kT;
+            const bool tin = kc + t < t1;
+            As[stg][t][r] = ra[e] * static_cast<E>((tin && i0 + r < f) ? 1 : 0);
+            Bs[stg][t][r] = rb[e] * static_cast<E>((tin && j0 + r < j1) ? 1 : 0);
+        }
+    };
+    auto mfma_stage = [&](int stg) {
+        if (!ACTIVE) return;
+#pragma unroll
+        for (int kk = 0; kk < kKc; kk += 4) {
+            E a[4], b[4];
+            const int kr = kk + (lane >> 4);
+#pragma unroll
+            for (int m = 0; m < 4; ++m) a[m] = Bs[stg][kr][wc * 64 + m * 16 + (lane & 15)];
+#pragma unroll
+            for (int nn = 0; nn < 4; ++nn) b[nn] = As[stg][kr][wr * 64 + nn * 16 + (lane & 15)];
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+#pragma unroll
+                for (int nn = 0; nn < 4; ++nn) acc[m][nn] = M::mfma(a[m], b[nn], acc[m][nn]);
+        }
+    };
+    int stg = 0;
+    if (kAhead == 1) {
+        Regs ra, rb;
+        gload(ra, rb, t0);
+        sstore(ra, rb, 0, t0);
+        __syncthreads();
+        for (int kc = t0; kc < t1; kc += kKc) {
+            const bool more = kc + kKc < t1;
+            if (more) gload(ra, rb, kc + kKc);
+            mfma_stage(stg);
+            if (more) sstore(ra, rb, stg ^ 1, kc + kKc);
+            __syncthreads();
+            stg ^= 1;
+        }
+    } else {
+        // chunk c in LDS stage stg, chunk c+1 in one register set; the
+        // step loads chunk c+2 into the other set (the one just stored)
+        Regs ra0, rb0, ra1, rb1;
+        gload(ra0, rb0, t0);
+        if (t0 + kKc < t1) gload(ra1, rb1, t0 + kKc);
+        sstore(ra0, rb0, 0, t0);
+        __syncthreads();
+        auto step = [&](Regs &la, Regs &lb, const Regs &sa, const Regs &sb, int kc) {
+            if (kc + 2 * kKc < t1) gload(la, lb, kc + 2 * kKc);
+            mfma_stage(stg);
+            if (kc + kKc < t1) sstore(sa, sb, stg ^ 1, kc + kKc);
+            __syncthreads();
+            stg ^= 1;
+        };
+        for (int kc = t0; kc < t1; kc += 2 * kKc) {
+            step(ra0, rb0, ra1, rb1, kc);
+            if (kc + kKc < t1) step(ra1, rb1, ra0, rb0, kc + kKc);
+        }
+    }
+}
+
+
+}  // namespace smfgpu
