@@ -213,6 +213,10 @@ SMFGPU_API int smfgpu_refine(smfgpu_factor_t f, const double *values, const doub
  * which is updated once its front's pivots are all factored (default 2048;
  * FP32 Schur updates accumulate in FP32 over one segment). */
 #define SMFGPU_TUNE_CB_SEG 4
+/* SMFGPU_TUNE_LDLT_BIG_MIN: LDL^T fronts with at least this many rows
+ * (default 2048, minimum 96) run the blocked algorithm over many
+ * workgroups (pivot logic per panel in one workgroup, MFMA updates). */
+#define SMFGPU_TUNE_LDLT_BIG_MIN 5
 SMFGPU_API int smfgpu_plan_tune(smfgpu_plan_t plan, int32_t key, int64_t value);
 
 /* Device time (ms) of the last factorization's dominant kernel family:

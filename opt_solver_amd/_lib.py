@@ -62,6 +62,7 @@ TUNE_SOLVE_FUSE_ROWS = 1
 TUNE_SYRK_AHEAD = 2
 TUNE_PANEL_OUTER = 3
 TUNE_CB_SEG = 4
+TUNE_LDLT_BIG_MIN = 5
 
 _lib = None
 

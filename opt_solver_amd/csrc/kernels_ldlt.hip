@@ -4,6 +4,7 @@
 // (solver/CMakeLists.txt:632-642), so the unblocked front kernel
 // reproduces ldlt_factor_front's pivot sequence and values bit for bit.
 #include "smfgpu_ldlt.hpp"
+#include "mfma_tile.hpp"
 
 #include <cfloat>
 
@@ -595,6 +596,386 @@ void launch_ldlt_blocked(hipStream_t st, const LdltDev &d, const int32_t *flist,
     if (nf <= 0) return;
     Prm prm{u, eff_small, small_pivot};
     k_ldlt_blocked<<<dim3(nf), kLT, 0, st>>>(d, flist, prm, nb);
+}
+
+// ---------------------------------------------------------------------------
+// Large fronts: ldlt_factor_front's blocked path (ldlt_kernel.cpp:1028-1139)
+// split over kernels. The panel kernel runs the pivot logic of one panel
+// (select_pivot / apply_pivot on the panel's columns, the unscaled columns
+// kept in W); the update kernel applies the deferred updates on MFMA; the
+// stall kernel does the partner pulls or the rotation of a stalled panel.
+// The deferred update of a panel (columns [pl, ne), rows [pl, f),
+// ldlt_kernel.cpp:705-761) is applied at once to the outer block's columns
+// and held back for the columns past it until a flush; sums differ from the
+// reference's only in order.
+
+// MFMA tile of C[i][j] -= sum_t A[i][t] * B[j][t - boff] over the lower
+// part of rows [c0, f) x columns [c0, c1) of front F; tile `tile` of the
+// region in column-block order.
+__device__ void big_tile(const LdltDev &d, const FrontLd &F, const double *Bm, int boff, int c0, int c1, int t0,
+                         int t1, int tile, unsigned char *smem) {
+    typedef SyrkMfma<double> M;
+    const int f = F.f;
+    const int nr = (f - c0 + kT - 1) / kT;
+    int cb = 0, rem = tile;
+    while (rem >= nr - cb) {
+        rem -= nr - cb;
+        ++cb;
+    }
+    const int i0 = c0 + (cb + rem) * kT, j0 = c0 + cb * kT;
+    double *G = d.arena + F.fofs;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int wr = w >> 1, wc = w & 1;
+    const bool active = (i0 + wr * 64 + 63) >= (j0 + wc * 64);
+    M::acc_t acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = M::zero();
+    if (active) tile_loop<double, true, 16, 2>(G, Bm, f, boff, i0, j0, f, c1, t0, t1, smem, acc);
+    else tile_loop<double, false, 16, 2>(G, Bm, f, boff, i0, j0, f, c1, t0, t1, smem, acc);
+    if (!active) return;
+    double *junk = d.junk + threadIdx.x;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        double *dst[4][4];
+        double old[4][4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int j = j0 + wc * 64 + m * 16 + M::row(v, lane);
+            double *col = G + static_cast<int64_t>(min(j, c1 - 1)) * f;
+#pragma unroll
+            for (int nn = 0; nn < 4; ++nn) {
+                const int i = i0 + wr * 64 + nn * 16 + (lane & 15);
+                dst[nn][v] = (i < f && j < c1 && i >= j) ? col + i : junk;
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+            for (int nn = 0; nn < 4; ++nn) old[nn][v] = *dst[nn][v];
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+            for (int nn = 0; nn < 4; ++nn) *dst[nn][v] = old[nn][v] - acc[m][nn][v];
+    }
+}
+
+__device__ __forceinline__ int big_tiles(int f, int c0, int c1) {
+    if (c1 <= c0) return 0;
+    const int nr = (f - c0 + kT - 1) / kT, nc = (c1 - c0 + kT - 1) / kT;
+    return nc * nr - nc * (nc - 1) / 2;
+}
+
+__global__ __launch_bounds__(kLT) void k_big_panel(LdltDev d, BigLd *states, Prm prm, int32_t nb) {
+    __shared__ RedScratch rs;
+    __shared__ int cnt[3];
+    BigLd &B = states[blockIdx.x];
+    const int pass = B.pass;
+    if (pass == 2) {
+        // done: nothing for this iteration's update kernel
+        if (threadIdx.x == 0) {
+            B.u_p1 = B.u_p0;
+            B.x_p1 = B.x_p0;
+            B.stalled = 0;
+        }
+        return;
+    }
+    FrontLd &Fd = d.fronts[B.s];
+    const int f = Fd.f, ne = Fd.ne;
+    int32_t *rows = d.rows + Fd.rofs;
+    int8_t *tags = d.tags + Fd.rofs;
+    double *W = d.work + Fd.wofs;
+    FV A{d.arena + Fd.fofs, f, f};
+    const int tid = threadIdx.x;
+    int k = B.k, oe = B.oe, pend0 = B.pend0, full = B.full;
+    const int live = B.live;
+    if (tid < 3) cnt[tid] = B.cnt[tid];
+    double flops = B.flops;
+    __syncthreads();
+    int u_c0 = 0, u_c1 = 0, u_p0 = 0, u_p1 = 0, x_c0 = ne, x_p0 = 0, x_p1 = 0, w_base = pend0;
+    int stalled = 0, npass = pass, ps = k, pl = k;
+    if (pass == 0) {
+        if (k >= live) {
+            // the first pass is over: what is held back goes out, then the
+            // second pass (single-level updates)
+            if (k > pend0 && oe < ne) {
+                x_c0 = oe;
+                x_p0 = pend0;
+                x_p1 = k;
+            }
+            pend0 = k;
+            oe = ne;
+            npass = 1;
+        } else {
+            ps = k;
+            pl = min(live, ps + nb);
+            while (k < pl) {
+                int kind, pa, pb;
+                select_pivot_wg(A, k, ne, pl, prm, rs, kind, pa, pb);
+                if (kind == 0) {
+                    stalled = 1;
+                    break;
+                }
+                apply_pivot_wg(A, kind, pa, pb, k, rows, tags, W + static_cast<int64_t>(ps - pend0) * f, f, k - ps, pl,
+                               prm, cnt, flops, 0);
+            }
+            if (k > ps && pl < oe) {
+                u_c0 = pl;
+                u_c1 = oe;
+                u_p0 = ps;
+                u_p1 = k;
+            }
+            // flush before pulls / rotation (they move columns past the
+            // outer block), at the end of the outer block and of the pass
+            if (stalled || k + nb > oe || k >= live) {
+                if (k > pend0 && oe < ne) {
+                    x_c0 = oe;
+                    x_p0 = pend0;
+                    x_p1 = k;
+                }
+                pend0 = k;
+                oe = min(ne, k + kBigOuter);
+            }
+        }
+    } else {
+        oe = ne;
+        if (full) {
+            // a stalled second-pass panel: one pivot from a search over all
+            // the remaining columns (ldlt_kernel.cpp:1119-1131)
+            const int ss = k;
+            int kind, pa, pb;
+            select_pivot_wg(A, k, ne, ne, prm, rs, kind, pa, pb);
+            if (kind == 0) {
+                npass = 2;
+            } else {
+                apply_pivot_wg(A, kind, pa, pb, k, rows, tags, W, f, 0, ss, prm, cnt, flops, 0);
+                u_c0 = k;
+                u_c1 = ne;
+                u_p0 = ss;
+                u_p1 = k;
+                w_base = ss;
+                pend0 = k;
+            }
+            full = 0;
+        } else if (k >= ne) {
+            npass = 2;
+        } else {
+            ps = k;
+            pl = min(ne, ps + nb);
+            while (k < pl) {
+                int kind, pa, pb;
+                select_pivot_wg(A, k, ne, pl, prm, rs, kind, pa, pb);
+                if (kind == 0) {
+                    stalled = 1;
+                    break;
+                }
+                apply_pivot_wg(A, kind, pa, pb, k, rows, tags, W, f, k - ps, pl, prm, cnt, flops, 0);
+            }
+            w_base = ps;
+            if (k > ps && pl < ne) {
+                u_c0 = pl;
+                u_c1 = ne;
+                u_p0 = ps;
+                u_p1 = k;
+            }
+            if (stalled) full = 1;
+            pend0 = k;
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        B.k = k;
+        B.pass = npass;
+        B.full = full;
+        B.stalled = stalled;
+        B.ps = ps;
+        B.pl = pl;
+        B.oe = oe;
+        B.pend0 = pend0;
+        B.u_c0 = u_c0;
+        B.u_c1 = u_c1;
+        B.u_p0 = u_p0;
+        B.u_p1 = u_p1;
+        B.x_c0 = x_c0;
+        B.x_p0 = x_p0;
+        B.x_p1 = x_p1;
+        B.w_base = w_base;
+        B.cnt[0] = cnt[0];
+        B.cnt[1] = cnt[1];
+        B.cnt[2] = cnt[2];
+        B.flops = flops;
+    }
+}
+
+// The requested update and flush of every front: grid (per_front, nbig),
+// tiles by grid stride.
+__global__ __launch_bounds__(kLT) void k_big_update(LdltDev d, const BigLd *__restrict__ states) {
+    __shared__ __attribute__((aligned(16))) unsigned char smem[4 * 16 * kT * sizeof(double)];
+    const BigLd B = states[blockIdx.y];
+    const FrontLd F = d.fronts[B.s];
+    const double *W = d.work + F.wofs;
+    const int t1 = (B.u_p1 > B.u_p0) ? big_tiles(F.f, B.u_c0, B.u_c1) : 0;
+    const int t2 = (B.x_p1 > B.x_p0) ? big_tiles(F.f, B.x_c0, F.ne) : 0;
+    for (int tile = blockIdx.x; tile < t1 + t2; tile += gridDim.x) {
+        if (tile < t1) big_tile(d, F, W, B.w_base, B.u_c0, B.u_c1, B.u_p0, B.u_p1, tile, smem);
+        else big_tile(d, F, W, B.w_base, B.x_c0, F.ne, B.x_p0, B.x_p1, tile - t1, smem);
+        __syncthreads();
+    }
+}
+
+// Partner pulls or rotation of a stalled first-pass panel
+// (ldlt_kernel.cpp:1052-1097); every column is current (flushed).
+__global__ __launch_bounds__(kLT) void k_big_stall(LdltDev d, BigLd *states, int32_t nb) {
+    __shared__ int32_t order[128];
+    __shared__ int np_s;
+    BigLd &B = states[blockIdx.x];
+    if (B.pass != 0 || !B.stalled) return;
+    FrontLd &Fd = d.fronts[B.s];
+    const int f = Fd.f, ne = Fd.ne;
+    int32_t *rows = d.rows + Fd.rofs;
+    FV A{d.arena + Fd.fofs, f, f};
+    const int tid = threadIdx.x;
+    const int k = B.k, pl = B.pl;
+    int live = B.live;
+    if (k != B.k_paired) {
+        if (tid == 0) {
+            int np = 0;
+            const int max_pairs = min(nb / 2, 64);
+            for (int pos = k; pos < pl && np / 2 < max_pairs; ++pos) {
+                double in_max = 0.0;
+                for (int i = k; i < pos; ++i) in_max = fmax(in_max, fabs(A.at(pos, i)));
+                for (int i = pos + 1; i < pl; ++i) in_max = fmax(in_max, fabs(A.at(i, pos)));
+                double out_max = 0.0;
+                int r = -1;
+                for (int i = pl; i < ne; ++i) {
+                    const double v = fabs(A.at(i, pos));
+                    if (v > out_max) {
+                        out_max = v;
+                        r = i;
+                    }
+                }
+                if (r < 0 || !(out_max > in_max)) continue;
+                bool taken = false;
+                for (int t = 1; t < np; t += 2) taken = taken || order[t] == rows[r];
+                if (taken) continue;
+                order[np++] = rows[pos];
+                order[np++] = rows[r];
+            }
+            np_s = np;
+        }
+        __syncthreads();
+        const int np = np_s;
+        if (np > 0) {
+            for (int t = 0; t < np; ++t) {
+                int q = k + t;
+                while (rows[q] != order[t]) ++q;
+                sym_swap_wg(A, k + t, q, rows, nullptr, 0, 0, 0);
+            }
+            if (tid == 0) {
+                B.live = max(live, k + np);
+                B.k_paired = k;
+                B.stalled = 0;
+            }
+            return;
+        }
+    }
+    const int m = pl - k;
+    const int mv = min(m, live - pl);
+    for (int t = 0; t < mv; ++t) sym_swap_wg(A, k + t, live - mv + t, rows, nullptr, 0, 0, 0);
+    if (tid == 0) {
+        B.live = live - m;
+        B.stalled = 0;
+    }
+}
+
+// L*D columns [c0, c1) of every done front, rows [ne, f), into W (the
+// column order of schur_update's fill_ld_column, ldlt_kernel.cpp:764-816)
+__global__ __launch_bounds__(kLT) void k_big_ld(LdltDev d, const BigLd *__restrict__ states, int32_t c0,
+                                               int32_t width) {
+    const BigLd B = states[blockIdx.y];
+    const FrontLd F = d.fronts[B.s];
+    const int f = F.f, ne = F.ne, p = B.k;
+    const int c1 = min(p, c0 + width);
+    if (c1 <= c0 || ne >= f) return;
+    const double *G = d.arena + F.fofs;
+    const int8_t *tags = d.tags + F.rofs;
+    double *W = d.work + F.wofs;
+    const int q = f - ne;
+    const int64_t tot = static_cast<int64_t>(c1 - c0) * q;
+    for (int64_t idx = static_cast<int64_t>(blockIdx.x) * kLT + threadIdx.x; idx < tot;
+         idx += static_cast<int64_t>(gridDim.x) * kLT) {
+        const int c = c0 + static_cast<int>(idx / q);
+        const int j = ne + static_cast<int>(idx % q);
+        const int8_t tg = tags[c];
+        double v = 0.0;
+        if (tg == 1) {
+            v = G[static_cast<int64_t>(c) * f + j] * G[static_cast<int64_t>(c) * f + c];
+        } else if (tg == 2) {
+            v = __dadd_rn(G[static_cast<int64_t>(c) * f + j] * G[static_cast<int64_t>(c) * f + c],
+                          G[static_cast<int64_t>(c + 1) * f + j] * G[static_cast<int64_t>(c) * f + c + 1]);
+        } else if (tg == -1) {
+            v = __dadd_rn(G[static_cast<int64_t>(c - 1) * f + j] * G[static_cast<int64_t>(c - 1) * f + c],
+                          G[static_cast<int64_t>(c) * f + j] * G[static_cast<int64_t>(c) * f + c]);
+        }
+        W[static_cast<int64_t>(c - c0) * f + j] = v;
+    }
+}
+
+__global__ __launch_bounds__(kLT) void k_big_cbupdate(LdltDev d, const BigLd *__restrict__ states, int32_t c0,
+                                                     int32_t width) {
+    __shared__ __attribute__((aligned(16))) unsigned char smem[4 * 16 * kT * sizeof(double)];
+    const BigLd B = states[blockIdx.y];
+    const FrontLd F = d.fronts[B.s];
+    const int c1 = min(B.k, c0 + width);
+    if (c1 <= c0 || F.ne >= F.f) return;
+    const double *W = d.work + F.wofs;
+    const int nt = big_tiles(F.f, F.ne, F.f);
+    for (int tile = blockIdx.x; tile < nt; tile += gridDim.x) {
+        big_tile(d, F, W, c0, F.ne, F.f, c0, c1, tile, smem);
+        __syncthreads();
+    }
+}
+
+__global__ void k_big_finish(LdltDev d, const BigLd *__restrict__ states, int32_t nbig) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbig) return;
+    const BigLd B = states[b];
+    FrontLd &F = d.fronts[B.s];
+    F.p = B.k;
+    F.npos = B.cnt[0];
+    F.nneg = B.cnt[1];
+    F.nzero = B.cnt[2];
+    F.flops = B.flops;
+}
+
+void launch_big_panel(hipStream_t st, const LdltDev &d, BigLd *states, int32_t nbig, double u, double eff_small,
+                      double small_pivot, int32_t nb) {
+    if (nbig <= 0) return;
+    Prm prm{u, eff_small, small_pivot};
+    k_big_panel<<<dim3(nbig), kLT, 0, st>>>(d, states, prm, nb);
+}
+
+void launch_big_update(hipStream_t st, const LdltDev &d, const BigLd *states, int32_t nbig, int32_t per_front) {
+    if (nbig <= 0) return;
+    k_big_update<<<dim3(per_front, nbig), kLT, 0, st>>>(d, states);
+}
+
+void launch_big_stall(hipStream_t st, const LdltDev &d, BigLd *states, int32_t nbig, int32_t nb) {
+    if (nbig <= 0) return;
+    k_big_stall<<<dim3(nbig), kLT, 0, st>>>(d, states, nb);
+}
+
+void launch_big_cb(hipStream_t st, const LdltDev &d, const BigLd *states, int32_t nbig, int32_t c0, int32_t width,
+                   int32_t per_front) {
+    if (nbig <= 0) return;
+    k_big_ld<<<dim3(per_front, nbig), kLT, 0, st>>>(d, states, c0, width);
+    k_big_cbupdate<<<dim3(per_front, nbig), kLT, 0, st>>>(d, states, c0, width);
+}
+
+void launch_big_finish(hipStream_t st, const LdltDev &d, const BigLd *states, int32_t nbig) {
+    if (nbig <= 0) return;
+    k_big_finish<<<dim3((nbig + 63) / 64), 64, 0, st>>>(d, states, nbig);
 }
 
 // ---------------------------------------------------------------------------

@@ -627,6 +627,10 @@ SMFGPU_API int smfgpu_plan_tune(smfgpu_plan_t P, int32_t key, int64_t value) {
         if (value < 64 || value > (int64_t{1} << 30)) return set_error(SMFGPU_ERR_INVALID_ARG, "tune: bad segment");
         P->cb_seg = static_cast<int32_t>(value);
         return P->posdef ? dense_schedule(P) : SMFGPU_OK;
+    case SMFGPU_TUNE_LDLT_BIG_MIN:
+        if (value < 96 || value > (int64_t{1} << 30)) return set_error(SMFGPU_ERR_INVALID_ARG, "tune: bad front size");
+        P->ldlt_big_min = static_cast<int32_t>(value);
+        return SMFGPU_OK;
     case SMFGPU_TUNE_SYRK_AHEAD:
         if (value != 1 && value != 2) return set_error(SMFGPU_ERR_INVALID_ARG, "tune: chunks ahead must be 1 or 2");
         P->syrk_ahead = static_cast<int32_t>(value);

@@ -46,6 +46,7 @@ LdltDev ldlt_dev(smfgpu_plan_s *P) {
     d.pcol_ptr = P->d_pcol_ptr.p;
     d.amap = P->d_amap.p;
     d.av = P->d_av.p;
+    d.junk = P->d_junk.p;
     return d;
 }
 
@@ -95,6 +96,53 @@ int ldlt_plan_build(smfgpu_plan_s *P, const smfgpu_analysis_desc *D) {
     return SMFGPU_OK;
 }
 
+// Large fronts of one level (multi-workgroup blocked LDL^T): the panel,
+// update and stall kernels in turn until every front is done (the state
+// read back every kBigCheck rounds), then the contribution blocks' update
+// in chunks of kBigWork pivots and the fronts' results.
+static int ldlt_big_fronts(smfgpu_plan_s *P, const LdltDev &d, const std::vector<int32_t> &big,
+                           const smfgpu_factor_options &opt, double eff_small, double small, int32_t nb,
+                           std::vector<BigLd> &h) {
+    constexpr int kBigCheck = 16;
+    hipStream_t st = P->stream;
+    const int32_t nbig = static_cast<int32_t>(big.size());
+    h.assign(nbig, BigLd{});
+    int64_t cap = 64;
+    for (int32_t b = 0; b < nbig; ++b) {
+        const FrontLd &F = P->fl[big[b]];
+        BigLd &B = h[b];
+        B.s = big[b];
+        B.live = F.ne;
+        B.k_paired = -1;
+        B.oe = std::min(F.ne, kBigOuter);
+        cap += 4 * (static_cast<int64_t>(F.ne) / std::max(nb, 1) + 2) + 2 * static_cast<int64_t>(F.ne);
+    }
+    int rc;
+    if ((rc = ensure(P->d_big, static_cast<size_t>(nbig), false))) return rc;
+    HIP_OK(hipMemcpyAsync(P->d_big.p, h.data(), nbig * sizeof(BigLd), hipMemcpyHostToDevice, st));
+    const int32_t per_front = std::max(64, 1024 / nbig);
+    for (int64_t it = 0;; ++it) {
+        launch_big_panel(st, d, P->d_big.p, nbig, opt.pivot_u, eff_small, small, nb);
+        launch_big_update(st, d, P->d_big.p, nbig, per_front);
+        launch_big_stall(st, d, P->d_big.p, nbig, nb);
+        if (it % kBigCheck == kBigCheck - 1) {
+            HIP_OK(hipMemcpyAsync(h.data(), P->d_big.p, nbig * sizeof(BigLd), hipMemcpyDeviceToHost, st));
+            HIP_OK(hipStreamSynchronize(st));
+            bool done = true;
+            for (const BigLd &B : h) done = done && B.pass == 2;
+            if (done) break;
+            if (it > cap) return set_error(SMFGPU_ERR_DEVICE, "factor: a large front did not finish");
+        }
+    }
+    HIP_OK(hipGetLastError());
+    int32_t pmax = 0;
+    for (const BigLd &B : h) pmax = std::max(pmax, B.k);
+    for (int32_t c0 = 0; c0 < pmax; c0 += kBigWork) launch_big_cb(st, d, P->d_big.p, nbig, c0, kBigWork, per_front);
+    launch_big_finish(st, d, P->d_big.p, nbig);
+    HIP_OK(hipGetLastError());
+    return SMFGPU_OK;
+}
+
 int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgpu_factor_options &opt,
                 smfgpu_factor_info &fi) {
     hipStream_t st = P->stream;
@@ -121,7 +169,8 @@ int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgp
     std::vector<AsmTask> tasks;
     std::vector<ChildCol> cc;
     std::vector<std::vector<ChildCol>> bucket;
-    std::vector<int32_t> ids, small_list, blocked_list;
+    std::vector<int32_t> ids, small_list, blocked_list, big_list;
+    std::vector<BigLd> big_h;
     std::vector<FrontLd> patch;
     std::vector<int32_t> lrows;
     const int32_t nlev = static_cast<int32_t>(P->level_fronts.size());
@@ -140,6 +189,7 @@ int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgp
         lrows.clear();
         small_list.clear();
         blocked_list.clear();
+        big_list.clear();
         const int64_t rows_start = P->rows_used;
         int64_t work_used = 0;
         for (int32_t s : fr) {
@@ -157,7 +207,11 @@ int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgp
             F.rofs = P->rows_used;
             F.dofs = P->dinv_used;
             F.wofs = 0;
-            if (F.f >= kBlockedMinFront) {
+            if (F.f >= P->ldlt_big_min && F.ne > 0) {
+                F.wofs = work_used;
+                work_used += static_cast<int64_t>(F.f) * kBigWork;
+                big_list.push_back(s);
+            } else if (F.f >= kBlockedMinFront) {
                 F.wofs = work_used;
                 work_used += static_cast<int64_t>(F.f) * nb;
                 blocked_list.push_back(s);
@@ -256,6 +310,7 @@ int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgp
                           opt.pivot_u, eff_small, small);
         launch_ldlt_blocked(st, d, reinterpret_cast<const int32_t *>(S + o_blk),
                             static_cast<int32_t>(blocked_list.size()), opt.pivot_u, eff_small, small, nb);
+        if (!big_list.empty() && (rc = ldlt_big_fronts(P, d, big_list, opt, eff_small, small, nb, big_h))) return rc;
         launch_ldlt_epilogue(st, d, reinterpret_cast<const int32_t *>(S + o_fr), static_cast<int32_t>(fr.size()));
         launch_front_gather(st, P->d_fl.p, reinterpret_cast<FrontLd *>(S + o_back),
                             reinterpret_cast<const int32_t *>(S + o_fr), static_cast<int32_t>(fr.size()));

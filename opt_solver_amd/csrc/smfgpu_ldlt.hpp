@@ -41,6 +41,35 @@ struct LdltDev {
     const int32_t *pcol_ptr;
     const int32_t *amap;      // per permuted entry: position among its front's symbolic rows
     const double *av;
+    double *junk;             // target of masked-out stores of the MFMA updates (contents meaningless)
+};
+
+// Large fronts (multi-workgroup blocked LDL^T): one state per front,
+// advanced by a panel kernel (one workgroup per front: the blocked
+// algorithm's pivot logic on a panel of nb columns), an update kernel (many
+// workgroups, MFMA) and a stall kernel (partner pulls / rotation), launched
+// in turn by the host until every front is done. The per-panel deferred
+// updates go to the columns of a kBigOuter-wide outer block; the pivots'
+// updates of the columns beyond it are held back (their unscaled columns
+// kept in W) and applied in one flush when the outer block ends, before a
+// stall's pulls or rotation, and before the second pass.
+constexpr int32_t kBigOuter = 256;
+constexpr int32_t kBigWork = 512;  // W columns per front (outer block + panel; L*D chunks of the final update)
+struct BigLd {
+    int32_t s;                  // supernode
+    int32_t k, live, k_paired;  // next pivot position, end of the live eligible columns, pulls' guard
+    int32_t pass;               // 0 first pass, 1 second pass, 2 done
+    int32_t full;               // second pass: next step is a full-search pivot
+    int32_t stalled;            // first pass: the last panel stalled
+    int32_t ps, pl;             // the last panel
+    int32_t oe;                 // outer block end
+    int32_t pend0;              // pivots [pend0, k) held back for the columns >= oe; W column 0
+    int32_t u_c0, u_c1, u_p0, u_p1;  // update request: columns [u_c0, u_c1), rows [u_c0, f), pivots [u_p0, u_p1)
+    int32_t x_c0, x_p0, x_p1;        // flush request: columns [x_c0, ne), rows [x_c0, f), pivots [x_p0, x_p1)
+    int32_t w_base;             // pivot of W column 0 for both requests
+    int32_t cnt[3];
+    int32_t pad;
+    double flops;
 };
 
 void launch_ldlt_small(hipStream_t st, const LdltDev &d, const int32_t *flist, int32_t nf, double u, double eff_small,
@@ -52,5 +81,14 @@ void launch_ldlt_epilogue(hipStream_t st, const LdltDev &d, const int32_t *flist
 void launch_ldlt_solve_level(hipStream_t st, const LdltDev &d, const int32_t *flist, int32_t nf, double *y, int32_t n,
                              int32_t nrhs, int32_t phase);
 void launch_front_patch(hipStream_t st, FrontLd *fronts, const FrontLd *patch, const int32_t *ids, int32_t np);
+void launch_big_panel(hipStream_t st, const LdltDev &d, BigLd *states, int32_t nbig, double u, double eff_small,
+                      double small_pivot, int32_t nb);
+void launch_big_update(hipStream_t st, const LdltDev &d, const BigLd *states, int32_t nbig, int32_t per_front);
+void launch_big_stall(hipStream_t st, const LdltDev &d, BigLd *states, int32_t nbig, int32_t nb);
+// final contribution-block update with pivots [c0, c0 + width) of every
+// front (L*D of the chunk into W first); then the fronts' results
+void launch_big_cb(hipStream_t st, const LdltDev &d, const BigLd *states, int32_t nbig, int32_t c0, int32_t width,
+                   int32_t per_front);
+void launch_big_finish(hipStream_t st, const LdltDev &d, const BigLd *states, int32_t nbig);
 
 }  // namespace smfgpu

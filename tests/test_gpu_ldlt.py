@@ -137,3 +137,63 @@ def test_inertia_limit():
         stop.solve(np.ones(A[0]))
     again = plan.factor(A[3], max_negative_pivots=-1)
     assert again.status == 0 and again.inertia == (16 * T + 12, 12 * T, 0)
+
+
+def _large_path_case(case):
+    from oracle import refshim
+    if case.startswith("kkt_grid"):
+        gsz = int(case.split("_")[2])
+        m = gsz * gsz // 5
+        A, _ = g.kkt_grid(gsz, m, seed=gsz)
+        return A, analyse(A, nemin=16).arrays(), (gsz * gsz, m, 0)
+    if case.startswith("ocp"):
+        T = int(case[5:])
+        A = g.ocp_kkt(T, seed=T)
+        return A, _arrays_for(A, refshim.ORDER_AMD), (16 * T + 12, 12 * T, 0)
+    gsz = int(case[4:])
+    m = gsz * gsz // 5
+    A, _ = g.kkt_grid(gsz, m, seed=gsz)
+    return A, _arrays_for(A, refshim.ORDER_METIS), (gsz * gsz, m, 0)
+
+
+@pytest.mark.parametrize("case", ["kkt_30", "kkt_48", "ocp_T200", "kkt_grid_60"])
+def test_large_front_path_against_oracle(oracle_lib, case):
+    """The multi-workgroup blocked path (forced on every blocked front):
+    the same pivot sequence, delays, 2x2 pivots and inertia as the oracle's
+    single-workgroup blocked kernel, L and D within 1e-9 (the deferred
+    updates run on MFMA in another summation order), solves to 1e-14."""
+    from opt_solver_amd._lib import TUNE_LDLT_BIG_MIN
+    print("case large path", case, flush=True)
+    A, arrays, inertia = _large_path_case(case)
+    plan = Plan(arrays, posdef=False)
+    plan.tune(TUNE_LDLT_BIG_MIN, 96)
+    fac = plan.factor(A[3], u=0.01)
+    orc = oracle_lib.Factor(arrays, A[3], posdef=False)
+    assert fac.inertia == orc.inertia == inertia
+    assert fac.info.delayed_pivots == orc.delayed and fac.info.two_by_two_pivots == orc.two_by_two
+    compare_factor(fac.export(), orc.export(), 1e-9)
+    x = np.random.default_rng(7).uniform(-1, 1, A[0])
+    b = matvec(A, x)
+    xs = fac.solve(b)
+    eta, _ = oracle_lib.backward_error(A, b, xs)
+    assert eta <= 1e-14, eta
+
+
+def test_large_front_default_threshold(oracle_lib):
+    """A grid KKT whose largest front (3016 rows) takes the large-front path
+    at the default threshold: Sylvester inertia and a backward error at
+    FP64 level (too large for the CPU oracle's dense kernel in a test)."""
+    print("case kkt_grid_120", flush=True)
+    gsz = 120
+    m = gsz * gsz // 5
+    A, _ = g.kkt_grid(gsz, m, seed=gsz)
+    an = analyse(A, nemin=16)
+    assert an.max_front >= 2048
+    plan = Plan(an, posdef=False)
+    fac = plan.factor(A[3], u=0.01)
+    assert fac.status == 0 and fac.inertia == (gsz * gsz, m, 0)
+    x = np.random.default_rng(9).uniform(-1, 1, A[0])
+    b = matvec(A, x)
+    xs = fac.solve(b)
+    eta, _ = oracle_lib.backward_error(A, b, xs)
+    assert eta <= 1e-13, eta
