@@ -197,6 +197,15 @@ __device__ void select_pivot_wg(const FV &A, int k, int ne, int limit, const Prm
     }
 }
 
+// Lanes per column when the workgroup sweeps the lower part of several
+// columns of up to `len` rows: log2 of a power of two covering len (at most
+// the workgroup), so kLT >> lg columns are in flight at a time.
+__device__ __forceinline__ int lg_rows(int len) {
+    int lg = 0;
+    while ((1 << lg) < len && (1 << lg) < kLT) ++lg;
+    return lg;
+}
+
 // sym_swap (ldlt_kernel.cpp:333-345): positions p < q. Columns [c_lo, p)
 // swap rows p and q (c_lo = 0: all earlier columns).
 __device__ void sym_swap_wg(const FV &A, int p, int q, int32_t *rows, double *W, int wld, int w_cols, int c_lo) {
@@ -249,16 +258,15 @@ __device__ void elim1_wg(const FV &A, int k, int ucol_end, double *W, int wld, i
     const double inv_d = 1.0 / A.at(k, k);
     if (W)
         for (int i = k + 1 + tid; i < f; i += kLT) W[static_cast<int64_t>(w_col) * wld + i] = A.at(i, k);
-    const int nc = ucol_end - (k + 1);
-    if (nc > 0) {
-        const int64_t tot = static_cast<int64_t>(nc) * f;
-        for (int64_t idx = tid; idx < tot; idx += kLT) {
-            const int j = k + 1 + static_cast<int>(idx / f);
-            const int i = static_cast<int>(idx % f);
-            if (i < j) continue;
+    if (ucol_end > k + 1) {
+        const int lg = lg_rows(f - (k + 1));
+        for (int jb = k + 1; jb < ucol_end; jb += kLT >> lg) {
+            const int j = jb + (tid >> lg);
+            if (j >= ucol_end) continue;
             const double lj = __dmul_rn(A.at(j, k), inv_d);
             if (lj == 0.0) continue;
-            A.at(i, j) = __dsub_rn(A.at(i, j), __dmul_rn(A.at(i, k), lj));
+            for (int i = j + (tid & ((1 << lg) - 1)); i < f; i += 1 << lg)
+                A.at(i, j) = __dsub_rn(A.at(i, j), __dmul_rn(A.at(i, k), lj));
         }
     }
     __syncthreads();
@@ -282,19 +290,19 @@ __device__ void elim2_wg(const FV &A, int k, int ucol_end, double *W, int wld, i
             W[static_cast<int64_t>(w_col) * wld + i] = A.at(i, k);
             W[static_cast<int64_t>(w_col + 1) * wld + i] = A.at(i, k + 1);
         }
-    const int nc = ucol_end - (k + 2);
-    if (nc > 0) {
-        const int64_t tot = static_cast<int64_t>(nc) * f;
-        for (int64_t idx = tid; idx < tot; idx += kLT) {
-            const int j = k + 2 + static_cast<int>(idx / f);
-            const int i = static_cast<int>(idx % f);
-            if (i < j) continue;
+    if (ucol_end > k + 2) {
+        const int lg = lg_rows(f - (k + 2));
+        for (int jb = k + 2; jb < ucol_end; jb += kLT >> lg) {
+            const int j = jb + (tid >> lg);
+            if (j >= ucol_end) continue;
             const double a0 = A.at(j, k), a1 = A.at(j, k + 1);
             const double l0 = __dadd_rn(__dmul_rn(i00, a0), __dmul_rn(i10, a1));
             const double l1 = __dadd_rn(__dmul_rn(i10, a0), __dmul_rn(i11, a1));
             if (l0 == 0.0 && l1 == 0.0) continue;
-            const double t = __dadd_rn(__dmul_rn(A.at(i, k), l0), __dmul_rn(A.at(i, k + 1), l1));
-            A.at(i, j) = __dsub_rn(A.at(i, j), t);
+            for (int i = j + (tid & ((1 << lg) - 1)); i < f; i += 1 << lg) {
+                const double t = __dadd_rn(__dmul_rn(A.at(i, k), l0), __dmul_rn(A.at(i, k + 1), l1));
+                A.at(i, j) = __dsub_rn(A.at(i, j), t);
+            }
         }
     }
     __syncthreads();
@@ -438,15 +446,16 @@ __device__ void deferred_update_wg(const FV &A, int p0, int p1, int c0, int e, c
     const int m = e - c0;
     if (jb <= 0 || m <= 0) return;
     const int f = A.f;
-    const int rowsN = f - c0;
-    const int64_t tot = static_cast<int64_t>(m) * rowsN;
-    for (int64_t idx = threadIdx.x; idx < tot; idx += kLT) {
-        const int j = c0 + static_cast<int>(idx / rowsN);
-        const int i = c0 + static_cast<int>(idx % rowsN);
-        if (i < j) continue;
-        double sacc = 0.0;
-        for (int t = 0; t < jb; ++t) sacc += A.at(i, p0 + t) * W[static_cast<int64_t>(t) * wld + j];
-        A.at(i, j) -= sacc;
+    const int tid = threadIdx.x;
+    const int lg = lg_rows(f - c0);
+    for (int cb = c0; cb < e; cb += kLT >> lg) {
+        const int j = cb + (tid >> lg);
+        if (j >= e) continue;
+        for (int i = j + (tid & ((1 << lg) - 1)); i < f; i += 1 << lg) {
+            double sacc = 0.0;
+            for (int t = 0; t < jb; ++t) sacc += A.at(i, p0 + t) * W[static_cast<int64_t>(t) * wld + j];
+            A.at(i, j) -= sacc;
+        }
     }
     __syncthreads();
 }
@@ -1041,17 +1050,26 @@ void launch_ldlt_assemble(hipStream_t st, const LdltDev &d, const AsmTask *tasks
 
 // ---------------------------------------------------------------------------
 // Front epilogue (multifrontal_serial.cpp:892-945): a root's unaccepted
-// eligible columns become zero pivots (tag 0); the stored factor keeps D
-// on the diagonal, a 2x2 block's off-diagonal moved to the strict upper
-// slot (c, c+1), zeros elsewhere above the diagonal; D^-1 per pivot as
-// build_inline_dinv (solve_diag.cpp:49-123): kinds 1 / 2 / 0.
-__global__ __launch_bounds__(kLT) void k_ldlt_epilogue(LdltDev d, const int32_t *__restrict__ flist) {
+// eligible columns become zero pivots (tag 0); D^-1 per pivot as
+// build_inline_dinv (solve_diag.cpp:49-123): kinds 1 / 2 / 0. The stored
+// factor columns keep D on the diagonal, a 2x2 block's off-diagonal moved
+// to the strict upper slot (c, c+1), zeros elsewhere above the diagonal
+// and in a zero pivot's column; the square itself is left as it is (its
+// trailing part is the contribution block).
+__device__ __forceinline__ double stored_entry(const double *G, int f, int c, int i, int8_t tg) {
+    if (i < c) return (tg == -1 && i == c - 1) ? G[static_cast<int64_t>(c - 1) * f + c] : 0.0;
+    if (tg == 0 || (tg == 2 && i == c + 1)) return 0.0;
+    return G[static_cast<int64_t>(c) * f + i];
+}
+
+__global__ __launch_bounds__(kLT) void k_ldlt_epilogue(LdltDev d, const int32_t *__restrict__ flist,
+                                                       int32_t n_inline) {
     const int s = flist[blockIdx.x];
     FrontLd &F = d.fronts[s];
     const int f = F.f;
     int p = F.p;
     int8_t *tags = d.tags + F.rofs;
-    double *G = d.arena + F.fofs;
+    const double *G = d.arena + F.fofs;
     const int tid = threadIdx.x;
     const bool root = F.root != 0;
     int ps = p;
@@ -1069,10 +1087,9 @@ __global__ __launch_bounds__(kLT) void k_ldlt_epilogue(LdltDev d, const int32_t 
         F.delayed = root ? 0 : F.ne - p;
     }
     for (int c = tid; c < ps; c += kLT) {
-        double *Lc = G + static_cast<int64_t>(c) * f;
+        const double *Lc = G + static_cast<int64_t>(c) * f;
         const int8_t tg = tags[c];
         const double d00 = Lc[c];
-        // D^-1 entries
         double *dv = d.dinv + 3 * (F.dofs + c);
         int8_t *dk = d.dkind + F.dofs + c;
         if (tg == 2) {
@@ -1103,21 +1120,25 @@ __global__ __launch_bounds__(kLT) void k_ldlt_epilogue(LdltDev d, const int32_t 
             *dk = 1;
         }
     }
-    __syncthreads();
-    // stored layout (the 2x2 off-diagonal read at its first column, before
-    // that slot is cleared, and written at the second)
-    double d10 = 0.0;
-    for (int c = 0; c < ps; ++c) {
-        double *Lc = G + static_cast<int64_t>(c) * f;
+    if (static_cast<int>(blockIdx.x) >= n_inline) return;
+    double *O = d.fac + F.lofs;
+    const int64_t tot = static_cast<int64_t>(ps) * f;
+    for (int64_t idx = tid; idx < tot; idx += kLT) {
+        const int c = static_cast<int>(idx / f), i = static_cast<int>(idx - static_cast<int64_t>(c) * f);
+        O[idx] = stored_entry(G, f, c, i, tags[c]);
+    }
+}
+
+// Stored columns of large fronts: grid (column stride, fronts).
+__global__ __launch_bounds__(kLT) void k_ldlt_store(LdltDev d, const int32_t *__restrict__ flist) {
+    const FrontLd F = d.fronts[flist[blockIdx.y]];
+    const int f = F.f;
+    const int8_t *tags = d.tags + F.rofs;
+    const double *G = d.arena + F.fofs;
+    double *O = d.fac + F.lofs;
+    for (int c = blockIdx.x; c < F.pstore; c += gridDim.x) {
         const int8_t tg = tags[c];
-        if (tg == 2) d10 = Lc[c + 1];
-        __syncthreads();
-        if (tg == 0)
-            for (int i = c + tid; i < f; i += kLT) Lc[i] = 0.0;
-        if (tg == 2 && tid == 0) Lc[c + 1] = 0.0;
-        for (int i = tid; i < c; i += kLT) Lc[i] = 0.0;
-        __syncthreads();
-        if (tg == -1 && tid == 0) Lc[c - 1] = d10;
+        for (int i = threadIdx.x; i < f; i += kLT) O[static_cast<int64_t>(c) * f + i] = stored_entry(G, f, c, i, tg);
     }
 }
 
@@ -1141,9 +1162,14 @@ void launch_front_patch(hipStream_t st, FrontLd *fronts, const FrontLd *patch, c
     k_front_patch<<<dim3((np + 127) / 128), 128, 0, st>>>(fronts, patch, ids, np);
 }
 
-void launch_ldlt_epilogue(hipStream_t st, const LdltDev &d, const int32_t *flist, int32_t nf) {
+void launch_ldlt_epilogue(hipStream_t st, const LdltDev &d, const int32_t *flist, int32_t nf, int32_t n_inline) {
     if (nf <= 0) return;
-    k_ldlt_epilogue<<<dim3(nf), kLT, 0, st>>>(d, flist);
+    k_ldlt_epilogue<<<dim3(nf), kLT, 0, st>>>(d, flist, n_inline);
+}
+
+void launch_ldlt_store(hipStream_t st, const LdltDev &d, const int32_t *flist, int32_t nf, int32_t max_cols) {
+    if (nf <= 0 || max_cols <= 0) return;
+    k_ldlt_store<<<dim3(max_cols < 2048 ? max_cols : 2048, nf), kLT, 0, st>>>(d, flist);
 }
 
 // ---------------------------------------------------------------------------
@@ -1155,7 +1181,7 @@ __global__ __launch_bounds__(kLT) void k_ldlt_fwd(LdltDev d, const int32_t *__re
     const int s = flist[blockIdx.x];
     const FrontLd F = d.fronts[s];
     const int f = F.f, p = F.pstore;
-    const double *G = d.arena + F.fofs;
+    const double *G = d.fac + F.lofs;
     const int32_t *rows = d.rows + F.rofs;
     const int tid = threadIdx.x;
     // pivot rows are own rows: sequential column sweep within the pivot
@@ -1210,7 +1236,7 @@ __global__ __launch_bounds__(kLT) void k_ldlt_bwd(LdltDev d, const int32_t *__re
     const int s = flist[blockIdx.x];
     const FrontLd F = d.fronts[s];
     const int f = F.f, p = F.pstore;
-    const double *G = d.arena + F.fofs;
+    const double *G = d.fac + F.lofs;
     const int32_t *rows = d.rows + F.rofs;
     const int tid = threadIdx.x;
     for (int r = 0; r < nrhs; ++r) {

@@ -15,6 +15,7 @@ namespace smfgpu {
 void launch_front_gather(hipStream_t st, const FrontLd *fronts, FrontLd *out, const int32_t *ids, int32_t np);
 
 constexpr int kBlockedMinFront = 96;  // ldlt_kernel.hpp:15 (blocked_min_front)
+constexpr int64_t kStoreWide = int64_t{1} << 20;  // stored entries from which a front gets the store kernel
 
 namespace {
 
@@ -37,6 +38,7 @@ LdltDev ldlt_dev(smfgpu_plan_s *P) {
     LdltDev d{};
     d.fronts = P->d_fl.p;
     d.arena = P->d_arena.p;
+    d.fac = P->d_fac.p;
     d.rows = P->d_lrows.p;
     d.tags = P->d_ltags.p;
     d.work = P->d_work.p;
@@ -57,6 +59,26 @@ size_t stage(std::vector<unsigned char> &buf, const std::vector<T> &v) {
     buf.resize(off + v.size() * sizeof(T));
     if (!v.empty()) std::memcpy(buf.data() + off, v.data(), v.size() * sizeof(T));
     return off;
+}
+
+// Peak of the square pool over a factorization without delays: each level
+// takes its fronts' squares, then gives back its children's.
+int64_t square_peak(const smfgpu_plan_s *P) {
+    RangePool pool;
+    std::vector<int64_t> off(P->fronts.size(), 0);
+    for (const auto &fr : P->level_fronts) {
+        for (int32_t s : fr) {
+            const int64_t f = P->fronts[s].f;
+            off[s] = pool.take(f * f);
+        }
+        for (int32_t s : fr)
+            for (int32_t c = P->child_ptr[s]; c < P->child_ptr[s + 1]; ++c) {
+                const int32_t ch = P->child_idx[c];
+                const int64_t fc = P->fronts[ch].f;
+                pool.give(off[ch], fc * fc);
+            }
+    }
+    return pool.end;
 }
 
 }  // namespace
@@ -80,14 +102,17 @@ int ldlt_plan_build(smfgpu_plan_s *P, const smfgpu_analysis_desc *D) {
     P->lv_flist_ofs[nlev] = static_cast<int64_t>(lf.size());
     int rc;
     if ((rc = P->d_flist.upload(lf)) || (rc = P->d_fl.alloc(std::max(ns, 1)))) return rc;
-    // initial arena: the symbolic fronts' squares plus a quarter for delays
-    int64_t sq = 0, rw = 0;
+    // initial capacities: the pool's peak when the levels are laid out in
+    // order without delays, and the symbolic factor, each with an eighth
+    // more for delays
+    int64_t fac = 0, rw = 0;
     for (int32_t s = 0; s < ns; ++s) {
-        const int64_t f = P->fronts[s].f;
-        sq += f * f;
-        rw += f;
+        fac += static_cast<int64_t>(P->fronts[s].f) * P->fronts[s].p;
+        rw += P->fronts[s].f;
     }
-    if ((rc = P->d_arena.alloc(static_cast<size_t>(sq + sq / 4 + 1024))) ||
+    const int64_t sq_peak = square_peak(P);
+    if ((rc = P->d_arena.alloc(static_cast<size_t>(sq_peak + sq_peak / 8 + 1024))) ||
+        (rc = P->d_fac.alloc(static_cast<size_t>(fac + fac / 8 + 1024))) ||
         (rc = P->d_lrows.alloc(static_cast<size_t>(rw + rw / 4 + 1024))) ||
         (rc = P->d_ltags.alloc(static_cast<size_t>(rw + rw / 4 + 1024))) ||
         (rc = P->d_dinv.alloc(static_cast<size_t>(3 * (P->n + 1024)))) ||
@@ -163,13 +188,14 @@ int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgp
     const double small = opt.small_pivot;
     const double eff_small = (max_abs > 0.0 && max_abs < small) ? small * max_abs : small;
     const int32_t nb = std::max(2, opt.panel);
-    P->arena_used = P->rows_used = P->dinv_used = 0;
+    P->fac_used = P->rows_used = P->dinv_used = 0;
+    P->sq_pool.reset();
     std::vector<int32_t> pos(n, -1);
     std::vector<int32_t> cmap;
     std::vector<AsmTask> tasks;
     std::vector<ChildCol> cc;
     std::vector<std::vector<ChildCol>> bucket;
-    std::vector<int32_t> ids, small_list, blocked_list, big_list;
+    std::vector<int32_t> ids, small_list, blocked_list, big_list, ep_list, store_list;
     std::vector<BigLd> big_h;
     std::vector<FrontLd> patch;
     std::vector<int32_t> lrows;
@@ -190,6 +216,8 @@ int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgp
         small_list.clear();
         blocked_list.clear();
         big_list.clear();
+        ep_list.clear();
+        store_list.clear();
         const int64_t rows_start = P->rows_used;
         int64_t work_used = 0;
         for (int32_t s : fr) {
@@ -203,7 +231,8 @@ int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgp
             F.f = nd + Sy.f;
             F.ne = nd + Sy.p;
             F.root = P->sn_parent[s] < 0 ? 1 : 0;
-            F.fofs = P->arena_used;
+            F.fofs = P->sq_pool.take(static_cast<int64_t>(F.f) * F.f);
+            F.lofs = P->fac_used;
             F.rofs = P->rows_used;
             F.dofs = P->dinv_used;
             F.wofs = 0;
@@ -218,7 +247,10 @@ int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgp
             } else {
                 small_list.push_back(s);
             }
-            P->arena_used += static_cast<int64_t>(F.f) * F.f;
+            P->fac_used += static_cast<int64_t>(F.f) * F.ne;
+            // large stored blocks get the many-workgroup store kernel
+            if (static_cast<int64_t>(F.f) * F.ne >= kStoreWide) store_list.push_back(s);
+            else ep_list.push_back(s);
             P->rows_used += F.f;
             P->dinv_used += F.ne;
             // rows: children's delayed columns in child order, then the
@@ -271,7 +303,8 @@ int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgp
         }
         if (cc.size() > static_cast<size_t>(INT32_MAX)) return set_error(SMFGPU_ERR_ALLOC, "factor: level too large");
         // capacities (arena contents stay: offsets are relative)
-        if ((rc = ensure(P->d_arena, static_cast<size_t>(P->arena_used), true)) ||
+        if ((rc = ensure(P->d_arena, static_cast<size_t>(P->sq_pool.end), true)) ||
+            (rc = ensure(P->d_fac, static_cast<size_t>(P->fac_used), true)) ||
             (rc = ensure(P->d_lrows, static_cast<size_t>(P->rows_used), true)) ||
             (rc = ensure(P->d_ltags, static_cast<size_t>(P->rows_used), true)) ||
             (rc = ensure(P->d_dinv, static_cast<size_t>(3 * P->dinv_used), true)) ||
@@ -291,6 +324,11 @@ int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgp
         const size_t o_small = stage(P->h_stage, small_list);
         const size_t o_blk = stage(P->h_stage, blocked_list);
         const size_t o_fr = stage(P->h_stage, fr);
+        const int32_t n_inline = static_cast<int32_t>(ep_list.size());
+        int32_t max_store_cols = 0;
+        for (int32_t s : store_list) max_store_cols = std::max(max_store_cols, P->fl[s].ne);
+        ep_list.insert(ep_list.end(), store_list.begin(), store_list.end());
+        const size_t o_ep = stage(P->h_stage, ep_list);
         const size_t o_back = (P->h_stage.size() + 7) & ~size_t{7};
         P->h_stage.resize(o_back + fr.size() * sizeof(FrontLd));
         if ((rc = ensure(P->d_stage, P->h_stage.size(), false))) return rc;
@@ -306,12 +344,23 @@ int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgp
         LdltDev d = ldlt_dev(P);
         launch_ldlt_assemble(st, d, reinterpret_cast<const AsmTask *>(S + o_tasks), static_cast<int64_t>(tasks.size()),
                              reinterpret_cast<const ChildCol *>(S + o_cc));
+        // the children's squares are free for the launches that follow
+        for (int32_t s : fr)
+            for (int32_t c = P->child_ptr[s]; c < P->child_ptr[s + 1]; ++c) {
+                const FrontLd &C = P->fl[P->child_idx[c]];
+                P->sq_pool.give(C.fofs, static_cast<int64_t>(C.f) * C.f);
+            }
         launch_ldlt_small(st, d, reinterpret_cast<const int32_t *>(S + o_small), static_cast<int32_t>(small_list.size()),
                           opt.pivot_u, eff_small, small);
         launch_ldlt_blocked(st, d, reinterpret_cast<const int32_t *>(S + o_blk),
                             static_cast<int32_t>(blocked_list.size()), opt.pivot_u, eff_small, small, nb);
         if (!big_list.empty() && (rc = ldlt_big_fronts(P, d, big_list, opt, eff_small, small, nb, big_h))) return rc;
-        launch_ldlt_epilogue(st, d, reinterpret_cast<const int32_t *>(S + o_fr), static_cast<int32_t>(fr.size()));
+        launch_ldlt_epilogue(st, d, reinterpret_cast<const int32_t *>(S + o_ep), static_cast<int32_t>(ep_list.size()),
+                             n_inline);
+        launch_ldlt_store(st, d, reinterpret_cast<const int32_t *>(S + o_ep) + n_inline,
+                          static_cast<int32_t>(store_list.size()), max_store_cols);
+        for (int32_t s : fr)
+            if (P->fl[s].root) P->sq_pool.give(P->fl[s].fofs, static_cast<int64_t>(P->fl[s].f) * P->fl[s].f);
         launch_front_gather(st, P->d_fl.p, reinterpret_cast<FrontLd *>(S + o_back),
                             reinterpret_cast<const int32_t *>(S + o_fr), static_cast<int32_t>(fr.size()));
         HIP_OK(hipGetLastError());
@@ -424,7 +473,7 @@ int ldlt_export(smfgpu_plan_s *P, int64_t *sizes, int64_t *steps, double *fv, in
             steps[4 * s + 3] = F.f;
         }
         if (fv && F.pstore > 0)
-            HIP_OK(hipMemcpy(fv + fo, P->d_arena.p + F.fofs, static_cast<size_t>(F.f) * F.pstore * sizeof(double),
+            HIP_OK(hipMemcpy(fv + fo, P->d_fac.p + F.lofs, static_cast<size_t>(F.f) * F.pstore * sizeof(double),
                              hipMemcpyDeviceToHost));
         if (rows) std::memcpy(rows + ro, P->rows_h.data() + F.rofs, static_cast<size_t>(F.f) * sizeof(int32_t));
         if (tags && F.pstore > 0) HIP_OK(hipMemcpy(tags + to, P->d_ltags.p + F.rofs, F.pstore, hipMemcpyDeviceToHost));

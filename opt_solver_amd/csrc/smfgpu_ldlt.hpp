@@ -9,11 +9,12 @@
 namespace smfgpu {
 
 // Front s of the current factorization. Its f x f square (column-major,
-// leading dimension f, lower triangle) lives in the arena at fofs and stays
-// there: columns [0, pstore) are the stored factor (FactorKeep layout:
-// unit L, D on the diagonal, a 2x2 block's off-diagonal in slot (c, c+1)),
-// the trailing (f - p) square is the contribution block the parent
-// assembles, delayed rows first.
+// leading dimension f, lower triangle) lives in the arena at fofs from its
+// assembly until its parent has assembled the trailing (f - p) square, the
+// contribution block (delayed rows first). Columns [0, pstore) are copied
+// to the factor store at lofs (leading dimension f) in the FactorKeep
+// layout: unit L, D on the diagonal, a 2x2 block's off-diagonal in slot
+// (c, c+1), zeros elsewhere above the diagonal.
 struct FrontLd {
     int32_t f, ne, nd, psym;  // rows, eligible columns, delayed columns received, own pivots
     int32_t col0;             // first own pivot column (permuted numbering)
@@ -23,6 +24,7 @@ struct FrontLd {
     int32_t delayed, pad;
     double flops;
     int64_t fofs;             // arena offset of the square
+    int64_t lofs;             // factor-store offset of the stored columns
     int64_t rofs;             // offset of the row ids / tags
     int64_t wofs;             // offset of the panel workspace W (blocked fronts)
     int64_t dofs;             // offset of the pivots' D^-1 entries
@@ -31,7 +33,8 @@ struct FrontLd {
 
 struct LdltDev {
     FrontLd *fronts;          // per supernode
-    double *arena;
+    double *arena;            // front squares
+    double *fac;              // stored factor columns
     int32_t *rows;            // per front: f row ids (permuted numbering), pivoted order
     int8_t *tags;             // per front: pivot tags (1, 2, -1, 0), same offsets as rows
     double *work;
@@ -77,7 +80,11 @@ void launch_ldlt_small(hipStream_t st, const LdltDev &d, const int32_t *flist, i
 void launch_ldlt_blocked(hipStream_t st, const LdltDev &d, const int32_t *flist, int32_t nf, double u,
                          double eff_small, double small_pivot, int32_t nb);
 void launch_ldlt_assemble(hipStream_t st, const LdltDev &d, const AsmTask *tasks, int64_t ntasks, const ChildCol *cc);
-void launch_ldlt_epilogue(hipStream_t st, const LdltDev &d, const int32_t *flist, int32_t nf);
+// epilogue (pivot counts, D^-1) of the fronts in flist; it also stores the
+// factor columns of the first n_inline fronts, launch_ldlt_store those of
+// the rest (many workgroups per front)
+void launch_ldlt_epilogue(hipStream_t st, const LdltDev &d, const int32_t *flist, int32_t nf, int32_t n_inline);
+void launch_ldlt_store(hipStream_t st, const LdltDev &d, const int32_t *flist, int32_t nf, int32_t max_cols);
 void launch_ldlt_solve_level(hipStream_t st, const LdltDev &d, const int32_t *flist, int32_t nf, double *y, int32_t n,
                              int32_t nrhs, int32_t phase);
 void launch_front_patch(hipStream_t st, FrontLd *fronts, const FrontLd *patch, const int32_t *ids, int32_t np);

@@ -5,6 +5,7 @@
 #include "smfgpu_internal.hpp"
 #include "smfgpu_ldlt.hpp"
 
+#include <map>
 #include <string>
 #include <vector>
 
@@ -49,6 +50,80 @@ struct DBuf {
         return SMFGPU_OK;
     }
     size_t bytes() const { return n * sizeof(T); }
+};
+
+// Host-side best-fit allocator of offsets (in elements) into one device
+// buffer; freed ranges merge with free neighbours. The LDL^T driver takes
+// each front's square from it when the front's level is laid out and gives
+// it back once the parent has assembled the contribution block, so the
+// buffer holds the live squares only.
+struct RangePool {
+    static constexpr int64_t kAlign = 64;
+    std::map<int64_t, int64_t> by_off;               // free ranges: offset -> length
+    std::multimap<int64_t, int64_t> by_len;          // length -> offset
+    int64_t end = 0;                                 // high-water mark
+    void reset() {
+        by_off.clear();
+        by_len.clear();
+        end = 0;
+    }
+    static int64_t round(int64_t len) { return (len + kAlign - 1) / kAlign * kAlign; }
+    void erase_len(int64_t len, int64_t off) {
+        auto r = by_len.equal_range(len);
+        for (auto it = r.first; it != r.second; ++it)
+            if (it->second == off) {
+                by_len.erase(it);
+                return;
+            }
+    }
+    int64_t take(int64_t len) {
+        len = round(len);
+        auto it = by_len.lower_bound(len);
+        if (it == by_len.end()) {
+            // the last free range may touch the end: extend it
+            if (!by_off.empty()) {
+                auto last = std::prev(by_off.end());
+                if (last->first + last->second == end) {
+                    const int64_t off = last->first;
+                    erase_len(last->second, off);
+                    by_off.erase(last);
+                    end = off + len;
+                    return off;
+                }
+            }
+            const int64_t off = end;
+            end += len;
+            return off;
+        }
+        const int64_t flen = it->first, off = it->second;
+        by_len.erase(it);
+        by_off.erase(off);
+        if (flen > len) {
+            by_off[off + len] = flen - len;
+            by_len.emplace(flen - len, off + len);
+        }
+        return off;
+    }
+    void give(int64_t off, int64_t len) {
+        len = round(len);
+        auto nx = by_off.lower_bound(off);
+        if (nx != by_off.end() && off + len == nx->first) {
+            len += nx->second;
+            erase_len(nx->second, nx->first);
+            nx = by_off.erase(nx);
+        }
+        if (nx != by_off.begin()) {
+            auto pv = std::prev(nx);
+            if (pv->first + pv->second == off) {
+                off = pv->first;
+                len += pv->second;
+                erase_len(pv->second, pv->first);
+                by_off.erase(pv);
+            }
+        }
+        by_off[off] = len;
+        by_len.emplace(len, off);
+    }
 };
 
 constexpr int kPanelOuter = 512;  // default outer panel (tunable per plan)
@@ -130,11 +205,13 @@ struct smfgpu_plan_s {
     std::vector<int8_t> tags_h;
     DBuf<FrontLd> d_fl;
     DBuf<double> d_arena, d_work, d_dinv;
+    DBuf<double> d_fac;                             // stored factor columns (f x ne per front)
+    RangePool sq_pool;                              // live front squares in d_arena
     DBuf<int32_t> d_lrows, d_lcmap, d_flist;
     DBuf<int8_t> d_ltags, d_dkind;
     DBuf<unsigned char> d_stage;                    // per-level upload staging
     std::vector<unsigned char> h_stage;
-    int64_t arena_used = 0, rows_used = 0, dinv_used = 0;
+    int64_t fac_used = 0, rows_used = 0, dinv_used = 0;
     std::vector<int64_t> lv_flist_ofs;              // per level: offset of its fronts in d_flist (solve)
     bool ldlt_ready = false;                        // a factorization's fronts are valid
 };
