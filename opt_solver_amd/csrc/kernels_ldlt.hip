@@ -1173,35 +1173,9 @@ void launch_ldlt_store(hipStream_t st, const LdltDev &d, const int32_t *flist, i
 }
 
 // ---------------------------------------------------------------------------
-// Solves of an LDL^T factor, one workgroup per front, level by level.
-// Forward: unit L over the stored pivots (rows through the front's row
-// ids), column by column in 64-wide blocks; ancestor rows by atomic add.
-__global__ __launch_bounds__(kLT) void k_ldlt_fwd(LdltDev d, const int32_t *__restrict__ flist, double *__restrict__ y,
-                                                 int32_t n, int32_t nrhs) {
-    const int s = flist[blockIdx.x];
-    const FrontLd F = d.fronts[s];
-    const int f = F.f, p = F.pstore;
-    const double *G = d.fac + F.lofs;
-    const int32_t *rows = d.rows + F.rofs;
-    const int tid = threadIdx.x;
-    // pivot rows are own rows: sequential column sweep within the pivot
-    // block, the rows below the pivots updated once per column block
-    for (int r = 0; r < nrhs; ++r) {
-        double *yr = y + static_cast<int64_t>(r) * n;
-        for (int c = 0; c < p; ++c) {
-            __syncthreads();
-            const double xc = yr[rows[c]];
-            for (int i = c + 1 + tid; i < f; i += kLT) {
-                const double l = G[static_cast<int64_t>(c) * f + i];
-                if (l == 0.0) continue;
-                if (i < p) yr[rows[i]] -= l * xc;
-                else atomicAdd(&yr[rows[i]], -l * xc);
-            }
-            __threadfence_block();
-        }
-    }
-}
-
+// D^-1 step of the LDL^T solve (apply_dinv, solve_backward.cpp:156-173): per pivot,
+// kind 1 scales, kind 2 applies the 2x2 inverse, kind 0 (second column of a
+// 2x2) is skipped. The L sweeps are in kernels_solve.hip.
 __global__ void k_ldlt_diag(LdltDev d, const int32_t *__restrict__ flist, int32_t nf, double *__restrict__ y,
                             int32_t n, int32_t nrhs) {
     const int fi = blockIdx.x;
@@ -1224,46 +1198,10 @@ __global__ void k_ldlt_diag(LdltDev d, const int32_t *__restrict__ flist, int32_
     }
 }
 
-__device__ __forceinline__ double wsum(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    return v;
-}
-
-__global__ __launch_bounds__(kLT) void k_ldlt_bwd(LdltDev d, const int32_t *__restrict__ flist, double *__restrict__ y,
-                                                 int32_t n, int32_t nrhs) {
-    __shared__ double part[kLT / 64];
-    const int s = flist[blockIdx.x];
-    const FrontLd F = d.fronts[s];
-    const int f = F.f, p = F.pstore;
-    const double *G = d.fac + F.lofs;
-    const int32_t *rows = d.rows + F.rofs;
-    const int tid = threadIdx.x;
-    for (int r = 0; r < nrhs; ++r) {
-        double *yr = y + static_cast<int64_t>(r) * n;
-        for (int c = p - 1; c >= 0; --c) {
-            double acc = 0.0;
-            for (int i = c + 1 + tid; i < f; i += kLT) acc += G[static_cast<int64_t>(c) * f + i] * yr[rows[i]];
-            acc = wsum(acc);
-            if ((tid & 63) == 0) part[tid >> 6] = acc;
-            __syncthreads();
-            if (tid == 0) {
-                double t = 0.0;
-                for (int w = 0; w < kLT / 64; ++w) t += part[w];
-                yr[rows[c]] -= t;
-            }
-            __threadfence_block();
-            __syncthreads();
-        }
-    }
-}
-
-void launch_ldlt_solve_level(hipStream_t st, const LdltDev &d, const int32_t *flist, int32_t nf, double *y, int32_t n,
-                             int32_t nrhs, int32_t phase) {
+void launch_ldlt_diag(hipStream_t st, const LdltDev &d, const int32_t *flist, int32_t nf, double *y, int32_t n,
+                      int32_t nrhs) {
     if (nf <= 0) return;
-    if (phase == 0) k_ldlt_fwd<<<dim3(nf), kLT, 0, st>>>(d, flist, y, n, nrhs);
-    else if (phase == 1) k_ldlt_diag<<<dim3(nf), 64, 0, st>>>(d, flist, nf, y, n, nrhs);
-    else k_ldlt_bwd<<<dim3(nf), kLT, 0, st>>>(d, flist, y, n, nrhs);
+    k_ldlt_diag<<<dim3(nf), 64, 0, st>>>(d, flist, nf, y, n, nrhs);
 }
 
 }  // namespace smfgpu

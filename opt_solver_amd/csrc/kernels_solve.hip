@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include "smfgpu_internal.hpp"
+#include "smfgpu_ldlt.hpp"
 
 namespace smfgpu {
 namespace {
@@ -329,7 +330,159 @@ __global__ __launch_bounds__(kSThreads) void k_bwd_step(DevPtrs d, const SolveIt
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// Sweeps of an LDL^T factor (unit L with D on the stored diagonal; the D^-1
+// step runs between them): the same step scheme with 64-pivot blocks and no
+// stored inverses. A triangle item takes the update of its own pivots by the
+// block solved in the step before, loads its unit triangle into LDS and
+// substitutes in one wave. Items: forward triangle {k, kb: previous block;
+// tk, tkb: this block}, forward chunk {rows [a0, a1), update by [k, k+kb)},
+// backward chunk {columns [a0, a1), contributions of rows [k, k+kb); flags
+// bit 0: atomic}, backward triangle {k, kb: block solved before; tk, tkb}.
+constexpr int kLB = 64;
+constexpr int kLThreads = 256;
+constexpr int kLSeg = 512;  // rows of a backward tail item
+constexpr int32_t kItemTri = 2;
+
+__device__ __forceinline__ double lwave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// sum_{c < kb} L(i, k + c) xs[c] for row i = r0 + (tid & 63) (< r1), the
+// four waves taking 16 columns each; the total lands in part[0][row].
+__device__ __forceinline__ void rows_times_block(const double *L, int f, int k, int kb, int r0, int r1,
+                                                 const double *xs, double (*part)[kLB]) {
+    const int row = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i = r0 + row;
+    double acc = 0.0;
+    if (i < r1) {
+        const int c1 = min(kb, w * 16 + 16);
+        for (int c = w * 16; c < c1; ++c) acc += L[static_cast<int64_t>(k + c) * f + i] * xs[c];
+    }
+    part[w][row] = acc;
+    __syncthreads();
+    if (w == 0) part[0][row] = (part[0][row] + part[1][row]) + (part[2][row] + part[3][row]);
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kLThreads) void k_ldlt_fwd_step(LdltDev d, const SolveItem *__restrict__ items,
+                                                            double *__restrict__ y, int32_t n, int32_t nrhs) {
+    __shared__ double xs[kLB];
+    __shared__ double part[4][kLB];
+    __shared__ double T[kLB][kLB + 1];  // T[c][l] = L(tk + l, tk + c)
+    const SolveItem it = items[blockIdx.x];
+    const FrontLd F = d.fronts[it.s];
+    const int f = F.f, p = F.pstore;
+    const double *L = d.fac + F.lofs;
+    const int32_t *rows = d.rows + F.rofs;
+    const int tid = threadIdx.x;
+    const bool tri = (it.flags & kItemTri) != 0;
+    if (tri) {
+        for (int e = tid; e < kLB * kLB; e += kLThreads) {
+            const int c = e >> 6, l = e & 63;
+            T[c][l] = (c < it.tkb && l < it.tkb && l > c) ? L[static_cast<int64_t>(it.tk + c) * f + it.tk + l] : 0.0;
+        }
+    }
+    const int r0 = tri ? it.tk : it.a0, r1 = tri ? it.tk + it.tkb : it.a1;
+    for (int r = 0; r < nrhs; ++r) {
+        double *yr = y + static_cast<int64_t>(r) * n;
+        if (tid < it.kb) xs[tid] = yr[rows[it.k + tid]];
+        __syncthreads();
+        if (it.kb > 0) rows_times_block(L, f, it.k, it.kb, r0, r1, xs, part);
+        else {
+            if (tid < kLB) part[0][tid] = 0.0;
+            __syncthreads();
+        }
+        if (tri) {
+            if (tid < 64) {
+                const int l = tid;
+                double yv = l < it.tkb ? yr[rows[it.tk + l]] - part[0][l] : 0.0;
+                for (int c = 0; c < it.tkb; ++c) {
+                    const double xc = __shfl(yv, c, 64);
+                    if (l > c) yv -= T[c][l] * xc;
+                }
+                if (l < it.tkb) yr[rows[it.tk + l]] = yv;
+            }
+        } else if (tid < 64) {
+            const int i = r0 + tid;
+            if (i < r1) {
+                if (i >= p) atomicAdd(&yr[rows[i]], -part[0][tid]);
+                else yr[rows[i]] -= part[0][tid];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kLThreads) void k_ldlt_bwd_step(LdltDev d, const SolveItem *__restrict__ items,
+                                                            double *__restrict__ y, int32_t n, int32_t nrhs) {
+    __shared__ double xs[kLSeg];
+    __shared__ double upd[kLB];
+    __shared__ double T[kLB][kLB + 1];  // T[c][l] = L(tk + l, tk + c)
+    const SolveItem it = items[blockIdx.x];
+    const FrontLd F = d.fronts[it.s];
+    const int f = F.f;
+    const double *L = d.fac + F.lofs;
+    const int32_t *rows = d.rows + F.rofs;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const bool tri = (it.flags & kItemTri) != 0;
+    if (tri) {
+        for (int e = tid; e < kLB * kLB; e += kLThreads) {
+            const int c = e >> 6, l = e & 63;
+            T[c][l] = (c < it.tkb && l < it.tkb && l > c) ? L[static_cast<int64_t>(it.tk + c) * f + it.tk + l] : 0.0;
+        }
+    }
+    const int c0 = tri ? it.tk : it.a0, c1 = tri ? it.tk + it.tkb : it.a1;
+    for (int r = 0; r < nrhs; ++r) {
+        double *yr = y + static_cast<int64_t>(r) * n;
+        for (int t = tid; t < it.kb; t += kLThreads) xs[t] = yr[rows[it.k + t]];
+        if (tid < kLB) upd[tid] = 0.0;
+        __syncthreads();
+        // contributions of rows [k, k + kb) to the columns, one wave per
+        // 16 columns
+        for (int j = 0; j < 16; ++j) {
+            const int c = c0 + w * 16 + j;
+            if (c >= c1) break;
+            const double *Lc = L + static_cast<int64_t>(c) * f + it.k;
+            double acc = 0.0;
+            for (int t = lane; t < it.kb; t += 64) acc += Lc[t] * xs[t];
+            acc = lwave_sum(acc);
+            if (lane == 0) {
+                if (tri) upd[c - c0] = acc;
+                else if (it.flags & 1) atomicAdd(&yr[rows[c]], -acc);
+                else yr[rows[c]] -= acc;
+            }
+        }
+        __syncthreads();
+        if (tri && tid < 64) {
+            const int l = tid;
+            double yv = l < it.tkb ? yr[rows[it.tk + l]] - upd[l] : 0.0;
+            for (int c = it.tkb - 1; c >= 0; --c) {
+                const double xc = __shfl(yv, c, 64);
+                if (l < c) yv -= T[l][c] * xc;
+            }
+            if (l < it.tkb) yr[rows[it.tk + l]] = yv;
+        }
+        __syncthreads();
+    }
+}
+
 }  // namespace
+
+void launch_ldlt_fwd_step(hipStream_t st, const LdltDev &d, const SolveItem *items, int64_t count, double *y,
+                          int32_t n, int32_t nrhs) {
+    if (count <= 0) return;
+    k_ldlt_fwd_step<<<dim3(static_cast<unsigned>(count)), kLThreads, 0, st>>>(d, items, y, n, nrhs);
+}
+
+void launch_ldlt_bwd_step(hipStream_t st, const LdltDev &d, const SolveItem *items, int64_t count, double *y,
+                          int32_t n, int32_t nrhs) {
+    if (count <= 0) return;
+    k_ldlt_bwd_step<<<dim3(static_cast<unsigned>(count)), kLThreads, 0, st>>>(d, items, y, n, nrhs);
+}
 
 void launch_fwd_step(hipStream_t st, const DevPtrs &d, const SolveItem *items, int64_t count, double *y, int32_t n,
                      int32_t nrhs) {

@@ -168,6 +168,93 @@ static int ldlt_big_fronts(smfgpu_plan_s *P, const LdltDev &d, const std::vector
     return SMFGPU_OK;
 }
 
+// Step lists of the blocked sweeps from the realized fronts (pivots per
+// front known after the factorization): 64-pivot blocks; forward steps
+// solve block t after block t-1's update of its rows, while chunk items
+// apply that update to the rest of the front (64-row chunks, rows past the
+// pivots by atomic add); backward steps first take the contributions of
+// the rows past the pivots (512-row segments, atomic), then solve the
+// blocks last to first, chunk items applying each solved block to the
+// columns before it. One launch per step covers every front of a level.
+static int ldlt_solve_lists(smfgpu_plan_s *P) {
+    constexpr int32_t B = 64, Seg = 512, Tri = 2;
+    const int32_t nlev = static_cast<int32_t>(P->level_fronts.size());
+    std::vector<SolveItem> fwd_items, bwd_items;
+    P->fwd_steps.clear();
+    P->bwd_steps.clear();
+    P->fwd_lv.assign(nlev + 1, 0);
+    P->bwd_lv.assign(nlev + 1, 0);
+    std::vector<std::vector<SolveItem>> fst, bst;
+    auto put = [](std::vector<std::vector<SolveItem>> &v, size_t t, const SolveItem &it) {
+        if (v.size() <= t) v.resize(t + 1);
+        v[t].push_back(it);
+    };
+    auto flush = [](std::vector<std::vector<SolveItem>> &stp, std::vector<SolveItem> &items,
+                    std::vector<std::pair<int64_t, int64_t>> &steps) {
+        for (auto &v : stp) {
+            if (v.empty()) continue;
+            steps.emplace_back(static_cast<int64_t>(items.size()), static_cast<int64_t>(v.size()));
+            items.insert(items.end(), v.begin(), v.end());
+        }
+        stp.clear();
+    };
+    for (int32_t lv = 0; lv < nlev; ++lv) {
+        P->fwd_lv[lv] = static_cast<int32_t>(P->fwd_steps.size());
+        P->bwd_lv[lv] = static_cast<int32_t>(P->bwd_steps.size());
+        for (int32_t s : P->level_fronts[lv]) {
+            const FrontLd &F = P->fl[s];
+            const int32_t f = F.f, p = F.pstore;
+            if (p <= 0) continue;
+            const int32_t nb = (p + B - 1) / B;
+            for (int32_t t = 0; t < nb; ++t) {
+                const int32_t tk = t * B, tkb = std::min(B, p - tk);
+                const int32_t k = t > 0 ? tk - B : 0, kb = t > 0 ? B : 0;
+                put(fst, t, SolveItem{s, k, kb, 0, 0, tk, tkb, Tri});
+                if (t > 0)
+                    for (int32_t r = tk + tkb; r < f; r += B) put(fst, t, SolveItem{s, k, kb, r, std::min(r + B, f), 0, 0, 0});
+            }
+            if (f > p) {
+                const int32_t k = (nb - 1) * B;
+                for (int32_t r = p; r < f; r += B) put(fst, nb, SolveItem{s, k, p - k, r, std::min(r + B, f), 0, 0, 0});
+            }
+            size_t t = 0;
+            if (f > p) {
+                for (int32_t r = p; r < f; r += Seg)
+                    for (int32_t c = 0; c < p; c += B)
+                        put(bst, 0, SolveItem{s, r, std::min(Seg, f - r), c, std::min(c + B, p), 0, 0, 1});
+                t = 1;
+            }
+            const int32_t kL = (nb - 1) * B;
+            put(bst, t++, SolveItem{s, 0, 0, 0, 0, kL, p - kL, Tri});
+            for (int32_t j = nb - 2; j >= 0; --j, ++t) {
+                const int32_t tk = j * B, k = tk + B, kb = std::min(B, p - k);
+                put(bst, t, SolveItem{s, k, kb, 0, 0, tk, B, Tri});
+                for (int32_t c = 0; c < tk; c += B) put(bst, t, SolveItem{s, k, kb, c, c + B, 0, 0, 0});
+            }
+        }
+        flush(fst, fwd_items, P->fwd_steps);
+        flush(bst, bwd_items, P->bwd_steps);
+    }
+    P->fwd_lv[nlev] = static_cast<int32_t>(P->fwd_steps.size());
+    P->bwd_lv[nlev] = static_cast<int32_t>(P->bwd_steps.size());
+    for (const auto &st : P->fwd_steps)
+        if (st.second > INT32_MAX) return set_error(SMFGPU_ERR_ALLOC, "factor: solve step too large");
+    for (const auto &st : P->bwd_steps)
+        if (st.second > INT32_MAX) return set_error(SMFGPU_ERR_ALLOC, "factor: solve step too large");
+    int rc;
+    if ((rc = ensure(P->d_fwd_items, std::max<size_t>(fwd_items.size(), 1), false)) ||
+        (rc = ensure(P->d_bwd_items, std::max<size_t>(bwd_items.size(), 1), false)))
+        return rc;
+    if (!fwd_items.empty())
+        HIP_OK(hipMemcpyAsync(P->d_fwd_items.p, fwd_items.data(), fwd_items.size() * sizeof(SolveItem),
+                              hipMemcpyHostToDevice, P->stream));
+    if (!bwd_items.empty())
+        HIP_OK(hipMemcpyAsync(P->d_bwd_items.p, bwd_items.data(), bwd_items.size() * sizeof(SolveItem),
+                              hipMemcpyHostToDevice, P->stream));
+    HIP_OK(hipStreamSynchronize(P->stream));
+    return SMFGPU_OK;
+}
+
 int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgpu_factor_options &opt,
                 smfgpu_factor_info &fi) {
     hipStream_t st = P->stream;
@@ -415,6 +502,7 @@ int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgp
         return SMFGPU_OK;
     }
     fi.status = fi.num_zero > 0 ? SMFGPU_STATUS_SINGULAR : SMFGPU_STATUS_SUCCESS;
+    if ((rc = ldlt_solve_lists(P))) return rc;
     P->ldlt_ready = true;
     return SMFGPU_OK;
 }
@@ -424,6 +512,14 @@ int ldlt_solve(smfgpu_plan_s *P, const double *b_dev, int64_t ldb, double *out_d
     const int32_t n = P->n;
     LdltDev d = ldlt_dev(P);
     const int32_t nl = static_cast<int32_t>(P->level_fronts.size());
+    const bool tm = P->timing;
+    if (tm && P->sev.size() < 3) {
+        for (size_t i = P->sev.size(); i < 3; ++i) {
+            hipEvent_t e;
+            HIP_OK(hipEventCreate(&e));
+            P->sev.push_back(e);
+        }
+    }
     for (int32_t r0 = 0; r0 < nrhs; r0 += 16) {
         const int32_t nr = std::min(16, nrhs - r0);
         if (P->d_y.n < static_cast<size_t>(n) * nr) {
@@ -431,15 +527,28 @@ int ldlt_solve(smfgpu_plan_s *P, const double *b_dev, int64_t ldb, double *out_d
             if (rc) return rc;
         }
         launch_gather_rhs(st, P->d_perm.p, b_dev + r0 * ldb, ldb, P->d_y.p, n, nr);
+        if (tm) HIP_OK(hipEventRecord(P->sev[0], st));
         for (int32_t lv = 0; lv < nl; ++lv)
-            launch_ldlt_solve_level(st, d, P->d_flist.p + P->lv_flist_ofs[lv],
-                                    static_cast<int32_t>(P->lv_flist_ofs[lv + 1] - P->lv_flist_ofs[lv]), P->d_y.p, n,
-                                    nr, 0);
-        launch_ldlt_solve_level(st, d, P->d_flist.p, static_cast<int32_t>(P->lv_flist_ofs[nl]), P->d_y.p, n, nr, 1);
+            for (int32_t i = P->fwd_lv[lv]; i < P->fwd_lv[lv + 1]; ++i)
+                launch_ldlt_fwd_step(st, d, P->d_fwd_items.p + P->fwd_steps[i].first, P->fwd_steps[i].second,
+                                     P->d_y.p, n, nr);
+        launch_ldlt_diag(st, d, P->d_flist.p, static_cast<int32_t>(P->lv_flist_ofs[nl]), P->d_y.p, n, nr);
+        if (tm) HIP_OK(hipEventRecord(P->sev[1], st));
         for (int32_t lv = nl - 1; lv >= 0; --lv)
-            launch_ldlt_solve_level(st, d, P->d_flist.p + P->lv_flist_ofs[lv],
-                                    static_cast<int32_t>(P->lv_flist_ofs[lv + 1] - P->lv_flist_ofs[lv]), P->d_y.p, n,
-                                    nr, 2);
+            for (int32_t i = P->bwd_lv[lv]; i < P->bwd_lv[lv + 1]; ++i)
+                launch_ldlt_bwd_step(st, d, P->d_bwd_items.p + P->bwd_steps[i].first, P->bwd_steps[i].second,
+                                     P->d_y.p, n, nr);
+        if (tm) {
+            HIP_OK(hipEventRecord(P->sev[2], st));
+            HIP_OK(hipStreamSynchronize(st));
+            float ms;
+            HIP_OK(hipEventElapsedTime(&ms, P->sev[0], P->sev[1]));
+            P->t_solve_phase[0] = ms;
+            HIP_OK(hipEventElapsedTime(&ms, P->sev[1], P->sev[2]));
+            P->t_solve_phase[1] = ms;
+            P->t_solve_phase[2] = static_cast<double>(P->fwd_steps.size());
+            P->t_solve_phase[3] = static_cast<double>(P->bwd_steps.size());
+        }
         launch_scatter_rhs(st, P->d_perm.p, P->d_y.p, out_dev + r0 * ldo, ldo, n, nr);
     }
     HIP_OK(hipGetLastError());

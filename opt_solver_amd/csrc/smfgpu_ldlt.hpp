@@ -85,9 +85,15 @@ void launch_ldlt_assemble(hipStream_t st, const LdltDev &d, const AsmTask *tasks
 // the rest (many workgroups per front)
 void launch_ldlt_epilogue(hipStream_t st, const LdltDev &d, const int32_t *flist, int32_t nf, int32_t n_inline);
 void launch_ldlt_store(hipStream_t st, const LdltDev &d, const int32_t *flist, int32_t nf, int32_t max_cols);
-void launch_ldlt_solve_level(hipStream_t st, const LdltDev &d, const int32_t *flist, int32_t nf, double *y, int32_t n,
-                             int32_t nrhs, int32_t phase);
+void launch_ldlt_diag(hipStream_t st, const LdltDev &d, const int32_t *flist, int32_t nf, double *y, int32_t n,
+                      int32_t nrhs);
 void launch_front_patch(hipStream_t st, FrontLd *fronts, const FrontLd *patch, const int32_t *ids, int32_t np);
+// one step of the blocked forward / backward sweep (items of all fronts of
+// a level; SolveItem flags bit 1: triangle item)
+void launch_ldlt_fwd_step(hipStream_t st, const LdltDev &d, const SolveItem *items, int64_t count, double *y,
+                          int32_t n, int32_t nrhs);
+void launch_ldlt_bwd_step(hipStream_t st, const LdltDev &d, const SolveItem *items, int64_t count, double *y,
+                          int32_t n, int32_t nrhs);
 void launch_big_panel(hipStream_t st, const LdltDev &d, BigLd *states, int32_t nbig, double u, double eff_small,
                       double small_pivot, int32_t nb);
 void launch_big_update(hipStream_t st, const LdltDev &d, const BigLd *states, int32_t nbig, int32_t per_front);

@@ -197,3 +197,30 @@ def test_large_front_default_threshold(oracle_lib):
     xs = fac.solve(b)
     eta, _ = oracle_lib.backward_error(A, b, xs)
     assert eta <= 1e-13, eta
+
+
+@pytest.mark.parametrize("case", ["ocp_T200", "kkt_grid_60"])
+def test_blocked_solve_multi_rhs(oracle_lib, case):
+    """The blocked LDL^T sweeps (64-pivot steps, the rows past a front's
+    pivots by atomic add) with 1, 5, 16 and 21 right-hand sides (21: two
+    passes of 16): against the known solutions, the reference backward-error
+    formula, and column by column against the single-RHS solve. ocp_T200
+    has delayed and 2x2 pivots, kkt_grid_60 fronts of several blocks."""
+    print("case blocked solve", case, flush=True)
+    A, arrays, _ = _large_path_case(case)
+    plan = Plan(arrays, posdef=False)
+    fac = plan.factor(A[3], u=0.01)
+    assert fac.status == 0
+    print(f"  delays {fac.info.delayed_pivots} 2x2 {fac.info.two_by_two_pivots}", flush=True)
+    if case == "ocp_T200":
+        assert fac.info.delayed_pivots > 0 and fac.info.two_by_two_pivots > 0
+    X = np.random.default_rng(11).uniform(-1, 1, (A[0], 21))
+    B = matvec(A, X)
+    for nr in (1, 5, 16, 21):
+        XS = fac.solve(B[:, :nr] if nr > 1 else B[:, 0]).reshape(A[0], nr)
+        eta, _ = oracle_lib.backward_error(A, B[:, :nr], XS)
+        assert eta <= 1e-14, (nr, eta)
+        assert np.abs(XS - X[:, :nr]).max() < 1e-8, nr
+        if nr == 21:
+            for r in (0, 15, 16, 20):
+                assert np.abs(fac.solve(B[:, r]) - XS[:, r]).max() <= 1e-12 * max(1.0, np.abs(XS[:, r]).max())
