@@ -217,6 +217,12 @@ SMFGPU_API int smfgpu_refine(smfgpu_factor_t f, const double *values, const doub
  * (default 2048, minimum 96) run the blocked algorithm over many
  * workgroups (pivot logic per panel in one workgroup, MFMA updates). */
 #define SMFGPU_TUNE_LDLT_BIG_MIN 5
+/* SMFGPU_TUNE_LDLT_GROUP: 1 (default) lets the large-front LDL^T path try
+ * groups of four first-pass panels at once (accepted when every pivot
+ * passes the first threshold test, otherwise the panel is redone with the
+ * full search; the same pivots either way), 0 factors one panel at a time.
+ * Groups need the panel width 16. */
+#define SMFGPU_TUNE_LDLT_GROUP 6
 SMFGPU_API int smfgpu_plan_tune(smfgpu_plan_t plan, int32_t key, int64_t value);
 
 /* Device time (ms) of the last factorization's dominant kernel family:

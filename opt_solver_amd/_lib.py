@@ -63,6 +63,7 @@ TUNE_SYRK_AHEAD = 2
 TUNE_PANEL_OUTER = 3
 TUNE_CB_SEG = 4
 TUNE_LDLT_BIG_MIN = 5
+TUNE_LDLT_GROUP = 6
 
 _lib = None
 

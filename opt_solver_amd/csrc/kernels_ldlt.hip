@@ -676,7 +676,279 @@ __device__ __forceinline__ int big_tiles(int f, int c0, int c1) {
     return nc * nr - nc * (nc - 1) / 2;
 }
 
-__global__ __launch_bounds__(kLT) void k_big_panel(LdltDev d, BigLd *states, Prm prm, int32_t nb) {
+// ---------------------------------------------------------------------------
+// Speculative panel groups (first pass of the large-front path). When every
+// pivot of a panel passes select_pivot's first test (the diagonal against u
+// times the largest entry of its column, ldlt_kernel.cpp:145-178), the
+// panel's pivots are 1x1, in order, without swaps, and each row of the front
+// goes through a fixed sequence of row-local operations: eliminate_1x1's
+// rank-1 updates within the panel (ldlt_kernel.cpp:350-432) and the deferred
+// update of the later columns (705-761). A group of up to four panels of 16
+// is evaluated under that assumption. The stats launch factors the group's
+// diagonal region in one wave of every workgroup (the multipliers and the
+// unscaled columns go to LDS), lets every thread carry one row below it
+// through the same operations, and records per pivot the largest entry of
+// its column. The panel kernel then applies the first test to each pivot in
+// order and accepts the leading panels that pass; the apply launch redoes the
+// accepted panels and writes their columns and W. A panel that fails is
+// factored by the panel kernel's full search. Per row, the operations and
+// their order are those of the one-panel path, so are the accepted values
+// and the pivot decisions.
+// Guarded loads are unconditional loads from clamped (in-front) addresses
+// times a 0 / 1 mask, so the compiler keeps every load of a row in flight
+// (inputs are finite: the factorization rejects non-finite values).
+__device__ __forceinline__ double masked(double v, bool keep) { return v * (keep ? 1.0 : 0.0); }
+
+__device__ __forceinline__ double rdlane(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane(static_cast<int>(b), l);
+    const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), l);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+
+// Columns of the group the one-panel path would factor next if no panel
+// stalled: panels of nb from k, each ending at min(live, start + nb), the
+// group ending after the panel whose end triggers the outer block's flush.
+// Every panel but the last has 16 columns.
+__device__ __forceinline__ int group_cols(const BigLd &B, int nb) {
+    if (B.pass != 0 || !B.spec || nb != 16 || B.k >= B.live) return 0;
+    int pos = B.k;
+    for (int m = 0; m < 4 && pos < B.live; ++m) {
+        const int pl = min(B.live, pos + nb);
+        if (pl > B.oe) break;
+        pos = pl;
+        if (pl + nb > B.oe || pl >= B.live) break;
+    }
+    return pos - B.k;
+}
+
+// Group buffer per front: [0, 64) largest entry per pivot column over the
+// diagonal region, [64, 128) the pivots, then the region's entries before
+// the group (64 x 64, column-major: the apply launch reads them there,
+// since its workgroup 0 rewrites the region in the front), then 64 column
+// maxima per workgroup.
+constexpr int64_t kGDiag = 2 * kGroupCols;
+constexpr int64_t kGMax = kGDiag + kGroupCols * kGroupCols;
+__device__ __forceinline__ int64_t gpart_stride(int nch) { return kGMax + static_cast<int64_t>(nch) * kGroupCols; }
+
+// Per-panel steps of a group (panel M: columns [16 M, 16 M + 16) of the
+// group), templated so that every index into a row's registers is a
+// constant. Diagonal region: lane r of wave 0 holds row ps + r.
+template <int M>
+__device__ __forceinline__ void gdiag_pivots(double (&a)[kGroupCols], int r, int ncols,
+                                             double (*Wd)[kGroupCols + 1], double (*Lm)[16], double *invd,
+                                             double *dd) {
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int kk = 16 * M + q;
+        if (kk < ncols) {
+            const double v = a[kk];
+            const double dk = rdlane(v, kk);
+            const double inv = 1.0 / dk;
+            if (r == kk) {
+                dd[kk] = dk;
+                invd[kk] = inv;
+            }
+            if (r > kk) Wd[kk][r] = v;
+#pragma unroll
+            for (int jq = q + 1; jq < 16; ++jq) {
+                const int j = 16 * M + jq;
+                if (j < ncols) {
+                    const double lj = __dmul_rn(rdlane(v, j), inv);
+                    if (r == 0) Lm[kk][jq] = lj;
+                    if (lj != 0.0 && r >= j) a[j] = __dsub_rn(a[j], __dmul_rn(v, lj));
+                }
+            }
+            if (r > kk) a[kk] = __dmul_rn(v, inv);
+        }
+    }
+}
+
+// Rows below the region (one per thread): the same pivots with the
+// multipliers from LDS; STATS keeps |entry| per pivot in red[q][tid].
+template <int M, bool STATS>
+__device__ __forceinline__ void grow_pivots(double (&a)[kGroupCols], int ncols, const double (*Lm)[16],
+                                            const double *invd, double (*red)[kLT + 1], double *W, int64_t wcol0,
+                                            int f, int i, bool rowok) {
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int kk = 16 * M + q;
+        if (kk < ncols) {
+            const double v = a[kk];
+            if (STATS) red[q][threadIdx.x] = fabs(v);
+            else if (rowok) W[(wcol0 + kk) * f + i] = v;
+#pragma unroll
+            for (int jq = q + 1; jq < 16; ++jq) {
+                const int j = 16 * M + jq;
+                if (j < ncols) {
+                    const double lj = Lm[kk][jq];
+                    if (lj != 0.0) a[j] = __dsub_rn(a[j], __dmul_rn(v, lj));
+                }
+            }
+            a[kk] = __dmul_rn(v, invd[kk]);
+        }
+    }
+}
+
+// Deferred update of the group's columns after panel M by its pivots
+// (rows at or below the column: lo = true for every row below the region).
+template <int M>
+__device__ __forceinline__ void gdeferred(double (&a)[kGroupCols], int ncols, const double (*Wd)[kGroupCols + 1],
+                                          int r, bool below) {
+#pragma unroll
+    for (int j = 16 * M + 16; j < kGroupCols; ++j) {
+        if (j < ncols) {
+            double sacc = 0.0;
+#pragma unroll
+            for (int t = 0; t < 16; ++t) sacc = __dadd_rn(sacc, __dmul_rn(a[16 * M + t], Wd[16 * M + t][j]));
+            if (below || r >= j) a[j] = __dsub_rn(a[j], sacc);
+        }
+    }
+}
+
+// Column maxima of panel M's pivots over the workgroup's rows (red holds
+// |entry| per pivot and thread).
+template <int M>
+__device__ __forceinline__ void gpanel_max(double (*red)[kLT + 1], double (*red2)[16], double *wgmax) {
+    const int tid = threadIdx.x;
+    __syncthreads();
+    {
+        const int q = tid & 15, part = tid >> 4;
+        double mx = 0.0;
+#pragma unroll
+        for (int rw = 0; rw < 16; ++rw) mx = fmax(mx, red[q][part * 16 + rw]);
+        red2[part][q] = mx;
+    }
+    __syncthreads();
+    if (tid < 16) {
+        double mx = 0.0;
+#pragma unroll
+        for (int pp = 0; pp < 16; ++pp) mx = fmax(mx, red2[pp][tid]);
+        wgmax[16 * M + tid] = mx;
+    }
+}
+
+// Rows of a group: the diagonal region [ps, ps + ncols) and rows
+// [ps + ncols, f) in chunks of kGroupRows per workgroup. STATS: column
+// maxima per pivot to gpart (workgroup 0 also the diagonal-region maxima
+// and the pivots); else the rows' group columns and W are written.
+template <bool STATS>
+__global__ __launch_bounds__(kLT) void k_big_group(LdltDev d, const BigLd *__restrict__ states,
+                                                   double *__restrict__ gpart, int32_t nch) {
+    __shared__ double Wd[kGroupCols][kGroupCols + 1];  // Wd[t][r]: row r of the region at pivot t, unscaled
+    __shared__ double Lm[kGroupCols][16];              // multipliers of pivot t for the rest of its panel
+    __shared__ double invd[kGroupCols], dd[kGroupCols], wgmax[kGroupCols];
+    __shared__ double red[STATS ? 16 : 1][kLT + 1];
+    __shared__ double red2[16][16];
+    const BigLd B = states[blockIdx.y];
+    int ps, ncols;
+    if (STATS) {
+        ps = B.k;
+        ncols = group_cols(B, 16);
+    } else {
+        ps = B.g_ps;
+        ncols = B.g_apply ? B.g_n : 0;
+    }
+    if (ncols <= 0) return;
+    const FrontLd F = d.fronts[B.s];
+    const int f = F.f;
+    const int gend = ps + ncols;
+    const int nact = max(1, (f - gend + kGroupRows - 1) / kGroupRows);
+    const int bx = blockIdx.x;
+    if (bx >= nact) return;
+    double *G = d.arena + F.fofs;
+    double *W = d.work + F.wofs;
+    const int64_t wcol0 = ps - B.w_base;  // W column of the group's first pivot
+    const int tid = threadIdx.x, wv = tid >> 6, r = tid & 63;
+    // ---- diagonal region, wave 0 ----
+    {
+        double a[kGroupCols];
+        const bool mine = wv == 0 && r < ncols;
+        const int rr = ps + min(r, ncols - 1);
+        double *gdiag = gpart + blockIdx.y * gpart_stride(nch) + kGDiag;
+        if (STATS) {
+#pragma unroll
+            for (int c = 0; c < kGroupCols; ++c)
+                a[c] = masked(G[static_cast<int64_t>(ps + min(c, ncols - 1)) * f + rr], mine && c <= r);
+            if (bx == 0 && wv == 0) {
+#pragma unroll
+                for (int c = 0; c < kGroupCols; ++c) gdiag[c * kGroupCols + r] = a[c];
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < kGroupCols; ++c) a[c] = masked(gdiag[c * kGroupCols + r], mine && c <= r);
+        }
+        if (wv == 0) gdiag_pivots<0>(a, r, ncols, Wd, Lm, invd, dd);
+        __syncthreads();
+        if (wv == 0 && 16 < ncols) {
+            gdeferred<0>(a, ncols, Wd, r, false);
+            gdiag_pivots<1>(a, r, ncols, Wd, Lm, invd, dd);
+        }
+        __syncthreads();
+        if (wv == 0 && 32 < ncols) {
+            gdeferred<1>(a, ncols, Wd, r, false);
+            gdiag_pivots<2>(a, r, ncols, Wd, Lm, invd, dd);
+        }
+        __syncthreads();
+        if (wv == 0 && 48 < ncols) {
+            gdeferred<2>(a, ncols, Wd, r, false);
+            gdiag_pivots<3>(a, r, ncols, Wd, Lm, invd, dd);
+        }
+        __syncthreads();
+        if (!STATS && bx == 0 && mine) {
+#pragma unroll
+            for (int c = 0; c < kGroupCols; ++c) {
+                if (c < ncols && c <= r) G[static_cast<int64_t>(ps + c) * f + ps + r] = a[c];
+                if (c < ncols && c < r) W[(wcol0 + c) * f + ps + r] = Wd[c][r];
+            }
+        }
+    }
+    if (STATS && bx == 0 && tid < ncols) {
+        double mx = 0.0;
+        for (int rw = tid + 1; rw < ncols; ++rw) mx = fmax(mx, fabs(Wd[tid][rw]));
+        double *gp = gpart + blockIdx.y * gpart_stride(nch);
+        gp[tid] = mx;
+        gp[kGroupCols + tid] = dd[tid];
+    }
+    // ---- rows below the diagonal region, one per thread ----
+    {
+        const int i = gend + bx * kGroupRows + tid;
+        const bool rowok = i < f;
+        const int il = min(i, f - 1);
+        double a[kGroupCols];
+#pragma unroll
+        for (int c = 0; c < kGroupCols; ++c)
+            a[c] = masked(G[static_cast<int64_t>(ps + min(c, ncols - 1)) * f + il], rowok && c < ncols);
+        grow_pivots<0, STATS>(a, ncols, Lm, invd, red, W, wcol0, f, i, rowok);
+        if (STATS) gpanel_max<0>(red, red2, wgmax);
+        if (16 < ncols) {
+            gdeferred<0>(a, ncols, Wd, 0, true);
+            grow_pivots<1, STATS>(a, ncols, Lm, invd, red, W, wcol0, f, i, rowok);
+            if (STATS) gpanel_max<1>(red, red2, wgmax);
+        }
+        if (32 < ncols) {
+            gdeferred<1>(a, ncols, Wd, 0, true);
+            grow_pivots<2, STATS>(a, ncols, Lm, invd, red, W, wcol0, f, i, rowok);
+            if (STATS) gpanel_max<2>(red, red2, wgmax);
+        }
+        if (48 < ncols) {
+            gdeferred<2>(a, ncols, Wd, 0, true);
+            grow_pivots<3, STATS>(a, ncols, Lm, invd, red, W, wcol0, f, i, rowok);
+            if (STATS) gpanel_max<3>(red, red2, wgmax);
+        }
+        if (STATS) {
+            __syncthreads();
+            if (tid < ncols) gpart[blockIdx.y * gpart_stride(nch) + kGMax + bx * kGroupCols + tid] = wgmax[tid];
+        } else if (rowok) {
+#pragma unroll
+            for (int c = 0; c < kGroupCols; ++c)
+                if (c < ncols) G[static_cast<int64_t>(ps + c) * f + i] = a[c];
+        }
+    }
+}
+
+__global__ __launch_bounds__(kLT) void k_big_panel(LdltDev d, BigLd *states, Prm prm, int32_t nb,
+                                                  const double *__restrict__ gpart, int32_t nch) {
     __shared__ RedScratch rs;
     __shared__ int cnt[3];
     BigLd &B = states[blockIdx.x];
@@ -687,6 +959,7 @@ __global__ __launch_bounds__(kLT) void k_big_panel(LdltDev d, BigLd *states, Prm
             B.u_p1 = B.u_p0;
             B.x_p1 = B.x_p0;
             B.stalled = 0;
+            B.g_apply = 0;
         }
         return;
     }
@@ -704,7 +977,83 @@ __global__ __launch_bounds__(kLT) void k_big_panel(LdltDev d, BigLd *states, Prm
     __syncthreads();
     int u_c0 = 0, u_c1 = 0, u_p0 = 0, u_p1 = 0, x_c0 = ne, x_p0 = 0, x_p1 = 0, w_base = pend0;
     int stalled = 0, npass = pass, ps = k, pl = k;
-    if (pass == 0) {
+    int spec = B.spec, g_apply = 0, g_ps = 0, g_n = 0;
+    bool group_done = false;
+    const int gcols = gpart ? group_cols(B, nb) : 0;
+    if (gcols > 0) {
+        // judge the group the stats launch evaluated: column maxima over
+        // its workgroups and the diagonal region, then the first test per
+        // pivot in order; whole panels are accepted
+        __shared__ double gmax[kGroupCols], gd[kGroupCols];
+        __shared__ int acc_s;
+        const int nact = max(1, (f - (k + gcols) + kGroupRows - 1) / kGroupRows);
+        const double *gp = gpart + blockIdx.x * gpart_stride(nch);
+        {
+            const int c = tid & 63, part = tid >> 6;
+            double mx = 0.0;
+            if (c < gcols)
+                for (int w = part; w < nact; w += 4) mx = fmax(mx, gp[kGMax + w * kGroupCols + c]);
+            rs.v[tid] = mx;
+        }
+        __syncthreads();
+        if (tid < kGroupCols) {
+            const double pm = fmax(fmax(rs.v[tid], rs.v[tid + 64]), fmax(rs.v[tid + 128], rs.v[tid + 192]));
+            gmax[tid] = tid < gcols ? fmax(pm, gp[tid]) : 0.0;
+            gd[tid] = tid < gcols ? gp[kGroupCols + tid] : 0.0;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            int nacc = 0;
+            for (int c0 = 0; c0 < gcols; c0 += 16) {
+                const int c1 = min(c0 + 16, gcols);
+                bool ok = true;
+                for (int kk = c0; kk < c1 && ok; ++kk) {
+                    const double a0 = fabs(gd[kk]), cm = gmax[kk];
+                    ok = a0 >= prm.eff_small && !(prm.u * cm > a0) && !(a0 < kWeakPivotRatio * cm);
+                }
+                if (!ok) break;
+                nacc = c1;
+            }
+            acc_s = nacc;
+            for (int kk = 0; kk < nacc; ++kk) {
+                inertia_1x1(gd[kk], prm.eff_small, cnt);
+                const double rem = static_cast<double>(f - (k + kk + 1));
+                flops += rem + rem * (rem + 1.0);
+            }
+        }
+        __syncthreads();
+        const int nacc = acc_s;
+        if (nacc > 0) {
+            for (int c = tid; c < nacc; c += kLT) tags[k + c] = 1;
+            const int kn = k + nacc;
+            if (kn < oe) {
+                u_c0 = kn;
+                u_c1 = oe;
+                u_p0 = k;
+                u_p1 = kn;
+            }
+            if (kn + nb > oe || kn >= live) {
+                if (kn > pend0 && oe < ne) {
+                    x_c0 = oe;
+                    x_p0 = pend0;
+                    x_p1 = kn;
+                }
+                pend0 = kn;
+                oe = min(ne, kn + kBigOuter);
+            }
+            g_apply = 1;
+            g_ps = k;
+            g_n = nacc;
+            spec = nacc == gcols ? 1 : 0;  // a panel failed inside: the next panel goes the one-panel way
+            ps = k + ((nacc - 1) / 16) * 16;
+            pl = kn;
+            k = kn;
+            group_done = true;
+        }
+    }
+    if (group_done) {
+    } else if (pass == 0) {
+        spec = 1;
         if (k >= live) {
             // the first pass is over: what is held back goes out, then the
             // second pass (single-level updates)
@@ -814,6 +1163,10 @@ __global__ __launch_bounds__(kLT) void k_big_panel(LdltDev d, BigLd *states, Prm
         B.cnt[1] = cnt[1];
         B.cnt[2] = cnt[2];
         B.flops = flops;
+        B.spec = spec;
+        B.g_apply = g_apply;
+        B.g_ps = g_ps;
+        B.g_n = g_n;
     }
 }
 
@@ -959,10 +1312,19 @@ __global__ void k_big_finish(LdltDev d, const BigLd *__restrict__ states, int32_
 }
 
 void launch_big_panel(hipStream_t st, const LdltDev &d, BigLd *states, int32_t nbig, double u, double eff_small,
-                      double small_pivot, int32_t nb) {
-    if (nbig <= 0) return;
+                      double small_pivot, int32_t nb, const double *gpart, int32_t nch) {
     Prm prm{u, eff_small, small_pivot};
-    k_big_panel<<<dim3(nbig), kLT, 0, st>>>(d, states, prm, nb);
+    k_big_panel<<<dim3(nbig), kLT, 0, st>>>(d, states, prm, nb, gpart, nch);
+}
+
+void launch_big_group_stats(hipStream_t st, const LdltDev &d, const BigLd *states, int32_t nbig, double *gpart,
+                            int32_t nch) {
+    k_big_group<true><<<dim3(nch, nbig), kLT, 0, st>>>(d, states, gpart, nch);
+}
+
+void launch_big_group_apply(hipStream_t st, const LdltDev &d, const BigLd *states, int32_t nbig, double *gpart,
+                            int32_t nch) {
+    k_big_group<false><<<dim3(nch, nbig), kLT, 0, st>>>(d, states, gpart, nch);
 }
 
 void launch_big_update(hipStream_t st, const LdltDev &d, const BigLd *states, int32_t nbig, int32_t per_front) {

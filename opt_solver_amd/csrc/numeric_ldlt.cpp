@@ -131,6 +131,7 @@ static int ldlt_big_fronts(smfgpu_plan_s *P, const LdltDev &d, const std::vector
     constexpr int kBigCheck = 16;
     hipStream_t st = P->stream;
     const int32_t nbig = static_cast<int32_t>(big.size());
+    const bool group = P->ldlt_group != 0 && nb == 16;
     h.assign(nbig, BigLd{});
     int64_t cap = 64;
     for (int32_t b = 0; b < nbig; ++b) {
@@ -140,14 +141,25 @@ static int ldlt_big_fronts(smfgpu_plan_s *P, const LdltDev &d, const std::vector
         B.live = F.ne;
         B.k_paired = -1;
         B.oe = std::min(F.ne, kBigOuter);
+        B.spec = group ? 1 : 0;
         cap += 4 * (static_cast<int64_t>(F.ne) / std::max(nb, 1) + 2) + 2 * static_cast<int64_t>(F.ne);
     }
     int rc;
     if ((rc = ensure(P->d_big, static_cast<size_t>(nbig), false))) return rc;
     HIP_OK(hipMemcpyAsync(P->d_big.p, h.data(), nbig * sizeof(BigLd), hipMemcpyHostToDevice, st));
     const int32_t per_front = std::max(64, 1024 / nbig);
+    // speculative groups: workgroups per front over the rows below a
+    // group's diagonal region, and their column maxima
+    int32_t maxf = 0;
+    for (int32_t s : big) maxf = std::max(maxf, P->fl[s].f);
+    const int32_t nch = (maxf + kGroupRows - 1) / kGroupRows + 1;
+    const size_t gstride = static_cast<size_t>(nch + 2 + kGroupCols) * kGroupCols;
+    if (group && (rc = ensure(P->d_gpart, static_cast<size_t>(nbig) * gstride, false))) return rc;
+    double *gpart = group ? P->d_gpart.p : nullptr;
     for (int64_t it = 0;; ++it) {
-        launch_big_panel(st, d, P->d_big.p, nbig, opt.pivot_u, eff_small, small, nb);
+        if (group) launch_big_group_stats(st, d, P->d_big.p, nbig, gpart, nch);
+        launch_big_panel(st, d, P->d_big.p, nbig, opt.pivot_u, eff_small, small, nb, gpart, nch);
+        if (group) launch_big_group_apply(st, d, P->d_big.p, nbig, gpart, nch);
         launch_big_update(st, d, P->d_big.p, nbig, per_front);
         launch_big_stall(st, d, P->d_big.p, nbig, nb);
         if (it % kBigCheck == kBigCheck - 1) {

@@ -71,9 +71,13 @@ struct BigLd {
     int32_t x_c0, x_p0, x_p1;        // flush request: columns [x_c0, ne), rows [x_c0, f), pivots [x_p0, x_p1)
     int32_t w_base;             // pivot of W column 0 for both requests
     int32_t cnt[3];
+    int32_t spec;               // 1: the next first-pass iteration tries a speculative group
+    int32_t g_apply, g_ps, g_n; // accepted group to write: columns [g_ps, g_ps + g_n)
     int32_t pad;
     double flops;
 };
+constexpr int32_t kGroupCols = 64;   // columns of a speculative group (four panels of 16)
+constexpr int32_t kGroupRows = 256;  // rows per workgroup below a group's diagonal region
 
 void launch_ldlt_small(hipStream_t st, const LdltDev &d, const int32_t *flist, int32_t nf, double u, double eff_small,
                        double small_pivot);
@@ -94,8 +98,15 @@ void launch_ldlt_fwd_step(hipStream_t st, const LdltDev &d, const SolveItem *ite
                           int32_t n, int32_t nrhs);
 void launch_ldlt_bwd_step(hipStream_t st, const LdltDev &d, const SolveItem *items, int64_t count, double *y,
                           int32_t n, int32_t nrhs);
+// one iteration's pivot logic; with gpart set it first judges the group
+// the stats launch evaluated (nch workgroups per front)
 void launch_big_panel(hipStream_t st, const LdltDev &d, BigLd *states, int32_t nbig, double u, double eff_small,
-                      double small_pivot, int32_t nb);
+                      double small_pivot, int32_t nb, const double *gpart, int32_t nch);
+// speculative group: column maxima (stats) and the accepted panels' rows (apply)
+void launch_big_group_stats(hipStream_t st, const LdltDev &d, const BigLd *states, int32_t nbig, double *gpart,
+                            int32_t nch);
+void launch_big_group_apply(hipStream_t st, const LdltDev &d, const BigLd *states, int32_t nbig, double *gpart,
+                            int32_t nch);
 void launch_big_update(hipStream_t st, const LdltDev &d, const BigLd *states, int32_t nbig, int32_t per_front);
 void launch_big_stall(hipStream_t st, const LdltDev &d, BigLd *states, int32_t nbig, int32_t nb);
 // final contribution-block update with pivots [c0, c0 + width) of every

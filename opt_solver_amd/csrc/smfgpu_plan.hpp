@@ -180,6 +180,8 @@ struct smfgpu_plan_s {
     int32_t cb_seg = 2048;                         // SPD: pivot columns per contribution-block update
     int32_t ldlt_big_min = 2048;                   // LDL^T: fronts with at least this many rows take the large-front path
     DBuf<BigLd> d_big;                             // LDL^T: large-front states of the level
+    int32_t ldlt_group = 1;                        // LDL^T: speculative panel groups on the large-front path
+    DBuf<double> d_gpart;                          // LDL^T: per-workgroup column maxima of a group
     int32_t syrk_ahead = 2;                        // Schur update: chunks with global loads in flight (1 or 2)
     int32_t solve_fuse_rows = 512;                 // rows below a last block its own workgroup updates
     double t_solve_phase[4] = {0, 0, 0, 0};        // fwd ms, bwd ms, fwd launches, bwd launches

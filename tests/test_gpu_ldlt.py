@@ -156,17 +156,20 @@ def _large_path_case(case):
     return A, _arrays_for(A, refshim.ORDER_METIS), (gsz * gsz, m, 0)
 
 
+@pytest.mark.parametrize("group", [1, 0])
 @pytest.mark.parametrize("case", ["kkt_30", "kkt_48", "ocp_T200", "kkt_grid_60"])
-def test_large_front_path_against_oracle(oracle_lib, case):
-    """The multi-workgroup blocked path (forced on every blocked front):
-    the same pivot sequence, delays, 2x2 pivots and inertia as the oracle's
-    single-workgroup blocked kernel, L and D within 1e-9 (the deferred
-    updates run on MFMA in another summation order), solves to 1e-14."""
-    from opt_solver_amd._lib import TUNE_LDLT_BIG_MIN
-    print("case large path", case, flush=True)
+def test_large_front_path_against_oracle(oracle_lib, case, group):
+    """The multi-workgroup blocked path (forced on every blocked front),
+    with speculative panel groups and one panel at a time: the same pivot
+    sequence, delays, 2x2 pivots and inertia as the oracle's blocked
+    kernel, L and D within 1e-9 (the deferred updates past a panel group
+    run on MFMA in another summation order), solves to 1e-14."""
+    from opt_solver_amd._lib import TUNE_LDLT_BIG_MIN, TUNE_LDLT_GROUP
+    print("case large path", case, "group", group, flush=True)
     A, arrays, inertia = _large_path_case(case)
     plan = Plan(arrays, posdef=False)
     plan.tune(TUNE_LDLT_BIG_MIN, 96)
+    plan.tune(TUNE_LDLT_GROUP, group)
     fac = plan.factor(A[3], u=0.01)
     orc = oracle_lib.Factor(arrays, A[3], posdef=False)
     assert fac.inertia == orc.inertia == inertia
