@@ -214,8 +214,9 @@ SMFGPU_API int smfgpu_refine(smfgpu_factor_t f, const double *values, const doub
  * FP32 Schur updates accumulate in FP32 over one segment). */
 #define SMFGPU_TUNE_CB_SEG 4
 /* SMFGPU_TUNE_LDLT_BIG_MIN: LDL^T fronts with at least this many rows
- * (default 2048, minimum 96) run the blocked algorithm over many
- * workgroups (pivot logic per panel in one workgroup, MFMA updates). */
+ * (default 256, minimum 96) run the blocked algorithm over many
+ * workgroups (speculative panel groups, pivot logic per panel in one
+ * workgroup, MFMA updates); smaller blocked fronts take one workgroup. */
 #define SMFGPU_TUNE_LDLT_BIG_MIN 5
 /* SMFGPU_TUNE_LDLT_GROUP: 1 (default) lets the large-front LDL^T path try
  * groups of four first-pass panels at once (accepted when every pivot

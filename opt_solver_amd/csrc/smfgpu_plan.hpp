@@ -178,7 +178,7 @@ struct smfgpu_plan_s {
     int32_t panel_outer = kPanelOuter;             // SPD: pivot columns per trailing Schur update
     std::vector<int64_t> asm_first;                // SPD: first assembly task of each level
     int32_t cb_seg = 2048;                         // SPD: pivot columns per contribution-block update
-    int32_t ldlt_big_min = 2048;                   // LDL^T: fronts with at least this many rows take the large-front path
+    int32_t ldlt_big_min = 256;                    // LDL^T: fronts with at least this many rows take the large-front path
     DBuf<BigLd> d_big;                             // LDL^T: large-front states of the level
     int32_t ldlt_group = 1;                        // LDL^T: speculative panel groups on the large-front path
     DBuf<double> d_gpart;                          // LDL^T: per-workgroup column maxima of a group

@@ -156,20 +156,21 @@ def _large_path_case(case):
     return A, _arrays_for(A, refshim.ORDER_METIS), (gsz * gsz, m, 0)
 
 
-@pytest.mark.parametrize("group", [1, 0])
+@pytest.mark.parametrize("mode", ["group", "panel", "one_wg"])
 @pytest.mark.parametrize("case", ["kkt_30", "kkt_48", "ocp_T200", "kkt_grid_60"])
-def test_large_front_path_against_oracle(oracle_lib, case, group):
-    """The multi-workgroup blocked path (forced on every blocked front),
-    with speculative panel groups and one panel at a time: the same pivot
-    sequence, delays, 2x2 pivots and inertia as the oracle's blocked
-    kernel, L and D within 1e-9 (the deferred updates past a panel group
-    run on MFMA in another summation order), solves to 1e-14."""
+def test_large_front_path_against_oracle(oracle_lib, case, mode):
+    """Every blocked front (f >= 96) on the multi-workgroup path with
+    speculative panel groups ("group") or one panel at a time ("panel"), or
+    on the one-workgroup blocked kernel ("one_wg"): the same pivot sequence,
+    delays, 2x2 pivots and inertia as the oracle's blocked kernel, L and D
+    within 1e-9 (the deferred updates past a panel group run on MFMA in
+    another summation order), solves to 1e-14."""
     from opt_solver_amd._lib import TUNE_LDLT_BIG_MIN, TUNE_LDLT_GROUP
-    print("case large path", case, "group", group, flush=True)
+    print("case large path", case, mode, flush=True)
     A, arrays, inertia = _large_path_case(case)
     plan = Plan(arrays, posdef=False)
-    plan.tune(TUNE_LDLT_BIG_MIN, 96)
-    plan.tune(TUNE_LDLT_GROUP, group)
+    plan.tune(TUNE_LDLT_BIG_MIN, 1 << 30 if mode == "one_wg" else 96)
+    plan.tune(TUNE_LDLT_GROUP, 1 if mode == "group" else 0)
     fac = plan.factor(A[3], u=0.01)
     orc = oracle_lib.Factor(arrays, A[3], posdef=False)
     assert fac.inertia == orc.inertia == inertia
@@ -183,9 +184,10 @@ def test_large_front_path_against_oracle(oracle_lib, case, group):
 
 
 def test_large_front_default_threshold(oracle_lib):
-    """A grid KKT whose largest front (3016 rows) takes the large-front path
-    at the default threshold: Sylvester inertia and a backward error at
-    FP64 level (too large for the CPU oracle's dense kernel in a test)."""
+    """A grid KKT whose fronts of 256 rows and more (the largest 3016) take
+    the large-front path at the default threshold: Sylvester inertia and a
+    backward error at FP64 level (too large for the CPU oracle's dense
+    kernel in a test)."""
     print("case kkt_grid_120", flush=True)
     gsz = 120
     m = gsz * gsz // 5
