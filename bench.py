@@ -162,6 +162,15 @@ def main():
     syrk_avg_s = kt["syrk_ms"] / max(kt["syrk_launches"], 1) * 1e-3
     syrk_flops_per_launch = kt["syrk_flops"] / max(kt["syrk_launches"], 1)
     achieved = syrk_flops_per_launch / syrk_avg_s if syrk_avg_s > 0 else 0.0
+    # HBM bytes per mixed-phase k_syrk launch: the committed summary of the
+    # two rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) over this
+    # workload, scaled by a plain copy's readings
+    traffic = None
+    tfile = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "k_syrk_spd27_100cube_traffic.txt")
+    if m == 100 and args.nrhs == 1 and os.path.exists(tfile):
+        for line in open(tfile):
+            if line.startswith("traffic_bytes_per_launch_mixed"):
+                traffic = int(line.split("=")[1])
     cpu = None
     if not args.no_cpu:
         try:
@@ -194,7 +203,7 @@ def main():
         "speedup_mixed_vs_fp64": round(fp64["ms_per_step"] / mixed["ms_per_step"], 3),
         "roofline": {"bound": "mfma", "kernel": "k_syrk (Schur update)", "achieved": achieved / 1e12,
                      "peak": PEAK["fp32"] / 1e12, "unit": "TFLOP/s", "frac": achieved / PEAK["fp32"],
-                     "traffic": None},
+                     "traffic": traffic},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
