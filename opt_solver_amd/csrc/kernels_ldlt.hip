@@ -1350,10 +1350,11 @@ void launch_big_finish(hipStream_t st, const LdltDev &d, const BigLd *states, in
 }
 
 // ---------------------------------------------------------------------------
-// Assembly of the level's fronts (full f x f squares in the arena). One
-// workgroup per front column: zero, original entries of the front's own
-// pivot columns, then each child's CB column in child order; the CB of a
-// child is the trailing (f_c - p_c) square of its own front.
+// Assembly of the level's fronts (f x f squares in the arena, lower
+// triangle). One workgroup per front column, rows from the diagonal down:
+// zero, original entries of the front's own pivot columns, then each
+// child's CB column in child order; the CB of a child is the trailing
+// (f_c - p_c) square of its own front.
 constexpr int kAsmChunkL = 2048;
 
 __device__ __forceinline__ int lbound(const int32_t *a, int lo, int hi, int32_t key) {
@@ -1374,7 +1375,7 @@ __global__ __launch_bounds__(kLT) void k_ldlt_assemble(LdltDev d, const AsmTask 
     const int f = F.f;
     double *colp = d.arena + F.fofs + static_cast<int64_t>(c) * f;
     const int tid = threadIdx.x;
-    for (int r = tid; r < c; r += kLT) colp[r] = 0.0;
+    // the strict upper triangle is never read: left as it is
     const int oc = c - F.nd;  // own pivot column index, when in [0, psym)
     for (int r0 = c; r0 < f; r0 += kAsmChunkL) {
         const int r1 = min(f, r0 + kAsmChunkL);

@@ -56,8 +56,9 @@ struct LdltDev {
 // updates of the columns beyond it are held back (their unscaled columns
 // kept in W) and applied in one flush when the outer block ends, before a
 // stall's pulls or rotation, and before the second pass.
-constexpr int32_t kBigOuter = 256;
-constexpr int32_t kBigWork = 512;  // W columns per front (outer block + panel; L*D chunks of the final update)
+constexpr int32_t kBigOuter = 512;
+constexpr int32_t kBigWork = 512;  // W columns per front (an outer block's pivots; L*D chunks of the final update)
+static_assert(kBigWork >= kBigOuter, "W holds the pivots of an outer block");
 struct BigLd {
     int32_t s;                  // supernode
     int32_t k, live, k_paired;  // next pivot position, end of the live eligible columns, pulls' guard
