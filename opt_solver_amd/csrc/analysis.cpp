@@ -7,9 +7,11 @@
 // drop-in producer of this data; this implementation is what the library
 // uses when no smf analysis is supplied. The ordering is an input (perm,
 // new -> old); amalgamation merges a supernode into its parent when it is
-// the parent's last child and the merged width stays within nemin, the
-// classic relaxed-supernode rule, not smf's cost model
-// (solver/src/supernode_detection.cpp:41, 508).
+// the parent's last child and the merged width stays within nemin (the
+// classic relaxed-supernode rule), then children whose contribution block
+// costs more to move than to eliminate inside the parent (a cost model of
+// its own, in the spirit of smf's, solver/src/supernode_detection.cpp:41,
+// 508).
 #include "smfgpu_internal.hpp"
 
 #include <algorithm>
@@ -121,6 +123,149 @@ std::vector<int64_t> column_counts(int32_t n, const std::vector<int32_t> &cp,
     for (int32_t j = 0; j < n; ++j)
         if (parent[j] != -1) delta[parent[j]] += delta[j];
     return delta;
+}
+
+// Supernodes of the column ranges [starts[k], starts[k+1]) (a topological
+// order of the elimination tree `parent`): parents, children (descending
+// order) and front rows (pivot columns, then the rows below from the
+// pattern and the children's fronts, sorted).
+int build_tree(const std::vector<int32_t> &starts, const std::vector<int32_t> &parent, Analysis &out) {
+    const int32_t n = out.n;
+    const int32_t nsn = static_cast<int32_t>(starts.size()) - 1;
+    out.sn_start.assign(starts.begin(), starts.end() - 1);
+    out.sn_end.assign(starts.begin() + 1, starts.end());
+    std::vector<int32_t> col2sn(n);
+    for (int32_t k = 0; k < nsn; ++k)
+        for (int32_t j = out.sn_start[k]; j < out.sn_end[k]; ++j) col2sn[j] = k;
+    out.sn_parent.assign(nsn, -1);
+    for (int32_t k = 0; k < nsn; ++k) {
+        const int32_t pj = parent[out.sn_end[k] - 1];
+        out.sn_parent[k] = pj < 0 ? -1 : col2sn[pj];
+        if (out.sn_parent[k] >= 0 && out.sn_parent[k] <= k)
+            return set_error(SMFGPU_ERR_INVALID_ARG, "analyse: supernode order is not topological");
+    }
+    out.child_ptr.assign(nsn + 1, 0);
+    for (int32_t k = 0; k < nsn; ++k)
+        if (out.sn_parent[k] >= 0) ++out.child_ptr[out.sn_parent[k] + 1];
+    for (int32_t k = 0; k < nsn; ++k) out.child_ptr[k + 1] += out.child_ptr[k];
+    out.child_idx.assign(out.child_ptr[nsn], 0);
+    {
+        std::vector<int32_t> fill(out.child_ptr.begin(), out.child_ptr.end() - 1);
+        for (int32_t k = nsn - 1; k >= 0; --k)
+            if (out.sn_parent[k] >= 0) out.child_idx[fill[out.sn_parent[k]]++] = k;
+    }
+    out.front_row_ptr.assign(nsn + 1, 0);
+    out.front_row_idx.clear();
+    std::vector<int32_t> mark(n, -1);
+    std::vector<int32_t> below;
+    for (int32_t k = 0; k < nsn; ++k) {
+        const int32_t a = out.sn_start[k], b = out.sn_end[k];
+        below.clear();
+        for (int32_t j = a; j < b; ++j)
+            for (int32_t p = out.pcol_ptr[j]; p < out.pcol_ptr[j + 1]; ++p) {
+                const int32_t r = out.prow_idx[p];
+                if (r >= b && mark[r] != k) {
+                    mark[r] = k;
+                    below.push_back(r);
+                }
+            }
+        for (int32_t c = out.child_ptr[k]; c < out.child_ptr[k + 1]; ++c) {
+            const int32_t ch = out.child_idx[c];
+            for (int32_t t = out.front_row_ptr[ch] + (out.sn_end[ch] - out.sn_start[ch]);
+                 t < out.front_row_ptr[ch + 1]; ++t) {
+                const int32_t r = out.front_row_idx[t];
+                if (r >= b && mark[r] != k) {
+                    mark[r] = k;
+                    below.push_back(r);
+                }
+            }
+        }
+        std::sort(below.begin(), below.end());
+        for (int32_t j = a; j < b; ++j) out.front_row_idx.push_back(j);
+        out.front_row_idx.insert(out.front_row_idx.end(), below.begin(), below.end());
+        if (out.front_row_idx.size() > static_cast<size_t>(INT32_MAX))
+            return set_error(SMFGPU_ERR_ALLOC, "analyse: front rows exceed 2^31");
+        out.front_row_ptr[k + 1] = static_cast<int32_t>(out.front_row_idx.size());
+    }
+    out.nsup = nsn;
+    return SMFGPU_OK;
+}
+
+// Cost-model amalgamation: child c joins its parent s when eliminating c's
+// p_c columns inside s's front costs less than moving c's contribution
+// block. The merged front has f_s + p_c rows (c's contribution rows are
+// rows of s), so the extra elimination work is p_c ((f_s + p_c)^2 - f_c^2)
+// flops, against about 12 q_c^2 bytes (q_c = f_c - p_c) of the block's
+// update, store and extend-add; with Schur updates near 30 TFLOP/s and
+// those moves near 3 TB/s, merge when the flops stay below 100 q_c^2. It
+// removes chains of narrow separators whose fronts carry the same large
+// row set (KKT systems whose constraint rows couple the whole grid).
+// Only blocks of at least 1024 rows are considered (below that the per-front
+// costs the model leaves out dominate, and merging slowed 3D grids down).
+// Bottom-up, a merged child's own merges count in its sizes; not applied
+// with nemin = 1 (exact fundamental supernodes). On a merge,
+// out.perm becomes the postorder of the merged tree (each group's
+// supernodes in their order) and starts the groups' column ranges.
+bool amalgamate(Analysis &out, std::vector<int32_t> &starts) {
+    constexpr double kMergeRatio = 100.0;
+    constexpr double kMergeMinCb = 1024.0;  // smaller blocks: per-front costs the model leaves out dominate
+    const int32_t nsn = out.nsup;
+    std::vector<double> fz(nsn), pz(nsn);
+    for (int32_t k = 0; k < nsn; ++k) {
+        fz[k] = out.front_row_ptr[k + 1] - out.front_row_ptr[k];
+        pz[k] = out.sn_end[k] - out.sn_start[k];
+    }
+    std::vector<int32_t> grp(nsn);
+    std::iota(grp.begin(), grp.end(), 0);
+    bool any = false;
+    for (int32_t k = 0; k < nsn; ++k)
+        for (int32_t t = out.child_ptr[k]; t < out.child_ptr[k + 1]; ++t) {
+            const int32_t c = out.child_idx[t];
+            const double qc = fz[c] - pz[c];
+            const double fm = fz[k] + pz[c];
+            const double extra = pz[c] * (fm * fm - fz[c] * fz[c]);
+            if (qc >= kMergeMinCb && extra < kMergeRatio * qc * qc) {
+                grp[c] = k;
+                fz[k] = fm;
+                pz[k] += pz[c];
+                any = true;
+            }
+        }
+    if (!any) return false;
+    // group roots, members (ascending) and the merged tree
+    std::vector<int32_t> root(nsn);
+    for (int32_t k = nsn - 1; k >= 0; --k) root[k] = grp[k] == k ? k : root[grp[k]];
+    std::vector<std::vector<int32_t>> members(nsn), kids(nsn);
+    for (int32_t k = 0; k < nsn; ++k) members[root[k]].push_back(k);
+    std::vector<int32_t> tops;
+    for (int32_t k = 0; k < nsn; ++k) {
+        if (root[k] != k) continue;
+        if (out.sn_parent[k] < 0) tops.push_back(k);
+        else kids[root[out.sn_parent[k]]].push_back(k);
+    }
+    // postorder of the merged tree (children in ascending order)
+    std::vector<int32_t> newperm, nstarts;
+    newperm.reserve(out.n);
+    std::vector<std::pair<int32_t, size_t>> stack;
+    for (int32_t t : tops) {
+        stack.emplace_back(t, 0);
+        while (!stack.empty()) {
+            auto &top = stack.back();
+            if (top.second < kids[top.first].size()) {
+                const int32_t ch = kids[top.first][top.second++];
+                stack.emplace_back(ch, 0);
+                continue;
+            }
+            nstarts.push_back(static_cast<int32_t>(newperm.size()));
+            for (int32_t m : members[top.first])
+                for (int32_t j = out.sn_start[m]; j < out.sn_end[m]; ++j) newperm.push_back(out.perm[j]);
+            stack.pop_back();
+        }
+    }
+    nstarts.push_back(out.n);
+    out.perm = newperm;
+    starts = nstarts;
+    return true;
 }
 
 }  // namespace
@@ -235,77 +380,29 @@ int analyse(int32_t n, const int32_t *col_ptr, const int32_t *row_idx, const int
         }
     }
     // renumber surviving supernodes (in column order = postorder)
-    std::vector<int32_t> newid(ns, -1);
-    int32_t nsn = 0;
     std::vector<int32_t> order_s;
     for (int32_t s = 0; s < ns; ++s)
         if (merged_into[s] == -1) order_s.push_back(s);
     std::sort(order_s.begin(), order_s.end(), [&](int32_t a, int32_t b) { return sstart[a] < sstart[b]; });
-    for (int32_t s : order_s) newid[s] = nsn++;
+    std::vector<int32_t> starts;
+    for (int32_t s : order_s) starts.push_back(sstart[s]);
+    starts.push_back(n);
     out.n = n;
-    out.sn_start.resize(nsn);
-    out.sn_end.resize(nsn);
-    out.sn_parent.resize(nsn);
-    for (int32_t s : order_s) {
-        const int32_t k = newid[s];
-        out.sn_start[k] = sstart[s];
-        out.sn_end[k] = sstart[s] + width[s];
+    int rc;
+    if ((rc = build_tree(starts, parent, out))) return rc;
+    if (nemin > 1 && amalgamate(out, starts)) {
+        // columns renumbered so that every merged group is contiguous: the
+        // same elimination tree in another topological order, so the same
+        // fill
+        std::vector<int32_t> ip(n);
+        for (int32_t k = 0; k < n; ++k) ip[out.perm[k]] = k;
+        permuted_pattern(ip, out.pcol_ptr, out.prow_idx, &out.orig_to_slot);
+        parent = etree_lower(n, out.pcol_ptr, out.prow_idx);
+        if ((rc = build_tree(starts, parent, out))) return rc;
     }
-    for (int32_t k = 0; k < nsn; ++k)
-        for (int32_t j = out.sn_start[k]; j < out.sn_end[k]; ++j) col2sn[j] = k;
-    for (int32_t k = 0; k < nsn; ++k) {
-        const int32_t pj = parent[out.sn_end[k] - 1];
-        out.sn_parent[k] = pj < 0 ? -1 : col2sn[pj];
-    }
-    // children CSR, descending order
-    out.child_ptr.assign(nsn + 1, 0);
-    for (int32_t k = 0; k < nsn; ++k)
-        if (out.sn_parent[k] >= 0) ++out.child_ptr[out.sn_parent[k] + 1];
-    for (int32_t k = 0; k < nsn; ++k) out.child_ptr[k + 1] += out.child_ptr[k];
-    out.child_idx.assign(out.child_ptr[nsn], 0);
-    {
-        std::vector<int32_t> fill(out.child_ptr.begin(), out.child_ptr.end() - 1);
-        for (int32_t k = nsn - 1; k >= 0; --k)
-            if (out.sn_parent[k] >= 0) out.child_idx[fill[out.sn_parent[k]]++] = k;
-    }
-    // front rows: pivot columns, then rows below from the pattern and the
-    // children's fronts
-    out.front_row_ptr.assign(nsn + 1, 0);
-    out.front_row_idx.clear();
-    std::vector<int32_t> mark(n, -1);
-    std::vector<int32_t> below;
-    for (int32_t k = 0; k < nsn; ++k) {
-        const int32_t a = out.sn_start[k], b = out.sn_end[k];
-        below.clear();
-        for (int32_t j = a; j < b; ++j)
-            for (int32_t p = out.pcol_ptr[j]; p < out.pcol_ptr[j + 1]; ++p) {
-                const int32_t r = out.prow_idx[p];
-                if (r >= b && mark[r] != k) {
-                    mark[r] = k;
-                    below.push_back(r);
-                }
-            }
-        for (int32_t c = out.child_ptr[k]; c < out.child_ptr[k + 1]; ++c) {
-            const int32_t ch = out.child_idx[c];
-            for (int32_t t = out.front_row_ptr[ch] + (out.sn_end[ch] - out.sn_start[ch]);
-                 t < out.front_row_ptr[ch + 1]; ++t) {
-                const int32_t r = out.front_row_idx[t];
-                if (r >= b && mark[r] != k) {
-                    mark[r] = k;
-                    below.push_back(r);
-                }
-            }
-        }
-        std::sort(below.begin(), below.end());
-        for (int32_t j = a; j < b; ++j) out.front_row_idx.push_back(j);
-        out.front_row_idx.insert(out.front_row_idx.end(), below.begin(), below.end());
-        if (out.front_row_idx.size() > static_cast<size_t>(INT32_MAX))
-            return set_error(SMFGPU_ERR_ALLOC, "analyse: front rows exceed 2^31");
-        out.front_row_ptr[k + 1] = static_cast<int32_t>(out.front_row_idx.size());
-    }
-    out.nsup = nsn;
     return SMFGPU_OK;
 }
+
 
 void Analysis::describe(smfgpu_analysis_desc &d) const {
     d.n = n;
