@@ -722,67 +722,77 @@ __device__ __forceinline__ int group_cols(const BigLd &B, int nb) {
     return pos - B.k;
 }
 
-// Group buffer per front: [0, 64) largest entry per pivot column over the
-// diagonal region, [64, 128) the pivots, then the region's entries before
-// the group (64 x 64, column-major: the apply launch reads them there,
-// since its workgroup 0 rewrites the region in the front), then 64 column
-// maxima per workgroup.
-constexpr int64_t kGDiag = 2 * kGroupCols;
-constexpr int64_t kGMax = kGDiag + kGroupCols * kGroupCols;
+// Group buffer per front (doubles): the diagonal region's results, written
+// by the region launch and read by the stats, panel and apply launches,
+// then 64 column maxima per workgroup of the stats launch.
+constexpr int64_t kGdMax = 0;                               // 64: largest |entry| below each pivot in the region
+constexpr int64_t kGdPiv = 64;                              // 64: pivots
+constexpr int64_t kGdInv = 128;                             // 64: 1 / pivot
+constexpr int64_t kGdLm = 192;                              // 64 x 16: multipliers of each pivot for its panel
+constexpr int64_t kGdWd = kGdLm + 64 * 16;                  // 64 x 64: [t][r] row r of the region at pivot t, unscaled
+constexpr int64_t kGdFin = kGdWd + 64 * 64;                 // 64 x 64: [c][r] the region's final entries
+constexpr int64_t kGMax = kGdFin + 64 * 64;
+static_assert(kGMax == kGroupBuf, "group buffer layout");
 __device__ __forceinline__ int64_t gpart_stride(int nch) { return kGMax + static_cast<int64_t>(nch) * kGroupCols; }
 
 // Per-panel steps of a group (panel M: columns [16 M, 16 M + 16) of the
 // group), templated so that every index into a row's registers is a
-// constant. Diagonal region: lane r of wave 0 holds row ps + r.
-template <int M>
+// constant; FULL: the group has all 64 columns (no column guards). A
+// skipped update (multiplier 0, or a row above the column) is a select,
+// the same value as the reference's skip. Diagonal region: lane r of
+// wave 0 holds row ps + r.
+template <int M, bool FULL>
 __device__ __forceinline__ void gdiag_pivots(double (&a)[kGroupCols], int r, int ncols,
                                              double (*Wd)[kGroupCols + 1], double (*Lm)[16], double *invd,
                                              double *dd) {
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
         const int kk = 16 * M + q;
-        if (kk < ncols) {
+        if (FULL || kk < ncols) {
+            // stores without conditions: every lane writes the same value
+            // to dd, invd and Lm; Wd[kk][r] for r <= kk is never read
             const double v = a[kk];
             const double dk = rdlane(v, kk);
             const double inv = 1.0 / dk;
-            if (r == kk) {
-                dd[kk] = dk;
-                invd[kk] = inv;
-            }
-            if (r > kk) Wd[kk][r] = v;
+            dd[kk] = dk;
+            invd[kk] = inv;
+            Wd[kk][r] = v;
 #pragma unroll
             for (int jq = q + 1; jq < 16; ++jq) {
                 const int j = 16 * M + jq;
-                if (j < ncols) {
+                if (FULL || j < ncols) {
                     const double lj = __dmul_rn(rdlane(v, j), inv);
-                    if (r == 0) Lm[kk][jq] = lj;
-                    if (lj != 0.0 && r >= j) a[j] = __dsub_rn(a[j], __dmul_rn(v, lj));
+                    Lm[kk][jq] = lj;
+                    const double t = __dsub_rn(a[j], __dmul_rn(v, lj));
+                    a[j] = (lj != 0.0 && r >= j) ? t : a[j];
                 }
             }
-            if (r > kk) a[kk] = __dmul_rn(v, inv);
+            const double sc = __dmul_rn(v, inv);
+            a[kk] = r > kk ? sc : a[kk];
         }
     }
 }
 
 // Rows below the region (one per thread): the same pivots with the
 // multipliers from LDS; STATS keeps |entry| per pivot in red[q][tid].
-template <int M, bool STATS>
+template <int M, bool STATS, bool FULL>
 __device__ __forceinline__ void grow_pivots(double (&a)[kGroupCols], int ncols, const double (*Lm)[16],
                                             const double *invd, double (*red)[kLT + 1], double *W, int64_t wcol0,
                                             int f, int i, bool rowok) {
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
         const int kk = 16 * M + q;
-        if (kk < ncols) {
+        if (FULL || kk < ncols) {
             const double v = a[kk];
             if (STATS) red[q][threadIdx.x] = fabs(v);
             else if (rowok) W[(wcol0 + kk) * f + i] = v;
 #pragma unroll
             for (int jq = q + 1; jq < 16; ++jq) {
                 const int j = 16 * M + jq;
-                if (j < ncols) {
+                if (FULL || j < ncols) {
                     const double lj = Lm[kk][jq];
-                    if (lj != 0.0) a[j] = __dsub_rn(a[j], __dmul_rn(v, lj));
+                    const double t = __dsub_rn(a[j], __dmul_rn(v, lj));
+                    a[j] = lj != 0.0 ? t : a[j];
                 }
             }
             a[kk] = __dmul_rn(v, invd[kk]);
@@ -791,17 +801,19 @@ __device__ __forceinline__ void grow_pivots(double (&a)[kGroupCols], int ncols, 
 }
 
 // Deferred update of the group's columns after panel M by its pivots
-// (rows at or below the column: lo = true for every row below the region).
-template <int M>
+// (below = true for every row under the region, else rows at or below the
+// column).
+template <int M, bool FULL>
 __device__ __forceinline__ void gdeferred(double (&a)[kGroupCols], int ncols, const double (*Wd)[kGroupCols + 1],
                                           int r, bool below) {
 #pragma unroll
     for (int j = 16 * M + 16; j < kGroupCols; ++j) {
-        if (j < ncols) {
+        if (FULL || j < ncols) {
             double sacc = 0.0;
 #pragma unroll
             for (int t = 0; t < 16; ++t) sacc = __dadd_rn(sacc, __dmul_rn(a[16 * M + t], Wd[16 * M + t][j]));
-            if (below || r >= j) a[j] = __dsub_rn(a[j], sacc);
+            const double u = __dsub_rn(a[j], sacc);
+            a[j] = (below || r >= j) ? u : a[j];
         }
     }
 }
@@ -828,16 +840,65 @@ __device__ __forceinline__ void gpanel_max(double (*red)[kLT + 1], double (*red2
     }
 }
 
-// Rows of a group: the diagonal region [ps, ps + ncols) and rows
-// [ps + ncols, f) in chunks of kGroupRows per workgroup. STATS: column
-// maxima per pivot to gpart (workgroup 0 also the diagonal-region maxima
-// and the pivots); else the rows' group columns and W are written.
+// The group's diagonal region [ps, ps + ncols), one wave per front: lane r
+// holds row ps + r. Results to the group buffer.
+__global__ __launch_bounds__(64) void k_big_gdiag(LdltDev d, const BigLd *__restrict__ states,
+                                                  double *__restrict__ gpart, int32_t nch) {
+    __shared__ double Wd[kGroupCols][kGroupCols + 1];
+    __shared__ double Lm[kGroupCols][16];
+    __shared__ double invd[kGroupCols], dd[kGroupCols];
+    const BigLd B = states[blockIdx.x];
+    const int ps = B.k, ncols = group_cols(B, 16);
+    if (ncols <= 0) return;
+    const FrontLd F = d.fronts[B.s];
+    const int f = F.f;
+    const double *G = d.arena + F.fofs;
+    double *gp = gpart + blockIdx.x * gpart_stride(nch);
+    const int r = threadIdx.x;
+    double a[kGroupCols];
+    const bool mine = r < ncols;
+    const int rr = ps + min(r, ncols - 1);
+#pragma unroll
+    for (int c = 0; c < kGroupCols; ++c)
+        a[c] = masked(G[static_cast<int64_t>(ps + min(c, ncols - 1)) * f + rr], mine && c <= r);
+    gdiag_pivots<0, false>(a, r, ncols, Wd, Lm, invd, dd);
+    __syncthreads();
+    if (16 < ncols) {
+        gdeferred<0, false>(a, ncols, Wd, r, false);
+        gdiag_pivots<1, false>(a, r, ncols, Wd, Lm, invd, dd);
+    }
+    __syncthreads();
+    if (32 < ncols) {
+        gdeferred<1, false>(a, ncols, Wd, r, false);
+        gdiag_pivots<2, false>(a, r, ncols, Wd, Lm, invd, dd);
+    }
+    __syncthreads();
+    if (48 < ncols) {
+        gdeferred<2, false>(a, ncols, Wd, r, false);
+        gdiag_pivots<3, false>(a, r, ncols, Wd, Lm, invd, dd);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < kGroupCols; ++c) gp[kGdFin + c * kGroupCols + r] = a[c];
+    for (int e = r; e < kGroupCols * kGroupCols; e += 64) gp[kGdWd + e] = Wd[e >> 6][e & 63];
+    for (int e = r; e < kGroupCols * 16; e += 64) gp[kGdLm + e] = Lm[e >> 4][e & 15];
+    gp[kGdPiv + r] = dd[r];
+    gp[kGdInv + r] = invd[r];
+    double mx = 0.0;
+    for (int rw = r + 1; rw < ncols; ++rw) mx = fmax(mx, fabs(Wd[r][rw]));
+    gp[kGdMax + r] = mx;
+}
+
+// Rows of a group below its diagonal region, [ps + ncols, f), in chunks of
+// kGroupRows per workgroup, one row per thread, with the region's tables.
+// STATS: column maxima per pivot to the group buffer; else the rows' group
+// columns and W are written, and workgroup 0 also writes the region.
 template <bool STATS>
 __global__ __launch_bounds__(kLT) void k_big_group(LdltDev d, const BigLd *__restrict__ states,
                                                    double *__restrict__ gpart, int32_t nch) {
-    __shared__ double Wd[kGroupCols][kGroupCols + 1];  // Wd[t][r]: row r of the region at pivot t, unscaled
-    __shared__ double Lm[kGroupCols][16];              // multipliers of pivot t for the rest of its panel
-    __shared__ double invd[kGroupCols], dd[kGroupCols], wgmax[kGroupCols];
+    __shared__ double Wd[kGroupCols][kGroupCols + 1];
+    __shared__ double Lm[kGroupCols][16];
+    __shared__ double invd[kGroupCols], wgmax[kGroupCols];
     __shared__ double red[STATS ? 16 : 1][kLT + 1];
     __shared__ double red2[16][16];
     const BigLd B = states[blockIdx.y];
@@ -853,97 +914,58 @@ __global__ __launch_bounds__(kLT) void k_big_group(LdltDev d, const BigLd *__res
     const FrontLd F = d.fronts[B.s];
     const int f = F.f;
     const int gend = ps + ncols;
-    const int nact = max(1, (f - gend + kGroupRows - 1) / kGroupRows);
     const int bx = blockIdx.x;
-    if (bx >= nact) return;
+    if (bx >= max(1, (f - gend + kGroupRows - 1) / kGroupRows)) return;
     double *G = d.arena + F.fofs;
     double *W = d.work + F.wofs;
+    const double *gp = gpart + blockIdx.y * gpart_stride(nch);
     const int64_t wcol0 = ps - B.w_base;  // W column of the group's first pivot
-    const int tid = threadIdx.x, wv = tid >> 6, r = tid & 63;
-    // ---- diagonal region, wave 0 ----
-    {
-        double a[kGroupCols];
-        const bool mine = wv == 0 && r < ncols;
-        const int rr = ps + min(r, ncols - 1);
-        double *gdiag = gpart + blockIdx.y * gpart_stride(nch) + kGDiag;
-        if (STATS) {
-#pragma unroll
-            for (int c = 0; c < kGroupCols; ++c)
-                a[c] = masked(G[static_cast<int64_t>(ps + min(c, ncols - 1)) * f + rr], mine && c <= r);
-            if (bx == 0 && wv == 0) {
-#pragma unroll
-                for (int c = 0; c < kGroupCols; ++c) gdiag[c * kGroupCols + r] = a[c];
-            }
-        } else {
-#pragma unroll
-            for (int c = 0; c < kGroupCols; ++c) a[c] = masked(gdiag[c * kGroupCols + r], mine && c <= r);
-        }
-        if (wv == 0) gdiag_pivots<0>(a, r, ncols, Wd, Lm, invd, dd);
-        __syncthreads();
-        if (wv == 0 && 16 < ncols) {
-            gdeferred<0>(a, ncols, Wd, r, false);
-            gdiag_pivots<1>(a, r, ncols, Wd, Lm, invd, dd);
-        }
-        __syncthreads();
-        if (wv == 0 && 32 < ncols) {
-            gdeferred<1>(a, ncols, Wd, r, false);
-            gdiag_pivots<2>(a, r, ncols, Wd, Lm, invd, dd);
-        }
-        __syncthreads();
-        if (wv == 0 && 48 < ncols) {
-            gdeferred<2>(a, ncols, Wd, r, false);
-            gdiag_pivots<3>(a, r, ncols, Wd, Lm, invd, dd);
-        }
-        __syncthreads();
-        if (!STATS && bx == 0 && mine) {
-#pragma unroll
-            for (int c = 0; c < kGroupCols; ++c) {
-                if (c < ncols && c <= r) G[static_cast<int64_t>(ps + c) * f + ps + r] = a[c];
-                if (c < ncols && c < r) W[(wcol0 + c) * f + ps + r] = Wd[c][r];
+    const int tid = threadIdx.x;
+    for (int e = tid; e < kGroupCols * kGroupCols; e += kLT) Wd[e >> 6][e & 63] = gp[kGdWd + e];
+    for (int e = tid; e < kGroupCols * 16; e += kLT) Lm[e >> 4][e & 15] = gp[kGdLm + e];
+    if (tid < kGroupCols) invd[tid] = gp[kGdInv + tid];
+    __syncthreads();
+    if (!STATS && bx == 0) {
+        // the region: final entries and W (rows below each pivot)
+        for (int e = tid; e < kGroupCols * kGroupCols; e += kLT) {
+            const int c = e >> 6, rw = e & 63;
+            if (c < ncols && rw < ncols && c <= rw) {
+                G[static_cast<int64_t>(ps + c) * f + ps + rw] = gp[kGdFin + e];
+                if (c < rw) W[(wcol0 + c) * f + ps + rw] = Wd[c][rw];
             }
         }
     }
-    if (STATS && bx == 0 && tid < ncols) {
-        double mx = 0.0;
-        for (int rw = tid + 1; rw < ncols; ++rw) mx = fmax(mx, fabs(Wd[tid][rw]));
-        double *gp = gpart + blockIdx.y * gpart_stride(nch);
-        gp[tid] = mx;
-        gp[kGroupCols + tid] = dd[tid];
+    const int i = gend + bx * kGroupRows + tid;
+    const bool rowok = i < f;
+    const int il = min(i, f - 1);
+    double a[kGroupCols];
+#pragma unroll
+    for (int c = 0; c < kGroupCols; ++c)
+        a[c] = masked(G[static_cast<int64_t>(ps + min(c, ncols - 1)) * f + il], rowok && c < ncols);
+    grow_pivots<0, STATS, false>(a, ncols, Lm, invd, red, W, wcol0, f, i, rowok);
+    if (STATS) gpanel_max<0>(red, red2, wgmax);
+    if (16 < ncols) {
+        gdeferred<0, false>(a, ncols, Wd, 0, true);
+        grow_pivots<1, STATS, false>(a, ncols, Lm, invd, red, W, wcol0, f, i, rowok);
+        if (STATS) gpanel_max<1>(red, red2, wgmax);
     }
-    // ---- rows below the diagonal region, one per thread ----
-    {
-        const int i = gend + bx * kGroupRows + tid;
-        const bool rowok = i < f;
-        const int il = min(i, f - 1);
-        double a[kGroupCols];
+    if (32 < ncols) {
+        gdeferred<1, false>(a, ncols, Wd, 0, true);
+        grow_pivots<2, STATS, false>(a, ncols, Lm, invd, red, W, wcol0, f, i, rowok);
+        if (STATS) gpanel_max<2>(red, red2, wgmax);
+    }
+    if (48 < ncols) {
+        gdeferred<2, false>(a, ncols, Wd, 0, true);
+        grow_pivots<3, STATS, false>(a, ncols, Lm, invd, red, W, wcol0, f, i, rowok);
+        if (STATS) gpanel_max<3>(red, red2, wgmax);
+    }
+    if (STATS) {
+        __syncthreads();
+        if (tid < ncols) gpart[blockIdx.y * gpart_stride(nch) + kGMax + bx * kGroupCols + tid] = wgmax[tid];
+    } else if (rowok) {
 #pragma unroll
         for (int c = 0; c < kGroupCols; ++c)
-            a[c] = masked(G[static_cast<int64_t>(ps + min(c, ncols - 1)) * f + il], rowok && c < ncols);
-        grow_pivots<0, STATS>(a, ncols, Lm, invd, red, W, wcol0, f, i, rowok);
-        if (STATS) gpanel_max<0>(red, red2, wgmax);
-        if (16 < ncols) {
-            gdeferred<0>(a, ncols, Wd, 0, true);
-            grow_pivots<1, STATS>(a, ncols, Lm, invd, red, W, wcol0, f, i, rowok);
-            if (STATS) gpanel_max<1>(red, red2, wgmax);
-        }
-        if (32 < ncols) {
-            gdeferred<1>(a, ncols, Wd, 0, true);
-            grow_pivots<2, STATS>(a, ncols, Lm, invd, red, W, wcol0, f, i, rowok);
-            if (STATS) gpanel_max<2>(red, red2, wgmax);
-        }
-        if (48 < ncols) {
-            gdeferred<2>(a, ncols, Wd, 0, true);
-            grow_pivots<3, STATS>(a, ncols, Lm, invd, red, W, wcol0, f, i, rowok);
-            if (STATS) gpanel_max<3>(red, red2, wgmax);
-        }
-        if (STATS) {
-            __syncthreads();
-            if (tid < ncols) gpart[blockIdx.y * gpart_stride(nch) + kGMax + bx * kGroupCols + tid] = wgmax[tid];
-        } else if (rowok) {
-#pragma unroll
-            for (int c = 0; c < kGroupCols; ++c)
-                if (c < ncols) G[static_cast<int64_t>(ps + c) * f + i] = a[c];
-        }
+            if (c < ncols) G[static_cast<int64_t>(ps + c) * f + i] = a[c];
     }
 }
 
@@ -998,8 +1020,8 @@ __global__ __launch_bounds__(kLT) void k_big_panel(LdltDev d, BigLd *states, Prm
         __syncthreads();
         if (tid < kGroupCols) {
             const double pm = fmax(fmax(rs.v[tid], rs.v[tid + 64]), fmax(rs.v[tid + 128], rs.v[tid + 192]));
-            gmax[tid] = tid < gcols ? fmax(pm, gp[tid]) : 0.0;
-            gd[tid] = tid < gcols ? gp[kGroupCols + tid] : 0.0;
+            gmax[tid] = tid < gcols ? fmax(pm, gp[kGdMax + tid]) : 0.0;
+            gd[tid] = tid < gcols ? gp[kGdPiv + tid] : 0.0;
         }
         __syncthreads();
         if (tid == 0) {
@@ -1319,6 +1341,7 @@ void launch_big_panel(hipStream_t st, const LdltDev &d, BigLd *states, int32_t n
 
 void launch_big_group_stats(hipStream_t st, const LdltDev &d, const BigLd *states, int32_t nbig, double *gpart,
                             int32_t nch) {
+    k_big_gdiag<<<dim3(nbig), 64, 0, st>>>(d, states, gpart, nch);
     k_big_group<true><<<dim3(nch, nbig), kLT, 0, st>>>(d, states, gpart, nch);
 }
 

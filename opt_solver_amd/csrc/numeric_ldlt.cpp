@@ -153,7 +153,7 @@ static int ldlt_big_fronts(smfgpu_plan_s *P, const LdltDev &d, const std::vector
     int32_t maxf = 0;
     for (int32_t s : big) maxf = std::max(maxf, P->fl[s].f);
     const int32_t nch = (maxf + kGroupRows - 1) / kGroupRows + 1;
-    const size_t gstride = static_cast<size_t>(nch + 2 + kGroupCols) * kGroupCols;
+    const size_t gstride = static_cast<size_t>(nch) * kGroupCols + kGroupBuf;
     if (group && (rc = ensure(P->d_gpart, static_cast<size_t>(nbig) * gstride, false))) return rc;
     double *gpart = group ? P->d_gpart.p : nullptr;
     for (int64_t it = 0;; ++it) {

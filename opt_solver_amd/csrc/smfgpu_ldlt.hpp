@@ -79,6 +79,7 @@ struct BigLd {
 };
 constexpr int32_t kGroupCols = 64;   // columns of a speculative group (four panels of 16)
 constexpr int32_t kGroupRows = 256;  // rows per workgroup below a group's diagonal region
+constexpr int64_t kGroupBuf = 192 + 64 * 16 + 2 * 64 * 64;  // group buffer per front before the per-workgroup maxima
 
 void launch_ldlt_small(hipStream_t st, const LdltDev &d, const int32_t *flist, int32_t nf, double u, double eff_small,
                        double small_pivot);
