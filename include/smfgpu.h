@@ -224,6 +224,10 @@ SMFGPU_API int smfgpu_refine(smfgpu_factor_t f, const double *values, const doub
  * full search; the same pivots either way), 0 factors one panel at a time.
  * Groups need the panel width 16. */
 #define SMFGPU_TUNE_LDLT_GROUP 6
+/* SMFGPU_TUNE_LDLT_UPDATE_WG: workgroups of the large-front MFMA update
+ * launches, shared by the level's large fronts (a multiple of 8, default
+ * 1024; at least 64 per front). */
+#define SMFGPU_TUNE_LDLT_UPDATE_WG 7
 SMFGPU_API int smfgpu_plan_tune(smfgpu_plan_t plan, int32_t key, int64_t value);
 
 /* Device time (ms) of the last factorization's dominant kernel family:

@@ -64,6 +64,7 @@ TUNE_PANEL_OUTER = 3
 TUNE_CB_SEG = 4
 TUNE_LDLT_BIG_MIN = 5
 TUNE_LDLT_GROUP = 6
+TUNE_LDLT_UPDATE_WG = 7
 
 _lib = None
 

@@ -670,6 +670,21 @@ __device__ void big_tile(const LdltDev &d, const FrontLd &F, const double *Bm, i
     }
 }
 
+// Tiles [lo, hi) of a launch's range of nt for this workgroup's XCD, which
+// its workgroups stride over: workgroups are dispatched to the 8 XCDs round
+// robin by linear index, so with a multiple of 8 per front, blockIdx.x % 8
+// is the XCD. Each XCD then works through a contiguous run of tiles (one
+// column block's operand panel stays in its L2) instead of every eighth.
+struct XcdRange {
+    int lo, hi, step;
+};
+__device__ __forceinline__ XcdRange xcd_range(int nt) {
+    const int x = blockIdx.x & 7, per = gridDim.x >> 3;
+    const int lo = static_cast<int>(static_cast<int64_t>(nt) * x / 8);
+    const int hi = static_cast<int>(static_cast<int64_t>(nt) * (x + 1) / 8);
+    return XcdRange{lo + static_cast<int>(blockIdx.x >> 3), hi, per};
+}
+
 __device__ __forceinline__ int big_tiles(int f, int c0, int c1) {
     if (c1 <= c0) return 0;
     const int nr = (f - c0 + kT - 1) / kT, nc = (c1 - c0 + kT - 1) / kT;
@@ -1201,7 +1216,8 @@ __global__ __launch_bounds__(kLT) void k_big_update(LdltDev d, const BigLd *__re
     const double *W = d.work + F.wofs;
     const int t1 = (B.u_p1 > B.u_p0) ? big_tiles(F.f, B.u_c0, B.u_c1) : 0;
     const int t2 = (B.x_p1 > B.x_p0) ? big_tiles(F.f, B.x_c0, F.ne) : 0;
-    for (int tile = blockIdx.x; tile < t1 + t2; tile += gridDim.x) {
+    const XcdRange xr = xcd_range(t1 + t2);
+    for (int tile = xr.lo; tile < xr.hi; tile += xr.step) {
         if (tile < t1) big_tile(d, F, W, B.w_base, B.u_c0, B.u_c1, B.u_p0, B.u_p1, tile, smem);
         else big_tile(d, F, W, B.w_base, B.x_c0, F.ne, B.x_p0, B.x_p1, tile - t1, smem);
         __syncthreads();
@@ -1314,8 +1330,8 @@ __global__ __launch_bounds__(kLT) void k_big_cbupdate(LdltDev d, const BigLd *__
     const int c1 = min(B.k, c0 + width);
     if (c1 <= c0 || F.ne >= F.f) return;
     const double *W = d.work + F.wofs;
-    const int nt = big_tiles(F.f, F.ne, F.f);
-    for (int tile = blockIdx.x; tile < nt; tile += gridDim.x) {
+    const XcdRange xr = xcd_range(big_tiles(F.f, F.ne, F.f));
+    for (int tile = xr.lo; tile < xr.hi; tile += xr.step) {
         big_tile(d, F, W, c0, F.ne, F.f, c0, c1, tile, smem);
         __syncthreads();
     }

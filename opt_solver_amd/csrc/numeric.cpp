@@ -635,6 +635,11 @@ SMFGPU_API int smfgpu_plan_tune(smfgpu_plan_t P, int32_t key, int64_t value) {
         if (value != 0 && value != 1) return set_error(SMFGPU_ERR_INVALID_ARG, "tune: group must be 0 or 1");
         P->ldlt_group = static_cast<int32_t>(value);
         return SMFGPU_OK;
+    case SMFGPU_TUNE_LDLT_UPDATE_WG:
+        if (value < 8 || value % 8 != 0 || value > (1 << 20))
+            return set_error(SMFGPU_ERR_INVALID_ARG, "tune: update workgroups must be a multiple of 8");
+        P->ldlt_update_wg = static_cast<int32_t>(value);
+        return SMFGPU_OK;
     case SMFGPU_TUNE_SYRK_AHEAD:
         if (value != 1 && value != 2) return set_error(SMFGPU_ERR_INVALID_ARG, "tune: chunks ahead must be 1 or 2");
         P->syrk_ahead = static_cast<int32_t>(value);

@@ -147,7 +147,8 @@ static int ldlt_big_fronts(smfgpu_plan_s *P, const LdltDev &d, const std::vector
     int rc;
     if ((rc = ensure(P->d_big, static_cast<size_t>(nbig), false))) return rc;
     HIP_OK(hipMemcpyAsync(P->d_big.p, h.data(), nbig * sizeof(BigLd), hipMemcpyHostToDevice, st));
-    const int32_t per_front = std::max(64, 1024 / nbig);
+    // update workgroups per front: a multiple of 8 (one run of tiles per XCD)
+    const int32_t per_front = std::max(64, P->ldlt_update_wg / nbig) / 8 * 8;
     // speculative groups: workgroups per front over the rows below a
     // group's diagonal region, and their column maxima
     int32_t maxf = 0;

@@ -181,6 +181,7 @@ struct smfgpu_plan_s {
     int32_t ldlt_big_min = 256;                    // LDL^T: fronts with at least this many rows take the large-front path
     DBuf<BigLd> d_big;                             // LDL^T: large-front states of the level
     int32_t ldlt_group = 1;                        // LDL^T: speculative panel groups on the large-front path
+    int32_t ldlt_update_wg = 1024;                 // LDL^T: workgroups of the large-front update launches
     DBuf<double> d_gpart;                          // LDL^T: per-workgroup column maxima of a group
     int32_t syrk_ahead = 2;                        // Schur update: chunks with global loads in flight (1 or 2)
     int32_t solve_fuse_rows = 512;                 // rows below a last block its own workgroup updates
