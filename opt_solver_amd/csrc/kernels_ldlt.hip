@@ -618,31 +618,72 @@ void launch_ldlt_blocked(hipStream_t st, const LdltDev &d, const int32_t *flist,
 // and held back for the columns past it until a flush; sums differ from the
 // reference's only in order.
 
-// MFMA tile of C[i][j] -= sum_t A[i][t] * B[j][t - boff] over the lower
-// part of rows [c0, f) x columns [c0, c1) of front F; tile `tile` of the
-// region in column-block order.
-__device__ void big_tile(const LdltDev &d, const FrontLd &F, const double *Bm, int boff, int c0, int c1, int t0,
-                         int t1, int tile, unsigned char *smem) {
-    typedef SyrkMfma<double> M;
-    const int f = F.f;
-    const int nr = (f - c0 + kT - 1) / kT;
-    int cb = 0, rem = tile;
-    while (rem >= nr - cb) {
-        rem -= nr - cb;
-        ++cb;
+// Tile `tile` of the lower part (row block >= column block) of nrb x ncb
+// tile blocks, in 8 x 8 super-blocks: super-columns left to right, each
+// from its diagonal super-block down, tiles column-major inside a
+// super-block. Consecutive tiles (one XCD's run) then share 8 row and 8
+// column operand panels, which stay in its L2.
+__device__ __forceinline__ void super_block_tile(int nrb, int ncb, int tile, int &ib, int &jb) {
+    int sj = 0, rem = tile;
+    for (;; ++sj) {
+        const int w = min(8, ncb - 8 * sj), h = min(8, nrb - 8 * sj);
+        // diagonal super-block: column jj holds rows jj .. h-1
+        const int diag = w * h - w * (w - 1) / 2;
+        const int total = diag + w * (nrb - 8 * sj - h);
+        if (rem < total || 8 * (sj + 1) >= ncb) {
+            if (rem < diag) {
+                int jj = 0;
+                while (rem >= h - jj) {
+                    rem -= h - jj;
+                    ++jj;
+                }
+                ib = 8 * sj + jj + rem;
+                jb = 8 * sj + jj;
+                return;
+            }
+            rem -= diag;
+            const int below = nrb - 8 * sj - h;  // rows under the diagonal super-block
+            const int nf = below / 8, lr = below % 8;
+            const int base = 8 * sj + h;
+            if (rem < nf * 8 * w) {
+                const int b = rem / (8 * w), q = rem % (8 * w);
+                ib = base + 8 * b + q % 8;
+                jb = 8 * sj + q / 8;
+            } else {
+                const int q = rem - nf * 8 * w;
+                ib = base + 8 * nf + q % lr;
+                jb = 8 * sj + q / lr;
+            }
+            return;
+        }
+        rem -= total;
     }
-    const int i0 = c0 + (cb + rem) * kT, j0 = c0 + cb * kT;
+}
+
+// MFMA tile of C[i][j] -= sum_t A[i][t] * B[j][t - boff] over the lower
+// part of rows [c0, f) x columns [c0, c1) of front F (c1 - c0 <= f - c0);
+// tile `tile` of the region in super-block order. E = float: FP32 MFMA on
+// the FP64 operands (rounded as they enter LDS), FP32 accumulation over
+// the panel, subtracted from the FP64 entries.
+template <typename E>
+__device__ __forceinline__ void big_tile(const LdltDev &d, const FrontLd &F, const double *Bm, int boff, int c0, int c1, int t0,
+                         int t1, int tile, unsigned char *smem) {
+    typedef SyrkMfma<E> M;
+    const int f = F.f;
+    int ib, jb;
+    super_block_tile((f - c0 + kT - 1) / kT, (c1 - c0 + kT - 1) / kT, tile, ib, jb);
+    const int i0 = c0 + ib * kT, j0 = c0 + jb * kT;
     double *G = d.arena + F.fofs;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int wr = w >> 1, wc = w & 1;
     const bool active = (i0 + wr * 64 + 63) >= (j0 + wc * 64);
-    M::acc_t acc[4][4];
+    typename M::acc_t acc[4][4];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = M::zero();
-    if (active) tile_loop<double, true, 16, 2>(G, Bm, f, boff, i0, j0, f, c1, t0, t1, smem, acc);
-    else tile_loop<double, false, 16, 2>(G, Bm, f, boff, i0, j0, f, c1, t0, t1, smem, acc);
+    if (active) tile_loop<E, true, 16, 2, double>(G, Bm, f, boff, i0, j0, f, c1, t0, t1, smem, acc);
+    else tile_loop<E, false, 16, 2, double>(G, Bm, f, boff, i0, j0, f, c1, t0, t1, smem, acc);
     if (!active) return;
     double *junk = d.junk + threadIdx.x;
 #pragma unroll
@@ -1209,17 +1250,20 @@ __global__ __launch_bounds__(kLT) void k_big_panel(LdltDev d, BigLd *states, Prm
 
 // The requested update and flush of every front: grid (per_front, nbig),
 // tiles by grid stride.
-__global__ __launch_bounds__(kLT) void k_big_update(LdltDev d, const BigLd *__restrict__ states) {
+template <typename E>
+__global__ __launch_bounds__(kLT, 2) void k_big_update(LdltDev d, const BigLd *__restrict__ states) {
     __shared__ __attribute__((aligned(16))) unsigned char smem[4 * 16 * kT * sizeof(double)];
     const BigLd B = states[blockIdx.y];
+    if (B.prec != (sizeof(E) == 4 ? 1 : 0)) return;  // the other precision's launch
     const FrontLd F = d.fronts[B.s];
     const double *W = d.work + F.wofs;
     const int t1 = (B.u_p1 > B.u_p0) ? big_tiles(F.f, B.u_c0, B.u_c1) : 0;
     const int t2 = (B.x_p1 > B.x_p0) ? big_tiles(F.f, B.x_c0, F.ne) : 0;
     const XcdRange xr = xcd_range(t1 + t2);
     for (int tile = xr.lo; tile < xr.hi; tile += xr.step) {
-        if (tile < t1) big_tile(d, F, W, B.w_base, B.u_c0, B.u_c1, B.u_p0, B.u_p1, tile, smem);
-        else big_tile(d, F, W, B.w_base, B.x_c0, F.ne, B.x_p0, B.x_p1, tile - t1, smem);
+        const bool u = tile < t1;
+        big_tile<E>(d, F, W, B.w_base, u ? B.u_c0 : B.x_c0, u ? B.u_c1 : F.ne, u ? B.u_p0 : B.x_p0,
+                    u ? B.u_p1 : B.x_p1, u ? tile : tile - t1, smem);
         __syncthreads();
     }
 }
@@ -1322,17 +1366,19 @@ __global__ __launch_bounds__(kLT) void k_big_ld(LdltDev d, const BigLd *__restri
     }
 }
 
-__global__ __launch_bounds__(kLT) void k_big_cbupdate(LdltDev d, const BigLd *__restrict__ states, int32_t c0,
+template <typename E>
+__global__ __launch_bounds__(kLT, 2) void k_big_cbupdate(LdltDev d, const BigLd *__restrict__ states, int32_t c0,
                                                      int32_t width) {
     __shared__ __attribute__((aligned(16))) unsigned char smem[4 * 16 * kT * sizeof(double)];
     const BigLd B = states[blockIdx.y];
+    if (B.prec != (sizeof(E) == 4 ? 1 : 0)) return;
     const FrontLd F = d.fronts[B.s];
     const int c1 = min(B.k, c0 + width);
     if (c1 <= c0 || F.ne >= F.f) return;
     const double *W = d.work + F.wofs;
     const XcdRange xr = xcd_range(big_tiles(F.f, F.ne, F.f));
     for (int tile = xr.lo; tile < xr.hi; tile += xr.step) {
-        big_tile(d, F, W, c0, F.ne, F.f, c0, c1, tile, smem);
+        big_tile<E>(d, F, W, c0, F.ne, F.f, c0, c1, tile, smem);
         __syncthreads();
     }
 }
@@ -1366,9 +1412,11 @@ void launch_big_group_apply(hipStream_t st, const LdltDev &d, const BigLd *state
     k_big_group<false><<<dim3(nch, nbig), kLT, 0, st>>>(d, states, gpart, nch);
 }
 
-void launch_big_update(hipStream_t st, const LdltDev &d, const BigLd *states, int32_t nbig, int32_t per_front) {
+void launch_big_update(hipStream_t st, const LdltDev &d, const BigLd *states, int32_t nbig, int32_t per_front,
+                       int32_t n64, int32_t n32) {
     if (nbig <= 0) return;
-    k_big_update<<<dim3(per_front, nbig), kLT, 0, st>>>(d, states);
+    if (n64 > 0) k_big_update<double><<<dim3(per_front, nbig), kLT, 0, st>>>(d, states);
+    if (n32 > 0) k_big_update<float><<<dim3(per_front, nbig), kLT, 0, st>>>(d, states);
 }
 
 void launch_big_stall(hipStream_t st, const LdltDev &d, BigLd *states, int32_t nbig, int32_t nb) {
@@ -1377,10 +1425,11 @@ void launch_big_stall(hipStream_t st, const LdltDev &d, BigLd *states, int32_t n
 }
 
 void launch_big_cb(hipStream_t st, const LdltDev &d, const BigLd *states, int32_t nbig, int32_t c0, int32_t width,
-                   int32_t per_front) {
+                   int32_t per_front, int32_t n64, int32_t n32) {
     if (nbig <= 0) return;
     k_big_ld<<<dim3(per_front, nbig), kLT, 0, st>>>(d, states, c0, width);
-    k_big_cbupdate<<<dim3(per_front, nbig), kLT, 0, st>>>(d, states, c0, width);
+    if (n64 > 0) k_big_cbupdate<double><<<dim3(per_front, nbig), kLT, 0, st>>>(d, states, c0, width);
+    if (n32 > 0) k_big_cbupdate<float><<<dim3(per_front, nbig), kLT, 0, st>>>(d, states, c0, width);
 }
 
 void launch_big_finish(hipStream_t st, const LdltDev &d, const BigLd *states, int32_t nbig) {

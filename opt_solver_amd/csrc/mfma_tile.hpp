@@ -53,9 +53,11 @@ struct SyrkMfma<float> {
 // false).
 // A operand element (row i, panel column t) = Ab[t * ld + i], B operand
 // element (row j, t) = Bb[(t - boff) * ld + j]; rows i < f, j < j1 are
-// valid, panel columns [t0, t1).
-template <typename E, bool ACTIVE, int kKc, int kAhead>
-__device__ __forceinline__ void tile_loop(const E *Ab, const E *Bb, int ld, int boff, int i0, int j0, int f, int j1,
+// valid, panel columns [t0, t1). GE is the operands' type in global memory:
+// GE = double with E = float rounds FP64 operands to FP32 as they are
+// written to LDS (FP32 MFMA on FP64 data).
+template <typename E, bool ACTIVE, int kKc, int kAhead, typename GE = E>
+__device__ __forceinline__ void tile_loop(const GE *Ab, const GE *Bb, int ld, int boff, int i0, int j0, int f, int j1,
                                           int t0, int t1, unsigned char *smem,
                                           typename SyrkMfma<E>::acc_t (&acc)[4][4]) {
     typedef SyrkMfma<E> M;
@@ -67,7 +69,7 @@ __device__ __forceinline__ void tile_loop(const E *Ab, const E *Bb, int ld, int 
     const int lane = tid & 63;
     const int w = tid >> 6;
     const int wr = w >> 1, wc = w & 1;
-    typedef E Regs[kLoadPer];
+    typedef GE Regs[kLoadPer];
     // Unconditional loads from clamped addresses (finite front data); the
     // 0 / 1 masks for rows past the front, columns past j1 and panel
     // columns past t1 are applied when the chunk is written to LDS, after
@@ -77,8 +79,8 @@ __device__ __forceinline__ void tile_loop(const E *Ab, const E *Bb, int ld, int 
     const int ia = min(i0 + (tid % kT), f - 1);
     const int jb = min(j0 + (tid % kT), j1 - 1);
     auto gload = [&](Regs &ra, Regs &rb, int kc) {
-        const E *ca = Ab + static_cast<int64_t>(kc) * ld;
-        const E *cb = Bb + static_cast<int64_t>(kc - boff) * ld;
+        const GE *ca = Ab + static_cast<int64_t>(kc) * ld;
+        const GE *cb = Bb + static_cast<int64_t>(kc - boff) * ld;
         const int tlim = t1 - 1 - kc;
 #pragma unroll
         for (int e = 0; e < kLoadPer; ++e) {
@@ -94,8 +96,8 @@ __device__ __forceinline__ void tile_loop(const E *Ab, const E *Bb, int ld, int 
             const int idx = tid + kThreads * e;
             const int t = idx / kT, r = idx % kT;
             const bool tin = kc + t < t1;
-            As[stg][t][r] = ra[e] * static_cast<E>((tin && i0 + r < f) ? 1 : 0);
-            Bs[stg][t][r] = rb[e] * static_cast<E>((tin && j0 + r < j1) ? 1 : 0);
+            As[stg][t][r] = static_cast<E>(ra[e]) * static_cast<E>((tin && i0 + r < f) ? 1 : 0);
+            Bs[stg][t][r] = static_cast<E>(rb[e]) * static_cast<E>((tin && j0 + r < j1) ? 1 : 0);
         }
     };
     auto mfma_stage = [&](int stg) {

@@ -127,13 +127,14 @@ int ldlt_plan_build(smfgpu_plan_s *P, const smfgpu_analysis_desc *D) {
 // in chunks of kBigWork pivots and the fronts' results.
 static int ldlt_big_fronts(smfgpu_plan_s *P, const LdltDev &d, const std::vector<int32_t> &big,
                            const smfgpu_factor_options &opt, double eff_small, double small, int32_t nb,
-                           std::vector<BigLd> &h) {
+                           std::vector<BigLd> &h, int32_t &reduced) {
     constexpr int kBigCheck = 16;
     hipStream_t st = P->stream;
     const int32_t nbig = static_cast<int32_t>(big.size());
     const bool group = P->ldlt_group != 0 && nb == 16;
     h.assign(nbig, BigLd{});
     int64_t cap = 64;
+    int32_t n32 = 0;
     for (int32_t b = 0; b < nbig; ++b) {
         const FrontLd &F = P->fl[big[b]];
         BigLd &B = h[b];
@@ -142,8 +143,17 @@ static int ldlt_big_fronts(smfgpu_plan_s *P, const LdltDev &d, const std::vector
         B.k_paired = -1;
         B.oe = std::min(F.ne, kBigOuter);
         B.spec = group ? 1 : 0;
+        // mixed precision: FP32 MFMA updates on the fronts of at least
+        // mixed_min_front rows (the pivot logic stays FP64)
+        B.prec = (opt.precision == SMFGPU_PREC_FP32 || opt.precision == SMFGPU_PREC_BF16) &&
+                         F.f >= opt.mixed_min_front
+                     ? 1
+                     : 0;
+        n32 += B.prec;
         cap += 4 * (static_cast<int64_t>(F.ne) / std::max(nb, 1) + 2) + 2 * static_cast<int64_t>(F.ne);
     }
+    reduced += n32;
+    const int32_t n64 = nbig - n32;
     int rc;
     if ((rc = ensure(P->d_big, static_cast<size_t>(nbig), false))) return rc;
     HIP_OK(hipMemcpyAsync(P->d_big.p, h.data(), nbig * sizeof(BigLd), hipMemcpyHostToDevice, st));
@@ -161,7 +171,7 @@ static int ldlt_big_fronts(smfgpu_plan_s *P, const LdltDev &d, const std::vector
         if (group) launch_big_group_stats(st, d, P->d_big.p, nbig, gpart, nch);
         launch_big_panel(st, d, P->d_big.p, nbig, opt.pivot_u, eff_small, small, nb, gpart, nch);
         if (group) launch_big_group_apply(st, d, P->d_big.p, nbig, gpart, nch);
-        launch_big_update(st, d, P->d_big.p, nbig, per_front);
+        launch_big_update(st, d, P->d_big.p, nbig, per_front, n64, n32);
         launch_big_stall(st, d, P->d_big.p, nbig, nb);
         if (it % kBigCheck == kBigCheck - 1) {
             HIP_OK(hipMemcpyAsync(h.data(), P->d_big.p, nbig * sizeof(BigLd), hipMemcpyDeviceToHost, st));
@@ -175,7 +185,7 @@ static int ldlt_big_fronts(smfgpu_plan_s *P, const LdltDev &d, const std::vector
     HIP_OK(hipGetLastError());
     int32_t pmax = 0;
     for (const BigLd &B : h) pmax = std::max(pmax, B.k);
-    for (int32_t c0 = 0; c0 < pmax; c0 += kBigWork) launch_big_cb(st, d, P->d_big.p, nbig, c0, kBigWork, per_front);
+    for (int32_t c0 = 0; c0 < pmax; c0 += kBigWork) launch_big_cb(st, d, P->d_big.p, nbig, c0, kBigWork, per_front, n64, n32);
     launch_big_finish(st, d, P->d_big.p, nbig);
     HIP_OK(hipGetLastError());
     return SMFGPU_OK;
@@ -302,7 +312,7 @@ int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgp
     const int32_t nlev = static_cast<int32_t>(P->level_fronts.size());
     int rc;
     int64_t negatives = 0;
-    int32_t levels_done = 0;
+    int32_t levels_done = 0, reduced = 0;
     bool inertia_exceeded = false;
     P->rows_h.resize(0);
     for (int32_t lv = 0; lv < nlev; ++lv) {
@@ -454,7 +464,8 @@ int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgp
                           opt.pivot_u, eff_small, small);
         launch_ldlt_blocked(st, d, reinterpret_cast<const int32_t *>(S + o_blk),
                             static_cast<int32_t>(blocked_list.size()), opt.pivot_u, eff_small, small, nb);
-        if (!big_list.empty() && (rc = ldlt_big_fronts(P, d, big_list, opt, eff_small, small, nb, big_h))) return rc;
+        if (!big_list.empty() && (rc = ldlt_big_fronts(P, d, big_list, opt, eff_small, small, nb, big_h, reduced)))
+            return rc;
         launch_ldlt_epilogue(st, d, reinterpret_cast<const int32_t *>(S + o_ep), static_cast<int32_t>(ep_list.size()),
                              n_inline);
         launch_ldlt_store(st, d, reinterpret_cast<const int32_t *>(S + o_ep) + n_inline,
@@ -510,6 +521,7 @@ int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgp
         fent += static_cast<int64_t>(F.f) * F.pstore;
     }
     fi.factor_entries = fent;
+    fi.fronts_reduced = reduced;
     if (inertia_exceeded) {
         fi.status = SMFGPU_STATUS_INERTIA_EXCEEDED;  // factor_indef.cpp:682-715: unusable
         return SMFGPU_OK;
