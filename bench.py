@@ -9,7 +9,11 @@ path) of the whole system, values and right-hand side already resident in
 HBM. The symbolic analysis (host) and the device plan are built once before
 timing, as smf reuses an AnalysisKeep across refactors.
 
-Prints one JSON line (see the contract in DESIGN.md)."""
+Beside it, the same comparison on an indefinite KKT system of the
+configuration-2 generator (300^2 grid, 30,000 constraints, n = 120,000):
+threshold-pivoted LDL^T in FP64 vs FP32 MFMA updates + FP64 refinement.
+
+Prints one JSON line."""
 from __future__ import annotations
 
 import argparse
@@ -60,6 +64,58 @@ def cpu_baseline(m_cpu, threads):
                       f"fwd err {np.abs(xs - x).max():.1e}"}
 
 
+def kkt_measure(steps, warmup, gsz=300, m=30000):
+    """FP64 vs mixed LDL^T factor + solve (+ refinement) of the KKT system
+    of the configuration-2 generator, values and right-hand side resident on
+    the device; the library's ordering and analysis, built once."""
+    import torch
+    from opt_solver_amd import generators as g, ordering as o
+    from opt_solver_amd.solver import PREC_FP32, PREC_FP64, Plan, analyse
+    A, _ = g.kkt_grid(gsz, m, seed=1)
+    n, cp, ri = A[0], A[1], A[2]
+    nx = gsz * gsz
+    cols = np.repeat(np.arange(n), np.diff(cp))
+    sel = (ri >= nx) & (cols < nx)
+    order = np.argsort(ri[sel], kind="stable")
+    acols = cols[sel][order].reshape(m, 8)
+    an = analyse(A, perm=o.kkt_grid_nd(gsz, m, acols, leaf=64), nemin=32)
+    plan = Plan(an, posdef=False)
+    dev = torch.device("cuda:0")
+    x = np.random.default_rng(0).uniform(-1, 1, n)
+    vals_d = torch.from_numpy(np.ascontiguousarray(A[3])).to(dev)
+    b_d = torch.from_numpy(_matvec(A, x)).to(dev)
+    x_d = torch.empty_like(b_d)
+    torch.cuda.synchronize()
+    out = {}
+    for name, prec in (("fp64", PREC_FP64), ("mixed", PREC_FP32)):
+        def step():
+            fac = plan.factor(values_ptr=vals_d.data_ptr(), u=0.01, precision=prec, mixed_min_front=1024)
+            x_d.copy_(b_d)
+            torch.cuda.current_stream().synchronize()
+            fac.solve_device(x_d.data_ptr(), 1)
+            info = {"steps": 0}
+            if prec != PREC_FP64:
+                info = fac.refine_device(b_d.data_ptr(), x_d.data_ptr(), 1, max_iters=10, tol=1e-12)
+            return fac, info
+        for _ in range(warmup):
+            step()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            fac, info = step()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t) / steps
+        chk = fac.refine_device(b_d.data_ptr(), x_d.data_ptr(), 1, max_iters=0)
+        out[name] = {"ms_per_step": round(dt * 1e3, 2), "gflops": round(fac.info.actual_flops / dt / 1e9, 1),
+                     "eta": chk["eta"], "refine_steps": info.get("steps", 0), "inertia": list(fac.inertia),
+                     "delayed": int(fac.info.delayed_pivots), "fronts_reduced": int(fac.info.fronts_reduced)}
+        log("kkt", name, out[name])
+    return {"workload": f"kkt_grid {gsz}^2, m={m} (n={n}), u=0.01, geometric ND + cost-model amalgamation",
+            "max_front": an.max_front, "flops_per_factor": fac.info.actual_flops, "fp64": out["fp64"],
+            "mixed": out["mixed"],
+            "speedup_mixed_vs_fp64": round(out["fp64"]["ms_per_step"] / out["mixed"]["ms_per_step"], 3)}
+
+
 def _matvec(A, x):
     n, cp, ri, v = A
     cols = np.repeat(np.arange(n), np.diff(cp))
@@ -79,11 +135,12 @@ def main():
     ap.add_argument("--nrhs", type=int, default=1)
     ap.add_argument("--cpu-m", type=int, default=76)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-kkt", action="store_true", help="skip the indefinite KKT comparison")
     ap.add_argument("--copy-calib", action="store_true",
                     help="run one plain 1 GiB device copy first (scale for the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU is a design in DESIGN.md",
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU path is a design, not built",
               file=sys.stderr)
         sys.exit(2)
 
@@ -171,6 +228,9 @@ def main():
         for line in open(tfile):
             if line.startswith("traffic_bytes_per_launch_mixed"):
                 traffic = int(line.split("=")[1])
+    kkt = None
+    if not args.no_kkt:
+        kkt = kkt_measure(args.steps, args.warmup)
     cpu = None
     if not args.no_cpu:
         try:
@@ -205,6 +265,7 @@ def main():
                      "peak": PEAK["fp32"] / 1e12, "unit": "TFLOP/s", "frac": achieved / PEAK["fp32"],
                      "traffic": traffic},
         "cpu_baseline": cpu,
+        "kkt": kkt,
     }
     print(json.dumps(out))
 
