@@ -113,7 +113,11 @@ typedef struct {
     int32_t mixed_min_front; /* fronts with fewer rows stay FP64 (default 1024) */
     int32_t max_negative_pivots; /* < 0: off */
     int32_t value_mem;       /* SMFGPU_MEM_* of the values pointer */
-    int32_t reserved[8];
+    int32_t work_limit_on;   /* 1: stop LDL^T once front work would pass work_limit */
+    double work_limit;       /* sum of num_elig * f^2 allowed (the scale-on-delay
+                                threshold, multifrontal_serial.cpp:653-671);
+                                a stopped run reports SMFGPU_STATUS_MAX_PIVOT_DELAYS */
+    int32_t reserved[4];
 } smfgpu_factor_options;
 
 /* Factor statistics (the smf::Info fields of a factorization,

@@ -38,7 +38,7 @@ class FactorOptions(ctypes.Structure):
     _fields_ = [
         ("posdef", _I), ("pivot_u", _D), ("small_pivot", _D), ("panel", _I), ("precision", _I),
         ("mixed_min_front", _I), ("max_negative_pivots", _I), ("value_mem", _I),
-        ("reserved", _I * 8),
+        ("work_limit_on", _I), ("work_limit", _D), ("reserved", _I * 4),
     ]
 
 
@@ -58,6 +58,11 @@ EXPORTED = [
     "smfgpu_refine", "smfgpu_set_timing", "smfgpu_kernel_times", "smfgpu_plan_tune",
     "smfgpu_factor_front_stats", "smfgpu_launch_profile",
 ]
+STATUS_SUCCESS = 0
+STATUS_NOT_POSDEF = 1
+STATUS_SINGULAR = 2
+STATUS_MAX_PIVOT_DELAYS = 3
+STATUS_INERTIA_EXCEEDED = 4
 TUNE_SOLVE_FUSE_ROWS = 1
 TUNE_SYRK_AHEAD = 2
 TUNE_PANEL_OUTER = 3

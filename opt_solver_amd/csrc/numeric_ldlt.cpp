@@ -313,7 +313,17 @@ int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgp
     int rc;
     int64_t negatives = 0;
     int32_t levels_done = 0, reduced = 0;
-    bool inertia_exceeded = false;
+    bool inertia_exceeded = false, work_exceeded = false;
+    // work limit (multifrontal_serial.cpp:653-671, 1198-1205): the work of a
+    // front, num_elig * f^2, is known once its level is laid out; the first
+    // front that would take the total past floor(work_limit) is counted and
+    // the factorization stops before its level. Work only grows, so whether
+    // a run stops does not depend on the order the fronts are taken in.
+    const bool work_armed = opt.work_limit_on != 0 && opt.work_limit >= 0.0;
+    const int64_t work_cap = !work_armed ? -1
+                             : (opt.work_limit >= 9.0e18 ? INT64_MAX
+                                                         : static_cast<int64_t>(std::floor(opt.work_limit)));
+    int64_t work_done = 0;
     P->rows_h.resize(0);
     for (int32_t lv = 0; lv < nlev; ++lv) {
         const auto &fr = P->level_fronts[lv];
@@ -356,6 +366,11 @@ int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgp
                 blocked_list.push_back(s);
             } else {
                 small_list.push_back(s);
+            }
+            if (work_armed && !work_exceeded) {
+                const int64_t w = static_cast<int64_t>(F.ne) * F.f * F.f;
+                if (work_done + w > work_cap) work_exceeded = true;
+                work_done += w;
             }
             P->fac_used += static_cast<int64_t>(F.f) * F.ne;
             // large stored blocks get the many-workgroup store kernel
@@ -411,6 +426,7 @@ int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgp
             ids.push_back(s);
             patch.push_back(F);
         }
+        if (work_exceeded) break;  // stopped before this level (nothing of it launched)
         if (cc.size() > static_cast<size_t>(INT32_MAX)) return set_error(SMFGPU_ERR_ALLOC, "factor: level too large");
         // capacities (arena contents stay: offsets are relative)
         if ((rc = ensure(P->d_arena, static_cast<size_t>(P->sq_pool.end), true)) ||
@@ -524,6 +540,13 @@ int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgp
     fi.fronts_reduced = reduced;
     if (inertia_exceeded) {
         fi.status = SMFGPU_STATUS_INERTIA_EXCEEDED;  // factor_indef.cpp:682-715: unusable
+        return SMFGPU_OK;
+    }
+    if (work_exceeded) {
+        // factor_indef.cpp:655-680: stopped at the scale-on-delay threshold,
+        // the factor is unusable; the front work counts the stopping front
+        fi.status = SMFGPU_STATUS_MAX_PIVOT_DELAYS;
+        fi.front_work = static_cast<double>(work_done);
         return SMFGPU_OK;
     }
     fi.status = fi.num_zero > 0 ? SMFGPU_STATUS_SINGULAR : SMFGPU_STATUS_SUCCESS;

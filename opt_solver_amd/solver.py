@@ -132,7 +132,7 @@ class Plan:
         check(lib().smfgpu_plan_tune(self.h, key, int(value)), "plan_tune")
 
     def factor(self, values=None, values_ptr=None, precision=PREC_FP64, mixed_min_front=1024, u=0.01,
-               small_pivot=1e-20, panel=16, max_negative_pivots=-1) -> "Factor":
+               small_pivot=1e-20, panel=16, max_negative_pivots=-1, work_limit=None) -> "Factor":
         opt = _lib.FactorOptions()
         lib().smfgpu_default_options(ctypes.byref(opt))
         opt.posdef = 1 if self.posdef else 0
@@ -142,6 +142,9 @@ class Plan:
         opt.precision = precision
         opt.mixed_min_front = mixed_min_front
         opt.max_negative_pivots = max_negative_pivots
+        if work_limit is not None:
+            opt.work_limit_on = 1
+            opt.work_limit = float(work_limit)
         if values_ptr is not None:
             opt.value_mem = MEM_DEVICE
             vp = ctypes.c_void_p(values_ptr)

@@ -252,3 +252,28 @@ def test_mixed_precision_ldlt(oracle_lib, case):
     eta, _ = oracle_lib.backward_error(A, b, xr)
     print(f"  reduced {fac.info.fronts_reduced} refine steps {st['steps']} eta {eta:.2e}", flush=True)
     assert eta <= 1e-12, eta
+
+
+def test_work_limit(oracle_lib):
+    """Work limit (multifrontal_serial.cpp:653-671): a run stops, with
+    MaxPivotDelays and an unusable factor, exactly when its total front
+    work sum(num_elig * f^2) (realized sizes, delays included) exceeds
+    floor(work_limit); the reported work then passes the limit. With the
+    limit at the total, or off, the factor is the unlimited one."""
+    from opt_solver_amd._lib import STATUS_MAX_PIVOT_DELAYS
+    print("case work limit", flush=True)
+    A, arrays, inertia = _large_path_case("ocp_T200")
+    plan = Plan(arrays, posdef=False)
+    full = plan.factor(A[3], u=0.01)
+    total = full.info.front_work
+    assert full.status == 0 and full.info.delayed_pivots > 0 and total == int(total)
+    ref = full.export()
+    at = plan.factor(A[3], u=0.01, work_limit=total + 0.5)
+    assert at.status == 0 and at.inertia == inertia
+    assert np.array_equal(at.export()["factor_values"], ref["factor_values"])
+    for lim in (total - 1, total / 2, 0.0):
+        st = plan.factor(A[3], u=0.01, work_limit=lim)
+        assert st.status == STATUS_MAX_PIVOT_DELAYS, lim
+        assert st.info.front_work > lim
+        with pytest.raises(SmfGpuError):
+            st.solve(np.ones(A[0]))
