@@ -89,13 +89,16 @@ void launch_map_values(hipStream_t st, const double *vals, int64_t nvals, const 
 }
 
 // ---------------------------------------------------------------------------
-// A3: assembly. One workgroup per front column; the column is built in LDS
-// in row chunks: zero, original entries (CSC order), then each child's CB
-// column in child order (the reference's summation order per entry,
-// multifrontal_serial.cpp:730-789), and written out once, coalesced. Rows
-// above the diagonal of a pivot column are written as zeros (the stored
-// factor's strict upper part).
-constexpr int kAsmChunk = 2048;
+// A3: assembly. One workgroup per front column; the column (rows from the
+// diagonal down) is built in LDS in row chunks: zero, original entries (CSC
+// order), then each child's CB column in child order (the reference's
+// summation order per entry, multifrontal_serial.cpp:730-789), and written
+// out once, coalesced. The chunk holds the level's longest column up to
+// kAsmMaxChunk rows, so nearly every column is one chunk, each child column
+// a single coalesced pass. Rows above the diagonal of a pivot column are not
+// written: no kernel reads them (the diagonal blocks' upper triangles are
+// written by the Cholesky kernel first) and the export zeroes them.
+constexpr int kAsmMaxChunk = 16384;
 
 __device__ __forceinline__ int lower_bound_i32(const int32_t *a, int lo, int hi, int32_t key) {
     while (lo < hi) {
@@ -111,8 +114,8 @@ __device__ __forceinline__ int lower_bound_i32(const int32_t *a, int lo, int hi,
 constexpr int kAsmCursors = 128;
 
 __global__ __launch_bounds__(kThreads) void k_assemble(DevPtrs d, const AsmTask *__restrict__ tasks,
-                                                       const ChildCol *__restrict__ cc) {
-    __shared__ double buf[kAsmChunk + 1];  // [kAsmChunk]: target of masked-out adds
+                                                       const ChildCol *__restrict__ cc, int chunk) {
+    extern __shared__ double buf[];  // chunk + 1: [chunk] is the target of masked-out adds
     __shared__ int cursor[kAsmCursors];
     __shared__ int first_out;
     const AsmTask t = tasks[blockIdx.x];
@@ -121,14 +124,12 @@ __global__ __launch_bounds__(kThreads) void k_assemble(DevPtrs d, const AsmTask 
     const int f = F.f;
     double *colp = col_base(d, F, c);
     const int tid = threadIdx.x;
-    if (c < F.p)
-        for (int r = tid; r < c; r += kThreads) colp[r] = 0.0;
     // child column jj lands in parent column cm[jj]: its rows ii >= jj go to
     // the increasing rows cm[ii], so the first chunk starts at ii = jj and
     // each chunk continues where the previous one stopped
     for (int e = t.cc0 + tid; e < min(t.cc1, t.cc0 + kAsmCursors); e += kThreads) cursor[e - t.cc0] = cc[e].jj;
-    for (int r0 = c; r0 < f; r0 += kAsmChunk) {
-        const int r1 = min(f, r0 + kAsmChunk);
+    for (int r0 = c; r0 < f; r0 += chunk) {
+        const int r1 = min(f, r0 + chunk);
         for (int r = r0 + tid; r < r1; r += kThreads) buf[r - r0] = 0.0;
         __syncthreads();
         if (c < F.p) {
@@ -137,57 +138,91 @@ __global__ __launch_bounds__(kThreads) void k_assemble(DevPtrs d, const AsmTask 
             for (int k = k0 + tid; k < k1; k += kThreads) {
                 const int lr = d.amap[k];
                 const double v = d.av[k];
-                buf[(lr >= r0 && lr < r1) ? lr - r0 : kAsmChunk] += v;
+                buf[(lr >= r0 && lr < r1) ? lr - r0 : chunk] += v;
             }
             __syncthreads();
         }
-        for (int e = t.cc0; e < t.cc1; ++e) {
-            const ChildCol ch = cc[e];
-            const FrontDev C = d.fronts[ch.ch];
-            const int q = C.q;
-            const int32_t *cm = d.cmap + d.cmofs[ch.ch];
-            const double *cbcol = d.CB + C.cbofs + static_cast<int64_t>(ch.jj) * q;
-            const bool tracked = e - t.cc0 < kAsmCursors;
-            int cur = tracked ? cursor[e - t.cc0] : lower_bound_i32(cm, ch.jj, q, r0);
-            // rounds of 4 x kThreads entries; the entries of this chunk are
-            // a prefix (cm increases), its end found by a minimum in LDS
-            while (cur < q) {
-                if (tid == 0) first_out = 4 * kThreads;
-                int x[4];
-                double v[4];
+        if (r0 == c && r1 == f) {
+            // the whole column in one chunk: every entry of each child
+            // column lands in it (4 loads in flight per thread)
+            for (int e = t.cc0; e < t.cc1; ++e) {
+                const ChildCol ch = cc[e];
+                const FrontDev C = d.fronts[ch.ch];
+                const int q = C.q;
+                const int32_t *cm = d.cmap + d.cmofs[ch.ch];
+                const double *cbcol = d.CB + C.cbofs + static_cast<int64_t>(ch.jj) * q;
+                for (int i0 = ch.jj; i0 < q; i0 += 4 * kThreads) {
+                    int x[4];
+                    double v[4];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int ii = min(cur + u * kThreads + tid, q - 1);
-                    x[u] = cm[ii];
-                    v[u] = cbcol[ii];
+                    for (int u = 0; u < 4; ++u) {
+                        const int ii = min(i0 + u * kThreads + tid, q - 1);
+                        x[u] = cm[ii];
+                        v[u] = cbcol[ii];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) buf[i0 + u * kThreads + tid < q ? x[u] - r0 : chunk] += v[u];
                 }
                 __syncthreads();
-                int mine = 4 * kThreads;
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const bool take = cur + u * kThreads + tid < q && x[u] < r1;
-                    buf[take ? x[u] - r0 : kAsmChunk] += v[u];
-                    if (!take) mine = min(mine, u * kThreads + tid);
-                }
-                if (mine < 4 * kThreads) atomicMin(&first_out, mine);
-                __syncthreads();
-                const int taken = first_out;
-                __syncthreads();
-                cur += taken;
-                if (taken < 4 * kThreads) break;
             }
-            if (tracked && tid == 0) cursor[e - t.cc0] = cur;
+        } else {
+            for (int e = t.cc0; e < t.cc1; ++e) {
+                const ChildCol ch = cc[e];
+                const FrontDev C = d.fronts[ch.ch];
+                const int q = C.q;
+                const int32_t *cm = d.cmap + d.cmofs[ch.ch];
+                const double *cbcol = d.CB + C.cbofs + static_cast<int64_t>(ch.jj) * q;
+                const bool tracked = e - t.cc0 < kAsmCursors;
+                int cur = tracked ? cursor[e - t.cc0] : lower_bound_i32(cm, ch.jj, q, r0);
+                // rounds of 4 x kThreads entries; the entries of this chunk
+                // are a prefix (cm increases), its end found by a minimum in
+                // LDS
+                while (cur < q) {
+                    if (tid == 0) first_out = 4 * kThreads;
+                    int x[4];
+                    double v[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int ii = min(cur + u * kThreads + tid, q - 1);
+                        x[u] = cm[ii];
+                        v[u] = cbcol[ii];
+                    }
+                    __syncthreads();
+                    int mine = 4 * kThreads;
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const bool take = cur + u * kThreads + tid < q && x[u] < r1;
+                        buf[take ? x[u] - r0 : chunk] += v[u];
+                        if (!take) mine = min(mine, u * kThreads + tid);
+                    }
+                    if (mine < 4 * kThreads) atomicMin(&first_out, mine);
+                    __syncthreads();
+                    const int taken = first_out;
+                    __syncthreads();
+                    cur += taken;
+                    if (taken < 4 * kThreads) break;
+                }
+                if (tracked && tid == 0) cursor[e - t.cc0] = cur;
+            }
+            __syncthreads();
         }
-        __syncthreads();
         for (int r = r0 + tid; r < r1; r += kThreads) colp[r] = buf[r - r0];
         __syncthreads();
     }
 }
 
 void launch_assemble(hipStream_t st, const DevPtrs &d, const AsmTask *tasks, int64_t ntasks,
-                     const ChildCol *cc) {
+                     const ChildCol *cc, int32_t fmax) {
     if (ntasks <= 0) return;
-    k_assemble<<<dim3(static_cast<unsigned>(ntasks)), kThreads, 0, st>>>(d, tasks, cc);
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_assemble),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (kAsmMaxChunk + 1) * sizeof(double));
+        attr = true;
+    }
+    const int chunk = std::max(1, std::min(fmax, kAsmMaxChunk));
+    k_assemble<<<dim3(static_cast<unsigned>(ntasks)), kThreads, (chunk + 1) * sizeof(double), st>>>(d, tasks, cc,
+                                                                                                  chunk);
 }
 
 // ---------------------------------------------------------------------------

@@ -760,7 +760,12 @@ SMFGPU_API int smfgpu_factor(smfgpu_plan_t P, const double *values, int64_t nval
             const bool rec = rec_all && (l.kind == K_SYRK || l.kind == K_ASSEMBLE);
             if (rec) mark(2 * li);
             switch (l.kind) {
-            case K_ASSEMBLE: launch_assemble(st, d, P->d_asm.p + l.first, l.count, P->d_cc.p); break;
+            case K_ASSEMBLE: {
+                int32_t fmax = 1;
+                for (int32_t s : P->level_fronts[l.level]) fmax = std::max(fmax, P->fronts[s].f);
+                launch_assemble(st, d, P->d_asm.p + l.first, l.count, P->d_cc.p, fmax);
+                break;
+            }
             case K_POTRF: launch_potrf(st, d, P->d_potrf.p + l.first, l.count); break;
             case K_TRSM: launch_trsm(st, d, P->d_trsm.p + l.first, l.count); break;
             case K_SYRK: launch_syrk(st, d, P->d_syrk.p + l.first, l.count, P->syrk_ahead); break;

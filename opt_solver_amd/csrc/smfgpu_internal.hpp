@@ -112,8 +112,9 @@ struct DevPtrs {
 
 void launch_map_values(hipStream_t st, const double *vals, int64_t nvals, const int64_t *slot_ptr,
                        const int64_t *slot_src, int64_t nslots, double *av, double *redbuf);
-void launch_assemble(hipStream_t st, const DevPtrs &d, const AsmTask *tasks, int64_t ntasks,
-                     const ChildCol *cc);
+// fmax: the longest front of the launch's level (sizes the LDS column chunk)
+void launch_assemble(hipStream_t st, const DevPtrs &d, const AsmTask *tasks, int64_t ntasks, const ChildCol *cc,
+                     int32_t fmax);
 void launch_potrf(hipStream_t st, const DevPtrs &d, const PotrfItem *it, int64_t n);
 void launch_trsm(hipStream_t st, const DevPtrs &d, const TrsmItem *it, int64_t n);
 void launch_syrk(hipStream_t st, const DevPtrs &d, const SyrkItem *it, int64_t n, int kc);
