@@ -463,10 +463,14 @@ __device__ __forceinline__ void syrk_tile(const DevPtrs &d, const SyrkItem &it, 
 constexpr int kSyrkKc64 = 16;
 constexpr int kSyrkKc32 = 32;
 
+// One kernel for both precisions: the small FP64 fronts and the large FP32
+// ones of a step share one launch (two launches per step serialise them).
 template <int kAhead>
 __global__ __launch_bounds__(kThreads) void k_syrk(DevPtrs d, const SyrkItem *__restrict__ items) {
-    __shared__ __attribute__((aligned(16))) unsigned char smem[4 * kSyrkKc64 * kT * sizeof(double)];
-    static_assert(kSyrkKc32 * sizeof(float) == kSyrkKc64 * sizeof(double), "LDS stages of equal size");
+    constexpr int kBytes = tile_lds_bytes<double, kSyrkKc64>() > tile_lds_bytes<float, kSyrkKc32>()
+                               ? tile_lds_bytes<double, kSyrkKc64>()
+                               : tile_lds_bytes<float, kSyrkKc32>();
+    __shared__ __attribute__((aligned(16))) unsigned char smem[kBytes];
     const SyrkItem it = items[xcd_contiguous(blockIdx.x, gridDim.x)];
     const int prec = d.prec != nullptr ? d.prec[it.s] : 0;
     if (prec == 0) syrk_tile<double, kSyrkKc64, kAhead>(d, it, smem);

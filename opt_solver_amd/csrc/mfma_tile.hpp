@@ -44,6 +44,15 @@ struct SyrkMfma<float> {
     static __device__ __forceinline__ int row(int v, int lane) { return 4 * (lane >> 4) + v; }
 };
 
+// LDS row length of a staged chunk: FP32 rows are padded by 16 floats so
+// that the four panel columns a wave reads at once (lanes 16 k .. 16 k + 15
+// take column k) fall on different banks; FP64 reads do not conflict.
+template <typename E>
+__host__ __device__ constexpr int tile_lds_ld() { return kT + (sizeof(E) == 4 ? 16 : 0); }
+// LDS bytes of the two stages of both operands
+template <typename E, int kKc>
+__host__ __device__ constexpr int tile_lds_bytes() { return 4 * kKc * tile_lds_ld<E>() * static_cast<int>(sizeof(E)); }
+
 // One 128 x 128 tile, operands of type E staged in a double-buffered LDS
 // ring. kAhead = 1: the global loads of chunk k+1 are in flight during the
 // MFMAs of chunk k; kAhead = 2: those of chunk k+2 (two register sets, the
@@ -63,8 +72,9 @@ __device__ __forceinline__ void tile_loop(const GE *Ab, const GE *Bb, int ld, in
     typedef SyrkMfma<E> M;
     constexpr int kThreads = kTileThreads;
     constexpr int kLoadPer = (kKc * kT) / kThreads;  // elements of A (and of B) per thread per chunk
-    E (*As)[kKc][kT] = reinterpret_cast<E (*)[kKc][kT]>(smem);                            // [2][kKc][kT]
-    E (*Bs)[kKc][kT] = reinterpret_cast<E (*)[kKc][kT]>(smem + 2 * kKc * kT * sizeof(E));  // [2][kKc][kT]
+    constexpr int kLd = tile_lds_ld<E>();
+    E (*As)[kKc][kLd] = reinterpret_cast<E (*)[kKc][kLd]>(smem);                             // [2][kKc][kLd]
+    E (*Bs)[kKc][kLd] = reinterpret_cast<E (*)[kKc][kLd]>(smem + 2 * kKc * kLd * sizeof(E));  // [2][kKc][kLd]
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int w = tid >> 6;

@@ -117,7 +117,7 @@ void launch_assemble(hipStream_t st, const DevPtrs &d, const AsmTask *tasks, int
                      int32_t fmax);
 void launch_potrf(hipStream_t st, const DevPtrs &d, const PotrfItem *it, int64_t n);
 void launch_trsm(hipStream_t st, const DevPtrs &d, const TrsmItem *it, int64_t n);
-void launch_syrk(hipStream_t st, const DevPtrs &d, const SyrkItem *it, int64_t n, int kc);
+void launch_syrk(hipStream_t st, const DevPtrs &d, const SyrkItem *it, int64_t n, int ahead);
 void launch_zero_upper(hipStream_t st, const DevPtrs &d, const PotrfItem *it, int64_t n);
 
 // Solves: y is the permuted right-hand side block, n x nrhs (ld n).
