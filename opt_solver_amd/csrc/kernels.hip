@@ -46,11 +46,12 @@ __device__ __forceinline__ int xcd_contiguous(int b, int n) {
 __global__ void k_map_values(const double *__restrict__ vals, const int64_t *__restrict__ slot_ptr,
                              const int64_t *__restrict__ slot_src, int64_t nslots,
                              double *__restrict__ av) {
-    const int64_t s = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (s >= nslots) return;
-    double acc = 0.0;
-    for (int64_t k = slot_ptr[s]; k < slot_ptr[s + 1]; ++k) acc += vals[slot_src[k]];
-    av[s] = acc;
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t s = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; s < nslots; s += stride) {
+        double acc = 0.0;
+        for (int64_t k = slot_ptr[s]; k < slot_ptr[s + 1]; ++k) acc += vals[slot_src[k]];
+        av[s] = acc;
+    }
 }
 
 __global__ void k_absmax(const double *__restrict__ v, int64_t n, unsigned long long *out) {
@@ -84,8 +85,8 @@ void launch_map_values(hipStream_t st, const double *vals, int64_t nvals, const 
         k_absmax<<<dim3(static_cast<unsigned>(g)), kThreads, 0, st>>>(vals, nvals, rb);
     }
     if (nslots > 0)
-        k_map_values<<<dim3(static_cast<unsigned>((nslots + kThreads - 1) / kThreads)), kThreads, 0, st>>>(
-            vals, slot_ptr, slot_src, nslots, av);
+        k_map_values<<<dim3(static_cast<unsigned>(std::min<int64_t>((nslots + kThreads - 1) / kThreads, 1 << 20))),
+                       kThreads, 0, st>>>(vals, slot_ptr, slot_src, nslots, av);
 }
 
 // ---------------------------------------------------------------------------
@@ -221,8 +222,9 @@ void launch_assemble(hipStream_t st, const DevPtrs &d, const AsmTask *tasks, int
         attr = true;
     }
     const int chunk = std::max(1, std::min(fmax, kAsmMaxChunk));
-    k_assemble<<<dim3(static_cast<unsigned>(ntasks)), kThreads, (chunk + 1) * sizeof(double), st>>>(d, tasks, cc,
-                                                                                                  chunk);
+    for (int64_t o = 0; o < ntasks; o += kMaxLaunchGroups)
+        k_assemble<<<dim3(static_cast<unsigned>(std::min(kMaxLaunchGroups, ntasks - o))), kThreads,
+                     (chunk + 1) * sizeof(double), st>>>(d, tasks + o, cc, chunk);
 }
 
 // ---------------------------------------------------------------------------
@@ -318,7 +320,8 @@ __global__ __launch_bounds__(kThreads) void k_potrf(DevPtrs d, const PotrfItem *
 
 void launch_potrf(hipStream_t st, const DevPtrs &d, const PotrfItem *it, int64_t n) {
     if (n <= 0) return;
-    k_potrf<<<dim3(static_cast<unsigned>(n)), kThreads, 0, st>>>(d, it);
+    for (int64_t o = 0; o < n; o += kMaxLaunchGroups)
+        k_potrf<<<dim3(static_cast<unsigned>(std::min(kMaxLaunchGroups, n - o))), kThreads, 0, st>>>(d, it + o);
 }
 
 // Panel TRSM: rows [r0, r0 + 64) of columns [k, k + kb): X = A M, with
@@ -390,7 +393,8 @@ __global__ __launch_bounds__(kThreads) void k_trsm(DevPtrs d, const TrsmItem *__
 
 void launch_trsm(hipStream_t st, const DevPtrs &d, const TrsmItem *it, int64_t n) {
     if (n <= 0) return;
-    k_trsm<<<dim3(static_cast<unsigned>(n)), kThreads, 0, st>>>(d, it);
+    for (int64_t o = 0; o < n; o += kMaxLaunchGroups)
+        k_trsm<<<dim3(static_cast<unsigned>(std::min(kMaxLaunchGroups, n - o))), kThreads, 0, st>>>(d, it + o);
 }
 
 // ---------------------------------------------------------------------------
@@ -479,8 +483,11 @@ __global__ __launch_bounds__(kThreads) void k_syrk(DevPtrs d, const SyrkItem *__
 
 void launch_syrk(hipStream_t st, const DevPtrs &d, const SyrkItem *it, int64_t n, int ahead) {
     if (n <= 0) return;
-    if (ahead == 1) k_syrk<1><<<dim3(static_cast<unsigned>(n)), kThreads, 0, st>>>(d, it);
-    else k_syrk<2><<<dim3(static_cast<unsigned>(n)), kThreads, 0, st>>>(d, it);
+    for (int64_t o = 0; o < n; o += kMaxLaunchGroups) {
+        const dim3 grid(static_cast<unsigned>(std::min(kMaxLaunchGroups, n - o)));
+        if (ahead == 1) k_syrk<1><<<grid, kThreads, 0, st>>>(d, it + o);
+        else k_syrk<2><<<grid, kThreads, 0, st>>>(d, it + o);
+    }
 }
 
 void launch_zero_upper(hipStream_t, const DevPtrs &, const PotrfItem *, int64_t) {}

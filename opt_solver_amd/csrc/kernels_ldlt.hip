@@ -432,7 +432,9 @@ void launch_ldlt_small(hipStream_t st, const LdltDev &d, const int32_t *flist, i
                        double small_pivot) {
     if (nf <= 0) return;
     Prm prm{u, eff_small, small_pivot};
-    k_ldlt_small<<<dim3(nf), kLT, 0, st>>>(d, flist, prm);
+    for (int64_t o = 0; o < nf; o += kMaxLaunchGroups)
+        k_ldlt_small<<<dim3(static_cast<unsigned>(std::min<int64_t>(kMaxLaunchGroups, nf - o))), kLT, 0, st>>>(
+            d, flist + o, prm);
 }
 
 // ---------------------------------------------------------------------------
@@ -604,7 +606,9 @@ void launch_ldlt_blocked(hipStream_t st, const LdltDev &d, const int32_t *flist,
                          double eff_small, double small_pivot, int32_t nb) {
     if (nf <= 0) return;
     Prm prm{u, eff_small, small_pivot};
-    k_ldlt_blocked<<<dim3(nf), kLT, 0, st>>>(d, flist, prm, nb);
+    for (int64_t o = 0; o < nf; o += kMaxLaunchGroups)
+        k_ldlt_blocked<<<dim3(static_cast<unsigned>(std::min<int64_t>(kMaxLaunchGroups, nf - o))), kLT, 0, st>>>(
+            d, flist + o, prm, nb);
 }
 
 // ---------------------------------------------------------------------------
@@ -1496,7 +1500,9 @@ __global__ __launch_bounds__(kLT) void k_ldlt_assemble(LdltDev d, const AsmTask 
 
 void launch_ldlt_assemble(hipStream_t st, const LdltDev &d, const AsmTask *tasks, int64_t ntasks, const ChildCol *cc) {
     if (ntasks <= 0) return;
-    k_ldlt_assemble<<<dim3(static_cast<unsigned>(ntasks)), kLT, 0, st>>>(d, tasks, cc);
+    for (int64_t o = 0; o < ntasks; o += kMaxLaunchGroups)
+        k_ldlt_assemble<<<dim3(static_cast<unsigned>(std::min(kMaxLaunchGroups, ntasks - o))), kLT, 0, st>>>(
+            d, tasks + o, cc);
 }
 
 // ---------------------------------------------------------------------------
@@ -1615,7 +1621,11 @@ void launch_front_patch(hipStream_t st, FrontLd *fronts, const FrontLd *patch, c
 
 void launch_ldlt_epilogue(hipStream_t st, const LdltDev &d, const int32_t *flist, int32_t nf, int32_t n_inline) {
     if (nf <= 0) return;
-    k_ldlt_epilogue<<<dim3(nf), kLT, 0, st>>>(d, flist, n_inline);
+    for (int64_t o = 0; o < nf; o += kMaxLaunchGroups) {
+        const int64_t m = std::min<int64_t>(kMaxLaunchGroups, nf - o);
+        const int32_t inl = static_cast<int32_t>(std::max<int64_t>(0, std::min<int64_t>(m, n_inline - o)));
+        k_ldlt_epilogue<<<dim3(static_cast<unsigned>(m)), kLT, 0, st>>>(d, flist + o, inl);
+    }
 }
 
 void launch_ldlt_store(hipStream_t st, const LdltDev &d, const int32_t *flist, int32_t nf, int32_t max_cols) {
@@ -1652,7 +1662,10 @@ __global__ void k_ldlt_diag(LdltDev d, const int32_t *__restrict__ flist, int32_
 void launch_ldlt_diag(hipStream_t st, const LdltDev &d, const int32_t *flist, int32_t nf, double *y, int32_t n,
                       int32_t nrhs) {
     if (nf <= 0) return;
-    k_ldlt_diag<<<dim3(nf), 64, 0, st>>>(d, flist, nf, y, n, nrhs);
+    for (int64_t o = 0; o < nf; o += kMaxLaunchGroups) {
+        const int32_t m = static_cast<int32_t>(std::min<int64_t>(kMaxLaunchGroups, nf - o));
+        k_ldlt_diag<<<dim3(static_cast<unsigned>(m)), 64, 0, st>>>(d, flist + o, m, y, n, nrhs);
+    }
 }
 
 }  // namespace smfgpu

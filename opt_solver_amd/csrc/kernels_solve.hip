@@ -475,29 +475,35 @@ __global__ __launch_bounds__(kLThreads) void k_ldlt_bwd_step(LdltDev d, const So
 void launch_ldlt_fwd_step(hipStream_t st, const LdltDev &d, const SolveItem *items, int64_t count, double *y,
                           int32_t n, int32_t nrhs) {
     if (count <= 0) return;
-    k_ldlt_fwd_step<<<dim3(static_cast<unsigned>(count)), kLThreads, 0, st>>>(d, items, y, n, nrhs);
+    for (int64_t o = 0; o < count; o += kMaxLaunchGroups)
+        k_ldlt_fwd_step<<<dim3(static_cast<unsigned>(std::min(kMaxLaunchGroups, count - o))), kLThreads, 0, st>>>(
+            d, items + o, y, n, nrhs);
 }
 
 void launch_ldlt_bwd_step(hipStream_t st, const LdltDev &d, const SolveItem *items, int64_t count, double *y,
                           int32_t n, int32_t nrhs) {
     if (count <= 0) return;
-    k_ldlt_bwd_step<<<dim3(static_cast<unsigned>(count)), kLThreads, 0, st>>>(d, items, y, n, nrhs);
+    for (int64_t o = 0; o < count; o += kMaxLaunchGroups)
+        k_ldlt_bwd_step<<<dim3(static_cast<unsigned>(std::min(kMaxLaunchGroups, count - o))), kLThreads, 0, st>>>(
+            d, items + o, y, n, nrhs);
 }
 
 void launch_fwd_step(hipStream_t st, const DevPtrs &d, const SolveItem *items, int64_t count, double *y, int32_t n,
                      int32_t nrhs) {
-    if (count <= 0) return;
-    const dim3 grid(static_cast<unsigned>(count));
-    if (nrhs == 1) k_fwd_step<1><<<grid, kSThreads, 0, st>>>(d, items, y, n, nrhs);
-    else k_fwd_step<4><<<grid, kSThreads, 0, st>>>(d, items, y, n, nrhs);
+    for (int64_t o = 0; o < count; o += kMaxLaunchGroups) {
+        const dim3 grid(static_cast<unsigned>(std::min(kMaxLaunchGroups, count - o)));
+        if (nrhs == 1) k_fwd_step<1><<<grid, kSThreads, 0, st>>>(d, items + o, y, n, nrhs);
+        else k_fwd_step<4><<<grid, kSThreads, 0, st>>>(d, items + o, y, n, nrhs);
+    }
 }
 
 void launch_bwd_step(hipStream_t st, const DevPtrs &d, const SolveItem *items, int64_t count, double *y, int32_t n,
                      int32_t nrhs) {
-    if (count <= 0) return;
-    const dim3 grid(static_cast<unsigned>(count));
-    if (nrhs == 1) k_bwd_step<1><<<grid, kSThreads, 0, st>>>(d, items, y, n, nrhs);
-    else k_bwd_step<4><<<grid, kSThreads, 0, st>>>(d, items, y, n, nrhs);
+    for (int64_t o = 0; o < count; o += kMaxLaunchGroups) {
+        const dim3 grid(static_cast<unsigned>(std::min(kMaxLaunchGroups, count - o)));
+        if (nrhs == 1) k_bwd_step<1><<<grid, kSThreads, 0, st>>>(d, items + o, y, n, nrhs);
+        else k_bwd_step<4><<<grid, kSThreads, 0, st>>>(d, items + o, y, n, nrhs);
+    }
 }
 
 }  // namespace smfgpu

@@ -20,13 +20,16 @@ constexpr int64_t kStoreWide = int64_t{1} << 20;  // stored entries from which a
 namespace {
 
 template <class T>
-int ensure(DBuf<T> &b, size_t need, bool keep) {
+int ensure(DBuf<T> &b, size_t need, bool keep, const char *what = "buffer") {
     if (b.n >= need) return SMFGPU_OK;
     size_t cap = std::max(need, b.n + b.n / 2);
-    if (!keep) return b.alloc(cap);
+    if (!keep) {
+        const int rc = b.alloc(cap);
+        return rc ? set_error(rc, std::string("factor: ") + what + ": " + last_error()) : SMFGPU_OK;
+    }
     DBuf<T> nb;
     int rc = nb.alloc(cap);
-    if (rc) return rc;
+    if (rc) return set_error(rc, std::string("factor: ") + what + ": " + last_error());
     if (b.n && hipMemcpy(nb.p, b.p, b.n * sizeof(T), hipMemcpyDeviceToDevice) != hipSuccess)
         return set_error(SMFGPU_ERR_DEVICE, "arena growth copy failed");
     std::swap(b.p, nb.p);
@@ -155,7 +158,7 @@ static int ldlt_big_fronts(smfgpu_plan_s *P, const LdltDev &d, const std::vector
     reduced += n32;
     const int32_t n64 = nbig - n32;
     int rc;
-    if ((rc = ensure(P->d_big, static_cast<size_t>(nbig), false))) return rc;
+    if ((rc = ensure(P->d_big, static_cast<size_t>(nbig), false, "large-front states"))) return rc;
     HIP_OK(hipMemcpyAsync(P->d_big.p, h.data(), nbig * sizeof(BigLd), hipMemcpyHostToDevice, st));
     // update workgroups per front: a multiple of 8 (one run of tiles per XCD)
     const int32_t per_front = std::max(64, P->ldlt_update_wg / nbig) / 8 * 8;
@@ -165,7 +168,7 @@ static int ldlt_big_fronts(smfgpu_plan_s *P, const LdltDev &d, const std::vector
     for (int32_t s : big) maxf = std::max(maxf, P->fl[s].f);
     const int32_t nch = (maxf + kGroupRows - 1) / kGroupRows + 1;
     const size_t gstride = static_cast<size_t>(nch) * kGroupCols + kGroupBuf;
-    if (group && (rc = ensure(P->d_gpart, static_cast<size_t>(nbig) * gstride, false))) return rc;
+    if (group && (rc = ensure(P->d_gpart, static_cast<size_t>(nbig) * gstride, false, "group buffer"))) return rc;
     double *gpart = group ? P->d_gpart.p : nullptr;
     for (int64_t it = 0;; ++it) {
         if (group) launch_big_group_stats(st, d, P->d_big.p, nbig, gpart, nch);
@@ -265,8 +268,8 @@ static int ldlt_solve_lists(smfgpu_plan_s *P) {
     for (const auto &st : P->bwd_steps)
         if (st.second > INT32_MAX) return set_error(SMFGPU_ERR_ALLOC, "factor: solve step too large");
     int rc;
-    if ((rc = ensure(P->d_fwd_items, std::max<size_t>(fwd_items.size(), 1), false)) ||
-        (rc = ensure(P->d_bwd_items, std::max<size_t>(bwd_items.size(), 1), false)))
+    if ((rc = ensure(P->d_fwd_items, std::max<size_t>(fwd_items.size(), 1), false, "forward solve items")) ||
+        (rc = ensure(P->d_bwd_items, std::max<size_t>(bwd_items.size(), 1), false, "backward solve items")))
         return rc;
     if (!fwd_items.empty())
         HIP_OK(hipMemcpyAsync(P->d_fwd_items.p, fwd_items.data(), fwd_items.size() * sizeof(SolveItem),
@@ -344,7 +347,17 @@ int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgp
             const FrontDev &Sy = P->fronts[s];
             FrontLd F{};
             int32_t nd = 0;
-            for (int32_t c = P->child_ptr[s]; c < P->child_ptr[s + 1]; ++c) nd += P->fl[P->child_idx[c]].delayed;
+            for (int32_t c = P->child_ptr[s]; c < P->child_ptr[s + 1]; ++c) {
+                const FrontLd &C = P->fl[P->child_idx[c]];
+                // a child passes at most its eligible columns up
+                if (C.delayed < 0 || C.delayed > C.ne || C.f > n)
+                    return set_error(SMFGPU_ERR_DEVICE, "factor: front " + std::to_string(P->child_idx[c]) +
+                                                            " came back inconsistent (f " + std::to_string(C.f) +
+                                                            ", eligible " + std::to_string(C.ne) + ", delayed " +
+                                                            std::to_string(C.delayed) + ", level " +
+                                                            std::to_string(lv) + ")");
+                nd += C.delayed;
+            }
             F.nd = nd;
             F.psym = Sy.p;
             F.col0 = Sy.col0;
@@ -429,14 +442,14 @@ int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgp
         if (work_exceeded) break;  // stopped before this level (nothing of it launched)
         if (cc.size() > static_cast<size_t>(INT32_MAX)) return set_error(SMFGPU_ERR_ALLOC, "factor: level too large");
         // capacities (arena contents stay: offsets are relative)
-        if ((rc = ensure(P->d_arena, static_cast<size_t>(P->sq_pool.end), true)) ||
-            (rc = ensure(P->d_fac, static_cast<size_t>(P->fac_used), true)) ||
-            (rc = ensure(P->d_lrows, static_cast<size_t>(P->rows_used), true)) ||
-            (rc = ensure(P->d_ltags, static_cast<size_t>(P->rows_used), true)) ||
-            (rc = ensure(P->d_dinv, static_cast<size_t>(3 * P->dinv_used), true)) ||
-            (rc = ensure(P->d_dkind, static_cast<size_t>(P->dinv_used), true)) ||
-            (rc = ensure(P->d_work, static_cast<size_t>(std::max<int64_t>(work_used, 1)), false)) ||
-            (rc = ensure(P->d_lcmap, std::max<size_t>(cmap.size(), 1), false)))
+        if ((rc = ensure(P->d_arena, static_cast<size_t>(P->sq_pool.end), true, "front squares")) ||
+            (rc = ensure(P->d_fac, static_cast<size_t>(P->fac_used), true, "factor store")) ||
+            (rc = ensure(P->d_lrows, static_cast<size_t>(P->rows_used), true, "row ids")) ||
+            (rc = ensure(P->d_ltags, static_cast<size_t>(P->rows_used), true, "pivot tags")) ||
+            (rc = ensure(P->d_dinv, static_cast<size_t>(3 * P->dinv_used), true, "D^-1 table")) ||
+            (rc = ensure(P->d_dkind, static_cast<size_t>(P->dinv_used), true, "D^-1 kinds")) ||
+            (rc = ensure(P->d_work, static_cast<size_t>(std::max<int64_t>(work_used, 1)), false, "panel workspace")) ||
+            (rc = ensure(P->d_lcmap, std::max<size_t>(cmap.size(), 1), false, "child maps")))
             return rc;
         // one staged upload: patch, ids, rows, cmap, tasks, contributions,
         // front lists
@@ -457,7 +470,7 @@ int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgp
         const size_t o_ep = stage(P->h_stage, ep_list);
         const size_t o_back = (P->h_stage.size() + 7) & ~size_t{7};
         P->h_stage.resize(o_back + fr.size() * sizeof(FrontLd));
-        if ((rc = ensure(P->d_stage, P->h_stage.size(), false))) return rc;
+        if ((rc = ensure(P->d_stage, P->h_stage.size(), false, "level staging"))) return rc;
         unsigned char *S = P->d_stage.p;
         HIP_OK(hipMemcpyAsync(S, P->h_stage.data(), o_back, hipMemcpyHostToDevice, st));
         HIP_OK(hipMemcpyAsync(P->d_lrows.p + rows_start, S + o_rows, lrows.size() * sizeof(int32_t),

@@ -12,7 +12,13 @@
 
 namespace smfgpu {
 
+// A dispatch packet counts a grid's work-items in 32 bits: a launch over a
+// list of items (one workgroup each, up to 1024 threads) covers at most
+// this many, and longer lists are launched in pieces.
+constexpr int64_t kMaxLaunchGroups = (int64_t{1} << 22) - 1;
+
 int set_error(int code, const std::string &msg);
+const char *last_error();  // message of the last set_error (this thread)
 
 // ---------------------------------------------------------------------
 // Host analysis result (the AnalysisKeep fields of the numeric phase).
