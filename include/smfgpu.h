@@ -232,6 +232,12 @@ SMFGPU_API int smfgpu_refine(smfgpu_factor_t f, const double *values, const doub
  * launches, shared by the level's large fronts (a multiple of 8, default
  * 1024; at least 64 per front). */
 #define SMFGPU_TUNE_LDLT_UPDATE_WG 7
+/* SMFGPU_TUNE_LDLT_DEVICE_FRONTS: LDL^T levels whose symbolic fronts all
+ * have at most min(64, value) rows are laid out and factored on the device
+ * (one workgroup per front, no host round trip per level); a level with a
+ * front that delays grow past value rows goes to the host layout. Default
+ * 95 (the largest unblocked front), 0 turns it off; at most 95. */
+#define SMFGPU_TUNE_LDLT_DEVICE_FRONTS 8
 SMFGPU_API int smfgpu_plan_tune(smfgpu_plan_t plan, int32_t key, int64_t value);
 
 /* Device time (ms) of the last factorization's dominant kernel family:
@@ -240,7 +246,8 @@ SMFGPU_API int smfgpu_plan_tune(smfgpu_plan_t plan, int32_t key, int64_t value);
  * out[6] solve bytes (last solve), out[7] forward sweep ms, out[8] backward
  * sweep ms (SPD, last right-hand-side group), out[9] forward launches,
  * out[10] backward launches. Measured with HIP events on the plan's
- * stream when timing is enabled. */
+ * stream when timing is enabled. out[11]: levels of the last LDL^T
+ * factorization laid out on the device (counted with timing off too). */
 SMFGPU_API int smfgpu_set_timing(smfgpu_plan_t plan, int32_t enabled);
 SMFGPU_API int smfgpu_kernel_times(smfgpu_plan_t plan, double *out);
 /* SPD launch sequence of the last timed factorization: 5 doubles per launch

@@ -70,6 +70,7 @@ TUNE_CB_SEG = 4
 TUNE_LDLT_BIG_MIN = 5
 TUNE_LDLT_GROUP = 6
 TUNE_LDLT_UPDATE_WG = 7
+TUNE_LDLT_DEVICE_FRONTS = 8
 
 _lib = None
 

@@ -1519,14 +1519,12 @@ __device__ __forceinline__ double stored_entry(const double *G, int f, int c, in
     return G[static_cast<int64_t>(c) * f + i];
 }
 
-__global__ __launch_bounds__(kLT) void k_ldlt_epilogue(LdltDev d, const int32_t *__restrict__ flist,
-                                                       int32_t n_inline) {
-    const int s = flist[blockIdx.x];
-    FrontLd &F = d.fronts[s];
+// The epilogue of one front whose square (leading dimension f) is at G, in
+// global memory or LDS; store: also write its stored factor columns.
+__device__ void finish_front_wg(const LdltDev &d, FrontLd &F, const double *G, bool store) {
     const int f = F.f;
     int p = F.p;
     int8_t *tags = d.tags + F.rofs;
-    const double *G = d.arena + F.fofs;
     const int tid = threadIdx.x;
     const bool root = F.root != 0;
     int ps = p;
@@ -1577,13 +1575,189 @@ __global__ __launch_bounds__(kLT) void k_ldlt_epilogue(LdltDev d, const int32_t 
             *dk = 1;
         }
     }
-    if (static_cast<int>(blockIdx.x) >= n_inline) return;
+    if (!store) return;
     double *O = d.fac + F.lofs;
     const int64_t tot = static_cast<int64_t>(ps) * f;
     for (int64_t idx = tid; idx < tot; idx += kLT) {
         const int c = static_cast<int>(idx / f), i = static_cast<int>(idx - static_cast<int64_t>(c) * f);
         O[idx] = stored_entry(G, f, c, i, tags[c]);
     }
+}
+
+__global__ __launch_bounds__(kLT) void k_ldlt_epilogue(LdltDev d, const int32_t *__restrict__ flist,
+                                                       int32_t n_inline) {
+    FrontLd &F = d.fronts[flist[blockIdx.x]];
+    finish_front_wg(d, F, d.arena + F.fofs, static_cast<int>(blockIdx.x) < n_inline);
+}
+
+// ---------------------------------------------------------------------------
+// Levels of small fronts laid out on the device.
+//
+// Layout: one thread per front. Its size follows from the children's
+// results (nd = their delayed columns, f = nd + symbolic rows, ne = nd +
+// symbolic pivots, as the host layout); its four offsets are a workgroup
+// scan plus one bump of each counter. A front larger than L.fmax, or a
+// bump past the room, refuses the level (atomicMin of the level index).
+__global__ __launch_bounds__(kLT) void k_dev_layout(LdltDev d, DevLevel L, const int32_t *__restrict__ flist,
+                                                    int32_t nf, int32_t lv) {
+    __shared__ long long sc[4][kLT];
+    __shared__ unsigned long long base[4];
+    __shared__ int refuse;
+    // after a refused level nothing more is laid out
+    if (__atomic_load_n(&L.ctr[kDcLevel], __ATOMIC_RELAXED) < static_cast<unsigned long long>(lv)) return;
+    const int tid = threadIdx.x;
+    const int i = blockIdx.x * kLT + tid;
+    FrontLd F{};
+    long long sz[4] = {0, 0, 0, 0};
+    bool big = false;
+    if (tid == 0) refuse = 0;
+    if (i < nf) {
+        const int s = flist[i];
+        const FrontDev Sy = L.sym[s];
+        int nd = 0;
+        for (int c = L.child_ptr[s]; c < L.child_ptr[s + 1]; ++c) nd += d.fronts[L.child_idx[c]].delayed;
+        F.nd = nd;
+        F.psym = Sy.p;
+        F.col0 = Sy.col0;
+        F.f = nd + Sy.f;
+        F.ne = nd + Sy.p;
+        F.root = L.parent[s] < 0 ? 1 : 0;
+        F.dev = 1;
+        big = F.f > L.fmax;
+        const long long f = F.f;
+        sz[0] = (f * f + 63) / 64 * 64;
+        sz[1] = f * F.ne;
+        sz[2] = f;
+        sz[3] = F.ne;
+    }
+    for (int k = 0; k < 4; ++k) sc[k][tid] = sz[k];
+    __syncthreads();
+    // inclusive scan (Hillis-Steele)
+    for (int w = 1; w < kLT; w <<= 1) {
+        long long v[4];
+        for (int k = 0; k < 4; ++k) v[k] = tid >= w ? sc[k][tid - w] : 0;
+        __syncthreads();
+        for (int k = 0; k < 4; ++k) sc[k][tid] += v[k];
+        __syncthreads();
+    }
+    if (tid == 0) {
+        for (int k = 0; k < 4; ++k) {
+            const unsigned long long tot = static_cast<unsigned long long>(sc[k][kLT - 1]);
+            base[k] = atomicAdd(&L.ctr[k], tot);
+            if (static_cast<long long>(base[k] + tot) > L.cap[k]) refuse = 1;
+        }
+    }
+    __syncthreads();
+    if (refuse && tid == 0) {
+        atomicMin(&L.ctr[kDcLevel], static_cast<unsigned long long>(lv));
+        atomicOr(&L.ctr[kDcWhy], kDevWhyRoom);
+    }
+    if (i >= nf) return;
+    if (big) {
+        atomicMin(&L.ctr[kDcLevel], static_cast<unsigned long long>(lv));
+        atomicOr(&L.ctr[kDcWhy], kDevWhySize);
+    }
+    F.fofs = static_cast<int64_t>(base[0] + (sc[0][tid] - sz[0]));
+    F.lofs = static_cast<int64_t>(base[1] + (sc[1][tid] - sz[1]));
+    F.rofs = static_cast<int64_t>(base[2] + (sc[2][tid] - sz[2]));
+    F.dofs = static_cast<int64_t>(base[3] + (sc[3][tid] - sz[3]));
+    d.fronts[flist[i]] = F;
+}
+
+void launch_dev_layout(hipStream_t st, const LdltDev &d, const DevLevel &L, const int32_t *flist, int32_t nf,
+                       int32_t lv) {
+    if (nf <= 0) return;
+    k_dev_layout<<<dim3((nf + kLT - 1) / kLT), kLT, 0, st>>>(d, L, flist, nf, lv);
+}
+
+// One front per workgroup, the square in LDS: row list (children's delayed
+// columns in child order, then the symbolic rows), assembly (zero, the
+// front's original entries, each child's contribution block in child
+// order: per entry the host layout's order of sums), the unblocked
+// factorization, the epilogue, and the contribution block written back to
+// the front's square for the parent.
+__global__ __launch_bounds__(kLT) void k_dev_fronts(LdltDev d, DevLevel L, const int32_t *__restrict__ flist,
+                                                    int32_t lv, Prm prm) {
+    __shared__ double Fs[kDevFrontMax * kDevFrontMax];
+    __shared__ RedScratch rs;
+    __shared__ int cnt[3];
+    __shared__ int32_t cpos[kDevFrontMax];
+    // a refused level is left whole to the host
+    if (__atomic_load_n(&L.ctr[kDcLevel], __ATOMIC_RELAXED) <= static_cast<unsigned long long>(lv)) return;
+    const int s = flist[blockIdx.x];
+    FrontLd &Fd = d.fronts[s];
+    const FrontLd F = Fd;
+    const FrontDev Sy = L.sym[s];
+    const int f = F.f, ne = F.ne, nd = F.nd;
+    int32_t *rows = d.rows + F.rofs;
+    int8_t *tags = d.tags + F.rofs;
+    const int tid = threadIdx.x;
+    for (int idx = tid; idx < f * f; idx += kLT) Fs[idx] = 0.0;
+    for (int t = tid; t < f; t += kLT) tags[t] = 0;
+    for (int t = tid; t < Sy.f; t += kLT) rows[nd + t] = L.sym_rows[Sy.rofs + t];
+    if (tid < 3) cnt[tid] = 0;
+    __syncthreads();
+    // original entries of the own pivot columns (one per row and column)
+    for (int oc = 0; oc < F.psym; ++oc) {
+        const int j = F.col0 + oc;
+        double *col = Fs + (nd + oc) * f + nd;
+        for (int k = d.pcol_ptr[j] + tid; k < d.pcol_ptr[j + 1]; k += kLT) col[d.amap[k]] += d.av[k];
+    }
+    __syncthreads();
+    int dp = 0;
+    for (int c = L.child_ptr[s]; c < L.child_ptr[s + 1]; ++c) {
+        const int ch = L.child_idx[c];
+        const FrontLd C = d.fronts[ch];
+        const int q = C.f - C.p, del = C.delayed;
+        const int32_t *crow = d.rows + C.rofs + C.p;
+        const int32_t *sm = L.sym_cmap + L.sym_cmofs[ch];
+        for (int ii = tid; ii < q; ii += kLT) cpos[ii] = ii < del ? dp + ii : nd + sm[ii - del];
+        for (int t = tid; t < del; t += kLT) rows[dp + t] = crow[t];
+        __syncthreads();
+        const double *cb = d.arena + C.fofs + static_cast<int64_t>(C.p) * C.f + C.p;
+        for (int idx = tid; idx < q * q; idx += kLT) {
+            const int jj = idx / q, ii = idx - jj * q;
+            if (ii >= jj) Fs[cpos[jj] * f + cpos[ii]] += cb[static_cast<int64_t>(jj) * C.f + ii];
+        }
+        __syncthreads();
+        dp += del;
+    }
+    FV A{Fs, f, f};
+    int k = 0;
+    double flops = 0.0;
+    while (k < ne) {
+        int kind, pa, pb;
+        select_pivot_wg(A, k, ne, ne, prm, rs, kind, pa, pb);
+        if (kind == 0) break;
+        apply_pivot_wg(A, kind, pa, pb, k, rows, tags, nullptr, 0, 0, f, prm, cnt, flops, 0);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        Fd.p = k;
+        Fd.npos = cnt[0];
+        Fd.nneg = cnt[1];
+        Fd.nzero = cnt[2];
+        Fd.flops = flops;
+    }
+    __syncthreads();
+    finish_front_wg(d, Fd, Fs, true);
+    if (F.root) return;
+    // contribution block (lower part of columns [k, f)) for the parent
+    double *G = d.arena + F.fofs;
+    const int q = f - k;
+    for (int idx = tid; idx < q * q; idx += kLT) {
+        const int jj = idx / q, ii = idx - jj * q;
+        if (ii >= jj) G[static_cast<int64_t>(k + jj) * f + k + ii] = Fs[(k + jj) * f + k + ii];
+    }
+}
+
+void launch_dev_fronts(hipStream_t st, const LdltDev &d, const DevLevel &L, const int32_t *flist, int32_t nf,
+                       int32_t lv, double u, double eff_small, double small_pivot) {
+    if (nf <= 0) return;
+    Prm prm{u, eff_small, small_pivot};
+    for (int64_t o = 0; o < nf; o += kMaxLaunchGroups)
+        k_dev_fronts<<<dim3(static_cast<unsigned>(std::min<int64_t>(kMaxLaunchGroups, nf - o))), kLT, 0, st>>>(
+            d, L, flist + o, lv, prm);
 }
 
 // Stored columns of large fronts: grid (column stride, fronts).

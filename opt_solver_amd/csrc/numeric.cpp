@@ -640,6 +640,10 @@ SMFGPU_API int smfgpu_plan_tune(smfgpu_plan_t P, int32_t key, int64_t value) {
             return set_error(SMFGPU_ERR_INVALID_ARG, "tune: update workgroups must be a multiple of 8");
         P->ldlt_update_wg = static_cast<int32_t>(value);
         return SMFGPU_OK;
+    case SMFGPU_TUNE_LDLT_DEVICE_FRONTS:
+        if (value < 0 || value > kDevFrontMax) return set_error(SMFGPU_ERR_INVALID_ARG, "tune: device fronts in [0, 95]");
+        P->dev_fmax = static_cast<int32_t>(value);
+        return SMFGPU_OK;
     case SMFGPU_TUNE_SYRK_AHEAD:
         if (value != 1 && value != 2) return set_error(SMFGPU_ERR_INVALID_ARG, "tune: chunks ahead must be 1 or 2");
         P->syrk_ahead = static_cast<int32_t>(value);
@@ -666,6 +670,7 @@ SMFGPU_API int smfgpu_kernel_times(smfgpu_plan_t P, double *out) {
     out[5] = P->t_solve_ms;
     out[6] = P->solve_bytes;
     for (int k = 0; k < 4; ++k) out[7 + k] = P->t_solve_phase[k];
+    out[11] = static_cast<double>(P->dev_levels);
     return SMFGPU_OK;
 }
 

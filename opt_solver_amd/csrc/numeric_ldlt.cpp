@@ -103,8 +103,14 @@ int ldlt_plan_build(smfgpu_plan_s *P, const smfgpu_analysis_desc *D) {
         lf.insert(lf.end(), P->level_fronts[lv].begin(), P->level_fronts[lv].end());
     }
     P->lv_flist_ofs[nlev] = static_cast<int64_t>(lf.size());
+    P->lv_fsym_max.assign(nlev, 0);
+    for (int32_t lv = 0; lv < nlev; ++lv)
+        for (int32_t s : P->level_fronts[lv]) P->lv_fsym_max[lv] = std::max(P->lv_fsym_max[lv], P->fronts[s].f);
     int rc;
-    if ((rc = P->d_flist.upload(lf)) || (rc = P->d_fl.alloc(std::max(ns, 1)))) return rc;
+    if ((rc = P->d_flist.upload(lf)) || (rc = P->d_fl.alloc(std::max(ns, 1))) ||
+        (rc = P->d_child_ptr.upload(P->child_ptr)) || (rc = P->d_child_idx.upload(P->child_idx)) ||
+        (rc = P->d_parent.upload(P->sn_parent)) || (rc = P->d_dctr.alloc(kDcN)))
+        return rc;
     // initial capacities: the pool's peak when the levels are laid out in
     // order without delays, and the symbolic factor, each with an eighth
     // more for delays
@@ -121,6 +127,121 @@ int ldlt_plan_build(smfgpu_plan_s *P, const smfgpu_analysis_desc *D) {
         (rc = P->d_dinv.alloc(static_cast<size_t>(3 * (P->n + 1024)))) ||
         (rc = P->d_dkind.alloc(static_cast<size_t>(P->n + 1024))))
         return rc;
+    return SMFGPU_OK;
+}
+
+// Levels of small fronts on the device, from lv0 through the run of levels
+// whose symbolic fronts are all small: room for the run (an estimate that
+// allows some delays) is reserved, the layout and front kernels of every
+// level are launched, and the counters and the done fronts are read back
+// once. A level refused for room is retried with twice the room; one
+// refused for a front too large is left to the host layout (hold).
+// next: the first level not done.
+static int ldlt_dev_levels(smfgpu_plan_s *P, int32_t lv0, double u, double eff_small, double small, int32_t &next,
+                           int32_t &hold) {
+    hipStream_t st = P->stream;
+    const int32_t nlev = static_cast<int32_t>(P->level_fronts.size());
+    const int32_t symmax = std::min(kDevSymMax, P->dev_fmax);
+    int32_t end = lv0;
+    while (end < nlev && P->lv_fsym_max[end] <= symmax) ++end;
+    next = lv0;
+    hold = -1;
+    int64_t grow = 1;
+    int32_t retried = -1;
+    int rc;
+    while (next < end) {
+        int64_t e_sq = 0, e_fac = 0, e_rows = 0, e_dinv = 0;
+        for (int32_t l = next; l < end; ++l)
+            for (int32_t s : P->level_fronts[l]) {
+                const int64_t fe = std::min<int64_t>(P->dev_fmax, P->fronts[s].f + 8);
+                const int64_t pe = std::min<int64_t>(fe, P->fronts[s].p + 8);
+                e_sq += RangePool::round(fe * fe);
+                e_fac += fe * pe;
+                e_rows += fe;
+                e_dinv += pe;
+            }
+        e_sq *= grow;
+        e_fac *= grow;
+        e_rows *= grow;
+        e_dinv *= grow;
+        const int64_t R = P->sq_pool.take(e_sq);
+        if ((rc = ensure(P->d_arena, static_cast<size_t>(P->sq_pool.end), true, "front squares")) ||
+            (rc = ensure(P->d_fac, static_cast<size_t>(P->fac_used + e_fac), true, "factor store")) ||
+            (rc = ensure(P->d_lrows, static_cast<size_t>(P->rows_used + e_rows), true, "row ids")) ||
+            (rc = ensure(P->d_ltags, static_cast<size_t>(P->rows_used + e_rows), true, "pivot tags")) ||
+            (rc = ensure(P->d_dinv, static_cast<size_t>(3 * (P->dinv_used + e_dinv)), true, "D^-1 table")) ||
+            (rc = ensure(P->d_dkind, static_cast<size_t>(P->dinv_used + e_dinv), true, "D^-1 kinds")))
+            return rc;
+        unsigned long long *h = P->h_dctr;
+        h[kDcSq] = static_cast<unsigned long long>(R);
+        h[kDcFac] = static_cast<unsigned long long>(P->fac_used);
+        h[kDcRows] = static_cast<unsigned long long>(P->rows_used);
+        h[kDcDinv] = static_cast<unsigned long long>(P->dinv_used);
+        h[kDcLevel] = ~0ull;
+        h[kDcWhy] = 0;
+        HIP_OK(hipMemcpyAsync(P->d_dctr.p, h, kDcN * sizeof(unsigned long long), hipMemcpyHostToDevice, st));
+        DevLevel L{};
+        L.sym = P->d_fronts.p;
+        L.sym_rows = P->d_rows.p;
+        L.sym_cmap = P->d_cmap.p;
+        L.sym_cmofs = P->d_cmofs.p;
+        L.child_ptr = P->d_child_ptr.p;
+        L.child_idx = P->d_child_idx.p;
+        L.parent = P->d_parent.p;
+        L.ctr = P->d_dctr.p;
+        L.cap[kDcSq] = R + e_sq;
+        L.cap[kDcFac] = P->fac_used + e_fac;
+        L.cap[kDcRows] = P->rows_used + e_rows;
+        L.cap[kDcDinv] = P->dinv_used + e_dinv;
+        L.fmax = P->dev_fmax;
+        const LdltDev d = ldlt_dev(P);
+        for (int32_t l = next; l < end; ++l) {
+            const int32_t *fl = P->d_flist.p + P->lv_flist_ofs[l];
+            const int32_t nf = static_cast<int32_t>(P->lv_flist_ofs[l + 1] - P->lv_flist_ofs[l]);
+            launch_dev_layout(st, d, L, fl, nf, l);
+            launch_dev_fronts(st, d, L, fl, nf, l, u, eff_small, small);
+        }
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipMemcpyAsync(h, P->d_dctr.p, kDcN * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+        HIP_OK(hipStreamSynchronize(st));
+        const int32_t done = static_cast<int32_t>(std::min<unsigned long long>(h[kDcLevel], end));
+        if (done > next) {
+            const int64_t o0 = P->lv_flist_ofs[next], o1 = P->lv_flist_ofs[done];
+            const int32_t nfr = static_cast<int32_t>(o1 - o0);
+            if ((rc = ensure(P->d_stage, static_cast<size_t>(nfr) * sizeof(FrontLd), false, "level staging")))
+                return rc;
+            std::vector<FrontLd> back(nfr);
+            launch_front_gather(st, P->d_fl.p, reinterpret_cast<FrontLd *>(P->d_stage.p), P->d_flist.p + o0, nfr);
+            HIP_OK(hipMemcpyAsync(back.data(), P->d_stage.p, back.size() * sizeof(FrontLd), hipMemcpyDeviceToHost, st));
+            HIP_OK(hipStreamSynchronize(st));
+            int64_t fac_end = P->fac_used, rows_end = P->rows_used, dinv_end = P->dinv_used;
+            int32_t i = 0;
+            for (int32_t l = next; l < done; ++l)
+                for (int32_t s : P->level_fronts[l]) {
+                    const FrontLd &F = back[i++];
+                    P->fl[s] = F;
+                    fac_end = std::max(fac_end, F.lofs + static_cast<int64_t>(F.f) * F.ne);
+                    rows_end = std::max(rows_end, F.rofs + F.f);
+                    dinv_end = std::max(dinv_end, F.dofs + F.ne);
+                }
+            P->rows_h.resize(rows_end);
+            HIP_OK(hipMemcpyAsync(P->rows_h.data() + P->rows_used, P->d_lrows.p + P->rows_used,
+                                  (rows_end - P->rows_used) * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+            HIP_OK(hipStreamSynchronize(st));
+            P->fac_used = fac_end;
+            P->rows_used = rows_end;
+            P->dinv_used = dinv_end;
+            P->dev_levels += done - next;
+            next = done;
+        }
+        if (next >= end) break;
+        if ((h[kDcWhy] & kDevWhySize) || retried == next) {
+            hold = next;
+            break;
+        }
+        retried = next;
+        grow *= 2;
+    }
     return SMFGPU_OK;
 }
 
@@ -328,7 +449,23 @@ int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgp
                                                          : static_cast<int64_t>(std::floor(opt.work_limit)));
     int64_t work_done = 0;
     P->rows_h.resize(0);
+    P->dev_levels = 0;
+    // levels of small fronts run on the device unless a limit needs the
+    // per-level checks
+    const bool dev_on = P->dev_fmax > 0 && !work_armed && opt.max_negative_pivots < 0;
+    int32_t dev_hold = -1;
     for (int32_t lv = 0; lv < nlev; ++lv) {
+        if (dev_on && lv != dev_hold && P->lv_fsym_max[lv] <= std::min(kDevSymMax, P->dev_fmax)) {
+            int32_t next = lv, hold = -1;
+            if ((rc = ldlt_dev_levels(P, lv, opt.pivot_u, eff_small, small, next, hold))) return rc;
+            levels_done = next;
+            dev_hold = hold;
+            if (next > lv) {
+                lv = next - 1;
+                continue;
+            }
+            dev_hold = lv;
+        }
         const auto &fr = P->level_fronts[lv];
         cmap.clear();
         tasks.clear();
@@ -487,7 +624,7 @@ int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgp
         for (int32_t s : fr)
             for (int32_t c = P->child_ptr[s]; c < P->child_ptr[s + 1]; ++c) {
                 const FrontLd &C = P->fl[P->child_idx[c]];
-                P->sq_pool.give(C.fofs, static_cast<int64_t>(C.f) * C.f);
+                if (!C.dev) P->sq_pool.give(C.fofs, static_cast<int64_t>(C.f) * C.f);
             }
         launch_ldlt_small(st, d, reinterpret_cast<const int32_t *>(S + o_small), static_cast<int32_t>(small_list.size()),
                           opt.pivot_u, eff_small, small);

@@ -21,7 +21,8 @@ struct FrontLd {
     int32_t root;             // 1: no parent (unaccepted columns become zero pivots)
     int32_t p, pstore;        // accepted pivots; stored pivot columns
     int32_t npos, nneg, nzero, n2x2;
-    int32_t delayed, pad;
+    int32_t delayed;
+    int32_t dev;              // 1: laid out on the device (its square is not in the host's pool)
     double flops;
     int64_t fofs;             // arena offset of the square
     int64_t lofs;             // factor-store offset of the stored columns
@@ -80,6 +81,33 @@ struct BigLd {
 constexpr int32_t kGroupCols = 64;   // columns of a speculative group (four panels of 16)
 constexpr int32_t kGroupRows = 256;  // rows per workgroup below a group's diagonal region
 constexpr int64_t kGroupBuf = 192 + 64 * 16 + 2 * 64 * 64;  // group buffer per front before the per-workgroup maxima
+
+// Levels of small fronts laid out on the device: per level, a layout
+// kernel (each front's size from its children's delays, offsets by a bump
+// of the counters below) and one workgroup per front that builds the row
+// list, assembles, factors (the unblocked kernel's steps) and stores the
+// front with the square in LDS. A level whose fronts do not all fit is
+// refused as a whole before any of it is factored: the counters record the
+// first refused level and the reason, and the launches of later levels
+// return at once.
+constexpr int32_t kDevFrontMax = 95;   // largest realized front of a device level (blocked_min_front - 1)
+constexpr int32_t kDevSymMax = 64;     // largest symbolic front of a level laid out on the device
+enum : int32_t { kDcSq = 0, kDcFac = 1, kDcRows = 2, kDcDinv = 3, kDcLevel = 4, kDcWhy = 5, kDcN = 6 };
+constexpr unsigned long long kDevWhySize = 1, kDevWhyRoom = 2;  // refusal: a front too large / a buffer full
+struct DevLevel {
+    const FrontDev *sym;        // symbolic fronts (f, p, col0, rofs)
+    const int32_t *sym_rows;    // symbolic row ids
+    const int32_t *sym_cmap;    // per child: its symbolic CB rows' positions among the parent's symbolic rows
+    const int64_t *sym_cmofs;
+    const int32_t *child_ptr, *child_idx, *parent;
+    unsigned long long *ctr;    // kDcN counters: next free square / factor / row / D^-1 offsets, refused level, reason
+    int64_t cap[4];             // ends of the room for the four offsets
+    int32_t fmax;               // largest realized front taken
+};
+void launch_dev_layout(hipStream_t st, const LdltDev &d, const DevLevel &L, const int32_t *flist, int32_t nf,
+                       int32_t lv);
+void launch_dev_fronts(hipStream_t st, const LdltDev &d, const DevLevel &L, const int32_t *flist, int32_t nf,
+                       int32_t lv, double u, double eff_small, double small_pivot);
 
 void launch_ldlt_small(hipStream_t st, const LdltDev &d, const int32_t *flist, int32_t nf, double u, double eff_small,
                        double small_pivot);

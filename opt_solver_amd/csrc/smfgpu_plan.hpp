@@ -216,6 +216,12 @@ struct smfgpu_plan_s {
     std::vector<unsigned char> h_stage;
     int64_t fac_used = 0, rows_used = 0, dinv_used = 0;
     std::vector<int64_t> lv_flist_ofs;              // per level: offset of its fronts in d_flist (solve)
+    std::vector<int32_t> lv_fsym_max;               // per level: largest symbolic front
+    int32_t dev_fmax = kDevFrontMax;                // largest realized front of a device level (0: host layout only)
+    int32_t dev_levels = 0;                         // levels the last factorization ran on the device
+    DBuf<int32_t> d_child_ptr, d_child_idx, d_parent;
+    DBuf<unsigned long long> d_dctr;                // device-level counters
+    unsigned long long h_dctr[kDcN] = {};
     bool ldlt_ready = false;                        // a factorization's fronts are valid
 };
 

@@ -2,7 +2,10 @@
 as one forest (every level's fronts of all systems in the same launches):
 per-system inertia, delayed pivots and 2x2 pivots against the reference's
 one-system-at-a-time factor, the whole factor bit for bit against the
-oracle, solves against the known solutions."""
+oracle, solves against the known solutions. The levels of small fronts run
+laid out on the device (default), partly (device fronts capped at 28 rows,
+so levels whose delays grow a front past that go to the host layout) and
+not at all: the same factor each way."""
 import numpy as np
 import pytest
 
@@ -15,13 +18,15 @@ if not gpu_available():
     pytest.skip("no GPU", allow_module_level=True)
 
 from opt_solver_amd import batch  # noqa: E402
+from opt_solver_amd._lib import TUNE_LDLT_DEVICE_FRONTS  # noqa: E402
 from opt_solver_amd.solver import Plan, analyse  # noqa: E402
 
 
-def test_ocp_batch_per_system_parity(oracle_lib):
+@pytest.mark.parametrize("dev_fronts", [95, 28, 0])
+def test_ocp_batch_per_system_parity(oracle_lib, dev_fronts):
     from oracle import refshim
     nsys = 32
-    print(f"case ocp batch {nsys}", flush=True)
+    print(f"case ocp batch {nsys} device fronts {dev_fronts}", flush=True)
     mats, Ts = batch.ocp_batch(nsys, 10, 60, seed=21)
     ref = refshim.available()
     if ref:
@@ -32,7 +37,17 @@ def test_ocp_batch_per_system_parity(oracle_lib):
     U, offs = batch.union_arrays(arrs)
     vals = np.concatenate([A[3] for A in mats])
     plan = Plan(U, posdef=False)
+    plan.tune(TUNE_LDLT_DEVICE_FRONTS, dev_fronts)
     fac = plan.factor(vals, u=0.01)
+    nlev = plan.stats()["levels"]
+    dev = plan.kernel_times()["device_levels"]
+    print(f"  levels {nlev:.0f} on the device {dev:.0f}", flush=True)
+    if dev_fronts == 95:
+        assert dev == nlev
+    elif dev_fronts == 0:
+        assert dev == 0
+    else:
+        assert 0 < dev < nlev
     per = batch.system_stats(fac.front_stats(), offs["sn"])
     for i, T in enumerate(Ts):
         assert tuple(per[i, :3]) == (16 * T + 12, 12 * T, 0), i
