@@ -247,7 +247,10 @@ SMFGPU_API int smfgpu_plan_tune(smfgpu_plan_t plan, int32_t key, int64_t value);
  * sweep ms (SPD, last right-hand-side group), out[9] forward launches,
  * out[10] backward launches. Measured with HIP events on the plan's
  * stream when timing is enabled. out[11]: levels of the last LDL^T
- * factorization laid out on the device (counted with timing off too). */
+ * factorization laid out on the device (counted with timing off too);
+ * out[12..15] host wall ms of the last LDL^T factorization's phases: the
+ * device levels (launches and wait), their read-back, the statistics, the
+ * solve schedule. out holds 16 doubles. */
 SMFGPU_API int smfgpu_set_timing(smfgpu_plan_t plan, int32_t enabled);
 SMFGPU_API int smfgpu_kernel_times(smfgpu_plan_t plan, double *out);
 /* SPD launch sequence of the last timed factorization: 5 doubles per launch

@@ -15,20 +15,38 @@ constexpr double kWeakPivotRatio = 1e-6;  // ldlt_kernel.hpp:100
 
 // ---------------------------------------------------------------------------
 // Workgroup reductions (exact: max and first argmax).
-struct RedScratch {
-    double v[kLT];
-    int i[kLT];
+template <int NT>
+struct RedScratchT {
+    double v[NT];
+    int i[NT];
 };
+using RedScratch = RedScratchT<kLT>;
 
 // (max, first index of the max) over the thread's candidates, reduced
 // across the workgroup: ties keep the smaller index, as a sequential scan
 // with a strict '>' does.
-__device__ __forceinline__ void red_maxfirst(RedScratch &rs, double v, int idx, double &outv, int &outi) {
+template <int NT>
+__device__ __forceinline__ void red_maxfirst(RedScratchT<NT> &rs, double v, int idx, double &outv, int &outi) {
+    if constexpr (NT == 64) {
+        // one wavefront: butterfly with the same rule (it is associative and
+        // commutative), then lane 0's result for every lane
+        for (int off = 32; off > 0; off >>= 1) {
+            const double b = __shfl_xor(v, off);
+            const int ib = __shfl_xor(idx, off);
+            if (b > v || (b == v && ib >= 0 && (idx < 0 || ib < idx))) {
+                v = b;
+                idx = ib;
+            }
+        }
+        outv = __shfl(v, 0);
+        outi = __shfl(idx, 0);
+        return;
+    }
     const int tid = threadIdx.x;
     rs.v[tid] = v;
     rs.i[tid] = idx;
     __syncthreads();
-    for (int w = kLT / 2; w > 0; w >>= 1) {
+    for (int w = NT / 2; w > 0; w >>= 1) {
         if (tid < w) {
             const double a = rs.v[tid], b = rs.v[tid + w];
             const int ia = rs.i[tid], ib = rs.i[tid + w];
@@ -44,11 +62,19 @@ __device__ __forceinline__ void red_maxfirst(RedScratch &rs, double v, int idx, 
     __syncthreads();
 }
 
-__device__ __forceinline__ double red_max(RedScratch &rs, double v) {
+template <int NT>
+__device__ __forceinline__ double red_max(RedScratchT<NT> &rs, double v) {
+    if constexpr (NT == 64) {
+        for (int off = 32; off > 0; off >>= 1) {
+            const double b = __shfl_xor(v, off);
+            if (b > v) v = b;
+        }
+        return __shfl(v, 0);
+    }
     const int tid = threadIdx.x;
     rs.v[tid] = v;
     __syncthreads();
-    for (int w = kLT / 2; w > 0; w >>= 1) {
+    for (int w = NT / 2; w > 0; w >>= 1) {
         if (tid < w && rs.v[tid + w] > rs.v[tid]) rs.v[tid] = rs.v[tid + w];
         __syncthreads();
     }
@@ -71,7 +97,8 @@ struct Prm {
 
 // select_pivot (ldlt_kernel.cpp:127-275) with workgroup reductions. Every
 // thread returns the same choice.
-__device__ void select_pivot_wg(const FV &A, int k, int ne, int limit, const Prm &prm, RedScratch &rs, int &kind,
+template <int NT>
+__device__ void select_pivot_wg(const FV &A, int k, int ne, int limit, const Prm &prm, RedScratchT<NT> &rs, int &kind,
                                 int &pa, int &pb) {
     const int tid = threadIdx.x;
     const int f = A.f;
@@ -81,7 +108,7 @@ __device__ void select_pivot_wg(const FV &A, int k, int ne, int limit, const Prm
         const double a0 = fabs(A.at(k, k));
         if (a0 >= prm.eff_small) {
             double m = 0.0;
-            for (int i = k + 1 + tid; i < f; i += kLT) m = fmax(m, fabs(A.at(i, k)));
+            for (int i = k + 1 + tid; i < f; i += NT) m = fmax(m, fabs(A.at(i, k)));
             const double colmax = red_max(rs, m);
             if (!(prm.u * colmax > a0) && !(a0 < kWeakPivotRatio * colmax)) {
                 kind = 1;
@@ -96,7 +123,7 @@ __device__ void select_pivot_wg(const FV &A, int k, int ne, int limit, const Prm
         // index on ties), then the largest of the rest
         double bv = -1.0;
         int bi = -1;
-        for (int i = k + tid; i < ne; i += kLT) {
+        for (int i = k + tid; i < ne; i += NT) {
             if (i == cand) continue;
             const double v = fabs(A.sym(i, cand));
             if (v > bv) {
@@ -112,12 +139,12 @@ __device__ void select_pivot_wg(const FV &A, int k, int ne, int limit, const Prm
             r_best = -1;
         }
         double m2l = 0.0;
-        for (int i = k + tid; i < ne; i += kLT)
+        for (int i = k + tid; i < ne; i += NT)
             if (i != cand && i != r_best) m2l = fmax(m2l, fabs(A.sym(i, cand)));
         const double m2 = red_max(rs, m2l);
         const int ext0 = max(ne, cand + 1);
         double mel = 0.0;
-        for (int i = ext0 + tid; i < f; i += kLT) mel = fmax(mel, fabs(A.at(i, cand)));
+        for (int i = ext0 + tid; i < f; i += NT) mel = fmax(mel, fabs(A.at(i, cand)));
         const double m_ext = red_max(rs, mel);
         bool weak = false;
         double weak_growth = 0.0;
@@ -143,7 +170,7 @@ __device__ void select_pivot_wg(const FV &A, int k, int ne, int limit, const Prm
         }
         // partner column: max excluding r_best, and excluding cand too
         double o2 = 0.0, o1 = 0.0;
-        for (int i = k + tid; i < f; i += kLT) {
+        for (int i = k + tid; i < f; i += NT) {
             if (i == r_best) continue;
             const double v = fabs(A.sym(i, r_best));
             o2 = fmax(o2, v);
@@ -199,15 +226,17 @@ __device__ void select_pivot_wg(const FV &A, int k, int ne, int limit, const Prm
 
 // Lanes per column when the workgroup sweeps the lower part of several
 // columns of up to `len` rows: log2 of a power of two covering len (at most
-// the workgroup), so kLT >> lg columns are in flight at a time.
+// the workgroup), so NT >> lg columns are in flight at a time.
+template <int NT = kLT>
 __device__ __forceinline__ int lg_rows(int len) {
     int lg = 0;
-    while ((1 << lg) < len && (1 << lg) < kLT) ++lg;
+    while ((1 << lg) < len && (1 << lg) < NT) ++lg;
     return lg;
 }
 
 // sym_swap (ldlt_kernel.cpp:333-345): positions p < q. Columns [c_lo, p)
 // swap rows p and q (c_lo = 0: all earlier columns).
+template <int NT = kLT>
 __device__ void sym_swap_wg(const FV &A, int p, int q, int32_t *rows, double *W, int wld, int w_cols, int c_lo) {
     if (p == q) return;
     if (p > q) {
@@ -226,22 +255,22 @@ __device__ void sym_swap_wg(const FV &A, int p, int q, int32_t *rows, double *W,
         rows[p] = rows[q];
         rows[q] = tr;
     }
-    for (int m = c_lo + tid; m < p; m += kLT) {
+    for (int m = c_lo + tid; m < p; m += NT) {
         const double t = A.at(p, m);
         A.at(p, m) = A.at(q, m);
         A.at(q, m) = t;
     }
-    for (int m = p + 1 + tid; m < q; m += kLT) {
+    for (int m = p + 1 + tid; m < q; m += NT) {
         const double t = A.at(m, p);
         A.at(m, p) = A.at(q, m);
         A.at(q, m) = t;
     }
-    for (int m = q + 1 + tid; m < f; m += kLT) {
+    for (int m = q + 1 + tid; m < f; m += NT) {
         const double t = A.at(m, p);
         A.at(m, p) = A.at(m, q);
         A.at(m, q) = t;
     }
-    for (int idx = tid; idx < w_cols; idx += kLT) {
+    for (int idx = tid; idx < w_cols; idx += NT) {
         const double t = W[static_cast<int64_t>(idx) * wld + p];
         W[static_cast<int64_t>(idx) * wld + p] = W[static_cast<int64_t>(idx) * wld + q];
         W[static_cast<int64_t>(idx) * wld + q] = t;
@@ -252,15 +281,16 @@ __device__ void sym_swap_wg(const FV &A, int p, int q, int32_t *rows, double *W,
 // eliminate_1x1 (ldlt_kernel.cpp:350-432): columns (k, ucol_end) get the
 // rank-1 update c_j[i] -= c_k[i] * (c_k[j] / d) (multiply, then subtract),
 // the unscaled column is saved in W column w_col, then c_k is scaled.
+template <int NT = kLT>
 __device__ void elim1_wg(const FV &A, int k, int ucol_end, double *W, int wld, int w_col) {
     const int tid = threadIdx.x;
     const int f = A.f;
     const double inv_d = 1.0 / A.at(k, k);
     if (W)
-        for (int i = k + 1 + tid; i < f; i += kLT) W[static_cast<int64_t>(w_col) * wld + i] = A.at(i, k);
+        for (int i = k + 1 + tid; i < f; i += NT) W[static_cast<int64_t>(w_col) * wld + i] = A.at(i, k);
     if (ucol_end > k + 1) {
-        const int lg = lg_rows(f - (k + 1));
-        for (int jb = k + 1; jb < ucol_end; jb += kLT >> lg) {
+        const int lg = lg_rows<NT>(f - (k + 1));
+        for (int jb = k + 1; jb < ucol_end; jb += NT >> lg) {
             const int j = jb + (tid >> lg);
             if (j >= ucol_end) continue;
             const double lj = __dmul_rn(A.at(j, k), inv_d);
@@ -270,11 +300,12 @@ __device__ void elim1_wg(const FV &A, int k, int ucol_end, double *W, int wld, i
         }
     }
     __syncthreads();
-    for (int i = k + 1 + tid; i < f; i += kLT) A.at(i, k) = __dmul_rn(A.at(i, k), inv_d);
+    for (int i = k + 1 + tid; i < f; i += NT) A.at(i, k) = __dmul_rn(A.at(i, k), inv_d);
     __syncthreads();
 }
 
 // eliminate_2x2 (ldlt_kernel.cpp:436-545)
+template <int NT = kLT>
 __device__ void elim2_wg(const FV &A, int k, int ucol_end, double *W, int wld, int w_col) {
     const int tid = threadIdx.x;
     const int f = A.f;
@@ -286,13 +317,13 @@ __device__ void elim2_wg(const FV &A, int k, int ucol_end, double *W, int wld, i
     const double inv = 1.0 / __dmul_rn(__dsub_rn(__dmul_rn(a, c), __dmul_rn(b, b)), s);
     const double i00 = __dmul_rn(c, inv), i10 = __dmul_rn(-b, inv), i11 = __dmul_rn(a, inv);
     if (W)
-        for (int i = k + 2 + tid; i < f; i += kLT) {
+        for (int i = k + 2 + tid; i < f; i += NT) {
             W[static_cast<int64_t>(w_col) * wld + i] = A.at(i, k);
             W[static_cast<int64_t>(w_col + 1) * wld + i] = A.at(i, k + 1);
         }
     if (ucol_end > k + 2) {
-        const int lg = lg_rows(f - (k + 2));
-        for (int jb = k + 2; jb < ucol_end; jb += kLT >> lg) {
+        const int lg = lg_rows<NT>(f - (k + 2));
+        for (int jb = k + 2; jb < ucol_end; jb += NT >> lg) {
             const int j = jb + (tid >> lg);
             if (j >= ucol_end) continue;
             const double a0 = A.at(j, k), a1 = A.at(j, k + 1);
@@ -306,7 +337,7 @@ __device__ void elim2_wg(const FV &A, int k, int ucol_end, double *W, int wld, i
         }
     }
     __syncthreads();
-    for (int i = k + 2 + tid; i < f; i += kLT) {
+    for (int i = k + 2 + tid; i < f; i += NT) {
         const double a0 = A.at(i, k), a1 = A.at(i, k + 1);
         A.at(i, k) = __dadd_rn(__dmul_rn(i00, a0), __dmul_rn(i10, a1));
         A.at(i, k + 1) = __dadd_rn(__dmul_rn(i10, a0), __dmul_rn(i11, a1));
@@ -356,16 +387,17 @@ __device__ void inertia_2x2(double d00, double d10, double d11, double small, in
 }
 
 // apply_pivot (ldlt_kernel.cpp:665-698). Thread 0 keeps the counters.
+template <int NT = kLT>
 __device__ void apply_pivot_wg(const FV &A, int kind, int pa, int pb, int &k, int32_t *rows, int8_t *tags, double *W,
                                int wld, int w_cols, int ucol_end, const Prm &prm, int *cnt, double &flops, int c_lo) {
     const int f = A.f;
     if (kind == 1) {
-        sym_swap_wg(A, k, pa, rows, W, wld, w_cols, c_lo);
+        sym_swap_wg<NT>(A, k, pa, rows, W, wld, w_cols, c_lo);
         if (threadIdx.x == 0) {
             inertia_1x1(A.at(k, k), prm.eff_small, cnt);
             tags[k] = 1;
         }
-        elim1_wg(A, k, ucol_end, W, wld, w_cols);
+        elim1_wg<NT>(A, k, ucol_end, W, wld, w_cols);
         const double rem = static_cast<double>(f - (k + 1));
         flops += rem + rem * (rem + 1.0);
         k += 1;
@@ -373,17 +405,17 @@ __device__ void apply_pivot_wg(const FV &A, int kind, int pa, int pb, int &k, in
     }
     int partner = pb;
     if (pa != k) {
-        sym_swap_wg(A, k, pa, rows, W, wld, w_cols, c_lo);
+        sym_swap_wg<NT>(A, k, pa, rows, W, wld, w_cols, c_lo);
         if (partner == k) partner = pa;
     }
-    if (partner != k + 1) sym_swap_wg(A, k + 1, partner, rows, W, wld, w_cols, c_lo);
+    if (partner != k + 1) sym_swap_wg<NT>(A, k + 1, partner, rows, W, wld, w_cols, c_lo);
     if (threadIdx.x == 0) {
         inertia_2x2(A.at(k, k), A.at(k + 1, k), A.at(k + 1, k + 1), prm.small_pivot, cnt);
         tags[k] = 2;
         tags[k + 1] = -1;
     }
     __syncthreads();
-    elim2_wg(A, k, ucol_end, W, wld, w_cols);
+    elim2_wg<NT>(A, k, ucol_end, W, wld, w_cols);
     const double rem = static_cast<double>(f - (k + 2));
     flops += 6.0 * rem + 2.0 * rem * (rem + 1.0);
     k += 2;
@@ -1521,6 +1553,7 @@ __device__ __forceinline__ double stored_entry(const double *G, int f, int c, in
 
 // The epilogue of one front whose square (leading dimension f) is at G, in
 // global memory or LDS; store: also write its stored factor columns.
+template <int NT = kLT>
 __device__ void finish_front_wg(const LdltDev &d, FrontLd &F, const double *G, bool store) {
     const int f = F.f;
     int p = F.p;
@@ -1529,7 +1562,7 @@ __device__ void finish_front_wg(const LdltDev &d, FrontLd &F, const double *G, b
     const bool root = F.root != 0;
     int ps = p;
     if (root && p < F.ne) {
-        for (int c = p + tid; c < F.ne; c += kLT) tags[c] = 0;
+        for (int c = p + tid; c < F.ne; c += NT) tags[c] = 0;
         ps = F.ne;
     }
     __syncthreads();
@@ -1541,7 +1574,7 @@ __device__ void finish_front_wg(const LdltDev &d, FrontLd &F, const double *G, b
         F.n2x2 = n2;
         F.delayed = root ? 0 : F.ne - p;
     }
-    for (int c = tid; c < ps; c += kLT) {
+    for (int c = tid; c < ps; c += NT) {
         const double *Lc = G + static_cast<int64_t>(c) * f;
         const int8_t tg = tags[c];
         const double d00 = Lc[c];
@@ -1578,7 +1611,7 @@ __device__ void finish_front_wg(const LdltDev &d, FrontLd &F, const double *G, b
     if (!store) return;
     double *O = d.fac + F.lofs;
     const int64_t tot = static_cast<int64_t>(ps) * f;
-    for (int64_t idx = tid; idx < tot; idx += kLT) {
+    for (int64_t idx = tid; idx < tot; idx += NT) {
         const int c = static_cast<int>(idx / f), i = static_cast<int>(idx - static_cast<int64_t>(c) * f);
         O[idx] = stored_entry(G, f, c, i, tags[c]);
     }
@@ -1676,12 +1709,13 @@ void launch_dev_layout(hipStream_t st, const LdltDev &d, const DevLevel &L, cons
 // order: per entry the host layout's order of sums), the unblocked
 // factorization, the epilogue, and the contribution block written back to
 // the front's square for the parent.
-__global__ __launch_bounds__(kLT) void k_dev_fronts(LdltDev d, DevLevel L, const int32_t *__restrict__ flist,
-                                                    int32_t lv, Prm prm) {
-    __shared__ double Fs[kDevFrontMax * kDevFrontMax];
-    __shared__ RedScratch rs;
+template <int NT, int FMIN, int FMAX>
+__global__ __launch_bounds__(NT) void k_dev_fronts(LdltDev d, DevLevel L, const int32_t *__restrict__ flist, int32_t lv,
+                                                   Prm prm) {
+    __shared__ double Fs[FMAX * FMAX];
+    __shared__ RedScratchT<NT> rs;
     __shared__ int cnt[3];
-    __shared__ int32_t cpos[kDevFrontMax];
+    __shared__ int32_t cpos[FMAX];
     // a refused level is left whole to the host
     if (__atomic_load_n(&L.ctr[kDcLevel], __ATOMIC_RELAXED) <= static_cast<unsigned long long>(lv)) return;
     const int s = flist[blockIdx.x];
@@ -1689,19 +1723,20 @@ __global__ __launch_bounds__(kLT) void k_dev_fronts(LdltDev d, DevLevel L, const
     const FrontLd F = Fd;
     const FrontDev Sy = L.sym[s];
     const int f = F.f, ne = F.ne, nd = F.nd;
+    if (f <= FMIN || f > FMAX) return;  // the other size class's launch
     int32_t *rows = d.rows + F.rofs;
     int8_t *tags = d.tags + F.rofs;
     const int tid = threadIdx.x;
-    for (int idx = tid; idx < f * f; idx += kLT) Fs[idx] = 0.0;
-    for (int t = tid; t < f; t += kLT) tags[t] = 0;
-    for (int t = tid; t < Sy.f; t += kLT) rows[nd + t] = L.sym_rows[Sy.rofs + t];
+    for (int idx = tid; idx < f * f; idx += NT) Fs[idx] = 0.0;
+    for (int t = tid; t < f; t += NT) tags[t] = 0;
+    for (int t = tid; t < Sy.f; t += NT) rows[nd + t] = L.sym_rows[Sy.rofs + t];
     if (tid < 3) cnt[tid] = 0;
     __syncthreads();
     // original entries of the own pivot columns (one per row and column)
     for (int oc = 0; oc < F.psym; ++oc) {
         const int j = F.col0 + oc;
         double *col = Fs + (nd + oc) * f + nd;
-        for (int k = d.pcol_ptr[j] + tid; k < d.pcol_ptr[j + 1]; k += kLT) col[d.amap[k]] += d.av[k];
+        for (int k = d.pcol_ptr[j] + tid; k < d.pcol_ptr[j + 1]; k += NT) col[d.amap[k]] += d.av[k];
     }
     __syncthreads();
     int dp = 0;
@@ -1711,11 +1746,11 @@ __global__ __launch_bounds__(kLT) void k_dev_fronts(LdltDev d, DevLevel L, const
         const int q = C.f - C.p, del = C.delayed;
         const int32_t *crow = d.rows + C.rofs + C.p;
         const int32_t *sm = L.sym_cmap + L.sym_cmofs[ch];
-        for (int ii = tid; ii < q; ii += kLT) cpos[ii] = ii < del ? dp + ii : nd + sm[ii - del];
-        for (int t = tid; t < del; t += kLT) rows[dp + t] = crow[t];
+        for (int ii = tid; ii < q; ii += NT) cpos[ii] = ii < del ? dp + ii : nd + sm[ii - del];
+        for (int t = tid; t < del; t += NT) rows[dp + t] = crow[t];
         __syncthreads();
         const double *cb = d.arena + C.fofs + static_cast<int64_t>(C.p) * C.f + C.p;
-        for (int idx = tid; idx < q * q; idx += kLT) {
+        for (int idx = tid; idx < q * q; idx += NT) {
             const int jj = idx / q, ii = idx - jj * q;
             if (ii >= jj) Fs[cpos[jj] * f + cpos[ii]] += cb[static_cast<int64_t>(jj) * C.f + ii];
         }
@@ -1729,7 +1764,7 @@ __global__ __launch_bounds__(kLT) void k_dev_fronts(LdltDev d, DevLevel L, const
         int kind, pa, pb;
         select_pivot_wg(A, k, ne, ne, prm, rs, kind, pa, pb);
         if (kind == 0) break;
-        apply_pivot_wg(A, kind, pa, pb, k, rows, tags, nullptr, 0, 0, f, prm, cnt, flops, 0);
+        apply_pivot_wg<NT>(A, kind, pa, pb, k, rows, tags, nullptr, 0, 0, f, prm, cnt, flops, 0);
     }
     __syncthreads();
     if (tid == 0) {
@@ -1740,12 +1775,12 @@ __global__ __launch_bounds__(kLT) void k_dev_fronts(LdltDev d, DevLevel L, const
         Fd.flops = flops;
     }
     __syncthreads();
-    finish_front_wg(d, Fd, Fs, true);
+    finish_front_wg<NT>(d, Fd, Fs, true);
     if (F.root) return;
     // contribution block (lower part of columns [k, f)) for the parent
     double *G = d.arena + F.fofs;
     const int q = f - k;
-    for (int idx = tid; idx < q * q; idx += kLT) {
+    for (int idx = tid; idx < q * q; idx += NT) {
         const int jj = idx / q, ii = idx - jj * q;
         if (ii >= jj) G[static_cast<int64_t>(k + jj) * f + k + ii] = Fs[(k + jj) * f + k + ii];
     }
@@ -1755,9 +1790,14 @@ void launch_dev_fronts(hipStream_t st, const LdltDev &d, const DevLevel &L, cons
                        int32_t lv, double u, double eff_small, double small_pivot) {
     if (nf <= 0) return;
     Prm prm{u, eff_small, small_pivot};
-    for (int64_t o = 0; o < nf; o += kMaxLaunchGroups)
-        k_dev_fronts<<<dim3(static_cast<unsigned>(std::min<int64_t>(kMaxLaunchGroups, nf - o))), kLT, 0, st>>>(
-            d, L, flist + o, lv, prm);
+    // two size classes: one wavefront and a 48 x 48 square (many fronts per
+    // CU), a workgroup of kLT and the largest square
+    static_assert(kDevFrontMax == 95, "size classes");
+    for (int64_t o = 0; o < nf; o += kMaxLaunchGroups) {
+        const dim3 grid(static_cast<unsigned>(std::min<int64_t>(kMaxLaunchGroups, nf - o)));
+        k_dev_fronts<64, 0, 48><<<grid, 64, 0, st>>>(d, L, flist + o, lv, prm);
+        k_dev_fronts<kLT, 48, 95><<<grid, kLT, 0, st>>>(d, L, flist + o, lv, prm);
+    }
 }
 
 // Stored columns of large fronts: grid (column stride, fronts).

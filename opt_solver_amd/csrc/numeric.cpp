@@ -671,6 +671,7 @@ SMFGPU_API int smfgpu_kernel_times(smfgpu_plan_t P, double *out) {
     out[6] = P->solve_bytes;
     for (int k = 0; k < 4; ++k) out[7 + k] = P->t_solve_phase[k];
     out[11] = static_cast<double>(P->dev_levels);
+    for (int k = 0; k < 4; ++k) out[12 + k] = P->t_ldlt_host[k] * 1e3;
     return SMFGPU_OK;
 }
 

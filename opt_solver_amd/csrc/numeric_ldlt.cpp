@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <chrono>
 #include <cstring>
 
 namespace smfgpu {
@@ -18,6 +19,10 @@ constexpr int kBlockedMinFront = 96;  // ldlt_kernel.hpp:15 (blocked_min_front)
 constexpr int64_t kStoreWide = int64_t{1} << 20;  // stored entries from which a front gets the store kernel
 
 namespace {
+
+double wall_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
 
 template <class T>
 int ensure(DBuf<T> &b, size_t need, bool keep, const char *what = "buffer") {
@@ -150,6 +155,7 @@ static int ldlt_dev_levels(smfgpu_plan_s *P, int32_t lv0, double u, double eff_s
     int32_t retried = -1;
     int rc;
     while (next < end) {
+        const double t_start = wall_s();
         int64_t e_sq = 0, e_fac = 0, e_rows = 0, e_dinv = 0;
         for (int32_t l = next; l < end; ++l)
             for (int32_t s : P->level_fronts[l]) {
@@ -204,6 +210,8 @@ static int ldlt_dev_levels(smfgpu_plan_s *P, int32_t lv0, double u, double eff_s
         HIP_OK(hipGetLastError());
         HIP_OK(hipMemcpyAsync(h, P->d_dctr.p, kDcN * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
         HIP_OK(hipStreamSynchronize(st));
+        const double t_run = wall_s();
+        P->t_ldlt_host[0] += t_run - t_start;
         const int32_t done = static_cast<int32_t>(std::min<unsigned long long>(h[kDcLevel], end));
         if (done > next) {
             const int64_t o0 = P->lv_flist_ofs[next], o1 = P->lv_flist_ofs[done];
@@ -234,6 +242,7 @@ static int ldlt_dev_levels(smfgpu_plan_s *P, int32_t lv0, double u, double eff_s
             P->dev_levels += done - next;
             next = done;
         }
+        P->t_ldlt_host[1] += wall_s() - t_run;
         if (next >= end) break;
         if ((h[kDcWhy] & kDevWhySize) || retried == next) {
             hold = next;
@@ -450,6 +459,7 @@ int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgp
     int64_t work_done = 0;
     P->rows_h.resize(0);
     P->dev_levels = 0;
+    for (double &t : P->t_ldlt_host) t = 0.0;
     // levels of small fronts run on the device unless a limit needs the
     // per-level checks
     const bool dev_on = P->dev_fmax > 0 && !work_armed && opt.max_negative_pivots < 0;
@@ -663,6 +673,7 @@ int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgp
             break;
         }
     }
+    const double t_stats = wall_s();
     // statistics (Info fields, factor_indef.cpp:725-770) over the fronts
     // factored
     fi.num_positive = fi.num_negative = fi.num_zero = 0;
@@ -700,7 +711,10 @@ int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgp
         return SMFGPU_OK;
     }
     fi.status = fi.num_zero > 0 ? SMFGPU_STATUS_SINGULAR : SMFGPU_STATUS_SUCCESS;
+    const double t_lists = wall_s();
+    P->t_ldlt_host[2] = t_lists - t_stats;
     if ((rc = ldlt_solve_lists(P))) return rc;
+    P->t_ldlt_host[3] = wall_s() - t_lists;
     P->ldlt_ready = true;
     return SMFGPU_OK;
 }

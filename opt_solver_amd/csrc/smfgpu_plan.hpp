@@ -219,6 +219,7 @@ struct smfgpu_plan_s {
     std::vector<int32_t> lv_fsym_max;               // per level: largest symbolic front
     int32_t dev_fmax = kDevFrontMax;                // largest realized front of a device level (0: host layout only)
     int32_t dev_levels = 0;                         // levels the last factorization ran on the device
+    double t_ldlt_host[4] = {0, 0, 0, 0};           // s: device levels (run), their read-back, statistics, solve lists
     DBuf<int32_t> d_child_ptr, d_child_idx, d_parent;
     DBuf<unsigned long long> d_dctr;                // device-level counters
     unsigned long long h_dctr[kDcN] = {};

@@ -117,8 +117,9 @@ class Plan:
         st = np.zeros(16)
         check(lib().smfgpu_kernel_times(self.h, _p(st)), "kernel_times")
         keys = ("syrk_ms", "syrk_launches", "syrk_flops", "asm_ms", "asm_bytes", "solve_ms", "solve_bytes",
-                "fwd_ms", "bwd_ms", "fwd_launches", "bwd_launches", "device_levels")
-        return dict(zip(keys, st[:12].tolist()))
+                "fwd_ms", "bwd_ms", "fwd_launches", "bwd_launches", "device_levels", "dev_run_ms",
+                "dev_readback_ms", "stats_ms", "solve_lists_ms")
+        return dict(zip(keys, st[:16].tolist()))
 
     def launch_profile(self):
         """Per launch of the last timed SPD factorization: kind, level, items,
