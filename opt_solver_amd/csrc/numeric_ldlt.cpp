@@ -98,7 +98,8 @@ int ldlt_plan_build(smfgpu_plan_s *P, const smfgpu_analysis_desc *D) {
     P->child_idx.assign(D->child_idx, D->child_idx + P->child_ptr[ns]);
     P->front_row_ptr.assign(D->front_row_ptr, D->front_row_ptr + ns + 1);
     P->front_row_idx.assign(D->front_row_idx, D->front_row_idx + P->front_row_ptr[ns]);
-    P->fl.assign(ns, FrontLd{});
+    int rc;
+    if ((rc = P->fl.assign(ns, FrontLd{}))) return rc;
     // level front lists (all fronts) for the solves
     std::vector<int32_t> lf;
     const int32_t nlev = static_cast<int32_t>(P->level_fronts.size());
@@ -108,10 +109,24 @@ int ldlt_plan_build(smfgpu_plan_s *P, const smfgpu_analysis_desc *D) {
         lf.insert(lf.end(), P->level_fronts[lv].begin(), P->level_fronts[lv].end());
     }
     P->lv_flist_ofs[nlev] = static_cast<int64_t>(lf.size());
+    // room a device level may need: squares, factor store, rows and D^-1
+    // entries of its fronts with up to 8 delayed columns each (a level that
+    // needs more is retried with more room)
     P->lv_fsym_max.assign(nlev, 0);
-    for (int32_t lv = 0; lv < nlev; ++lv)
-        for (int32_t s : P->level_fronts[lv]) P->lv_fsym_max[lv] = std::max(P->lv_fsym_max[lv], P->fronts[s].f);
-    int rc;
+    P->lv_room.assign(4 * static_cast<size_t>(nlev + 1), 0);
+    for (int32_t lv = 0; lv < nlev; ++lv) {
+        int64_t *r = &P->lv_room[4 * static_cast<size_t>(lv + 1)];
+        for (int k = 0; k < 4; ++k) r[k] = r[k - 4];
+        for (int32_t s : P->level_fronts[lv]) {
+            P->lv_fsym_max[lv] = std::max(P->lv_fsym_max[lv], P->fronts[s].f);
+            const int64_t fe = std::min<int64_t>(kDevFrontMax, P->fronts[s].f + 8);
+            const int64_t pe = std::min<int64_t>(fe, P->fronts[s].p + 8);
+            r[0] += RangePool::round(fe * fe);
+            r[1] += fe * pe;
+            r[2] += fe;
+            r[3] += pe;
+        }
+    }
     if ((rc = P->d_flist.upload(lf)) || (rc = P->d_fl.alloc(std::max(ns, 1))) ||
         (rc = P->d_child_ptr.upload(P->child_ptr)) || (rc = P->d_child_idx.upload(P->child_idx)) ||
         (rc = P->d_parent.upload(P->sn_parent)) || (rc = P->d_dctr.alloc(kDcN)))
@@ -156,20 +171,9 @@ static int ldlt_dev_levels(smfgpu_plan_s *P, int32_t lv0, double u, double eff_s
     int rc;
     while (next < end) {
         const double t_start = wall_s();
-        int64_t e_sq = 0, e_fac = 0, e_rows = 0, e_dinv = 0;
-        for (int32_t l = next; l < end; ++l)
-            for (int32_t s : P->level_fronts[l]) {
-                const int64_t fe = std::min<int64_t>(P->dev_fmax, P->fronts[s].f + 8);
-                const int64_t pe = std::min<int64_t>(fe, P->fronts[s].p + 8);
-                e_sq += RangePool::round(fe * fe);
-                e_fac += fe * pe;
-                e_rows += fe;
-                e_dinv += pe;
-            }
-        e_sq *= grow;
-        e_fac *= grow;
-        e_rows *= grow;
-        e_dinv *= grow;
+        const int64_t *r0 = &P->lv_room[4 * static_cast<size_t>(next)], *r1 = &P->lv_room[4 * static_cast<size_t>(end)];
+        const int64_t e_sq = (r1[0] - r0[0]) * grow, e_fac = (r1[1] - r0[1]) * grow;
+        const int64_t e_rows = (r1[2] - r0[2]) * grow, e_dinv = (r1[3] - r0[3]) * grow;
         const int64_t R = P->sq_pool.take(e_sq);
         if ((rc = ensure(P->d_arena, static_cast<size_t>(P->sq_pool.end), true, "front squares")) ||
             (rc = ensure(P->d_fac, static_cast<size_t>(P->fac_used + e_fac), true, "factor store")) ||
@@ -214,24 +218,25 @@ static int ldlt_dev_levels(smfgpu_plan_s *P, int32_t lv0, double u, double eff_s
         P->t_ldlt_host[0] += t_run - t_start;
         const int32_t done = static_cast<int32_t>(std::min<unsigned long long>(h[kDcLevel], end));
         if (done > next) {
-            const int64_t o0 = P->lv_flist_ofs[next], o1 = P->lv_flist_ofs[done];
-            const int32_t nfr = static_cast<int32_t>(o1 - o0);
-            if ((rc = ensure(P->d_stage, static_cast<size_t>(nfr) * sizeof(FrontLd), false, "level staging")))
-                return rc;
-            std::vector<FrontLd> back(nfr);
-            launch_front_gather(st, P->d_fl.p, reinterpret_cast<FrontLd *>(P->d_stage.p), P->d_flist.p + o0, nfr);
-            HIP_OK(hipMemcpyAsync(back.data(), P->d_stage.p, back.size() * sizeof(FrontLd), hipMemcpyDeviceToHost, st));
+            // the whole front array (the host mirror of every front done
+            // before agrees with it; later fronts are laid out again)
+            HIP_OK(hipMemcpyAsync(P->fl.data(), P->d_fl.p, P->fronts.size() * sizeof(FrontLd), hipMemcpyDeviceToHost,
+                                  st));
             HIP_OK(hipStreamSynchronize(st));
             int64_t fac_end = P->fac_used, rows_end = P->rows_used, dinv_end = P->dinv_used;
-            int32_t i = 0;
-            for (int32_t l = next; l < done; ++l)
-                for (int32_t s : P->level_fronts[l]) {
-                    const FrontLd &F = back[i++];
-                    P->fl[s] = F;
-                    fac_end = std::max(fac_end, F.lofs + static_cast<int64_t>(F.f) * F.ne);
-                    rows_end = std::max(rows_end, F.rofs + F.f);
-                    dinv_end = std::max(dinv_end, F.dofs + F.ne);
-                }
+            if (done == end) {
+                fac_end = static_cast<int64_t>(h[kDcFac]);
+                rows_end = static_cast<int64_t>(h[kDcRows]);
+                dinv_end = static_cast<int64_t>(h[kDcDinv]);
+            } else {
+                for (int32_t l = next; l < done; ++l)
+                    for (int32_t s : P->level_fronts[l]) {
+                        const FrontLd &F = P->fl[s];
+                        fac_end = std::max(fac_end, F.lofs + static_cast<int64_t>(F.f) * F.ne);
+                        rows_end = std::max(rows_end, F.rofs + F.f);
+                        dinv_end = std::max(dinv_end, F.dofs + F.ne);
+                    }
+            }
             P->rows_h.resize(rows_end);
             HIP_OK(hipMemcpyAsync(P->rows_h.data() + P->rows_used, P->d_lrows.p + P->rows_used,
                                   (rows_end - P->rows_used) * sizeof(int32_t), hipMemcpyDeviceToHost, st));

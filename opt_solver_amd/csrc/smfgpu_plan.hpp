@@ -52,6 +52,37 @@ struct DBuf {
     size_t bytes() const { return n * sizeof(T); }
 };
 
+// Host array in pinned memory: device-to-host copies land in it directly.
+template <class T>
+struct PinnedArr {
+    T *p = nullptr;
+    size_t n = 0;
+    PinnedArr() = default;
+    PinnedArr(const PinnedArr &) = delete;
+    PinnedArr &operator=(const PinnedArr &) = delete;
+    ~PinnedArr() {
+        if (p) (void)hipHostFree(p);
+    }
+    int assign(size_t count, const T &v) {
+        if (count > n) {
+            if (p) (void)hipHostFree(p);
+            p = nullptr;
+            n = 0;
+            if (hipHostMalloc(reinterpret_cast<void **>(&p), count * sizeof(T), hipHostMallocDefault) != hipSuccess) {
+                p = nullptr;
+                return set_error(SMFGPU_ERR_ALLOC, "pinned host allocation of " + std::to_string(count * sizeof(T)) +
+                                                       " bytes failed");
+            }
+            n = count;
+        }
+        for (size_t i = 0; i < count; ++i) p[i] = v;
+        return SMFGPU_OK;
+    }
+    T &operator[](size_t i) { return p[i]; }
+    const T &operator[](size_t i) const { return p[i]; }
+    T *data() { return p; }
+};
+
 // Host-side best-fit allocator of offsets (in elements) into one device
 // buffer; freed ranges merge with free neighbours. The LDL^T driver takes
 // each front's square from it when the front's level is laid out and gives
@@ -203,7 +234,7 @@ struct smfgpu_plan_s {
     // ---- LDL^T (posdef = 0) ----
     std::vector<int32_t> sn_parent, child_ptr, child_idx, front_row_ptr, front_row_idx;
     std::vector<int32_t> amap_h;                    // host copy of the symbolic-position map
-    std::vector<FrontLd> fl;                        // host mirror of the fronts of the last factorization
+    PinnedArr<FrontLd> fl;                          // host mirror of the fronts of the last factorization
     std::vector<int32_t> rows_h;                    // host mirror of the final row ids (rofs offsets)
     std::vector<int8_t> tags_h;
     DBuf<FrontLd> d_fl;
@@ -217,6 +248,7 @@ struct smfgpu_plan_s {
     int64_t fac_used = 0, rows_used = 0, dinv_used = 0;
     std::vector<int64_t> lv_flist_ofs;              // per level: offset of its fronts in d_flist (solve)
     std::vector<int32_t> lv_fsym_max;               // per level: largest symbolic front
+    std::vector<int64_t> lv_room;                   // prefix over levels of the device-level room estimates (4 per level)
     int32_t dev_fmax = kDevFrontMax;                // largest realized front of a device level (0: host layout only)
     int32_t dev_levels = 0;                         // levels the last factorization ran on the device
     double t_ldlt_host[4] = {0, 0, 0, 0};           // s: device levels (run), their read-back, statistics, solve lists
