@@ -25,21 +25,66 @@ using RedScratch = RedScratchT<kLT>;
 // (max, first index of the max) over the thread's candidates, reduced
 // across the workgroup: ties keep the smaller index, as a sequential scan
 // with a strict '>' does.
+// One wavefront (every lane active): within each row of 16 lanes the
+// partners of four DPP steps (quad swaps of 1 and 2, half-row mirror, row
+// mirror) bring every lane the row's result; the four rows' results are
+// then read from lanes 0, 16, 32, 48. Max and first-argmax do not depend on
+// the order of combination.
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int v) {
+    return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_i<CTRL>(static_cast<int>(b));
+    const int hi = dpp_i<CTRL>(static_cast<int>(b >> 32));
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+__device__ __forceinline__ double lane_d(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane(static_cast<int>(b), l);
+    const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), l);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+constexpr int kDppXor1 = 0xB1, kDppXor2 = 0x4E, kDppHalfMirror = 0x141, kDppMirror = 0x140;
+__device__ __forceinline__ bool better_first(double b, int ib, double a, int ia) {
+    return b > a || (b == a && ib >= 0 && (ia < 0 || ib < ia));
+}
+template <int CTRL>
+__device__ __forceinline__ void maxfirst_step(double &v, int &idx) {
+    const double b = dpp_d<CTRL>(v);
+    const int ib = dpp_i<CTRL>(idx);
+    if (better_first(b, ib, v, idx)) {
+        v = b;
+        idx = ib;
+    }
+}
+template <int CTRL>
+__device__ __forceinline__ void max_step(double &v) {
+    const double b = dpp_d<CTRL>(v);
+    if (b > v) v = b;
+}
+
 template <int NT>
 __device__ __forceinline__ void red_maxfirst(RedScratchT<NT> &rs, double v, int idx, double &outv, int &outi) {
     if constexpr (NT == 64) {
-        // one wavefront: butterfly with the same rule (it is associative and
-        // commutative), then lane 0's result for every lane
-        for (int off = 32; off > 0; off >>= 1) {
-            const double b = __shfl_xor(v, off);
-            const int ib = __shfl_xor(idx, off);
-            if (b > v || (b == v && ib >= 0 && (idx < 0 || ib < idx))) {
-                v = b;
-                idx = ib;
+        maxfirst_step<kDppXor1>(v, idx);
+        maxfirst_step<kDppXor2>(v, idx);
+        maxfirst_step<kDppHalfMirror>(v, idx);
+        maxfirst_step<kDppMirror>(v, idx);
+        double bv = lane_d(v, 0);
+        int bi = __builtin_amdgcn_readlane(idx, 0);
+        for (int r = 16; r < 64; r += 16) {
+            const double c = lane_d(v, r);
+            const int ic = __builtin_amdgcn_readlane(idx, r);
+            if (better_first(c, ic, bv, bi)) {
+                bv = c;
+                bi = ic;
             }
         }
-        outv = __shfl(v, 0);
-        outi = __shfl(idx, 0);
+        outv = bv;
+        outi = bi;
         return;
     }
     const int tid = threadIdx.x;
@@ -65,11 +110,16 @@ __device__ __forceinline__ void red_maxfirst(RedScratchT<NT> &rs, double v, int 
 template <int NT>
 __device__ __forceinline__ double red_max(RedScratchT<NT> &rs, double v) {
     if constexpr (NT == 64) {
-        for (int off = 32; off > 0; off >>= 1) {
-            const double b = __shfl_xor(v, off);
-            if (b > v) v = b;
+        max_step<kDppXor1>(v);
+        max_step<kDppXor2>(v);
+        max_step<kDppHalfMirror>(v);
+        max_step<kDppMirror>(v);
+        double m = lane_d(v, 0);
+        for (int r = 16; r < 64; r += 16) {
+            const double c = lane_d(v, r);
+            if (c > m) m = c;
         }
-        return __shfl(v, 0);
+        return m;
     }
     const int tid = threadIdx.x;
     rs.v[tid] = v;
