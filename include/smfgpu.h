@@ -188,8 +188,16 @@ SMFGPU_API int smfgpu_export_factor(smfgpu_factor_t f, int64_t *sizes, int64_t *
  * each system's inertia and delays. */
 SMFGPU_API int smfgpu_factor_front_stats(smfgpu_factor_t f, int32_t *out);
 
-/* ---- solves: b is n x nrhs, leading dimension ldb >= n, in place,
- * original (unpermuted) ordering. job: 0 full. mem: SMFGPU_MEM_*. */
+/* ---- solves: b is n x nrhs, leading dimension ldb >= n, in place.
+ * job as smf::SolveJob (solver/include/smf/types.hpp:250-256; solve_core,
+ * solver/src/solve.cpp:1050-1068):
+ *   0 full       A x = b, b and x in the original ordering;
+ *   1 forward    x = L^-1 P^T b: b original, x in the permuted ordering;
+ *   2 diagonal   x = D^-1 b, both permuted (the reference's solve_diag:
+ *                division by d, scaled 2x2 solve; a Cholesky factor copies);
+ *   3 backward   x = P L^-T b: b permuted, x original;
+ *   4 diag+back  x = P L^-T D^-1 b: b permuted, x original.
+ * mem: SMFGPU_MEM_*. */
 SMFGPU_API int smfgpu_solve(smfgpu_factor_t f, double *b, int32_t nrhs, int32_t ldb,
                             int32_t job, int32_t mem);
 

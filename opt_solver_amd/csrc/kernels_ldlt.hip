@@ -1923,6 +1923,72 @@ __global__ void k_ldlt_diag(LdltDev d, const int32_t *__restrict__ flist, int32_
     }
 }
 
+// The diagonal-only solve job (solve_diag, solve_diag.cpp:236-261, over
+// the entries process_front pushes, multifrontal_serial.cpp:921-942): D
+// read from the stored columns (d on the diagonal, a 2x2 block's
+// off-diagonal in slot (c, c+1)); a 1x1 pivot divides (a zero or
+// non-finite d, and a root's zero pivot, give 0), a 2x2 block takes
+// solve_2x2_scaled (pivoting.cpp:30-62).
+__global__ void k_ldlt_diag_div(LdltDev d, const int32_t *__restrict__ flist, int32_t nf, double *__restrict__ y,
+                                int32_t n, int32_t nrhs) {
+    const int fi = blockIdx.x;
+    if (fi >= nf) return;
+    const FrontLd F = d.fronts[flist[fi]];
+    const int32_t *rows = d.rows + F.rofs;
+    const int8_t *tags = d.tags + F.rofs;
+    const double *L = d.fac + F.lofs;
+    const int64_t f = F.f;
+    for (int c = threadIdx.x; c < F.pstore; c += blockDim.x) {
+        const int8_t tg = tags[c];
+        if (tg == -1) continue;
+        const double d00 = L[c * f + c];
+        for (int r = 0; r < nrhs; ++r) {
+            double *yr = y + static_cast<int64_t>(r) * n;
+            if (tg != 2) {
+                yr[rows[c]] = (d00 == 0.0 || !isfinite(d00)) ? 0.0 : yr[rows[c]] / d00;
+                continue;
+            }
+            const double d10 = L[(c + 1) * f + c], d11 = L[(c + 1) * f + c + 1];
+            const double x0 = yr[rows[c]], x1 = yr[rows[c + 1]];
+            double s = fabs(d00);
+            if (s < fabs(d10)) s = fabs(d10);
+            if (s < fabs(d11)) s = fabs(d11);
+            double y0 = 0.0, y1 = 0.0;
+            if (isfinite(s) && s != 0.0) {
+                const double a = d00 / s, b = d10 / s, cc = d11 / s;
+                const double det_s = a * cc - b * b;
+                if (isfinite(det_s) && det_s != 0.0) {
+                    if (!isfinite(x0) || !isfinite(x1)) {
+                        y0 = x0;
+                        y1 = x1;
+                    } else {
+                        const double rs = fmax(fabs(x0), fabs(x1));
+                        if (rs != 0.0) {
+                            const double r0 = x0 / rs, r1 = x1 / rs;
+                            const double u0 = (cc * r0 - b * r1) / det_s;
+                            const double u1 = (-b * r0 + a * r1) / det_s;
+                            const double ratio = rs / s;
+                            y0 = u0 * ratio;
+                            y1 = u1 * ratio;
+                        }
+                    }
+                }
+            }
+            yr[rows[c]] = y0;
+            yr[rows[c + 1]] = y1;
+        }
+    }
+}
+
+void launch_ldlt_diag_div(hipStream_t st, const LdltDev &d, const int32_t *flist, int32_t nf, double *y, int32_t n,
+                          int32_t nrhs) {
+    if (nf <= 0) return;
+    for (int64_t o = 0; o < nf; o += kMaxLaunchGroups) {
+        const int32_t m = static_cast<int32_t>(std::min<int64_t>(kMaxLaunchGroups, nf - o));
+        k_ldlt_diag_div<<<dim3(static_cast<unsigned>(m)), 64, 0, st>>>(d, flist + o, m, y, n, nrhs);
+    }
+}
+
 void launch_ldlt_diag(hipStream_t st, const LdltDev &d, const int32_t *flist, int32_t nf, double *y, int32_t n,
                       int32_t nrhs) {
     if (nf <= 0) return;

@@ -928,9 +928,10 @@ SMFGPU_API int smfgpu_export_factor(smfgpu_factor_t f, int64_t *sizes, int64_t *
     return SMFGPU_OK;
 }
 
+// job as ldlt_solve; for a Cholesky factor D = I (the diagonal job copies)
 static int solve_device(smfgpu_plan_s *P, const double *b_dev, int64_t ldb, double *out_dev, int64_t ldo,
-                        int32_t nrhs) {
-    if (!P->posdef) return ldlt_solve(P, b_dev, ldb, out_dev, ldo, nrhs);
+                        int32_t nrhs, int32_t job = 0) {
+    if (!P->posdef) return ldlt_solve(P, b_dev, ldb, out_dev, ldo, nrhs, job);
     hipStream_t st = P->stream;
     const int32_t n = P->n;
     DevPtrs d = dev_ptrs(P);
@@ -941,8 +942,12 @@ static int solve_device(smfgpu_plan_s *P, const double *b_dev, int64_t ldb, doub
             const int rc = P->d_y.alloc(static_cast<size_t>(n) * 4);
             if (rc) return rc;
         }
-        launch_gather_rhs(st, P->d_perm.p, b_dev + r0 * ldb, ldb, P->d_y.p, n, nr);
-        const bool tm = P->timing;
+        const bool fwd = job == 0 || job == 1;
+        if (fwd) launch_gather_rhs(st, P->d_perm.p, b_dev + r0 * ldb, ldb, P->d_y.p, n, nr);
+        else
+            HIP_OK(hipMemcpy2DAsync(P->d_y.p, n * sizeof(double), b_dev + r0 * ldb, ldb * sizeof(double),
+                                    n * sizeof(double), nr, hipMemcpyDeviceToDevice, st));
+        const bool tm = P->timing && job == 0;
         if (tm && P->sev.size() < 3) {
             for (size_t i = P->sev.size(); i < 3; ++i) {
                 hipEvent_t e;
@@ -951,11 +956,17 @@ static int solve_device(smfgpu_plan_s *P, const double *b_dev, int64_t ldb, doub
             }
         }
         if (tm) HIP_OK(hipEventRecord(P->sev[0], st));
-        for (int32_t lv = 0; lv < nl; ++lv)
-            for (int32_t i = P->fwd_lv[lv]; i < P->fwd_lv[lv + 1]; ++i)
-                launch_fwd_step(st, d, P->d_fwd_items.p + P->fwd_steps[i].first, P->fwd_steps[i].second, P->d_y.p, n,
-                                nr);
+        if (fwd)
+            for (int32_t lv = 0; lv < nl; ++lv)
+                for (int32_t i = P->fwd_lv[lv]; i < P->fwd_lv[lv + 1]; ++i)
+                    launch_fwd_step(st, d, P->d_fwd_items.p + P->fwd_steps[i].first, P->fwd_steps[i].second, P->d_y.p,
+                                    n, nr);
         if (tm) HIP_OK(hipEventRecord(P->sev[1], st));
+        if (job == 1 || job == 2) {
+            HIP_OK(hipMemcpy2DAsync(out_dev + r0 * ldo, ldo * sizeof(double), P->d_y.p, n * sizeof(double),
+                                    n * sizeof(double), nr, hipMemcpyDeviceToDevice, st));
+            continue;
+        }
         for (int32_t lv = nl - 1; lv >= 0; --lv)
             for (int32_t i = P->bwd_lv[lv]; i < P->bwd_lv[lv + 1]; ++i)
                 launch_bwd_step(st, d, P->d_bwd_items.p + P->bwd_steps[i].first, P->bwd_steps[i].second, P->d_y.p, n,
@@ -981,7 +992,7 @@ SMFGPU_API int smfgpu_solve(smfgpu_factor_t f, double *b, int32_t nrhs, int32_t 
     if (!f || !f->plan || !b) return set_error(SMFGPU_ERR_INVALID_ARG, "null argument");
     smfgpu_plan_s *P = f->plan;
     if (P->live != f || !f->valid) return set_error(SMFGPU_ERR_STATE, "solve: factor not usable");
-    if (job != 0) return set_error(SMFGPU_ERR_UNSUPPORTED, "solve: only the full solve (job 0)");
+    if (job < 0 || job > 4) return set_error(SMFGPU_ERR_INVALID_ARG, "solve: job must be 0..4");
     if (nrhs <= 0 || ldb < P->n) return set_error(SMFGPU_ERR_INVALID_ARG, "solve: bad nrhs/ldb");
     hipStream_t st = P->stream;
     const int32_t n = P->n;
@@ -993,14 +1004,14 @@ SMFGPU_API int smfgpu_solve(smfgpu_factor_t f, double *b, int32_t nrhs, int32_t 
     int rc;
     if (mem == SMFGPU_MEM_DEVICE) {
         if (P->timing) HIP_OK(hipEventRecord(e0, st));
-        rc = solve_device(P, b, ldb, b, ldb, nrhs);
+        rc = solve_device(P, b, ldb, b, ldb, nrhs, job);
         if (P->timing) HIP_OK(hipEventRecord(e1, st));
     } else {
         if (P->d_b.n < static_cast<size_t>(n) * nrhs && (rc = P->d_b.alloc(static_cast<size_t>(n) * nrhs))) return rc;
         HIP_OK(hipMemcpy2DAsync(P->d_b.p, n * sizeof(double), b, ldb * sizeof(double), n * sizeof(double), nrhs,
                                 hipMemcpyHostToDevice, st));
         if (P->timing) HIP_OK(hipEventRecord(e0, st));
-        rc = solve_device(P, P->d_b.p, n, P->d_b.p, n, nrhs);
+        rc = solve_device(P, P->d_b.p, n, P->d_b.p, n, nrhs, job);
         if (P->timing) HIP_OK(hipEventRecord(e1, st));
         HIP_OK(hipMemcpy2DAsync(b, ldb * sizeof(double), P->d_b.p, n * sizeof(double), n * sizeof(double), nrhs,
                                 hipMemcpyDeviceToHost, st));

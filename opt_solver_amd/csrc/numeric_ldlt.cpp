@@ -724,12 +724,13 @@ int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgp
     return SMFGPU_OK;
 }
 
-int ldlt_solve(smfgpu_plan_s *P, const double *b_dev, int64_t ldb, double *out_dev, int64_t ldo, int32_t nrhs) {
+int ldlt_solve(smfgpu_plan_s *P, const double *b_dev, int64_t ldb, double *out_dev, int64_t ldo, int32_t nrhs,
+               int32_t job) {
     hipStream_t st = P->stream;
     const int32_t n = P->n;
     LdltDev d = ldlt_dev(P);
     const int32_t nl = static_cast<int32_t>(P->level_fronts.size());
-    const bool tm = P->timing;
+    const bool tm = P->timing && job == 0;
     if (tm && P->sev.size() < 3) {
         for (size_t i = P->sev.size(); i < 3; ++i) {
             hipEvent_t e;
@@ -743,13 +744,26 @@ int ldlt_solve(smfgpu_plan_s *P, const double *b_dev, int64_t ldb, double *out_d
             const int rc = P->d_y.alloc(static_cast<size_t>(n) * 16);
             if (rc) return rc;
         }
-        launch_gather_rhs(st, P->d_perm.p, b_dev + r0 * ldb, ldb, P->d_y.p, n, nr);
+        const bool fwd = job == 0 || job == 1;
+        if (fwd) launch_gather_rhs(st, P->d_perm.p, b_dev + r0 * ldb, ldb, P->d_y.p, n, nr);
+        else
+            HIP_OK(hipMemcpy2DAsync(P->d_y.p, n * sizeof(double), b_dev + r0 * ldb, ldb * sizeof(double),
+                                    n * sizeof(double), nr, hipMemcpyDeviceToDevice, st));
         if (tm) HIP_OK(hipEventRecord(P->sev[0], st));
-        for (int32_t lv = 0; lv < nl; ++lv)
-            for (int32_t i = P->fwd_lv[lv]; i < P->fwd_lv[lv + 1]; ++i)
-                launch_ldlt_fwd_step(st, d, P->d_fwd_items.p + P->fwd_steps[i].first, P->fwd_steps[i].second,
-                                     P->d_y.p, n, nr);
-        launch_ldlt_diag(st, d, P->d_flist.p, static_cast<int32_t>(P->lv_flist_ofs[nl]), P->d_y.p, n, nr);
+        if (fwd)
+            for (int32_t lv = 0; lv < nl; ++lv)
+                for (int32_t i = P->fwd_lv[lv]; i < P->fwd_lv[lv + 1]; ++i)
+                    launch_ldlt_fwd_step(st, d, P->d_fwd_items.p + P->fwd_steps[i].first, P->fwd_steps[i].second,
+                                         P->d_y.p, n, nr);
+        if (job == 2) launch_ldlt_diag_div(st, d, P->d_flist.p, static_cast<int32_t>(P->lv_flist_ofs[nl]), P->d_y.p, n, nr);
+        if (job == 1 || job == 2) {
+            // the permuted-order result
+            HIP_OK(hipMemcpy2DAsync(out_dev + r0 * ldo, ldo * sizeof(double), P->d_y.p, n * sizeof(double),
+                                    n * sizeof(double), nr, hipMemcpyDeviceToDevice, st));
+            continue;
+        }
+        if (job == 0 || job == 4)
+            launch_ldlt_diag(st, d, P->d_flist.p, static_cast<int32_t>(P->lv_flist_ofs[nl]), P->d_y.p, n, nr);
         if (tm) HIP_OK(hipEventRecord(P->sev[1], st));
         for (int32_t lv = nl - 1; lv >= 0; --lv)
             for (int32_t i = P->bwd_lv[lv]; i < P->bwd_lv[lv + 1]; ++i)

@@ -121,6 +121,9 @@ void launch_ldlt_epilogue(hipStream_t st, const LdltDev &d, const int32_t *flist
 void launch_ldlt_store(hipStream_t st, const LdltDev &d, const int32_t *flist, int32_t nf, int32_t max_cols);
 void launch_ldlt_diag(hipStream_t st, const LdltDev &d, const int32_t *flist, int32_t nf, double *y, int32_t n,
                       int32_t nrhs);
+// the diagonal-only solve job: D from the stored columns, by division
+void launch_ldlt_diag_div(hipStream_t st, const LdltDev &d, const int32_t *flist, int32_t nf, double *y, int32_t n,
+                          int32_t nrhs);
 void launch_front_patch(hipStream_t st, FrontLd *fronts, const FrontLd *patch, const int32_t *ids, int32_t np);
 // one step of the blocked forward / backward sweep (items of all fronts of
 // a level; SolveItem flags bit 1: triangle item)

@@ -270,6 +270,10 @@ namespace smfgpu {
 int ldlt_plan_build(smfgpu_plan_s *P, const smfgpu_analysis_desc *D);
 int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgpu_factor_options &opt,
                 smfgpu_factor_info &fi);
-int ldlt_solve(smfgpu_plan_s *P, const double *b_dev, int64_t ldb, double *out_dev, int64_t ldo, int32_t nrhs);
+// job: SolveJob (types.hpp:250-256): 0 full, 1 forward (in: original
+// order, out: permuted), 2 diagonal only, 3 backward, 4 diagonal then
+// backward (in: permuted order; 3 and 4 out: original)
+int ldlt_solve(smfgpu_plan_s *P, const double *b_dev, int64_t ldb, double *out_dev, int64_t ldo, int32_t nrhs,
+               int32_t job);
 int ldlt_export(smfgpu_plan_s *P, int64_t *sizes, int64_t *steps, double *fv, int32_t *rows, int8_t *tags);
 }  // namespace smfgpu

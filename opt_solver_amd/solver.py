@@ -16,6 +16,9 @@ import numpy as np
 from . import _lib
 from ._lib import MEM_DEVICE, MEM_HOST, PREC_BF16, PREC_FP32, PREC_FP64, SmfGpuError, check, lib
 
+# smf::SolveJob (solver/include/smf/types.hpp:250-256)
+SOLVE_FULL, SOLVE_FORWARD, SOLVE_DIAG, SOLVE_BACKWARD, SOLVE_DIAG_BACK = 0, 1, 2, 3, 4
+
 __all__ = ["analyse", "Analysis", "Plan", "Factor", "SmfGpuError", "PREC_FP64", "PREC_FP32", "PREC_BF16"]
 
 
@@ -178,10 +181,13 @@ class Factor:
         self.flops = float(info.actual_flops)
         self.seconds = float(info.factor_seconds)
 
-    def solve(self, b):
+    def solve(self, b, job=SOLVE_FULL):
+        """job: SOLVE_FULL, SOLVE_FORWARD (result in the permuted ordering),
+        SOLVE_DIAG (permuted in and out), SOLVE_BACKWARD and SOLVE_DIAG_BACK
+        (permuted in, original out), as smf's SolveJob."""
         b = np.array(b, dtype=np.float64, order="F", copy=True)
         nrhs = 1 if b.ndim == 1 else b.shape[1]
-        check(lib().smfgpu_solve(self.h, _p(b), nrhs, self.plan.n, 0, MEM_HOST), "solve")
+        check(lib().smfgpu_solve(self.h, _p(b), nrhs, self.plan.n, int(job), MEM_HOST), "solve")
         return b
 
     def front_stats(self):
@@ -190,8 +196,8 @@ class Factor:
         check(lib().smfgpu_factor_front_stats(self.h, _p(out)), "front_stats")
         return out
 
-    def solve_device(self, ptr, nrhs, ldb=None):
-        check(lib().smfgpu_solve(self.h, ctypes.c_void_p(ptr), nrhs, ldb or self.plan.n, 0, MEM_DEVICE),
+    def solve_device(self, ptr, nrhs, ldb=None, job=SOLVE_FULL):
+        check(lib().smfgpu_solve(self.h, ctypes.c_void_p(ptr), nrhs, ldb or self.plan.n, int(job), MEM_DEVICE),
               "solve")
 
     def refine(self, b, x, max_iters=10, tol=1e-14):

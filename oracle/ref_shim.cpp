@@ -254,6 +254,13 @@ int ref_solve(void *handle, double *b, int n, int nrhs, double *seconds) {
     return rc;
 }
 
+// Solver::solve with a SolveJob (solver.hpp:101-114, types.hpp:250-256).
+int ref_solve_job(void *handle, double *b, int n, int nrhs, int job) {
+    auto *h = static_cast<RefFactor *>(handle);
+    Solver s;
+    return s.solve(h->fk, h->ctrl, h->info, b, n, nrhs, static_cast<SolveJob>(job));
+}
+
 void ref_factor_free(void *handle) { delete static_cast<RefFactor *>(handle); }
 
 // The reference's dense TPP / in-order LDL^T front kernel on F (f x f,

@@ -46,6 +46,8 @@ def lib():
         L.ref_pivot_types_size.argtypes = [_P]
         L.ref_solve.restype = _I
         L.ref_solve.argtypes = [_P, _P, _I, _I, _P]
+        L.ref_solve_job.restype = _I
+        L.ref_solve_job.argtypes = [_P, _P, _I, _I, _I]
         L.ref_factor_free.argtypes = [_P]
         L.ref_ldlt_front.restype = _I
         L.ref_ldlt_front.argtypes = [_P, _I, _I, _P, _D, _D, _D, _I, _I, _D, _P, _P]
@@ -153,6 +155,15 @@ class RefFactor:
         if rc != 0:
             raise RuntimeError(f"reference solve returned {rc}")
         self.solve_seconds = float(sec[0])
+        return b
+
+    def solve_job(self, b, job):
+        """The reference's partial solve jobs (SolveJob 1..4) on a copy of b."""
+        b = np.array(b, dtype=np.float64, order="F", copy=True)
+        nrhs = 1 if b.ndim == 1 else b.shape[1]
+        rc = lib().ref_solve_job(self.h, _ptr(b), self.an.n, nrhs, int(job))
+        if rc != 0:
+            raise RuntimeError(f"reference solve job {job} returned {rc}")
         return b
 
     def __del__(self):

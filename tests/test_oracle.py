@@ -201,3 +201,25 @@ def test_kkt_inertia_sylvester(oracle_lib):
         an = analyse(A, perm=None, nemin=8)
         of = oracle_lib.Factor(an.arrays(), A[3], posdef=False)
         assert of.inertia == inertia
+
+
+def test_reference_solve_jobs_compose(ref_lib):
+    """The reference's partial solve jobs (SolveJob 1..4), which the GPU job
+    tests compare with: forward then diagonal-and-backward, and forward,
+    diagonal, backward (division by d instead of the D^-1 table) give the
+    full solve to rounding (the full solve of several right-hand sides takes
+    its own route); for a Cholesky factor the diagonal job is the identity."""
+    from opt_solver_amd import generators as g
+    for name, A, posdef, order in _cases(g, ref_lib):
+        an = ref_lib.RefAnalysis(A, ordering=order, posdef=posdef)
+        rf = ref_lib.RefFactor(an, A[3])
+        b = np.random.default_rng(2).uniform(-1, 1, (A[0], 2))
+        full = rf.solve(b)
+        y = rf.solve_job(b, 1)
+        db = rf.solve_job(y, 4)
+        assert np.abs(db - full).max() <= 1e-13 * np.abs(full).max(), name
+        d = rf.solve_job(y, 2)
+        if posdef:
+            assert np.array_equal(d, y), name
+        x = rf.solve_job(d, 3)
+        assert np.abs(x - full).max() <= 1e-12 * np.abs(full).max(), name
