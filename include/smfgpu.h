@@ -117,8 +117,23 @@ typedef struct {
     double work_limit;       /* sum of num_elig * f^2 allowed (the scale-on-delay
                                 threshold, multifrontal_serial.cpp:653-671);
                                 a stopped run reports SMFGPU_STATUS_MAX_PIVOT_DELAYS */
-    int32_t reserved[4];
+    /* Static pivoting (LDL^T; control.hpp:333-364, factor_indef.cpp:642-645):
+     * static_pivot_tol > 0: a column with no acceptable 1x1/2x2 pivot is
+     * taken as a 1x1 pivot, |d| raised to static_pivot_tol * max|A|
+     * (force_static_pivot, ldlt_kernel.cpp:285-320); no delays.
+     * fallback_static_pivot_tol > 0: the same only in roots and in fronts
+     * that receive more delayed columns than their own width, with hybrid
+     * inertia (multifrontal_serial.cpp:562, 615-629). */
+    double static_pivot_tol;
+    double fallback_static_pivot_tol;
+    int32_t static_flags;    /* SMFGPU_STATIC_* bits */
+    int32_t reserved[3];
 } smfgpu_factor_options;
+
+/* static_flags (control.hpp:349-364) */
+#define SMFGPU_STATIC_AS_ZERO 1         /* static_pivots_as_zero */
+#define SMFGPU_STATIC_ZERO_STAYS_ZERO 2 /* static_zero_stays_zero */
+#define SMFGPU_STATIC_IN_ORDER 4        /* static_pivot_in_order (no search) */
 
 /* Factor statistics (the smf::Info fields of a factorization,
  * solver/include/smf/info.hpp:24-221). */
@@ -136,7 +151,10 @@ typedef struct {
     double max_abs;          /* max |a_ij| of the input */
     double factor_seconds;   /* device time of the numeric factorization */
     int64_t factor_entries;  /* sum f * p */
-    double reserved[4];
+    int32_t static_pivots;   /* pivots whose |d| static pivoting raised */
+    int32_t pad;
+    double pivot_growth;     /* largest column max / |d| over static pivots */
+    double reserved[2];
 } smfgpu_factor_info;
 
 SMFGPU_API const char *smfgpu_error_string(void);

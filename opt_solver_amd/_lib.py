@@ -38,7 +38,8 @@ class FactorOptions(ctypes.Structure):
     _fields_ = [
         ("posdef", _I), ("pivot_u", _D), ("small_pivot", _D), ("panel", _I), ("precision", _I),
         ("mixed_min_front", _I), ("max_negative_pivots", _I), ("value_mem", _I),
-        ("work_limit_on", _I), ("work_limit", _D), ("reserved", _I * 4),
+        ("work_limit_on", _I), ("work_limit", _D), ("static_pivot_tol", _D),
+        ("fallback_static_pivot_tol", _D), ("static_flags", _I), ("reserved", _I * 3),
     ]
 
 
@@ -47,7 +48,8 @@ class FactorInfo(ctypes.Structure):
         ("status", _I), ("num_positive", _I), ("num_negative", _I), ("num_zero", _I),
         ("delayed_pivots", _I), ("two_by_two_pivots", _I), ("max_front", _I), ("fronts_reduced", _I),
         ("actual_flops", _D), ("front_work", _D), ("max_abs", _D), ("factor_seconds", _D),
-        ("factor_entries", _L), ("reserved", _D * 4),
+        ("factor_entries", _L), ("static_pivots", _I), ("pad", _I), ("pivot_growth", _D),
+        ("reserved", _D * 2),
     ]
 
 
@@ -58,6 +60,9 @@ EXPORTED = [
     "smfgpu_refine", "smfgpu_set_timing", "smfgpu_kernel_times", "smfgpu_plan_tune",
     "smfgpu_factor_front_stats", "smfgpu_launch_profile",
 ]
+STATIC_AS_ZERO = 1
+STATIC_ZERO_STAYS_ZERO = 2
+STATIC_IN_ORDER = 4
 STATUS_SUCCESS = 0
 STATUS_NOT_POSDEF = 1
 STATUS_SINGULAR = 2

@@ -141,9 +141,34 @@ struct FV {
     __device__ __forceinline__ double sym(int i, int j) const { return i >= j ? at(i, j) : at(j, i); }
 };
 
+// min_abs: the front's in-order static pivot (static_pivot_in_order, 0 off);
+// S: static pivoting of the launch (fronts pick theirs with
+// front_static_value)
 struct Prm {
     double u, eff_small, small_pivot;
+    double min_abs;
+    StaticPrm S;
 };
+
+// The pivoting parameters of front F: its static value (0: none) and, in
+// the in-order mode, min_abs (ldlt_kernel.cpp:1131-1132 and
+// multifrontal_serial.cpp:624-629: a fallback front keeps in-order only when
+// static pivoting is on overall).
+__device__ __forceinline__ double front_prm(const Prm &prm, const FrontLd &F, Prm &fp) {
+    fp = prm;
+    const double sp = front_static_value(prm.S, F.root, F.nd, F.psym);
+    fp.min_abs = (prm.S.sp > 0.0 && (prm.S.flags & kStaticInOrder)) ? sp : 0.0;
+    return sp;
+}
+
+// static_counts_as_zero (ldlt_kernel.cpp:325-328): a fallback front keeps a
+// pivot raised from an exact zero as a zero (static_zero_stays_zero set).
+__device__ __forceinline__ bool static_as_zero(int raised, const Prm &prm, const FrontLd &F) {
+    if (raised == 0) return false;
+    const bool fallback = prm.S.fallback > 0.0 && (F.root != 0 || F.nd > F.psym);
+    const bool stays = fallback || (prm.S.flags & kStaticZeroStaysZero);
+    return (prm.S.flags & kStaticAsZero) || (raised == 2 && stays);
+}
 
 // select_pivot (ldlt_kernel.cpp:127-275) with workgroup reductions. Every
 // thread returns the same choice.
@@ -154,6 +179,15 @@ __device__ void select_pivot_wg(const FV &A, int k, int ne, int limit, const Prm
     const int f = A.f;
     kind = 0;
     pa = pb = -1;
+    if (prm.min_abs > 0.0) {
+        // in order, no search: column k or nothing (then the static path)
+        const double d0 = A.at(k, k);
+        if (isfinite(d0) && fabs(d0) >= prm.min_abs) {
+            kind = 1;
+            pa = k;
+        }
+        return;
+    }
     if (k < limit) {
         const double a0 = fabs(A.at(k, k));
         if (a0 >= prm.eff_small) {
@@ -436,15 +470,58 @@ __device__ void inertia_2x2(double d00, double d10, double d11, double small, in
     }
 }
 
+// force_static_pivot (ldlt_kernel.cpp:285-320): the column of [k, ne)
+// whose diagonal is largest relative to its column (first on ties; column k
+// in the in-order mode), |d| raised to sp keeping its sign when smaller.
+// raised: 0 as it is, 1 from a nonzero |d|, 2 from an exact zero (below
+// eff_small or not finite). Every thread returns the same column.
+template <int NT>
+__device__ int force_static_wg(const FV &A, int k, int ne, double sp, const Prm &prm, RedScratchT<NT> &rs,
+                               int &raised, int &nstatic, double &growth) {
+    const int tid = threadIdx.x;
+    const int f = A.f;
+    int best = k;
+    double best_ratio = -1.0, best_colmax = 0.0;
+    const int search_end = prm.min_abs > 0.0 ? k : ne;
+    for (int cand = k; cand < search_end; ++cand) {
+        double m = 0.0;
+        for (int i = k + tid; i < cand; i += NT) m = fmax(m, fabs(A.at(cand, i)));
+        for (int i = cand + 1 + tid; i < f; i += NT) m = fmax(m, fabs(A.at(i, cand)));
+        const double colmax = red_max(rs, m);
+        const double a = fabs(A.at(cand, cand));
+        const double ratio = colmax > 0.0 ? a / colmax : a;
+        if (ratio > best_ratio) {
+            best_ratio = ratio;
+            best = cand;
+            best_colmax = colmax;
+        }
+    }
+    __syncthreads();
+    double dv = A.at(best, best);
+    raised = 0;
+    if (!(fabs(dv) >= sp)) {
+        raised = (isfinite(dv) && fabs(dv) >= prm.eff_small) ? 1 : 2;
+        dv = dv < 0.0 ? -sp : sp;
+        ++nstatic;
+        __syncthreads();
+        if (tid == 0) A.at(best, best) = dv;
+    }
+    if (best_colmax > 0.0 && fabs(dv) > 0.0) growth = fmax(growth, best_colmax / fabs(dv));
+    __syncthreads();
+    return best;
+}
+
 // apply_pivot (ldlt_kernel.cpp:665-698). Thread 0 keeps the counters.
 template <int NT = kLT>
 __device__ void apply_pivot_wg(const FV &A, int kind, int pa, int pb, int &k, int32_t *rows, int8_t *tags, double *W,
-                               int wld, int w_cols, int ucol_end, const Prm &prm, int *cnt, double &flops, int c_lo) {
+                               int wld, int w_cols, int ucol_end, const Prm &prm, int *cnt, double &flops, int c_lo,
+                               bool as_zero = false) {
     const int f = A.f;
     if (kind == 1) {
         sym_swap_wg<NT>(A, k, pa, rows, W, wld, w_cols, c_lo);
         if (threadIdx.x == 0) {
-            inertia_1x1(A.at(k, k), prm.eff_small, cnt);
+            if (as_zero) ++cnt[2];
+            else inertia_1x1(A.at(k, k), prm.eff_small, cnt);
             tags[k] = 1;
         }
         elim1_wg<NT>(A, k, ucol_end, W, wld, w_cols);
@@ -493,11 +570,22 @@ __global__ __launch_bounds__(kLT) void k_ldlt_small(LdltDev d, const int32_t *__
     FV A{Fs, f, f};
     int k = 0;
     double flops = 0.0;
+    Prm fp;
+    const double sp = front_prm(prm, Fd, fp);
+    int nstatic = 0;
+    double growth = 0.0;
     while (k < ne) {
         int kind, pa, pb;
-        select_pivot_wg(A, k, ne, ne, prm, rs, kind, pa, pb);
-        if (kind == 0) break;
-        apply_pivot_wg(A, kind, pa, pb, k, rows, tags, nullptr, 0, 0, f, prm, cnt, flops, 0);
+        select_pivot_wg(A, k, ne, ne, fp, rs, kind, pa, pb);
+        bool az = false;
+        if (kind == 0) {
+            if (!(sp > 0.0)) break;
+            int raised;
+            pa = force_static_wg(A, k, ne, sp, fp, rs, raised, nstatic, growth);
+            kind = 1;
+            az = static_as_zero(raised, fp, Fd);
+        }
+        apply_pivot_wg(A, kind, pa, pb, k, rows, tags, nullptr, 0, 0, f, fp, cnt, flops, 0, az);
     }
     __syncthreads();
     for (int idx = tid; idx < f * f; idx += kLT) G[idx] = Fs[idx];
@@ -507,13 +595,15 @@ __global__ __launch_bounds__(kLT) void k_ldlt_small(LdltDev d, const int32_t *__
         Fd.nneg = cnt[1];
         Fd.nzero = cnt[2];
         Fd.flops = flops;
+        Fd.nstatic = nstatic;
+        Fd.growth = growth;
     }
 }
 
 void launch_ldlt_small(hipStream_t st, const LdltDev &d, const int32_t *flist, int32_t nf, double u, double eff_small,
-                       double small_pivot) {
+                       double small_pivot, const StaticPrm &S) {
     if (nf <= 0) return;
-    Prm prm{u, eff_small, small_pivot};
+    Prm prm{u, eff_small, small_pivot, 0.0, S};
     for (int64_t o = 0; o < nf; o += kMaxLaunchGroups)
         k_ldlt_small<<<dim3(static_cast<unsigned>(std::min<int64_t>(kMaxLaunchGroups, nf - o))), kLT, 0, st>>>(
             d, flist + o, prm);
@@ -587,7 +677,14 @@ __global__ __launch_bounds__(kLT) void k_ldlt_blocked(LdltDev d, const int32_t *
     __syncthreads();
     int k = 0;
     double flops = 0.0;
-    {
+    Prm fp;
+    const double sp = front_prm(prm, Fd, fp);
+    const Prm &prm_f = fp;
+    int nstatic = 0;
+    double growth = 0.0;
+    // the first pass with stall handling is for threshold pivoting only
+    // (ldlt_kernel.cpp:1028); static fronts go straight to the second
+    if (!(sp > 0.0) && !(fp.min_abs > 0.0)) {
         int live = ne, k_paired = -1;
         while (k < live) {
             const int ps = k;
@@ -595,12 +692,12 @@ __global__ __launch_bounds__(kLT) void k_ldlt_blocked(LdltDev d, const int32_t *
             bool stalled = false;
             while (k < pl) {
                 int kind, pa, pb;
-                select_pivot_wg(A, k, ne, pl, prm, rs, kind, pa, pb);
+                select_pivot_wg(A, k, ne, pl, prm_f, rs, kind, pa, pb);
                 if (kind == 0) {
                     stalled = true;
                     break;
                 }
-                apply_pivot_wg(A, kind, pa, pb, k, rows, tags, W, wld, k - ps, pl, prm, cnt, flops, 0);
+                apply_pivot_wg(A, kind, pa, pb, k, rows, tags, W, wld, k - ps, pl, prm_f, cnt, flops, 0);
             }
             deferred_update_wg(A, ps, k, pl, ne, W, wld);
             if (stalled && k != k_paired) {
@@ -657,20 +754,27 @@ __global__ __launch_bounds__(kLT) void k_ldlt_blocked(LdltDev d, const int32_t *
         bool stalled = false;
         while (k < pl) {
             int kind, pa, pb;
-            select_pivot_wg(A, k, ne, pl, prm, rs, kind, pa, pb);
+            select_pivot_wg(A, k, ne, pl, prm_f, rs, kind, pa, pb);
             if (kind == 0) {
                 stalled = true;
                 break;
             }
-            apply_pivot_wg(A, kind, pa, pb, k, rows, tags, W, wld, k - ps, pl, prm, cnt, flops, 0);
+            apply_pivot_wg(A, kind, pa, pb, k, rows, tags, W, wld, k - ps, pl, prm_f, cnt, flops, 0);
         }
         deferred_update_wg(A, ps, k, pl, ne, W, wld);
         if (stalled && k < ne) {
             const int ss = k;
             int kind, pa, pb;
-            select_pivot_wg(A, k, ne, ne, prm, rs, kind, pa, pb);
-            if (kind == 0) break;
-            apply_pivot_wg(A, kind, pa, pb, k, rows, tags, W, wld, 0, ss, prm, cnt, flops, 0);
+            select_pivot_wg(A, k, ne, ne, prm_f, rs, kind, pa, pb);
+            bool az = false;
+            if (kind == 0) {
+                if (!(sp > 0.0)) break;
+                int raised;
+                pa = force_static_wg(A, k, ne, sp, prm_f, rs, raised, nstatic, growth);
+                kind = 1;
+                az = static_as_zero(raised, prm_f, Fd);
+            }
+            apply_pivot_wg(A, kind, pa, pb, k, rows, tags, W, wld, 0, ss, prm_f, cnt, flops, 0, az);
             deferred_update_wg(A, ss, k, k, ne, W, wld);
         }
     }
@@ -681,13 +785,15 @@ __global__ __launch_bounds__(kLT) void k_ldlt_blocked(LdltDev d, const int32_t *
         Fd.nneg = cnt[1];
         Fd.nzero = cnt[2];
         Fd.flops = flops;
+        Fd.nstatic = nstatic;
+        Fd.growth = growth;
     }
 }
 
 void launch_ldlt_blocked(hipStream_t st, const LdltDev &d, const int32_t *flist, int32_t nf, double u,
-                         double eff_small, double small_pivot, int32_t nb) {
+                         double eff_small, double small_pivot, int32_t nb, const StaticPrm &S) {
     if (nf <= 0) return;
-    Prm prm{u, eff_small, small_pivot};
+    Prm prm{u, eff_small, small_pivot, 0.0, S};
     for (int64_t o = 0; o < nf; o += kMaxLaunchGroups)
         k_ldlt_blocked<<<dim3(static_cast<unsigned>(std::min<int64_t>(kMaxLaunchGroups, nf - o))), kLT, 0, st>>>(
             d, flist + o, prm, nb);
@@ -1810,11 +1916,22 @@ __global__ __launch_bounds__(NT) void k_dev_fronts(LdltDev d, DevLevel L, const 
     FV A{Fs, f, f};
     int k = 0;
     double flops = 0.0;
+    Prm fp;
+    const double sp = front_prm(prm, F, fp);
+    int nstatic = 0;
+    double growth = 0.0;
     while (k < ne) {
         int kind, pa, pb;
-        select_pivot_wg(A, k, ne, ne, prm, rs, kind, pa, pb);
-        if (kind == 0) break;
-        apply_pivot_wg<NT>(A, kind, pa, pb, k, rows, tags, nullptr, 0, 0, f, prm, cnt, flops, 0);
+        select_pivot_wg(A, k, ne, ne, fp, rs, kind, pa, pb);
+        bool az = false;
+        if (kind == 0) {
+            if (!(sp > 0.0)) break;
+            int raised;
+            pa = force_static_wg(A, k, ne, sp, fp, rs, raised, nstatic, growth);
+            kind = 1;
+            az = static_as_zero(raised, fp, F);
+        }
+        apply_pivot_wg<NT>(A, kind, pa, pb, k, rows, tags, nullptr, 0, 0, f, fp, cnt, flops, 0, az);
     }
     __syncthreads();
     if (tid == 0) {
@@ -1823,6 +1940,8 @@ __global__ __launch_bounds__(NT) void k_dev_fronts(LdltDev d, DevLevel L, const 
         Fd.nneg = cnt[1];
         Fd.nzero = cnt[2];
         Fd.flops = flops;
+        Fd.nstatic = nstatic;
+        Fd.growth = growth;
     }
     __syncthreads();
     finish_front_wg<NT>(d, Fd, Fs, true);
@@ -1837,9 +1956,9 @@ __global__ __launch_bounds__(NT) void k_dev_fronts(LdltDev d, DevLevel L, const 
 }
 
 void launch_dev_fronts(hipStream_t st, const LdltDev &d, const DevLevel &L, const int32_t *flist, int32_t nf,
-                       int32_t lv, double u, double eff_small, double small_pivot) {
+                       int32_t lv, double u, double eff_small, double small_pivot, const StaticPrm &S) {
     if (nf <= 0) return;
-    Prm prm{u, eff_small, small_pivot};
+    Prm prm{u, eff_small, small_pivot, 0.0, S};
     // two size classes: one wavefront and a 48 x 48 square (many fronts per
     // CU), a workgroup of kLT and the largest square
     static_assert(kDevFrontMax == 95, "size classes");

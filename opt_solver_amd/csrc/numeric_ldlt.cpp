@@ -157,8 +157,8 @@ int ldlt_plan_build(smfgpu_plan_s *P, const smfgpu_analysis_desc *D) {
 // once. A level refused for room is retried with twice the room; one
 // refused for a front too large is left to the host layout (hold).
 // next: the first level not done.
-static int ldlt_dev_levels(smfgpu_plan_s *P, int32_t lv0, double u, double eff_small, double small, int32_t &next,
-                           int32_t &hold) {
+static int ldlt_dev_levels(smfgpu_plan_s *P, int32_t lv0, double u, double eff_small, double small,
+                           const StaticPrm &S, int32_t &next, int32_t &hold) {
     hipStream_t st = P->stream;
     const int32_t nlev = static_cast<int32_t>(P->level_fronts.size());
     const int32_t symmax = std::min(kDevSymMax, P->dev_fmax);
@@ -209,7 +209,7 @@ static int ldlt_dev_levels(smfgpu_plan_s *P, int32_t lv0, double u, double eff_s
             const int32_t *fl = P->d_flist.p + P->lv_flist_ofs[l];
             const int32_t nf = static_cast<int32_t>(P->lv_flist_ofs[l + 1] - P->lv_flist_ofs[l]);
             launch_dev_layout(st, d, L, fl, nf, l);
-            launch_dev_fronts(st, d, L, fl, nf, l, u, eff_small, small);
+            launch_dev_fronts(st, d, L, fl, nf, l, u, eff_small, small, S);
         }
         HIP_OK(hipGetLastError());
         HIP_OK(hipMemcpyAsync(h, P->d_dctr.p, kDcN * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
@@ -436,6 +436,11 @@ int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgp
     const double small = opt.small_pivot;
     const double eff_small = (max_abs > 0.0 && max_abs < small) ? small * max_abs : small;
     const int32_t nb = std::max(2, opt.panel);
+    // static pivoting values (factor_indef.cpp:642-645)
+    StaticPrm spr{};
+    spr.sp = opt.static_pivot_tol > 0.0 ? opt.static_pivot_tol * max_abs : 0.0;
+    spr.fallback = opt.fallback_static_pivot_tol > 0.0 ? opt.fallback_static_pivot_tol * max_abs : 0.0;
+    spr.flags = opt.static_flags;
     P->fac_used = P->rows_used = P->dinv_used = 0;
     P->sq_pool.reset();
     std::vector<int32_t> pos(n, -1);
@@ -472,7 +477,7 @@ int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgp
     for (int32_t lv = 0; lv < nlev; ++lv) {
         if (dev_on && lv != dev_hold && P->lv_fsym_max[lv] <= std::min(kDevSymMax, P->dev_fmax)) {
             int32_t next = lv, hold = -1;
-            if ((rc = ldlt_dev_levels(P, lv, opt.pivot_u, eff_small, small, next, hold))) return rc;
+            if ((rc = ldlt_dev_levels(P, lv, opt.pivot_u, eff_small, small, spr, next, hold))) return rc;
             levels_done = next;
             dev_hold = hold;
             if (next > lv) {
@@ -521,7 +526,10 @@ int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgp
             F.rofs = P->rows_used;
             F.dofs = P->dinv_used;
             F.wofs = 0;
-            if (F.f >= P->ldlt_big_min && F.ne > 0) {
+            // the large-front state machine has no static path: a static
+            // front takes the one-workgroup blocked kernel
+            const bool stat = front_static_value(spr, F.root, F.nd, F.psym) > 0.0;
+            if (F.f >= P->ldlt_big_min && F.ne > 0 && !stat) {
                 F.wofs = work_used;
                 work_used += static_cast<int64_t>(F.f) * kBigWork;
                 big_list.push_back(s);
@@ -642,9 +650,9 @@ int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgp
                 if (!C.dev) P->sq_pool.give(C.fofs, static_cast<int64_t>(C.f) * C.f);
             }
         launch_ldlt_small(st, d, reinterpret_cast<const int32_t *>(S + o_small), static_cast<int32_t>(small_list.size()),
-                          opt.pivot_u, eff_small, small);
+                          opt.pivot_u, eff_small, small, spr);
         launch_ldlt_blocked(st, d, reinterpret_cast<const int32_t *>(S + o_blk),
-                            static_cast<int32_t>(blocked_list.size()), opt.pivot_u, eff_small, small, nb);
+                            static_cast<int32_t>(blocked_list.size()), opt.pivot_u, eff_small, small, nb, spr);
         if (!big_list.empty() && (rc = ldlt_big_fronts(P, d, big_list, opt, eff_small, small, nb, big_h, reduced)))
             return rc;
         launch_ldlt_epilogue(st, d, reinterpret_cast<const int32_t *>(S + o_ep), static_cast<int32_t>(ep_list.size()),
@@ -686,6 +694,8 @@ int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgp
     fi.actual_flops = 0;
     fi.front_work = 0;
     fi.max_front = 0;
+    fi.static_pivots = 0;
+    fi.pivot_growth = 0.0;
     int64_t fent = 0;
     std::vector<int32_t> done_fronts;
     for (int32_t lv = 0; lv < levels_done; ++lv)
@@ -700,6 +710,8 @@ int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgp
         fi.actual_flops += F.flops;
         fi.front_work += static_cast<double>(F.ne) * F.f * static_cast<double>(F.f);
         fi.max_front = std::max(fi.max_front, F.f);
+        fi.static_pivots += F.nstatic;
+        fi.pivot_growth = std::max(fi.pivot_growth, F.growth);
         fent += static_cast<int64_t>(F.f) * F.pstore;
     }
     fi.factor_entries = fent;

@@ -23,6 +23,8 @@ struct FrontLd {
     int32_t npos, nneg, nzero, n2x2;
     int32_t delayed;
     int32_t dev;              // 1: laid out on the device (its square is not in the host's pool)
+    int32_t nstatic;          // pivots whose |d| static pivoting raised
+    double growth;            // largest column max / |d| over its static pivots
     double flops;
     int64_t fofs;             // arena offset of the square
     int64_t lofs;             // factor-store offset of the stored columns
@@ -82,6 +84,20 @@ constexpr int32_t kGroupCols = 64;   // columns of a speculative group (four pan
 constexpr int32_t kGroupRows = 256;  // rows per workgroup below a group's diagonal region
 constexpr int64_t kGroupBuf = 192 + 64 * 16 + 2 * 64 * 64;  // group buffer per front before the per-workgroup maxima
 
+// Static pivoting (control.hpp:333-364): sp = static_pivot_tol * max|A|
+// (0: off), fallback = fallback_static_pivot_tol * max|A| for roots and
+// fronts that receive more delayed columns than their width; flags as
+// SMFGPU_STATIC_*.
+struct StaticPrm {
+    double sp, fallback;
+    int32_t flags;
+};
+constexpr int32_t kStaticAsZero = 1, kStaticZeroStaysZero = 2, kStaticInOrder = 4;
+// whether front F pivots statically, and with which value
+__host__ __device__ inline double front_static_value(const StaticPrm &S, int32_t root, int32_t nd, int32_t psym) {
+    return (S.fallback > 0.0 && (root != 0 || nd > psym)) ? S.fallback : S.sp;
+}
+
 // Levels of small fronts laid out on the device: per level, a layout
 // kernel (each front's size from its children's delays, offsets by a bump
 // of the counters below) and one workgroup per front that builds the row
@@ -107,12 +123,12 @@ struct DevLevel {
 void launch_dev_layout(hipStream_t st, const LdltDev &d, const DevLevel &L, const int32_t *flist, int32_t nf,
                        int32_t lv);
 void launch_dev_fronts(hipStream_t st, const LdltDev &d, const DevLevel &L, const int32_t *flist, int32_t nf,
-                       int32_t lv, double u, double eff_small, double small_pivot);
+                       int32_t lv, double u, double eff_small, double small_pivot, const StaticPrm &S);
 
 void launch_ldlt_small(hipStream_t st, const LdltDev &d, const int32_t *flist, int32_t nf, double u, double eff_small,
-                       double small_pivot);
+                       double small_pivot, const StaticPrm &S);
 void launch_ldlt_blocked(hipStream_t st, const LdltDev &d, const int32_t *flist, int32_t nf, double u,
-                         double eff_small, double small_pivot, int32_t nb);
+                         double eff_small, double small_pivot, int32_t nb, const StaticPrm &S);
 void launch_ldlt_assemble(hipStream_t st, const LdltDev &d, const AsmTask *tasks, int64_t ntasks, const ChildCol *cc);
 // epilogue (pivot counts, D^-1) of the fronts in flist; it also stores the
 // factor columns of the first n_inline fronts, launch_ldlt_store those of

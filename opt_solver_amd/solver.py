@@ -136,7 +136,8 @@ class Plan:
         check(lib().smfgpu_plan_tune(self.h, key, int(value)), "plan_tune")
 
     def factor(self, values=None, values_ptr=None, precision=PREC_FP64, mixed_min_front=1024, u=0.01,
-               small_pivot=1e-20, panel=16, max_negative_pivots=-1, work_limit=None) -> "Factor":
+               small_pivot=1e-20, panel=16, max_negative_pivots=-1, work_limit=None, static_pivot_tol=0.0,
+               fallback_static_pivot_tol=0.0, static_flags=0) -> "Factor":
         opt = _lib.FactorOptions()
         lib().smfgpu_default_options(ctypes.byref(opt))
         opt.posdef = 1 if self.posdef else 0
@@ -146,6 +147,9 @@ class Plan:
         opt.precision = precision
         opt.mixed_min_front = mixed_min_front
         opt.max_negative_pivots = max_negative_pivots
+        opt.static_pivot_tol = static_pivot_tol
+        opt.fallback_static_pivot_tol = fallback_static_pivot_tol
+        opt.static_flags = static_flags
         if work_limit is not None:
             opt.work_limit_on = 1
             opt.work_limit = float(work_limit)

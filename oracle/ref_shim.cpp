@@ -156,13 +156,21 @@ void ref_analysis_free(void *handle) { delete static_cast<RefAnalysis *>(handle)
 // [0] status (FactorStatus) [1] ErrorCode [2] pos [3] neg [4] zero
 // [5] delayed pivots [6] 2x2 pivots [7] actual flops [8] front work
 // [9] seconds [10] static pivots [11] max front
-void *ref_factor(void *handle, const double *values, int64_t nvalues, double u,
-                 int num_threads, double *stats) {
+// static pivoting (control.hpp:337-364): tolerances and flags bit 0
+// static_pivots_as_zero, bit 1 static_zero_stays_zero, bit 2
+// static_pivot_in_order; stats[12] = Info::pivot_growth.
+void *ref_factor_ex(void *handle, const double *values, int64_t nvalues, double u, int num_threads,
+                    double static_tol, double fallback_tol, int static_flags, double *stats) {
     auto *an = static_cast<RefAnalysis *>(handle);
     auto *h = new RefFactor;
     h->an = an;
     h->ctrl = base_control(an->posdef, num_threads);
     h->ctrl.pivot_u = u;
+    h->ctrl.static_pivot_tol = static_tol;
+    h->ctrl.fallback_static_pivot_tol = fallback_tol;
+    h->ctrl.static_pivots_as_zero = (static_flags & 1) != 0;
+    h->ctrl.static_zero_stays_zero = (static_flags & 2) != 0;
+    h->ctrl.static_pivot_in_order = (static_flags & 4) != 0;
     h->ctrl.scaling = ScalingMethod::None;
     h->ctrl.scale_on_delay_ratio = -1.0;
     h->ctrl.matching_trigger = MatchingTrigger::Predicted;
@@ -184,8 +192,13 @@ void *ref_factor(void *handle, const double *values, int64_t nvalues, double u,
         stats[9] = t1 - t0;
         stats[10] = h->info.static_pivots;
         stats[11] = h->info.max_front_size;
+        stats[12] = h->info.pivot_growth;
     }
     return h;
+}
+
+void *ref_factor(void *handle, const double *values, int64_t nvalues, double u, int num_threads, double *stats) {
+    return ref_factor_ex(handle, values, nvalues, u, num_threads, 0.0, 0.0, 0, stats);
 }
 
 // out: [0] steps [1] factor extent (doubles) [2] row index extent

@@ -40,6 +40,8 @@ def lib():
         L.ref_analysis_free.argtypes = [_P]
         L.ref_factor.restype = _P
         L.ref_factor.argtypes = [_P, _P, _L, _D, _I, _P]
+        L.ref_factor_ex.restype = _P
+        L.ref_factor_ex.argtypes = [_P, _P, _L, _D, _I, _D, _D, _I, _P]
         L.ref_factor_sizes.argtypes = [_P, _P]
         L.ref_factor_export.argtypes = [_P, _P, _P, _P, _P, _P]
         L.ref_pivot_types_size.restype = _L
@@ -117,11 +119,14 @@ class RefAnalysis:
 class RefFactor:
     STATUS = {0: "Success", 1: "NotPositiveDefinite", 2: "Singular", 3: "MaxPivotDelays", 4: "InertiaExceeded"}
 
-    def __init__(self, an: RefAnalysis, values, u=0.01, threads=1):
+    def __init__(self, an: RefAnalysis, values, u=0.01, threads=1, static_pivot_tol=0.0,
+                 fallback_static_pivot_tol=0.0, static_flags=0):
         self.an = an
         vals = np.ascontiguousarray(values, np.float64)
         st = np.zeros(16)
-        self.h = lib().ref_factor(an.h, _ptr(vals), vals.size, u, threads, _ptr(st))
+        self.h = lib().ref_factor_ex(an.h, _ptr(vals), vals.size, u, threads, static_pivot_tol,
+                                     fallback_static_pivot_tol, static_flags, _ptr(st))
+        self.pivot_growth = float(st[12])
         self.status = int(st[0])
         self.error = int(st[1])
         self.inertia = (int(st[2]), int(st[3]), int(st[4]))
