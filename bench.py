@@ -116,6 +116,89 @@ def kkt_measure(steps, warmup, gsz=300, m=30000):
             "speedup_mixed_vs_fp64": round(out["fp64"]["ms_per_step"] / out["mixed"]["ms_per_step"], 3)}
 
 
+def batch_systems(nsys):
+    """The configuration-3 batch: nsys trajectory-optimisation KKTs with
+    horizons T ~ U{10..200}, seeded."""
+    from opt_solver_amd import batch
+    return batch.ocp_batch(nsys, 10, 200, seed=0)
+
+
+def batch_measure(steps, warmup, nsys=1024):
+    """Factor + solve of nsys independent OCP KKT systems as one forest
+    (every level's fronts of all systems in the same launches), values and
+    right-hand sides resident on the device; the library's analysis of each
+    system, built once."""
+    import torch
+    from opt_solver_amd import batch
+    from opt_solver_amd.solver import Plan, analyse
+    t0 = time.perf_counter()
+    mats, Ts = batch_systems(nsys)
+    U, offs = batch.union_arrays([analyse(A, nemin=16).arrays() for A in mats])
+    plan = Plan(U, posdef=False)
+    vals = np.concatenate([A[3] for A in mats])
+    X = np.concatenate([np.random.default_rng(i).uniform(-1, 1, A[0]) for i, A in enumerate(mats)])
+    starts = np.cumsum([0] + [A[0] for A in mats])
+    b = np.concatenate([_matvec(A, X[starts[i]:starts[i + 1]]) for i, A in enumerate(mats)])
+    dev = torch.device("cuda:0")
+    vals_d = torch.from_numpy(vals).to(dev)
+    b_d = torch.from_numpy(b).to(dev)
+    x_d = torch.empty_like(b_d)
+    torch.cuda.synchronize()
+    log(f"batch set-up {time.perf_counter() - t0:.1f}s n={U['n']} nsup={U['sn_start'].size}")
+
+    def step():
+        fac = plan.factor(values_ptr=vals_d.data_ptr(), u=0.01)
+        x_d.copy_(b_d)
+        torch.cuda.current_stream().synchronize()
+        fac.solve_device(x_d.data_ptr(), 1)
+        return fac
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(steps):
+        fac = step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / steps
+    per = batch.system_stats(fac.front_stats(), offs["sn"])
+    inertia_ok = all(tuple(per[i, :3]) == (16 * T + 12, 12 * T, 0) for i, T in enumerate(Ts))
+    err = float(np.abs(x_d.cpu().numpy() - X).max())
+    kt = plan.kernel_times()
+    out = {"workload": f"{nsys} OCP KKTs, T ~ U{{10..200}} (n={U['n']}), u=0.01, one forest",
+           "nsup": int(U["sn_start"].size), "levels": int(plan.stats()["levels"]),
+           "device_levels": int(kt["device_levels"]), "ms_per_step": round(dt * 1e3, 2),
+           "systems_per_s": round(nsys / dt, 1), "flops_per_factor": fac.info.actual_flops,
+           "delayed": int(fac.info.delayed_pivots), "inertia_ok": inertia_ok, "fwd_err": err}
+    log("batch", out)
+    return out
+
+
+def batch_cpu_baseline(sample, threads):
+    """The reference's Solver::factor + solve of the first `sample` systems
+    of the batch, one system per call, on `threads` host threads (each call
+    single-threaded; the calls run concurrently)."""
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle import refshim
+    if not refshim.available():
+        return None
+    mats, _ = batch_systems(sample)
+    ans = [refshim.RefAnalysis(A, ordering=refshim.ORDER_AMD, posdef=False) for A in mats]
+    rhs = [_matvec(A, np.ones(A[0])) for A in mats]
+
+    def one(i):
+        rf = refshim.RefFactor(ans[i], mats[i][3], u=0.01, threads=1)
+        rf.solve(rhs[i])
+        return rf.flops
+    t = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        fl = sum(ex.map(one, range(sample)))
+    dt = time.perf_counter() - t
+    return {"value": round(sample / dt, 1), "unit": "systems/s", "cores": threads, "kind": "reference",
+            "sample": f"smf 0.2.3 Solver::factor+solve of the first {sample} systems of the batch, one system "
+                      f"per call on {threads} threads, AMD (fallback minimum degree), {dt:.2f} s, "
+                      f"{fl:.3e} flops"}
+
+
 def _matvec(A, x):
     n, cp, ri, v = A
     cols = np.repeat(np.arange(n), np.diff(cp))
@@ -136,6 +219,7 @@ def main():
     ap.add_argument("--cpu-m", type=int, default=76)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-kkt", action="store_true", help="skip the indefinite KKT comparison")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batch of small systems")
     ap.add_argument("--copy-calib", action="store_true",
                     help="run one plain 1 GiB device copy first (scale for the HBM byte counters)")
     args = ap.parse_args()
@@ -231,10 +315,15 @@ def main():
     kkt = None
     if not args.no_kkt:
         kkt = kkt_measure(args.steps, args.warmup)
+    bat = None
+    if not args.no_batch:
+        bat = batch_measure(args.steps, args.warmup)
     cpu = None
     if not args.no_cpu:
         try:
             cpu = cpu_baseline(args.cpu_m, min(os.cpu_count() or 1, 16))
+            if bat is not None:
+                cpu["batch"] = batch_cpu_baseline(1024, min(os.cpu_count() or 1, 16))
         except Exception as e:  # the baseline is informative only
             log("cpu baseline failed:", e)
     out = {
@@ -266,6 +355,7 @@ def main():
                      "traffic": traffic},
         "cpu_baseline": cpu,
         "kkt": kkt,
+        "batch": bat,
     }
     print(json.dumps(out))
 
