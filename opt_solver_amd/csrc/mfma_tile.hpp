@@ -44,6 +44,21 @@ struct SyrkMfma<float> {
     static __device__ __forceinline__ int row(int v, int lane) { return 4 * (lane >> 4) + v; }
 };
 
+// Word 3 of a raw buffer resource on gfx9-family targets (gfx950): the
+// data format field that makes a raw buffer read dwords.
+constexpr int kBufferWord3 = 0x00020000;
+
+template <typename GE>
+__device__ __forceinline__ GE buffer_load(__amdgpu_buffer_rsrc_t r, uint32_t off);
+template <>
+__device__ __forceinline__ float buffer_load<float>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, static_cast<int>(off), 0, 0));
+}
+template <>
+__device__ __forceinline__ double buffer_load<double>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, static_cast<int>(off), 0, 0));
+}
+
 // LDS row length of a staged chunk: FP32 rows are padded by 16 floats so
 // that the four panel columns a wave reads at once (lanes 16 k .. 16 k + 15
 // take column k) fall on different banks; FP64 reads do not conflict.
@@ -56,8 +71,8 @@ __host__ __device__ constexpr int tile_lds_bytes() { return 4 * kKc * tile_lds_l
 // One 128 x 128 tile, operands of type E staged in a double-buffered LDS
 // ring. kAhead = 1: the global loads of chunk k+1 are in flight during the
 // MFMAs of chunk k; kAhead = 2: those of chunk k+2 (two register sets, the
-// loads get two MFMA phases to arrive). Loads are branch-free (indices
-// clamped into the front, out-of-range values selected to zero); waves
+// loads get two MFMA phases to arrive). Loads are branch-free buffer loads
+// (out-of-range rows and columns read 0); waves
 // wholly above the diagonal run the same loop without MFMAs (ACTIVE =
 // false).
 // A operand element (row i, panel column t) = Ab[t * ld + i], B operand
@@ -80,34 +95,37 @@ __device__ __forceinline__ void tile_loop(const GE *Ab, const GE *Bb, int ld, in
     const int w = tid >> 6;
     const int wr = w >> 1, wc = w & 1;
     typedef GE Regs[kLoadPer];
-    // Unconditional loads from clamped addresses (finite front data); the
-    // 0 / 1 masks for rows past the front, columns past j1 and panel
-    // columns past t1 are applied when the chunk is written to LDS, after
-    // the MFMAs of the previous chunk, so nothing waits for the loads early.
-    // wave-uniform chunk base + 32-bit per-lane offsets (scalar-base
-    // global loads: one offset VGPR per load instead of a 64-bit address)
-    const int ia = min(i0 + (tid % kT), f - 1);
-    const int jb = min(j0 + (tid % kT), j1 - 1);
+    // Buffer loads: per chunk one descriptor over its panel columns
+    // [kc, min(kc + kKc, t1)) and per operand one lane offset. Columns past
+    // t1, and rows past the front (A) or past j1 (B), whose offsets are
+    // pushed past the descriptor's end, read 0 without a branch, a clamp or
+    // a mask (the range check covers the lane offset).
+    constexpr int kPass = kThreads / kT;  // panel columns of one load instruction
+    constexpr uint32_t kOut = 0x80000000u;
+    const int rl = tid % kT, th = tid / kT;
+    const uint32_t colb = static_cast<uint32_t>(ld) * static_cast<uint32_t>(sizeof(GE));
+    const uint32_t voa = (i0 + rl < f) ? th * colb + static_cast<uint32_t>(i0 + rl) * sizeof(GE) : kOut;
+    const uint32_t vob = (j0 + rl < j1) ? th * colb + static_cast<uint32_t>(j0 + rl) * sizeof(GE) : kOut;
     auto gload = [&](Regs &ra, Regs &rb, int kc) {
-        const GE *ca = Ab + static_cast<int64_t>(kc) * ld;
-        const GE *cb = Bb + static_cast<int64_t>(kc - boff) * ld;
-        const int tlim = t1 - 1 - kc;
+        const int nrec = min(kKc, t1 - kc) * static_cast<int>(colb);
+        const __amdgpu_buffer_rsrc_t sa = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<GE *>(Ab + static_cast<int64_t>(kc) * ld), 0, nrec, kBufferWord3);
+        const __amdgpu_buffer_rsrc_t sb = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<GE *>(Bb + static_cast<int64_t>(kc - boff) * ld), 0, nrec, kBufferWord3);
 #pragma unroll
         for (int e = 0; e < kLoadPer; ++e) {
-            const int t = min((tid + kThreads * e) / kT, tlim);
-            const uint32_t col = static_cast<uint32_t>(t) * static_cast<uint32_t>(ld);
-            ra[e] = ca[col + static_cast<uint32_t>(ia)];
-            rb[e] = cb[col + static_cast<uint32_t>(jb)];
+            const uint32_t co = static_cast<uint32_t>(e * kPass) * colb;
+            ra[e] = buffer_load<GE>(sa, voa + co);
+            rb[e] = buffer_load<GE>(sb, vob + co);
         }
     };
     auto sstore = [&](const Regs &ra, const Regs &rb, int stg, int kc) {
+        (void)kc;
 #pragma unroll
         for (int e = 0; e < kLoadPer; ++e) {
-            const int idx = tid + kThreads * e;
-            const int t = idx / kT, r = idx % kT;
-            const bool tin = kc + t < t1;
-            As[stg][t][r] = static_cast<E>(ra[e]) * static_cast<E>((tin && i0 + r < f) ? 1 : 0);
-            Bs[stg][t][r] = static_cast<E>(rb[e]) * static_cast<E>((tin && j0 + r < j1) ? 1 : 0);
+            const int t = kPass * e + th;
+            As[stg][t][rl] = static_cast<E>(ra[e]);
+            Bs[stg][t][rl] = static_cast<E>(rb[e]);
         }
     };
     auto mfma_stage = [&](int stg) {
