@@ -264,6 +264,11 @@ SMFGPU_API int smfgpu_refine(smfgpu_factor_t f, const double *values, const doub
  * front that delays grow past value rows goes to the host layout. Default
  * 95 (the largest unblocked front), 0 turns it off; at most 95. */
 #define SMFGPU_TUNE_LDLT_DEVICE_FRONTS 8
+/* SMFGPU_TUNE_ASM_CHUNK: SPD assembly's LDS column chunk is the level's
+ * longest column up to this many rows (default 16384, in [256, 16384]):
+ * longer chunks pass each child column once, shorter ones let more
+ * workgroups share a CU. */
+#define SMFGPU_TUNE_ASM_CHUNK 9
 SMFGPU_API int smfgpu_plan_tune(smfgpu_plan_t plan, int32_t key, int64_t value);
 
 /* Device time (ms) of the last factorization's dominant kernel family:

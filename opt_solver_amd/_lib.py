@@ -76,6 +76,7 @@ TUNE_LDLT_BIG_MIN = 5
 TUNE_LDLT_GROUP = 6
 TUNE_LDLT_UPDATE_WG = 7
 TUNE_LDLT_DEVICE_FRONTS = 8
+TUNE_ASM_CHUNK = 9
 
 _lib = None
 

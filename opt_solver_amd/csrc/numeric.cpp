@@ -644,6 +644,11 @@ SMFGPU_API int smfgpu_plan_tune(smfgpu_plan_t P, int32_t key, int64_t value) {
         if (value < 0 || value > kDevFrontMax) return set_error(SMFGPU_ERR_INVALID_ARG, "tune: device fronts in [0, 95]");
         P->dev_fmax = static_cast<int32_t>(value);
         return SMFGPU_OK;
+    case SMFGPU_TUNE_ASM_CHUNK:
+        if (value < 256 || value > 16384) return set_error(SMFGPU_ERR_INVALID_ARG, "tune: chunk in [256, 16384]");
+        P->asm_chunk = static_cast<int32_t>(value);
+        release_graph(P);
+        return SMFGPU_OK;
     case SMFGPU_TUNE_SYRK_AHEAD:
         if (value != 1 && value != 2) return set_error(SMFGPU_ERR_INVALID_ARG, "tune: chunks ahead must be 1 or 2");
         P->syrk_ahead = static_cast<int32_t>(value);
@@ -769,7 +774,7 @@ SMFGPU_API int smfgpu_factor(smfgpu_plan_t P, const double *values, int64_t nval
             case K_ASSEMBLE: {
                 int32_t fmax = 1;
                 for (int32_t s : P->level_fronts[l.level]) fmax = std::max(fmax, P->fronts[s].f);
-                launch_assemble(st, d, P->d_asm.p + l.first, l.count, P->d_cc.p, fmax);
+                launch_assemble(st, d, P->d_asm.p + l.first, l.count, P->d_cc.p, std::min(fmax, P->asm_chunk));
                 break;
             }
             case K_POTRF: launch_potrf(st, d, P->d_potrf.p + l.first, l.count); break;

@@ -215,6 +215,7 @@ struct smfgpu_plan_s {
     int32_t ldlt_update_wg = 1024;                 // LDL^T: workgroups of the large-front update launches
     DBuf<double> d_gpart;                          // LDL^T: per-workgroup column maxima of a group
     int32_t syrk_ahead = 2;                        // Schur update: chunks with global loads in flight (1 or 2)
+    int32_t asm_chunk = 16384;                     // SPD assembly: largest LDS column chunk (rows)
     int32_t solve_fuse_rows = 512;                 // rows below a last block its own workgroup updates
     double t_solve_phase[4] = {0, 0, 0, 0};        // fwd ms, bwd ms, fwd launches, bwd launches
     // refinement CSR (full symmetric, permuted numbering)

@@ -70,7 +70,10 @@ class Analysis:
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().smfgpu_analysis_free(ctypes.byref(self.h))
+            try:
+                lib().smfgpu_analysis_free(ctypes.byref(self.h))
+            except Exception:  # interpreter shutdown: the library is going away anyway
+                pass
             self.h = None
 
 
@@ -171,7 +174,10 @@ class Plan:
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().smfgpu_plan_free(ctypes.byref(self.h))
+            try:
+                lib().smfgpu_plan_free(ctypes.byref(self.h))
+            except Exception:  # interpreter shutdown: the library is going away anyway
+                pass
             self.h = None
 
 
@@ -233,5 +239,8 @@ class Factor:
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().smfgpu_factor_free(ctypes.byref(self.h))
+            try:
+                lib().smfgpu_factor_free(ctypes.byref(self.h))
+            except Exception:  # interpreter shutdown: the library is going away anyway
+                pass
             self.h = None
