@@ -269,6 +269,10 @@ SMFGPU_API int smfgpu_refine(smfgpu_factor_t f, const double *values, const doub
  * longer chunks pass each child column once, shorter ones let more
  * workgroups share a CU. */
 #define SMFGPU_TUNE_ASM_CHUNK 9
+/* SMFGPU_TUNE_LDLT_DEVICE_WAVE: 1 (default) factors the device levels'
+ * fronts of at most 48 rows with one wavefront each (many fronts per CU),
+ * 0 gives every front of a device level a 256-thread workgroup. */
+#define SMFGPU_TUNE_LDLT_DEVICE_WAVE 10
 SMFGPU_API int smfgpu_plan_tune(smfgpu_plan_t plan, int32_t key, int64_t value);
 
 /* Device time (ms) of the last factorization's dominant kernel family:

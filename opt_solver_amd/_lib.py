@@ -77,6 +77,7 @@ TUNE_LDLT_GROUP = 6
 TUNE_LDLT_UPDATE_WG = 7
 TUNE_LDLT_DEVICE_FRONTS = 8
 TUNE_ASM_CHUNK = 9
+TUNE_LDLT_DEVICE_WAVE = 10
 
 _lib = None
 

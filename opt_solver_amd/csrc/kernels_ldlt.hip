@@ -1956,9 +1956,17 @@ __global__ __launch_bounds__(NT) void k_dev_fronts(LdltDev d, DevLevel L, const 
 }
 
 void launch_dev_fronts(hipStream_t st, const LdltDev &d, const DevLevel &L, const int32_t *flist, int32_t nf,
-                       int32_t lv, double u, double eff_small, double small_pivot, const StaticPrm &S) {
+                       int32_t lv, double u, double eff_small, double small_pivot, const StaticPrm &S, bool wave) {
     if (nf <= 0) return;
     Prm prm{u, eff_small, small_pivot, 0.0, S};
+    if (!wave) {
+        // every front on kLT threads (shorter per-front latency where a
+        // level has few fronts)
+        for (int64_t o = 0; o < nf; o += kMaxLaunchGroups)
+            k_dev_fronts<kLT, 0, 95><<<dim3(static_cast<unsigned>(std::min<int64_t>(kMaxLaunchGroups, nf - o))), kLT, 0,
+                                        st>>>(d, L, flist + o, lv, prm);
+        return;
+    }
     // two size classes: one wavefront and a 48 x 48 square (many fronts per
     // CU), a workgroup of kLT and the largest square
     static_assert(kDevFrontMax == 95, "size classes");

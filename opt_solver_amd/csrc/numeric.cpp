@@ -644,6 +644,10 @@ SMFGPU_API int smfgpu_plan_tune(smfgpu_plan_t P, int32_t key, int64_t value) {
         if (value < 0 || value > kDevFrontMax) return set_error(SMFGPU_ERR_INVALID_ARG, "tune: device fronts in [0, 95]");
         P->dev_fmax = static_cast<int32_t>(value);
         return SMFGPU_OK;
+    case SMFGPU_TUNE_LDLT_DEVICE_WAVE:
+        if (value != 0 && value != 1) return set_error(SMFGPU_ERR_INVALID_ARG, "tune: 0 or 1");
+        P->dev_wave = static_cast<int32_t>(value);
+        return SMFGPU_OK;
     case SMFGPU_TUNE_ASM_CHUNK:
         if (value < 256 || value > 16384) return set_error(SMFGPU_ERR_INVALID_ARG, "tune: chunk in [256, 16384]");
         P->asm_chunk = static_cast<int32_t>(value);

@@ -209,7 +209,7 @@ static int ldlt_dev_levels(smfgpu_plan_s *P, int32_t lv0, double u, double eff_s
             const int32_t *fl = P->d_flist.p + P->lv_flist_ofs[l];
             const int32_t nf = static_cast<int32_t>(P->lv_flist_ofs[l + 1] - P->lv_flist_ofs[l]);
             launch_dev_layout(st, d, L, fl, nf, l);
-            launch_dev_fronts(st, d, L, fl, nf, l, u, eff_small, small, S);
+            launch_dev_fronts(st, d, L, fl, nf, l, u, eff_small, small, S, P->dev_wave != 0);
         }
         HIP_OK(hipGetLastError());
         HIP_OK(hipMemcpyAsync(h, P->d_dctr.p, kDcN * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
@@ -337,81 +337,114 @@ static int ldlt_big_fronts(smfgpu_plan_s *P, const LdltDev &d, const std::vector
 // the rows past the pivots (512-row segments, atomic), then solve the
 // blocks last to first, chunk items applying each solved block to the
 // columns before it. One launch per step covers every front of a level.
-static int ldlt_solve_lists(smfgpu_plan_s *P) {
+// The items of one front's forward (bwd = false) or backward sweep, step
+// by step: emit(step, item).
+template <class Emit>
+static void front_solve_items(int32_t f, int32_t p, int32_t s, bool bwd, Emit &&emit) {
     constexpr int32_t B = 64, Seg = 512, Tri = 2;
+    if (p <= 0) return;
+    const int32_t nb = (p + B - 1) / B;
+    if (!bwd) {
+        for (int32_t t = 0; t < nb; ++t) {
+            const int32_t tk = t * B, tkb = std::min(B, p - tk);
+            const int32_t k = t > 0 ? tk - B : 0, kb = t > 0 ? B : 0;
+            emit(t, SolveItem{s, k, kb, 0, 0, tk, tkb, Tri});
+            if (t > 0)
+                for (int32_t r = tk + tkb; r < f; r += B) emit(t, SolveItem{s, k, kb, r, std::min(r + B, f), 0, 0, 0});
+        }
+        if (f > p) {
+            const int32_t k = (nb - 1) * B;
+            for (int32_t r = p; r < f; r += B) emit(nb, SolveItem{s, k, p - k, r, std::min(r + B, f), 0, 0, 0});
+        }
+        return;
+    }
+    int32_t t = 0;
+    if (f > p) {
+        for (int32_t r = p; r < f; r += Seg)
+            for (int32_t c = 0; c < p; c += B) emit(0, SolveItem{s, r, std::min(Seg, f - r), c, std::min(c + B, p), 0, 0, 1});
+        t = 1;
+    }
+    const int32_t kL = (nb - 1) * B;
+    emit(t++, SolveItem{s, 0, 0, 0, 0, kL, p - kL, Tri});
+    for (int32_t j = nb - 2; j >= 0; --j, ++t) {
+        const int32_t tk = j * B, k = tk + B, kb = std::min(B, p - k);
+        emit(t, SolveItem{s, k, kb, 0, 0, tk, B, Tri});
+        for (int32_t c = 0; c < tk; c += B) emit(t, SolveItem{s, k, kb, c, c + B, 0, 0, 0});
+    }
+}
+
+// Solve schedule of the factorization: per level, the items of each step
+// (items of all the level's fronts), counted first and then written in
+// place.
+static int ldlt_solve_lists(smfgpu_plan_s *P) {
     const int32_t nlev = static_cast<int32_t>(P->level_fronts.size());
-    std::vector<SolveItem> fwd_items, bwd_items;
     P->fwd_steps.clear();
     P->bwd_steps.clear();
     P->fwd_lv.assign(nlev + 1, 0);
     P->bwd_lv.assign(nlev + 1, 0);
-    std::vector<std::vector<SolveItem>> fst, bst;
-    auto put = [](std::vector<std::vector<SolveItem>> &v, size_t t, const SolveItem &it) {
-        if (v.size() <= t) v.resize(t + 1);
-        v[t].push_back(it);
-    };
-    auto flush = [](std::vector<std::vector<SolveItem>> &stp, std::vector<SolveItem> &items,
-                    std::vector<std::pair<int64_t, int64_t>> &steps) {
-        for (auto &v : stp) {
-            if (v.empty()) continue;
-            steps.emplace_back(static_cast<int64_t>(items.size()), static_cast<int64_t>(v.size()));
-            items.insert(items.end(), v.begin(), v.end());
-        }
-        stp.clear();
-    };
-    for (int32_t lv = 0; lv < nlev; ++lv) {
-        P->fwd_lv[lv] = static_cast<int32_t>(P->fwd_steps.size());
-        P->bwd_lv[lv] = static_cast<int32_t>(P->bwd_steps.size());
-        for (int32_t s : P->level_fronts[lv]) {
-            const FrontLd &F = P->fl[s];
-            const int32_t f = F.f, p = F.pstore;
-            if (p <= 0) continue;
-            const int32_t nb = (p + B - 1) / B;
-            for (int32_t t = 0; t < nb; ++t) {
-                const int32_t tk = t * B, tkb = std::min(B, p - tk);
-                const int32_t k = t > 0 ? tk - B : 0, kb = t > 0 ? B : 0;
-                put(fst, t, SolveItem{s, k, kb, 0, 0, tk, tkb, Tri});
-                if (t > 0)
-                    for (int32_t r = tk + tkb; r < f; r += B) put(fst, t, SolveItem{s, k, kb, r, std::min(r + B, f), 0, 0, 0});
-            }
-            if (f > p) {
-                const int32_t k = (nb - 1) * B;
-                for (int32_t r = p; r < f; r += B) put(fst, nb, SolveItem{s, k, p - k, r, std::min(r + B, f), 0, 0, 0});
-            }
-            size_t t = 0;
-            if (f > p) {
-                for (int32_t r = p; r < f; r += Seg)
-                    for (int32_t c = 0; c < p; c += B)
-                        put(bst, 0, SolveItem{s, r, std::min(Seg, f - r), c, std::min(c + B, p), 0, 0, 1});
-                t = 1;
-            }
-            const int32_t kL = (nb - 1) * B;
-            put(bst, t++, SolveItem{s, 0, 0, 0, 0, kL, p - kL, Tri});
-            for (int32_t j = nb - 2; j >= 0; --j, ++t) {
-                const int32_t tk = j * B, k = tk + B, kb = std::min(B, p - k);
-                put(bst, t, SolveItem{s, k, kb, 0, 0, tk, B, Tri});
-                for (int32_t c = 0; c < tk; c += B) put(bst, t, SolveItem{s, k, kb, c, c + B, 0, 0, 0});
-            }
-        }
-        flush(fst, fwd_items, P->fwd_steps);
-        flush(bst, bwd_items, P->bwd_steps);
+    std::vector<int64_t> cnt, pos;
+    // (rows, stored pivots) of every front: one sequential pass over the
+    // front array; the passes below read these compact arrays
+    const size_t ns = P->fronts.size();
+    std::vector<int32_t> lf(ns), lp(ns);
+    for (size_t s = 0; s < ns; ++s) {
+        lf[s] = P->fl[s].f;
+        lp[s] = P->fl[s].pstore;
     }
-    P->fwd_lv[nlev] = static_cast<int32_t>(P->fwd_steps.size());
-    P->bwd_lv[nlev] = static_cast<int32_t>(P->bwd_steps.size());
-    for (const auto &st : P->fwd_steps)
-        if (st.second > INT32_MAX) return set_error(SMFGPU_ERR_ALLOC, "factor: solve step too large");
-    for (const auto &st : P->bwd_steps)
-        if (st.second > INT32_MAX) return set_error(SMFGPU_ERR_ALLOC, "factor: solve step too large");
+    // counts per level and step give the steps' offsets; the items are then
+    // written in place into pinned staging and uploaded once
+    auto build = [&](bool bwd, PinnedArr<SolveItem> &items, DBuf<SolveItem> &dev,
+                     std::vector<std::pair<int64_t, int64_t>> &steps, std::vector<int32_t> &lvs) -> int {
+        std::vector<int64_t> lv_step0(nlev + 1, 0);
+        int64_t total = 0;
+        for (int32_t lv = 0; lv < nlev; ++lv) {
+            lvs[lv] = static_cast<int32_t>(steps.size());
+            lv_step0[lv] = static_cast<int64_t>(steps.size());
+            cnt.clear();
+            for (int32_t s : P->level_fronts[lv])
+                front_solve_items(lf[s], lp[s], s, bwd, [&](int32_t t, const SolveItem &) {
+                    if (cnt.size() <= static_cast<size_t>(t)) cnt.resize(t + 1, 0);
+                    ++cnt[t];
+                });
+            for (size_t t = 0; t < cnt.size(); ++t)
+                if (cnt[t] > 0) {
+                    steps.emplace_back(total, cnt[t]);
+                    total += cnt[t];
+                }
+        }
+        lvs[nlev] = static_cast<int32_t>(steps.size());
+        lv_step0[nlev] = static_cast<int64_t>(steps.size());
+        int rc;
+        if ((rc = items.reserve(static_cast<size_t>(std::max<int64_t>(total, 1))))) return rc;
+        for (int32_t lv = 0; lv < nlev; ++lv) {
+            // the level's non-empty steps in order of step index
+            int64_t k = lv_step0[lv];
+            cnt.clear();
+            for (int32_t s : P->level_fronts[lv])
+                front_solve_items(lf[s], lp[s], s, bwd, [&](int32_t t, const SolveItem &) {
+                    if (cnt.size() <= static_cast<size_t>(t)) cnt.resize(t + 1, 0);
+                    ++cnt[t];
+                });
+            pos.assign(cnt.size(), 0);
+            for (size_t t = 0; t < cnt.size(); ++t)
+                if (cnt[t] > 0) pos[t] = steps[k++].first;
+            for (int32_t s : P->level_fronts[lv])
+                front_solve_items(lf[s], lp[s], s, bwd, [&](int32_t t, const SolveItem &it) { items[pos[t]++] = it; });
+        }
+        for (const auto &st : steps)
+            if (st.second > INT32_MAX) return set_error(SMFGPU_ERR_ALLOC, "factor: solve step too large");
+        if ((rc = ensure(dev, static_cast<size_t>(std::max<int64_t>(total, 1)), false,
+                         bwd ? "backward solve items" : "forward solve items")))
+            return rc;
+        if (total > 0)
+            HIP_OK(hipMemcpyAsync(dev.p, items.data(), static_cast<size_t>(total) * sizeof(SolveItem),
+                                  hipMemcpyHostToDevice, P->stream));
+        return SMFGPU_OK;
+    };
     int rc;
-    if ((rc = ensure(P->d_fwd_items, std::max<size_t>(fwd_items.size(), 1), false, "forward solve items")) ||
-        (rc = ensure(P->d_bwd_items, std::max<size_t>(bwd_items.size(), 1), false, "backward solve items")))
+    if ((rc = build(false, P->h_fwd_items, P->d_fwd_items, P->fwd_steps, P->fwd_lv)) ||
+        (rc = build(true, P->h_bwd_items, P->d_bwd_items, P->bwd_steps, P->bwd_lv)))
         return rc;
-    if (!fwd_items.empty())
-        HIP_OK(hipMemcpyAsync(P->d_fwd_items.p, fwd_items.data(), fwd_items.size() * sizeof(SolveItem),
-                              hipMemcpyHostToDevice, P->stream));
-    if (!bwd_items.empty())
-        HIP_OK(hipMemcpyAsync(P->d_bwd_items.p, bwd_items.data(), bwd_items.size() * sizeof(SolveItem),
-                              hipMemcpyHostToDevice, P->stream));
     HIP_OK(hipStreamSynchronize(P->stream));
     return SMFGPU_OK;
 }
@@ -698,8 +731,14 @@ int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgp
     fi.pivot_growth = 0.0;
     int64_t fent = 0;
     std::vector<int32_t> done_fronts;
-    for (int32_t lv = 0; lv < levels_done; ++lv)
-        done_fronts.insert(done_fronts.end(), P->level_fronts[lv].begin(), P->level_fronts[lv].end());
+    if (levels_done == nlev) {
+        // every front: in supernode order (sequential over the front array)
+        done_fronts.resize(P->fronts.size());
+        for (size_t s = 0; s < done_fronts.size(); ++s) done_fronts[s] = static_cast<int32_t>(s);
+    } else {
+        for (int32_t lv = 0; lv < levels_done; ++lv)
+            done_fronts.insert(done_fronts.end(), P->level_fronts[lv].begin(), P->level_fronts[lv].end());
+    }
     for (int32_t s : done_fronts) {
         const FrontLd &F = P->fl[s];
         fi.num_positive += F.npos;

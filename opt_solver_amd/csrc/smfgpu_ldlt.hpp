@@ -123,7 +123,7 @@ struct DevLevel {
 void launch_dev_layout(hipStream_t st, const LdltDev &d, const DevLevel &L, const int32_t *flist, int32_t nf,
                        int32_t lv);
 void launch_dev_fronts(hipStream_t st, const LdltDev &d, const DevLevel &L, const int32_t *flist, int32_t nf,
-                       int32_t lv, double u, double eff_small, double small_pivot, const StaticPrm &S);
+                       int32_t lv, double u, double eff_small, double small_pivot, const StaticPrm &S, bool wave);
 
 void launch_ldlt_small(hipStream_t st, const LdltDev &d, const int32_t *flist, int32_t nf, double u, double eff_small,
                        double small_pivot, const StaticPrm &S);

@@ -78,6 +78,20 @@ struct PinnedArr {
         for (size_t i = 0; i < count; ++i) p[i] = v;
         return SMFGPU_OK;
     }
+    // room for count elements; the contents are not kept
+    int reserve(size_t count) {
+        if (count <= n) return SMFGPU_OK;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+        if (hipHostMalloc(reinterpret_cast<void **>(&p), count * sizeof(T), hipHostMallocDefault) != hipSuccess) {
+            p = nullptr;
+            return set_error(SMFGPU_ERR_ALLOC, "pinned host allocation of " + std::to_string(count * sizeof(T)) +
+                                                   " bytes failed");
+        }
+        n = count;
+        return SMFGPU_OK;
+    }
     T &operator[](size_t i) { return p[i]; }
     const T &operator[](size_t i) const { return p[i]; }
     T *data() { return p; }
@@ -204,6 +218,7 @@ struct smfgpu_plan_s {
     // forward sweep is fwd_steps[i] = {first item, count}, level lv owns
     // steps [fwd_lv[lv], fwd_lv[lv + 1]) (backward likewise)
     DBuf<SolveItem> d_fwd_items, d_bwd_items;
+    PinnedArr<SolveItem> h_fwd_items, h_bwd_items;  // LDL^T: staging of the solve items
     std::vector<std::pair<int64_t, int64_t>> fwd_steps, bwd_steps;
     std::vector<int32_t> fwd_lv, bwd_lv;
     int32_t panel_outer = kPanelOuter;             // SPD: pivot columns per trailing Schur update
@@ -252,6 +267,7 @@ struct smfgpu_plan_s {
     std::vector<int64_t> lv_room;                   // prefix over levels of the device-level room estimates (4 per level)
     int32_t dev_fmax = kDevFrontMax;                // largest realized front of a device level (0: host layout only)
     int32_t dev_levels = 0;                         // levels the last factorization ran on the device
+    int32_t dev_wave = 1;                           // device levels: one wavefront per front of <= 48 rows
     double t_ldlt_host[4] = {0, 0, 0, 0};           // s: device levels (run), their read-back, statistics, solve lists
     DBuf<int32_t> d_child_ptr, d_child_idx, d_parent;
     DBuf<unsigned long long> d_dctr;                // device-level counters

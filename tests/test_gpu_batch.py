@@ -4,8 +4,9 @@ per-system inertia, delayed pivots and 2x2 pivots against the reference's
 one-system-at-a-time factor, the whole factor bit for bit against the
 oracle, solves against the known solutions. The levels of small fronts run
 laid out on the device (default), partly (device fronts capped at 28 rows,
-so levels whose delays grow a front past that go to the host layout) and
-not at all: the same factor each way."""
+so levels whose delays grow a front past that go to the host layout), not
+at all, and on the device with 256 threads for every front instead of one
+wavefront for the small ones: the same factor each way."""
 import numpy as np
 import pytest
 
@@ -18,15 +19,15 @@ if not gpu_available():
     pytest.skip("no GPU", allow_module_level=True)
 
 from opt_solver_amd import batch  # noqa: E402
-from opt_solver_amd._lib import TUNE_LDLT_DEVICE_FRONTS  # noqa: E402
+from opt_solver_amd._lib import TUNE_LDLT_DEVICE_FRONTS, TUNE_LDLT_DEVICE_WAVE  # noqa: E402
 from opt_solver_amd.solver import Plan, analyse  # noqa: E402
 
 
-@pytest.mark.parametrize("dev_fronts", [95, 28, 0])
-def test_ocp_batch_per_system_parity(oracle_lib, dev_fronts):
+@pytest.mark.parametrize("dev_fronts,wave", [(95, 1), (28, 1), (0, 1), (95, 0)])
+def test_ocp_batch_per_system_parity(oracle_lib, dev_fronts, wave):
     from oracle import refshim
     nsys = 32
-    print(f"case ocp batch {nsys} device fronts {dev_fronts}", flush=True)
+    print(f"case ocp batch {nsys} device fronts {dev_fronts} one-wave class {wave}", flush=True)
     mats, Ts = batch.ocp_batch(nsys, 10, 60, seed=21)
     ref = refshim.available()
     if ref:
@@ -38,6 +39,7 @@ def test_ocp_batch_per_system_parity(oracle_lib, dev_fronts):
     vals = np.concatenate([A[3] for A in mats])
     plan = Plan(U, posdef=False)
     plan.tune(TUNE_LDLT_DEVICE_FRONTS, dev_fronts)
+    plan.tune(TUNE_LDLT_DEVICE_WAVE, wave)
     fac = plan.factor(vals, u=0.01)
     nlev = plan.stats()["levels"]
     dev = plan.kernel_times()["device_levels"]
