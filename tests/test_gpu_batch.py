@@ -69,3 +69,31 @@ def test_ocp_batch_per_system_parity(oracle_lib, dev_fronts, wave):
     xs = fac.solve(b)
     assert np.abs(xs - np.concatenate(X)).max() < 1e-8
     print("  delays", int(per[:, 4].sum()), "2x2", int(per[:, 3].sum()), flush=True)
+
+
+def test_full_batch_host_layout_large_launches():
+    """The 1024-system batch on the reference's analysis with the host layout
+    (device levels off): its first level's assembly has more than 2^22
+    columns, so the launch goes out in pieces (a dispatch counts its
+    work-items in 32 bits). Every system's inertia and the solve are checked."""
+    from oracle import refshim
+    if not refshim.available():
+        pytest.skip("reference library not built")
+    nsys = 1024
+    print(f"case ocp batch {nsys}, host layout", flush=True)
+    mats, Ts = batch.ocp_batch(nsys, 10, 200, seed=0)
+    arrs = [refshim.RefAnalysis(A, ordering=refshim.ORDER_AMD, posdef=False).plan_arrays() for A in mats]
+    U, offs = batch.union_arrays(arrs)
+    vals = np.concatenate([A[3] for A in mats])
+    plan = Plan(U, posdef=False)
+    plan.tune(TUNE_LDLT_DEVICE_FRONTS, 0)
+    fac = plan.factor(vals, u=0.01)
+    per = batch.system_stats(fac.front_stats(), offs["sn"])
+    for i, T in enumerate(Ts):
+        assert tuple(per[i, :3]) == (16 * T + 12, 12 * T, 0), i
+    X = np.concatenate([np.random.default_rng(i).uniform(-1, 1, A[0]) for i, A in enumerate(mats)])
+    starts = np.cumsum([0] + [A[0] for A in mats])
+    b = np.concatenate([matvec(A, X[starts[i]:starts[i + 1]]) for i, A in enumerate(mats)])
+    xs = fac.solve(b)
+    print(f"  n {U['n']} delays {int(per[:, 4].sum())} err {np.abs(xs - X).max():.1e}", flush=True)
+    assert np.abs(xs - X).max() < 1e-8
