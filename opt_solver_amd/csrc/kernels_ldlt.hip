@@ -1872,6 +1872,7 @@ __global__ __launch_bounds__(NT) void k_dev_fronts(LdltDev d, DevLevel L, const 
     __shared__ RedScratchT<NT> rs;
     __shared__ int cnt[3];
     __shared__ int32_t cpos[FMAX];
+    __shared__ int32_t cptr[FMAX + 1];
     // a refused level is left whole to the host
     if (__atomic_load_n(&L.ctr[kDcLevel], __ATOMIC_RELAXED) <= static_cast<unsigned long long>(lv)) return;
     const int s = flist[blockIdx.x];
@@ -1888,11 +1889,22 @@ __global__ __launch_bounds__(NT) void k_dev_fronts(LdltDev d, DevLevel L, const 
     for (int t = tid; t < Sy.f; t += NT) rows[nd + t] = L.sym_rows[Sy.rofs + t];
     if (tid < 3) cnt[tid] = 0;
     __syncthreads();
-    // original entries of the own pivot columns (one per row and column)
-    for (int oc = 0; oc < F.psym; ++oc) {
-        const int j = F.col0 + oc;
-        double *col = Fs + (nd + oc) * f + nd;
-        for (int k = d.pcol_ptr[j] + tid; k < d.pcol_ptr[j + 1]; k += NT) col[d.amap[k]] += d.av[k];
+    // original entries of the own pivot columns (one per row and column):
+    // the columns' entries are contiguous; their pointers go to LDS once and
+    // each lane finds the column of its entries there
+    for (int t = tid; t <= F.psym; t += NT) cptr[t] = d.pcol_ptr[F.col0 + t];
+    __syncthreads();
+    {
+        const int e0 = cptr[0], e1 = cptr[F.psym];
+        for (int e = e0 + tid; e < e1; e += NT) {
+            int lo = 0, hi = F.psym;  // the column oc with cptr[oc] <= e < cptr[oc + 1]
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (cptr[mid] <= e) lo = mid;
+                else hi = mid;
+            }
+            Fs[(nd + lo) * f + nd + d.amap[e]] += d.av[e];
+        }
     }
     __syncthreads();
     int dp = 0;
