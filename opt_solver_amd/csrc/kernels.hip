@@ -283,7 +283,12 @@ __global__ __launch_bounds__(kThreads) void k_potrf(DevPtrs d, const PotrfItem *
             }
             __syncthreads();
             const double djj = colb[bs][j];
-            const double rs = 1.0 / sqrt(djj);
+            // 1 / sqrt(djj): the hardware estimate refined by two Newton
+            // steps (y <- y (3 - d y^2) / 2), off the division and square
+            // root sequences on this step's critical path
+            double rs = __builtin_amdgcn_rsq(djj);
+            rs = rs * (1.5 - 0.5 * djj * rs * rs);
+            rs = rs * (1.5 - 0.5 * djj * rs * rs);
             const double gr = r > j ? colb[bs][r] * (rs * rs) : 0.0;
             if (cg > jb) {
 #pragma unroll
