@@ -1251,7 +1251,7 @@ __global__ __launch_bounds__(kLT) void k_big_panel(LdltDev d, BigLd *states, Prm
     bool group_done = false;
     const int gcols = gpart ? group_cols(B, nb) : 0;
     if (gcols > 0) {
-        // judge the group the stats launch evaluated: column maxima over
+        // decide on the group the stats launch evaluated: column maxima over
         // its workgroups and the diagonal region, then the first test per
         // pivot in order; whole panels are accepted
         __shared__ double gmax[kGroupCols], gd[kGroupCols];

@@ -147,7 +147,7 @@ void launch_ldlt_fwd_step(hipStream_t st, const LdltDev &d, const SolveItem *ite
                           int32_t n, int32_t nrhs);
 void launch_ldlt_bwd_step(hipStream_t st, const LdltDev &d, const SolveItem *items, int64_t count, double *y,
                           int32_t n, int32_t nrhs);
-// one iteration's pivot logic; with gpart set it first judges the group
+// one iteration's pivot logic; with gpart set it first decides on the group
 // the stats launch evaluated (nch workgroups per front)
 void launch_big_panel(hipStream_t st, const LdltDev &d, BigLd *states, int32_t nbig, double u, double eff_small,
                       double small_pivot, int32_t nb, const double *gpart, int32_t nch);
