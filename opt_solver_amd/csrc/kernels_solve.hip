@@ -344,6 +344,7 @@ constexpr int kLB = 64;
 constexpr int kLThreads = 256;
 constexpr int kLSeg = 512;  // rows of a backward tail item
 constexpr int32_t kItemTri = 2;
+constexpr int32_t kItemFused = 4;  // forward: the triangle, then the rows [a0, a1) below it in the same workgroup
 
 __device__ __forceinline__ double lwave_sum(double v) {
 #pragma unroll
@@ -404,7 +405,21 @@ __global__ __launch_bounds__(kLThreads) void k_ldlt_fwd_step(LdltDev d, const So
                     const double xc = __shfl(yv, c, 64);
                     if (l > c) yv -= T[c][l] * xc;
                 }
-                if (l < it.tkb) yr[rows[it.tk + l]] = yv;
+                if (l < it.tkb) {
+                    yr[rows[it.tk + l]] = yv;
+                    xs[l] = yv;
+                }
+            }
+            if (it.flags & kItemFused) {
+                // the rows below, with the block just solved (the same sums
+                // as the separate row items of the next step)
+                __syncthreads();
+                for (int rb = it.a0; rb < it.a1; rb += kLB) {
+                    const int re = min(rb + kLB, it.a1);
+                    rows_times_block(L, f, it.tk, it.tkb, rb, re, xs, part);
+                    if (tid < 64 && rb + tid < re) atomicAdd(&yr[rows[rb + tid]], -part[0][tid]);
+                    __syncthreads();
+                }
             }
         } else if (tid < 64) {
             const int i = r0 + tid;

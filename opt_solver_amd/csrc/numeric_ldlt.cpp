@@ -345,6 +345,12 @@ static void front_solve_items(int32_t f, int32_t p, int32_t s, bool bwd, Emit &&
     if (p <= 0) return;
     const int32_t nb = (p + B - 1) / B;
     if (!bwd) {
+        constexpr int32_t Fused = 4, FusedRows = 512;
+        if (nb == 1 && f > p && f - p <= FusedRows) {
+            // one workgroup: the triangle, then the rows below it
+            emit(0, SolveItem{s, 0, 0, p, f, 0, p, Tri | Fused});
+            return;
+        }
         for (int32_t t = 0; t < nb; ++t) {
             const int32_t tk = t * B, tkb = std::min(B, p - tk);
             const int32_t k = t > 0 ? tk - B : 0, kb = t > 0 ? B : 0;
