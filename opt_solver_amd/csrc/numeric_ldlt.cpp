@@ -364,6 +364,12 @@ static void front_solve_items(int32_t f, int32_t p, int32_t s, bool bwd, Emit &&
         }
         return;
     }
+    if (nb == 1 && f > p && f - p <= Seg) {
+        // one workgroup: the rows below (one segment) folded into the
+        // triangle item's own column sums, then the triangle
+        emit(0, SolveItem{s, p, f - p, 0, 0, 0, p, Tri});
+        return;
+    }
     int32_t t = 0;
     if (f > p) {
         for (int32_t r = p; r < f; r += Seg)
