@@ -407,39 +407,34 @@ static int ldlt_solve_lists(smfgpu_plan_s *P) {
     // written in place into pinned staging and uploaded once
     auto build = [&](bool bwd, PinnedArr<SolveItem> &items, DBuf<SolveItem> &dev,
                      std::vector<std::pair<int64_t, int64_t>> &steps, std::vector<int32_t> &lvs) -> int {
-        std::vector<int64_t> lv_step0(nlev + 1, 0);
+        std::vector<int32_t> lv_nstep(nlev, 0);
+        std::vector<int64_t> lv_t0(nlev, 0), tfirst;
         int64_t total = 0;
         for (int32_t lv = 0; lv < nlev; ++lv) {
             lvs[lv] = static_cast<int32_t>(steps.size());
-            lv_step0[lv] = static_cast<int64_t>(steps.size());
             cnt.clear();
             for (int32_t s : P->level_fronts[lv])
                 front_solve_items(lf[s], lp[s], s, bwd, [&](int32_t t, const SolveItem &) {
                     if (cnt.size() <= static_cast<size_t>(t)) cnt.resize(t + 1, 0);
                     ++cnt[t];
                 });
-            for (size_t t = 0; t < cnt.size(); ++t)
+            lv_t0[lv] = static_cast<int64_t>(tfirst.size());
+            for (size_t t = 0; t < cnt.size(); ++t) {
+                tfirst.push_back(total);  // first item of step index t (none if cnt[t] == 0)
                 if (cnt[t] > 0) {
                     steps.emplace_back(total, cnt[t]);
                     total += cnt[t];
                 }
+            }
+            lv_nstep[lv] = static_cast<int32_t>(cnt.size());
         }
         lvs[nlev] = static_cast<int32_t>(steps.size());
-        lv_step0[nlev] = static_cast<int64_t>(steps.size());
         int rc;
         if ((rc = items.reserve(static_cast<size_t>(std::max<int64_t>(total, 1))))) return rc;
         for (int32_t lv = 0; lv < nlev; ++lv) {
-            // the level's non-empty steps in order of step index
-            int64_t k = lv_step0[lv];
-            cnt.clear();
-            for (int32_t s : P->level_fronts[lv])
-                front_solve_items(lf[s], lp[s], s, bwd, [&](int32_t t, const SolveItem &) {
-                    if (cnt.size() <= static_cast<size_t>(t)) cnt.resize(t + 1, 0);
-                    ++cnt[t];
-                });
-            pos.assign(cnt.size(), 0);
-            for (size_t t = 0; t < cnt.size(); ++t)
-                if (cnt[t] > 0) pos[t] = steps[k++].first;
+            // the first item of each of the level's step indices, from the
+            // counting pass
+            pos.assign(tfirst.begin() + lv_t0[lv], tfirst.begin() + lv_t0[lv] + lv_nstep[lv]);
             for (int32_t s : P->level_fronts[lv])
                 front_solve_items(lf[s], lp[s], s, bwd, [&](int32_t t, const SolveItem &it) { items[pos[t]++] = it; });
         }
