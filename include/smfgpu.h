@@ -29,8 +29,9 @@
  *   smfgpu_export_factor    the FactorKeep layout
  *                           (solver/include/smf/factor_posdef.hpp:16-220)
  *   smfgpu_solve            solve_core (solver/src/solve.cpp:858-1075)
- *   smfgpu_refine           refine / compute_residual_and_backward_error
- *                           (solver/src/refinement.cpp:39-130, 643-724)
+ *   smfgpu_refine_ex,       refine / refine_core / classical_refine /
+ *   smfgpu_refine           fgmres_refine / compute_residual_and_backward_error
+ *                           (solver/src/refinement.cpp:39-130, 300-724)
  */
 #pragma once
 
@@ -62,12 +63,15 @@ extern "C" {
 #define SMFGPU_STATUS_MAX_PIVOT_DELAYS 3
 #define SMFGPU_STATUS_INERTIA_EXCEEDED 4
 
-/* Precision of the Schur-complement updates of large fronts. Pivot tests,
- * D, inertia, residuals and the refined solution are always FP64. */
+/* Precision of the Schur-complement updates of large fronts (fronts of at
+ * least mixed_min_front rows). Pivot tests, D, inertia, residuals and the
+ * refined solution are always FP64. */
 #define SMFGPU_PREC_FP64 0
-#define SMFGPU_PREC_FP32 1 /* FP32 MFMA, FP32 accumulation */
-#define SMFGPU_PREC_BF16 2 /* BF16 MFMA, FP32 accumulation */
-#define SMFGPU_PREC_AUTO 3 /* per-supernode choice by the learned selector */
+#define SMFGPU_PREC_FP32 1   /* FP32 MFMA, FP32 accumulation */
+#define SMFGPU_PREC_BF16 2   /* BF16 MFMA (operands rounded to bf16), FP32 accumulation */
+#define SMFGPU_PREC_AUTO 3   /* per-supernode choice by the learned selector */
+#define SMFGPU_PREC_BF16X3 4 /* split BF16: operands as hi + lo bf16 pairs, three BF16
+                                MFMAs per product (hi*hi + hi*lo + lo*hi), FP32 accumulation */
 
 /* Memory kind of value / right-hand-side pointers. */
 #define SMFGPU_MEM_HOST 0
@@ -219,11 +223,44 @@ SMFGPU_API int smfgpu_factor_front_stats(smfgpu_factor_t f, int32_t *out);
 SMFGPU_API int smfgpu_solve(smfgpu_factor_t f, double *b, int32_t nrhs, int32_t ldb,
                             int32_t job, int32_t mem);
 
-/* ---- FP64 iterative refinement against the analysed matrix (values as
- * given to smfgpu_factor, same memory kind): x (n x nrhs, device or host
- * per mem) is refined in place until the scaled backward error
- * eta_inf <= tol or max_iters steps or a stall. out[0] eta_inf,
- * out[1] omega, out[2] steps taken. */
+/* ---- FP64 iterative refinement against the analysed matrix (refine /
+ * refine_core, solver/src/refinement.cpp:300-724). values: null for the
+ * factored values, else values of the analysed pattern (one per entry, as
+ * given to smfgpu_factor; memory kind value_mem) to refine against; b and x
+ * are n x nrhs (leading dimensions ldb, ldx >= n; device or host per mem);
+ * x is refined in place and left at the best iterate. Residuals use
+ * double-double products and sums; eta_inf = ||b - A x||_inf /
+ * (||A||_inf ||x||_inf + ||b||_inf), omega = max_i |b - A x|_i /
+ * (|A||x| + |b|)_i (refinement.cpp:39-130), over all columns. */
+#define SMFGPU_REFINE_CLASSICAL 0 /* x += M^-1 r; stop on no improvement or two steps contracting < 0.85 */
+#define SMFGPU_REFINE_AUTO 1      /* classical, escalating to FGMRES on a step contracting < 0.5 */
+#define SMFGPU_REFINE_FGMRES 2    /* right-preconditioned FGMRES(restart), one column at a time */
+typedef struct {
+    int32_t method;        /* SMFGPU_REFINE_* (Control::refine_method, default AUTO) */
+    int32_t max_iters;     /* classical steps (Control::refinement_max_iters, default 5) */
+    int32_t max_steps;     /* AUTO / FGMRES: solves in total (Control::refine_max_steps, default 10) */
+    int32_t restart;       /* FGMRES restart length (Control::fgmres_restart, default 20) */
+    double eta_target;     /* stop once eta_inf <= eta_target ... (default 1e-10) */
+    double omega_target;   /* ... and omega <= omega_target (default 1e-8) */
+    int32_t value_mem;     /* SMFGPU_MEM_* of values (when not null) */
+    int32_t reserved[3];
+} smfgpu_refine_options;
+typedef struct {
+    double eta, omega;             /* of the returned x */
+    double eta_start, omega_start; /* of the given x */
+    int32_t steps;                 /* solves taken */
+    int32_t method_used;           /* SMFGPU_REFINE_CLASSICAL or SMFGPU_REFINE_FGMRES */
+    int32_t reached;               /* 1: both targets met */
+    int32_t stalled;               /* 1: stopped with a target unmet */
+    double reserved[2];
+} smfgpu_refine_info;
+SMFGPU_API int smfgpu_refine_default_options(smfgpu_refine_options *opt);
+SMFGPU_API int smfgpu_refine_ex(smfgpu_factor_t f, const double *values, const double *b, int32_t ldb, double *x,
+                                int32_t ldx, int32_t nrhs, const smfgpu_refine_options *opt, int32_t mem,
+                                smfgpu_refine_info *info);
+/* Shorthand: AUTO with max_iters classical steps and at most max_iters
+ * solves in total, eta_target = tol, no omega target; ldb = ldx = n.
+ * out[0] eta_inf, out[1] omega, out[2] steps taken. */
 SMFGPU_API int smfgpu_refine(smfgpu_factor_t f, const double *values, const double *b,
                              double *x, int32_t nrhs, int32_t max_iters, double tol,
                              int32_t mem, double *out);

@@ -17,7 +17,7 @@ _L = ctypes.c_int64
 _D = ctypes.c_double
 
 SMFGPU_OK = 0
-PREC_FP64, PREC_FP32, PREC_BF16, PREC_AUTO = 0, 1, 2, 3
+PREC_FP64, PREC_FP32, PREC_BF16, PREC_AUTO, PREC_BF16X3 = 0, 1, 2, 3, 4
 MEM_HOST, MEM_DEVICE = 0, 1
 
 STATUS = {0: "Success", 1: "NotPositiveDefinite", 2: "Singular", 3: "MaxPivotDelays", 4: "InertiaExceeded"}
@@ -43,6 +43,23 @@ class FactorOptions(ctypes.Structure):
     ]
 
 
+class RefineOptions(ctypes.Structure):
+    _fields_ = [
+        ("method", _I), ("max_iters", _I), ("max_steps", _I), ("restart", _I), ("eta_target", _D),
+        ("omega_target", _D), ("value_mem", _I), ("reserved", _I * 3),
+    ]
+
+
+class RefineInfo(ctypes.Structure):
+    _fields_ = [
+        ("eta", _D), ("omega", _D), ("eta_start", _D), ("omega_start", _D), ("steps", _I), ("method_used", _I),
+        ("reached", _I), ("stalled", _I), ("reserved", _D * 2),
+    ]
+
+
+REFINE_CLASSICAL, REFINE_AUTO, REFINE_FGMRES = 0, 1, 2
+
+
 class FactorInfo(ctypes.Structure):
     _fields_ = [
         ("status", _I), ("num_positive", _I), ("num_negative", _I), ("num_zero", _I),
@@ -58,7 +75,7 @@ EXPORTED = [
     "smfgpu_analysis_stats", "smfgpu_analysis_free", "smfgpu_plan_create", "smfgpu_plan_free",
     "smfgpu_plan_stats", "smfgpu_factor", "smfgpu_factor_free", "smfgpu_export_factor", "smfgpu_solve",
     "smfgpu_refine", "smfgpu_set_timing", "smfgpu_kernel_times", "smfgpu_plan_tune",
-    "smfgpu_factor_front_stats", "smfgpu_launch_profile",
+    "smfgpu_factor_front_stats", "smfgpu_launch_profile", "smfgpu_refine_default_options", "smfgpu_refine_ex",
 ]
 STATIC_AS_ZERO = 1
 STATIC_ZERO_STAYS_ZERO = 2
@@ -107,6 +124,8 @@ def lib():
     L.smfgpu_export_factor.argtypes = [_P, _P, _P, _P, _P, _P]
     L.smfgpu_solve.argtypes = [_P, _P, _I, _I, _I, _I]
     L.smfgpu_refine.argtypes = [_P, _P, _P, _P, _I, _I, _D, _I, _P]
+    L.smfgpu_refine_default_options.argtypes = [_P]
+    L.smfgpu_refine_ex.argtypes = [_P, _P, _P, _I, _P, _I, _I, _P, _I, _P]
     L.smfgpu_set_timing.argtypes = [_P, _I]
     L.smfgpu_kernel_times.argtypes = [_P, _P]
     L.smfgpu_plan_tune.argtypes = [_P, _I, _L]

@@ -378,9 +378,9 @@ __global__ __launch_bounds__(kThreads) void k_trsm(DevPtrs d, const TrsmItem *__
             if (kk < 4 * cb + 4) acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(Ms[t][16 * cb + lr], bop, acc[cb], 0, 0, 0);
     }
     const int rr = 16 * w + lr;
-    // fronts with FP32 Schur updates also get the panel in FP32 (the Schur
-    // kernel's operands: half the bytes)
-    float *b32 = (d.L32 != nullptr && d.prec != nullptr && d.prec[it.s] == 1) ? d.L32 + F.lofs : nullptr;
+    // fronts with reduced-precision Schur updates also get the panel in
+    // FP32 (the reduced tiles' operands: half the bytes)
+    float *b32 = (d.L32 != nullptr && d.prec != nullptr && d.prec[it.s] != 0) ? d.L32 + F.lofs : nullptr;
     if (rr < nr) {
 #pragma unroll
         for (int cb = 0; cb < 4; ++cb)
@@ -404,16 +404,24 @@ void launch_trsm(hipStream_t st, const DevPtrs &d, const TrsmItem *it, int64_t n
 
 // ---------------------------------------------------------------------------
 // A10/A11: Schur-complement update on MFMA (tile machinery in mfma_tile.hpp).
-template <typename E, int kKc, int kAhead>
+// Precision codes of a front's Schur updates (DevPtrs::prec): 0 FP64 MFMA;
+// 1 FP32 MFMA on the FP32 panel copy; 2 BF16 MFMA (operands rounded to
+// bf16), 3 split BF16 (hi / lo operands, three MFMAs); 1-3 accumulate in
+// FP32 over the panel and subtract the result from the FP64 Schur
+// complement.
+constexpr int kSyrkKc64 = 16;
+constexpr int kSyrkKc32 = 32;
+constexpr int kSyrkKcBf = 64;
+constexpr int kSyrkKcBf3 = 32;
+
+template <int MODE, int kAhead>
 __device__ __forceinline__ void syrk_tile(const DevPtrs &d, const SyrkItem &it, unsigned char *smem) {
+    typedef typename std::conditional<MODE == 0, double, float>::type E;
     typedef SyrkMfma<E> M;
     const FrontDev F = d.fronts[it.s];
     const int f = F.f;
     const int i0 = it.i0, j0 = it.j0;
     const int j1 = min(it.j1, f);
-    // FP64 tiles read the factor, FP32 tiles its FP32 panel copy
-    const E *Lb = std::is_same<E, double>::value ? reinterpret_cast<const E *>(d.L + F.lofs)
-                                                 : reinterpret_cast<const E *>(d.L32 + F.lofs);
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int w = tid >> 6;
@@ -424,8 +432,21 @@ __device__ __forceinline__ void syrk_tile(const DevPtrs &d, const SyrkItem &it, 
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = M::zero();
-    if (active) tile_loop<E, true, kKc, kAhead>(Lb, Lb, f, 0, i0, j0, f, j1, it.t0, it.t1, smem, acc);
-    else tile_loop<E, false, kKc, kAhead>(Lb, Lb, f, 0, i0, j0, f, j1, it.t0, it.t1, smem, acc);
+    if constexpr (MODE == 0) {
+        const double *Lb = d.L + F.lofs;
+        if (active) tile_loop<double, true, kSyrkKc64, kAhead>(Lb, Lb, f, 0, i0, j0, f, j1, it.t0, it.t1, smem, acc);
+        else tile_loop<double, false, kSyrkKc64, kAhead>(Lb, Lb, f, 0, i0, j0, f, j1, it.t0, it.t1, smem, acc);
+    } else if constexpr (MODE == 1) {
+        const float *Lb = d.L32 + F.lofs;
+        if (active) tile_loop<float, true, kSyrkKc32, kAhead>(Lb, Lb, f, 0, i0, j0, f, j1, it.t0, it.t1, smem, acc);
+        else tile_loop<float, false, kSyrkKc32, kAhead>(Lb, Lb, f, 0, i0, j0, f, j1, it.t0, it.t1, smem, acc);
+    } else {
+        constexpr bool kSplit = MODE == 3;
+        constexpr int kKc = kSplit ? kSyrkKcBf3 : kSyrkKcBf;
+        const float *Lb = d.L32 + F.lofs;
+        if (active) tile_loop_bf16<kSplit, true, kKc, float>(Lb, Lb, f, 0, i0, j0, f, j1, it.t0, it.t1, smem, acc);
+        else tile_loop_bf16<kSplit, false, kKc, float>(Lb, Lb, f, 0, i0, j0, f, j1, it.t0, it.t1, smem, acc);
+    }
     if (!active) return;
     // C -= acc, 16 elements at a time: all 16 loads issued before the
     // stores; elements outside the tile's lower part read and write a junk
@@ -458,40 +479,52 @@ __device__ __forceinline__ void syrk_tile(const DevPtrs &d, const SyrkItem &it, 
 
 // A10/A11: Schur-complement update on MFMA. Tile 128 x 128 of
 // C[i, j] -= sum_{t in [t0, t1)} L[i, t] L[j, t], i >= j, four waves of
-// 64 x 64, each 4 x 4 MFMA 16x16x4 blocks. The MFMA computes the
-// transposed block (A operand = the j rows, B operand = the i rows) so that
-// the 16 lanes of a row group hold 16 consecutive rows of one column on the
-// store. Per-front precision: FP64 MFMA, or operands rounded to FP32 with
-// FP32 MFMA accumulation over the panel, subtracted from the FP64 Schur
-// complement.
-// Panel columns per LDS stage: 16 for FP64 operands, 32 for FP32 (the
-// same 64 KB for the two stages of both operands, so two workgroups share a
-// CU; FP32 MFMAs take half the time, the longer chunk keeps as much MFMA
-// work in front of each chunk's loads); kAhead: chunks whose global loads
-// are in flight (1 or 2).
-constexpr int kSyrkKc64 = 16;
-constexpr int kSyrkKc32 = 32;
-
-// One kernel for both precisions: the small FP64 fronts and the large FP32
-// ones of a step share one launch (two launches per step serialise them).
+// 64 x 64 (4 x 4 MFMA blocks of 16 x 16). The MFMA computes the transposed
+// block (A operand = the j rows, B operand = the i rows) so that the 16
+// lanes of a row group hold 16 consecutive rows of one column on the store.
+// Two kernels share each step's item list: k_syrk64 runs the items of FP64
+// fronts, k_syrk_red those of reduced-precision fronts; each returns at
+// once on the other's items. Separate kernels keep the FP32-accumulator
+// tiles free of the FP64 tile's register count. The FP64 tile stages 16
+// panel columns per LDS stage (64 KB for both operands' two stages: two
+// workgroups per CU); the FP32 tile 32; the BF16 tile 64, the split-BF16
+// tile 32 (hi and lo planes).
 template <int kAhead>
-__global__ __launch_bounds__(kThreads) void k_syrk(DevPtrs d, const SyrkItem *__restrict__ items) {
-    constexpr int kBytes = tile_lds_bytes<double, kSyrkKc64>() > tile_lds_bytes<float, kSyrkKc32>()
-                               ? tile_lds_bytes<double, kSyrkKc64>()
-                               : tile_lds_bytes<float, kSyrkKc32>();
+__global__ __launch_bounds__(kThreads) void k_syrk64(DevPtrs d, const SyrkItem *__restrict__ items) {
+    __shared__ __attribute__((aligned(16))) unsigned char smem[tile_lds_bytes<double, kSyrkKc64>()];
+    const SyrkItem it = items[xcd_contiguous(blockIdx.x, gridDim.x)];
+    if (d.prec != nullptr && d.prec[it.s] != 0) return;
+    syrk_tile<0, kAhead>(d, it, smem);
+}
+
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+
+template <int kAhead>
+__global__ __launch_bounds__(kThreads) void k_syrk_red(DevPtrs d, const SyrkItem *__restrict__ items) {
+    constexpr int kBytes = cmax(tile_lds_bytes<float, kSyrkKc32>(),
+                                cmax(bf_lds_bytes<false, kSyrkKcBf>(), bf_lds_bytes<true, kSyrkKcBf3>()));
     __shared__ __attribute__((aligned(16))) unsigned char smem[kBytes];
     const SyrkItem it = items[xcd_contiguous(blockIdx.x, gridDim.x)];
     const int prec = d.prec != nullptr ? d.prec[it.s] : 0;
-    if (prec == 0) syrk_tile<double, kSyrkKc64, kAhead>(d, it, smem);
-    else syrk_tile<float, kSyrkKc32, kAhead>(d, it, smem);
+    if (prec == 1) syrk_tile<1, kAhead>(d, it, smem);
+    else if (prec == 2) syrk_tile<2, kAhead>(d, it, smem);
+    else if (prec == 3) syrk_tile<3, kAhead>(d, it, smem);
 }
 
-void launch_syrk(hipStream_t st, const DevPtrs &d, const SyrkItem *it, int64_t n, int ahead) {
+// n items; the reduced kernel covers the last n_red of them (the items of
+// fronts that may run below FP64)
+void launch_syrk(hipStream_t st, const DevPtrs &d, const SyrkItem *it, int64_t n, int64_t n_red, int ahead) {
     if (n <= 0) return;
     for (int64_t o = 0; o < n; o += kMaxLaunchGroups) {
         const dim3 grid(static_cast<unsigned>(std::min(kMaxLaunchGroups, n - o)));
-        if (ahead == 1) k_syrk<1><<<grid, kThreads, 0, st>>>(d, it + o);
-        else k_syrk<2><<<grid, kThreads, 0, st>>>(d, it + o);
+        if (ahead == 1) k_syrk64<1><<<grid, kThreads, 0, st>>>(d, it + o);
+        else k_syrk64<2><<<grid, kThreads, 0, st>>>(d, it + o);
+    }
+    const SyrkItem *ir = it + (n - n_red);
+    for (int64_t o = 0; o < n_red; o += kMaxLaunchGroups) {
+        const dim3 grid(static_cast<unsigned>(std::min(kMaxLaunchGroups, n_red - o)));
+        if (ahead == 1) k_syrk_red<1><<<grid, kThreads, 0, st>>>(d, ir + o);
+        else k_syrk_red<2><<<grid, kThreads, 0, st>>>(d, ir + o);
     }
 }
 
@@ -529,54 +562,201 @@ void launch_scatter_rhs(hipStream_t st, const int32_t *perm, const double *y, do
 }
 
 // ---------------------------------------------------------------------------
-// A16: residual r = b - A x in permuted numbering over the full symmetric
-// CSR, products and sums in double-double (TwoProd by fma, TwoSum), and the
-// per-row terms of the backward errors (refinement.cpp:39-130):
-// rowstat[4i] = |r_i|, [4i+1] = (|A||x|)_i + |b_i|, [4i+2] = |x_i|,
-// [4i+3] = |b_i|.
+// A16: refinement. Residual r = b - A x in permuted numbering over the full
+// symmetric CSR, products and sums in double-double (TwoProd by fma,
+// TwoSum), as the reference's compute_residual_and_backward_error
+// (refinement.cpp:39-130) sums in long double, and per column the four
+// maxima the backward errors are formed from, reduced on the device:
+// stat[4c] = max |r_i|, stat[4c+1] = max_i |r_i| / ((|A||x|)_i + |b_i|)
+// (+inf when that denominator is 0 and r_i is not), stat[4c+2] = max |x_i|,
+// stat[4c+3] = max |b_i|. Maxima are taken over the bit patterns of the
+// non-negative values (an Inf or NaN sorts above every finite value), by one
+// atomic max per workgroup; stat is cleared by the caller.
 __device__ __forceinline__ void two_sum(double a, double b, double &s, double &e) {
     s = a + b;
     const double bb = s - a;
     e = (a - (s - bb)) + (b - bb);
 }
 
-__global__ void k_residual(int32_t n, const int64_t *__restrict__ rp, const int32_t *__restrict__ ci,
-                           const int64_t *__restrict__ slot, const double *__restrict__ av,
-                           const double *__restrict__ x, const double *__restrict__ b,
-                           double *__restrict__ r, double *__restrict__ rowstat, int32_t nrhs) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__device__ __forceinline__ unsigned long long ubits(double v) {
+    return static_cast<unsigned long long>(__double_as_longlong(v));
+}
+
+// workgroup max of m (kThreads threads) into *out by one atomic
+__device__ __forceinline__ void block_max_atomic(unsigned long long m, unsigned long long *out) {
+    __shared__ unsigned long long red[kThreads / 64];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long v = __shfl_xor(m, o, 64);
+        m = v > m ? v : m;
+    }
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long b = red[0];
+        for (int w = 1; w < kThreads / 64; ++w) b = red[w] > b ? red[w] : b;
+        atomicMax(out, b);
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kThreads) void k_residual(int32_t n, const int64_t *__restrict__ rp,
+                                                       const int32_t *__restrict__ ci,
+                                                       const int64_t *__restrict__ slot,
+                                                       const double *__restrict__ av, const double *__restrict__ x,
+                                                       const double *__restrict__ b, double *__restrict__ r,
+                                                       unsigned long long *__restrict__ stat, int32_t nrhs) {
     for (int c = 0; c < nrhs; ++c) {
         const double *xc = x + static_cast<int64_t>(c) * n;
-        const double bi = b[i + static_cast<int64_t>(c) * n];
-        double sh = bi, sl = 0.0, absax = 0.0;
-        for (int64_t k = rp[i]; k < rp[i + 1]; ++k) {
-            const double a = av[slot[k]];
-            const double xv = xc[ci[k]];
-            const double ph = -a * xv;
-            const double pl = fma(-a, xv, -ph);
-            double s, e;
-            two_sum(sh, ph, s, e);
-            sh = s;
-            sl += e + pl;
-            absax += fabs(a) * fabs(xv);
+        unsigned long long m[4] = {0, 0, 0, 0};
+        for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+             i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+            const double bi = b[i + static_cast<int64_t>(c) * n];
+            double sh = bi, sl = 0.0, absax = 0.0;
+            for (int64_t k = rp[i]; k < rp[i + 1]; ++k) {
+                const double a = av[slot[k]];
+                const double xv = xc[ci[k]];
+                const double ph = -a * xv;
+                const double pl = fma(-a, xv, -ph);
+                double s2, e;
+                two_sum(sh, ph, s2, e);
+                sh = s2;
+                sl += e + pl;
+                absax += fabs(a) * fabs(xv);
+            }
+            const double res = sh + sl;
+            r[i + static_cast<int64_t>(c) * n] = res;
+            const double ar = fabs(res), den = absax + fabs(bi);
+            const double om = den > 0.0 ? ar / den : (ar == 0.0 ? 0.0 : INFINITY);
+            const unsigned long long v[4] = {ubits(ar), ubits(fabs(om)), ubits(fabs(xc[i])), ubits(fabs(bi))};
+#pragma unroll
+            for (int k = 0; k < 4; ++k) m[k] = v[k] > m[k] ? v[k] : m[k];
         }
-        const double res = sh + sl;
-        r[i + static_cast<int64_t>(c) * n] = res;
-        double *rs = rowstat + 4 * (i + static_cast<int64_t>(c) * n);
-        rs[0] = fabs(res);
-        rs[1] = absax + fabs(bi);
-        rs[2] = fabs(xc[i]);
-        rs[3] = fabs(bi);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) block_max_atomic(m[k], stat + 4 * c + k);
     }
 }
 
 void launch_residual(hipStream_t st, int32_t n, const int64_t *rp, const int32_t *ci, const int64_t *slot,
-                     const double *av, const double *x, const double *b, double *r, double *rowstat,
+                     const double *av, const double *x, const double *b, double *r, double *stat,
                      int32_t nrhs) {
+    auto *o = reinterpret_cast<unsigned long long *>(stat);
+    k_fill_u64<<<dim3((4 * nrhs + 63) / 64), 64, 0, st>>>(o, 4 * nrhs, 0ull);
     if (n <= 0) return;
-    k_residual<<<dim3((n + kThreads - 1) / kThreads), kThreads, 0, st>>>(n, rp, ci, slot, av, x, b, r, rowstat,
-                                                                         nrhs);
+    const int64_t g = std::min<int64_t>((n + kThreads - 1) / kThreads, 2048);
+    k_residual<<<dim3(static_cast<unsigned>(g)), kThreads, 0, st>>>(n, rp, ci, slot, av, x, b, r, o, nrhs);
+}
+
+// w = A z (FP64, full symmetric CSR, permuted numbering)
+__global__ __launch_bounds__(kThreads) void k_spmv(int32_t n, const int64_t *__restrict__ rp,
+                                                   const int32_t *__restrict__ ci, const int64_t *__restrict__ slot,
+                                                   const double *__restrict__ av, const double *__restrict__ z,
+                                                   double *__restrict__ w) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int64_t k = rp[i]; k < rp[i + 1]; ++k) s += av[slot[k]] * z[ci[k]];
+    w[i] = s;
+}
+
+void launch_spmv(hipStream_t st, int32_t n, const int64_t *rp, const int32_t *ci, const int64_t *slot,
+                 const double *av, const double *z, double *w) {
+    if (n <= 0) return;
+    k_spmv<<<dim3((n + kThreads - 1) / kThreads), kThreads, 0, st>>>(n, rp, ci, slot, av, z, w);
+}
+
+// Dot products of w with nvec columns of V (ld n), in two passes with a
+// fixed summation order (deterministic): partial sums per workgroup, then
+// one workgroup per column sums them; out[i] = sum (acc: out[i] += sum;
+// root: out[i] = sqrt of the sum).
+__global__ __launch_bounds__(kThreads) void k_dot_part(int64_t n, const double *__restrict__ V, int32_t nvec,
+                                                       const double *__restrict__ w, double *__restrict__ part) {
+    __shared__ double red[kThreads];
+    const int i = blockIdx.y;
+    const double *v = V + static_cast<int64_t>(i) * n;
+    double s = 0.0;
+    for (int64_t k = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; k < n;
+         k += static_cast<int64_t>(gridDim.x) * kThreads)
+        s += v[k] * w[k];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = kThreads / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[static_cast<int64_t>(i) * gridDim.x + blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(kThreads) void k_dot_finish(const double *__restrict__ part, int32_t nblk,
+                                                         double *__restrict__ out, int32_t mode) {
+    __shared__ double red[kThreads];
+    const int i = blockIdx.x;
+    double s = 0.0;
+    for (int k = threadIdx.x; k < nblk; k += kThreads) s += part[static_cast<int64_t>(i) * nblk + k];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = kThreads / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (mode == 1) out[i] += red[0];
+        else if (mode == 2) out[i] = sqrt(red[0]);
+        else out[i] = red[0];
+    }
+}
+
+void launch_dots(hipStream_t st, int64_t n, const double *V, int32_t nvec, const double *w, double *part,
+                 double *out, int32_t mode) {
+    if (nvec <= 0) return;
+    const int nblk = static_cast<int>(std::min<int64_t>(kDotParts, std::max<int64_t>(1, (n + kThreads - 1) / kThreads)));
+    k_dot_part<<<dim3(nblk, nvec), kThreads, 0, st>>>(n, V, nvec, w, part);
+    k_dot_finish<<<dim3(nvec), kThreads, 0, st>>>(part, nblk, out, mode);
+}
+
+// w -= V h (nvec columns, ld n; h on the device)
+__global__ __launch_bounds__(kThreads) void k_vsub(int64_t n, const double *__restrict__ V, int32_t nvec,
+                                                   const double *__restrict__ h, double *__restrict__ w) {
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (k >= n) return;
+    double s = w[k];
+    for (int i = 0; i < nvec; ++i) s -= V[static_cast<int64_t>(i) * n + k] * h[i];
+    w[k] = s;
+}
+
+void launch_vsub(hipStream_t st, int64_t n, const double *V, int32_t nvec, const double *h, double *w) {
+    if (n <= 0 || nvec <= 0) return;
+    k_vsub<<<dim3(static_cast<unsigned>((n + kThreads - 1) / kThreads)), kThreads, 0, st>>>(n, V, nvec, h, w);
+}
+
+// out = w / *d (d on the device)
+__global__ __launch_bounds__(kThreads) void k_vdiv(int64_t n, const double *__restrict__ w,
+                                                   const double *__restrict__ d, double *__restrict__ out) {
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (k < n) out[k] = w[k] / d[0];
+}
+
+void launch_vdiv(hipStream_t st, int64_t n, const double *w, const double *d, double *out) {
+    if (n <= 0) return;
+    k_vdiv<<<dim3(static_cast<unsigned>((n + kThreads - 1) / kThreads)), kThreads, 0, st>>>(n, w, d, out);
+}
+
+// out = x0 + Z y (nvec columns of Z, ld n; y on the device)
+__global__ __launch_bounds__(kThreads) void k_lincomb(int64_t n, const double *__restrict__ x0,
+                                                      const double *__restrict__ Z, int32_t nvec,
+                                                      const double *__restrict__ y, double *__restrict__ out) {
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (k >= n) return;
+    double s = x0[k];
+    for (int i = 0; i < nvec; ++i) s += y[i] * Z[static_cast<int64_t>(i) * n + k];
+    out[k] = s;
+}
+
+void launch_lincomb(hipStream_t st, int64_t n, const double *x0, const double *Z, int32_t nvec, const double *y,
+                    double *out) {
+    if (n <= 0) return;
+    k_lincomb<<<dim3(static_cast<unsigned>((n + kThreads - 1) / kThreads)), kThreads, 0, st>>>(n, x0, Z, nvec, y,
+                                                                                                out);
 }
 
 // row sums of |A| (for ||A||_inf)
@@ -595,14 +775,15 @@ void launch_rowabs(hipStream_t st, int32_t n, const int64_t *rp, const int64_t *
     k_rowabs<<<dim3((n + kThreads - 1) / kThreads), kThreads, 0, st>>>(n, rp, slot, av, out);
 }
 
-__global__ void k_axpy(int64_t n, const double *__restrict__ dlt, double *__restrict__ x) {
+// x = y + d (y may be x)
+__global__ void k_axpy(int64_t n, const double *__restrict__ y, const double *__restrict__ dlt, double *x) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) x[i] += dlt[i];
+    if (i < n) x[i] = y[i] + dlt[i];
 }
 
-void launch_axpy(hipStream_t st, int64_t n, const double *dlt, double *x) {
+void launch_axpy(hipStream_t st, int64_t n, const double *y, const double *dlt, double *x) {
     if (n <= 0) return;
-    k_axpy<<<dim3(static_cast<unsigned>((n + kThreads - 1) / kThreads)), kThreads, 0, st>>>(n, dlt, x);
+    k_axpy<<<dim3(static_cast<unsigned>((n + kThreads - 1) / kThreads)), kThreads, 0, st>>>(n, y, dlt, x);
 }
 
 // max over i of v[i] (v >= 0), into *out (bit-pattern max, out zeroed first)

@@ -7,6 +7,7 @@
 #include "mfma_tile.hpp"
 
 #include <cfloat>
+#include <type_traits>
 
 namespace smfgpu {
 
@@ -854,12 +855,20 @@ __device__ __forceinline__ void super_block_tile(int nrb, int ncb, int tile, int
 
 // MFMA tile of C[i][j] -= sum_t A[i][t] * B[j][t - boff] over the lower
 // part of rows [c0, f) x columns [c0, c1) of front F (c1 - c0 <= f - c0);
-// tile `tile` of the region in super-block order. E = float: FP32 MFMA on
-// the FP64 operands (rounded as they enter LDS), FP32 accumulation over
-// the panel, subtracted from the FP64 entries.
-template <typename E>
+// tile `tile` of the region in super-block order. MODE 0: FP64 MFMA;
+// 1: FP32 MFMA on the FP64 operands (rounded as they enter LDS); 2: BF16
+// MFMA (operands rounded to bf16); 3: split BF16 (hi / lo operands, three
+// MFMAs). Modes 1-3 accumulate in FP32 over the panel and subtract the
+// result from the FP64 entries.
+constexpr int kBigKcBf = 32;  // panel columns per LDS stage of the BF16 tiles (FP64 operands in registers)
+template <int MODE>
+__host__ __device__ constexpr int big_lds_bytes() {
+    return MODE <= 1 ? 4 * 16 * kT * static_cast<int>(sizeof(double)) : bf_lds_bytes<MODE == 3, kBigKcBf>();
+}
+template <int MODE>
 __device__ __forceinline__ void big_tile(const LdltDev &d, const FrontLd &F, const double *Bm, int boff, int c0, int c1, int t0,
                          int t1, int tile, unsigned char *smem) {
+    typedef typename std::conditional<MODE == 0, double, float>::type E;
     typedef SyrkMfma<E> M;
     const int f = F.f;
     int ib, jb;
@@ -874,8 +883,13 @@ __device__ __forceinline__ void big_tile(const LdltDev &d, const FrontLd &F, con
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = M::zero();
-    if (active) tile_loop<E, true, 16, 2, double>(G, Bm, f, boff, i0, j0, f, c1, t0, t1, smem, acc);
-    else tile_loop<E, false, 16, 2, double>(G, Bm, f, boff, i0, j0, f, c1, t0, t1, smem, acc);
+    if constexpr (MODE <= 1) {
+        if (active) tile_loop<E, true, 16, 2, double>(G, Bm, f, boff, i0, j0, f, c1, t0, t1, smem, acc);
+        else tile_loop<E, false, 16, 2, double>(G, Bm, f, boff, i0, j0, f, c1, t0, t1, smem, acc);
+    } else {
+        if (active) tile_loop_bf16<MODE == 3, true, kBigKcBf, double>(G, Bm, f, boff, i0, j0, f, c1, t0, t1, smem, acc);
+        else tile_loop_bf16<MODE == 3, false, kBigKcBf, double>(G, Bm, f, boff, i0, j0, f, c1, t0, t1, smem, acc);
+    }
     if (!active) return;
     double *junk = d.junk + threadIdx.x;
 #pragma unroll
@@ -1442,11 +1456,11 @@ __global__ __launch_bounds__(kLT) void k_big_panel(LdltDev d, BigLd *states, Prm
 
 // The requested update and flush of every front: grid (per_front, nbig),
 // tiles by grid stride.
-template <typename E>
+template <int MODE>
 __global__ __launch_bounds__(kLT, 2) void k_big_update(LdltDev d, const BigLd *__restrict__ states) {
-    __shared__ __attribute__((aligned(16))) unsigned char smem[4 * 16 * kT * sizeof(double)];
+    __shared__ __attribute__((aligned(16))) unsigned char smem[big_lds_bytes<MODE>()];
     const BigLd B = states[blockIdx.y];
-    if (B.prec != (sizeof(E) == 4 ? 1 : 0)) return;  // the other precision's launch
+    if (B.prec != MODE) return;  // another precision's launch
     const FrontLd F = d.fronts[B.s];
     const double *W = d.work + F.wofs;
     const int t1 = (B.u_p1 > B.u_p0) ? big_tiles(F.f, B.u_c0, B.u_c1) : 0;
@@ -1454,7 +1468,7 @@ __global__ __launch_bounds__(kLT, 2) void k_big_update(LdltDev d, const BigLd *_
     const XcdRange xr = xcd_range(t1 + t2);
     for (int tile = xr.lo; tile < xr.hi; tile += xr.step) {
         const bool u = tile < t1;
-        big_tile<E>(d, F, W, B.w_base, u ? B.u_c0 : B.x_c0, u ? B.u_c1 : F.ne, u ? B.u_p0 : B.x_p0,
+        big_tile<MODE>(d, F, W, B.w_base, u ? B.u_c0 : B.x_c0, u ? B.u_c1 : F.ne, u ? B.u_p0 : B.x_p0,
                     u ? B.u_p1 : B.x_p1, u ? tile : tile - t1, smem);
         __syncthreads();
     }
@@ -1558,19 +1572,19 @@ __global__ __launch_bounds__(kLT) void k_big_ld(LdltDev d, const BigLd *__restri
     }
 }
 
-template <typename E>
+template <int MODE>
 __global__ __launch_bounds__(kLT, 2) void k_big_cbupdate(LdltDev d, const BigLd *__restrict__ states, int32_t c0,
                                                      int32_t width) {
-    __shared__ __attribute__((aligned(16))) unsigned char smem[4 * 16 * kT * sizeof(double)];
+    __shared__ __attribute__((aligned(16))) unsigned char smem[big_lds_bytes<MODE>()];
     const BigLd B = states[blockIdx.y];
-    if (B.prec != (sizeof(E) == 4 ? 1 : 0)) return;
+    if (B.prec != MODE) return;
     const FrontLd F = d.fronts[B.s];
     const int c1 = min(B.k, c0 + width);
     if (c1 <= c0 || F.ne >= F.f) return;
     const double *W = d.work + F.wofs;
     const XcdRange xr = xcd_range(big_tiles(F.f, F.ne, F.f));
     for (int tile = xr.lo; tile < xr.hi; tile += xr.step) {
-        big_tile<E>(d, F, W, c0, F.ne, F.f, c0, c1, tile, smem);
+        big_tile<MODE>(d, F, W, c0, F.ne, F.f, c0, c1, tile, smem);
         __syncthreads();
     }
 }
@@ -1605,10 +1619,13 @@ void launch_big_group_apply(hipStream_t st, const LdltDev &d, const BigLd *state
 }
 
 void launch_big_update(hipStream_t st, const LdltDev &d, const BigLd *states, int32_t nbig, int32_t per_front,
-                       int32_t n64, int32_t n32) {
+                       const int32_t *nmode) {
     if (nbig <= 0) return;
-    if (n64 > 0) k_big_update<double><<<dim3(per_front, nbig), kLT, 0, st>>>(d, states);
-    if (n32 > 0) k_big_update<float><<<dim3(per_front, nbig), kLT, 0, st>>>(d, states);
+    const dim3 g(per_front, nbig);
+    if (nmode[0] > 0) k_big_update<0><<<g, kLT, 0, st>>>(d, states);
+    if (nmode[1] > 0) k_big_update<1><<<g, kLT, 0, st>>>(d, states);
+    if (nmode[2] > 0) k_big_update<2><<<g, kLT, 0, st>>>(d, states);
+    if (nmode[3] > 0) k_big_update<3><<<g, kLT, 0, st>>>(d, states);
 }
 
 void launch_big_stall(hipStream_t st, const LdltDev &d, BigLd *states, int32_t nbig, int32_t nb) {
@@ -1617,11 +1634,14 @@ void launch_big_stall(hipStream_t st, const LdltDev &d, BigLd *states, int32_t n
 }
 
 void launch_big_cb(hipStream_t st, const LdltDev &d, const BigLd *states, int32_t nbig, int32_t c0, int32_t width,
-                   int32_t per_front, int32_t n64, int32_t n32) {
+                   int32_t per_front, const int32_t *nmode) {
     if (nbig <= 0) return;
-    k_big_ld<<<dim3(per_front, nbig), kLT, 0, st>>>(d, states, c0, width);
-    if (n64 > 0) k_big_cbupdate<double><<<dim3(per_front, nbig), kLT, 0, st>>>(d, states, c0, width);
-    if (n32 > 0) k_big_cbupdate<float><<<dim3(per_front, nbig), kLT, 0, st>>>(d, states, c0, width);
+    const dim3 g(per_front, nbig);
+    k_big_ld<<<g, kLT, 0, st>>>(d, states, c0, width);
+    if (nmode[0] > 0) k_big_cbupdate<0><<<g, kLT, 0, st>>>(d, states, c0, width);
+    if (nmode[1] > 0) k_big_cbupdate<1><<<g, kLT, 0, st>>>(d, states, c0, width);
+    if (nmode[2] > 0) k_big_cbupdate<2><<<g, kLT, 0, st>>>(d, states, c0, width);
+    if (nmode[3] > 0) k_big_cbupdate<3><<<g, kLT, 0, st>>>(d, states, c0, width);
 }
 
 void launch_big_finish(hipStream_t st, const LdltDev &d, const BigLd *states, int32_t nbig) {

@@ -180,5 +180,153 @@ __device__ __forceinline__ void tile_loop(const GE *Ab, const GE *Bb, int ld, in
     }
 }
 
+// ---------------------------------------------------------------------------
+// BF16 MFMA tile (v_mfma_f32_16x16x32_bf16, FP32 accumulation). Same tile
+// geometry and output layout as the FP32 tile (accumulator value v of a lane
+// is row 4 (lane / 16) + v of the 16 x 16 block), but the operands sit in
+// LDS row-major ([row][k], kKc k-values plus 8 of padding per row) so that
+// each lane reads the 8 consecutive k-values of one row that the
+// instruction takes per lane as one 16-byte read. The A and B operands use
+// the same lane -> k assignment, so the order of k inside the 32-wide step
+// does not matter for the dot products.
+// SPLIT = false: each operand x is rounded to bf16 (8 significant bits).
+// SPLIT = true: x = hi + lo with hi = bf16(x), lo = bf16(x - hi), and the
+// product is hi*hi + hi*lo + lo*hi in three MFMAs (the lo*lo term is
+// dropped): about 16 significant bits per operand at a third of the BF16
+// rate, the split-bfloat16 scheme of Henry, Tang and Heinecke, "Leveraging
+// the bfloat16 artificial intelligence datatype for higher-precision
+// computations" (ARITH 2019).
+typedef __bf16 v8bf_t __attribute__((ext_vector_type(8)));
+
+template <int kKc>
+__host__ __device__ constexpr int bf_lds_ld() { return kKc + 8; }
+// LDS bytes: 2 stages x 2 operands (x 2 for hi / lo) of kT rows
+template <bool SPLIT, int kKc>
+__host__ __device__ constexpr int bf_lds_bytes() { return (SPLIT ? 2 : 1) * 2 * 2 * kT * bf_lds_ld<kKc>() * 2; }
+
+__device__ __forceinline__ v4f_t mfma_bf16(v8bf_t a, v8bf_t b, v4f_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// GE: operand type in global memory (float: an FP32 panel copy; double: FP64
+// data rounded on the way into LDS). Element (row i, panel column t) of A is
+// Ab[t * ld + i], of B Bb[(t - boff) * ld + j]; rows i < f, j < j1 and panel
+// columns [t0, t1) are valid, the rest read as 0 (buffer loads past the
+// descriptor). Each thread loads one row's 8 consecutive k-values per group
+// (the 256 threads cover 128 rows x kKc columns of each operand), converts
+// them and writes one 16-byte LDS vector per operand (two with SPLIT).
+template <bool SPLIT, bool ACTIVE, int kKc, typename GE>
+__device__ __forceinline__ void tile_loop_bf16(const GE *Ab, const GE *Bb, int ld, int boff, int i0, int j0, int f,
+                                               int j1, int t0, int t1, unsigned char *smem, v4f_t (&acc)[4][4]) {
+    constexpr int kLd = bf_lds_ld<kKc>();
+    constexpr int kGroups = kKc / 16;  // 8-column groups per thread and operand
+    constexpr int kLoads = 8 * kGroups;
+    constexpr int kPlane = 2 * kT * kLd;  // one operand's two stages, in bf16 elements
+    __bf16 *lds = reinterpret_cast<__bf16 *>(smem);
+    // planes: A (hi), B (hi), then with SPLIT A (lo), B (lo); stage s of a
+    // plane starts at s * kT * kLd
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int w = tid >> 6;
+    const int wr = w >> 1, wc = w & 1;
+    constexpr uint32_t kOut = 0x80000000u;
+    const int rl = tid & (kT - 1), th = tid >> 7;
+    const uint32_t colb = static_cast<uint32_t>(ld) * static_cast<uint32_t>(sizeof(GE));
+    const uint32_t voa = (i0 + rl < f) ? static_cast<uint32_t>(i0 + rl) * sizeof(GE) : kOut;
+    const uint32_t vob = (j0 + rl < j1) ? static_cast<uint32_t>(j0 + rl) * sizeof(GE) : kOut;
+    typedef GE Regs[kLoads];
+    auto gload = [&](Regs &ra, Regs &rb, int kc) {
+        const int nrec = min(kKc, t1 - kc) * static_cast<int>(colb);
+        const __amdgpu_buffer_rsrc_t sa = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<GE *>(Ab + static_cast<int64_t>(kc) * ld), 0, nrec, kBufferWord3);
+        const __amdgpu_buffer_rsrc_t sb = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<GE *>(Bb + static_cast<int64_t>(kc - boff) * ld), 0, nrec, kBufferWord3);
+#pragma unroll
+        for (int e = 0; e < kGroups; ++e)
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const uint32_t co = static_cast<uint32_t>(8 * (th + 2 * e) + u) * colb;
+                ra[8 * e + u] = buffer_load<GE>(sa, voa + co);
+                rb[8 * e + u] = buffer_load<GE>(sb, vob + co);
+            }
+    };
+    auto sstore = [&](const Regs &ra, const Regs &rb, int stg) {
+#pragma unroll
+        for (int e = 0; e < kGroups; ++e) {
+            const int o = stg * kT * kLd + rl * kLd + 8 * (th + 2 * e);
+            v8bf_t ha, hb, la, lb;
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const float xa = static_cast<float>(ra[8 * e + u]);
+                const float xb = static_cast<float>(rb[8 * e + u]);
+                ha[u] = static_cast<__bf16>(xa);
+                hb[u] = static_cast<__bf16>(xb);
+                if (SPLIT) {
+                    la[u] = static_cast<__bf16>(xa - static_cast<float>(ha[u]));
+                    lb[u] = static_cast<__bf16>(xb - static_cast<float>(hb[u]));
+                }
+            }
+            *reinterpret_cast<v8bf_t *>(lds + o) = ha;
+            *reinterpret_cast<v8bf_t *>(lds + kPlane + o) = hb;
+            if (SPLIT) {
+                *reinterpret_cast<v8bf_t *>(lds + 2 * kPlane + o) = la;
+                *reinterpret_cast<v8bf_t *>(lds + 3 * kPlane + o) = lb;
+            }
+        }
+    };
+    auto mfma_stage = [&](int stg) {
+        if (!ACTIVE) return;
+        const __bf16 *As = lds + stg * kT * kLd;
+        const __bf16 *Bs = lds + kPlane + stg * kT * kLd;
+#pragma unroll
+        for (int kk = 0; kk < kKc; kk += 32) {
+            const int ko = kk + 8 * (lane >> 4);
+            v8bf_t a[4], b[4];
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+                a[m] = *reinterpret_cast<const v8bf_t *>(Bs + (wc * 64 + m * 16 + (lane & 15)) * kLd + ko);
+#pragma unroll
+            for (int nn = 0; nn < 4; ++nn)
+                b[nn] = *reinterpret_cast<const v8bf_t *>(As + (wr * 64 + nn * 16 + (lane & 15)) * kLd + ko);
+            if (SPLIT) {
+                v8bf_t al[4], bl[4];
+#pragma unroll
+                for (int m = 0; m < 4; ++m)
+                    al[m] = *reinterpret_cast<const v8bf_t *>(Bs + 2 * kPlane + (wc * 64 + m * 16 + (lane & 15)) * kLd + ko);
+#pragma unroll
+                for (int nn = 0; nn < 4; ++nn)
+                    bl[nn] = *reinterpret_cast<const v8bf_t *>(As + 2 * kPlane + (wr * 64 + nn * 16 + (lane & 15)) * kLd + ko);
+                // small terms first
+#pragma unroll
+                for (int m = 0; m < 4; ++m)
+#pragma unroll
+                    for (int nn = 0; nn < 4; ++nn) {
+                        acc[m][nn] = mfma_bf16(al[m], b[nn], acc[m][nn]);
+                        acc[m][nn] = mfma_bf16(a[m], bl[nn], acc[m][nn]);
+                    }
+            }
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+#pragma unroll
+                for (int nn = 0; nn < 4; ++nn) acc[m][nn] = mfma_bf16(a[m], b[nn], acc[m][nn]);
+        }
+    };
+    // one register set: the loads of chunk c+1 are in flight during the
+    // MFMAs of chunk c
+    int stg = 0;
+    Regs ra, rb;
+    gload(ra, rb, t0);
+    sstore(ra, rb, 0);
+    __syncthreads();
+    for (int kc = t0; kc < t1; kc += kKc) {
+        const bool more = kc + kKc < t1;
+        if (more) gload(ra, rb, kc + kKc);
+        mfma_stage(stg);
+        if (more) sstore(ra, rb, stg ^ 1);
+        __syncthreads();
+        stg ^= 1;
+    }
+}
+
 
 }  // namespace smfgpu

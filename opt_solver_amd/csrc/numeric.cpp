@@ -26,6 +26,19 @@ int set_error(int code, const std::string &msg) {
 
 const char *last_error() { return g_err.c_str(); }
 
+// device precision code of a SMFGPU_PREC_* request (DevPtrs::prec), -1
+// when the request cannot be served
+int prec_code(int32_t precision) {
+    switch (precision) {
+    case SMFGPU_PREC_FP64: return 0;
+    case SMFGPU_PREC_FP32: return 1;
+    case SMFGPU_PREC_BF16: return 2;
+    case SMFGPU_PREC_BF16X3: return 3;
+    default: return -1;
+    }
+}
+
+
 }  // namespace smfgpu
 
 using namespace smfgpu;
@@ -51,10 +64,12 @@ DevPtrs dev_ptrs(smfgpu_plan_s *P) {
 }
 
 static void release_graph(smfgpu_plan_s *P) {
-    if (P->fexec) (void)hipGraphExecDestroy(P->fexec);
-    if (P->fgraph) (void)hipGraphDestroy(P->fgraph);
-    P->fexec = nullptr;
-    P->fgraph = nullptr;
+    for (int g = 0; g < 2; ++g) {
+        if (P->fexec[g]) (void)hipGraphExecDestroy(P->fexec[g]);
+        if (P->fgraph[g]) (void)hipGraphDestroy(P->fgraph[g]);
+        P->fexec[g] = nullptr;
+        P->fgraph[g] = nullptr;
+    }
 }
 
 // Solve schedule (static for SPD). Per front, pivot blocks of
@@ -189,9 +204,23 @@ int dense_schedule(smfgpu_plan_s *P) {
                     for (int32_t rb = std::max(R, cb); rb < std::min(R + kSyrkGroup, nr); ++rb)
                         syrk.push_back(SyrkItem{s, jlo + rb * kTile, jlo + cb * kTile, jhi, t0, t1, 0, 0});
     };
+    // a Schur launch: the items of the fronts that always run in FP64
+    // first, then those of the fronts that may run below it (f >=
+    // red_min), counted in Launch::red
+    const int32_t red_min = P->sched_red_min;
+    auto syrk_launch = [&](int32_t lv, const std::vector<int32_t> &fl, auto &&emit) {
+        const int64_t y0 = static_cast<int64_t>(syrk.size());
+        for (int32_t s : fl)
+            if (P->fronts[s].f < red_min) emit(s);
+        const int64_t yr = static_cast<int64_t>(syrk.size());
+        for (int32_t s : fl)
+            if (P->fronts[s].f >= red_min) emit(s);
+        const int64_t y1 = static_cast<int64_t>(syrk.size());
+        P->launches.push_back(Launch{K_SYRK, lv, y0, y1 - y0, y1 - yr});
+    };
     for (int32_t lv = 0; lv < nlev; ++lv) {
         const auto &fl = P->level_fronts[lv];
-        P->launches.push_back(Launch{K_ASSEMBLE, lv, P->asm_first[lv], P->asm_first[lv + 1] - P->asm_first[lv]});
+        P->launches.push_back(Launch{K_ASSEMBLE, lv, P->asm_first[lv], P->asm_first[lv + 1] - P->asm_first[lv], 0});
         int32_t maxp = 0;
         for (int32_t s : fl) maxp = std::max(maxp, P->fronts[s].p);
         for (int32_t K = 0; K < maxp; K += outer) {
@@ -205,40 +234,34 @@ int dense_schedule(smfgpu_plan_s *P) {
                     potrf.push_back(PotrfItem{s, k, kb, 0});
                     for (int32_t r0 = k + kb; r0 < F.f; r0 += kPanelInner) trsm.push_back(TrsmItem{s, k, kb, r0});
                 }
-                P->launches.push_back(Launch{K_POTRF, lv, p0, static_cast<int64_t>(potrf.size()) - p0});
-                P->launches.push_back(Launch{K_TRSM, lv, t0, static_cast<int64_t>(trsm.size()) - t0});
-                const int64_t y0 = static_cast<int64_t>(syrk.size());
-                for (int32_t s : fl) {
+                P->launches.push_back(Launch{K_POTRF, lv, p0, static_cast<int64_t>(potrf.size()) - p0, 0});
+                P->launches.push_back(Launch{K_TRSM, lv, t0, static_cast<int64_t>(trsm.size()) - t0, 0});
+                syrk_launch(lv, fl, [&](int32_t s) {
                     const FrontDev &F = P->fronts[s];
-                    if (F.p <= k) continue;
+                    if (F.p <= k) return;
                     const int32_t kb = std::min(kPanelInner, F.p - k);
                     const int32_t kend = std::min(K + outer, F.p);
                     if (k + kb < kend) add_syrk(s, k + kb, kend, k, k + kb);
-                }
-                P->launches.push_back(Launch{K_SYRK, lv, y0, static_cast<int64_t>(syrk.size()) - y0});
+                });
             }
             // trailing update of the remaining pivot columns only (all
             // their rows, the ones below the pivots included)
-            const int64_t y0 = static_cast<int64_t>(syrk.size());
-            for (int32_t s : fl) {
+            syrk_launch(lv, fl, [&](int32_t s) {
                 const FrontDev &F = P->fronts[s];
-                if (F.p <= K) continue;
+                if (F.p <= K) return;
                 const int32_t kend = std::min(K + outer, F.p);
                 if (kend < F.p) add_syrk(s, kend, F.p, K, kend);
-            }
-            P->launches.push_back(Launch{K_SYRK, lv, y0, static_cast<int64_t>(syrk.size()) - y0});
+            });
         }
         // the contribution blocks, once per cb_seg pivot columns (the
         // reference's final schur_update, ldlt_kernel.cpp:1135): far less
         // read-modify-write traffic than one update per outer panel
         for (int32_t t0 = 0; t0 < maxp; t0 += P->cb_seg) {
-            const int64_t y0 = static_cast<int64_t>(syrk.size());
-            for (int32_t s : fl) {
+            syrk_launch(lv, fl, [&](int32_t s) {
                 const FrontDev &F = P->fronts[s];
-                if (F.p <= t0 || F.q <= 0) continue;
+                if (F.p <= t0 || F.q <= 0) return;
                 add_syrk(s, F.p, F.f, t0, std::min(t0 + P->cb_seg, F.p));
-            }
-            P->launches.push_back(Launch{K_SYRK, lv, y0, static_cast<int64_t>(syrk.size()) - y0});
+            });
         }
     }
     P->launches.erase(std::remove_if(P->launches.begin(), P->launches.end(),
@@ -565,9 +588,13 @@ SMFGPU_API void smfgpu_plan_free(smfgpu_plan_t *plan) {
     }
     for (hipEvent_t e : P->ev) (void)hipEventDestroy(e);
     for (hipEvent_t e : P->sev) (void)hipEventDestroy(e);
-    if (P->fexec) (void)hipGraphExecDestroy(P->fexec);
-    if (P->fgraph) (void)hipGraphDestroy(P->fgraph);
-    if (P->live) P->live->plan = nullptr;
+    release_graph(P);
+    // factor handles outlive the plan only as empty shells: every later
+    // call on them reports SMFGPU_ERR_STATE
+    for (smfgpu_factor_s *f : P->issued) {
+        f->plan = nullptr;
+        f->valid = false;
+    }
     delete P;
     *plan = nullptr;
 }
@@ -694,6 +721,9 @@ SMFGPU_API int smfgpu_factor(smfgpu_plan_t P, const double *values, int64_t nval
     if ((opt.posdef != 0) != (P->posdef != 0))
         return set_error(SMFGPU_ERR_INVALID_ARG, "factor: matrix type differs from the plan's");
     if (nvalues != P->n_orig) return set_error(SMFGPU_ERR_INVALID_ARG, "factor: wrong number of values");
+    if (prec_code(opt.precision) < 0)
+        return set_error(opt.precision == SMFGPU_PREC_AUTO ? SMFGPU_ERR_UNSUPPORTED : SMFGPU_ERR_INVALID_ARG,
+                         "factor: precision " + std::to_string(opt.precision) + " is not available");
     if (nvalues > 0 && !values) return set_error(SMFGPU_ERR_INVALID_ARG, "factor: null values");
     hipStream_t st = P->stream;
     const double *dv = values;
@@ -726,21 +756,31 @@ SMFGPU_API int smfgpu_factor(smfgpu_plan_t P, const double *values, int64_t nval
         // solve_core refuses a singular factor (solve.cpp:888-903)
         F->valid = fi.status == SMFGPU_STATUS_SUCCESS;
         P->live = F.get();
+        P->issued.insert(F.get());
         P->max_abs = fi.max_abs;
         *out = F.release();
         return SMFGPU_OK;
     }
-    // per-front Schur precision
+    // per-front Schur precision: fronts of at least mixed_min_front rows
+    // run the requested reduced precision
     auto F = std::make_unique<smfgpu_factor_s>();
     F->plan = P;
     F->prec.assign(P->nsup, 0);
     int reduced = 0;
-    if (opt.precision == SMFGPU_PREC_FP32 || opt.precision == SMFGPU_PREC_BF16) {
+    const int8_t code = static_cast<int8_t>(prec_code(opt.precision));
+    if (code > 0) {
         for (int32_t s = 0; s < P->nsup; ++s)
             if (P->fronts[s].f >= opt.mixed_min_front) {
-                F->prec[s] = 1;  // BF16 requests run as FP32 until the BF16 tile kernel exists
+                F->prec[s] = code;
                 ++reduced;
             }
+    }
+    const int gi = reduced > 0 ? 1 : 0;
+    if (reduced > 0 && P->sched_red_min != opt.mixed_min_front) {
+        // the Schur launches put the items of fronts that may run reduced last
+        P->sched_red_min = std::max(opt.mixed_min_front, 1);
+        const int rc = dense_schedule(P);
+        if (rc) return rc;
     }
     if (reduced > 0 && P->d_L32.p == nullptr) {
         // FP32 panel copies (the reduced fronts' Schur operands), once per plan
@@ -783,7 +823,7 @@ SMFGPU_API int smfgpu_factor(smfgpu_plan_t P, const double *values, int64_t nval
             }
             case K_POTRF: launch_potrf(st, d, P->d_potrf.p + l.first, l.count); break;
             case K_TRSM: launch_trsm(st, d, P->d_trsm.p + l.first, l.count); break;
-            case K_SYRK: launch_syrk(st, d, P->d_syrk.p + l.first, l.count, P->syrk_ahead); break;
+            case K_SYRK: launch_syrk(st, d, P->d_syrk.p + l.first, l.count, gi ? l.red : 0, P->syrk_ahead); break;
             default: break;
             }
             if (rec) mark(2 * li + 1);
@@ -795,7 +835,7 @@ SMFGPU_API int smfgpu_factor(smfgpu_plan_t P, const double *values, int64_t nval
     if (timing || nvalues == 0) {
         HIP_OK(enqueue(timing));
     } else {
-        if (!P->fexec) {
+        if (!P->fexec[gi]) {
             HIP_OK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             const hipError_t e1 = enqueue(false);
             hipGraph_t g = nullptr;
@@ -804,13 +844,13 @@ SMFGPU_API int smfgpu_factor(smfgpu_plan_t P, const double *values, int64_t nval
                 if (g) (void)hipGraphDestroy(g);
                 return set_error(SMFGPU_ERR_DEVICE, "factor: capture of the launch sequence failed");
             }
-            P->fgraph = g;
-            if (hipGraphInstantiate(&P->fexec, g, nullptr, nullptr, 0) != hipSuccess) {
+            P->fgraph[gi] = g;
+            if (hipGraphInstantiate(&P->fexec[gi], g, nullptr, nullptr, 0) != hipSuccess) {
                 release_graph(P);
                 return set_error(SMFGPU_ERR_DEVICE, "factor: graph instantiation failed");
             }
         }
-        HIP_OK(hipGraphLaunch(P->fexec, st));
+        HIP_OK(hipGraphLaunch(P->fexec[gi], st));
     }
     int32_t flag = 0;
     unsigned long long maxbits = 0;
@@ -870,6 +910,7 @@ SMFGPU_API int smfgpu_factor(smfgpu_plan_t P, const double *values, int64_t nval
     F->valid = fi.status == SMFGPU_STATUS_SUCCESS;
     if (P->live) P->live->valid = false;
     P->live = F.get();
+    P->issued.insert(F.get());
     P->max_abs = max_abs;
     if (info) *info = fi;
     *out = F.release();
@@ -878,13 +919,17 @@ SMFGPU_API int smfgpu_factor(smfgpu_plan_t P, const double *values, int64_t nval
 
 SMFGPU_API void smfgpu_factor_free(smfgpu_factor_t *f) {
     if (!f || !*f) return;
-    if ((*f)->plan && (*f)->plan->live == *f) (*f)->plan->live = nullptr;
+    if ((*f)->plan) {
+        if ((*f)->plan->live == *f) (*f)->plan->live = nullptr;
+        (*f)->plan->issued.erase(*f);
+    }
     delete *f;
     *f = nullptr;
 }
 
 SMFGPU_API int smfgpu_factor_front_stats(smfgpu_factor_t f, int32_t *out) {
-    if (!f || !f->plan || !out) return set_error(SMFGPU_ERR_INVALID_ARG, "null argument");
+    if (!f || !out) return set_error(SMFGPU_ERR_INVALID_ARG, "null argument");
+    if (!f->plan) return set_error(SMFGPU_ERR_STATE, "front stats: the factor's plan was freed");
     smfgpu_plan_s *P = f->plan;
     if (P->live != f) return set_error(SMFGPU_ERR_STATE, "front stats: factor was superseded by a newer factorization");
     for (int32_t s = 0; s < P->nsup; ++s) {
@@ -903,7 +948,8 @@ SMFGPU_API int smfgpu_factor_front_stats(smfgpu_factor_t f, int32_t *out) {
 
 SMFGPU_API int smfgpu_export_factor(smfgpu_factor_t f, int64_t *sizes, int64_t *steps, double *factor_values,
                                     int32_t *row_indices, int8_t *pivot_types) {
-    if (!f || !f->plan) return set_error(SMFGPU_ERR_INVALID_ARG, "null factor");
+    if (!f) return set_error(SMFGPU_ERR_INVALID_ARG, "null factor");
+    if (!f->plan) return set_error(SMFGPU_ERR_STATE, "export: the factor's plan was freed");
     smfgpu_plan_s *P = f->plan;
     if (P->live != f) return set_error(SMFGPU_ERR_STATE, "export: factor was superseded by a newer factorization");
     if (!P->posdef) return ldlt_export(P, sizes, steps, factor_values, row_indices, pivot_types);
@@ -937,9 +983,12 @@ SMFGPU_API int smfgpu_export_factor(smfgpu_factor_t f, int64_t *sizes, int64_t *
     return SMFGPU_OK;
 }
 
+}  // extern "C"
+
+namespace smfgpu {
 // job as ldlt_solve; for a Cholesky factor D = I (the diagonal job copies)
-static int solve_device(smfgpu_plan_s *P, const double *b_dev, int64_t ldb, double *out_dev, int64_t ldo,
-                        int32_t nrhs, int32_t job = 0) {
+int solve_device(smfgpu_plan_s *P, const double *b_dev, int64_t ldb, double *out_dev, int64_t ldo, int32_t nrhs,
+                 int32_t job) {
     if (!P->posdef) return ldlt_solve(P, b_dev, ldb, out_dev, ldo, nrhs, job);
     hipStream_t st = P->stream;
     const int32_t n = P->n;
@@ -997,8 +1046,13 @@ static int solve_device(smfgpu_plan_s *P, const double *b_dev, int64_t ldb, doub
     return SMFGPU_OK;
 }
 
+}  // namespace smfgpu
+
+extern "C" {
+
 SMFGPU_API int smfgpu_solve(smfgpu_factor_t f, double *b, int32_t nrhs, int32_t ldb, int32_t job, int32_t mem) {
-    if (!f || !f->plan || !b) return set_error(SMFGPU_ERR_INVALID_ARG, "null argument");
+    if (!f || !b) return set_error(SMFGPU_ERR_INVALID_ARG, "null argument");
+    if (!f->plan) return set_error(SMFGPU_ERR_STATE, "solve: the factor's plan was freed");
     smfgpu_plan_s *P = f->plan;
     if (P->live != f || !f->valid) return set_error(SMFGPU_ERR_STATE, "solve: factor not usable");
     if (job < 0 || job > 4) return set_error(SMFGPU_ERR_INVALID_ARG, "solve: job must be 0..4");
@@ -1037,85 +1091,6 @@ SMFGPU_API int smfgpu_solve(smfgpu_factor_t f, double *b, int32_t nrhs, int32_t 
         (void)hipEventDestroy(e1);
     }
     return rc;
-}
-
-SMFGPU_API int smfgpu_refine(smfgpu_factor_t f, const double *values, const double *b, double *x, int32_t nrhs,
-                             int32_t max_iters, double tol, int32_t mem, double *out) {
-    if (!f || !f->plan || !b || !x) return set_error(SMFGPU_ERR_INVALID_ARG, "null argument");
-    smfgpu_plan_s *P = f->plan;
-    if (P->live != f || !f->valid) return set_error(SMFGPU_ERR_STATE, "refine: factor not usable");
-    if (nrhs <= 0) return set_error(SMFGPU_ERR_INVALID_ARG, "refine: bad nrhs");
-    (void)values;  // the factorization's permuted values are already on the device
-    hipStream_t st = P->stream;
-    const int32_t n = P->n;
-    const size_t nn = static_cast<size_t>(n) * nrhs;
-    int rc;
-    if (P->d_x.n < nn && (rc = P->d_x.alloc(nn))) return rc;
-    if (P->d_bb.n < nn && (rc = P->d_bb.alloc(nn))) return rc;
-    if (P->d_r.n < nn && (rc = P->d_r.alloc(nn))) return rc;
-    if (P->d_b.n < nn && (rc = P->d_b.alloc(nn))) return rc;
-    if (P->d_rowstat.n < 4 * nn && (rc = P->d_rowstat.alloc(4 * nn))) return rc;
-    const hipMemcpyKind kin = mem == SMFGPU_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
-    const hipMemcpyKind kout = mem == SMFGPU_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
-    // permuted copies of b and x (the residual runs in the permuted numbering)
-    HIP_OK(hipMemcpyAsync(P->d_b.p, b, nn * sizeof(double), kin, st));
-    launch_gather_rhs(st, P->d_perm.p, P->d_b.p, n, P->d_bb.p, n, nrhs);
-    HIP_OK(hipMemcpyAsync(P->d_b.p, x, nn * sizeof(double), kin, st));
-    launch_gather_rhs(st, P->d_perm.p, P->d_b.p, n, P->d_x.p, n, nrhs);
-    // ||A||_inf = max row sum of |A|
-    launch_rowabs(st, n, P->d_fr_ptr.p, P->d_fr_slot.p, P->d_av.p, P->d_r.p);
-    launch_reduce_max(st, P->d_r.p, n, P->d_red.p + 1);
-    double normA = 0.0;
-    HIP_OK(hipMemcpyAsync(&normA, P->d_red.p + 1, sizeof(double), hipMemcpyDeviceToHost, st));
-    std::vector<double> rs(4 * nn);
-    auto eval = [&](double &eta, double &omega) -> int {
-        launch_residual(st, n, P->d_fr_ptr.p, P->d_fr_col.p, P->d_fr_slot.p, P->d_av.p, P->d_x.p, P->d_bb.p,
-                        P->d_r.p, P->d_rowstat.p, nrhs);
-        HIP_OK(hipMemcpyAsync(rs.data(), P->d_rowstat.p, rs.size() * sizeof(double), hipMemcpyDeviceToHost, st));
-        HIP_OK(hipStreamSynchronize(st));
-        eta = 0;
-        omega = 0;
-        for (int32_t c = 0; c < nrhs; ++c) {
-            double rmax = 0, xmax = 0, bmax = 0, om = 0;
-            const double *r = rs.data() + 4 * static_cast<size_t>(c) * n;
-            for (int32_t i = 0; i < n; ++i) {
-                rmax = std::max(rmax, r[4 * i]);
-                xmax = std::max(xmax, r[4 * i + 2]);
-                bmax = std::max(bmax, r[4 * i + 3]);
-                const double den = r[4 * i + 1];
-                if (den > 0) om = std::max(om, r[4 * i] / den);
-                else if (r[4 * i] != 0) om = INFINITY;
-            }
-            const double den = normA * xmax + bmax;
-            eta = std::max(eta, den > 0 ? rmax / den : (rmax == 0 ? 0.0 : INFINITY));
-            omega = std::max(omega, om);
-        }
-        return SMFGPU_OK;
-    };
-    double eta = 0, omega = 0;
-    int steps = 0;
-    if ((rc = eval(eta, omega))) return rc;
-    while (steps < max_iters && eta > tol) {
-        // correction d = A^{-1} r: r is in the permuted numbering, the solve
-        // takes the original one
-        launch_scatter_rhs(st, P->d_perm.p, P->d_r.p, P->d_b.p, n, n, nrhs);
-        if ((rc = solve_device(P, P->d_b.p, n, P->d_b.p, n, nrhs))) return rc;
-        launch_gather_rhs(st, P->d_perm.p, P->d_b.p, n, P->d_r.p, n, nrhs);
-        launch_axpy(st, static_cast<int64_t>(nn), P->d_r.p, P->d_x.p);
-        ++steps;
-        const double eta_prev = eta;
-        if ((rc = eval(eta, omega))) return rc;
-        if (!(eta < 0.5 * eta_prev)) break;  // stalled
-    }
-    launch_scatter_rhs(st, P->d_perm.p, P->d_x.p, P->d_b.p, n, n, nrhs);
-    HIP_OK(hipMemcpyAsync(x, P->d_b.p, nn * sizeof(double), kout, st));
-    HIP_OK(hipStreamSynchronize(st));
-    if (out) {
-        out[0] = eta;
-        out[1] = omega;
-        out[2] = steps;
-    }
-    return SMFGPU_OK;
 }
 
 }  // extern "C"

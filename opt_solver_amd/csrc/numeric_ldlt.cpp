@@ -272,7 +272,8 @@ static int ldlt_big_fronts(smfgpu_plan_s *P, const LdltDev &d, const std::vector
     const bool group = P->ldlt_group != 0 && nb == 16;
     h.assign(nbig, BigLd{});
     int64_t cap = 64;
-    int32_t n32 = 0;
+    int32_t nmode[4] = {0, 0, 0, 0};
+    const int code = prec_code(opt.precision);
     for (int32_t b = 0; b < nbig; ++b) {
         const FrontLd &F = P->fl[big[b]];
         BigLd &B = h[b];
@@ -281,17 +282,13 @@ static int ldlt_big_fronts(smfgpu_plan_s *P, const LdltDev &d, const std::vector
         B.k_paired = -1;
         B.oe = std::min(F.ne, kBigOuter);
         B.spec = group ? 1 : 0;
-        // mixed precision: FP32 MFMA updates on the fronts of at least
-        // mixed_min_front rows (the pivot logic stays FP64)
-        B.prec = (opt.precision == SMFGPU_PREC_FP32 || opt.precision == SMFGPU_PREC_BF16) &&
-                         F.f >= opt.mixed_min_front
-                     ? 1
-                     : 0;
-        n32 += B.prec;
+        // mixed precision: reduced-precision MFMA updates on the fronts of
+        // at least mixed_min_front rows (the pivot logic stays FP64)
+        B.prec = code > 0 && F.f >= opt.mixed_min_front ? code : 0;
+        ++nmode[B.prec];
         cap += 4 * (static_cast<int64_t>(F.ne) / std::max(nb, 1) + 2) + 2 * static_cast<int64_t>(F.ne);
     }
-    reduced += n32;
-    const int32_t n64 = nbig - n32;
+    reduced += nbig - nmode[0];
     int rc;
     if ((rc = ensure(P->d_big, static_cast<size_t>(nbig), false, "large-front states"))) return rc;
     HIP_OK(hipMemcpyAsync(P->d_big.p, h.data(), nbig * sizeof(BigLd), hipMemcpyHostToDevice, st));
@@ -309,7 +306,7 @@ static int ldlt_big_fronts(smfgpu_plan_s *P, const LdltDev &d, const std::vector
         if (group) launch_big_group_stats(st, d, P->d_big.p, nbig, gpart, nch);
         launch_big_panel(st, d, P->d_big.p, nbig, opt.pivot_u, eff_small, small, nb, gpart, nch);
         if (group) launch_big_group_apply(st, d, P->d_big.p, nbig, gpart, nch);
-        launch_big_update(st, d, P->d_big.p, nbig, per_front, n64, n32);
+        launch_big_update(st, d, P->d_big.p, nbig, per_front, nmode);
         launch_big_stall(st, d, P->d_big.p, nbig, nb);
         if (it % kBigCheck == kBigCheck - 1) {
             HIP_OK(hipMemcpyAsync(h.data(), P->d_big.p, nbig * sizeof(BigLd), hipMemcpyDeviceToHost, st));
@@ -323,7 +320,7 @@ static int ldlt_big_fronts(smfgpu_plan_s *P, const LdltDev &d, const std::vector
     HIP_OK(hipGetLastError());
     int32_t pmax = 0;
     for (const BigLd &B : h) pmax = std::max(pmax, B.k);
-    for (int32_t c0 = 0; c0 < pmax; c0 += kBigWork) launch_big_cb(st, d, P->d_big.p, nbig, c0, kBigWork, per_front, n64, n32);
+    for (int32_t c0 = 0; c0 < pmax; c0 += kBigWork) launch_big_cb(st, d, P->d_big.p, nbig, c0, kBigWork, per_front, nmode);
     launch_big_finish(st, d, P->d_big.p, nbig);
     HIP_OK(hipGetLastError());
     return SMFGPU_OK;

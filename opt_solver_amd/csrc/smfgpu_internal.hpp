@@ -96,6 +96,7 @@ struct Launch {
     int32_t level;
     int64_t first;  // first item in the kind's item array
     int64_t count;  // number of items (= workgroups)
+    int64_t red;    // K_SYRK: the last `red` items belong to fronts that may run below FP64
 };
 
 // ---------------------------------------------------------------------
@@ -109,10 +110,10 @@ struct DevPtrs {
     const int32_t *amap;    // per permuted entry: local row in its column's front
     const double *av;       // permuted values
     double *L;
-    float *L32;             // FP32 copy of the panel columns of FP32-Schur fronts (null: none)
+    float *L32;             // FP32 copy of the panel columns of reduced-precision fronts (null: none)
     double *CB;
     int32_t *flag;          // [0]: smallest failing supernode + 1 (SPD); INT32_MAX when none
-    const int8_t *prec;     // per supernode: Schur-update precision (null: all FP64)
+    const int8_t *prec;     // per supernode: Schur-update precision 0-3 (null: all FP64)
     double *junk;           // kThreads doubles: target of masked-out stores (contents meaningless)
 };
 
@@ -123,7 +124,8 @@ void launch_assemble(hipStream_t st, const DevPtrs &d, const AsmTask *tasks, int
                      int32_t fmax);
 void launch_potrf(hipStream_t st, const DevPtrs &d, const PotrfItem *it, int64_t n);
 void launch_trsm(hipStream_t st, const DevPtrs &d, const TrsmItem *it, int64_t n);
-void launch_syrk(hipStream_t st, const DevPtrs &d, const SyrkItem *it, int64_t n, int ahead);
+// n items; the reduced-precision kernel runs the last n_red of them
+void launch_syrk(hipStream_t st, const DevPtrs &d, const SyrkItem *it, int64_t n, int64_t n_red, int ahead);
 void launch_zero_upper(hipStream_t st, const DevPtrs &d, const PotrfItem *it, int64_t n);
 
 // Solves: y is the permuted right-hand side block, n x nrhs (ld n).
@@ -140,12 +142,28 @@ void launch_bwd_step(hipStream_t st, const DevPtrs &d, const SolveItem *items, i
                      int32_t n, int32_t nrhs);
 
 // Refinement: full symmetric CSR in permuted numbering (slot into av).
+// r = b - A x (double-double) and per column c the maxima stat[4c..4c+3] =
+// max |r_i|, max_i |r_i| / ((|A||x|)_i + |b_i|), max |x_i|, max |b_i|
+// (x, b, r: n x nrhs, ld n; stat: 4 nrhs doubles, cleared by the launch).
 void launch_residual(hipStream_t st, int32_t n, const int64_t *rp, const int32_t *ci,
                      const int64_t *slot, const double *av, const double *x, const double *b,
-                     double *r, double *rowstat, int32_t nrhs);
+                     double *r, double *stat, int32_t nrhs);
+void launch_spmv(hipStream_t st, int32_t n, const int64_t *rp, const int32_t *ci, const int64_t *slot,
+                 const double *av, const double *z, double *w);
 void launch_rowabs(hipStream_t st, int32_t n, const int64_t *rp, const int64_t *slot, const double *av,
                    double *out);
-void launch_axpy(hipStream_t st, int64_t n, const double *d, double *x);
+// x = y + d
+void launch_axpy(hipStream_t st, int64_t n, const double *y, const double *d, double *x);
+// FGMRES building blocks (vectors of length n, columns ld n, scalars on
+// the device). dots: out[i] = V_i . w for nvec columns (mode 1: out[i] +=,
+// mode 2: out[i] = sqrt); part: kDotParts * nvec doubles of scratch.
+constexpr int kDotParts = 256;
+void launch_dots(hipStream_t st, int64_t n, const double *V, int32_t nvec, const double *w, double *part,
+                 double *out, int32_t mode);
+void launch_vsub(hipStream_t st, int64_t n, const double *V, int32_t nvec, const double *h, double *w);
+void launch_vdiv(hipStream_t st, int64_t n, const double *w, const double *d, double *out);
+void launch_lincomb(hipStream_t st, int64_t n, const double *x0, const double *Z, int32_t nvec, const double *y,
+                    double *out);
 void launch_reduce_max(hipStream_t st, const double *v, int64_t n, double *out);
 
 }  // namespace smfgpu

@@ -77,7 +77,7 @@ struct BigLd {
     int32_t cnt[3];
     int32_t spec;               // 1: the next first-pass iteration tries a speculative group
     int32_t g_apply, g_ps, g_n; // accepted group to write: columns [g_ps, g_ps + g_n)
-    int32_t prec;               // 1: the MFMA updates run in FP32 (mixed precision), 0: FP64
+    int32_t prec;               // precision code of the MFMA updates: 0 FP64, 1 FP32, 2 BF16, 3 split BF16
     double flops;
 };
 constexpr int32_t kGroupCols = 64;   // columns of a speculative group (four panels of 16)
@@ -156,14 +156,15 @@ void launch_big_group_stats(hipStream_t st, const LdltDev &d, const BigLd *state
                             int32_t nch);
 void launch_big_group_apply(hipStream_t st, const LdltDev &d, const BigLd *states, int32_t nbig, double *gpart,
                             int32_t nch);
-// n64 / n32: fronts whose updates run in FP64 / FP32 (a launch per precision in use)
+// nmode[m]: fronts of the level whose updates run in precision code m
+// (0 FP64, 1 FP32, 2 BF16, 3 split BF16); a launch per code in use
 void launch_big_update(hipStream_t st, const LdltDev &d, const BigLd *states, int32_t nbig, int32_t per_front,
-                       int32_t n64, int32_t n32);
+                       const int32_t *nmode);
 void launch_big_stall(hipStream_t st, const LdltDev &d, BigLd *states, int32_t nbig, int32_t nb);
 // final contribution-block update with pivots [c0, c0 + width) of every
 // front (L*D of the chunk into W first); then the fronts' results
 void launch_big_cb(hipStream_t st, const LdltDev &d, const BigLd *states, int32_t nbig, int32_t c0, int32_t width,
-                   int32_t per_front, int32_t n64, int32_t n32);
+                   int32_t per_front, const int32_t *nmode);
 void launch_big_finish(hipStream_t st, const LdltDev &d, const BigLd *states, int32_t nbig);
 
 }  // namespace smfgpu

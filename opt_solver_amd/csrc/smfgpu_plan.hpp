@@ -5,7 +5,9 @@
 #include "smfgpu_internal.hpp"
 #include "smfgpu_ldlt.hpp"
 
+#include <climits>
 #include <map>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -189,9 +191,11 @@ struct smfgpu_plan_s {
     int32_t posdef = 1;
     hipStream_t stream = nullptr;
     // SPD: the factorization's launch sequence, captured once and replayed
-    // (it reads the values from d_vals)
-    hipGraph_t fgraph = nullptr;
-    hipGraphExec_t fexec = nullptr;
+    // (it reads the values from d_vals); [0]: every front in FP64, [1]:
+    // with the reduced-precision Schur launches
+    hipGraph_t fgraph[2] = {nullptr, nullptr};
+    hipGraphExec_t fexec[2] = {nullptr, nullptr};
+    int32_t sched_red_min = INT32_MAX;             // SPD schedule: fronts with f >= this may run below FP64
     std::vector<FrontDev> fronts;
     std::vector<int32_t> level;                    // per supernode
     std::vector<std::vector<int32_t>> level_fronts;
@@ -238,6 +242,12 @@ struct smfgpu_plan_s {
     DBuf<int32_t> d_fr_col;
     // solve / refine work
     DBuf<double> d_y, d_b, d_r, d_rowstat, d_x, d_bb;
+    // refinement: vectors of the iterations, per-column maxima, dot-product
+    // scratch, values of a matrix other than the factored one, and
+    // ||A||_inf of the factored values (valid until the next factorization)
+    DBuf<double> d_rvec, d_stat, d_part, d_h, d_av_ref;
+    double norm_a = 0.0;
+    bool norm_a_valid = false;
     std::vector<int64_t> syrk_flops_of_launch;
     // timing
     bool timing = false;
@@ -246,6 +256,7 @@ struct smfgpu_plan_s {
     std::vector<hipEvent_t> ev, sev;
     int8_t last_prec_any = 0;
     smfgpu_factor_s *live = nullptr;
+    std::set<smfgpu_factor_s *> issued;             // every factor handle of this plan not yet freed
     double max_abs = 0.0;
     // ---- LDL^T (posdef = 0) ----
     std::vector<int32_t> sn_parent, child_ptr, child_idx, front_row_ptr, front_row_idx;
@@ -284,6 +295,9 @@ struct smfgpu_factor_s {
 
 
 namespace smfgpu {
+// device precision code of a SMFGPU_PREC_* request (0 FP64, 1 FP32, 2 BF16,
+// 3 split BF16), -1 when the request cannot be served
+int prec_code(int32_t precision);
 int ldlt_plan_build(smfgpu_plan_s *P, const smfgpu_analysis_desc *D);
 int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgpu_factor_options &opt,
                 smfgpu_factor_info &fi);
@@ -293,4 +307,8 @@ int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgp
 int ldlt_solve(smfgpu_plan_s *P, const double *b_dev, int64_t ldb, double *out_dev, int64_t ldo, int32_t nrhs,
                int32_t job);
 int ldlt_export(smfgpu_plan_s *P, int64_t *sizes, int64_t *steps, double *fv, int32_t *rows, int8_t *tags);
+// solve on device buffers (b, out: n x nrhs, leading dimensions ldb / ldo;
+// may alias), job as ldlt_solve, for either factor kind
+int solve_device(smfgpu_plan_s *P, const double *b_dev, int64_t ldb, double *out_dev, int64_t ldo, int32_t nrhs,
+                 int32_t job = 0);
 }  // namespace smfgpu

@@ -14,12 +14,14 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import MEM_DEVICE, MEM_HOST, PREC_BF16, PREC_FP32, PREC_FP64, SmfGpuError, check, lib
+from ._lib import (MEM_DEVICE, MEM_HOST, PREC_AUTO, PREC_BF16, PREC_BF16X3, PREC_FP32, PREC_FP64, SmfGpuError,
+                   check, lib)
 
 # smf::SolveJob (solver/include/smf/types.hpp:250-256)
 SOLVE_FULL, SOLVE_FORWARD, SOLVE_DIAG, SOLVE_BACKWARD, SOLVE_DIAG_BACK = 0, 1, 2, 3, 4
 
-__all__ = ["analyse", "Analysis", "Plan", "Factor", "SmfGpuError", "PREC_FP64", "PREC_FP32", "PREC_BF16"]
+__all__ = ["analyse", "Analysis", "Plan", "Factor", "SmfGpuError", "PREC_FP64", "PREC_FP32", "PREC_BF16",
+           "PREC_BF16X3", "PREC_AUTO"]
 
 
 def _p(a):
@@ -210,19 +212,54 @@ class Factor:
         check(lib().smfgpu_solve(self.h, ctypes.c_void_p(ptr), nrhs, ldb or self.plan.n, int(job), MEM_DEVICE),
               "solve")
 
-    def refine(self, b, x, max_iters=10, tol=1e-14):
+    def refine(self, b, x, max_iters=10, tol=1e-14, method=None, max_steps=None, omega_target=None,
+               restart=20, values=None):
+        """Refine x against A (the factored values, or `values`): smf's
+        refine with Control::refine_method = method (default Auto),
+        refinement_max_iters = max_iters, refine_max_steps = max_steps
+        (default max_iters), targets eta_inf <= tol and omega <= omega_target
+        (default: none). Returns (x, info)."""
         b = np.array(b, dtype=np.float64, order="F", copy=True)
         x = np.array(x, dtype=np.float64, order="F", copy=True)
         nrhs = 1 if b.ndim == 1 else b.shape[1]
-        out = np.zeros(3)
-        check(lib().smfgpu_refine(self.h, None, _p(b), _p(x), nrhs, max_iters, tol, MEM_HOST, _p(out)), "refine")
-        return x, dict(eta=float(out[0]), omega=float(out[1]), steps=int(out[2]))
+        o = self._refine_opts(max_iters, tol, method, max_steps, omega_target, restart)
+        keep = None
+        if values is not None:
+            keep = np.ascontiguousarray(values, np.float64)
+            o.value_mem = MEM_HOST
+        inf = _lib.RefineInfo()
+        n = self.plan.n
+        check(lib().smfgpu_refine_ex(self.h, _p(keep), _p(b), n, _p(x), n, nrhs, ctypes.byref(o), MEM_HOST,
+                                     ctypes.byref(inf)), "refine")
+        return x, self._refine_dict(inf)
 
-    def refine_device(self, b_ptr, x_ptr, nrhs, max_iters=10, tol=1e-14):
-        out = np.zeros(3)
-        check(lib().smfgpu_refine(self.h, None, ctypes.c_void_p(b_ptr), ctypes.c_void_p(x_ptr), nrhs, max_iters,
-                                  tol, MEM_DEVICE, _p(out)), "refine")
-        return dict(eta=float(out[0]), omega=float(out[1]), steps=int(out[2]))
+    def refine_device(self, b_ptr, x_ptr, nrhs, max_iters=10, tol=1e-14, method=None, max_steps=None,
+                      omega_target=None, restart=20):
+        o = self._refine_opts(max_iters, tol, method, max_steps, omega_target, restart)
+        inf = _lib.RefineInfo()
+        n = self.plan.n
+        check(lib().smfgpu_refine_ex(self.h, None, ctypes.c_void_p(b_ptr), n, ctypes.c_void_p(x_ptr), n, nrhs,
+                                     ctypes.byref(o), MEM_DEVICE, ctypes.byref(inf)), "refine")
+        return self._refine_dict(inf)
+
+    @staticmethod
+    def _refine_opts(max_iters, tol, method, max_steps, omega_target, restart):
+        o = _lib.RefineOptions()
+        lib().smfgpu_refine_default_options(ctypes.byref(o))
+        o.method = _lib.REFINE_AUTO if method is None else int(method)
+        o.max_iters = int(max_iters)
+        o.max_steps = int(max_iters if max_steps is None else max_steps)
+        o.restart = int(restart)
+        o.eta_target = float(tol) if tol > 0 else 1e-300
+        o.omega_target = 1.7e308 if omega_target is None else float(omega_target)
+        return o
+
+    @staticmethod
+    def _refine_dict(inf):
+        return dict(eta=float(inf.eta), omega=float(inf.omega), eta_start=float(inf.eta_start),
+                    omega_start=float(inf.omega_start), steps=int(inf.steps),
+                    method="fgmres" if inf.method_used == _lib.REFINE_FGMRES else "classical",
+                    reached=bool(inf.reached), stalled=bool(inf.stalled))
 
     def export(self):
         """The FactorKeep layout: steps (row_offset, factor_offset, p, f per

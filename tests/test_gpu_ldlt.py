@@ -231,24 +231,24 @@ def test_blocked_solve_multi_rhs(oracle_lib, case):
                 assert np.abs(fac.solve(B[:, r]) - XS[:, r]).max() <= 1e-12 * max(1.0, np.abs(XS[:, r]).max())
 
 
+@pytest.mark.parametrize("prec", [1, 4, 2])
 @pytest.mark.parametrize("case", ["kkt_grid_60", "kkt_48"])
-def test_mixed_precision_ldlt(oracle_lib, case):
-    """FP32 MFMA updates on every large front (the pivot logic in FP64), then
-    FP64 refinement against A: scaled backward error <= 1e-12 (the mixed
-    gate) and the Sylvester inertia."""
+def test_mixed_precision_ldlt(oracle_lib, case, prec):
+    """FP32 / split-BF16 / BF16 MFMA updates on every large front (the pivot
+    logic in FP64), then FP64 refinement against A: scaled backward error
+    <= 1e-12 (the mixed gate) and the Sylvester inertia."""
     from opt_solver_amd._lib import TUNE_LDLT_BIG_MIN
-    from opt_solver_amd.solver import PREC_FP32
-    print("case mixed ldlt", case, flush=True)
+    print("case mixed ldlt", case, "prec", prec, flush=True)
     A, arrays, inertia = _large_path_case(case)
     plan = Plan(arrays, posdef=False)
     plan.tune(TUNE_LDLT_BIG_MIN, 96)
-    fac = plan.factor(A[3], u=0.01, precision=PREC_FP32, mixed_min_front=0)
+    fac = plan.factor(A[3], u=0.01, precision=prec, mixed_min_front=0)
     assert fac.info.fronts_reduced > 0
     assert fac.inertia == inertia
     x = np.random.default_rng(13).uniform(-1, 1, A[0])
     b = matvec(A, x)
     x0 = fac.solve(b)
-    xr, st = fac.refine(b, x0, max_iters=20, tol=1e-14)
+    xr, st = fac.refine(b, x0, max_iters=30, tol=1e-14)
     eta, _ = oracle_lib.backward_error(A, b, xr)
     print(f"  reduced {fac.info.fronts_reduced} refine steps {st['steps']} eta {eta:.2e}", flush=True)
     assert eta <= 1e-12, eta

@@ -16,7 +16,7 @@ if not gpu_available():
 from opt_solver_amd import generators as g  # noqa: E402
 from opt_solver_amd import ordering as o  # noqa: E402
 from opt_solver_amd._lib import TUNE_CB_SEG, TUNE_PANEL_OUTER, TUNE_SOLVE_FUSE_ROWS  # noqa: E402
-from opt_solver_amd.solver import PREC_FP32, Plan, SmfGpuError, analyse  # noqa: E402
+from opt_solver_amd.solver import PREC_BF16, PREC_BF16X3, PREC_FP32, Plan, SmfGpuError, analyse  # noqa: E402
 
 
 def block_rel_err(e1, e2):
@@ -118,24 +118,34 @@ def test_large_front_stepped_solve(oracle_lib):
             assert np.abs(XS - X[:, :nr]).max() < 1e-10
 
 
-def test_mixed_precision_refinement(oracle_lib):
-    """FP32 Schur updates on every front, FP64 refinement against A: scaled
-    backward error <= 1e-12 (the mixed-precision gate), inertia unchanged."""
+@pytest.mark.parametrize("prec", [PREC_FP32, PREC_BF16X3, PREC_BF16])
+def test_mixed_precision_refinement(oracle_lib, prec):
+    """Reduced-precision Schur updates on every front (FP32, split BF16, or
+    BF16 MFMA, FP32 accumulation), FP64 refinement against A: scaled
+    backward error <= 1e-12 (the mixed-precision gate), inertia unchanged,
+    and the factor close to the FP64 one by the precision's unit roundoff."""
     m = 16
-    print("case mixed spd27_16", flush=True)
+    print("case mixed spd27_16 prec", prec, flush=True)
     A = g.spd27_3d(m, seed=5)
     an = analyse(A, perm=o.grid_nd((m, m, m), leaf=32))
     plan = Plan(an, posdef=True)
-    fac = plan.factor(A[3], precision=PREC_FP32, mixed_min_front=0)
+    ref = plan.factor(A[3]).export()
+    fac = plan.factor(A[3], precision=prec, mixed_min_front=0)
     assert fac.status == 0 and fac.inertia == (A[0], 0, 0)
     assert fac.info.fronts_reduced == an.nsup
+    err = block_rel_err(fac.export(), ref)
+    # FP32: 2^-24 per operand rounding; split BF16: about 2^-16; BF16: 2^-9
+    lo, hi = {PREC_FP32: (1e-9, 1e-5), PREC_BF16X3: (1e-7, 1e-3), PREC_BF16: (1e-4, 0.2)}[prec]
+    print(f"  factor rel diff vs FP64 {err:.2e}", flush=True)
+    assert lo < err < hi
     x = np.random.default_rng(2).uniform(-1, 1, A[0])
     b = matvec(A, x)
     x0 = fac.solve(b)
     eta0, _ = oracle_lib.backward_error(A, b, x0)
-    xr, st = fac.refine(b, x0, max_iters=10, tol=1e-14)
+    xr, st = fac.refine(b, x0, max_iters=30, tol=1e-14)
     eta, om = oracle_lib.backward_error(A, b, xr)
-    assert eta0 > 1e-12  # the FP32 Schur updates are visible before refinement
+    print(f"  eta0 {eta0:.2e} refined {eta:.2e} in {st['steps']} steps", flush=True)
+    assert eta0 > 1e-12  # the reduced Schur updates are visible before refinement
     assert eta <= 1e-12 and st["steps"] >= 1
     assert abs(st["eta"] - eta) <= 0.5 * eta + 1e-18  # device residual agrees with the oracle formula
 
