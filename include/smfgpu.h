@@ -223,6 +223,20 @@ SMFGPU_API int smfgpu_factor_front_stats(smfgpu_factor_t f, int32_t *out);
 SMFGPU_API int smfgpu_solve(smfgpu_factor_t f, double *b, int32_t nrhs, int32_t ldb,
                             int32_t job, int32_t mem);
 
+/* smfgpu_solve with b in device memory that allocates nothing: no
+ * counterpart in the reference. It is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). The right-hand-side work block is a
+ * fixed 64 MiB device array per GPU (resolved when a plan is created on that
+ * device, held until the process ends) instead of a plan buffer allocated on
+ * first use; right-hand sides pass through the sweeps in groups of at most
+ * min(4 (Cholesky) / 16 (LDL^T), 2^23 / n) columns. Plans with n > 2^23
+ * rows are refused with SMFGPU_ERR_INVALID_ARG. Every call on one GPU
+ * shares that array: calls that overlap on one GPU (from several threads or
+ * plans) race. Returns after the solve has finished. */
+SMFGPU_API int smfgpu_solve_noalloc(smfgpu_factor_t f, double *b, int32_t nrhs, int32_t ldb,
+                                    int32_t job);
+
 /* ---- FP64 iterative refinement against the analysed matrix (refine /
  * refine_core, solver/src/refinement.cpp:300-724). values: null for the
  * factored values, else values of the analysed pattern (one per entry, as

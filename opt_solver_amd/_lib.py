@@ -76,6 +76,7 @@ EXPORTED = [
     "smfgpu_plan_stats", "smfgpu_factor", "smfgpu_factor_free", "smfgpu_export_factor", "smfgpu_solve",
     "smfgpu_refine", "smfgpu_set_timing", "smfgpu_kernel_times", "smfgpu_plan_tune",
     "smfgpu_factor_front_stats", "smfgpu_launch_profile", "smfgpu_refine_default_options", "smfgpu_refine_ex",
+    "smfgpu_solve_noalloc",
 ]
 STATIC_AS_ZERO = 1
 STATIC_ZERO_STAYS_ZERO = 2
@@ -123,6 +124,7 @@ def lib():
     L.smfgpu_factor_free.argtypes = [_P]
     L.smfgpu_export_factor.argtypes = [_P, _P, _P, _P, _P, _P]
     L.smfgpu_solve.argtypes = [_P, _P, _I, _I, _I, _I]
+    L.smfgpu_solve_noalloc.argtypes = [_P, _P, _I, _I, _I]
     L.smfgpu_refine.argtypes = [_P, _P, _P, _P, _I, _I, _D, _I, _P]
     L.smfgpu_refine_default_options.argtypes = [_P]
     L.smfgpu_refine_ex.argtypes = [_P, _P, _P, _I, _P, _I, _I, _P, _I, _P]

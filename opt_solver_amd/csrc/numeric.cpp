@@ -566,6 +566,9 @@ SMFGPU_API int smfgpu_plan_create(const smfgpu_analysis_desc *desc, int32_t posd
     try {
         auto P = std::make_unique<smfgpu_plan_s>();
         P->posdef = posdef;
+        HIP_OK(hipGetDevice(&P->device));
+        // the no-allocation solve's work block on this device
+        if (const int rc = scratch_init(P->device); rc != SMFGPU_OK) return rc;
         HIP_OK(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking));
         const int rc = plan_build(P.get(), desc);
         if (rc != SMFGPU_OK) {
@@ -604,7 +607,20 @@ SMFGPU_API int smfgpu_plan_stats(smfgpu_plan_t P, double *out) {
     double bytes = 0;
     bytes += P->d_L.bytes() + P->d_CB.bytes() + P->d_av.bytes() + P->d_rows.bytes() + P->d_cmap.bytes() +
              P->d_asm.bytes() + P->d_cc.bytes() + P->d_potrf.bytes() + P->d_trsm.bytes() + P->d_syrk.bytes() +
-             P->d_amap.bytes() + P->d_slot_src.bytes() + P->d_fr_col.bytes() + P->d_fr_slot.bytes();
+             P->d_amap.bytes() + P->d_slot_src.bytes() + P->d_fr_col.bytes() + P->d_fr_slot.bytes() +
+             P->d_fronts.bytes() + P->d_pcol_ptr.bytes() + P->d_perm.bytes() + P->d_level_fronts.bytes() +
+             P->d_cmofs.bytes() + P->d_slot_ptr.bytes() + P->d_vals.bytes() + P->d_L32.bytes() +
+             P->d_fwd_items.bytes() + P->d_bwd_items.bytes() + P->d_gpart.bytes() + P->d_big.bytes() +
+             P->d_fr_ptr.bytes();
+    // solve and refinement work buffers (allocated on first use)
+    bytes += P->d_y.bytes() + P->d_b.bytes() + P->d_r.bytes() + P->d_rowstat.bytes() + P->d_x.bytes() +
+             P->d_bb.bytes() + P->d_rvec.bytes() + P->d_stat.bytes() + P->d_part.bytes() + P->d_h.bytes() +
+             P->d_av_ref.bytes();
+    // LDL^T fronts, factor store and staging
+    bytes += P->d_fl.bytes() + P->d_arena.bytes() + P->d_work.bytes() + P->d_dinv.bytes() + P->d_fac.bytes() +
+             P->d_lrows.bytes() + P->d_lcmap.bytes() + P->d_flist.bytes() + P->d_ltags.bytes() +
+             P->d_dkind.bytes() + P->d_stage.bytes() + P->d_child_ptr.bytes() + P->d_child_idx.bytes() +
+             P->d_parent.bytes();
     out[0] = bytes;
     out[1] = static_cast<double>(P->level_fronts.size());
     out[2] = static_cast<double>(P->launches.size());
@@ -988,24 +1004,29 @@ SMFGPU_API int smfgpu_export_factor(smfgpu_factor_t f, int64_t *sizes, int64_t *
 namespace smfgpu {
 // job as ldlt_solve; for a Cholesky factor D = I (the diagonal job copies)
 int solve_device(smfgpu_plan_s *P, const double *b_dev, int64_t ldb, double *out_dev, int64_t ldo, int32_t nrhs,
-                 int32_t job) {
-    if (!P->posdef) return ldlt_solve(P, b_dev, ldb, out_dev, ldo, nrhs, job);
+                 int32_t job, double *ybuf, int64_t ycap) {
+    if (!P->posdef) return ldlt_solve(P, b_dev, ldb, out_dev, ldo, nrhs, job, ybuf, ycap);
     hipStream_t st = P->stream;
     const int32_t n = P->n;
     DevPtrs d = dev_ptrs(P);
     const int32_t nl = static_cast<int32_t>(P->level_fronts.size());
-    for (int32_t r0 = 0; r0 < nrhs; r0 += 4) {
-        const int32_t nr = std::min(4, nrhs - r0);
-        if (P->d_y.n < static_cast<size_t>(n) * nr) {
+    // right-hand sides per pass through the sweeps: 4, or what a
+    // caller-given work block holds
+    const int32_t group = ybuf ? static_cast<int32_t>(std::min<int64_t>(4, ycap / std::max(n, 1))) : 4;
+    if (group < 1) return set_error(SMFGPU_ERR_INVALID_ARG, "solve: work block smaller than one column");
+    for (int32_t r0 = 0; r0 < nrhs; r0 += group) {
+        const int32_t nr = std::min(group, nrhs - r0);
+        if (!ybuf && P->d_y.n < static_cast<size_t>(n) * nr) {
             const int rc = P->d_y.alloc(static_cast<size_t>(n) * 4);
             if (rc) return rc;
         }
+        double *y = ybuf ? ybuf : P->d_y.p;
         const bool fwd = job == 0 || job == 1;
-        if (fwd) launch_gather_rhs(st, P->d_perm.p, b_dev + r0 * ldb, ldb, P->d_y.p, n, nr);
+        if (fwd) launch_gather_rhs(st, P->d_perm.p, b_dev + r0 * ldb, ldb, y, n, nr);
         else
-            HIP_OK(hipMemcpy2DAsync(P->d_y.p, n * sizeof(double), b_dev + r0 * ldb, ldb * sizeof(double),
+            HIP_OK(hipMemcpy2DAsync(y, n * sizeof(double), b_dev + r0 * ldb, ldb * sizeof(double),
                                     n * sizeof(double), nr, hipMemcpyDeviceToDevice, st));
-        const bool tm = P->timing && job == 0;
+        const bool tm = P->timing && job == 0 && !ybuf;
         if (tm && P->sev.size() < 3) {
             for (size_t i = P->sev.size(); i < 3; ++i) {
                 hipEvent_t e;
@@ -1017,17 +1038,17 @@ int solve_device(smfgpu_plan_s *P, const double *b_dev, int64_t ldb, double *out
         if (fwd)
             for (int32_t lv = 0; lv < nl; ++lv)
                 for (int32_t i = P->fwd_lv[lv]; i < P->fwd_lv[lv + 1]; ++i)
-                    launch_fwd_step(st, d, P->d_fwd_items.p + P->fwd_steps[i].first, P->fwd_steps[i].second, P->d_y.p,
+                    launch_fwd_step(st, d, P->d_fwd_items.p + P->fwd_steps[i].first, P->fwd_steps[i].second, y,
                                     n, nr);
         if (tm) HIP_OK(hipEventRecord(P->sev[1], st));
         if (job == 1 || job == 2) {
-            HIP_OK(hipMemcpy2DAsync(out_dev + r0 * ldo, ldo * sizeof(double), P->d_y.p, n * sizeof(double),
+            HIP_OK(hipMemcpy2DAsync(out_dev + r0 * ldo, ldo * sizeof(double), y, n * sizeof(double),
                                     n * sizeof(double), nr, hipMemcpyDeviceToDevice, st));
             continue;
         }
         for (int32_t lv = nl - 1; lv >= 0; --lv)
             for (int32_t i = P->bwd_lv[lv]; i < P->bwd_lv[lv + 1]; ++i)
-                launch_bwd_step(st, d, P->d_bwd_items.p + P->bwd_steps[i].first, P->bwd_steps[i].second, P->d_y.p, n,
+                launch_bwd_step(st, d, P->d_bwd_items.p + P->bwd_steps[i].first, P->bwd_steps[i].second, y, n,
                                 nr);
         if (tm) {
             HIP_OK(hipEventRecord(P->sev[2], st));
@@ -1040,7 +1061,7 @@ int solve_device(smfgpu_plan_s *P, const double *b_dev, int64_t ldb, double *out
             P->t_solve_phase[2] = static_cast<double>(P->fwd_steps.size());
             P->t_solve_phase[3] = static_cast<double>(P->bwd_steps.size());
         }
-        launch_scatter_rhs(st, P->d_perm.p, P->d_y.p, out_dev + r0 * ldo, ldo, n, nr);
+        launch_scatter_rhs(st, P->d_perm.p, y, out_dev + r0 * ldo, ldo, n, nr);
     }
     HIP_OK(hipGetLastError());
     return SMFGPU_OK;

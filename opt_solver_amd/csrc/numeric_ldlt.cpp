@@ -780,12 +780,16 @@ int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgp
 }
 
 int ldlt_solve(smfgpu_plan_s *P, const double *b_dev, int64_t ldb, double *out_dev, int64_t ldo, int32_t nrhs,
-               int32_t job) {
+               int32_t job, double *ybuf, int64_t ycap) {
     hipStream_t st = P->stream;
     const int32_t n = P->n;
     LdltDev d = ldlt_dev(P);
     const int32_t nl = static_cast<int32_t>(P->level_fronts.size());
-    const bool tm = P->timing && job == 0;
+    const bool tm = P->timing && job == 0 && !ybuf;
+    // right-hand sides per pass through the sweeps: 16, or what a
+    // caller-given work block holds
+    const int32_t group = ybuf ? static_cast<int32_t>(std::min<int64_t>(16, ycap / std::max(n, 1))) : 16;
+    if (group < 1) return set_error(SMFGPU_ERR_INVALID_ARG, "solve: work block smaller than one column");
     if (tm && P->sev.size() < 3) {
         for (size_t i = P->sev.size(); i < 3; ++i) {
             hipEvent_t e;
@@ -793,37 +797,38 @@ int ldlt_solve(smfgpu_plan_s *P, const double *b_dev, int64_t ldb, double *out_d
             P->sev.push_back(e);
         }
     }
-    for (int32_t r0 = 0; r0 < nrhs; r0 += 16) {
-        const int32_t nr = std::min(16, nrhs - r0);
-        if (P->d_y.n < static_cast<size_t>(n) * nr) {
+    for (int32_t r0 = 0; r0 < nrhs; r0 += group) {
+        const int32_t nr = std::min(group, nrhs - r0);
+        if (!ybuf && P->d_y.n < static_cast<size_t>(n) * nr) {
             const int rc = P->d_y.alloc(static_cast<size_t>(n) * 16);
             if (rc) return rc;
         }
+        double *y = ybuf ? ybuf : P->d_y.p;
         const bool fwd = job == 0 || job == 1;
-        if (fwd) launch_gather_rhs(st, P->d_perm.p, b_dev + r0 * ldb, ldb, P->d_y.p, n, nr);
+        if (fwd) launch_gather_rhs(st, P->d_perm.p, b_dev + r0 * ldb, ldb, y, n, nr);
         else
-            HIP_OK(hipMemcpy2DAsync(P->d_y.p, n * sizeof(double), b_dev + r0 * ldb, ldb * sizeof(double),
+            HIP_OK(hipMemcpy2DAsync(y, n * sizeof(double), b_dev + r0 * ldb, ldb * sizeof(double),
                                     n * sizeof(double), nr, hipMemcpyDeviceToDevice, st));
         if (tm) HIP_OK(hipEventRecord(P->sev[0], st));
         if (fwd)
             for (int32_t lv = 0; lv < nl; ++lv)
                 for (int32_t i = P->fwd_lv[lv]; i < P->fwd_lv[lv + 1]; ++i)
                     launch_ldlt_fwd_step(st, d, P->d_fwd_items.p + P->fwd_steps[i].first, P->fwd_steps[i].second,
-                                         P->d_y.p, n, nr);
-        if (job == 2) launch_ldlt_diag_div(st, d, P->d_flist.p, static_cast<int32_t>(P->lv_flist_ofs[nl]), P->d_y.p, n, nr);
+                                         y, n, nr);
+        if (job == 2) launch_ldlt_diag_div(st, d, P->d_flist.p, static_cast<int32_t>(P->lv_flist_ofs[nl]), y, n, nr);
         if (job == 1 || job == 2) {
             // the permuted-order result
-            HIP_OK(hipMemcpy2DAsync(out_dev + r0 * ldo, ldo * sizeof(double), P->d_y.p, n * sizeof(double),
+            HIP_OK(hipMemcpy2DAsync(out_dev + r0 * ldo, ldo * sizeof(double), y, n * sizeof(double),
                                     n * sizeof(double), nr, hipMemcpyDeviceToDevice, st));
             continue;
         }
         if (job == 0 || job == 4)
-            launch_ldlt_diag(st, d, P->d_flist.p, static_cast<int32_t>(P->lv_flist_ofs[nl]), P->d_y.p, n, nr);
+            launch_ldlt_diag(st, d, P->d_flist.p, static_cast<int32_t>(P->lv_flist_ofs[nl]), y, n, nr);
         if (tm) HIP_OK(hipEventRecord(P->sev[1], st));
         for (int32_t lv = nl - 1; lv >= 0; --lv)
             for (int32_t i = P->bwd_lv[lv]; i < P->bwd_lv[lv + 1]; ++i)
                 launch_ldlt_bwd_step(st, d, P->d_bwd_items.p + P->bwd_steps[i].first, P->bwd_steps[i].second,
-                                     P->d_y.p, n, nr);
+                                     y, n, nr);
         if (tm) {
             HIP_OK(hipEventRecord(P->sev[2], st));
             HIP_OK(hipStreamSynchronize(st));
@@ -835,7 +840,7 @@ int ldlt_solve(smfgpu_plan_s *P, const double *b_dev, int64_t ldb, double *out_d
             P->t_solve_phase[2] = static_cast<double>(P->fwd_steps.size());
             P->t_solve_phase[3] = static_cast<double>(P->bwd_steps.size());
         }
-        launch_scatter_rhs(st, P->d_perm.p, P->d_y.p, out_dev + r0 * ldo, ldo, n, nr);
+        launch_scatter_rhs(st, P->d_perm.p, y, out_dev + r0 * ldo, ldo, n, nr);
     }
     HIP_OK(hipGetLastError());
     return SMFGPU_OK;

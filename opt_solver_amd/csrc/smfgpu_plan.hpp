@@ -189,6 +189,7 @@ struct smfgpu_analysis_s {
 struct smfgpu_plan_s {
     int32_t n = 0, nsup = 0;
     int32_t posdef = 1;
+    int32_t device = 0;                            // HIP device current at plan creation
     hipStream_t stream = nullptr;
     // SPD: the factorization's launch sequence, captured once and replayed
     // (it reads the values from d_vals); [0]: every front in FP64, [1]:
@@ -304,11 +305,19 @@ int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgp
 // job: SolveJob (types.hpp:250-256): 0 full, 1 forward (in: original
 // order, out: permuted), 2 diagonal only, 3 backward, 4 diagonal then
 // backward (in: permuted order; 3 and 4 out: original)
+// ybuf (optional): caller-owned device block of ycap >= n doubles that
+// replaces the plan's right-hand-side work block; the right-hand sides then
+// go through in groups of at most ycap / n columns, nothing is allocated and
+// nothing is timed
 int ldlt_solve(smfgpu_plan_s *P, const double *b_dev, int64_t ldb, double *out_dev, int64_t ldo, int32_t nrhs,
-               int32_t job);
+               int32_t job, double *ybuf = nullptr, int64_t ycap = 0);
 int ldlt_export(smfgpu_plan_s *P, int64_t *sizes, int64_t *steps, double *fv, int32_t *rows, int8_t *tags);
 // solve on device buffers (b, out: n x nrhs, leading dimensions ldb / ldo;
 // may alias), job as ldlt_solve, for either factor kind
 int solve_device(smfgpu_plan_s *P, const double *b_dev, int64_t ldb, double *out_dev, int64_t ldo, int32_t nrhs,
-                 int32_t job = 0);
+                 int32_t job = 0, double *ybuf = nullptr, int64_t ycap = 0);
+// device address of the no-allocation solve's work block on `device`,
+// resolved once per device (noalloc.hip); the first call must be made with
+// that device current
+int scratch_init(int device);
 }  // namespace smfgpu
