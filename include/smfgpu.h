@@ -69,7 +69,8 @@ extern "C" {
 #define SMFGPU_PREC_FP64 0
 #define SMFGPU_PREC_FP32 1   /* FP32 MFMA, FP32 accumulation */
 #define SMFGPU_PREC_BF16 2   /* BF16 MFMA (operands rounded to bf16), FP32 accumulation */
-#define SMFGPU_PREC_AUTO 3   /* per-supernode choice by the learned selector */
+#define SMFGPU_PREC_AUTO 3   /* per-supernode choice by the learned selector (weights from
+                                smfgpu_selector_set; fronts of at least mixed_min_front rows) */
 #define SMFGPU_PREC_BF16X3 4 /* split BF16: operands as hi + lo bf16 pairs, three BF16
                                 MFMAs per product (hi*hi + hi*lo + lo*hi), FP32 accumulation */
 
@@ -131,7 +132,18 @@ typedef struct {
     double static_pivot_tol;
     double fallback_static_pivot_tol;
     int32_t static_flags;    /* SMFGPU_STATIC_* bits */
-    int32_t reserved[3];
+    /* Learned selector (precision SMFGPU_PREC_AUTO): each front of at least
+     * mixed_min_front rows takes the cheapest precision in selector_mask
+     * (bits 1 << SMFGPU_PREC_FP32 / _BF16 / _BF16X3; 0: all three) whose
+     * predicted relative error of the Schur update is <= selector_tol, else
+     * FP64. The prediction runs on the device between the front's assembly
+     * and its factorization. */
+    int32_t selector_mask;
+    double selector_tol;     /* default 1e-4 */
+    int32_t scaled;          /* 1: the values are scaled (the selector's scaling-state feature) */
+    int32_t collect;         /* 1 (FP64 only): keep the features of the fronts of at least
+                                mixed_min_front rows for smfgpu_selector_pairs */
+    int32_t reserved[2];
 } smfgpu_factor_options;
 
 /* static_flags (control.hpp:349-364) */
@@ -209,6 +221,44 @@ SMFGPU_API int smfgpu_export_factor(smfgpu_factor_t f, int64_t *sizes, int64_t *
  * Lets a caller that factors many independent systems as one forest report
  * each system's inertia and delays. */
 SMFGPU_API int smfgpu_factor_front_stats(smfgpu_factor_t f, int32_t *out);
+/* Precision code of each supernode's Schur updates in the last
+ * factorization (one int8 per supernode: SMFGPU_PREC_FP64 / _FP32 / _BF16 /
+ * _BF16X3), as the static policy or the learned selector set them. */
+SMFGPU_API int smfgpu_factor_precisions(smfgpu_factor_t f, int8_t *out);
+
+/* ---- learned per-supernode precision selector. An MLP (10 features ->
+ * 32 -> 32 -> 3, tanh) predicts, from cheap features of an assembled front
+ * (front order, pivot columns, contribution-block share, spread of the
+ * pivot-column norms, diagonal dominance (worst and mean), spread of the
+ * diagonal, column norms against max|A|, delayed columns received, scaling
+ * state), the log10 relative error max|dZ| / max|S| of its Schur update in
+ * FP32, split BF16 and BF16. Training and inference are HIP kernels; no
+ * reference counterpart (the reference factors in FP64 only). */
+#define SMFGPU_SELECTOR_FEATURES 10
+#define SMFGPU_SELECTOR_OUTPUTS 3
+#define SMFGPU_SELECTOR_WEIGHTS 1527 /* W1 b1 W2 b2 W3 b3, input mean, input 1/std (floats) */
+SMFGPU_API int smfgpu_selector_set(smfgpu_plan_t plan, const float *weights);
+/* Training pairs of an FP64 factorization made with opt.collect = 1: for
+ * each front of at least mixed_min_front rows with a contribution block,
+ * its features (X, SMFGPU_SELECTOR_FEATURES floats) and the measured log10
+ * max|Z_m - Z_64| / max|S| of its contribution-block update Z = L21 L21^T
+ * run on the MFMA tiles in FP32, split BF16 and BF16 from the FP64 factor
+ * (Y, 3 floats; S the FP64 contribution block), and its supernode. Returns
+ * the number of pairs (at most cap; with null arrays, the number there
+ * are), or a negative error code. Cholesky factors: the updates recomputed
+ * from the stored factor; LDL^T factors: measured during the factorization
+ * on the large fronts (the multi-workgroup path) by running each one's final
+ * contribution-block update in every precision from the same square. */
+SMFGPU_API int smfgpu_selector_pairs(smfgpu_factor_t f, float *X, float *Y, int32_t *fronts, int32_t cap);
+/* Adam training of the MLP on n pairs on the device (kernels of the
+ * library): minimises the squared error over the labels whose mask entry
+ * is 1 (M null: all). weights holds the start point and receives the
+ * result (its input mean / 1/std are set from X first); loss, when not
+ * null, receives the mean loss of each epoch (epochs floats). */
+SMFGPU_API int smfgpu_selector_train(const float *X, const float *Y, const float *M, int32_t n, int32_t epochs,
+                                     float lr, float *weights, float *loss);
+/* Inference kernel over n feature rows: Y (n x 3) predicted log10 errors. */
+SMFGPU_API int smfgpu_selector_predict(const float *weights, const float *X, int32_t n, float *Y);
 
 /* ---- solves: b is n x nrhs, leading dimension ldb >= n, in place.
  * job as smf::SolveJob (solver/include/smf/types.hpp:250-256; solve_core,
@@ -266,7 +316,10 @@ typedef struct {
     int32_t method_used;           /* SMFGPU_REFINE_CLASSICAL or SMFGPU_REFINE_FGMRES */
     int32_t reached;               /* 1: both targets met */
     int32_t stalled;               /* 1: stopped with a target unmet */
-    double reserved[2];
+    int32_t fallback;              /* smfgpu_factor_solve: 1 when the reduced-precision factor's
+                                      refinement stalled and the plain FP64 factor was used */
+    int32_t pad;
+    double reserved[1];
 } smfgpu_refine_info;
 SMFGPU_API int smfgpu_refine_default_options(smfgpu_refine_options *opt);
 SMFGPU_API int smfgpu_refine_ex(smfgpu_factor_t f, const double *values, const double *b, int32_t ldb, double *x,
@@ -278,6 +331,19 @@ SMFGPU_API int smfgpu_refine_ex(smfgpu_factor_t f, const double *values, const d
 SMFGPU_API int smfgpu_refine(smfgpu_factor_t f, const double *values, const double *b,
                              double *x, int32_t nrhs, int32_t max_iters, double tol,
                              int32_t mem, double *out);
+
+/* Factor, solve and refine in one call (Solver::factor_solve,
+ * solver/src/solver.cpp:14-73, on an existing analysis): x = A^-1 b, b and
+ * x n x nrhs with leading dimension ld (memory kind mem). When the Schur
+ * updates ran below FP64 (opt->precision FP32 / BF16 / split BF16 / AUTO)
+ * and refinement stops with a target unmet, the plan is refactored with
+ * every front in FP64 and the solve and refinement are rerun: the plain FP64
+ * factor judges the error (rinfo->fallback = 1). *out receives the factor
+ * that produced x. */
+SMFGPU_API int smfgpu_factor_solve(smfgpu_plan_t plan, const double *values, int64_t nvalues,
+                                   const smfgpu_factor_options *opt, const double *b, double *x, int32_t nrhs,
+                                   int32_t ld, const smfgpu_refine_options *ropt, int32_t mem,
+                                   smfgpu_factor_t *out, smfgpu_factor_info *finfo, smfgpu_refine_info *rinfo);
 
 /* Per-plan tuning knobs (the autotuner's parameters). Keys:
  * SMFGPU_TUNE_SOLVE_FUSE_ROWS: an SPD front whose last pivot block has at

@@ -39,7 +39,8 @@ class FactorOptions(ctypes.Structure):
         ("posdef", _I), ("pivot_u", _D), ("small_pivot", _D), ("panel", _I), ("precision", _I),
         ("mixed_min_front", _I), ("max_negative_pivots", _I), ("value_mem", _I),
         ("work_limit_on", _I), ("work_limit", _D), ("static_pivot_tol", _D),
-        ("fallback_static_pivot_tol", _D), ("static_flags", _I), ("reserved", _I * 3),
+        ("fallback_static_pivot_tol", _D), ("static_flags", _I), ("selector_mask", _I), ("selector_tol", _D),
+        ("scaled", _I), ("collect", _I), ("reserved", _I * 2),
     ]
 
 
@@ -53,7 +54,7 @@ class RefineOptions(ctypes.Structure):
 class RefineInfo(ctypes.Structure):
     _fields_ = [
         ("eta", _D), ("omega", _D), ("eta_start", _D), ("omega_start", _D), ("steps", _I), ("method_used", _I),
-        ("reached", _I), ("stalled", _I), ("reserved", _D * 2),
+        ("reached", _I), ("stalled", _I), ("fallback", _I), ("pad", _I), ("reserved", _D * 1),
     ]
 
 
@@ -76,8 +77,10 @@ EXPORTED = [
     "smfgpu_plan_stats", "smfgpu_factor", "smfgpu_factor_free", "smfgpu_export_factor", "smfgpu_solve",
     "smfgpu_refine", "smfgpu_set_timing", "smfgpu_kernel_times", "smfgpu_plan_tune",
     "smfgpu_factor_front_stats", "smfgpu_launch_profile", "smfgpu_refine_default_options", "smfgpu_refine_ex",
-    "smfgpu_solve_noalloc",
+    "smfgpu_solve_noalloc", "smfgpu_factor_precisions", "smfgpu_selector_set", "smfgpu_selector_pairs",
+    "smfgpu_selector_train", "smfgpu_selector_predict", "smfgpu_factor_solve",
 ]
+SELECTOR_FEATURES, SELECTOR_OUTPUTS, SELECTOR_WEIGHTS = 10, 3, 1527
 STATIC_AS_ZERO = 1
 STATIC_ZERO_STAYS_ZERO = 2
 STATIC_IN_ORDER = 4
@@ -125,6 +128,12 @@ def lib():
     L.smfgpu_export_factor.argtypes = [_P, _P, _P, _P, _P, _P]
     L.smfgpu_solve.argtypes = [_P, _P, _I, _I, _I, _I]
     L.smfgpu_solve_noalloc.argtypes = [_P, _P, _I, _I, _I]
+    L.smfgpu_factor_precisions.argtypes = [_P, _P]
+    L.smfgpu_selector_set.argtypes = [_P, _P]
+    L.smfgpu_selector_pairs.argtypes = [_P, _P, _P, _P, _I]
+    L.smfgpu_selector_train.argtypes = [_P, _P, _P, _I, _I, ctypes.c_float, _P, _P]
+    L.smfgpu_selector_predict.argtypes = [_P, _P, _I, _P]
+    L.smfgpu_factor_solve.argtypes = [_P, _P, _L, _P, _P, _P, _I, _I, _P, _I, _P, _P, _P]
     L.smfgpu_refine.argtypes = [_P, _P, _P, _P, _I, _I, _D, _I, _P]
     L.smfgpu_refine_default_options.argtypes = [_P]
     L.smfgpu_refine_ex.argtypes = [_P, _P, _P, _I, _P, _I, _I, _P, _I, _P]

@@ -125,6 +125,7 @@ __global__ __launch_bounds__(kThreads) void k_assemble(DevPtrs d, const AsmTask 
     const int f = F.f;
     double *colp = col_base(d, F, c);
     const int tid = threadIdx.x;
+    double ssq = 0.0, dg = 0.0, offmax = 0.0;  // this thread's share of the column statistics
     // child column jj lands in parent column cm[jj]: its rows ii >= jj go to
     // the increasing rows cm[ii], so the first chunk starts at ii = jj and
     // each chunk continues where the previous one stopped
@@ -207,8 +208,40 @@ __global__ __launch_bounds__(kThreads) void k_assemble(DevPtrs d, const AsmTask 
             }
             __syncthreads();
         }
-        for (int r = r0 + tid; r < r1; r += kThreads) colp[r] = buf[r - r0];
+        for (int r = r0 + tid; r < r1; r += kThreads) {
+            const double v = buf[r - r0];
+            colp[r] = v;
+            ssq += v * v;
+            if (r == c) dg = fabs(v);
+            else offmax = fmax(offmax, fabs(v));
+        }
         __syncthreads();
+    }
+    if (d.colstat != nullptr && c < F.p) {
+        // the selector's column statistics: 2-norm, |diagonal|, largest
+        // entry below the diagonal
+        __shared__ double red[3][kThreads / 64];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            ssq += __shfl_xor(ssq, o);
+            dg = fmax(dg, __shfl_xor(dg, o));
+            offmax = fmax(offmax, __shfl_xor(offmax, o));
+        }
+        if ((tid & 63) == 0) {
+            red[0][tid >> 6] = ssq;
+            red[1][tid >> 6] = dg;
+            red[2][tid >> 6] = offmax;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double a = 0.0, b = 0.0, m = 0.0;
+            for (int w = 0; w < kThreads / 64; ++w) {
+                a += red[0][w];
+                b = fmax(b, red[1][w]);
+                m = fmax(m, red[2][w]);
+            }
+            d.colstat[F.col0 + c] = make_float4(static_cast<float>(sqrt(a)), static_cast<float>(b), static_cast<float>(m), 0.f);
+        }
     }
 }
 
@@ -237,96 +270,104 @@ void launch_assemble(hipStream_t st, const DevPtrs &d, const AsmTask *tasks, int
 // NotPositiveDefinite.
 constexpr int kNb = 64;
 
-__global__ __launch_bounds__(kThreads) void k_potrf(DevPtrs d, const PotrfItem *__restrict__ items) {
-    __shared__ double colb[2][kNb];
-    __shared__ double rowb[2][kNb];
+// Two wavefronts per diagonal block: thread (r, h), r the lane and h the
+// wave, holds row r of the block's lower triangle in columns [32 h, 32 h +
+// 32) (a[i] = A[r][32 h + i]) and rows [32 h, 32 h + 32) of column r of W,
+// the identity turned into the unit lower inverse by the same row
+// operations (w[i] = W[32 h + i][r]). Step j: the wave owning column j puts
+// the multipliers g_c = A[c][j] / d_jj, the column A[.][j] and the row
+// W[j][.] in LDS (two alternating buffers: one barrier per step); then each
+// thread updates A[r][c] -= A[r][j] g_c for c > j and W[i][r] -= g_i W[j][r]
+// for i > j in its half. inv(L_kk) = D^-1/2 W. Rows and columns past kb are
+// the identity.
+__device__ __forceinline__ void potrf_steps(double (&a)[32], double (&w)[32], int r, int h, int kb, double (*gl)[kNb],
+                                            double (*cola)[kNb], double (*colw)[kNb], double *rsv, bool &bad) {
+    for (int jb = 0; jb < 2; ++jb) {
+#pragma unroll
+        for (int jj = 0; jj < 32; ++jj) {
+            const int j = 32 * jb + jj;
+            const int bs = jj & 1;
+            if (h == jb) {
+                const double ajj = a[jj];
+                const double djj = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(ajj), j),
+                                                    __builtin_amdgcn_readlane(__double2loint(ajj), j));
+                // 1 / sqrt(djj): the hardware estimate refined by two
+                // Newton steps (y <- y (3 - d y^2) / 2)
+                double rs = __builtin_amdgcn_rsq(djj);
+                rs = rs * (1.5 - 0.5 * djj * rs * rs);
+                rs = rs * (1.5 - 0.5 * djj * rs * rs);
+                gl[bs][r] = r > j ? ajj * (rs * rs) : 0.0;
+                cola[bs][r] = ajj;
+                colw[bs][r] = w[jj];
+                if (r == 0) rsv[j] = rs;
+                bad = bad || (j < kb && !(djj > 0.0 && djj <= DBL_MAX));
+                a[jj] = r > j ? ajj * rs : (r == j ? djj * rs : ajj);
+            }
+            __syncthreads();
+            const double arj = cola[bs][r], wjr = colw[bs][r];
+            if (h == jb) {
+#pragma unroll
+                for (int i = jj + 1; i < 32; ++i) {
+                    a[i] -= arj * gl[bs][32 * jb + i];
+                    w[i] -= gl[bs][32 * jb + i] * wjr;
+                }
+            } else if (h > jb) {
+#pragma unroll
+                for (int i = 0; i < 32; ++i) {
+                    a[i] -= arj * gl[bs][32 * h + i];
+                    w[i] -= gl[bs][32 * h + i] * wjr;
+                }
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(128) void k_potrf(DevPtrs d, const PotrfItem *__restrict__ items) {
+    __shared__ double gl[2][kNb];
+    __shared__ double cola[2][kNb];
+    __shared__ double colw[2][kNb];
     __shared__ double rsv[kNb];
-    __shared__ double T[kNb][kNb + 1];
+    __shared__ int flag_bad;
     const PotrfItem it = items[blockIdx.x];
     const FrontDev F = d.fronts[it.s];
     const int f = F.f, k = it.k, kb = it.kb;
     double *base = d.L + F.lofs;
-    const int tid = threadIdx.x;
-    const int r = tid & 63;
-    const int cg = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave cg owns columns 16 cg .. 16 cg + 15
-    // a: the block's lower triangle (padded with the identity past kb);
-    // w: the identity, turned into inv(L_kk) (rows unscaled) by the same
-    // row operations. Upper entries of a are never read.
-    double a[16], w[16];
+    const int r = threadIdx.x & 63;
+    const int h = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (threadIdx.x == 0) flag_bad = 0;
+    bool bad = false;
+    double a[32], w[32];
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        const int c = 16 * cg + i;
+    for (int i = 0; i < 32; ++i) {
+        const int c = 32 * h + i;
         a[i] = base[static_cast<int64_t>(k + min(c, kb - 1)) * f + k + min(r, kb - 1)];
     }
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        const int c = 16 * cg + i;
+    for (int i = 0; i < 32; ++i) {
+        const int c = 32 * h + i;
         const bool in = r < kb && c < kb && r >= c;
         a[i] = masked(a[i], in) + ((!in && r == c) ? 1.0 : 0.0);
-        w[i] = (r == c) ? 1.0 : 0.0;
+        w[i] = (c == r) ? 1.0 : 0.0;
     }
-    bool bad = false;
-    // Step j: column j of A and row j of W through LDS (alternating
-    // buffers: one barrier per step). With g = A[r][j] / d_jj for rows
-    // r > j (0 above): A[r][c] -= g A[c][j] (c > j), W[r][c] -= g W[j][c]
-    // (c <= j), i.e. L[r][j] L[c][j] and L[r][j] X[j][c] with
-    // X[j] = W[j] / L_jj. Rows of W are scaled by 1 / L_rr at the end.
-    for (int jb = 0; jb < 4; ++jb) {
+    potrf_steps(a, w, r, h, kb, gl, cola, colw, rsv, bad);
+    if (bad) flag_bad = 1;
+    __syncthreads();
+    if (threadIdx.x == 0 && flag_bad) atomicMin(d.flag, it.s + 1);
+    // lower (c <= r): L_kk; strict upper (r < c): M[r][c] = inv(L_kk)[c][r]
+    // = W[c][r] / L_cc
+    if (r < kb) {
 #pragma unroll
-        for (int jj = 0; jj < 16; ++jj) {
-            const int j = 16 * jb + jj;
-            const int bs = jj & 1;
-            if (cg == jb) colb[bs][r] = a[jj];
-            if (r == j) {
-#pragma unroll
-                for (int i = 0; i < 16; ++i) rowb[bs][16 * cg + i] = w[i];
-            }
-            __syncthreads();
-            const double djj = colb[bs][j];
-            // 1 / sqrt(djj): the hardware estimate refined by two Newton
-            // steps (y <- y (3 - d y^2) / 2), off the division and square
-            // root sequences on this step's critical path
-            double rs = __builtin_amdgcn_rsq(djj);
-            rs = rs * (1.5 - 0.5 * djj * rs * rs);
-            rs = rs * (1.5 - 0.5 * djj * rs * rs);
-            const double gr = r > j ? colb[bs][r] * (rs * rs) : 0.0;
-            if (cg > jb) {
-#pragma unroll
-                for (int i = 0; i < 16; ++i) a[i] -= gr * colb[bs][16 * cg + i];
-            } else if (cg < jb) {
-#pragma unroll
-                for (int i = 0; i < 16; ++i) w[i] -= gr * rowb[bs][16 * cg + i];
-            } else {
-#pragma unroll
-                for (int i = 0; i < 16; ++i) {
-                    if (i > jj) a[i] -= gr * colb[bs][16 * cg + i];
-                    else w[i] -= gr * rowb[bs][16 * cg + i];
-                }
-                a[jj] = r > j ? colb[bs][r] * rs : (r == j ? djj * rs : a[jj]);
-            }
-            if (tid == 0) rsv[j] = rs;
-            bad = bad || (j < kb && !(djj > 0.0 && djj <= DBL_MAX));
+        for (int i = 0; i < 32; ++i) {
+            const int c = 32 * h + i;
+            if (c < kb) base[static_cast<int64_t>(k + c) * f + k + r] = c <= r ? a[i] : w[i] * rsv[c];
         }
-    }
-    if (tid == 0 && bad) atomicMin(d.flag, it.s + 1);
-    __syncthreads();
-    // lower: L_kk (diagonal sqrt(d_jj)); strict upper (r < c):
-    // M[r][c] = inv(L_kk)[c][r] = W[c][r] / L_cc, through a transpose in LDS
-    const double rr = rsv[r];
-#pragma unroll
-    for (int i = 0; i < 16; ++i) T[r][16 * cg + i] = w[i] * rr;
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        const int c = 16 * cg + i;
-        if (r < kb && c < kb) base[static_cast<int64_t>(k + c) * f + k + r] = (r >= c) ? a[i] : T[c][r];
     }
 }
 
 void launch_potrf(hipStream_t st, const DevPtrs &d, const PotrfItem *it, int64_t n) {
     if (n <= 0) return;
     for (int64_t o = 0; o < n; o += kMaxLaunchGroups)
-        k_potrf<<<dim3(static_cast<unsigned>(std::min(kMaxLaunchGroups, n - o))), kThreads, 0, st>>>(d, it + o);
+        k_potrf<<<dim3(static_cast<unsigned>(std::min(kMaxLaunchGroups, n - o))), 128, 0, st>>>(d, it + o);
 }
 
 // Panel TRSM: rows [r0, r0 + 64) of columns [k, k + kb): X = A M, with
@@ -379,8 +420,37 @@ __global__ __launch_bounds__(kThreads) void k_trsm(DevPtrs d, const TrsmItem *__
     }
     const int rr = 16 * w + lr;
     // fronts with reduced-precision Schur updates also get the panel in
-    // FP32 (the reduced tiles' operands: half the bytes)
-    float *b32 = (d.L32 != nullptr && d.prec != nullptr && d.prec[it.s] != 0) ? d.L32 + F.lofs : nullptr;
+    // the precision their tiles read: FP32 (half the bytes), or bf16 in the
+    // packed layout (high part, and low part for split BF16)
+    const int pc = d.prec != nullptr ? d.prec[it.s] : 0;
+    float *b32 = (d.L32 != nullptr && pc == 1) ? d.L32 + F.lofs : nullptr;
+    if ((pc == 2 || pc == 3) && d.L16h != nullptr) {
+        __syncthreads();  // every wave is done with As
+        if (rr < nr) {
+#pragma unroll
+            for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) As[rr][16 * cb + 4 * v + lk] = acc[cb][v];
+        }
+        __syncthreads();
+        // 8-column groups of one row as 16-byte vectors; columns past kb
+        // (the front's last pivot) stay zero
+        const bool split = pc == 3 && d.L16l != nullptr;
+        for (int e = tid; e < kNb * (kNb / 8); e += kThreads) {
+            const int row = e & 63, g = e >> 6;
+            if (row >= nr || 8 * g >= kb) continue;
+            v8bf_t hi, lo;
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const float x = 8 * g + u < kb ? static_cast<float>(As[row][8 * g + u]) : 0.f;
+                hi[u] = static_cast<__bf16>(x);
+                lo[u] = static_cast<__bf16>(x - static_cast<float>(hi[u]));
+            }
+            const int64_t o = F.l16ofs + static_cast<int64_t>(k / 8 + g) * f * 8 + static_cast<int64_t>(r0 + row) * 8;
+            *reinterpret_cast<v8bf_t *>(d.L16h + o) = hi;
+            if (split) *reinterpret_cast<v8bf_t *>(d.L16l + o) = lo;
+        }
+    }
     if (rr < nr) {
 #pragma unroll
         for (int cb = 0; cb < 4; ++cb)
@@ -443,9 +513,10 @@ __device__ __forceinline__ void syrk_tile(const DevPtrs &d, const SyrkItem &it, 
     } else {
         constexpr bool kSplit = MODE == 3;
         constexpr int kKc = kSplit ? kSyrkKcBf3 : kSyrkKcBf;
-        const float *Lb = d.L32 + F.lofs;
-        if (active) tile_loop_bf16<kSplit, true, kKc, float>(Lb, Lb, f, 0, i0, j0, f, j1, it.t0, it.t1, smem, acc);
-        else tile_loop_bf16<kSplit, false, kKc, float>(Lb, Lb, f, 0, i0, j0, f, j1, it.t0, it.t1, smem, acc);
+        const uint16_t *Lh = d.L16h + F.l16ofs;
+        const uint16_t *Ll = kSplit ? d.L16l + F.l16ofs : Lh;
+        if (active) tile_loop_bf16p<kSplit, true, kKc>(Lh, Ll, Lh, Ll, f, i0, j0, f, j1, it.t0, it.t1, smem, acc);
+        else tile_loop_bf16p<kSplit, false, kKc>(Lh, Ll, Lh, Ll, f, i0, j0, f, j1, it.t0, it.t1, smem, acc);
     }
     if (!active) return;
     // C -= acc, 16 elements at a time: all 16 loads issued before the
@@ -497,23 +568,28 @@ __global__ __launch_bounds__(kThreads) void k_syrk64(DevPtrs d, const SyrkItem *
     syrk_tile<0, kAhead>(d, it, smem);
 }
 
-constexpr int cmax(int a, int b) { return a > b ? a : b; }
+template <int MODE>
+struct RedLds {
+    static constexpr int bytes = MODE == 1   ? tile_lds_bytes<float, kSyrkKc32>()
+                                 : MODE == 2 ? bf_lds_bytes<false, kSyrkKcBf>()
+                                             : bf_lds_bytes<true, kSyrkKcBf3>();
+};
 
-template <int kAhead>
+// one kernel per reduced precision (its own register count); each returns
+// at once on the items of fronts of another precision
+template <int MODE, int kAhead>
 __global__ __launch_bounds__(kThreads) void k_syrk_red(DevPtrs d, const SyrkItem *__restrict__ items) {
-    constexpr int kBytes = cmax(tile_lds_bytes<float, kSyrkKc32>(),
-                                cmax(bf_lds_bytes<false, kSyrkKcBf>(), bf_lds_bytes<true, kSyrkKcBf3>()));
-    __shared__ __attribute__((aligned(16))) unsigned char smem[kBytes];
+    __shared__ __attribute__((aligned(16))) unsigned char smem[RedLds<MODE>::bytes];
     const SyrkItem it = items[xcd_contiguous(blockIdx.x, gridDim.x)];
-    const int prec = d.prec != nullptr ? d.prec[it.s] : 0;
-    if (prec == 1) syrk_tile<1, kAhead>(d, it, smem);
-    else if (prec == 2) syrk_tile<2, kAhead>(d, it, smem);
-    else if (prec == 3) syrk_tile<3, kAhead>(d, it, smem);
+    if (d.prec == nullptr || d.prec[it.s] != MODE) return;
+    syrk_tile<MODE, kAhead>(d, it, smem);
 }
 
-// n items; the reduced kernel covers the last n_red of them (the items of
-// fronts that may run below FP64)
-void launch_syrk(hipStream_t st, const DevPtrs &d, const SyrkItem *it, int64_t n, int64_t n_red, int ahead) {
+// n items; the reduced kernels cover the last n_red of them (the items of
+// fronts that may run below FP64), one launch per precision code in
+// red_modes (bit c: code c)
+void launch_syrk(hipStream_t st, const DevPtrs &d, const SyrkItem *it, int64_t n, int64_t n_red, int ahead,
+                 int red_modes) {
     if (n <= 0) return;
     for (int64_t o = 0; o < n; o += kMaxLaunchGroups) {
         const dim3 grid(static_cast<unsigned>(std::min(kMaxLaunchGroups, n - o)));
@@ -523,8 +599,13 @@ void launch_syrk(hipStream_t st, const DevPtrs &d, const SyrkItem *it, int64_t n
     const SyrkItem *ir = it + (n - n_red);
     for (int64_t o = 0; o < n_red; o += kMaxLaunchGroups) {
         const dim3 grid(static_cast<unsigned>(std::min(kMaxLaunchGroups, n_red - o)));
-        if (ahead == 1) k_syrk_red<1><<<grid, kThreads, 0, st>>>(d, ir + o);
-        else k_syrk_red<2><<<grid, kThreads, 0, st>>>(d, ir + o);
+        if (red_modes & 2) {
+            if (ahead == 1) k_syrk_red<1, 1><<<grid, kThreads, 0, st>>>(d, ir + o);
+            else k_syrk_red<1, 2><<<grid, kThreads, 0, st>>>(d, ir + o);
+        }
+        // the bf16 tiles keep one register set of loads in flight
+        if (red_modes & 4) k_syrk_red<2, 1><<<grid, kThreads, 0, st>>>(d, ir + o);
+        if (red_modes & 8) k_syrk_red<3, 1><<<grid, kThreads, 0, st>>>(d, ir + o);
     }
 }
 
@@ -572,7 +653,11 @@ void launch_scatter_rhs(hipStream_t st, const int32_t *perm, const double *y, do
 // stat[4c+3] = max |b_i|. Maxima are taken over the bit patterns of the
 // non-negative values (an Inf or NaN sorts above every finite value), by one
 // atomic max per workgroup; stat is cleared by the caller.
+// Knuth's TwoSum: s + e = a + b exactly. FMA contraction is off here and
+// in the residual loop: a product fused into these sums (the default under
+// -ffp-contract=fast) would break the error-free transformation.
 __device__ __forceinline__ void two_sum(double a, double b, double &s, double &e) {
+#pragma clang fp contract(off)
     s = a + b;
     const double bb = s - a;
     e = (a - (s - bb)) + (b - bb);
@@ -606,6 +691,7 @@ __global__ __launch_bounds__(kThreads) void k_residual(int32_t n, const int64_t 
                                                        const double *__restrict__ av, const double *__restrict__ x,
                                                        const double *__restrict__ b, double *__restrict__ r,
                                                        unsigned long long *__restrict__ stat, int32_t nrhs) {
+#pragma clang fp contract(off)
     for (int c = 0; c < nrhs; ++c) {
         const double *xc = x + static_cast<int64_t>(c) * n;
         unsigned long long m[4] = {0, 0, 0, 0};
@@ -616,8 +702,8 @@ __global__ __launch_bounds__(kThreads) void k_residual(int32_t n, const int64_t 
             for (int64_t k = rp[i]; k < rp[i + 1]; ++k) {
                 const double a = av[slot[k]];
                 const double xv = xc[ci[k]];
-                const double ph = -a * xv;
-                const double pl = fma(-a, xv, -ph);
+                const double ph = -a * xv;           // rounded product
+                const double pl = fma(-a, xv, -ph);  // its exact rounding error
                 double s2, e;
                 two_sum(sh, ph, s2, e);
                 sh = s2;

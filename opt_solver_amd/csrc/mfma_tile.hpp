@@ -328,5 +328,119 @@ __device__ __forceinline__ void tile_loop_bf16(const GE *Ab, const GE *Bb, int l
     }
 }
 
+// ---------------------------------------------------------------------------
+// BF16 tile on operands already stored as bf16 in the packed panel layout:
+// element (row i, panel column t) of a front with ld rows sits at
+// P[(t / 8) * ld * 8 + i * 8 + t % 8], so one row's 8 consecutive k-values
+// are one 16-byte vector, loaded by one raw buffer load and written to LDS
+// as is (the LDS layout is tile_loop_bf16's). SPLIT: a second plane holds
+// the low parts. Panel columns [t0, t1) with t0 a multiple of 8; the plane
+// holds zeros in a front's columns past its last pivot, so the last group of
+// a chunk ending at the front's last pivot reads zeros there.
+typedef unsigned int v4u_t __attribute__((ext_vector_type(4)));
+template <bool SPLIT, bool ACTIVE, int kKc>
+__device__ __forceinline__ void tile_loop_bf16p(const uint16_t *Ah, const uint16_t *Al, const uint16_t *Bh,
+                                                const uint16_t *Bl, int ld, int i0, int j0, int f, int j1, int t0,
+                                                int t1, unsigned char *smem, v4f_t (&acc)[4][4]) {
+    constexpr int kLd = bf_lds_ld<kKc>();
+    constexpr int kVec = kKc / 16;        // 16-byte vectors per thread, operand and plane
+    constexpr int kPlane = 2 * kT * kLd;  // one operand's two stages, in bf16 elements
+    __bf16 *lds = reinterpret_cast<__bf16 *>(smem);
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int w = tid >> 6;
+    const int wr = w >> 1, wc = w & 1;
+    constexpr uint32_t kOut = 0x80000000u;
+    const int rl = tid & (kT - 1), th = tid >> 7;
+    const uint32_t grp = static_cast<uint32_t>(ld) * 16u;  // bytes of one 8-column group
+    const uint32_t voa = (i0 + rl < f) ? static_cast<uint32_t>(i0 + rl) * 16u : kOut;
+    const uint32_t vob = (j0 + rl < j1) ? static_cast<uint32_t>(j0 + rl) * 16u : kOut;
+    typedef v4u_t Regs[SPLIT ? 2 * kVec : kVec];
+    auto gload = [&](Regs &ra, Regs &rb, int kc) {
+        const int ng = (min(kKc, t1 - kc) + 7) / 8;  // groups of the chunk inside [t0, t1)
+        const int nrec = ng * static_cast<int>(grp);
+        const int64_t go = static_cast<int64_t>(kc / 8) * ld * 8;
+        const __amdgpu_buffer_rsrc_t sa = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint16_t *>(Ah + go), 0, nrec, kBufferWord3);
+        const __amdgpu_buffer_rsrc_t sb = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint16_t *>(Bh + go), 0, nrec, kBufferWord3);
+#pragma unroll
+        for (int e = 0; e < kVec; ++e) {
+            const uint32_t co = static_cast<uint32_t>(th + 2 * e) * grp;
+            ra[e] = __builtin_amdgcn_raw_buffer_load_b128(sa, static_cast<int>(voa + co), 0, 0);
+            rb[e] = __builtin_amdgcn_raw_buffer_load_b128(sb, static_cast<int>(vob + co), 0, 0);
+        }
+        if (SPLIT) {
+            const __amdgpu_buffer_rsrc_t la = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint16_t *>(Al + go), 0, nrec, kBufferWord3);
+            const __amdgpu_buffer_rsrc_t lb = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint16_t *>(Bl + go), 0, nrec, kBufferWord3);
+#pragma unroll
+            for (int e = 0; e < kVec; ++e) {
+                const uint32_t co = static_cast<uint32_t>(th + 2 * e) * grp;
+                ra[kVec + e] = __builtin_amdgcn_raw_buffer_load_b128(la, static_cast<int>(voa + co), 0, 0);
+                rb[kVec + e] = __builtin_amdgcn_raw_buffer_load_b128(lb, static_cast<int>(vob + co), 0, 0);
+            }
+        }
+    };
+    auto sstore = [&](const Regs &ra, const Regs &rb, int stg) {
+#pragma unroll
+        for (int e = 0; e < kVec; ++e) {
+            const int o = stg * kT * kLd + rl * kLd + 8 * (th + 2 * e);
+            *reinterpret_cast<v4u_t *>(lds + o) = ra[e];
+            *reinterpret_cast<v4u_t *>(lds + kPlane + o) = rb[e];
+            if (SPLIT) {
+                *reinterpret_cast<v4u_t *>(lds + 2 * kPlane + o) = ra[kVec + e];
+                *reinterpret_cast<v4u_t *>(lds + 3 * kPlane + o) = rb[kVec + e];
+            }
+        }
+    };
+    auto mfma_stage = [&](int stg) {
+        if (!ACTIVE) return;
+        const __bf16 *As = lds + stg * kT * kLd;
+        const __bf16 *Bs = lds + kPlane + stg * kT * kLd;
+#pragma unroll
+        for (int kk = 0; kk < kKc; kk += 32) {
+            const int ko = kk + 8 * (lane >> 4);
+            v8bf_t a[4], b[4];
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+                a[m] = *reinterpret_cast<const v8bf_t *>(Bs + (wc * 64 + m * 16 + (lane & 15)) * kLd + ko);
+#pragma unroll
+            for (int nn = 0; nn < 4; ++nn)
+                b[nn] = *reinterpret_cast<const v8bf_t *>(As + (wr * 64 + nn * 16 + (lane & 15)) * kLd + ko);
+            if (SPLIT) {
+                v8bf_t al[4], bl[4];
+#pragma unroll
+                for (int m = 0; m < 4; ++m)
+                    al[m] = *reinterpret_cast<const v8bf_t *>(Bs + 2 * kPlane + (wc * 64 + m * 16 + (lane & 15)) * kLd + ko);
+#pragma unroll
+                for (int nn = 0; nn < 4; ++nn)
+                    bl[nn] = *reinterpret_cast<const v8bf_t *>(As + 2 * kPlane + (wr * 64 + nn * 16 + (lane & 15)) * kLd + ko);
+#pragma unroll
+                for (int m = 0; m < 4; ++m)
+#pragma unroll
+                    for (int nn = 0; nn < 4; ++nn) {
+                        acc[m][nn] = mfma_bf16(al[m], b[nn], acc[m][nn]);
+                        acc[m][nn] = mfma_bf16(a[m], bl[nn], acc[m][nn]);
+                    }
+            }
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+#pragma unroll
+                for (int nn = 0; nn < 4; ++nn) acc[m][nn] = mfma_bf16(a[m], b[nn], acc[m][nn]);
+        }
+    };
+    // global loads of chunk c+1 in flight during the MFMAs of chunk c
+    int stg = 0;
+    Regs ra, rb;
+    gload(ra, rb, t0);
+    sstore(ra, rb, 0);
+    __syncthreads();
+    for (int kc = t0; kc < t1; kc += kKc) {
+        const bool more = kc + kKc < t1;
+        if (more) gload(ra, rb, kc + kKc);
+        mfma_stage(stg);
+        if (more) sstore(ra, rb, stg ^ 1);
+        __syncthreads();
+        stg ^= 1;
+    }
+}
 
 }  // namespace smfgpu

@@ -56,15 +56,18 @@ DevPtrs dev_ptrs(smfgpu_plan_s *P) {
     d.av = P->d_av.p;
     d.L = P->d_L.p;
     d.L32 = P->d_L32.p;
+    d.L16h = P->d_L16h.p;
+    d.L16l = P->d_L16l.p;
     d.CB = P->d_CB.p;
     d.flag = P->d_flag.p;
     d.prec = P->d_prec.p;
     d.junk = P->d_junk.p;
+    d.colstat = nullptr;
     return d;
 }
 
 static void release_graph(smfgpu_plan_s *P) {
-    for (int g = 0; g < 2; ++g) {
+    for (int g = 0; g < 3; ++g) {
         if (P->fexec[g]) (void)hipGraphExecDestroy(P->fexec[g]);
         if (P->fgraph[g]) (void)hipGraphDestroy(P->fgraph[g]);
         P->fexec[g] = nullptr;
@@ -218,9 +221,17 @@ int dense_schedule(smfgpu_plan_s *P) {
         const int64_t y1 = static_cast<int64_t>(syrk.size());
         P->launches.push_back(Launch{K_SYRK, lv, y0, y1 - y0, y1 - yr});
     };
+    P->selfr.clear();
     for (int32_t lv = 0; lv < nlev; ++lv) {
         const auto &fl = P->level_fronts[lv];
         P->launches.push_back(Launch{K_ASSEMBLE, lv, P->asm_first[lv], P->asm_first[lv + 1] - P->asm_first[lv], 0});
+        // the selector's candidates of the level (fronts that may run below
+        // FP64), between their assembly and their factorization
+        const int64_t s0 = static_cast<int64_t>(P->selfr.size());
+        for (int32_t s : fl)
+            if (P->fronts[s].f >= red_min)
+                P->selfr.push_back(SelFront{s, P->fronts[s].f, P->fronts[s].p, 0, P->fronts[s].col0});
+        P->launches.push_back(Launch{K_SELECT, lv, s0, static_cast<int64_t>(P->selfr.size()) - s0, 0});
         int32_t maxp = 0;
         for (int32_t s : fl) maxp = std::max(maxp, P->fronts[s].p);
         for (int32_t K = 0; K < maxp; K += outer) {
@@ -286,10 +297,34 @@ int dense_schedule(smfgpu_plan_s *P) {
         P->syrk_flops_of_launch.push_back(fl);
     }
     int rc;
-    if ((rc = P->d_potrf.upload(potrf)) || (rc = P->d_trsm.upload(trsm)) || (rc = P->d_syrk.upload(syrk))) return rc;
+    if ((rc = P->d_potrf.upload(potrf)) || (rc = P->d_trsm.upload(trsm)) || (rc = P->d_syrk.upload(syrk)) ||
+        (rc = P->d_selfr.upload(P->selfr)))
+        return rc;
     release_graph(P);
     return SMFGPU_OK;
 }
+
+}  // namespace
+
+SelPrm smfgpu::sel_prm(smfgpu_plan_s *P, const smfgpu_factor_options &opt) {
+    SelPrm sp{};
+    const bool autosel = opt.precision == SMFGPU_PREC_AUTO;
+    sp.w = autosel ? P->d_selw.p : nullptr;
+    sp.log_tol = static_cast<float>(std::log10(autosel ? opt.selector_tol : 1.0));
+    int32_t mask = 0;  // device codes: bit 1 FP32, bit 2 BF16, bit 3 split BF16
+    const int32_t am = opt.selector_mask != 0
+                           ? opt.selector_mask
+                           : (1 << SMFGPU_PREC_FP32) | (1 << SMFGPU_PREC_BF16) | (1 << SMFGPU_PREC_BF16X3);
+    if (am & (1 << SMFGPU_PREC_FP32)) mask |= 2;
+    if (am & (1 << SMFGPU_PREC_BF16)) mask |= 4;
+    if (am & (1 << SMFGPU_PREC_BF16X3)) mask |= 8;
+    sp.mask = mask;
+    sp.scaled = opt.scaled != 0;
+    sp.maxabs_bits = reinterpret_cast<const unsigned long long *>(P->d_red.p);
+    return sp;
+}
+
+namespace {
 
 int plan_build(smfgpu_plan_s *P, const smfgpu_analysis_desc *D) {
     const int32_t n = D->n, ns = D->nsup;
@@ -301,7 +336,7 @@ int plan_build(smfgpu_plan_s *P, const smfgpu_analysis_desc *D) {
     std::vector<int32_t> col2sn(n, -1);
     P->fronts.resize(ns);
     P->front_flops.assign(ns, 0.0);
-    int64_t lofs = 0, cbofs = 0;
+    int64_t lofs = 0, cbofs = 0, l16ofs = 0;
     for (int32_t s = 0; s < ns; ++s) {
         FrontDev &F = P->fronts[s];
         F.f = D->front_row_ptr[s + 1] - D->front_row_ptr[s];
@@ -311,9 +346,11 @@ int plan_build(smfgpu_plan_s *P, const smfgpu_analysis_desc *D) {
         F.lofs = lofs;
         F.cbofs = cbofs;
         F.rofs = D->front_row_ptr[s];
+        F.l16ofs = l16ofs;
         if (F.p <= 0 || F.q < 0) return set_error(SMFGPU_ERR_INVALID_ARG, "plan: bad supernode sizes");
         for (int32_t j = D->sn_start[s]; j < D->sn_end[s]; ++j) col2sn[j] = s;
         lofs += static_cast<int64_t>(F.f) * F.p;
+        l16ofs += static_cast<int64_t>(F.f) * ((F.p + 7) / 8 * 8);
         cbofs += static_cast<int64_t>(F.q) * F.q;
         double fl = 0.0;
         for (int c = 0; c < F.p; ++c) {
@@ -324,6 +361,7 @@ int plan_build(smfgpu_plan_s *P, const smfgpu_analysis_desc *D) {
         P->flops += fl;
     }
     P->factor_entries = lofs;
+    P->l16_entries = l16ofs;
     P->cb_entries = cbofs;
     P->rows_total = D->front_row_ptr[ns];
     // levels (children before parents; supernodes are in postorder)
@@ -495,6 +533,7 @@ SMFGPU_API int smfgpu_default_options(smfgpu_factor_options *opt) {
     opt->mixed_min_front = 1024;
     opt->max_negative_pivots = -1;
     opt->value_mem = SMFGPU_MEM_HOST;
+    opt->selector_tol = 1e-4;
     return SMFGPU_OK;
 }
 
@@ -610,6 +649,7 @@ SMFGPU_API int smfgpu_plan_stats(smfgpu_plan_t P, double *out) {
              P->d_amap.bytes() + P->d_slot_src.bytes() + P->d_fr_col.bytes() + P->d_fr_slot.bytes() +
              P->d_fronts.bytes() + P->d_pcol_ptr.bytes() + P->d_perm.bytes() + P->d_level_fronts.bytes() +
              P->d_cmofs.bytes() + P->d_slot_ptr.bytes() + P->d_vals.bytes() + P->d_L32.bytes() +
+             P->d_L16h.bytes() + P->d_L16l.bytes() +
              P->d_fwd_items.bytes() + P->d_bwd_items.bytes() + P->d_gpart.bytes() + P->d_big.bytes() +
              P->d_fr_ptr.bytes();
     // solve and refinement work buffers (allocated on first use)
@@ -737,11 +777,20 @@ SMFGPU_API int smfgpu_factor(smfgpu_plan_t P, const double *values, int64_t nval
     if ((opt.posdef != 0) != (P->posdef != 0))
         return set_error(SMFGPU_ERR_INVALID_ARG, "factor: matrix type differs from the plan's");
     if (nvalues != P->n_orig) return set_error(SMFGPU_ERR_INVALID_ARG, "factor: wrong number of values");
-    if (prec_code(opt.precision) < 0)
-        return set_error(opt.precision == SMFGPU_PREC_AUTO ? SMFGPU_ERR_UNSUPPORTED : SMFGPU_ERR_INVALID_ARG,
-                         "factor: precision " + std::to_string(opt.precision) + " is not available");
+    const bool autosel = opt.precision == SMFGPU_PREC_AUTO;
+    if (!autosel && prec_code(opt.precision) < 0)
+        return set_error(SMFGPU_ERR_INVALID_ARG, "factor: precision " + std::to_string(opt.precision) + " is not available");
+    if (autosel && !P->sel_loaded)
+        return set_error(SMFGPU_ERR_UNSUPPORTED, "factor: precision AUTO needs selector weights (smfgpu_selector_set)");
+    if (opt.collect && opt.precision != SMFGPU_PREC_FP64)
+        return set_error(SMFGPU_ERR_INVALID_ARG, "factor: collect needs an FP64 factorization");
+    if (autosel && !(opt.selector_tol > 0.0))
+        return set_error(SMFGPU_ERR_INVALID_ARG, "factor: selector_tol must be > 0");
     if (nvalues > 0 && !values) return set_error(SMFGPU_ERR_INVALID_ARG, "factor: null values");
     hipStream_t st = P->stream;
+    // ||A||_inf of the factored values, cached by the refinement, belongs to
+    // the previous values
+    P->norm_a_valid = false;
     const double *dv = values;
     // values go through d_vals when they are on the host, and for the SPD
     // replayed sequence (its nodes read a fixed buffer)
@@ -763,11 +812,24 @@ SMFGPU_API int smfgpu_factor(smfgpu_plan_t P, const double *values, int64_t nval
         smfgpu_factor_info fi{};
         if (P->live) P->live->valid = false;
         P->live = nullptr;
+        F->auto_prec = autosel;
+        F->collected = opt.collect != 0;
+        F->mixed_min_front = opt.mixed_min_front;
+        if (autosel || opt.collect) {
+            if ((P->d_feat.p == nullptr && P->d_feat.alloc(static_cast<size_t>(std::max(P->nsup, 1)) * kSelNF)))
+                return SMFGPU_ERR_ALLOC;
+            if (opt.collect) {
+                if (P->d_lab.p == nullptr && P->d_lab.alloc(static_cast<size_t>(std::max(P->nsup, 1)) * 4))
+                    return SMFGPU_ERR_ALLOC;
+                HIP_OK(hipMemsetAsync(P->d_lab.p, 0, static_cast<size_t>(P->nsup) * 4 * sizeof(unsigned int), st));
+            }
+        }
         const double t0 = now_s();
         const int rc = ldlt_factor(P, dv, nvalues, opt, fi);
         fi.factor_seconds = now_s() - t0;
         if (info) *info = fi;
         if (rc != SMFGPU_OK) return rc;
+        F->prec = P->ldlt_prec;
         F->info = fi;
         // solve_core refuses a singular factor (solve.cpp:888-903)
         F->valid = fi.status == SMFGPU_STATUS_SUCCESS;
@@ -778,27 +840,58 @@ SMFGPU_API int smfgpu_factor(smfgpu_plan_t P, const double *values, int64_t nval
         return SMFGPU_OK;
     }
     // per-front Schur precision: fronts of at least mixed_min_front rows
-    // run the requested reduced precision
+    // run the requested reduced precision, or, with AUTO, the precision the
+    // learned selector picks on the device once the front is assembled
     auto F = std::make_unique<smfgpu_factor_s>();
     F->plan = P;
     F->prec.assign(P->nsup, 0);
+    F->auto_prec = autosel;
+    F->collected = opt.collect != 0;
+    F->mixed_min_front = opt.mixed_min_front;
     int reduced = 0;
-    const int8_t code = static_cast<int8_t>(prec_code(opt.precision));
-    if (code > 0) {
-        for (int32_t s = 0; s < P->nsup; ++s)
-            if (P->fronts[s].f >= opt.mixed_min_front) {
+    const int8_t code = autosel ? 0 : static_cast<int8_t>(prec_code(opt.precision));
+    int candidates = 0;
+    for (int32_t s = 0; s < P->nsup; ++s)
+        if (P->fronts[s].f >= opt.mixed_min_front) {
+            ++candidates;
+            if (code > 0) {
                 F->prec[s] = code;
                 ++reduced;
             }
+        }
+    const bool sel_on = (autosel || opt.collect) && candidates > 0;
+    const int gi = autosel && candidates > 0 ? 2 : (reduced > 0 ? 1 : 0);
+    if (sel_on) {
+        if ((P->d_colstat.p == nullptr && P->d_colstat.alloc(std::max(P->n, 1))) ||
+            (P->d_feat.p == nullptr && P->d_feat.alloc(static_cast<size_t>(std::max(P->nsup, 1)) * kSelNF)))
+            return SMFGPU_ERR_ALLOC;
+        // the graph of the selector's launches bakes its parameters in
+        const double key[3] = {opt.selector_tol, static_cast<double>(opt.selector_mask), static_cast<double>(opt.scaled)};
+        if (autosel && (key[0] != P->sel_key[0] || key[1] != P->sel_key[1] || key[2] != P->sel_key[2])) {
+            release_graph(P);
+            std::copy(key, key + 3, P->sel_key);
+        }
     }
-    const int gi = reduced > 0 ? 1 : 0;
-    if (reduced > 0 && P->sched_red_min != opt.mixed_min_front) {
+    if ((reduced > 0 || sel_on) && P->sched_red_min != opt.mixed_min_front) {
         // the Schur launches put the items of fronts that may run reduced last
         P->sched_red_min = std::max(opt.mixed_min_front, 1);
         const int rc = dense_schedule(P);
         if (rc) return rc;
     }
-    if (reduced > 0 && P->d_L32.p == nullptr) {
+    const bool need16 = (autosel && candidates > 0) || (reduced > 0 && (code == 2 || code == 3));
+    const bool need16lo = (autosel && candidates > 0) || (reduced > 0 && code == 3);
+    if ((need16 && P->d_L16h.p == nullptr) || (need16lo && P->d_L16l.p == nullptr)) {
+        // packed bf16 panel planes (the BF16 tiles' operands); zeros in the
+        // padding columns past each front's last pivot stay zero
+        for (DBuf<uint16_t> *pb : {&P->d_L16h, &P->d_L16l}) {
+            const bool want = pb == &P->d_L16h ? need16 : need16lo;
+            if (!want || pb->p != nullptr) continue;
+            if (const int rc = pb->alloc(std::max<int64_t>(P->l16_entries, 1))) return rc;
+            HIP_OK(hipMemsetAsync(pb->p, 0, pb->bytes(), st));
+        }
+        release_graph(P);
+    }
+    if ((reduced > 0 || (autosel && candidates > 0)) && P->d_L32.p == nullptr) {
         // FP32 panel copies (the reduced fronts' Schur operands), once per plan
         int rc = P->d_L32.alloc(std::max<int64_t>(P->factor_entries, 1));
         if (rc) return rc;
@@ -808,6 +901,14 @@ SMFGPU_API int smfgpu_factor(smfgpu_plan_t P, const double *values, int64_t nval
     const int32_t big = INT32_MAX;
     HIP_OK(hipMemcpyAsync(P->d_flag.p, &big, sizeof(int32_t), hipMemcpyHostToDevice, st));
     DevPtrs d = dev_ptrs(P);
+    SelPrm sprm{};
+    if (sel_on) {
+        d.colstat = P->d_colstat.p;
+        sprm = sel_prm(P, opt);
+    }
+    // reduced-precision Schur kernels to launch: the static code, or every
+    // code the selector may pick
+    const int red_modes = autosel ? sprm.mask : (code > 0 ? (1 << code) : 0);
     // timing events
     const bool timing = P->timing;
     const size_t nl = P->launches.size();
@@ -839,7 +940,13 @@ SMFGPU_API int smfgpu_factor(smfgpu_plan_t P, const double *values, int64_t nval
             }
             case K_POTRF: launch_potrf(st, d, P->d_potrf.p + l.first, l.count); break;
             case K_TRSM: launch_trsm(st, d, P->d_trsm.p + l.first, l.count); break;
-            case K_SYRK: launch_syrk(st, d, P->d_syrk.p + l.first, l.count, gi ? l.red : 0, P->syrk_ahead); break;
+            case K_SYRK:
+                launch_syrk(st, d, P->d_syrk.p + l.first, l.count, gi ? l.red : 0, P->syrk_ahead, red_modes);
+                break;
+            case K_SELECT:
+                if (sel_on) launch_select(st, P->d_selfr.p + l.first, static_cast<int32_t>(l.count), P->d_colstat.p, sprm,
+                                          P->d_feat.p, P->d_prec.p);
+                break;
             default: break;
             }
             if (rec) mark(2 * li + 1);
@@ -848,10 +955,18 @@ SMFGPU_API int smfgpu_factor(smfgpu_plan_t P, const double *values, int64_t nval
         const hipError_t e2 = hipGetLastError();
         return e != hipSuccess ? e : e2;
     };
-    if (timing || nvalues == 0) {
+    if (timing || nvalues == 0 || opt.collect) {
         HIP_OK(enqueue(timing));
     } else {
+        if (P->fexec[gi] && P->graph_red_modes[gi] != red_modes) {
+            // captured with another set of reduced-precision kernels
+            (void)hipGraphExecDestroy(P->fexec[gi]);
+            (void)hipGraphDestroy(P->fgraph[gi]);
+            P->fexec[gi] = nullptr;
+            P->fgraph[gi] = nullptr;
+        }
         if (!P->fexec[gi]) {
+            P->graph_red_modes[gi] = red_modes;
             HIP_OK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             const hipError_t e1 = enqueue(false);
             hipGraph_t g = nullptr;
@@ -870,6 +985,8 @@ SMFGPU_API int smfgpu_factor(smfgpu_plan_t P, const double *values, int64_t nval
     }
     int32_t flag = 0;
     unsigned long long maxbits = 0;
+    if (autosel && candidates > 0)
+        HIP_OK(hipMemcpyAsync(F->prec.data(), P->d_prec.p, P->nsup, hipMemcpyDeviceToHost, st));
     HIP_OK(hipMemcpyAsync(&flag, P->d_flag.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
     HIP_OK(hipMemcpyAsync(&maxbits, P->d_red.p, sizeof(maxbits), hipMemcpyDeviceToHost, st));
     HIP_OK(hipStreamSynchronize(st));
@@ -880,6 +997,10 @@ SMFGPU_API int smfgpu_factor(smfgpu_plan_t P, const double *values, int64_t nval
     fi.max_abs = max_abs;
     fi.max_front = 0;
     for (const FrontDev &Fd : P->fronts) fi.max_front = std::max(fi.max_front, Fd.f);
+    if (autosel) {
+        reduced = 0;
+        for (int8_t c : F->prec) reduced += c != 0;
+    }
     fi.fronts_reduced = reduced;
     fi.factor_entries = P->factor_entries;
     fi.factor_seconds = t1 - t0;

@@ -270,10 +270,50 @@ static int ldlt_big_fronts(smfgpu_plan_s *P, const LdltDev &d, const std::vector
     hipStream_t st = P->stream;
     const int32_t nbig = static_cast<int32_t>(big.size());
     const bool group = P->ldlt_group != 0 && nb == 16;
+    const bool autosel = opt.precision == SMFGPU_PREC_AUTO;
+    const bool collect = opt.collect != 0;
     h.assign(nbig, BigLd{});
     int64_t cap = 64;
     int32_t nmode[4] = {0, 0, 0, 0};
-    const int code = prec_code(opt.precision);
+    const int code = autosel ? 0 : prec_code(opt.precision);
+    int rc;
+    // learned selector: the candidates' features from their assembled
+    // squares, then (AUTO) each one's precision, read back for the launches
+    std::vector<SelFront> sf;
+    std::vector<int32_t> sf_big;  // index into big of each candidate
+    int32_t maxne = 0, maxq = 0;
+    int64_t snap_total = 0;
+    if (autosel || collect) {
+        int64_t cs = 0;
+        for (int32_t b = 0; b < nbig; ++b) {
+            const FrontLd &F = P->fl[big[b]];
+            if (F.f < opt.mixed_min_front) continue;
+            sf.push_back(SelFront{big[b], F.f, F.ne, F.nd, cs, F.fofs, snap_total});
+            sf_big.push_back(b);
+            cs += F.ne;
+            snap_total += static_cast<int64_t>(F.f) * F.f;
+            maxne = std::max(maxne, F.ne);
+            maxq = std::max(maxq, F.f - F.ne);
+        }
+        if (!sf.empty()) {
+            if ((rc = ensure(P->d_selfr, sf.size(), false, "selector fronts")) ||
+                (rc = ensure(P->d_colstat, static_cast<size_t>(std::max<int64_t>(cs, 1)), false, "selector statistics")))
+                return rc;
+            HIP_OK(hipMemcpyAsync(P->d_selfr.p, sf.data(), sf.size() * sizeof(SelFront), hipMemcpyHostToDevice, st));
+            launch_sq_colstat(st, P->d_arena.p, P->d_selfr.p, static_cast<int32_t>(sf.size()), maxne, P->d_colstat.p);
+            launch_select(st, P->d_selfr.p, static_cast<int32_t>(sf.size()), P->d_colstat.p, sel_prm(P, opt),
+                          P->d_feat.p, P->d_prec.p);
+            if (autosel) {
+                std::vector<int8_t> pc(P->nsup);
+                HIP_OK(hipMemcpyAsync(pc.data(), P->d_prec.p, P->nsup, hipMemcpyDeviceToHost, st));
+                HIP_OK(hipStreamSynchronize(st));
+                for (const SelFront &x : sf) P->ldlt_prec[x.s] = pc[x.s];
+            }
+            if (collect)
+                for (const SelFront &x : sf)
+                    if (x.p < x.f) P->sel_cand.push_back(x.s);
+        }
+    }
     for (int32_t b = 0; b < nbig; ++b) {
         const FrontLd &F = P->fl[big[b]];
         BigLd &B = h[b];
@@ -283,13 +323,15 @@ static int ldlt_big_fronts(smfgpu_plan_s *P, const LdltDev &d, const std::vector
         B.oe = std::min(F.ne, kBigOuter);
         B.spec = group ? 1 : 0;
         // mixed precision: reduced-precision MFMA updates on the fronts of
-        // at least mixed_min_front rows (the pivot logic stays FP64)
-        B.prec = code > 0 && F.f >= opt.mixed_min_front ? code : 0;
+        // at least mixed_min_front rows (the pivot logic stays FP64), the
+        // requested one or the selector's choice
+        if (autosel) B.prec = F.f >= opt.mixed_min_front ? P->ldlt_prec[big[b]] : 0;
+        else B.prec = code > 0 && F.f >= opt.mixed_min_front ? code : 0;
+        P->ldlt_prec[big[b]] = static_cast<int8_t>(B.prec);
         ++nmode[B.prec];
         cap += 4 * (static_cast<int64_t>(F.ne) / std::max(nb, 1) + 2) + 2 * static_cast<int64_t>(F.ne);
     }
     reduced += nbig - nmode[0];
-    int rc;
     if ((rc = ensure(P->d_big, static_cast<size_t>(nbig), false, "large-front states"))) return rc;
     HIP_OK(hipMemcpyAsync(P->d_big.p, h.data(), nbig * sizeof(BigLd), hipMemcpyHostToDevice, st));
     // update workgroups per front: a multiple of 8 (one run of tiles per XCD)
@@ -320,7 +362,44 @@ static int ldlt_big_fronts(smfgpu_plan_s *P, const LdltDev &d, const std::vector
     HIP_OK(hipGetLastError());
     int32_t pmax = 0;
     for (const BigLd &B : h) pmax = std::max(pmax, B.k);
+    const bool labels = collect && !sf.empty() && maxq > 0;
+    if (labels) {
+        // training pairs: the candidates' contribution-block update run in
+        // FP32, split BF16 and BF16 from the same square (copies in slots
+        // 0-2 of the snapshot buffer, the square itself kept in slot 3 and
+        // put back after each run), then in FP64 as the factorization's own
+        if (4.0 * static_cast<double>(snap_total) * sizeof(double) > 32.0e9)
+            return set_error(SMFGPU_ERR_ALLOC, "factor: training-pair snapshots would pass 32 GB");
+        if ((rc = ensure(P->d_snap, static_cast<size_t>(4 * snap_total), false, "training-pair snapshots"))) return rc;
+        auto copy_sq = [&](bool to_snap, int slot) -> int {
+            for (const SelFront &x : sf) {
+                double *sq = P->d_arena.p + x.fofs;
+                double *sn = P->d_snap.p + slot * snap_total + x.sofs;
+                const size_t bytes = static_cast<size_t>(x.f) * x.f * sizeof(double);
+                HIP_OK(hipMemcpyAsync(to_snap ? sn : sq, to_snap ? sq : sn, bytes, hipMemcpyDeviceToDevice, st));
+            }
+            return SMFGPU_OK;
+        };
+        if ((rc = copy_sq(true, 3))) return rc;
+        std::vector<int8_t> is_cand(nbig, 0);
+        for (int32_t b : sf_big) is_cand[b] = 1;
+        for (int m : {1, 3, 2}) {
+            int32_t nm[4] = {0, 0, 0, 0};
+            for (int32_t b = 0; b < nbig; ++b) {
+                h[b].prec = is_cand[b] ? m : -1;  // -1: no update kernel touches the front
+                if (is_cand[b]) ++nm[m];
+            }
+            HIP_OK(hipMemcpyAsync(P->d_big.p, h.data(), nbig * sizeof(BigLd), hipMemcpyHostToDevice, st));
+            for (int32_t c0 = 0; c0 < pmax; c0 += kBigWork) launch_big_cb(st, d, P->d_big.p, nbig, c0, kBigWork, per_front, nm);
+            if ((rc = copy_sq(true, kSelOutOfCode[m])) || (rc = copy_sq(false, 3))) return rc;
+        }
+        for (BigLd &B : h) B.prec = 0;
+        HIP_OK(hipMemcpyAsync(P->d_big.p, h.data(), nbig * sizeof(BigLd), hipMemcpyHostToDevice, st));
+    }
     for (int32_t c0 = 0; c0 < pmax; c0 += kBigWork) launch_big_cb(st, d, P->d_big.p, nbig, c0, kBigWork, per_front, nmode);
+    if (labels)
+        launch_label_sq(st, P->d_arena.p, P->d_snap.p, snap_total, P->d_selfr.p, static_cast<int32_t>(sf.size()), maxq,
+                        P->d_lab.p);
     launch_big_finish(st, d, P->d_big.p, nbig);
     HIP_OK(hipGetLastError());
     return SMFGPU_OK;
@@ -480,6 +559,8 @@ int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgp
     spr.flags = opt.static_flags;
     P->fac_used = P->rows_used = P->dinv_used = 0;
     P->sq_pool.reset();
+    P->ldlt_prec.assign(P->nsup, 0);
+    P->sel_cand.clear();
     std::vector<int32_t> pos(n, -1);
     std::vector<int32_t> cmap;
     std::vector<AsmTask> tasks;

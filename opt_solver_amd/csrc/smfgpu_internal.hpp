@@ -50,6 +50,7 @@ struct FrontDev {
     int64_t lofs;           // factor offset
     int64_t cbofs;          // contribution-block offset
     int64_t rofs;           // row-index offset (rows[rofs .. rofs + f))
+    int64_t l16ofs;         // offset of the pivot columns in the packed bf16 planes (f x p rounded up to 8 columns)
 };
 
 // One column of one front to assemble (a workgroup per task).
@@ -89,6 +90,7 @@ enum LaunchKind : int32_t {
     K_POTRF = 1,
     K_TRSM = 2,
     K_SYRK = 3,
+    K_SELECT = 4,  // learned precision selector over the level's candidate fronts (items: SelFront)
 };
 
 struct Launch {
@@ -111,10 +113,14 @@ struct DevPtrs {
     const double *av;       // permuted values
     double *L;
     float *L32;             // FP32 copy of the panel columns of reduced-precision fronts (null: none)
+    uint16_t *L16h, *L16l;  // bf16 copies (high part, low part) of the panel columns of BF16 / split-BF16
+                            // fronts in the packed layout: (row i, column t) at l16ofs + (t / 8) f 8 + 8 i + t % 8
     double *CB;
     int32_t *flag;          // [0]: smallest failing supernode + 1 (SPD); INT32_MAX when none
     const int8_t *prec;     // per supernode: Schur-update precision 0-3 (null: all FP64)
     double *junk;           // kThreads doubles: target of masked-out stores (contents meaningless)
+    float4 *colstat;        // per pivot column (permuted numbering), written by the assembly when not null:
+                            // 2-norm of the column's lower part, |diagonal|, max |below the diagonal|
 };
 
 void launch_map_values(hipStream_t st, const double *vals, int64_t nvals, const int64_t *slot_ptr,
@@ -124,8 +130,10 @@ void launch_assemble(hipStream_t st, const DevPtrs &d, const AsmTask *tasks, int
                      int32_t fmax);
 void launch_potrf(hipStream_t st, const DevPtrs &d, const PotrfItem *it, int64_t n);
 void launch_trsm(hipStream_t st, const DevPtrs &d, const TrsmItem *it, int64_t n);
-// n items; the reduced-precision kernel runs the last n_red of them
-void launch_syrk(hipStream_t st, const DevPtrs &d, const SyrkItem *it, int64_t n, int64_t n_red, int ahead);
+// n items; the reduced-precision kernels run the last n_red of them, one
+// launch per precision code in red_modes (bit c: code c)
+void launch_syrk(hipStream_t st, const DevPtrs &d, const SyrkItem *it, int64_t n, int64_t n_red, int ahead,
+                 int red_modes);
 void launch_zero_upper(hipStream_t st, const DevPtrs &d, const PotrfItem *it, int64_t n);
 
 // Solves: y is the permuted right-hand side block, n x nrhs (ld n).
@@ -165,5 +173,69 @@ void launch_vdiv(hipStream_t st, int64_t n, const double *w, const double *d, do
 void launch_lincomb(hipStream_t st, int64_t n, const double *x0, const double *Z, int32_t nvec, const double *y,
                     double *out);
 void launch_reduce_max(hipStream_t st, const double *v, int64_t n, double *out);
+
+// ---------------------------------------------------------------------
+// Learned per-supernode precision selector (selector.hip). An MLP
+// kSelNF -> kSelH -> kSelH -> kSelNO (tanh hidden layers) maps a front's
+// features to the predicted log10 relative error its Schur update would
+// have in FP32, split BF16 and BF16 (outputs 0, 1, 2). Weight layout
+// (floats): W1[kSelH][kSelNF], b1[kSelH], W2[kSelH][kSelH], b2[kSelH],
+// W3[kSelNO][kSelH], b3[kSelNO], then the inputs' mean[kSelNF] and
+// inverse standard deviation[kSelNF].
+constexpr int kSelNF = 10, kSelH = 32, kSelNO = 3;
+constexpr int kSelParams = kSelH * kSelNF + kSelH + kSelH * kSelH + kSelH + kSelNO * kSelH + kSelNO;
+constexpr int kSelWeights = kSelParams + 2 * kSelNF;
+// output index of each precision code (1 FP32, 2 BF16, 3 split BF16)
+constexpr int kSelOutOfCode[4] = {-1, 0, 2, 1};
+
+// One front handed to the selector: rows, pivot columns (eligible columns
+// for LDL^T), delayed columns received, and where its pivot columns' stats
+// are (colstat index of column 0).
+struct SelFront {
+    int32_t s, f, p, nd;
+    int64_t cs0;
+    int64_t fofs;  // LDL^T: the front's square in the arena
+    int64_t sofs;  // LDL^T training pairs: the square's copies in the snapshot buffer
+};
+struct SelPrm {
+    const float *w;            // kSelWeights (null: features only, precision 0)
+    float log_tol;             // a precision is taken when its predicted log10 error <= log_tol
+    int32_t mask;              // allowed precision codes: bit c for code c (1 FP32, 2 BF16, 3 split BF16)
+    int32_t scaled;            // scaling state of the values (feature 9)
+    const unsigned long long *maxabs_bits;  // max |A| as the bits of a double
+};
+// features of each front into feat[s * kSelNF ..] and its precision into
+// prec[s] (or, for LDL^T, BigLd::prec of big[i] when big is not null)
+void launch_select(hipStream_t st, const SelFront *fr, int32_t nf, const float4 *colstat, const SelPrm &prm,
+                   float *feat, int8_t *prec);
+// inference over rows of raw features: Y[i * kSelNO ..]
+void launch_mlp_predict(hipStream_t st, const float *w, const float *X, int32_t n, float *Y);
+// one training step's gradient (mean squared error over the labels with
+// mask 1), per-block partial sums into part (nblk x (kSelParams + 1): the
+// last entry the block's summed squared error); returns nblk
+int32_t launch_mlp_grad(hipStream_t st, const float *w, const float *X, const float *Y, const float *M, int32_t n,
+                        float *part);
+// Adam step over the parameters from nblk partial gradients (divided by
+// the label count cnt); the loss of the step into loss[0]
+void launch_mlp_adam(hipStream_t st, float *w, float *m, float *v, const float *part, int32_t nblk, float cnt,
+                     int32_t step, float lr, float *loss);
+// labels of the SPD candidate fronts of a factorization: for each CB tile
+// item (t0 = 0, t1 = p), max |Z_64 - Z_m| over the tile (m = FP32, split
+// BF16, BF16: slots 0, 1, 2) and max |S| (slot 3) into lab[4 s ..] as float
+// bits (cleared by the caller); zs: n x 128 x 128 doubles of scratch
+// (n <= kMaxLaunchGroups)
+void launch_label_spd(hipStream_t st, const DevPtrs &d, const SyrkItem *items, int64_t n, double *zs,
+                      unsigned int *lab);
+// LDL^T: statistics of the eligible columns (fr[i].p of them, rows from the
+// diagonal down) of the f x f squares at arena + fr[i].fofs, into
+// colstat[fr[i].cs0 ..]; maxcols: the largest fr[i].p
+void launch_sq_colstat(hipStream_t st, const double *arena, const SelFront *fr, int32_t nf, int32_t maxcols,
+                       float4 *colstat);
+// LDL^T labels: over the contribution block [p, f)^2 (lower) of each square,
+// max |snap[k * stride + sofs ..] - square| for k = 0, 1, 2 (the updates in
+// FP32, split BF16, BF16) into lab[4 s + k] and max |square| into
+// lab[4 s + 3], as float bits; maxq: the largest f - p
+void launch_label_sq(hipStream_t st, const double *arena, const double *snap, int64_t stride, const SelFront *fr,
+                     int32_t nf, int32_t maxq, unsigned int *lab);
 
 }  // namespace smfgpu

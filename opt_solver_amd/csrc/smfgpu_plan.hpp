@@ -193,9 +193,11 @@ struct smfgpu_plan_s {
     hipStream_t stream = nullptr;
     // SPD: the factorization's launch sequence, captured once and replayed
     // (it reads the values from d_vals); [0]: every front in FP64, [1]:
-    // with the reduced-precision Schur launches
-    hipGraph_t fgraph[2] = {nullptr, nullptr};
-    hipGraphExec_t fexec[2] = {nullptr, nullptr};
+    // with the reduced-precision Schur launches, [2]: with the learned
+    // selector's launches too
+    hipGraph_t fgraph[3] = {nullptr, nullptr, nullptr};
+    hipGraphExec_t fexec[3] = {nullptr, nullptr, nullptr};
+    int32_t graph_red_modes[3] = {-1, -1, -1};     // reduced Schur kernels each graph launches
     int32_t sched_red_min = INT32_MAX;             // SPD schedule: fronts with f >= this may run below FP64
     std::vector<FrontDev> fronts;
     std::vector<int32_t> level;                    // per supernode
@@ -219,6 +221,8 @@ struct smfgpu_plan_s {
     DBuf<int8_t> d_prec;
     DBuf<double> d_junk;
     DBuf<float> d_L32;  // allocated at the first factorization with FP32 Schur updates
+    DBuf<uint16_t> d_L16h, d_L16l;  // packed bf16 planes, allocated (zeroed) at the first BF16 factorization
+    int64_t l16_entries = 0;
     // solve schedule: items of every step of every level; step i of the
     // forward sweep is fwd_steps[i] = {first item, count}, level lv owns
     // steps [fwd_lv[lv], fwd_lv[lv + 1]) (backward likewise)
@@ -250,6 +254,22 @@ struct smfgpu_plan_s {
     double norm_a = 0.0;
     bool norm_a_valid = false;
     std::vector<int64_t> syrk_flops_of_launch;
+    // learned selector: weights, per-column statistics of the assembled
+    // pivot columns, per-front features, candidate fronts per level (SPD:
+    // the K_SELECT launches' items)
+    DBuf<float> d_selw, d_feat;
+    DBuf<float4> d_colstat;
+    DBuf<SelFront> d_selfr;
+    std::vector<SelFront> selfr;
+    bool sel_loaded = false;
+    double sel_key[3] = {-1.0, -1.0, -1.0};         // tolerance, mask, scaling state baked into the graph [2]
+    int32_t sel_fallbacks = 0;                      // refinements that fell back to an FP64 refactorization
+    // LDL^T: per supernode precision code of the last factorization; the
+    // training pairs' fronts and labels of a collecting factorization
+    std::vector<int8_t> ldlt_prec;
+    std::vector<int32_t> sel_cand;
+    DBuf<unsigned int> d_lab;
+    DBuf<double> d_snap;
     // timing
     bool timing = false;
     double t_syrk_ms = 0, t_asm_ms = 0, t_solve_ms = 0;
@@ -291,7 +311,10 @@ struct smfgpu_factor_s {
     smfgpu_plan_s *plan = nullptr;
     smfgpu_factor_info info{};
     bool valid = false;
-    std::vector<int8_t> prec;
+    std::vector<int8_t> prec;                       // per supernode: device precision code of its Schur updates
+    bool collected = false;                         // features kept for smfgpu_selector_pairs
+    bool auto_prec = false;                         // precisions chosen by the learned selector
+    int32_t mixed_min_front = 0;
 };
 
 
@@ -320,4 +343,7 @@ int solve_device(smfgpu_plan_s *P, const double *b_dev, int64_t ldb, double *out
 // resolved once per device (noalloc.hip); the first call must be made with
 // that device current
 int scratch_init(int device);
+// the selector launches' parameters for a factorization's options (w null
+// unless the precision is AUTO)
+SelPrm sel_prm(smfgpu_plan_s *P, const smfgpu_factor_options &opt);
 }  // namespace smfgpu

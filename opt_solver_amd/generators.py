@@ -30,6 +30,7 @@ __all__ = [
     "spd27_3d",
     "kkt_grid",
     "kkt_sigma_sequence",
+    "kkt_grid_refactor_values",
     "ocp_kkt",
     "coo_lower_to_csc",
     "csc_to_dense",
@@ -175,6 +176,19 @@ def kkt_sigma_sequence(sigma0, k: int, seed: int = 2):
     rng = np.random.default_rng(seed)
     s = rng.uniform(-0.3, 0.3, size=np.asarray(sigma0).size)
     return np.clip(10.0 ** (np.log10(sigma0) + k * s), 1e-10, 1e10)
+
+
+def kkt_grid_refactor_values(A, sigma, nx: int):
+    """Values of a kkt_grid matrix A (same pattern and entry order) with the
+    barrier diagonal Sigma replaced: H_ii = 4 + sigma_i for the first nx
+    columns (the diagonal is each column's first entry)."""
+    n, cp, ri, v = A
+    first = cp[:nx]
+    if not np.array_equal(ri[first], np.arange(nx)):
+        raise ValueError("not a kkt_grid matrix: a column's first entry is not its diagonal")
+    out = np.array(v, dtype=np.float64, copy=True)
+    out[first] = 4.0 + np.asarray(sigma, dtype=np.float64)
+    return out
 
 
 def ocp_kkt(T: int, seed: int = 0, reg: float = 1e-9):

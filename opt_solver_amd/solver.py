@@ -140,9 +140,63 @@ class Plan:
     def tune(self, key, value):
         check(lib().smfgpu_plan_tune(self.h, key, int(value)), "plan_tune")
 
+    def selector_set(self, weights):
+        """Load the learned precision selector's weights (SELECTOR_WEIGHTS
+        floats) into the plan; PREC_AUTO factorizations use them."""
+        w = np.ascontiguousarray(weights, np.float32)
+        if w.size != _lib.SELECTOR_WEIGHTS:
+            raise ValueError(f"expected {_lib.SELECTOR_WEIGHTS} weights, got {w.size}")
+        check(lib().smfgpu_selector_set(self.h, _p(w)), "selector_set")
+        self.selector_loaded = True
+
     def factor(self, values=None, values_ptr=None, precision=PREC_FP64, mixed_min_front=1024, u=0.01,
                small_pivot=1e-20, panel=16, max_negative_pivots=-1, work_limit=None, static_pivot_tol=0.0,
-               fallback_static_pivot_tol=0.0, static_flags=0) -> "Factor":
+               fallback_static_pivot_tol=0.0, static_flags=0, selector_tol=1e-4, selector_mask=0, scaled=False,
+               collect=False) -> "Factor":
+        if precision == PREC_AUTO and not getattr(self, "selector_loaded", False):
+            w = default_selector_weights()
+            if w is not None:
+                self.selector_set(w)
+        opt, vp, keep = self._options(values, values_ptr, precision, mixed_min_front, u, small_pivot, panel,
+                                      max_negative_pivots, work_limit, static_pivot_tol, fallback_static_pivot_tol,
+                                      static_flags, selector_tol, selector_mask, scaled, collect)
+        info = _lib.FactorInfo()
+        h = ctypes.c_void_p()
+        check(lib().smfgpu_factor(self.h, vp, self.n_orig, ctypes.byref(opt), ctypes.byref(h), ctypes.byref(info)),
+              "factor")
+        return Factor(self, h, info)
+
+    def factor_solve(self, values, b, precision=PREC_FP64, mixed_min_front=1024, u=0.01, selector_tol=1e-4,
+                     selector_mask=0, max_iters=10, tol=1e-14, max_steps=None, omega_target=None, method=None,
+                     **kw):
+        """smfgpu_factor_solve: factor, solve and refine (host arrays); a
+        reduced-precision factor whose refinement stalls is replaced by the
+        plain FP64 one. Returns (factor, x, refine info)."""
+        if precision == PREC_AUTO and not getattr(self, "selector_loaded", False):
+            w = default_selector_weights()
+            if w is not None:
+                self.selector_set(w)
+        opt, vp, keep = self._options(values, None, precision, mixed_min_front, u, kw.get("small_pivot", 1e-20),
+                                      kw.get("panel", 16), -1, None, kw.get("static_pivot_tol", 0.0),
+                                      kw.get("fallback_static_pivot_tol", 0.0), kw.get("static_flags", 0),
+                                      selector_tol, selector_mask, kw.get("scaled", False), False)
+        b = np.array(b, dtype=np.float64, order="F", copy=True)
+        nrhs = 1 if b.ndim == 1 else b.shape[1]
+        x = np.zeros_like(b)
+        ro = Factor._refine_opts(max_iters, tol, method, max_steps, omega_target, 20)
+        info = _lib.FactorInfo()
+        rinf = _lib.RefineInfo()
+        h = ctypes.c_void_p()
+        check(lib().smfgpu_factor_solve(self.h, vp, self.n_orig, ctypes.byref(opt), _p(b), _p(x), nrhs, self.n,
+                                        ctypes.byref(ro), MEM_HOST, ctypes.byref(h), ctypes.byref(info),
+                                        ctypes.byref(rinf)), "factor_solve")
+        rd = Factor._refine_dict(rinf)
+        rd["fallback"] = bool(rinf.fallback)
+        return Factor(self, h, info), x, rd
+
+    def _options(self, values, values_ptr, precision, mixed_min_front, u, small_pivot, panel, max_negative_pivots,
+                 work_limit, static_pivot_tol, fallback_static_pivot_tol, static_flags, selector_tol, selector_mask,
+                 scaled, collect):
         opt = _lib.FactorOptions()
         lib().smfgpu_default_options(ctypes.byref(opt))
         opt.posdef = 1 if self.posdef else 0
@@ -155,6 +209,10 @@ class Plan:
         opt.static_pivot_tol = static_pivot_tol
         opt.fallback_static_pivot_tol = fallback_static_pivot_tol
         opt.static_flags = static_flags
+        opt.selector_tol = selector_tol
+        opt.selector_mask = selector_mask
+        opt.scaled = 1 if scaled else 0
+        opt.collect = 1 if collect else 0
         if work_limit is not None:
             opt.work_limit_on = 1
             opt.work_limit = float(work_limit)
@@ -168,11 +226,7 @@ class Plan:
                 raise ValueError(f"expected {self.n_orig} values, got {keep.size}")
             opt.value_mem = MEM_HOST
             vp = _p(keep)
-        info = _lib.FactorInfo()
-        h = ctypes.c_void_p()
-        check(lib().smfgpu_factor(self.h, vp, self.n_orig, ctypes.byref(opt), ctypes.byref(h), ctypes.byref(info)),
-              "factor")
-        return Factor(self, h, info)
+        return opt, vp, keep
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -201,6 +255,27 @@ class Factor:
         nrhs = 1 if b.ndim == 1 else b.shape[1]
         check(lib().smfgpu_solve(self.h, _p(b), nrhs, self.plan.n, int(job), MEM_HOST), "solve")
         return b
+
+    def precisions(self):
+        """Per supernode: the precision code (PREC_*) of its Schur updates."""
+        out = np.zeros(self.plan.nsup, np.int8)
+        check(lib().smfgpu_factor_precisions(self.h, _p(out)), "factor_precisions")
+        return out
+
+    def selector_pairs(self):
+        """Training pairs of a collect=True FP64 factorization: features X
+        (k x 10), labels Y (k x 3: log10 relative Schur-update error in FP32,
+        split BF16, BF16) and the fronts' supernodes."""
+        k = lib().smfgpu_selector_pairs(self.h, None, None, None, 0)
+        if k < 0:
+            check(k, "selector_pairs")
+        X = np.zeros((max(k, 1), _lib.SELECTOR_FEATURES), np.float32)
+        Y = np.zeros((max(k, 1), _lib.SELECTOR_OUTPUTS), np.float32)
+        fr = np.zeros(max(k, 1), np.int32)
+        got = lib().smfgpu_selector_pairs(self.h, _p(X), _p(Y), _p(fr), k)
+        if got < 0:
+            check(got, "selector_pairs")
+        return X[:got], Y[:got], fr[:got]
 
     def front_stats(self):
         """Per supernode: f, p, positive, negative, zero, 2x2, delayed, eligible."""
@@ -281,3 +356,56 @@ class Factor:
             except Exception:  # interpreter shutdown: the library is going away anyway
                 pass
             self.h = None
+
+
+# ---- learned precision selector -------------------------------------------
+import json as _json  # noqa: E402
+import os as _os  # noqa: E402
+
+SELECTOR_WEIGHTS_FILE = _os.path.join(_os.path.dirname(_os.path.abspath(__file__)), "selector_weights.json")
+
+
+def default_selector_weights():
+    """The weights trained by opt_solver_amd/train_selector.py, or None."""
+    if not _os.path.exists(SELECTOR_WEIGHTS_FILE):
+        return None
+    with open(SELECTOR_WEIGHTS_FILE) as fh:
+        return np.asarray(_json.load(fh)["weights"], np.float32)
+
+
+def selector_init(seed=0):
+    """Glorot-uniform start weights (biases 0, identity input scaling)."""
+    rng = np.random.default_rng(seed)
+    nf, nh, no = _lib.SELECTOR_FEATURES, 32, _lib.SELECTOR_OUTPUTS
+
+    def glorot(o, i):
+        a = np.sqrt(6.0 / (i + o))
+        return rng.uniform(-a, a, (o, i))
+    parts = [glorot(nh, nf).ravel(), np.zeros(nh), glorot(nh, nh).ravel(), np.zeros(nh), glorot(no, nh).ravel(),
+             np.zeros(no), np.zeros(nf), np.ones(nf)]
+    w = np.concatenate(parts).astype(np.float32)
+    assert w.size == _lib.SELECTOR_WEIGHTS
+    return w
+
+
+def selector_train(X, Y, M=None, epochs=2000, lr=3e-3, weights=None, seed=0):
+    """Adam training of the selector MLP on the device (the library's HIP
+    kernels). Returns (weights, per-epoch loss)."""
+    X = np.ascontiguousarray(X, np.float32)
+    Y = np.ascontiguousarray(Y, np.float32)
+    Mk = None if M is None else np.ascontiguousarray(M, np.float32)
+    w = selector_init(seed) if weights is None else np.array(weights, np.float32, copy=True)
+    loss = np.zeros(max(epochs, 1), np.float32)
+    check(lib().smfgpu_selector_train(_p(X), _p(Y), _p(Mk), X.shape[0], int(epochs), float(lr), _p(w), _p(loss)),
+          "selector_train")
+    return w, loss[:epochs]
+
+
+def selector_predict(weights, X):
+    """The selector's inference kernel over feature rows: (n x 3) predicted
+    log10 relative errors."""
+    w = np.ascontiguousarray(weights, np.float32)
+    X = np.ascontiguousarray(X, np.float32)
+    Y = np.zeros((X.shape[0], _lib.SELECTOR_OUTPUTS), np.float32)
+    check(lib().smfgpu_selector_predict(_p(w), _p(X), X.shape[0], _p(Y)), "selector_predict")
+    return Y

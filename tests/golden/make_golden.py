@@ -9,7 +9,10 @@ Fixtures (numpy .npz, no pickles):
                      (AnalysisKeep arrays), values, and its factorization in
                      the FactorKeep layout, inertia, delays, 2x2 count, flops,
                      and a right-hand side with the reference's solution
-Run: python tests/golden/make_golden.py"""
+  metis_<case>.npz   the reference's METIS ordering of a benchmark matrix
+                     (first differences), so the benchmark factors the
+                     configuration's own elimination tree
+Run: python tests/golden/make_golden.py [metis]"""
 import os
 import sys
 
@@ -64,9 +67,26 @@ def mf_case(name, A, posdef, ordering):
         b=b, x_ref=xs)
 
 
+def metis_perms():
+    """The reference's METIS orderings (its own analysis, ordering_metis.cpp
+    with the vendored METIS 5.2.1) of the configuration-4 operator (27-point
+    100^3, seed 0) and the configuration-2 KKT (300^2, m = 30000, seed 1),
+    stored as first differences (perm = cumsum(perm_delta))."""
+    for name, A, posdef in (("metis_spd27_100", g.spd27_3d(100, seed=0), True),
+                            ("metis_kkt_300", g.kkt_grid(300, 30000, seed=1)[0], False)):
+        an = R.RefAnalysis(A, ordering=R.ORDER_METIS, posdef=posdef)
+        perm = np.asarray(an.perm, np.int64)
+        np.savez_compressed(os.path.join(HERE, f"{name}.npz"),
+                            perm_delta=np.diff(perm, prepend=0).astype(np.int32), n=np.array(A[0]))
+
+
 if __name__ == "__main__":
     if not R.available():
         sys.exit("oracle/_ref/libsmfref.so is not built")
+    if len(sys.argv) > 1 and sys.argv[1] == "metis":
+        metis_perms()
+        sys.exit(0)
+    metis_perms()
     front_cases()
     mf_case("laplace7_8", g.laplace7_3d(8), True, R.ORDER_METIS)
     mf_case("ocp_T10", g.ocp_kkt(10, seed=0), False, R.ORDER_AMD)

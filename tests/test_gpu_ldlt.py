@@ -88,7 +88,7 @@ def test_indefinite_against_oracle(oracle_lib, case):
     assert fac.inertia == orc.inertia == inertia
     assert fac.info.delayed_pivots == orc.delayed and fac.info.two_by_two_pivots == orc.two_by_two
     assert fac.flops == orc.flops
-    err = compare_factor(fac.export(), orc.export(), 1e-11)
+    err = compare_factor(fac.export(), orc.export(), 1e-12)
     x = np.random.default_rng(1).uniform(-1, 1, A[0])
     b = matvec(A, x)
     xs = fac.solve(b)
@@ -163,8 +163,8 @@ def test_large_front_path_against_oracle(oracle_lib, case, mode):
     speculative panel groups ("group") or one panel at a time ("panel"), or
     on the one-workgroup blocked kernel ("one_wg"): the same pivot sequence,
     delays, 2x2 pivots and inertia as the oracle's blocked kernel, L and D
-    within 1e-9 (the deferred updates past a panel group run on MFMA in
-    another summation order), solves to 1e-14."""
+    within 1e-12 per block (the deferred updates past a panel group run on
+    MFMA in another summation order), solves to 1e-14."""
     from opt_solver_amd._lib import TUNE_LDLT_BIG_MIN, TUNE_LDLT_GROUP
     print("case large path", case, mode, flush=True)
     A, arrays, inertia = _large_path_case(case)
@@ -175,7 +175,8 @@ def test_large_front_path_against_oracle(oracle_lib, case, mode):
     orc = oracle_lib.Factor(arrays, A[3], posdef=False)
     assert fac.inertia == orc.inertia == inertia
     assert fac.info.delayed_pivots == orc.delayed and fac.info.two_by_two_pivots == orc.two_by_two
-    compare_factor(fac.export(), orc.export(), 1e-9)
+    err = compare_factor(fac.export(), orc.export(), 1e-12)
+    print(f"  L/D block rel err {err:.1e}", flush=True)
     x = np.random.default_rng(7).uniform(-1, 1, A[0])
     b = matvec(A, x)
     xs = fac.solve(b)

@@ -147,7 +147,10 @@ def test_mixed_precision_refinement(oracle_lib, prec):
     print(f"  eta0 {eta0:.2e} refined {eta:.2e} in {st['steps']} steps", flush=True)
     assert eta0 > 1e-12  # the reduced Schur updates are visible before refinement
     assert eta <= 1e-12 and st["steps"] >= 1
-    assert abs(st["eta"] - eta) <= 0.5 * eta + 1e-18  # device residual agrees with the oracle formula
+    # the device's double-double residual and reductions against the
+    # oracle's long-double formula on the same x
+    print(f"  device eta {st['eta']:.6e} oracle {eta:.6e}", flush=True)
+    assert abs(st["eta"] - eta) <= 1e-3 * eta + 1e-30
 
 
 def test_not_positive_definite_detected():

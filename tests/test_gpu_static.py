@@ -81,7 +81,9 @@ def test_static_pivoting_against_reference(case, kw):
     assert np.array_equal(ex["row_indices"], eo["row_indices"])
     assert np.array_equal(ex["pivot_types"], eo["pivot_types"])
     fv, fo = ex["factor_values"], eo["factor_values"]
-    assert np.abs(fv - fo).max() <= 1e-10 * np.abs(fo).max()
+    err = np.abs(fv - fo).max() / np.abs(fo).max()
+    print(f"  L/D rel err {err:.1e}", flush=True)
+    assert err <= 1e-12
     if case.startswith("ocp") and fac.status == 0 and fac.inertia[2] == 0:
         # a perturbed factor: refinement recovers the solution
         x = np.random.default_rng(1).uniform(-1, 1, A[0])

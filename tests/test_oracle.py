@@ -223,3 +223,39 @@ def test_reference_solve_jobs_compose(ref_lib):
             assert np.array_equal(d, y), name
         x = rf.solve_job(d, 3)
         assert np.abs(x - full).max() <= 1e-12 * np.abs(full).max(), name
+
+
+def test_backward_error_against_exact_arithmetic(oracle_lib):
+    """eta_inf and omega of the oracle (the reference's long-double residual,
+    refinement.cpp:39-130) against the same formulas in exact rational
+    arithmetic, at a backward error near the roundoff level where a residual
+    evaluated in plain double would be off by tens of percent."""
+    from fractions import Fraction
+    from opt_solver_amd import generators as g
+    A = g.spd27_3d(5, seed=5)
+    n, cp, ri, v = A
+    rng = np.random.default_rng(0)
+    x = rng.uniform(-1, 1, n)
+    cols = np.repeat(np.arange(n), np.diff(cp))
+    vf = [Fraction(float(t)) for t in v]
+    xf = [Fraction(float(t)) for t in x]
+    ax = [Fraction(0)] * n
+    absax = [Fraction(0)] * n
+    rowabs = np.zeros(n)
+    for k in range(len(v)):
+        i, j = ri[k], cols[k]
+        ax[i] += vf[k] * xf[j]
+        absax[i] += abs(vf[k] * xf[j])
+        rowabs[i] += abs(v[k])
+        if i != j:
+            ax[j] += vf[k] * xf[i]
+            absax[j] += abs(vf[k] * xf[i])
+            rowabs[j] += abs(v[k])
+    b = np.array([float(t) for t in ax])
+    b[::5] = np.nextafter(b[::5], np.inf)  # one-ulp perturbations
+    r = [Fraction(float(b[i])) - ax[i] for i in range(n)]
+    eta = float(max(abs(t) for t in r)) / (rowabs.max() * np.abs(x).max() + np.abs(b).max())
+    om = max(float(abs(r[i]) / (absax[i] + abs(Fraction(float(b[i]))))) for i in range(n))
+    e2, o2 = oracle_lib.backward_error(A, b, x)
+    assert eta < 1e-15
+    assert abs(e2 - eta) <= 1e-3 * eta and abs(o2 - om) <= 1e-3 * om
