@@ -171,30 +171,41 @@ def c4_measure(args, torch):
 
 
 def kkt_measure(args, torch):
-    """Configuration 2: the KKT of the 300^2 grid with 30000 constraints,
-    u = 0.01, the reference's METIS ordering; FP64, FP32 and learned."""
+    """Configuration 2's KKT (300^2 grid, 30000 constraints, u = 0.01):
+    FP64, FP32 and learned steps on the library's ordering (geometric
+    nested dissection of the grid and constraints + cost-model
+    amalgamation); and one FP64 factor + solve on the reference's METIS
+    tree, whose tall fronts (up to 38k rows over a few thousand eligible
+    columns) stall the threshold search of many panels."""
     from opt_solver_amd import generators as g
     from opt_solver_amd.solver import PREC_AUTO, PREC_FP32, PREC_FP64, Plan, analyse
     gsz, m = 300, 30000
     A, _ = g.kkt_grid(gsz, m, seed=1)
-    perm = metis_perm("kkt_300")
-    ordering = "METIS (the reference's, fixture)"
-    if perm is None:
-        perm, ordering = _kkt_perm(gsz, m, A), "geometric nested dissection"
-    an = analyse(A, perm=perm, nemin=32)
-    plan = Plan(an, posdef=False)
     dev = torch.device("cuda:0")
     x = np.random.default_rng(0).uniform(-1, 1, A[0])
     vals_d = torch.from_numpy(np.ascontiguousarray(A[3])).to(dev)
     b_d = torch.from_numpy(_matvec(A, x)).to(dev)
     torch.cuda.synchronize()
+    an = analyse(A, perm=_kkt_perm(gsz, m, A), nemin=32)
+    plan = Plan(an, posdef=False)
     run = Runner(plan, vals_d, b_d, 1, an.flops, False)
     out = {}
     for name, prec in (("fp64", PREC_FP64), ("fp32", PREC_FP32), ("learned", PREC_AUTO)):
         out[name] = run.measure(f"kkt {name}", prec, args.steps, args.warmup, timed_run=False)
-    return {"workload": f"kkt_grid {gsz}^2, m={m} (n={A[0]}), u=0.01, {ordering}", "nsup": an.nsup,
-            "max_front": an.max_front, "flops_per_factor_delay_free": an.flops, **out,
-            "speedup_learned_vs_fp64": round(out["fp64"]["ms_per_step"] / out["learned"]["ms_per_step"], 3)}
+    res = {"workload": f"kkt_grid {gsz}^2, m={m} (n={A[0]}), u=0.01, geometric ND + cost-model amalgamation",
+           "nsup": an.nsup, "max_front": an.max_front, "flops_per_factor_delay_free": an.flops, **out,
+           "speedup_learned_vs_fp64": round(out["fp64"]["ms_per_step"] / out["learned"]["ms_per_step"], 3)}
+    perm = metis_perm("kkt_300")
+    if perm is not None and not args.no_kkt_metis:
+        del plan, run
+        anm = analyse(A, perm=perm, nemin=32)
+        planm = Plan(anm, posdef=False)
+        runm = Runner(planm, vals_d, b_d, 1, anm.flops, False)
+        r = runm.measure("kkt metis fp64", PREC_FP64, 1, 0, timed_run=False)
+        res["metis_fp64"] = {"workload": "the same KKT on the reference's METIS ordering (fixture)",
+                             "nsup": anm.nsup, "max_front": anm.max_front,
+                             "flops_per_factor_delay_free": anm.flops, **r}
+    return res
 
 
 def seq_measure(args, torch):
@@ -380,6 +391,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-rhs16", action="store_true")
     ap.add_argument("--no-kkt", action="store_true")
+    ap.add_argument("--no-kkt-metis", action="store_true")
     ap.add_argument("--no-seq", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-c1", action="store_true")

@@ -357,7 +357,7 @@ SMFGPU_API int smfgpu_factor_solve(smfgpu_plan_t plan, const double *values, int
  * (a multiple of 64, default 512); rebuilds the launch sequence. */
 #define SMFGPU_TUNE_PANEL_OUTER 3
 /* SMFGPU_TUNE_CB_SEG: SPD pivot columns per update of a contribution block,
- * which is updated once its front's pivots are all factored (default 2048;
+ * which is updated once its front's pivots are all factored (default 4096;
  * FP32 Schur updates accumulate in FP32 over one segment). */
 #define SMFGPU_TUNE_CB_SEG 4
 /* SMFGPU_TUNE_LDLT_BIG_MIN: LDL^T fronts with at least this many rows
@@ -382,7 +382,7 @@ SMFGPU_API int smfgpu_factor_solve(smfgpu_plan_t plan, const double *values, int
  * 95 (the largest unblocked front), 0 turns it off; at most 95. */
 #define SMFGPU_TUNE_LDLT_DEVICE_FRONTS 8
 /* SMFGPU_TUNE_ASM_CHUNK: SPD assembly's LDS column chunk is the level's
- * longest column up to this many rows (default 16384, in [256, 16384]):
+ * longest column up to this many rows (default 2048, in [256, 16384]):
  * longer chunks pass each child column once, shorter ones let more
  * workgroups share a CU. */
 #define SMFGPU_TUNE_ASM_CHUNK 9

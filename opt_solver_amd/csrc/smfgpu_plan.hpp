@@ -232,14 +232,14 @@ struct smfgpu_plan_s {
     std::vector<int32_t> fwd_lv, bwd_lv;
     int32_t panel_outer = kPanelOuter;             // SPD: pivot columns per trailing Schur update
     std::vector<int64_t> asm_first;                // SPD: first assembly task of each level
-    int32_t cb_seg = 2048;                         // SPD: pivot columns per contribution-block update
+    int32_t cb_seg = 4096;                         // SPD: pivot columns per contribution-block update
     int32_t ldlt_big_min = 256;                    // LDL^T: fronts with at least this many rows take the large-front path
     DBuf<BigLd> d_big;                             // LDL^T: large-front states of the level
     int32_t ldlt_group = 1;                        // LDL^T: speculative panel groups on the large-front path
     int32_t ldlt_update_wg = 1024;                 // LDL^T: workgroups of the large-front update launches
     DBuf<double> d_gpart;                          // LDL^T: per-workgroup column maxima of a group
     int32_t syrk_ahead = 2;                        // Schur update: chunks with global loads in flight (1 or 2)
-    int32_t asm_chunk = 16384;                     // SPD assembly: largest LDS column chunk (rows)
+    int32_t asm_chunk = 2048;                      // SPD assembly: largest LDS column chunk (rows)
     int32_t solve_fuse_rows = 512;                 // rows below a last block its own workgroup updates
     double t_solve_phase[4] = {0, 0, 0, 0};        // fwd ms, bwd ms, fwd launches, bwd launches
     // refinement CSR (full symmetric, permuted numbering)

@@ -191,7 +191,7 @@ def test_refactor_same_plan():
 
 def test_outer_panel_width_invariance():
     """The trailing-update width (64, 256, the default 512, 1024) and the
-    contribution-block segment (64, 100, unbounded, the default 2048)
+    contribution-block segment (64, 100, unbounded, the default 4096)
     change only the order of the Schur sums: factors agree to 1e-13
     relative."""
     print("case outer panel widths", flush=True)
@@ -205,7 +205,7 @@ def test_outer_panel_width_invariance():
         fac = plan.factor(A[3])
         assert block_rel_err(fac.export(), ref) < 1e-13, w
         assert np.abs(fac.solve(matvec(A, x)) - x).max() < 1e-12, w
-    for seg in (64, 100, 1 << 20, 2048):
+    for seg in (64, 100, 1 << 20, 4096):
         plan.tune(TUNE_CB_SEG, seg)
         fac = plan.factor(A[3])
         assert block_rel_err(fac.export(), ref) < 1e-13, seg
