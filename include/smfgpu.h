@@ -221,6 +221,14 @@ SMFGPU_API int smfgpu_export_factor(smfgpu_factor_t f, int64_t *sizes, int64_t *
  * Lets a caller that factors many independent systems as one forest report
  * each system's inertia and delays. */
 SMFGPU_API int smfgpu_factor_front_stats(smfgpu_factor_t f, int32_t *out);
+/* Blocked-path counters of the last factorization, 4 int32 per supernode:
+ * first-pass panels that stalled, partner pairs pulled in front of a
+ * stalled panel, failed columns rotated to the end of the live range, and
+ * second-pass full searches (the stall handling of ldlt_factor_front's
+ * blocked path, ldlt_kernel.cpp:1028-1131). No counterpart in the
+ * reference's Info: a diagnostic of how often a front took those branches.
+ * Zero for SPD plans and for fronts below the blocked size. */
+SMFGPU_API int smfgpu_factor_stall_stats(smfgpu_factor_t f, int32_t *out);
 /* Precision code of each supernode's Schur updates in the last
  * factorization (one int8 per supernode: SMFGPU_PREC_FP64 / _FP32 / _BF16 /
  * _BF16X3), as the static policy or the learned selector set them. */

@@ -78,7 +78,7 @@ EXPORTED = [
     "smfgpu_refine", "smfgpu_set_timing", "smfgpu_kernel_times", "smfgpu_plan_tune",
     "smfgpu_factor_front_stats", "smfgpu_launch_profile", "smfgpu_refine_default_options", "smfgpu_refine_ex",
     "smfgpu_solve_noalloc", "smfgpu_factor_precisions", "smfgpu_selector_set", "smfgpu_selector_pairs",
-    "smfgpu_selector_train", "smfgpu_selector_predict", "smfgpu_factor_solve",
+    "smfgpu_selector_train", "smfgpu_selector_predict", "smfgpu_factor_solve", "smfgpu_factor_stall_stats",
 ]
 SELECTOR_FEATURES, SELECTOR_OUTPUTS, SELECTOR_WEIGHTS = 10, 3, 1527
 STATIC_AS_ZERO = 1
@@ -141,6 +141,7 @@ def lib():
     L.smfgpu_kernel_times.argtypes = [_P, _P]
     L.smfgpu_plan_tune.argtypes = [_P, _I, _L]
     L.smfgpu_factor_front_stats.argtypes = [_P, _P]
+    L.smfgpu_factor_stall_stats.argtypes = [_P, _P]
     L.smfgpu_launch_profile.argtypes = [_P, _P, _L]
     L.smfgpu_launch_profile.restype = _L
     _lib = L

@@ -206,14 +206,26 @@ __device__ void select_pivot_wg(const FV &A, int k, int ne, int limit, const Prm
         const double a_cand = fabs(A.at(cand, cand));
         // largest entry over the eligible rows [k, ne) other than cand (first
         // index on ties), then the largest of the rest
-        double bv = -1.0;
+        // One sweep of the column: each thread keeps its largest entry (first
+        // index), the largest of its other entries, and the maximum over
+        // rows [ne, f). The thread holding the overall first argmax then
+        // contributes its second value to m2, every other thread its first.
+        const int ext0 = max(ne, cand + 1);
+        double bv = -1.0, b2 = 0.0, mel = 0.0;
         int bi = -1;
-        for (int i = k + tid; i < ne; i += NT) {
-            if (i == cand) continue;
-            const double v = fabs(A.sym(i, cand));
-            if (v > bv) {
-                bv = v;
-                bi = i;
+        for (int i = k + tid; i < f; i += NT) {
+            if (i < ne) {
+                if (i == cand) continue;
+                const double v = fabs(A.sym(i, cand));
+                if (v > bv) {
+                    b2 = fmax(b2, bv);
+                    bv = v;
+                    bi = i;
+                } else {
+                    b2 = fmax(b2, v);
+                }
+            } else if (i >= ext0) {
+                mel = fmax(mel, fabs(A.at(i, cand)));
             }
         }
         double m1;
@@ -223,13 +235,7 @@ __device__ void select_pivot_wg(const FV &A, int k, int ne, int limit, const Prm
             m1 = 0.0;
             r_best = -1;
         }
-        double m2l = 0.0;
-        for (int i = k + tid; i < ne; i += NT)
-            if (i != cand && i != r_best) m2l = fmax(m2l, fabs(A.sym(i, cand)));
-        const double m2 = red_max(rs, m2l);
-        const int ext0 = max(ne, cand + 1);
-        double mel = 0.0;
-        for (int i = ext0 + tid; i < f; i += NT) mel = fmax(mel, fabs(A.at(i, cand)));
+        const double m2 = red_max(rs, (r_best >= 0 && bi == r_best) ? b2 : fmax(bv, 0.0));
         const double m_ext = red_max(rs, mel);
         bool weak = false;
         double weak_growth = 0.0;
@@ -307,6 +313,50 @@ __device__ void select_pivot_wg(const FV &A, int k, int ne, int limit, const Prm
             return;
         }
     }
+}
+
+// Partner rows of a stalled first-pass panel (ldlt_kernel.cpp:1052-1075):
+// for each failed column pos of [k, pl) in order, the first row of [pl, ne)
+// holding the column's largest entry, kept when that entry exceeds the
+// column's largest entry inside the panel and the row is no earlier
+// column's partner. (column, partner) row ids go to order (at most 64
+// pairs, nb / 2 when smaller); returns the number of ids written. Every
+// thread returns the same count.
+template <int NT>
+__device__ int partner_pairs_wg(const FV &A, int k, int pl, int ne, int nb, const int32_t *rows, int32_t *order,
+                                RedScratchT<NT> &rs) {
+    const int tid = threadIdx.x;
+    int np = 0;
+    const int max_pairs = min(nb / 2, 64);
+    for (int pos = k; pos < pl && np / 2 < max_pairs; ++pos) {
+        double inl = 0.0;
+        for (int i = k + tid; i < pl; i += NT)
+            if (i != pos) inl = fmax(inl, fabs(A.sym(i, pos)));
+        const double in_max = red_max(rs, inl);
+        double bv = 0.0;
+        int bi = -1;
+        for (int i = pl + tid; i < ne; i += NT) {
+            const double v = fabs(A.at(i, pos));
+            if (v > bv) {
+                bv = v;
+                bi = i;
+            }
+        }
+        double out_max;
+        int r;
+        red_maxfirst(rs, bv, bi, out_max, r);
+        if (r < 0 || !(out_max > in_max)) continue;
+        bool taken = false;
+        for (int t = 1; t < np; t += 2) taken = taken || order[t] == rows[r];
+        if (taken) continue;
+        if (tid == 0) {
+            order[np] = rows[pos];
+            order[np + 1] = rows[r];
+        }
+        np += 2;
+        __syncthreads();
+    }
+    return np;
 }
 
 // Lanes per column when the workgroup sweeps the lower part of several
@@ -598,6 +648,7 @@ __global__ __launch_bounds__(kLT) void k_ldlt_small(LdltDev d, const int32_t *__
         Fd.flops = flops;
         Fd.nstatic = nstatic;
         Fd.growth = growth;
+        Fd.stalls = Fd.pulls = Fd.rotated = Fd.searches = 0;  // unblocked: no panels
     }
 }
 
@@ -664,7 +715,6 @@ __global__ __launch_bounds__(kLT) void k_ldlt_blocked(LdltDev d, const int32_t *
     __shared__ RedScratch rs;
     __shared__ int cnt[3];
     __shared__ int32_t order[128];
-    __shared__ int np_s;
     const int s = flist[blockIdx.x];
     FrontLd &Fd = d.fronts[s];
     const int f = Fd.f, ne = Fd.ne;
@@ -683,6 +733,7 @@ __global__ __launch_bounds__(kLT) void k_ldlt_blocked(LdltDev d, const int32_t *
     const Prm &prm_f = fp;
     int nstatic = 0;
     double growth = 0.0;
+    int n_stall = 0, n_pull = 0, n_rot = 0, n_search = 0;
     // the first pass with stall handling is for threshold pivoting only
     // (ldlt_kernel.cpp:1028); static fronts go straight to the second
     if (!(sp > 0.0) && !(fp.min_abs > 0.0)) {
@@ -701,36 +752,12 @@ __global__ __launch_bounds__(kLT) void k_ldlt_blocked(LdltDev d, const int32_t *
                 apply_pivot_wg(A, kind, pa, pb, k, rows, tags, W, wld, k - ps, pl, prm_f, cnt, flops, 0);
             }
             deferred_update_wg(A, ps, k, pl, ne, W, wld);
+            n_stall += stalled ? 1 : 0;
             if (stalled && k != k_paired) {
                 // (failed column, partner) row-id pairs, distinct partners
-                if (tid == 0) {
-                    int np = 0;
-                    const int max_pairs = min(nb / 2, 64);
-                    for (int pos = k; pos < pl && np / 2 < max_pairs; ++pos) {
-                        double in_max = 0.0;
-                        for (int i = k; i < pos; ++i) in_max = fmax(in_max, fabs(A.at(pos, i)));
-                        for (int i = pos + 1; i < pl; ++i) in_max = fmax(in_max, fabs(A.at(i, pos)));
-                        double out_max = 0.0;
-                        int r = -1;
-                        for (int i = pl; i < ne; ++i) {
-                            const double v = fabs(A.at(i, pos));
-                            if (v > out_max) {
-                                out_max = v;
-                                r = i;
-                            }
-                        }
-                        if (r < 0 || !(out_max > in_max)) continue;
-                        bool taken = false;
-                        for (int t = 1; t < np; t += 2) taken = taken || order[t] == rows[r];
-                        if (taken) continue;
-                        order[np++] = rows[pos];
-                        order[np++] = rows[r];
-                    }
-                    np_s = np;
-                }
-                __syncthreads();
-                const int np = np_s;
+                const int np = partner_pairs_wg(A, k, pl, ne, nb, rows, order, rs);
                 if (np > 0) {
+                    n_pull += np / 2;
                     for (int t = 0; t < np; ++t) {
                         int q = k + t;
                         while (rows[q] != order[t]) ++q;
@@ -746,6 +773,7 @@ __global__ __launch_bounds__(kLT) void k_ldlt_blocked(LdltDev d, const int32_t *
                 const int mv = min(m, live - pl);
                 for (int t = 0; t < mv; ++t) sym_swap_wg(A, k + t, live - mv + t, rows, nullptr, 0, 0, 0);
                 live -= m;
+                n_rot += m;
             }
         }
     }
@@ -764,6 +792,7 @@ __global__ __launch_bounds__(kLT) void k_ldlt_blocked(LdltDev d, const int32_t *
         }
         deferred_update_wg(A, ps, k, pl, ne, W, wld);
         if (stalled && k < ne) {
+            ++n_search;
             const int ss = k;
             int kind, pa, pb;
             select_pivot_wg(A, k, ne, ne, prm_f, rs, kind, pa, pb);
@@ -788,6 +817,10 @@ __global__ __launch_bounds__(kLT) void k_ldlt_blocked(LdltDev d, const int32_t *
         Fd.flops = flops;
         Fd.nstatic = nstatic;
         Fd.growth = growth;
+        Fd.stalls = n_stall;
+        Fd.pulls = n_pull;
+        Fd.rotated = n_rot;
+        Fd.searches = n_search;
     }
 }
 
@@ -1261,7 +1294,7 @@ __global__ __launch_bounds__(kLT) void k_big_panel(LdltDev d, BigLd *states, Prm
     __syncthreads();
     int u_c0 = 0, u_c1 = 0, u_p0 = 0, u_p1 = 0, x_c0 = ne, x_p0 = 0, x_p1 = 0, w_base = pend0;
     int stalled = 0, npass = pass, ps = k, pl = k;
-    int spec = B.spec, g_apply = 0, g_ps = 0, g_n = 0;
+    int spec = B.spec, g_apply = 0, g_ps = 0, g_n = 0, searched = 0;
     bool group_done = false;
     const int gcols = gpart ? group_cols(B, nb) : 0;
     if (gcols > 0) {
@@ -1385,6 +1418,7 @@ __global__ __launch_bounds__(kLT) void k_big_panel(LdltDev d, BigLd *states, Prm
         if (full) {
             // a stalled second-pass panel: one pivot from a search over all
             // the remaining columns (ldlt_kernel.cpp:1119-1131)
+            searched = 1;
             const int ss = k;
             int kind, pa, pb;
             select_pivot_wg(A, k, ne, ne, prm, rs, kind, pa, pb);
@@ -1447,6 +1481,7 @@ __global__ __launch_bounds__(kLT) void k_big_panel(LdltDev d, BigLd *states, Prm
         B.cnt[1] = cnt[1];
         B.cnt[2] = cnt[2];
         B.flops = flops;
+        B.searches += searched;
         B.spec = spec;
         B.g_apply = g_apply;
         B.g_ps = g_ps;
@@ -1477,8 +1512,8 @@ __global__ __launch_bounds__(kLT, 2) void k_big_update(LdltDev d, const BigLd *_
 // Partner pulls or rotation of a stalled first-pass panel
 // (ldlt_kernel.cpp:1052-1097); every column is current (flushed).
 __global__ __launch_bounds__(kLT) void k_big_stall(LdltDev d, BigLd *states, int32_t nb) {
+    __shared__ RedScratch rs;
     __shared__ int32_t order[128];
-    __shared__ int np_s;
     BigLd &B = states[blockIdx.x];
     if (B.pass != 0 || !B.stalled) return;
     FrontLd &Fd = d.fronts[B.s];
@@ -1489,33 +1524,7 @@ __global__ __launch_bounds__(kLT) void k_big_stall(LdltDev d, BigLd *states, int
     const int k = B.k, pl = B.pl;
     int live = B.live;
     if (k != B.k_paired) {
-        if (tid == 0) {
-            int np = 0;
-            const int max_pairs = min(nb / 2, 64);
-            for (int pos = k; pos < pl && np / 2 < max_pairs; ++pos) {
-                double in_max = 0.0;
-                for (int i = k; i < pos; ++i) in_max = fmax(in_max, fabs(A.at(pos, i)));
-                for (int i = pos + 1; i < pl; ++i) in_max = fmax(in_max, fabs(A.at(i, pos)));
-                double out_max = 0.0;
-                int r = -1;
-                for (int i = pl; i < ne; ++i) {
-                    const double v = fabs(A.at(i, pos));
-                    if (v > out_max) {
-                        out_max = v;
-                        r = i;
-                    }
-                }
-                if (r < 0 || !(out_max > in_max)) continue;
-                bool taken = false;
-                for (int t = 1; t < np; t += 2) taken = taken || order[t] == rows[r];
-                if (taken) continue;
-                order[np++] = rows[pos];
-                order[np++] = rows[r];
-            }
-            np_s = np;
-        }
-        __syncthreads();
-        const int np = np_s;
+        const int np = partner_pairs_wg(A, k, pl, ne, nb, rows, order, rs);
         if (np > 0) {
             for (int t = 0; t < np; ++t) {
                 int q = k + t;
@@ -1526,6 +1535,8 @@ __global__ __launch_bounds__(kLT) void k_big_stall(LdltDev d, BigLd *states, int
                 B.live = max(live, k + np);
                 B.k_paired = k;
                 B.stalled = 0;
+                B.stalls += 1;
+                B.pulls += np / 2;
             }
             return;
         }
@@ -1536,6 +1547,8 @@ __global__ __launch_bounds__(kLT) void k_big_stall(LdltDev d, BigLd *states, int
     if (tid == 0) {
         B.live = live - m;
         B.stalled = 0;
+        B.stalls += 1;
+        B.rotated += m;
     }
 }
 
@@ -1599,6 +1612,10 @@ __global__ void k_big_finish(LdltDev d, const BigLd *__restrict__ states, int32_
     F.nneg = B.cnt[1];
     F.nzero = B.cnt[2];
     F.flops = B.flops;
+    F.stalls = B.stalls;
+    F.pulls = B.pulls;
+    F.rotated = B.rotated;
+    F.searches = B.searches;
 }
 
 void launch_big_panel(hipStream_t st, const LdltDev &d, BigLd *states, int32_t nbig, double u, double eff_small,
@@ -1974,6 +1991,7 @@ __global__ __launch_bounds__(NT) void k_dev_fronts(LdltDev d, DevLevel L, const 
         Fd.flops = flops;
         Fd.nstatic = nstatic;
         Fd.growth = growth;
+        Fd.stalls = Fd.pulls = Fd.rotated = Fd.searches = 0;  // unblocked: no panels
     }
     __syncthreads();
     finish_front_wg<NT>(d, Fd, Fs, true);

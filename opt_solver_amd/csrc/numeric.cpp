@@ -1083,6 +1083,23 @@ SMFGPU_API int smfgpu_factor_front_stats(smfgpu_factor_t f, int32_t *out) {
     return SMFGPU_OK;
 }
 
+SMFGPU_API int smfgpu_factor_stall_stats(smfgpu_factor_t f, int32_t *out) {
+    if (!f || !out) return set_error(SMFGPU_ERR_INVALID_ARG, "null argument");
+    if (!f->plan) return set_error(SMFGPU_ERR_STATE, "stall stats: the factor's plan was freed");
+    smfgpu_plan_s *P = f->plan;
+    if (P->live != f) return set_error(SMFGPU_ERR_STATE, "stall stats: factor was superseded by a newer factorization");
+    for (int32_t s = 0; s < P->nsup; ++s) {
+        int32_t *o = out + 4 * static_cast<int64_t>(s);
+        if (P->posdef) {
+            o[0] = o[1] = o[2] = o[3] = 0;
+        } else {
+            const FrontLd &F = P->fl[s];
+            o[0] = F.stalls; o[1] = F.pulls; o[2] = F.rotated; o[3] = F.searches;
+        }
+    }
+    return SMFGPU_OK;
+}
+
 SMFGPU_API int smfgpu_export_factor(smfgpu_factor_t f, int64_t *sizes, int64_t *steps, double *factor_values,
                                     int32_t *row_indices, int8_t *pivot_types) {
     if (!f) return set_error(SMFGPU_ERR_INVALID_ARG, "null factor");

@@ -572,19 +572,33 @@ int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgp
     std::vector<int32_t> lrows;
     const int32_t nlev = static_cast<int32_t>(P->level_fronts.size());
     int rc;
-    int64_t negatives = 0;
     int32_t levels_done = 0, reduced = 0;
     bool inertia_exceeded = false, work_exceeded = false;
-    // work limit (multifrontal_serial.cpp:653-671, 1198-1205): the work of a
-    // front, num_elig * f^2, is known once its level is laid out; the first
-    // front that would take the total past floor(work_limit) is counted and
-    // the factorization stops before its level. Work only grows, so whether
-    // a run stops does not depend on the order the fronts are taken in.
+    // Early stops (multifrontal_serial.cpp:653-671, 1196-1205, 1612-1622).
+    // The reference takes the fronts one by one in supernode order (its
+    // postorder is the identity, symbolic_analysis.cpp:1200-1237): before a
+    // front, a work limit stops the run when the front's work num_elig * f^2
+    // would take the total past floor(work_limit) (that work is counted, the
+    // front is not factored); after it, max_negative_pivots stops the run
+    // once the negative pivots exceed it. Levels here finish fronts out of
+    // that order, so a cursor walks the supernodes in order through the
+    // finished ones and applies both tests there: the stopping front and the
+    // partial counts (the fronts before it, and for the inertia stop the
+    // front itself) are the reference's. A front's results do not depend on
+    // when it runs, only on its children.
     const bool work_armed = opt.work_limit_on != 0 && opt.work_limit >= 0.0;
+    const bool neg_armed = opt.max_negative_pivots >= 0;
     const int64_t work_cap = !work_armed ? -1
                              : (opt.work_limit >= 9.0e18 ? INT64_MAX
                                                          : static_cast<int64_t>(std::floor(opt.work_limit)));
-    int64_t work_done = 0;
+    std::vector<int64_t> fwork;  // per supernode once laid out, -1 before
+    std::vector<char> fdone;     // per supernode once factored (armed runs)
+    if (work_armed || neg_armed) {
+        fwork.assign(P->nsup, -1);
+        fdone.assign(P->nsup, 0);
+    }
+    int32_t cur = 0, stop_s = -1;
+    int64_t work_prefix = 0, neg_prefix = 0;
     P->rows_h.resize(0);
     P->dev_levels = 0;
     for (double &t : P->t_ldlt_host) t = 0.0;
@@ -658,11 +672,7 @@ int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgp
             } else {
                 small_list.push_back(s);
             }
-            if (work_armed && !work_exceeded) {
-                const int64_t w = static_cast<int64_t>(F.ne) * F.f * F.f;
-                if (work_done + w > work_cap) work_exceeded = true;
-                work_done += w;
-            }
+            if (!fwork.empty()) fwork[s] = static_cast<int64_t>(F.ne) * F.f * F.f;
             P->fac_used += static_cast<int64_t>(F.f) * F.ne;
             // large stored blocks get the many-workgroup store kernel
             if (static_cast<int64_t>(F.f) * F.ne >= kStoreWide) store_list.push_back(s);
@@ -717,7 +727,14 @@ int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgp
             ids.push_back(s);
             patch.push_back(F);
         }
-        if (work_exceeded) break;  // stopped before this level (nothing of it launched)
+        // the next front in order is in this level and its work crosses the
+        // limit: stop before it, with nothing of this level launched
+        if (work_armed && cur < P->nsup && !fdone[cur] && fwork[cur] >= 0 && work_prefix + fwork[cur] > work_cap) {
+            work_exceeded = true;
+            stop_s = cur;
+            work_prefix += fwork[cur];
+            break;
+        }
         if (cc.size() > static_cast<size_t>(INT32_MAX)) return set_error(SMFGPU_ERR_ALLOC, "factor: level too large");
         // capacities (arena contents stay: offsets are relative)
         if ((rc = ensure(P->d_arena, static_cast<size_t>(P->sq_pool.end), true, "front squares")) ||
@@ -789,19 +806,28 @@ int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgp
         HIP_OK(hipMemcpyAsync(P->rows_h.data() + rows_start, P->d_lrows.p + rows_start, lrows.size() * sizeof(int32_t),
                               hipMemcpyDeviceToHost, st));
         HIP_OK(hipStreamSynchronize(st));
-        for (size_t i = 0; i < fr.size(); ++i) {
-            P->fl[fr[i]] = back[i];
-            negatives += back[i].nneg;
-        }
+        for (size_t i = 0; i < fr.size(); ++i) P->fl[fr[i]] = back[i];
         levels_done = lv + 1;
-        // inertia limit (multifrontal_serial.cpp:1619-1627): the reference
-        // stops after the front that takes the negative pivots past the
-        // limit; here the check runs at level boundaries, so the status is
-        // the same (counts only grow) and the partial counts cover whole
-        // levels
-        if (opt.max_negative_pivots >= 0 && negatives > opt.max_negative_pivots) {
-            inertia_exceeded = true;
-            break;
+        if (!fdone.empty()) {
+            for (int32_t s : fr) fdone[s] = 1;
+            while (cur < P->nsup && fdone[cur]) {
+                if (work_armed && work_prefix + fwork[cur] > work_cap) {
+                    work_exceeded = true;
+                    work_prefix += fwork[cur];
+                    break;
+                }
+                work_prefix += fwork[cur];
+                neg_prefix += P->fl[cur].nneg;
+                if (neg_armed && neg_prefix > opt.max_negative_pivots) {
+                    inertia_exceeded = true;
+                    break;
+                }
+                ++cur;
+            }
+            if (work_exceeded || inertia_exceeded) {
+                stop_s = cur;
+                break;
+            }
         }
     }
     const double t_stats = wall_s();
@@ -815,15 +841,12 @@ int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgp
     fi.static_pivots = 0;
     fi.pivot_growth = 0.0;
     int64_t fent = 0;
+    // every front, or those the reference factors before its stop (the
+    // stopping front too after an inertia stop), in supernode order
     std::vector<int32_t> done_fronts;
-    if (levels_done == nlev) {
-        // every front: in supernode order (sequential over the front array)
-        done_fronts.resize(P->fronts.size());
-        for (size_t s = 0; s < done_fronts.size(); ++s) done_fronts[s] = static_cast<int32_t>(s);
-    } else {
-        for (int32_t lv = 0; lv < levels_done; ++lv)
-            done_fronts.insert(done_fronts.end(), P->level_fronts[lv].begin(), P->level_fronts[lv].end());
-    }
+    const int32_t n_done = stop_s < 0 ? P->nsup : (inertia_exceeded ? stop_s + 1 : stop_s);
+    done_fronts.resize(n_done);
+    for (int32_t s = 0; s < n_done; ++s) done_fronts[s] = s;
     for (int32_t s : done_fronts) {
         const FrontLd &F = P->fl[s];
         fi.num_positive += F.npos;
@@ -848,7 +871,7 @@ int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgp
         // factor_indef.cpp:655-680: stopped at the scale-on-delay threshold,
         // the factor is unusable; the front work counts the stopping front
         fi.status = SMFGPU_STATUS_MAX_PIVOT_DELAYS;
-        fi.front_work = static_cast<double>(work_done);
+        fi.front_work = static_cast<double>(work_prefix);
         return SMFGPU_OK;
     }
     fi.status = fi.num_zero > 0 ? SMFGPU_STATUS_SINGULAR : SMFGPU_STATUS_SUCCESS;

@@ -24,6 +24,9 @@ struct FrontLd {
     int32_t delayed;
     int32_t dev;              // 1: laid out on the device (its square is not in the host's pool)
     int32_t nstatic;          // pivots whose |d| static pivoting raised
+    // blocked fronts: first-pass panels that stalled, partner pairs pulled,
+    // failed columns rotated to the end, second-pass full searches
+    int32_t stalls, pulls, rotated, searches;
     double growth;            // largest column max / |d| over its static pivots
     double flops;
     int64_t fofs;             // arena offset of the square
@@ -75,6 +78,7 @@ struct BigLd {
     int32_t x_c0, x_p0, x_p1;        // flush request: columns [x_c0, ne), rows [x_c0, f), pivots [x_p0, x_p1)
     int32_t w_base;             // pivot of W column 0 for both requests
     int32_t cnt[3];
+    int32_t stalls, pulls, rotated, searches;  // as FrontLd's
     int32_t spec;               // 1: the next first-pass iteration tries a speculative group
     int32_t g_apply, g_ps, g_n; // accepted group to write: columns [g_ps, g_ps + g_n)
     int32_t prec;               // precision code of the MFMA updates: 0 FP64, 1 FP32, 2 BF16, 3 split BF16

@@ -283,6 +283,13 @@ class Factor:
         check(lib().smfgpu_factor_front_stats(self.h, _p(out)), "front_stats")
         return out
 
+    def stall_stats(self):
+        """Per supernode: stalled first-pass panels, partner pairs pulled,
+        columns rotated, second-pass full searches (blocked fronts)."""
+        out = np.zeros((self.plan.nsup, 4), np.int32)
+        check(lib().smfgpu_factor_stall_stats(self.h, _p(out)), "stall_stats")
+        return out
+
     def solve_device(self, ptr, nrhs, ldb=None, job=SOLVE_FULL):
         check(lib().smfgpu_solve(self.h, ctypes.c_void_p(ptr), nrhs, ldb or self.plan.n, int(job), MEM_DEVICE),
               "solve")

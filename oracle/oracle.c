@@ -40,10 +40,15 @@ typedef struct {
     int nb, posdef;
 } orc_params;
 
+/* stalls / pulls / rotated / searches: instrumentation of the blocked
+   path with no counterpart in the reference's Info: first-pass panels that
+   stalled, partner pairs pulled in front of a stalled panel, failed columns
+   rotated to the end of the live range, and second-pass full searches. */
 typedef struct {
     int p;
     int pos, neg, zero;
     double flops;
+    int stalls, pulls, rotated, searches;
 } orc_result;
 
 static int is_fin(double v) { return isfinite(v); }
@@ -425,6 +430,7 @@ int orc_ldlt_front(double *F, int f, int num_elig, int32_t *rows, const orc_para
                 apply_pivot(kind, pa, pb, F, f, &k, rows, tags, W, k - ps, pl, prm, res);
             }
             deferred_update(F, f, ps, k, pl, num_elig, W);
+            res->stalls += stalled;
             if (stalled && k != k_paired) {
                 int32_t order[128];
                 int np = 0;
@@ -451,6 +457,7 @@ int orc_ldlt_front(double *F, int f, int num_elig, int32_t *rows, const orc_para
                     order[np++] = rows[r];
                 }
                 if (np > 0) {
+                    res->pulls += np / 2;
                     for (int t = 0; t < np; ++t) {
                         int q = k + t;
                         while (rows[q] != order[t]) ++q;
@@ -466,6 +473,7 @@ int orc_ldlt_front(double *F, int f, int num_elig, int32_t *rows, const orc_para
                 const int mv = m < live - pl ? m : live - pl;
                 for (int t = 0; t < mv; ++t) sym_swap(F, sf, f, k + t, live - mv + t, rows, NULL, 0);
                 live -= m;
+                res->rotated += m;
             }
         }
     }
@@ -484,6 +492,7 @@ int orc_ldlt_front(double *F, int f, int num_elig, int32_t *rows, const orc_para
         }
         deferred_update(F, f, ps, k, pl, num_elig, W);
         if (stalled && k < num_elig) {
+            ++res->searches;
             const int ss = k;
             int kind, pa, pb;
             select_pivot(F, sf, f, k, num_elig, num_elig, prm, &kind, &pa, &pb);
@@ -520,6 +529,7 @@ typedef struct {
     int8_t *tags;        /* per stored pivot, in step order */
     int64_t ntags;
     int32_t posdef;
+    int32_t stalls, pulls, rotated, searches; /* sums of the fronts' orc_result counters */
 } orc_factor;
 
 void orc_factor_free(orc_factor *o) {
@@ -639,6 +649,10 @@ int orc_factor_run(const orc_analysis *a, const double *values, double u, double
         out->neg += kr.neg;
         out->zero += kr.zero;
         out->flops += kr.flops;
+        out->stalls += kr.stalls;
+        out->pulls += kr.pulls;
+        out->rotated += kr.rotated;
+        out->searches += kr.searches;
         out->front_work += (double)num_elig * (double)f * (double)f;
         if (f > out->max_front) out->max_front = f;
         const int is_root = a->sn_parent[s] == -1;

@@ -21,7 +21,8 @@ class _Params(ctypes.Structure):
 
 class _Result(ctypes.Structure):
     _fields_ = [("p", ctypes.c_int), ("pos", ctypes.c_int), ("neg", ctypes.c_int), ("zero", ctypes.c_int),
-                ("flops", ctypes.c_double)]
+                ("flops", ctypes.c_double), ("stalls", ctypes.c_int), ("pulls", ctypes.c_int),
+                ("rotated", ctypes.c_int), ("searches", ctypes.c_int)]
 
 
 class _Analysis(ctypes.Structure):
@@ -37,7 +38,9 @@ class _Factor(ctypes.Structure):
                 ("delayed", ctypes.c_int32), ("two_by_two", ctypes.c_int32), ("max_front", ctypes.c_int32),
                 ("flops", ctypes.c_double), ("front_work", ctypes.c_double), ("max_abs", ctypes.c_double),
                 ("steps", _P), ("L", _P), ("nL", ctypes.c_int64), ("nrows", ctypes.c_int64), ("rows", _P),
-                ("tags", _P), ("ntags", ctypes.c_int64), ("posdef", ctypes.c_int32)]
+                ("tags", _P), ("ntags", ctypes.c_int64), ("posdef", ctypes.c_int32),
+                ("stalls", ctypes.c_int32), ("pulls", ctypes.c_int32), ("rotated", ctypes.c_int32),
+                ("searches", ctypes.c_int32)]
 
 
 def available():
@@ -72,7 +75,8 @@ def ldlt_front(F, num_elig, u=0.01, eff_small=1e-20, small_pivot=1e-20, nb=16, p
     prm = _Params(u, eff_small, small_pivot, nb, 1 if posdef else 0)
     res = _Result()
     lib().orc_ldlt_front(_p(Fc), f, num_elig, _p(r), ctypes.byref(prm), _p(tags), ctypes.byref(res))
-    return Fc, r, tags[:f], dict(p=res.p, pos=res.pos, neg=res.neg, zero=res.zero, flops=res.flops)
+    return Fc, r, tags[:f], dict(p=res.p, pos=res.pos, neg=res.neg, zero=res.zero, flops=res.flops,
+                                 stalls=res.stalls, pulls=res.pulls, rotated=res.rotated, searches=res.searches)
 
 
 class Factor:
@@ -100,6 +104,7 @@ class Factor:
         self.two_by_two = int(f.two_by_two)
         self.flops = float(f.flops)
         self.max_front = int(f.max_front)
+        self.stall_stats = (int(f.stalls), int(f.pulls), int(f.rotated), int(f.searches))
 
     def export(self):
         f = self.f

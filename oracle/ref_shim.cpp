@@ -9,9 +9,11 @@
 //   AnalysisKeep fields                      solver/include/smf/analysis.hpp:25-144
 //   FactorKeep layout                        solver/include/smf/factor_posdef.hpp:16-220
 //   ldlt_factor_front                        solver/include/smf/ldlt_kernel.hpp:66-70
+//   factor_indef (work_limit)                solver/include/smf/factor_indef.hpp:28-39
 //   posdef_factor_front_lapack               solver/include/smf/ldlt_kernel.hpp:95
 #include "smf/analysis.hpp"
 #include "smf/control.hpp"
+#include "smf/factor_indef.hpp"
 #include "smf/factor_posdef.hpp"
 #include "smf/info.hpp"
 #include "smf/ldlt_kernel.hpp"
@@ -178,6 +180,42 @@ void *ref_factor_ex(void *handle, const double *values, int64_t nvalues, double 
     const double t0 = now_s();
     const FactorStatus st = s.factor(*an->ak, std::span<const double>(values, static_cast<std::size_t>(nvalues)),
                                      h->ctrl, h->info, h->fk, h->ws);
+    const double t1 = now_s();
+    if (stats) {
+        stats[0] = static_cast<double>(st);
+        stats[1] = static_cast<double>(h->info.status);
+        stats[2] = h->info.num_positive;
+        stats[3] = h->info.num_negative;
+        stats[4] = h->info.num_zero;
+        stats[5] = h->info.delayed_pivots;
+        stats[6] = h->info.two_by_two_pivots;
+        stats[7] = h->info.actual_flops;
+        stats[8] = h->info.front_work;
+        stats[9] = t1 - t0;
+        stats[10] = h->info.static_pivots;
+        stats[11] = h->info.max_front_size;
+        stats[12] = h->info.pivot_growth;
+    }
+    return h;
+}
+
+// The early stops on the values the analysis was built from:
+// Control::max_negative_pivots (-1 off) and factor_indef's work_limit
+// argument (factor_indef.hpp:28-39, -1 off), one thread, no scaling. stats
+// as ref_factor_ex ([1] ErrorCode, [0] FactorStatus: 3 MaxPivotDelays,
+// 4 InertiaExceeded; the partial counts of a stopped run).
+void *ref_factor_limits(void *handle, double u, int max_negative_pivots, double work_limit, double *stats) {
+    auto *an = static_cast<RefAnalysis *>(handle);
+    auto *h = new RefFactor;
+    h->an = an;
+    h->ctrl = base_control(an->posdef, 1);
+    h->ctrl.pivot_u = u;
+    h->ctrl.max_negative_pivots = max_negative_pivots;
+    h->ctrl.scaling = ScalingMethod::None;
+    h->ctrl.scale_on_delay_ratio = -1.0;
+    h->ctrl.matching_trigger = MatchingTrigger::Predicted;
+    const double t0 = now_s();
+    const FactorStatus st = factor_indef(*an->ak, h->ctrl, h->info, h->fk, work_limit);
     const double t1 = now_s();
     if (stats) {
         stats[0] = static_cast<double>(st);

@@ -42,6 +42,8 @@ def lib():
         L.ref_factor.argtypes = [_P, _P, _L, _D, _I, _P]
         L.ref_factor_ex.restype = _P
         L.ref_factor_ex.argtypes = [_P, _P, _L, _D, _I, _D, _D, _I, _P]
+        L.ref_factor_limits.restype = _P
+        L.ref_factor_limits.argtypes = [_P, _D, _I, _D, _P]
         L.ref_factor_sizes.argtypes = [_P, _P]
         L.ref_factor_export.argtypes = [_P, _P, _P, _P, _P, _P]
         L.ref_pivot_types_size.restype = _L
@@ -120,12 +122,18 @@ class RefFactor:
     STATUS = {0: "Success", 1: "NotPositiveDefinite", 2: "Singular", 3: "MaxPivotDelays", 4: "InertiaExceeded"}
 
     def __init__(self, an: RefAnalysis, values, u=0.01, threads=1, static_pivot_tol=0.0,
-                 fallback_static_pivot_tol=0.0, static_flags=0):
+                 fallback_static_pivot_tol=0.0, static_flags=0, max_negative_pivots=None, work_limit=None):
+        """max_negative_pivots / work_limit: the early stops, on the values the
+        analysis was built from (values must be those)."""
         self.an = an
         vals = np.ascontiguousarray(values, np.float64)
         st = np.zeros(16)
-        self.h = lib().ref_factor_ex(an.h, _ptr(vals), vals.size, u, threads, static_pivot_tol,
-                                     fallback_static_pivot_tol, static_flags, _ptr(st))
+        if max_negative_pivots is not None or work_limit is not None:
+            self.h = lib().ref_factor_limits(an.h, u, -1 if max_negative_pivots is None else int(max_negative_pivots),
+                                             -1.0 if work_limit is None else float(work_limit), _ptr(st))
+        else:
+            self.h = lib().ref_factor_ex(an.h, _ptr(vals), vals.size, u, threads, static_pivot_tol,
+                                         fallback_static_pivot_tol, static_flags, _ptr(st))
         self.pivot_growth = float(st[12])
         self.status = int(st[0])
         self.error = int(st[1])

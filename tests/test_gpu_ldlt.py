@@ -175,8 +175,11 @@ def test_large_front_path_against_oracle(oracle_lib, case, mode):
     orc = oracle_lib.Factor(arrays, A[3], posdef=False)
     assert fac.inertia == orc.inertia == inertia
     assert fac.info.delayed_pivots == orc.delayed and fac.info.two_by_two_pivots == orc.two_by_two
+    # stalled panels, partner pulls, rotated columns, second-pass searches
+    stalls = tuple(int(v) for v in fac.stall_stats().sum(axis=0))
+    assert stalls == orc.stall_stats, (stalls, orc.stall_stats)
     err = compare_factor(fac.export(), orc.export(), 1e-12)
-    print(f"  L/D block rel err {err:.1e}", flush=True)
+    print(f"  L/D block rel err {err:.1e} stall counters {stalls}", flush=True)
     x = np.random.default_rng(7).uniform(-1, 1, A[0])
     b = matvec(A, x)
     xs = fac.solve(b)
@@ -253,6 +256,84 @@ def test_mixed_precision_ldlt(oracle_lib, case, prec):
     eta, _ = oracle_lib.backward_error(A, b, xr)
     print(f"  reduced {fac.info.fronts_reduced} refine steps {st['steps']} eta {eta:.2e}", flush=True)
     assert eta <= 1e-12, eta
+
+
+@pytest.mark.parametrize("case", ["ocp_T60", "ocp_T200", "kkt_30", "kkt_48"])
+def test_tiny_u_against_reference(oracle_lib, case):
+    """IPOPT's pivot tolerance u = 1e-8 on the reference's own analysis: the
+    GPU factor against the compiled reference's (steps, row order, pivot
+    types, inertia, delays, 2x2 pivots; L and D within 1e-12 per block) and
+    against the oracle's."""
+    from oracle import refshim
+    if not refshim.available():
+        pytest.skip("reference library not built")
+    print("case tiny u sparse", case, flush=True)
+    if case.startswith("ocp"):
+        T = int(case[5:])
+        A = g.ocp_kkt(T, seed=T)
+        order = refshim.ORDER_AMD
+    else:
+        gsz = int(case[4:])
+        A, _ = g.kkt_grid(gsz, gsz * gsz // 5, seed=gsz)
+        order = refshim.ORDER_METIS
+    an = refshim.RefAnalysis(A, ordering=order, posdef=False)
+    arrays = an.plan_arrays()
+    ref = refshim.RefFactor(an, A[3], u=1e-8)
+    plan = Plan(arrays, posdef=False)
+    fac = plan.factor(A[3], u=1e-8)
+    orc = oracle_lib.Factor(arrays, A[3], posdef=False, u=1e-8)
+    assert fac.status == ref.status == 0
+    assert fac.inertia == ref.inertia == orc.inertia
+    assert fac.info.delayed_pivots == ref.delayed == orc.delayed
+    assert fac.info.two_by_two_pivots == ref.two_by_two == orc.two_by_two
+    ex = fac.export()
+    e_ref = compare_factor(ex, ref.export(), 1e-12)
+    e_orc = compare_factor(ex, orc.export(), 1e-12)
+    print(f"  delays {ref.delayed} 2x2 {ref.two_by_two} L/D vs reference {e_ref:.1e} vs oracle {e_orc:.1e}",
+          flush=True)
+
+
+def test_limits_against_reference():
+    """max_negative_pivots and the work limit against the compiled
+    reference's own stopped runs (one thread, serial postorder): the same
+    status and, for every limit, the partial counts the reference reports
+    (inertia for an inertia stop; delays, 2x2 pivots, front work and flops
+    for both)."""
+    from oracle import refshim
+    from opt_solver_amd._lib import STATUS_INERTIA_EXCEEDED, STATUS_MAX_PIVOT_DELAYS
+    if not refshim.available():
+        pytest.skip("reference library not built")
+    print("case limits vs reference", flush=True)
+    T = 200
+    A = g.ocp_kkt(T, seed=T)
+    an = refshim.RefAnalysis(A, ordering=refshim.ORDER_AMD, posdef=False)
+    assert np.array_equal(an.postorder, np.arange(an.nsup))  # supernode order is the reference's order
+    plan = Plan(an.plan_arrays(), posdef=False)
+    full = refshim.RefFactor(an, A[3], u=0.01)
+    total = full.front_work
+    outcomes = set()
+    for lim in (12 * T, 12 * T - 1, 6 * T, 100, 0):
+        ref = refshim.RefFactor(an, A[3], u=0.01, max_negative_pivots=lim)
+        fac = plan.factor(A[3], u=0.01, max_negative_pivots=lim)
+        print(f"  negatives <= {lim}: status {fac.status} / {ref.status} inertia {fac.inertia} / {ref.inertia}",
+              flush=True)
+        assert fac.status == ref.status
+        assert fac.inertia == ref.inertia
+        assert fac.info.delayed_pivots == ref.delayed and fac.info.two_by_two_pivots == ref.two_by_two
+        assert fac.info.front_work == ref.front_work and fac.flops == ref.flops
+        outcomes.add(fac.status)
+    for lim in (total + 0.5, total - 1, total / 2, 1000.0, 0.0):
+        ref = refshim.RefFactor(an, A[3], u=0.01, work_limit=lim)
+        fac = plan.factor(A[3], u=0.01, work_limit=lim)
+        print(f"  work <= {lim}: status {fac.status} / {ref.status} work {fac.info.front_work} / {ref.front_work}",
+              flush=True)
+        assert fac.status == ref.status
+        assert fac.info.delayed_pivots == ref.delayed and fac.info.two_by_two_pivots == ref.two_by_two
+        assert fac.info.front_work == ref.front_work and fac.flops == ref.flops
+        if fac.status == 0:
+            assert fac.inertia == ref.inertia
+        outcomes.add(fac.status)
+    assert {0, STATUS_INERTIA_EXCEEDED, STATUS_MAX_PIVOT_DELAYS} <= outcomes
 
 
 def test_work_limit(oracle_lib):

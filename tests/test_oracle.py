@@ -134,6 +134,63 @@ def test_stalled_panel_pulls_partners(oracle_lib):
     assert (st["pos"], st["neg"], st["zero"]) == (int((ev > 0).sum()), int((ev < 0).sum()), 0)
 
 
+def test_stall_counters_by_construction(oracle_lib):
+    """The blocked path's counters against what a front's structure fixes:
+    a diagonally dominant front never stalls; in the far-partner front every
+    stalled panel is rescued by pulls (no rotation, no second-pass search);
+    columns with no entry inside the eligible block have no partner, so
+    each is rotated and finally delayed after a full search."""
+    rng = np.random.default_rng(11)
+    D = _random_front(rng, 160) + 400.0 * np.eye(160)
+    _, _, _, st = oracle_lib.ldlt_front(_lower(D), 160, nb=16)
+    assert (st["stalls"], st["pulls"], st["rotated"], st["searches"]) == (0, 0, 0, 0)
+    m, nv, q, kfar = 40, 120, 40, 100
+    e = m + nv
+    rng = np.random.default_rng(5)
+    A = np.zeros((e + q, e + q))
+    B = rng.standard_normal((nv, nv))
+    A[m:e, m:e] = B @ B.T + nv * np.eye(nv)
+    for c in range(m):
+        v = m + ((c + kfar) % nv)
+        A[v, c] = A[c, v] = 1.0 + rng.uniform()
+    A[e:, e:] = np.eye(q)
+    _, _, _, st = oracle_lib.ldlt_front(_lower(A), e, nb=16)
+    assert st["p"] == e and st["stalls"] > 0 and st["pulls"] > 0
+    assert st["rotated"] == 0 and st["searches"] == 0
+    f, ne = 200, 150
+    rng = np.random.default_rng(3)
+    M = rng.standard_normal((f, f))
+    M = M + M.T
+    M[np.arange(f), np.arange(f)] = rng.uniform(5, 10, f)
+    z = list(range(0, ne, 7))
+    for c in z:
+        M[c, :ne] = 0.0
+        M[:ne, c] = 0.0
+    _, rows, _, st = oracle_lib.ldlt_front(_lower(M), ne, nb=16)
+    assert st["p"] == ne - len(z) and sorted(rows[st["p"]:ne]) == z
+    assert st["pulls"] == 0 and st["rotated"] >= len(z) and st["searches"] >= 1
+
+
+@pytest.mark.parametrize("f,ne", [(20, 12), (60, 60), (130, 90), (200, 150)])
+def test_dense_front_problem(oracle_lib, f, ne):
+    """tests.helpers.dense_front_problem: the multifrontal oracle on the
+    one- or two-supernode problem stores exactly what the front kernel
+    leaves in the first front (rows, pivot types, L and D bitwise)."""
+    from tests.helpers import dense_front_problem, stored_columns
+    M = _random_front(np.random.default_rng(f), f)
+    A, arrays = dense_front_problem(M, ne)
+    ex = oracle_lib.Factor(arrays, A[3], posdef=False).export()
+    Fo, rows, tags, st = oracle_lib.ldlt_front(_lower(M), ne, nb=16)
+    ro, fo, p, ff = ex["steps"][0]
+    p_store = ne if ne == f else st["p"]
+    tags[st["p"]:p_store] = 0
+    assert ff == f and p == p_store
+    assert np.array_equal(ex["row_indices"][:f], rows)
+    assert np.array_equal(ex["pivot_types"][:p], tags[:p])
+    L = ex["factor_values"][fo:fo + f * p].reshape(p, f).T
+    assert np.array_equal(L, stored_columns(Fo, tags, p_store))
+
+
 def test_backward_error_closed_form(oracle_lib):
     # A = [[2, 1], [1, 3]] (lower CSC), x = [1, 1], b = A x + [1e-3, 0]
     A = (2, np.array([0, 2, 3], np.int32), np.array([0, 1, 1], np.int32), np.array([2.0, 1.0, 3.0]))
