@@ -585,9 +585,27 @@ __global__ __launch_bounds__(kThreads) void k_syrk_red(DevPtrs d, const SyrkItem
     syrk_tile<MODE, kAhead>(d, it, smem);
 }
 
+// all three reduced precisions in one kernel, for the learned selector's
+// runs (the precisions are chosen on the device, so one launch covers
+// whichever codes the fronts got): LDS and registers of the largest
+template <int MODE>
+struct RedMax {
+    static constexpr int a = RedLds<1>::bytes, b = RedLds<2>::bytes, c = RedLds<3>::bytes;
+    static constexpr int bytes = a > b ? (a > c ? a : c) : (b > c ? b : c);
+};
+__global__ __launch_bounds__(kThreads) void k_syrk_red_any(DevPtrs d, const SyrkItem *__restrict__ items) {
+    __shared__ __attribute__((aligned(16))) unsigned char smem[RedMax<0>::bytes];
+    const SyrkItem it = items[xcd_contiguous(blockIdx.x, gridDim.x)];
+    const int pc = d.prec == nullptr ? 0 : d.prec[it.s];
+    if (pc == 1) syrk_tile<1, 1>(d, it, smem);
+    else if (pc == 2) syrk_tile<2, 1>(d, it, smem);
+    else if (pc == 3) syrk_tile<3, 1>(d, it, smem);
+}
+
 // n items; the reduced kernels cover the last n_red of them (the items of
 // fronts that may run below FP64), one launch per precision code in
-// red_modes (bit c: code c)
+// red_modes (bit c: code c), or one launch of the combined kernel when
+// several codes are possible
 void launch_syrk(hipStream_t st, const DevPtrs &d, const SyrkItem *it, int64_t n, int64_t n_red, int ahead,
                  int red_modes) {
     if (n <= 0) return;
@@ -599,6 +617,10 @@ void launch_syrk(hipStream_t st, const DevPtrs &d, const SyrkItem *it, int64_t n
     const SyrkItem *ir = it + (n - n_red);
     for (int64_t o = 0; o < n_red; o += kMaxLaunchGroups) {
         const dim3 grid(static_cast<unsigned>(std::min(kMaxLaunchGroups, n_red - o)));
+        if (__builtin_popcount(red_modes & 14) > 1) {
+            k_syrk_red_any<<<grid, kThreads, 0, st>>>(d, ir + o);
+            continue;
+        }
         if (red_modes & 2) {
             if (ahead == 1) k_syrk_red<1, 1><<<grid, kThreads, 0, st>>>(d, ir + o);
             else k_syrk_red<1, 2><<<grid, kThreads, 0, st>>>(d, ir + o);

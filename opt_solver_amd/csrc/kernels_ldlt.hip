@@ -142,6 +142,97 @@ struct FV {
     __device__ __forceinline__ double sym(int i, int j) const { return i >= j ? at(i, j) : at(j, i); }
 };
 
+// Column sweeps keep kU independent loads in flight per thread: the loads
+// of a group are issued from clamped (in-range) addresses before any use,
+// and only the in-range elements are used or stored. Updates read and
+// write different columns, which the compiler cannot tell apart, so
+// without the explicit grouping every element waits out a full memory
+// latency.
+constexpr int kU = 4;
+
+// f(i, v) for v = ld(i), i = lo, lo + st, ... < hi, in increasing i per
+// thread
+template <class Ld, class Fn>
+__device__ __forceinline__ void sweep(int lo, int hi, int st, Ld &&ld, Fn &&fn) {
+    for (int i0 = lo; i0 < hi; i0 += kU * st) {
+        double v[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) v[u] = ld(min(i0 + u * st, hi - 1));
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+            if (i0 + u * st < hi) fn(i0 + u * st, v[u]);
+    }
+}
+
+// c[i] = c[i] - a[i] * l (multiply, then subtract), i = lo, lo + st, ... < hi;
+// c and a are different columns
+__device__ __forceinline__ void col_axpy(double *c, const double *a, double l, int lo, int hi, int st) {
+    for (int i0 = lo; i0 < hi; i0 += kU * st) {
+        double cv[kU], av[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int i = min(i0 + u * st, hi - 1);
+            cv[u] = c[i];
+            av[u] = a[i];
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+            if (i0 + u * st < hi) c[i0 + u * st] = __dsub_rn(cv[u], __dmul_rn(av[u], l));
+    }
+}
+
+// c[i] = c[i] - (a0[i] * l0 + a1[i] * l1)
+__device__ __forceinline__ void col_axpy2(double *c, const double *a0, const double *a1, double l0, double l1, int lo,
+                                          int hi, int st) {
+    for (int i0 = lo; i0 < hi; i0 += kU * st) {
+        double cv[kU], x0[kU], x1[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int i = min(i0 + u * st, hi - 1);
+            cv[u] = c[i];
+            x0[u] = a0[i];
+            x1[u] = a1[i];
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+            if (i0 + u * st < hi)
+                c[i0 + u * st] = __dsub_rn(cv[u], __dadd_rn(__dmul_rn(x0[u], l0), __dmul_rn(x1[u], l1)));
+    }
+}
+
+// dst[i] = src[i] (different arrays)
+__device__ __forceinline__ void col_copy(double *dst, const double *src, int lo, int hi, int st) {
+    for (int i0 = lo; i0 < hi; i0 += kU * st) {
+        double v[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) v[u] = src[min(i0 + u * st, hi - 1)];
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+            if (i0 + u * st < hi) dst[i0 + u * st] = v[u];
+    }
+}
+
+// exchange a[i] and b[i] (different columns, or a row and a column that
+// share no element over the range)
+template <class PA, class PB>
+__device__ __forceinline__ void swap_range(PA &&pa, PB &&pb, int lo, int hi, int st) {
+    for (int i0 = lo; i0 < hi; i0 += kU * st) {
+        double x[kU], y[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int i = min(i0 + u * st, hi - 1);
+            x[u] = *pa(i);
+            y[u] = *pb(i);
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+            if (i0 + u * st < hi) {
+                *pa(i0 + u * st) = y[u];
+                *pb(i0 + u * st) = x[u];
+            }
+    }
+}
+
 // min_abs: the front's in-order static pivot (static_pivot_in_order, 0 off);
 // S: static pivoting of the launch (fronts pick theirs with
 // front_static_value)
@@ -193,7 +284,8 @@ __device__ void select_pivot_wg(const FV &A, int k, int ne, int limit, const Prm
         const double a0 = fabs(A.at(k, k));
         if (a0 >= prm.eff_small) {
             double m = 0.0;
-            for (int i = k + 1 + tid; i < f; i += NT) m = fmax(m, fabs(A.at(i, k)));
+            const double *ck = &A.at(0, k);
+            sweep(k + 1 + tid, f, NT, [&](int i) { return ck[i]; }, [&](int, double v) { m = fmax(m, fabs(v)); });
             const double colmax = red_max(rs, m);
             if (!(prm.u * colmax > a0) && !(a0 < kWeakPivotRatio * colmax)) {
                 kind = 1;
@@ -213,20 +305,28 @@ __device__ void select_pivot_wg(const FV &A, int k, int ne, int limit, const Prm
         const int ext0 = max(ne, cand + 1);
         double bv = -1.0, b2 = 0.0, mel = 0.0;
         int bi = -1;
-        for (int i = k + tid; i < f; i += NT) {
-            if (i < ne) {
-                if (i == cand) continue;
-                const double v = fabs(A.sym(i, cand));
-                if (v > bv) {
-                    b2 = fmax(b2, bv);
-                    bv = v;
-                    bi = i;
-                } else {
-                    b2 = fmax(b2, v);
-                }
-            } else if (i >= ext0) {
-                mel = fmax(mel, fabs(A.at(i, cand)));
+        const auto top2 = [&](int i, double x) {
+            const double v = fabs(x);
+            if (v > bv) {
+                b2 = fmax(b2, bv);
+                bv = v;
+                bi = i;
+            } else {
+                b2 = fmax(b2, v);
             }
+        };
+        // rows [k, cand) of the candidate's row, then rows (cand, ne) and
+        // [ext0, f) of its column: each thread still meets its indices in
+        // increasing order
+        const double *cc = &A.at(0, cand);
+        const int k1 = k + tid;
+        sweep(k1, min(cand, ne), NT, [&](int i) { return A.at(cand, i); }, top2);
+        {
+            // first index of this thread's sequence past cand
+            const int c1 = cand + 1 + (((k1 - (cand + 1)) % NT) + NT) % NT;
+            sweep(c1, ne, NT, [&](int i) { return cc[i]; }, top2);
+            const int e1 = ext0 + (((k1 - ext0) % NT) + NT) % NT;
+            sweep(e1, f, NT, [&](int i) { return cc[i]; }, [&](int, double x) { mel = fmax(mel, fabs(x)); });
         }
         double m1;
         int r_best;
@@ -261,12 +361,14 @@ __device__ void select_pivot_wg(const FV &A, int k, int ne, int limit, const Prm
         }
         // partner column: max excluding r_best, and excluding cand too
         double o2 = 0.0, o1 = 0.0;
-        for (int i = k + tid; i < f; i += NT) {
-            if (i == r_best) continue;
-            const double v = fabs(A.sym(i, r_best));
+        const auto part = [&](int i, double x) {
+            const double v = fabs(x);
             o2 = fmax(o2, v);
             if (i != cand) o1 = fmax(o1, v);
-        }
+        };
+        const double *cr = &A.at(0, r_best);
+        sweep(k + tid, r_best, NT, [&](int i) { return A.at(r_best, i); }, part);
+        sweep(r_best + 1 + (((k + tid - (r_best + 1)) % NT) + NT) % NT, f, NT, [&](int i) { return cr[i]; }, part);
         const double om2 = red_max(rs, o2);
         const double off1 = red_max(rs, o1);
         // two_by_two_ok (ldlt_kernel.cpp:97-122)
@@ -335,13 +437,14 @@ __device__ int partner_pairs_wg(const FV &A, int k, int pl, int ne, int nb, cons
         const double in_max = red_max(rs, inl);
         double bv = 0.0;
         int bi = -1;
-        for (int i = pl + tid; i < ne; i += NT) {
-            const double v = fabs(A.at(i, pos));
+        const double *cp = &A.at(0, pos);
+        sweep(pl + tid, ne, NT, [&](int i) { return cp[i]; }, [&](int i, double x) {
+            const double v = fabs(x);
             if (v > bv) {
                 bv = v;
                 bi = i;
             }
-        }
+        });
         double out_max;
         int r;
         red_maxfirst(rs, bv, bi, out_max, r);
@@ -390,26 +493,13 @@ __device__ void sym_swap_wg(const FV &A, int p, int q, int32_t *rows, double *W,
         rows[p] = rows[q];
         rows[q] = tr;
     }
-    for (int m = c_lo + tid; m < p; m += NT) {
-        const double t = A.at(p, m);
-        A.at(p, m) = A.at(q, m);
-        A.at(q, m) = t;
-    }
-    for (int m = p + 1 + tid; m < q; m += NT) {
-        const double t = A.at(m, p);
-        A.at(m, p) = A.at(q, m);
-        A.at(q, m) = t;
-    }
-    for (int m = q + 1 + tid; m < f; m += NT) {
-        const double t = A.at(m, p);
-        A.at(m, p) = A.at(m, q);
-        A.at(m, q) = t;
-    }
-    for (int idx = tid; idx < w_cols; idx += NT) {
-        const double t = W[static_cast<int64_t>(idx) * wld + p];
-        W[static_cast<int64_t>(idx) * wld + p] = W[static_cast<int64_t>(idx) * wld + q];
-        W[static_cast<int64_t>(idx) * wld + q] = t;
-    }
+    // rows p and q of the columns before p; column p and row q between
+    // them; columns p and q below q
+    swap_range([&](int m) { return &A.at(p, m); }, [&](int m) { return &A.at(q, m); }, c_lo + tid, p, NT);
+    swap_range([&](int m) { return &A.at(m, p); }, [&](int m) { return &A.at(q, m); }, p + 1 + tid, q, NT);
+    swap_range([&](int m) { return &A.at(m, p); }, [&](int m) { return &A.at(m, q); }, q + 1 + tid, f, NT);
+    swap_range([&](int idx) { return &W[static_cast<int64_t>(idx) * wld + p]; },
+               [&](int idx) { return &W[static_cast<int64_t>(idx) * wld + q]; }, tid, w_cols, NT);
     __syncthreads();
 }
 
@@ -421,8 +511,8 @@ __device__ void elim1_wg(const FV &A, int k, int ucol_end, double *W, int wld, i
     const int tid = threadIdx.x;
     const int f = A.f;
     const double inv_d = 1.0 / A.at(k, k);
-    if (W)
-        for (int i = k + 1 + tid; i < f; i += NT) W[static_cast<int64_t>(w_col) * wld + i] = A.at(i, k);
+    double *ck = &A.at(0, k);
+    if (W) col_copy(W + static_cast<int64_t>(w_col) * wld, ck, k + 1 + tid, f, NT);
     if (ucol_end > k + 1) {
         const int lg = lg_rows<NT>(f - (k + 1));
         for (int jb = k + 1; jb < ucol_end; jb += NT >> lg) {
@@ -430,12 +520,11 @@ __device__ void elim1_wg(const FV &A, int k, int ucol_end, double *W, int wld, i
             if (j >= ucol_end) continue;
             const double lj = __dmul_rn(A.at(j, k), inv_d);
             if (lj == 0.0) continue;
-            for (int i = j + (tid & ((1 << lg) - 1)); i < f; i += 1 << lg)
-                A.at(i, j) = __dsub_rn(A.at(i, j), __dmul_rn(A.at(i, k), lj));
+            col_axpy(&A.at(0, j), ck, lj, j + (tid & ((1 << lg) - 1)), f, 1 << lg);
         }
     }
     __syncthreads();
-    for (int i = k + 1 + tid; i < f; i += NT) A.at(i, k) = __dmul_rn(A.at(i, k), inv_d);
+    for (int i = k + 1 + tid; i < f; i += NT) ck[i] = __dmul_rn(ck[i], inv_d);
     __syncthreads();
 }
 
@@ -451,11 +540,11 @@ __device__ void elim2_wg(const FV &A, int k, int ucol_end, double *W, int wld, i
     const double a = d00 / s, b = d10 / s, c = d11 / s;
     const double inv = 1.0 / __dmul_rn(__dsub_rn(__dmul_rn(a, c), __dmul_rn(b, b)), s);
     const double i00 = __dmul_rn(c, inv), i10 = __dmul_rn(-b, inv), i11 = __dmul_rn(a, inv);
-    if (W)
-        for (int i = k + 2 + tid; i < f; i += NT) {
-            W[static_cast<int64_t>(w_col) * wld + i] = A.at(i, k);
-            W[static_cast<int64_t>(w_col + 1) * wld + i] = A.at(i, k + 1);
-        }
+    double *ck0 = &A.at(0, k), *ck1 = &A.at(0, k + 1);
+    if (W) {
+        col_copy(W + static_cast<int64_t>(w_col) * wld, ck0, k + 2 + tid, f, NT);
+        col_copy(W + static_cast<int64_t>(w_col + 1) * wld, ck1, k + 2 + tid, f, NT);
+    }
     if (ucol_end > k + 2) {
         const int lg = lg_rows<NT>(f - (k + 2));
         for (int jb = k + 2; jb < ucol_end; jb += NT >> lg) {
@@ -465,10 +554,7 @@ __device__ void elim2_wg(const FV &A, int k, int ucol_end, double *W, int wld, i
             const double l0 = __dadd_rn(__dmul_rn(i00, a0), __dmul_rn(i10, a1));
             const double l1 = __dadd_rn(__dmul_rn(i10, a0), __dmul_rn(i11, a1));
             if (l0 == 0.0 && l1 == 0.0) continue;
-            for (int i = j + (tid & ((1 << lg) - 1)); i < f; i += 1 << lg) {
-                const double t = __dadd_rn(__dmul_rn(A.at(i, k), l0), __dmul_rn(A.at(i, k + 1), l1));
-                A.at(i, j) = __dsub_rn(A.at(i, j), t);
-            }
+            col_axpy2(&A.at(0, j), ck0, ck1, l0, l1, j + (tid & ((1 << lg) - 1)), f, 1 << lg);
         }
     }
     __syncthreads();
@@ -536,8 +622,9 @@ __device__ int force_static_wg(const FV &A, int k, int ne, double sp, const Prm 
     const int search_end = prm.min_abs > 0.0 ? k : ne;
     for (int cand = k; cand < search_end; ++cand) {
         double m = 0.0;
-        for (int i = k + tid; i < cand; i += NT) m = fmax(m, fabs(A.at(cand, i)));
-        for (int i = cand + 1 + tid; i < f; i += NT) m = fmax(m, fabs(A.at(i, cand)));
+        const double *cc = &A.at(0, cand);
+        sweep(k + tid, cand, NT, [&](int i) { return A.at(cand, i); }, [&](int, double v) { m = fmax(m, fabs(v)); });
+        sweep(cand + 1 + tid, f, NT, [&](int i) { return cc[i]; }, [&](int, double v) { m = fmax(m, fabs(v)); });
         const double colmax = red_max(rs, m);
         const double a = fabs(A.at(cand, cand));
         const double ratio = colmax > 0.0 ? a / colmax : a;
@@ -674,13 +761,28 @@ __device__ void deferred_update_wg(const FV &A, int p0, int p1, int c0, int e, c
     const int f = A.f;
     const int tid = threadIdx.x;
     const int lg = lg_rows(f - c0);
+    const int st = 1 << lg;
     for (int cb = c0; cb < e; cb += kLT >> lg) {
         const int j = cb + (tid >> lg);
         if (j >= e) continue;
-        for (int i = j + (tid & ((1 << lg) - 1)); i < f; i += 1 << lg) {
-            double sacc = 0.0;
-            for (int t = 0; t < jb; ++t) sacc += A.at(i, p0 + t) * W[static_cast<int64_t>(t) * wld + j];
-            A.at(i, j) -= sacc;
+        double *cj = &A.at(0, j);
+        // kU rows at a time, each summed over t in order
+        for (int i0 = j + (tid & (st - 1)); i0 < f; i0 += kU * st) {
+            double sacc[kU];
+#pragma unroll
+            for (int u = 0; u < kU; ++u) sacc[u] = 0.0;
+            for (int t = 0; t < jb; ++t) {
+                const double w = W[static_cast<int64_t>(t) * wld + j];
+                const double *ct = &A.at(0, p0 + t);
+                double v[kU];
+#pragma unroll
+                for (int u = 0; u < kU; ++u) v[u] = ct[min(i0 + u * st, f - 1)];
+#pragma unroll
+                for (int u = 0; u < kU; ++u) sacc[u] += v[u] * w;
+            }
+#pragma unroll
+            for (int u = 0; u < kU; ++u)
+                if (i0 + u * st < f) cj[i0 + u * st] -= sacc[u];
         }
     }
     __syncthreads();
