@@ -503,6 +503,32 @@ __device__ void sym_swap_wg(const FV &A, int p, int q, int32_t *rows, double *W,
     __syncthreads();
 }
 
+// Brings the rows order[0..np) to positions k, k + 1, ... in turn with
+// sym_swap (ldlt_kernel.cpp:1076-1082): each row id is unique and, when
+// its turn comes, lies in [k + t, ne); the workgroup finds it in one
+// parallel sweep instead of a sequential scan.
+template <int NT>
+__device__ void place_rows_wg(const FV &A, int k, int ne, int np, const int32_t *order, int32_t *rows,
+                              RedScratchT<NT> &rs) {
+    const int tid = threadIdx.x;
+    for (int t = 0; t < np; ++t) {
+        const int32_t want = order[t];
+        int mine = -1;
+        for (int i0 = k + t + tid; i0 < ne; i0 += kU * NT) {
+            int32_t v[kU];
+#pragma unroll
+            for (int u = 0; u < kU; ++u) v[u] = rows[min(i0 + u * NT, ne - 1)];
+#pragma unroll
+            for (int u = 0; u < kU; ++u)
+                if (i0 + u * NT < ne && v[u] == want) mine = i0 + u * NT;
+        }
+        double hit;
+        int q;
+        red_maxfirst(rs, mine >= 0 ? 1.0 : 0.0, mine, hit, q);
+        if (q >= k + t) sym_swap_wg<NT>(A, k + t, q, rows, nullptr, 0, 0, 0);
+    }
+}
+
 // eliminate_1x1 (ldlt_kernel.cpp:350-432): columns (k, ucol_end) get the
 // rank-1 update c_j[i] -= c_k[i] * (c_k[j] / d) (multiply, then subtract),
 // the unscaled column is saved in W column w_col, then c_k is scaled.
@@ -860,11 +886,7 @@ __global__ __launch_bounds__(kLT) void k_ldlt_blocked(LdltDev d, const int32_t *
                 const int np = partner_pairs_wg(A, k, pl, ne, nb, rows, order, rs);
                 if (np > 0) {
                     n_pull += np / 2;
-                    for (int t = 0; t < np; ++t) {
-                        int q = k + t;
-                        while (rows[q] != order[t]) ++q;
-                        sym_swap_wg(A, k + t, q, rows, nullptr, 0, 0, 0);
-                    }
+                    place_rows_wg(A, k, ne, np, order, rows, rs);
                     live = max(live, k + np);
                     k_paired = k;
                     continue;
@@ -1628,11 +1650,7 @@ __global__ __launch_bounds__(kLT) void k_big_stall(LdltDev d, BigLd *states, int
     if (k != B.k_paired) {
         const int np = partner_pairs_wg(A, k, pl, ne, nb, rows, order, rs);
         if (np > 0) {
-            for (int t = 0; t < np; ++t) {
-                int q = k + t;
-                while (rows[q] != order[t]) ++q;
-                sym_swap_wg(A, k + t, q, rows, nullptr, 0, 0, 0);
-            }
+            place_rows_wg(A, k, ne, np, order, rows, rs);
             if (tid == 0) {
                 B.live = max(live, k + np);
                 B.k_paired = k;
