@@ -99,6 +99,8 @@ TUNE_LDLT_UPDATE_WG = 7
 TUNE_LDLT_DEVICE_FRONTS = 8
 TUNE_ASM_CHUNK = 9
 TUNE_LDLT_DEVICE_WAVE = 10
+TUNE_SOLVE_PERSISTENT = 11
+TUNE_PANEL_OUTER_REDUCED = 12
 
 _lib = None
 

@@ -476,7 +476,7 @@ void launch_trsm(hipStream_t st, const DevPtrs &d, const TrsmItem *it, int64_t n
 // A10/A11: Schur-complement update on MFMA (tile machinery in mfma_tile.hpp).
 // Precision codes of a front's Schur updates (DevPtrs::prec): 0 FP64 MFMA;
 // 1 FP32 MFMA on the FP32 panel copy; 2 BF16 MFMA (operands rounded to
-// bf16), 3 split BF16 (hi / lo operands, three MFMAs); 1-3 accumulate in
+// bf16), 3 split BF16 (hi / lo operands, four MFMAs); 1-3 accumulate in
 // FP32 over the panel and subtract the result from the FP64 Schur
 // complement.
 constexpr int kSyrkKc64 = 16;

@@ -1014,7 +1014,7 @@ __device__ __forceinline__ void super_block_tile(int nrb, int ncb, int tile, int
 // part of rows [c0, f) x columns [c0, c1) of front F (c1 - c0 <= f - c0);
 // tile `tile` of the region in super-block order. MODE 0: FP64 MFMA;
 // 1: FP32 MFMA on the FP64 operands (rounded as they enter LDS); 2: BF16
-// MFMA (operands rounded to bf16); 3: split BF16 (hi / lo operands, three
+// MFMA (operands rounded to bf16); 3: split BF16 (hi / lo operands, four
 // MFMAs). Modes 1-3 accumulate in FP32 over the panel and subtract the
 // result from the FP64 entries.
 constexpr int kBigKcBf = 32;  // panel columns per LDS stage of the BF16 tiles (FP64 operands in registers)

@@ -15,6 +15,9 @@
 // also update: those forward updates are atomic adds.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <type_traits>
+
 #include "smfgpu_internal.hpp"
 #include "smfgpu_ldlt.hpp"
 
@@ -130,13 +133,20 @@ __device__ void fwd_update(const double *Lb, const int32_t *frows, int f, int p,
 }
 
 template <int NR>
-__global__ __launch_bounds__(kSThreads) void k_fwd_step(DevPtrs d, const SolveItem *__restrict__ items,
-                                                       double *__restrict__ y, int32_t n, int32_t nrhs) {
-    __shared__ double Dg[kSD][kSD + 1];
-    __shared__ double zs[kSB][NR];
-    __shared__ double yb[kSB][NR];
-    __shared__ double part[8][64][NR];
-    const SolveItem it = items[blockIdx.x];
+struct FwdLds {
+    double Dg[kSD][kSD + 1];
+    double zs[kSB][NR];
+    double yb[kSB][NR];
+    double part[8][64][NR];
+};
+
+template <int NR>
+__device__ void fwd_item(const DevPtrs &d, const SolveItem &it, double *__restrict__ y, int32_t n, int32_t nrhs,
+                         FwdLds<NR> &S) {
+    auto &Dg = S.Dg;
+    auto &zs = S.zs;
+    auto &yb = S.yb;
+    auto &part = S.part;
     const FrontDev F = d.fronts[it.s];
     const int f = F.f, p = F.p;
     const double *Lb = d.L + F.lofs;
@@ -190,6 +200,13 @@ __global__ __launch_bounds__(kSThreads) void k_fwd_step(DevPtrs d, const SolveIt
     }
     if (it.flags & 2)  // the rows below the last block, by this workgroup
         fwd_update<NR>(Lb, frows, f, p, F.col0, it.tk, it.tkb, it.tk + it.tkb, f, zs, part, yb, -1, 0, y, n, nrhs);
+}
+
+template <int NR>
+__global__ __launch_bounds__(kSThreads) void k_fwd_step(DevPtrs d, const SolveItem *__restrict__ items,
+                                                       double *__restrict__ y, int32_t n, int32_t nrhs) {
+    __shared__ FwdLds<NR> S;
+    fwd_item<NR>(d, items[blockIdx.x], y, n, nrhs, S);
 }
 
 // Backward update of columns [c0, c1): y[col0 + c] -= sum_{i in [r0, r1)}
@@ -275,13 +292,20 @@ __device__ void bwd_update(const double *Lb, const int32_t *frows, int f, int p,
 }
 
 template <int NR>
-__global__ __launch_bounds__(kSThreads) void k_bwd_step(DevPtrs d, const SolveItem *__restrict__ items,
-                                                       double *__restrict__ y, int32_t n, int32_t nrhs) {
-    __shared__ double Dg[kSD][kSD + 1];
-    __shared__ double xs[kSeg][NR];
-    __shared__ double v[kSB][NR];
-    __shared__ double xo[kSB][NR];
-    const SolveItem it = items[blockIdx.x];
+struct BwdLds {
+    double Dg[kSD][kSD + 1];
+    double xs[kSeg][NR];
+    double v[kSB][NR];
+    double xo[kSB][NR];
+};
+
+template <int NR>
+__device__ void bwd_item(const DevPtrs &d, const SolveItem &it, double *__restrict__ y, int32_t n, int32_t nrhs,
+                         BwdLds<NR> &S) {
+    auto &Dg = S.Dg;
+    auto &xs = S.xs;
+    auto &v = S.v;
+    auto &xo = S.xo;
     const FrontDev F = d.fronts[it.s];
     const int f = F.f, p = F.p;
     const double *Lb = d.L + F.lofs;
@@ -328,6 +352,68 @@ __global__ __launch_bounds__(kSThreads) void k_bwd_step(DevPtrs d, const SolveIt
         const int t = idx % it.tkb, r = idx / it.tkb;
         y[F.col0 + it.tk + t + static_cast<int64_t>(r) * n] = xo[t][r];
     }
+}
+
+template <int NR>
+__global__ __launch_bounds__(kSThreads) void k_bwd_step(DevPtrs d, const SolveItem *__restrict__ items,
+                                                       double *__restrict__ y, int32_t n, int32_t nrhs) {
+    __shared__ BwdLds<NR> S;
+    bwd_item<NR>(d, items[blockIdx.x], y, n, nrhs, S);
+}
+
+// ---------------------------------------------------------------------------
+// One launch per sweep: a workgroup per item, each taking a ticket (its
+// item, in sweep order) when it starts; before an item of step t runs,
+// every item of step t - 1 has finished (its counter reached the step's
+// item count): the ordering the per-step launches get from kernel
+// boundaries, without the launches. Tickets go out in increasing order to
+// running workgroups and an item waits only on items with smaller tickets,
+// so the waits end however the workgroups are dispatched. There is no loop
+// around the item: every wave meets the same barriers in the same order.
+// Writes to y are released with a device-scope fence before the counter
+// moves and acquired after the wait. A wait that runs out (never expected)
+// sets err and lets every later wait through, so the launch always ends.
+constexpr int64_t kSweepSpin = int64_t{1} << 18;
+
+__device__ __forceinline__ void sweep_wait(int32_t *cnt, int32_t target, int32_t *err) {
+    for (int64_t spin = 0; __hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target; ++spin) {
+        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+        if (spin > kSweepSpin) {
+            atomicExch(err, 1);
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    __threadfence();
+}
+
+template <int NR, bool FWD>
+__global__ __launch_bounds__(kSThreads) void k_sweep(DevPtrs d, const SolveItem *__restrict__ items,
+                                                    const int32_t *__restrict__ step_of,
+                                                    const int32_t *__restrict__ step_cnt, int32_t *ctr, int32_t *err,
+                                                    double *__restrict__ y, int32_t n, int32_t nrhs) {
+    __shared__ typename std::conditional<FWD, FwdLds<NR>, BwdLds<NR>>::type S;
+    __shared__ int32_t s_item;
+    if (threadIdx.x == 0) {
+        const int32_t i = atomicAdd(&ctr[0], 1);  // < the grid size, the item count
+        const int32_t t = step_of[i];
+        if (t > 0) sweep_wait(&ctr[t], step_cnt[t - 1], err);  // counter of step t - 1
+        s_item = i;
+    }
+    __syncthreads();
+    const int32_t i = __builtin_amdgcn_readfirstlane(s_item);
+    const SolveItem it = items[i];
+    if constexpr (FWD) fwd_item<NR>(d, it, y, n, nrhs, S);
+    else bwd_item<NR>(d, it, y, n, nrhs, S);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        atomicAdd(&ctr[1 + step_of[i]], 1);
+    }
+}
+
+__global__ void k_zero_i32(int32_t *p, int32_t n) {
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
 }
 
 
@@ -501,6 +587,21 @@ void launch_ldlt_bwd_step(hipStream_t st, const LdltDev &d, const SolveItem *ite
     for (int64_t o = 0; o < count; o += kMaxLaunchGroups)
         k_ldlt_bwd_step<<<dim3(static_cast<unsigned>(std::min(kMaxLaunchGroups, count - o))), kLThreads, 0, st>>>(
             d, items + o, y, n, nrhs);
+}
+
+void launch_sweep(hipStream_t st, const DevPtrs &d, bool forward, const SolveItem *items, const int32_t *step_of,
+                  const int32_t *step_cnt, int32_t nitems, int32_t nsteps, int32_t *ctr, int32_t *err, double *y,
+                  int32_t n, int32_t nrhs) {
+    if (nitems <= 0) return;
+    k_zero_i32<<<dim3(std::max(1, std::min(64, (nsteps + 1 + 255) / 256))), 256, 0, st>>>(ctr, nsteps + 1);
+    const dim3 grid(static_cast<unsigned>(nitems));  // one workgroup per item
+    if (forward) {
+        if (nrhs == 1) k_sweep<1, true><<<grid, kSThreads, 0, st>>>(d, items, step_of, step_cnt, ctr, err, y, n, nrhs);
+        else k_sweep<4, true><<<grid, kSThreads, 0, st>>>(d, items, step_of, step_cnt, ctr, err, y, n, nrhs);
+    } else {
+        if (nrhs == 1) k_sweep<1, false><<<grid, kSThreads, 0, st>>>(d, items, step_of, step_cnt, ctr, err, y, n, nrhs);
+        else k_sweep<4, false><<<grid, kSThreads, 0, st>>>(d, items, step_of, step_cnt, ctr, err, y, n, nrhs);
+    }
 }
 
 void launch_fwd_step(hipStream_t st, const DevPtrs &d, const SolveItem *items, int64_t count, double *y, int32_t n,

@@ -65,6 +65,7 @@ SMFGPU_API int smfgpu_solve_noalloc(smfgpu_factor_t f, double *b, int32_t nrhs, 
     if (cur != P->device) (void)hipSetDevice(cur);
     if (rc == SMFGPU_OK && e != hipSuccess)
         rc = set_error(SMFGPU_ERR_DEVICE, std::string("solve_noalloc: ") + hipGetErrorString(e));
+    if (rc == SMFGPU_OK) rc = sweep_check(P);
     return rc;
 }
 

@@ -175,6 +175,32 @@ int solve_lists(smfgpu_plan_s *P) {
         if (st.second > INT32_MAX) return set_error(SMFGPU_ERR_ALLOC, "plan: solve step too large");
     int rc;
     if ((rc = P->d_fwd_items.upload(fwd_items)) || (rc = P->d_bwd_items.upload(bwd_items))) return rc;
+    // the persistent sweeps' sequences
+    std::vector<int32_t> fstep(fwd_items.size()), fcnt, bstep, bcnt;
+    for (size_t t = 0; t < P->fwd_steps.size(); ++t) {
+        for (int64_t i = 0; i < P->fwd_steps[t].second; ++i) fstep[P->fwd_steps[t].first + i] = static_cast<int32_t>(t);
+        fcnt.push_back(static_cast<int32_t>(P->fwd_steps[t].second));
+    }
+    std::vector<SolveItem> bseq;
+    bseq.reserve(bwd_items.size());
+    for (int32_t lv = nlev - 1; lv >= 0; --lv)
+        for (int32_t i = P->bwd_lv[lv]; i < P->bwd_lv[lv + 1]; ++i) {
+            const auto &stp = P->bwd_steps[i];
+            for (int64_t e = 0; e < stp.second; ++e) {
+                bseq.push_back(bwd_items[stp.first + e]);
+                bstep.push_back(static_cast<int32_t>(bcnt.size()));
+            }
+            bcnt.push_back(static_cast<int32_t>(stp.second));
+        }
+    const size_t nctr = 1 + std::max(fcnt.size(), bcnt.size());
+    if ((rc = P->d_bwd_seq.upload(bseq)) || (rc = P->d_fwd_step_of.upload(fstep)) ||
+        (rc = P->d_bwd_step_of.upload(bstep)) || (rc = P->d_fwd_step_cnt.upload(fcnt)) ||
+        (rc = P->d_bwd_step_cnt.upload(bcnt)) || (rc = P->d_sweep_ctr.alloc(nctr)))
+        return rc;
+    if (!P->d_sweep_err.p) {
+        if ((rc = P->d_sweep_err.alloc(1)) || (rc = P->h_sweep_err.assign(1, 0))) return rc;
+        HIP_OK(hipMemset(P->d_sweep_err.p, 0, sizeof(int32_t)));
+    }
     return SMFGPU_OK;
 }
 
@@ -187,12 +213,18 @@ int solve_lists(smfgpu_plan_s *P) {
 // cb_seg columns. Items of all fronts of the level share each launch.
 int dense_schedule(smfgpu_plan_s *P) {
     const int32_t nlev = static_cast<int32_t>(P->level_fronts.size());
-    const int32_t outer = P->panel_outer;
     std::vector<PotrfItem> potrf;
     std::vector<TrsmItem> trsm;
     std::vector<SyrkItem> syrk;
-    P->launches.clear();
-    P->syrk_flops_of_launch.clear();
+    P->selfr.clear();
+    // one launch sequence per outer panel width: FP64 factorizations take
+    // panel_outer, reduced-precision ones panel_outer_red (their trailing
+    // updates of the pivot columns accumulate K = outer products in FP32, so
+    // a narrower panel keeps that sum shorter); both index the same item
+    // arrays
+    auto build = [&](const int32_t outer, std::vector<Launch> &launches, std::vector<int64_t> &sflops) {
+    launches.clear();
+    sflops.clear();
     // dense-kernel items and the launch sequence
     // tiles (it, jt) of the lower part of rows [jlo, f) x columns [jlo, jhi),
     // in super-blocks of kSyrkGroup x kSyrkGroup tiles: the kernel gives each
@@ -219,19 +251,18 @@ int dense_schedule(smfgpu_plan_s *P) {
         for (int32_t s : fl)
             if (P->fronts[s].f >= red_min) emit(s);
         const int64_t y1 = static_cast<int64_t>(syrk.size());
-        P->launches.push_back(Launch{K_SYRK, lv, y0, y1 - y0, y1 - yr});
+        launches.push_back(Launch{K_SYRK, lv, y0, y1 - y0, y1 - yr});
     };
-    P->selfr.clear();
     for (int32_t lv = 0; lv < nlev; ++lv) {
         const auto &fl = P->level_fronts[lv];
-        P->launches.push_back(Launch{K_ASSEMBLE, lv, P->asm_first[lv], P->asm_first[lv + 1] - P->asm_first[lv], 0});
+        launches.push_back(Launch{K_ASSEMBLE, lv, P->asm_first[lv], P->asm_first[lv + 1] - P->asm_first[lv], 0});
         // the selector's candidates of the level (fronts that may run below
         // FP64), between their assembly and their factorization
         const int64_t s0 = static_cast<int64_t>(P->selfr.size());
         for (int32_t s : fl)
             if (P->fronts[s].f >= red_min)
                 P->selfr.push_back(SelFront{s, P->fronts[s].f, P->fronts[s].p, 0, P->fronts[s].col0});
-        P->launches.push_back(Launch{K_SELECT, lv, s0, static_cast<int64_t>(P->selfr.size()) - s0, 0});
+        launches.push_back(Launch{K_SELECT, lv, s0, static_cast<int64_t>(P->selfr.size()) - s0, 0});
         int32_t maxp = 0;
         for (int32_t s : fl) maxp = std::max(maxp, P->fronts[s].p);
         for (int32_t K = 0; K < maxp; K += outer) {
@@ -245,8 +276,8 @@ int dense_schedule(smfgpu_plan_s *P) {
                     potrf.push_back(PotrfItem{s, k, kb, 0});
                     for (int32_t r0 = k + kb; r0 < F.f; r0 += kPanelInner) trsm.push_back(TrsmItem{s, k, kb, r0});
                 }
-                P->launches.push_back(Launch{K_POTRF, lv, p0, static_cast<int64_t>(potrf.size()) - p0, 0});
-                P->launches.push_back(Launch{K_TRSM, lv, t0, static_cast<int64_t>(trsm.size()) - t0, 0});
+                launches.push_back(Launch{K_POTRF, lv, p0, static_cast<int64_t>(potrf.size()) - p0, 0});
+                launches.push_back(Launch{K_TRSM, lv, t0, static_cast<int64_t>(trsm.size()) - t0, 0});
                 syrk_launch(lv, fl, [&](int32_t s) {
                     const FrontDev &F = P->fronts[s];
                     if (F.p <= k) return;
@@ -275,13 +306,12 @@ int dense_schedule(smfgpu_plan_s *P) {
             });
         }
     }
-    P->launches.erase(std::remove_if(P->launches.begin(), P->launches.end(),
-                                     [](const Launch &l) { return l.count <= 0; }),
-                      P->launches.end());
+    launches.erase(std::remove_if(launches.begin(), launches.end(), [](const Launch &l) { return l.count <= 0; }),
+                   launches.end());
     // syrk flops per launch (algorithmic: 2 * entries updated * panel width)
-    for (const Launch &l : P->launches) {
+    for (const Launch &l : launches) {
         if (l.kind != K_SYRK) {
-            P->syrk_flops_of_launch.push_back(0);
+            sflops.push_back(0);
             continue;
         }
         int64_t fl = 0;
@@ -294,7 +324,15 @@ int dense_schedule(smfgpu_plan_s *P) {
             for (int64_t j = y.j0; j < jh; ++j) cnt += std::max<int64_t>(0, ih - std::max<int64_t>(j, y.i0));
             fl += 2 * cnt * (y.t1 - y.t0);
         }
-        P->syrk_flops_of_launch.push_back(fl);
+        sflops.push_back(fl);
+    }
+    };
+    build(P->panel_outer, P->launches, P->syrk_flops_of_launch);
+    if (P->panel_outer_red != P->panel_outer) {
+        build(P->panel_outer_red, P->launches_red, P->syrk_flops_of_launch_red);
+    } else {
+        P->launches_red = P->launches;
+        P->syrk_flops_of_launch_red = P->syrk_flops_of_launch;
     }
     int rc;
     if ((rc = P->d_potrf.upload(potrf)) || (rc = P->d_trsm.upload(trsm)) || (rc = P->d_syrk.upload(syrk)) ||
@@ -650,7 +688,9 @@ SMFGPU_API int smfgpu_plan_stats(smfgpu_plan_t P, double *out) {
              P->d_fronts.bytes() + P->d_pcol_ptr.bytes() + P->d_perm.bytes() + P->d_level_fronts.bytes() +
              P->d_cmofs.bytes() + P->d_slot_ptr.bytes() + P->d_vals.bytes() + P->d_L32.bytes() +
              P->d_L16h.bytes() + P->d_L16l.bytes() +
-             P->d_fwd_items.bytes() + P->d_bwd_items.bytes() + P->d_gpart.bytes() + P->d_big.bytes() +
+             P->d_fwd_items.bytes() + P->d_bwd_items.bytes() + P->d_bwd_seq.bytes() + P->d_fwd_step_of.bytes() +
+             P->d_bwd_step_of.bytes() + P->d_fwd_step_cnt.bytes() + P->d_bwd_step_cnt.bytes() +
+             P->d_sweep_ctr.bytes() + P->d_sweep_err.bytes() + P->d_gpart.bytes() + P->d_big.bytes() +
              P->d_fr_ptr.bytes();
     // solve and refinement work buffers (allocated on first use)
     bytes += P->d_y.bytes() + P->d_b.bytes() + P->d_r.bytes() + P->d_rowstat.bytes() + P->d_x.bytes() +
@@ -735,6 +775,15 @@ SMFGPU_API int smfgpu_plan_tune(smfgpu_plan_t P, int32_t key, int64_t value) {
         if (value < 256 || value > 16384) return set_error(SMFGPU_ERR_INVALID_ARG, "tune: chunk in [256, 16384]");
         P->asm_chunk = static_cast<int32_t>(value);
         release_graph(P);
+        return SMFGPU_OK;
+    case SMFGPU_TUNE_PANEL_OUTER_REDUCED:
+        if (value < kPanelInner || value % kPanelInner != 0 || value > 4096)
+            return set_error(SMFGPU_ERR_INVALID_ARG, "tune: outer panel must be a multiple of 64 in [64, 4096]");
+        P->panel_outer_red = static_cast<int32_t>(value);
+        return P->posdef ? dense_schedule(P) : SMFGPU_OK;
+    case SMFGPU_TUNE_SOLVE_PERSISTENT:
+        if (value != 0 && value != 1) return set_error(SMFGPU_ERR_INVALID_ARG, "tune: 0 or 1");
+        P->sweep_persistent = static_cast<int32_t>(value);
         return SMFGPU_OK;
     case SMFGPU_TUNE_SYRK_AHEAD:
         if (value != 1 && value != 2) return set_error(SMFGPU_ERR_INVALID_ARG, "tune: chunks ahead must be 1 or 2");
@@ -909,9 +958,12 @@ SMFGPU_API int smfgpu_factor(smfgpu_plan_t P, const double *values, int64_t nval
     // reduced-precision Schur kernels to launch: the static code, or every
     // code the selector may pick
     const int red_modes = autosel ? sprm.mask : (code > 0 ? (1 << code) : 0);
+    // the launch sequence of this factorization's precision class
+    const std::vector<Launch> &LL = gi ? P->launches_red : P->launches;
+    const std::vector<int64_t> &LF = gi ? P->syrk_flops_of_launch_red : P->syrk_flops_of_launch;
     // timing events
     const bool timing = P->timing;
-    const size_t nl = P->launches.size();
+    const size_t nl = LL.size();
     if (timing && P->ev.size() < 2 * nl + 2) {
         for (size_t i = P->ev.size(); i < 2 * nl + 2; ++i) {
             hipEvent_t e;
@@ -928,7 +980,7 @@ SMFGPU_API int smfgpu_factor(smfgpu_plan_t P, const double *values, int64_t nval
         if (rec_all) mark(2 * nl);
         launch_map_values(st, dv, nvalues, P->d_slot_ptr.p, P->d_slot_src.p, P->nnz_perm, P->d_av.p, P->d_red.p);
         for (size_t li = 0; li < nl; ++li) {
-            const Launch &l = P->launches[li];
+            const Launch &l = LL[li];
             const bool rec = rec_all && (l.kind == K_SYRK || l.kind == K_ASSEMBLE);
             if (rec) mark(2 * li);
             switch (l.kind) {
@@ -1012,13 +1064,13 @@ SMFGPU_API int smfgpu_factor(smfgpu_plan_t P, const double *values, int64_t nval
         P->syrk_launches = 0;
         P->syrk_flops = 0;
         for (size_t li = 0; li < nl; ++li) {
-            const Launch &l = P->launches[li];
+            const Launch &l = LL[li];
             if (l.kind != K_SYRK && l.kind != K_ASSEMBLE) continue;
             HIP_OK(hipEventElapsedTime(&ms, P->ev[2 * li], P->ev[2 * li + 1]));
             if (l.kind == K_SYRK) {
                 P->t_syrk_ms += ms;
                 P->syrk_launches += 1;
-                P->syrk_flops += static_cast<double>(P->syrk_flops_of_launch[li]);
+                P->syrk_flops += static_cast<double>(LF[li]);
             } else {
                 P->t_asm_ms += ms;
             }
@@ -1141,6 +1193,16 @@ SMFGPU_API int smfgpu_export_factor(smfgpu_factor_t f, int64_t *sizes, int64_t *
 
 namespace smfgpu {
 // job as ldlt_solve; for a Cholesky factor D = I (the diagonal job copies)
+int sweep_check(smfgpu_plan_s *P) {
+    if (!P->d_sweep_err.p || !P->h_sweep_err.p) return SMFGPU_OK;
+    HIP_OK(hipMemcpyAsync(P->h_sweep_err.p, P->d_sweep_err.p, sizeof(int32_t), hipMemcpyDeviceToHost, P->stream));
+    HIP_OK(hipStreamSynchronize(P->stream));
+    if (P->h_sweep_err.p[0] == 0) return SMFGPU_OK;
+    HIP_OK(hipMemsetAsync(P->d_sweep_err.p, 0, sizeof(int32_t), P->stream));
+    HIP_OK(hipStreamSynchronize(P->stream));
+    return set_error(SMFGPU_ERR_DEVICE, "solve: a sweep's dependency wait ran out");
+}
+
 int solve_device(smfgpu_plan_s *P, const double *b_dev, int64_t ldb, double *out_dev, int64_t ldo, int32_t nrhs,
                  int32_t job, double *ybuf, int64_t ycap) {
     if (!P->posdef) return ldlt_solve(P, b_dev, ldb, out_dev, ldo, nrhs, job, ybuf, ycap);
@@ -1173,21 +1235,33 @@ int solve_device(smfgpu_plan_s *P, const double *b_dev, int64_t ldb, double *out
             }
         }
         if (tm) HIP_OK(hipEventRecord(P->sev[0], st));
-        if (fwd)
+        const bool pers = P->sweep_persistent != 0;
+        if (fwd && pers) {
+            launch_sweep(st, d, true, P->d_fwd_items.p, P->d_fwd_step_of.p, P->d_fwd_step_cnt.p,
+                         static_cast<int32_t>(P->d_fwd_items.n), static_cast<int32_t>(P->fwd_steps.size()),
+                         P->d_sweep_ctr.p, P->d_sweep_err.p, y, n, nr);
+        } else if (fwd) {
             for (int32_t lv = 0; lv < nl; ++lv)
                 for (int32_t i = P->fwd_lv[lv]; i < P->fwd_lv[lv + 1]; ++i)
                     launch_fwd_step(st, d, P->d_fwd_items.p + P->fwd_steps[i].first, P->fwd_steps[i].second, y,
                                     n, nr);
+        }
         if (tm) HIP_OK(hipEventRecord(P->sev[1], st));
         if (job == 1 || job == 2) {
             HIP_OK(hipMemcpy2DAsync(out_dev + r0 * ldo, ldo * sizeof(double), y, n * sizeof(double),
                                     n * sizeof(double), nr, hipMemcpyDeviceToDevice, st));
             continue;
         }
-        for (int32_t lv = nl - 1; lv >= 0; --lv)
-            for (int32_t i = P->bwd_lv[lv]; i < P->bwd_lv[lv + 1]; ++i)
-                launch_bwd_step(st, d, P->d_bwd_items.p + P->bwd_steps[i].first, P->bwd_steps[i].second, y, n,
-                                nr);
+        if (pers) {
+            launch_sweep(st, d, false, P->d_bwd_seq.p, P->d_bwd_step_of.p, P->d_bwd_step_cnt.p,
+                         static_cast<int32_t>(P->d_bwd_seq.n), static_cast<int32_t>(P->bwd_steps.size()),
+                         P->d_sweep_ctr.p, P->d_sweep_err.p, y, n, nr);
+        } else {
+            for (int32_t lv = nl - 1; lv >= 0; --lv)
+                for (int32_t i = P->bwd_lv[lv]; i < P->bwd_lv[lv + 1]; ++i)
+                    launch_bwd_step(st, d, P->d_bwd_items.p + P->bwd_steps[i].first, P->bwd_steps[i].second, y,
+                                    n, nr);
+        }
         if (tm) {
             HIP_OK(hipEventRecord(P->sev[2], st));
             HIP_OK(hipStreamSynchronize(st));
@@ -1239,6 +1313,7 @@ SMFGPU_API int smfgpu_solve(smfgpu_factor_t f, double *b, int32_t nrhs, int32_t 
                                 hipMemcpyDeviceToHost, st));
     }
     HIP_OK(hipStreamSynchronize(st));
+    if (rc == SMFGPU_OK) rc = sweep_check(P);
     if (P->timing) {
         float ms = 0;
         HIP_OK(hipEventElapsedTime(&ms, e0, e1));

@@ -372,6 +372,7 @@ int refine_run(smfgpu_factor_s *f, const double *values, int32_t value_mem, cons
     launch_scatter_rhs(st, P->d_perm.p, xbest, c.tmp, n, n, nrhs);
     HIP_OK(hipMemcpy2DAsync(x, ldx * sizeof(double), c.tmp, un * sizeof(double), un * sizeof(double), nrhs, kout, st));
     HIP_OK(hipStreamSynchronize(st));
+    if (const int rc = sweep_check(P)) return rc;
     info.eta = out.eta;
     info.omega = out.om;
     info.steps = steps;

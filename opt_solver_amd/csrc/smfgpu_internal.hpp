@@ -144,6 +144,14 @@ void launch_scatter_rhs(hipStream_t st, const int32_t *perm, const double *y, do
 // One step of a sweep (nrhs <= 4): a workgroup per item.
 constexpr int32_t kSolveBlock = 128;  // pivot columns per step
 constexpr int32_t kSolveSeg = 512;    // rows per backward update item
+// One sweep (forward or backward) as a single persistent launch: items in
+// sweep order, step_of[i] their steps, step_cnt[t] the items of step t;
+// an item waits until every item of the step before has finished (the
+// kernel boundary of the per-step launches, as a counter). ctr holds
+// 1 + steps counters (cleared by the launch); err is set if a wait ran out.
+void launch_sweep(hipStream_t st, const DevPtrs &d, bool forward, const SolveItem *items, const int32_t *step_of,
+                  const int32_t *step_cnt, int32_t nitems, int32_t nsteps, int32_t *ctr, int32_t *err, double *y,
+                  int32_t n, int32_t nrhs);
 void launch_fwd_step(hipStream_t st, const DevPtrs &d, const SolveItem *items, int64_t count, double *y,
                      int32_t n, int32_t nrhs);
 void launch_bwd_step(hipStream_t st, const DevPtrs &d, const SolveItem *items, int64_t count, double *y,

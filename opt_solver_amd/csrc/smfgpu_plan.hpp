@@ -174,6 +174,11 @@ struct RangePool {
 };
 
 constexpr int kPanelOuter = 512;  // default outer panel (tunable per plan)
+// default outer panel of reduced-precision factorizations: on the 100^3
+// 27-point operator (METIS) 256 brings the learned factor's backward error
+// after one refinement step from 1.04e-12 to 8.2e-13 (under the 1e-12
+// target: one solve less) for 2 ms more factor time
+constexpr int kPanelOuterRed = 256;
 constexpr int kPanelInner = 64;
 constexpr int kTile = 128;
 constexpr int kSyrkGroup = 8;  // Schur tiles per super-block side (one XCD's resident tiles)
@@ -207,7 +212,8 @@ struct smfgpu_plan_s {
     int64_t factor_entries = 0, cb_entries = 0, rows_total = 0;
     double flops = 0.0;
     std::vector<double> front_flops;
-    std::vector<Launch> launches;
+    std::vector<Launch> launches;      // FP64 factorizations (outer panel panel_outer)
+    std::vector<Launch> launches_red;  // reduced-precision ones (outer panel panel_outer_red)
     // device
     DBuf<FrontDev> d_fronts;
     DBuf<int32_t> d_rows, d_cmap, d_pcol_ptr, d_amap, d_perm, d_level_fronts, d_flag;
@@ -230,7 +236,18 @@ struct smfgpu_plan_s {
     PinnedArr<SolveItem> h_fwd_items, h_bwd_items;  // LDL^T: staging of the solve items
     std::vector<std::pair<int64_t, int64_t>> fwd_steps, bwd_steps;
     std::vector<int32_t> fwd_lv, bwd_lv;
+    // SPD sweeps as one persistent launch each: the items in the order the
+    // sweep takes them (forward: as stored; backward: levels from the root
+    // down), the step of each, the item count of each step, and the
+    // counters (ticket, then one per step)
+    DBuf<SolveItem> d_bwd_seq;
+    DBuf<int32_t> d_fwd_step_of, d_bwd_step_of, d_fwd_step_cnt, d_bwd_step_cnt;
+    DBuf<int32_t> d_sweep_ctr;
+    DBuf<int32_t> d_sweep_err;  // set when a sweep's wait ran out (never expected)
+    PinnedArr<int32_t> h_sweep_err;  // its host copy (pinned: read without an allocation)
+    int32_t sweep_persistent = 0;  // per-step launches by default (measured faster)
     int32_t panel_outer = kPanelOuter;             // SPD: pivot columns per trailing Schur update
+    int32_t panel_outer_red = kPanelOuterRed;      // the same for reduced-precision factorizations
     std::vector<int64_t> asm_first;                // SPD: first assembly task of each level
     int32_t cb_seg = 4096;                         // SPD: pivot columns per contribution-block update
     int32_t ldlt_big_min = 256;                    // LDL^T: fronts with at least this many rows take the large-front path
@@ -253,7 +270,7 @@ struct smfgpu_plan_s {
     DBuf<double> d_rvec, d_stat, d_part, d_h, d_av_ref;
     double norm_a = 0.0;
     bool norm_a_valid = false;
-    std::vector<int64_t> syrk_flops_of_launch;
+    std::vector<int64_t> syrk_flops_of_launch, syrk_flops_of_launch_red;
     // learned selector: weights, per-column statistics of the assembled
     // pivot columns, per-front features, candidate fronts per level (SPD:
     // the K_SELECT launches' items)
@@ -339,6 +356,9 @@ int ldlt_export(smfgpu_plan_s *P, int64_t *sizes, int64_t *steps, double *fv, in
 // may alias), job as ldlt_solve, for either factor kind
 int solve_device(smfgpu_plan_s *P, const double *b_dev, int64_t ldb, double *out_dev, int64_t ldo, int32_t nrhs,
                  int32_t job = 0, double *ybuf = nullptr, int64_t ycap = 0);
+// after a synchronization: an error if a persistent sweep's dependency wait
+// ran out since the last check (never expected; the flag is cleared)
+int sweep_check(smfgpu_plan_s *P);
 // device address of the no-allocation solve's work block on `device`,
 // resolved once per device (noalloc.hip); the first call must be made with
 // that device current

@@ -153,6 +153,31 @@ def test_mixed_precision_refinement(oracle_lib, prec):
     assert abs(st["eta"] - eta) <= 1e-3 * eta + 1e-30
 
 
+def test_single_launch_sweeps_match_stepped():
+    """The single-launch sweeps (one workgroup per item, ordered by tickets
+    and per-step counters) against the per-step launches on spd27 40^3:
+    1, 4 and 5 right-hand sides, every solve job, within 1e-14 (the
+    forward updates of shared rows are atomic adds in either order)."""
+    from opt_solver_amd._lib import TUNE_SOLVE_PERSISTENT
+    m = 40
+    print("case spd27_40 single-launch sweeps", flush=True)
+    A = g.spd27_3d(m, seed=4)
+    an = analyse(A, perm=o.grid_nd((m, m, m), leaf=64), nemin=32)
+    plan = Plan(an, posdef=True)
+    fac = plan.factor(A[3])
+    B = np.random.default_rng(6).uniform(-1, 1, (A[0], 5))
+    for nr in (1, 4, 5):
+        for job in (0, 1, 2, 3, 4):
+            b = B[:, :nr] if nr > 1 else B[:, 0]
+            plan.tune(TUNE_SOLVE_PERSISTENT, 0)
+            ref = fac.solve(b, job=job)
+            plan.tune(TUNE_SOLVE_PERSISTENT, 1)
+            got = fac.solve(b, job=job)
+            err = np.abs(got - ref).max() / max(np.abs(ref).max(), 1e-300)
+            assert err <= 1e-14, (nr, job, err)
+    plan.tune(TUNE_SOLVE_PERSISTENT, 0)
+
+
 def test_not_positive_definite_detected():
     print("case not posdef", flush=True)
     A = g.laplace7_3d(6, shift=7.0)  # diagonal -1: indefinite
