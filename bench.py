@@ -230,7 +230,8 @@ def seq_measure(args, torch):
     bs = [torch.from_numpy(_matvec((A0[0], A0[1], A0[2], v.cpu().numpy()), x)).to(dev) for v in vals]
     torch.cuda.synchronize()
     out = {}
-    for name, prec in (("fp64", PREC_FP64), ("fp32", PREC_FP32), ("learned", PREC_AUTO)):
+    paths = (("fp64", PREC_FP64), ("fp32", PREC_FP32), ("learned", PREC_AUTO))
+    for name, prec in [pp for pp in paths if pp[0] in args.seq_paths.split(",")]:
         run = Runner(plan, vals[0], bs[0], 1, an.flops, False)
         run.step(prec)  # warm-up
         times, etas, steps, flops, ok = [], [], [], 0.0, True
@@ -246,6 +247,8 @@ def seq_measure(args, torch):
             steps.append(info.get("steps", 0))
             flops += fac.flops
             ok = ok and tuple(fac.inertia) == (nx, m, 0)
+            if (k + 1) % 10 == 0:
+                log("seq", name, f"refactor {k + 1}/{nref}: {times[-1] * 1e3:.1f} ms, eta {etas[-1]:.2e}")
         tot = sum(times)
         out[name] = {"total_s": round(tot, 3), "ms_per_refactor": round(tot / nref * 1e3, 2),
                      "gflops": round(flops / tot / 1e9, 1), "inertia_ok_every_refactor": ok,
@@ -254,7 +257,8 @@ def seq_measure(args, torch):
     return {"workload": f"kkt_grid {G}^2, m={m} (n={A0[0]}), {nref} refactors of Sigma_k, u=0.01, "
                         f"geometric ND of the grid + constraints", "nsup": an.nsup, "max_front": an.max_front,
             "flops_per_factor_delay_free": an.flops, **out,
-            "speedup_learned_vs_fp64": round(out["fp64"]["total_s"] / out["learned"]["total_s"], 3)}
+            "speedup_learned_vs_fp64": (round(out["fp64"]["total_s"] / out["learned"]["total_s"], 3)
+                                        if "fp64" in out and "learned" in out else None)}
 
 
 def batch_systems(nsys):
@@ -388,6 +392,8 @@ def main():
     ap.add_argument("--cpu-m", type=int, default=76)
     ap.add_argument("--seq-grid", type=int, default=300, help="grid side of the refactor sequence")
     ap.add_argument("--seq-refactors", type=int, default=40)
+    ap.add_argument("--seq-paths", default="fp64,fp32,learned", help="paths of the refactor sequence")
+    ap.add_argument("--only-seq", action="store_true", help="run the refactor sequence only (no SPD legs)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-rhs16", action="store_true")
     ap.add_argument("--no-kkt", action="store_true")
@@ -413,6 +419,15 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    if args.only_seq:
+        # the refactor sequence alone (a long run at a larger grid): one JSON
+        # line with the sequence's own numbers
+        seq = seq_measure(args, torch)
+        tot = seq["learned"]["total_s"] if "learned" in seq else seq["fp64"]["total_s"]
+        print(json.dumps({"metric": "factor+solve s (40-refactor IPM sequence)", "value": round(tot, 3),
+                          "unit": "s", "n_gpus": 1, "higher_is_better": False, "data": "synthetic",
+                          "config": {"workload": seq["workload"]}, "ipm_sequence": seq}))
+        return
     an, ps, ordering, res = c4_measure(args, torch)
     r1 = res[1]
     learned, fp64 = r1["learned"], r1["fp64"]
