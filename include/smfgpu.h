@@ -410,6 +410,12 @@ SMFGPU_API int smfgpu_factor_solve(smfgpu_plan_t plan, const double *values, int
  * factorizations whose fronts may run below FP64 (default 256: their
  * trailing updates accumulate that many products in FP32). */
 #define SMFGPU_TUNE_PANEL_OUTER_REDUCED 12
+/* SMFGPU_TUNE_LDLT_PLANES: 1 (default) gives the large LDL^T fronts of BF16
+ * and split-BF16 precision their rank-512 updates (the held-back pivots'
+ * flush and the contribution block's) on bf16 copies of the operands,
+ * written once per update; 0 rounds the FP64 operands inside every tile.
+ * Both round and sum the same way: the factors agree bit for bit. */
+#define SMFGPU_TUNE_LDLT_PLANES 13
 SMFGPU_API int smfgpu_plan_tune(smfgpu_plan_t plan, int32_t key, int64_t value);
 
 /* Device time (ms) of the last factorization's dominant kernel family:

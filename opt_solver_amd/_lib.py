@@ -101,6 +101,7 @@ TUNE_ASM_CHUNK = 9
 TUNE_LDLT_DEVICE_WAVE = 10
 TUNE_SOLVE_PERSISTENT = 11
 TUNE_PANEL_OUTER_REDUCED = 12
+TUNE_LDLT_PLANES = 13
 
 _lib = None
 

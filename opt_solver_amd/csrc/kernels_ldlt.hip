@@ -1016,7 +1016,10 @@ __device__ __forceinline__ void super_block_tile(int nrb, int ncb, int tile, int
 // 1: FP32 MFMA on the FP64 operands (rounded as they enter LDS); 2: BF16
 // MFMA (operands rounded to bf16); 3: split BF16 (hi / lo operands, four
 // MFMAs). Modes 1-3 accumulate in FP32 over the panel and subtract the
-// result from the FP64 entries.
+// result from the FP64 entries. Modes 2-3 with pl: the operands come from
+// the bf16 planes k_big_pack wrote (plane column t - t0, same rounding and
+// the same chunks of kBigKcBf columns, so the same sums), at half (split)
+// or a quarter (BF16) of the FP64 operands' bytes.
 constexpr int kBigKcBf = 32;  // panel columns per LDS stage of the BF16 tiles (FP64 operands in registers)
 template <int MODE>
 __host__ __device__ constexpr int big_lds_bytes() {
@@ -1024,7 +1027,7 @@ __host__ __device__ constexpr int big_lds_bytes() {
 }
 template <int MODE>
 __device__ __forceinline__ void big_tile(const LdltDev &d, const FrontLd &F, const double *Bm, int boff, int c0, int c1, int t0,
-                         int t1, int tile, unsigned char *smem) {
+                         int t1, int tile, unsigned char *smem, const uint16_t *pl = nullptr) {
     typedef typename std::conditional<MODE == 0, double, float>::type E;
     typedef SyrkMfma<E> M;
     const int f = F.f;
@@ -1043,6 +1046,14 @@ __device__ __forceinline__ void big_tile(const LdltDev &d, const FrontLd &F, con
     if constexpr (MODE <= 1) {
         if (active) tile_loop<E, true, 16, 2, double>(G, Bm, f, boff, i0, j0, f, c1, t0, t1, smem, acc);
         else tile_loop<E, false, 16, 2, double>(G, Bm, f, boff, i0, j0, f, c1, t0, t1, smem, acc);
+    } else if (pl != nullptr) {
+        const int64_t ps = static_cast<int64_t>(kBigWork) * f;
+        if (active)
+            tile_loop_bf16p<MODE == 3, true, kBigKcBf>(pl, pl + ps, pl + 2 * ps, pl + 3 * ps, f, i0, j0, f, c1, 0,
+                                                       t1 - t0, smem, acc);
+        else
+            tile_loop_bf16p<MODE == 3, false, kBigKcBf>(pl, pl + ps, pl + 2 * ps, pl + 3 * ps, f, i0, j0, f, c1, 0,
+                                                        t1 - t0, smem, acc);
     } else {
         if (active) tile_loop_bf16<MODE == 3, true, kBigKcBf, double>(G, Bm, f, boff, i0, j0, f, c1, t0, t1, smem, acc);
         else tile_loop_bf16<MODE == 3, false, kBigKcBf, double>(G, Bm, f, boff, i0, j0, f, c1, t0, t1, smem, acc);
@@ -1625,10 +1636,12 @@ __global__ __launch_bounds__(kLT, 2) void k_big_update(LdltDev d, const BigLd *_
     const int t1 = (B.u_p1 > B.u_p0) ? big_tiles(F.f, B.u_c0, B.u_c1) : 0;
     const int t2 = (B.x_p1 > B.x_p0) ? big_tiles(F.f, B.x_c0, F.ne) : 0;
     const XcdRange xr = xcd_range(t1 + t2);
+    // the flush reads the planes k_big_pack wrote (BF16 modes)
+    const uint16_t *pl = (MODE >= 2 && d.xpl != nullptr) ? d.xpl + 4 * F.wofs : nullptr;
     for (int tile = xr.lo; tile < xr.hi; tile += xr.step) {
         const bool u = tile < t1;
         big_tile<MODE>(d, F, W, B.w_base, u ? B.u_c0 : B.x_c0, u ? B.u_c1 : F.ne, u ? B.u_p0 : B.x_p0,
-                    u ? B.u_p1 : B.x_p1, u ? tile : tile - t1, smem);
+                    u ? B.u_p1 : B.x_p1, u ? tile : tile - t1, smem, u ? nullptr : pl);
         __syncthreads();
     }
 }
@@ -1715,10 +1728,71 @@ __global__ __launch_bounds__(kLT, 2) void k_big_cbupdate(LdltDev d, const BigLd 
     const int c1 = min(B.k, c0 + width);
     if (c1 <= c0 || F.ne >= F.f) return;
     const double *W = d.work + F.wofs;
+    const uint16_t *pl = (MODE >= 2 && d.xpl != nullptr) ? d.xpl + 4 * F.wofs : nullptr;
     const XcdRange xr = xcd_range(big_tiles(F.f, F.ne, F.f));
     for (int tile = xr.lo; tile < xr.hi; tile += xr.step) {
-        big_tile<MODE>(d, F, W, c0, F.ne, F.f, c0, c1, tile, smem);
+        big_tile<MODE>(d, F, W, c0, F.ne, F.f, c0, c1, tile, smem, pl);
         __syncthreads();
+    }
+}
+
+// bf16 planes of a rank-kBigWork update's operands in the packed layout of
+// tile_loop_bf16p (row i, plane column c at (c / 8) f 8 + 8 i + c % 8), for
+// the fronts of BF16 (high part) or split-BF16 (both parts) precision:
+// A = front columns [p0, p0 + np), rows [r0, f); B = W columns from p0 - w0,
+// rows [r0, rb); plane columns [np, np rounded up to 8) zero. CB: the
+// contribution-block chunk at c0 (W from k_big_ld), else the flush request.
+template <bool CB>
+__global__ __launch_bounds__(kLT) void k_big_pack(LdltDev d, const BigLd *__restrict__ states, int32_t c0,
+                                                  int32_t width) {
+    const BigLd B = states[blockIdx.y];
+    if (B.prec < 2) return;
+    const FrontLd F = d.fronts[B.s];
+    const int f = F.f;
+    int p0, np, w0, r0, rb;
+    if (CB) {
+        const int c1 = min(B.k, c0 + width);
+        if (c1 <= c0 || F.ne >= f) return;
+        p0 = c0;
+        np = c1 - c0;
+        w0 = c0;
+        r0 = F.ne;
+        rb = f;
+    } else {
+        if (B.x_p1 <= B.x_p0) return;
+        p0 = B.x_p0;
+        np = B.x_p1 - B.x_p0;
+        w0 = B.w_base;
+        r0 = B.x_c0;
+        rb = F.ne;
+    }
+    const bool split = B.prec == 3;
+    const double *G = d.arena + F.fofs;
+    const double *W = d.work + F.wofs;
+    uint16_t *pl = d.xpl + 4 * F.wofs;
+    const int64_t ps = static_cast<int64_t>(kBigWork) * f;
+    const int ng = (np + 7) / 8;
+    const int nr = f - r0;
+    const int64_t tot = 2 * static_cast<int64_t>(ng) * nr;
+    for (int64_t idx = static_cast<int64_t>(blockIdx.x) * kLT + threadIdx.x; idx < tot;
+         idx += static_cast<int64_t>(gridDim.x) * kLT) {
+        const int i = r0 + static_cast<int>(idx % nr);
+        const int64_t q = idx / nr;
+        const int g = static_cast<int>(q % ng);
+        const int op = static_cast<int>(q / ng);  // 0: A, 1: B
+        if (op == 1 && i >= rb) continue;
+        const double *src = op == 0 ? G + static_cast<int64_t>(p0) * f : W + static_cast<int64_t>(p0 - w0) * f;
+        v8bf_t hi, lo;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int c = 8 * g + u;
+            const float x = c < np ? static_cast<float>(src[static_cast<int64_t>(c) * f + i]) : 0.f;
+            hi[u] = static_cast<__bf16>(x);
+            lo[u] = static_cast<__bf16>(x - static_cast<float>(hi[u]));
+        }
+        const int64_t o = static_cast<int64_t>(op) * 2 * ps + static_cast<int64_t>(g) * f * 8 + static_cast<int64_t>(i) * 8;
+        *reinterpret_cast<v8bf_t *>(pl + o) = hi;
+        if (split) *reinterpret_cast<v8bf_t *>(pl + o + ps) = lo;
     }
 }
 
@@ -1759,6 +1833,7 @@ void launch_big_update(hipStream_t st, const LdltDev &d, const BigLd *states, in
                        const int32_t *nmode) {
     if (nbig <= 0) return;
     const dim3 g(per_front, nbig);
+    if (d.xpl != nullptr && nmode[2] + nmode[3] > 0) k_big_pack<false><<<g, kLT, 0, st>>>(d, states, 0, 0);
     if (nmode[0] > 0) k_big_update<0><<<g, kLT, 0, st>>>(d, states);
     if (nmode[1] > 0) k_big_update<1><<<g, kLT, 0, st>>>(d, states);
     if (nmode[2] > 0) k_big_update<2><<<g, kLT, 0, st>>>(d, states);
@@ -1775,6 +1850,7 @@ void launch_big_cb(hipStream_t st, const LdltDev &d, const BigLd *states, int32_
     if (nbig <= 0) return;
     const dim3 g(per_front, nbig);
     k_big_ld<<<g, kLT, 0, st>>>(d, states, c0, width);
+    if (d.xpl != nullptr && nmode[2] + nmode[3] > 0) k_big_pack<true><<<g, kLT, 0, st>>>(d, states, c0, width);
     if (nmode[0] > 0) k_big_cbupdate<0><<<g, kLT, 0, st>>>(d, states, c0, width);
     if (nmode[1] > 0) k_big_cbupdate<1><<<g, kLT, 0, st>>>(d, states, c0, width);
     if (nmode[2] > 0) k_big_cbupdate<2><<<g, kLT, 0, st>>>(d, states, c0, width);

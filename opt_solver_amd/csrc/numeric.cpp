@@ -700,7 +700,7 @@ SMFGPU_API int smfgpu_plan_stats(smfgpu_plan_t P, double *out) {
     bytes += P->d_fl.bytes() + P->d_arena.bytes() + P->d_work.bytes() + P->d_dinv.bytes() + P->d_fac.bytes() +
              P->d_lrows.bytes() + P->d_lcmap.bytes() + P->d_flist.bytes() + P->d_ltags.bytes() +
              P->d_dkind.bytes() + P->d_stage.bytes() + P->d_child_ptr.bytes() + P->d_child_idx.bytes() +
-             P->d_parent.bytes();
+             P->d_parent.bytes() + P->d_xpl.bytes();
     out[0] = bytes;
     out[1] = static_cast<double>(P->level_fronts.size());
     out[2] = static_cast<double>(P->launches.size());
@@ -784,6 +784,10 @@ SMFGPU_API int smfgpu_plan_tune(smfgpu_plan_t P, int32_t key, int64_t value) {
     case SMFGPU_TUNE_SOLVE_PERSISTENT:
         if (value != 0 && value != 1) return set_error(SMFGPU_ERR_INVALID_ARG, "tune: 0 or 1");
         P->sweep_persistent = static_cast<int32_t>(value);
+        return SMFGPU_OK;
+    case SMFGPU_TUNE_LDLT_PLANES:
+        if (value != 0 && value != 1) return set_error(SMFGPU_ERR_INVALID_ARG, "tune: 0 or 1");
+        P->ldlt_planes = static_cast<int32_t>(value);
         return SMFGPU_OK;
     case SMFGPU_TUNE_SYRK_AHEAD:
         if (value != 1 && value != 2) return set_error(SMFGPU_ERR_INVALID_ARG, "tune: chunks ahead must be 1 or 2");

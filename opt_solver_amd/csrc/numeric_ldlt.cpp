@@ -263,11 +263,13 @@ static int ldlt_dev_levels(smfgpu_plan_s *P, int32_t lv0, double u, double eff_s
 // update and stall kernels in turn until every front is done (the state
 // read back every kBigCheck rounds), then the contribution blocks' update
 // in chunks of kBigWork pivots and the fronts' results.
-static int ldlt_big_fronts(smfgpu_plan_s *P, const LdltDev &d, const std::vector<int32_t> &big,
+static int ldlt_big_fronts(smfgpu_plan_s *P, const LdltDev &d0, const std::vector<int32_t> &big,
                            const smfgpu_factor_options &opt, double eff_small, double small, int32_t nb,
                            std::vector<BigLd> &h, int32_t &reduced) {
     constexpr int kBigCheck = 16;
     hipStream_t st = P->stream;
+    LdltDev d = d0;
+    d.xpl = nullptr;
     const int32_t nbig = static_cast<int32_t>(big.size());
     const bool group = P->ldlt_group != 0 && nb == 16;
     const bool autosel = opt.precision == SMFGPU_PREC_AUTO;
@@ -333,6 +335,14 @@ static int ldlt_big_fronts(smfgpu_plan_s *P, const LdltDev &d, const std::vector
     }
     reduced += nbig - nmode[0];
     if ((rc = ensure(P->d_big, static_cast<size_t>(nbig), false, "large-front states"))) return rc;
+    // bf16 planes of the rank-kBigWork updates' operands (BF16 and split-BF16
+    // fronts, and the training pairs' split-BF16 and BF16 runs)
+    if (P->ldlt_planes != 0 && (nmode[2] + nmode[3] > 0 || (collect && !sf.empty()))) {
+        int64_t planes = 0;
+        for (int32_t s : big) planes = std::max(planes, 4 * (P->fl[s].wofs + static_cast<int64_t>(P->fl[s].f) * kBigWork));
+        if ((rc = ensure(P->d_xpl, static_cast<size_t>(planes), false, "update planes"))) return rc;
+        d.xpl = P->d_xpl.p;
+    }
     HIP_OK(hipMemcpyAsync(P->d_big.p, h.data(), nbig * sizeof(BigLd), hipMemcpyHostToDevice, st));
     // update workgroups per front: a multiple of 8 (one run of tiles per XCD)
     const int32_t per_front = std::max(64, P->ldlt_update_wg / nbig) / 8 * 8;

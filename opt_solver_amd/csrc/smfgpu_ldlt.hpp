@@ -51,6 +51,8 @@ struct LdltDev {
     const int32_t *amap;      // per permuted entry: position among its front's symbolic rows
     const double *av;
     double *junk;             // target of masked-out stores of the MFMA updates (contents meaningless)
+    uint16_t *xpl;            // bf16 operand planes of the large fronts' rank-kBigWork updates (BF16 / split BF16):
+                              // per front at 4 wofs, planes A hi, A lo, B hi, B lo of kBigWork f entries each
 };
 
 // Large fronts (multi-workgroup blocked LDL^T): one state per front,

@@ -226,6 +226,7 @@ struct smfgpu_plan_s {
     DBuf<SyrkItem> d_syrk;
     DBuf<int8_t> d_prec;
     DBuf<double> d_junk;
+    DBuf<uint16_t> d_xpl;  // large fronts' bf16 update operands (allocated when a BF16 / split-BF16 front needs them)
     DBuf<float> d_L32;  // allocated at the first factorization with FP32 Schur updates
     DBuf<uint16_t> d_L16h, d_L16l;  // packed bf16 planes, allocated (zeroed) at the first BF16 factorization
     int64_t l16_entries = 0;
@@ -252,6 +253,7 @@ struct smfgpu_plan_s {
     int32_t cb_seg = 4096;                         // SPD: pivot columns per contribution-block update
     int32_t ldlt_big_min = 256;                    // LDL^T: fronts with at least this many rows take the large-front path
     DBuf<BigLd> d_big;                             // LDL^T: large-front states of the level
+    int32_t ldlt_planes = 1;                       // LDL^T: bf16 operand planes for the large fronts' rank-512 updates
     int32_t ldlt_group = 1;                        // LDL^T: speculative panel groups on the large-front path
     int32_t ldlt_update_wg = 1024;                 // LDL^T: workgroups of the large-front update launches
     DBuf<double> d_gpart;                          // LDL^T: per-workgroup column maxima of a group

@@ -208,6 +208,36 @@ def test_large_front_default_threshold(oracle_lib):
     assert eta <= 1e-13, eta
 
 
+@pytest.mark.parametrize("prec", ["bf16x3", "bf16"])
+def test_large_front_planes_bitwise(prec):
+    """The large fronts' rank-512 updates in BF16 and split BF16 read bf16
+    copies of their operands written once per update (the held-back
+    pivots' flush past each 512-column outer block, the contribution
+    block's chunks): the same factor, bit for bit, as rounding the FP64
+    operands inside every tile (the same roundings, chunks and MFMA order),
+    on a grid KKT whose largest fronts (3016 rows) have several outer
+    blocks and contribution blocks."""
+    from opt_solver_amd._lib import TUNE_LDLT_PLANES
+    from opt_solver_amd.solver import PREC_BF16, PREC_BF16X3
+    print("case planes", prec, flush=True)
+    gsz = 120
+    m = gsz * gsz // 5
+    A, _ = g.kkt_grid(gsz, m, seed=gsz)
+    an = analyse(A, nemin=16)
+    plan = Plan(an, posdef=False)
+    code = {"bf16x3": PREC_BF16X3, "bf16": PREC_BF16}[prec]
+    ex = {}
+    for planes in (1, 0):
+        plan.tune(TUNE_LDLT_PLANES, planes)
+        fac = plan.factor(A[3], u=0.01, precision=code, mixed_min_front=512)
+        assert fac.status == 0 and fac.info.fronts_reduced > 0
+        ex[planes] = (fac.export(), fac.inertia, fac.info.delayed_pivots)
+    assert ex[0][1:] == ex[1][1:]
+    for key in ("steps", "row_indices", "pivot_types", "factor_values"):
+        assert np.array_equal(ex[0][0][key], ex[1][0][key]), key
+    print(f"  reduced fronts {fac.info.fronts_reduced} inertia {ex[1][1]}", flush=True)
+
+
 @pytest.mark.parametrize("case", ["ocp_T200", "kkt_grid_60"])
 def test_blocked_solve_multi_rhs(oracle_lib, case):
     """The blocked LDL^T sweeps (64-pivot steps, the rows past a front's
