@@ -411,11 +411,19 @@ SMFGPU_API int smfgpu_factor_solve(smfgpu_plan_t plan, const double *values, int
  * trailing updates accumulate that many products in FP32). */
 #define SMFGPU_TUNE_PANEL_OUTER_REDUCED 12
 /* SMFGPU_TUNE_LDLT_PLANES: 1 (default) gives the large LDL^T fronts of BF16
- * and split-BF16 precision their rank-512 updates (the held-back pivots'
- * flush and the contribution block's) on bf16 copies of the operands,
- * written once per update; 0 rounds the FP64 operands inside every tile.
- * Both round and sum the same way: the factors agree bit for bit. */
+ * and split-BF16 precision their outer-block updates (the held-back pivots'
+ * flush and the contribution block's chunks) on bf16 copies of the
+ * operands, written once per update, in 256 x 256 tiles; 0 rounds the FP64
+ * operands inside every 128 x 128 tile. With the same outer block (512:
+ * SMFGPU_TUNE_LDLT_OUTER_WIDE) both round and sum the same way and the
+ * factors agree bit for bit. */
 #define SMFGPU_TUNE_LDLT_PLANES 13
+/* SMFGPU_TUNE_LDLT_OUTER_WIDE: outer block (and contribution-block chunk)
+ * of the large LDL^T fronts updated on bf16 planes, a multiple of 64 in
+ * [64, 1024] (default 1024; the other fronts use 512). Each rank-w update
+ * reads and writes the FP64 entries once, and its FP32 accumulators sum w
+ * products. */
+#define SMFGPU_TUNE_LDLT_OUTER_WIDE 14
 SMFGPU_API int smfgpu_plan_tune(smfgpu_plan_t plan, int32_t key, int64_t value);
 
 /* Device time (ms) of the last factorization's dominant kernel family:

@@ -102,6 +102,7 @@ TUNE_LDLT_DEVICE_WAVE = 10
 TUNE_SOLVE_PERSISTENT = 11
 TUNE_PANEL_OUTER_REDUCED = 12
 TUNE_LDLT_PLANES = 13
+TUNE_LDLT_OUTER_WIDE = 14
 
 _lib = None
 

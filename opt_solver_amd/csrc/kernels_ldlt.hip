@@ -1016,10 +1016,7 @@ __device__ __forceinline__ void super_block_tile(int nrb, int ncb, int tile, int
 // 1: FP32 MFMA on the FP64 operands (rounded as they enter LDS); 2: BF16
 // MFMA (operands rounded to bf16); 3: split BF16 (hi / lo operands, four
 // MFMAs). Modes 1-3 accumulate in FP32 over the panel and subtract the
-// result from the FP64 entries. Modes 2-3 with pl: the operands come from
-// the bf16 planes k_big_pack wrote (plane column t - t0, same rounding and
-// the same chunks of kBigKcBf columns, so the same sums), at half (split)
-// or a quarter (BF16) of the FP64 operands' bytes.
+// result from the FP64 entries.
 constexpr int kBigKcBf = 32;  // panel columns per LDS stage of the BF16 tiles (FP64 operands in registers)
 template <int MODE>
 __host__ __device__ constexpr int big_lds_bytes() {
@@ -1027,7 +1024,7 @@ __host__ __device__ constexpr int big_lds_bytes() {
 }
 template <int MODE>
 __device__ __forceinline__ void big_tile(const LdltDev &d, const FrontLd &F, const double *Bm, int boff, int c0, int c1, int t0,
-                         int t1, int tile, unsigned char *smem, const uint16_t *pl = nullptr) {
+                         int t1, int tile, unsigned char *smem) {
     typedef typename std::conditional<MODE == 0, double, float>::type E;
     typedef SyrkMfma<E> M;
     const int f = F.f;
@@ -1046,14 +1043,6 @@ __device__ __forceinline__ void big_tile(const LdltDev &d, const FrontLd &F, con
     if constexpr (MODE <= 1) {
         if (active) tile_loop<E, true, 16, 2, double>(G, Bm, f, boff, i0, j0, f, c1, t0, t1, smem, acc);
         else tile_loop<E, false, 16, 2, double>(G, Bm, f, boff, i0, j0, f, c1, t0, t1, smem, acc);
-    } else if (pl != nullptr) {
-        const int64_t ps = static_cast<int64_t>(kBigWork) * f;
-        if (active)
-            tile_loop_bf16p<MODE == 3, true, kBigKcBf>(pl, pl + ps, pl + 2 * ps, pl + 3 * ps, f, i0, j0, f, c1, 0,
-                                                       t1 - t0, smem, acc);
-        else
-            tile_loop_bf16p<MODE == 3, false, kBigKcBf>(pl, pl + ps, pl + 2 * ps, pl + 3 * ps, f, i0, j0, f, c1, 0,
-                                                        t1 - t0, smem, acc);
     } else {
         if (active) tile_loop_bf16<MODE == 3, true, kBigKcBf, double>(G, Bm, f, boff, i0, j0, f, c1, t0, t1, smem, acc);
         else tile_loop_bf16<MODE == 3, false, kBigKcBf, double>(G, Bm, f, boff, i0, j0, f, c1, t0, t1, smem, acc);
@@ -1491,7 +1480,7 @@ __global__ __launch_bounds__(kLT) void k_big_panel(LdltDev d, BigLd *states, Prm
                     x_p1 = kn;
                 }
                 pend0 = kn;
-                oe = min(ne, kn + kBigOuter);
+                oe = min(ne, kn + B.outer);
             }
             g_apply = 1;
             g_ps = k;
@@ -1545,7 +1534,7 @@ __global__ __launch_bounds__(kLT) void k_big_panel(LdltDev d, BigLd *states, Prm
                     x_p1 = k;
                 }
                 pend0 = k;
-                oe = min(ne, k + kBigOuter);
+                oe = min(ne, k + B.outer);
             }
         }
     } else {
@@ -1634,14 +1623,14 @@ __global__ __launch_bounds__(kLT, 2) void k_big_update(LdltDev d, const BigLd *_
     const FrontLd F = d.fronts[B.s];
     const double *W = d.work + F.wofs;
     const int t1 = (B.u_p1 > B.u_p0) ? big_tiles(F.f, B.u_c0, B.u_c1) : 0;
-    const int t2 = (B.x_p1 > B.x_p0) ? big_tiles(F.f, B.x_c0, F.ne) : 0;
+    // BF16 modes on planes: the flush is k_big_wide's
+    const bool wide = MODE >= 2 && d.xpl != nullptr;
+    const int t2 = (!wide && B.x_p1 > B.x_p0) ? big_tiles(F.f, B.x_c0, F.ne) : 0;
     const XcdRange xr = xcd_range(t1 + t2);
-    // the flush reads the planes k_big_pack wrote (BF16 modes)
-    const uint16_t *pl = (MODE >= 2 && d.xpl != nullptr) ? d.xpl + 4 * F.wofs : nullptr;
     for (int tile = xr.lo; tile < xr.hi; tile += xr.step) {
         const bool u = tile < t1;
         big_tile<MODE>(d, F, W, B.w_base, u ? B.u_c0 : B.x_c0, u ? B.u_c1 : F.ne, u ? B.u_p0 : B.x_p0,
-                    u ? B.u_p1 : B.x_p1, u ? tile : tile - t1, smem, u ? nullptr : pl);
+                    u ? B.u_p1 : B.x_p1, u ? tile : tile - t1, smem);
         __syncthreads();
     }
 }
@@ -1688,8 +1677,9 @@ __global__ __launch_bounds__(kLT) void k_big_stall(LdltDev d, BigLd *states, int
 // L*D columns [c0, c1) of every done front, rows [ne, f), into W (the
 // column order of schur_update's fill_ld_column, ldlt_kernel.cpp:764-816)
 __global__ __launch_bounds__(kLT) void k_big_ld(LdltDev d, const BigLd *__restrict__ states, int32_t c0,
-                                               int32_t width) {
+                                               int32_t width, int32_t wide) {
     const BigLd B = states[blockIdx.y];
+    if ((B.prec >= 2 && d.xpl != nullptr) != (wide != 0)) return;  // the other chunk width's fronts
     const FrontLd F = d.fronts[B.s];
     const int f = F.f, ne = F.ne, p = B.k;
     const int c1 = min(p, c0 + width);
@@ -1728,10 +1718,97 @@ __global__ __launch_bounds__(kLT, 2) void k_big_cbupdate(LdltDev d, const BigLd 
     const int c1 = min(B.k, c0 + width);
     if (c1 <= c0 || F.ne >= F.f) return;
     const double *W = d.work + F.wofs;
-    const uint16_t *pl = (MODE >= 2 && d.xpl != nullptr) ? d.xpl + 4 * F.wofs : nullptr;
     const XcdRange xr = xcd_range(big_tiles(F.f, F.ne, F.f));
     for (int tile = xr.lo; tile < xr.hi; tile += xr.step) {
-        big_tile<MODE>(d, F, W, c0, F.ne, F.f, c0, c1, tile, smem, pl);
+        big_tile<MODE>(d, F, W, c0, F.ne, F.f, c0, c1, tile, smem);
+        __syncthreads();
+    }
+}
+
+// 256 x 256 tile of the rank-kBigWork updates on the bf16 planes (modes 2,
+// 3): C[i][j] -= sum_c A[i][c] B[j][c] over plane columns [0, np), lower
+// part of rows [c0, f) x columns [c0, c1) of front F, tile `tile` in
+// super-block order.
+template <int MODE>
+__device__ __forceinline__ void big_tile_w(const LdltDev &d, const FrontLd &F, const uint16_t *pl, int c0, int c1,
+                                           int np, int tile, unsigned char *smem) {
+    const int f = F.f;
+    int ib, jb;
+    super_block_tile((f - c0 + kTW - 1) / kTW, (c1 - c0 + kTW - 1) / kTW, tile, ib, jb);
+    const int i0 = c0 + ib * kTW, j0 = c0 + jb * kTW;
+    double *G = d.arena + F.fofs;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int wr = w & 3, wc = w >> 2;
+    const bool active = (i0 + wr * 64 + 63) >= (j0 + wc * 128);
+    v4f_t acc[8][4];
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = v4f_t{0.f, 0.f, 0.f, 0.f};
+    const int64_t ps = static_cast<int64_t>(kBigWork) * f;
+    tile_loop_bf16p_w<MODE == 3, kBigKcBf>(pl, pl + ps, pl + 2 * ps, pl + 3 * ps, f, i0, j0, f, c1, 0, np, active, smem,
+                                           acc);
+    if (!active) return;
+    double *junk = d.junk + threadIdx.x;
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        double *dst[4][4];
+        double old[4][4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int j = j0 + wc * 128 + m * 16 + 4 * (lane >> 4) + v;
+            double *col = G + static_cast<int64_t>(min(j, c1 - 1)) * f;
+#pragma unroll
+            for (int nn = 0; nn < 4; ++nn) {
+                const int i = i0 + wr * 64 + nn * 16 + (lane & 15);
+                dst[nn][v] = (i < f && j < c1 && i >= j) ? col + i : junk;
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+            for (int nn = 0; nn < 4; ++nn) old[nn][v] = *dst[nn][v];
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+            for (int nn = 0; nn < 4; ++nn) *dst[nn][v] = old[nn][v] - acc[m][nn][v];
+    }
+}
+
+__device__ __forceinline__ int big_tiles_w(int f, int c0, int c1) {
+    if (c1 <= c0) return 0;
+    const int nr = (f - c0 + kTW - 1) / kTW, nc = (c1 - c0 + kTW - 1) / kTW;
+    return nc * nr - nc * (nc - 1) / 2;
+}
+
+// The rank-kBigWork updates of BF16 / split-BF16 fronts on their planes, in
+// 256 x 256 tiles: CB, the contribution block's chunk at c0 (after
+// k_big_ld and k_big_pack<true>); else the flush request (after
+// k_big_pack<false>).
+template <int MODE, bool CB>
+__global__ __launch_bounds__(kWideThreads) void k_big_wide(LdltDev d, const BigLd *__restrict__ states, int32_t c0,
+                                                           int32_t width) {
+    __shared__ __attribute__((aligned(16))) unsigned char smem[bfw_lds_bytes<MODE == 3, kBigKcBf>()];
+    const BigLd B = states[blockIdx.y];
+    if (B.prec != MODE) return;
+    const FrontLd F = d.fronts[B.s];
+    int r0, rc, np;
+    if (CB) {
+        const int c1 = min(B.k, c0 + width);
+        if (c1 <= c0 || F.ne >= F.f) return;
+        r0 = F.ne;
+        rc = F.f;
+        np = c1 - c0;
+    } else {
+        if (B.x_p1 <= B.x_p0) return;
+        r0 = B.x_c0;
+        rc = F.ne;
+        np = B.x_p1 - B.x_p0;
+    }
+    const uint16_t *pl = d.xpl + 4 * F.wofs;
+    const XcdRange xr = xcd_range(big_tiles_w(F.f, r0, rc));
+    for (int tile = xr.lo; tile < xr.hi; tile += xr.step) {
+        big_tile_w<MODE>(d, F, pl, r0, rc, np, tile, smem);
         __syncthreads();
     }
 }
@@ -1829,6 +1906,9 @@ void launch_big_group_apply(hipStream_t st, const LdltDev &d, const BigLd *state
     k_big_group<false><<<dim3(nch, nbig), kLT, 0, st>>>(d, states, gpart, nch);
 }
 
+// workgroups per front of the 256 x 256 tiles (one per CU; a multiple of 8)
+static int32_t wide_per_front(int32_t per_front) { return std::max(8, per_front / 16 * 8); }
+
 void launch_big_update(hipStream_t st, const LdltDev &d, const BigLd *states, int32_t nbig, int32_t per_front,
                        const int32_t *nmode) {
     if (nbig <= 0) return;
@@ -1838,6 +1918,11 @@ void launch_big_update(hipStream_t st, const LdltDev &d, const BigLd *states, in
     if (nmode[1] > 0) k_big_update<1><<<g, kLT, 0, st>>>(d, states);
     if (nmode[2] > 0) k_big_update<2><<<g, kLT, 0, st>>>(d, states);
     if (nmode[3] > 0) k_big_update<3><<<g, kLT, 0, st>>>(d, states);
+    if (d.xpl != nullptr) {
+        const dim3 gw(wide_per_front(per_front), nbig);
+        if (nmode[2] > 0) k_big_wide<2, false><<<gw, kWideThreads, 0, st>>>(d, states, 0, 0);
+        if (nmode[3] > 0) k_big_wide<3, false><<<gw, kWideThreads, 0, st>>>(d, states, 0, 0);
+    }
 }
 
 void launch_big_stall(hipStream_t st, const LdltDev &d, BigLd *states, int32_t nbig, int32_t nb) {
@@ -1846,15 +1931,24 @@ void launch_big_stall(hipStream_t st, const LdltDev &d, BigLd *states, int32_t n
 }
 
 void launch_big_cb(hipStream_t st, const LdltDev &d, const BigLd *states, int32_t nbig, int32_t c0, int32_t width,
-                   int32_t per_front, const int32_t *nmode) {
+                   int32_t per_front, const int32_t *nmode, bool wide) {
     if (nbig <= 0) return;
     const dim3 g(per_front, nbig);
-    k_big_ld<<<g, kLT, 0, st>>>(d, states, c0, width);
-    if (d.xpl != nullptr && nmode[2] + nmode[3] > 0) k_big_pack<true><<<g, kLT, 0, st>>>(d, states, c0, width);
+    k_big_ld<<<g, kLT, 0, st>>>(d, states, c0, width, wide ? 1 : 0);
+    if (wide) {
+        // the BF16 modes' fronts on planes
+        const dim3 gw(wide_per_front(per_front), nbig);
+        k_big_pack<true><<<g, kLT, 0, st>>>(d, states, c0, width);
+        if (nmode[2] > 0) k_big_wide<2, true><<<gw, kWideThreads, 0, st>>>(d, states, c0, width);
+        if (nmode[3] > 0) k_big_wide<3, true><<<gw, kWideThreads, 0, st>>>(d, states, c0, width);
+        return;
+    }
     if (nmode[0] > 0) k_big_cbupdate<0><<<g, kLT, 0, st>>>(d, states, c0, width);
     if (nmode[1] > 0) k_big_cbupdate<1><<<g, kLT, 0, st>>>(d, states, c0, width);
-    if (nmode[2] > 0) k_big_cbupdate<2><<<g, kLT, 0, st>>>(d, states, c0, width);
-    if (nmode[3] > 0) k_big_cbupdate<3><<<g, kLT, 0, st>>>(d, states, c0, width);
+    if (d.xpl == nullptr) {
+        if (nmode[2] > 0) k_big_cbupdate<2><<<g, kLT, 0, st>>>(d, states, c0, width);
+        if (nmode[3] > 0) k_big_cbupdate<3><<<g, kLT, 0, st>>>(d, states, c0, width);
+    }
 }
 
 void launch_big_finish(hipStream_t st, const LdltDev &d, const BigLd *states, int32_t nbig) {

@@ -448,4 +448,117 @@ __device__ __forceinline__ void tile_loop_bf16p(const uint16_t *Ah, const uint16
     }
 }
 
+// ---------------------------------------------------------------------------
+// 256 x 256 tile of eight waves on operands in the packed bf16 layout (for
+// the rank-512 updates of large fronts). Wave w covers A rows
+// [i0 + 64 (w % 4), +64) x B rows [j0 + 128 (w / 4), +128): 8 x 4 MFMA
+// blocks of 16 x 16 in acc[m][nn] (m: B-row block, nn: A-row block, the
+// transposed blocks of tile_loop_bf16p). Per 32-column chunk the workgroup
+// stages 64 KB (split) for four times the products of the 128 x 128 tile's
+// 32 KB. Each accumulator receives the same MFMAs on the same fragments in
+// the same order as in tile_loop_bf16p, so the sums agree bit for bit.
+// LDS: two stages of the four planes, 160 KB (one workgroup per CU).
+constexpr int kTW = 256;
+constexpr int kWideThreads = 512;
+template <bool SPLIT, int kKc>
+__host__ __device__ constexpr int bfw_lds_bytes() { return (SPLIT ? 2 : 1) * 2 * 2 * kTW * bf_lds_ld<kKc>() * 2; }
+template <bool SPLIT, int kKc>
+__device__ __forceinline__ void tile_loop_bf16p_w(const uint16_t *Ah, const uint16_t *Al, const uint16_t *Bh,
+                                                  const uint16_t *Bl, int ld, int i0, int j0, int f, int j1, int t0,
+                                                  int t1, bool active, unsigned char *smem, v4f_t (&acc)[8][4]) {
+    constexpr int kLd = bf_lds_ld<kKc>();
+    constexpr int kVec = kKc / 16;         // 16-byte vectors per thread, operand and plane
+    constexpr int kPlane = 2 * kTW * kLd;  // one operand's two stages, in bf16 elements
+    __bf16 *lds = reinterpret_cast<__bf16 *>(smem);
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int w = tid >> 6;
+    const int wr = w & 3, wc = w >> 2;
+    constexpr uint32_t kOut = 0x80000000u;
+    const int rl = tid & (kTW - 1), th = tid >> 8;
+    const uint32_t grp = static_cast<uint32_t>(ld) * 16u;  // bytes of one 8-column group
+    const uint32_t voa = (i0 + rl < f) ? static_cast<uint32_t>(i0 + rl) * 16u : kOut;
+    const uint32_t vob = (j0 + rl < j1) ? static_cast<uint32_t>(j0 + rl) * 16u : kOut;
+    typedef v4u_t Regs[SPLIT ? 2 * kVec : kVec];
+    auto gload = [&](Regs &ra, Regs &rb, int kc) {
+        const int ng = (min(kKc, t1 - kc) + 7) / 8;
+        const int nrec = ng * static_cast<int>(grp);
+        const int64_t go = static_cast<int64_t>(kc / 8) * ld * 8;
+        const __amdgpu_buffer_rsrc_t sa = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint16_t *>(Ah + go), 0, nrec, kBufferWord3);
+        const __amdgpu_buffer_rsrc_t sb = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint16_t *>(Bh + go), 0, nrec, kBufferWord3);
+#pragma unroll
+        for (int e = 0; e < kVec; ++e) {
+            const uint32_t co = static_cast<uint32_t>(th + 2 * e) * grp;
+            ra[e] = __builtin_amdgcn_raw_buffer_load_b128(sa, static_cast<int>(voa + co), 0, 0);
+            rb[e] = __builtin_amdgcn_raw_buffer_load_b128(sb, static_cast<int>(vob + co), 0, 0);
+        }
+        if (SPLIT) {
+            const __amdgpu_buffer_rsrc_t la = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint16_t *>(Al + go), 0, nrec, kBufferWord3);
+            const __amdgpu_buffer_rsrc_t lb = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint16_t *>(Bl + go), 0, nrec, kBufferWord3);
+#pragma unroll
+            for (int e = 0; e < kVec; ++e) {
+                const uint32_t co = static_cast<uint32_t>(th + 2 * e) * grp;
+                ra[kVec + e] = __builtin_amdgcn_raw_buffer_load_b128(la, static_cast<int>(voa + co), 0, 0);
+                rb[kVec + e] = __builtin_amdgcn_raw_buffer_load_b128(lb, static_cast<int>(vob + co), 0, 0);
+            }
+        }
+    };
+    auto sstore = [&](const Regs &ra, const Regs &rb, int stg) {
+#pragma unroll
+        for (int e = 0; e < kVec; ++e) {
+            const int o = stg * kTW * kLd + rl * kLd + 8 * (th + 2 * e);
+            *reinterpret_cast<v4u_t *>(lds + o) = ra[e];
+            *reinterpret_cast<v4u_t *>(lds + kPlane + o) = rb[e];
+            if (SPLIT) {
+                *reinterpret_cast<v4u_t *>(lds + 2 * kPlane + o) = ra[kVec + e];
+                *reinterpret_cast<v4u_t *>(lds + 3 * kPlane + o) = rb[kVec + e];
+            }
+        }
+    };
+    auto mfma_stage = [&](int stg) {
+        const __bf16 *As = lds + stg * kTW * kLd;
+        const __bf16 *Bs = lds + kPlane + stg * kTW * kLd;
+#pragma unroll
+        for (int kk = 0; kk < kKc; kk += 32) {
+            const int ko = kk + 8 * (lane >> 4);
+            v8bf_t b[4], bl[4];
+#pragma unroll
+            for (int nn = 0; nn < 4; ++nn) {
+                b[nn] = *reinterpret_cast<const v8bf_t *>(As + (wr * 64 + nn * 16 + (lane & 15)) * kLd + ko);
+                if (SPLIT)
+                    bl[nn] = *reinterpret_cast<const v8bf_t *>(As + 2 * kPlane + (wr * 64 + nn * 16 + (lane & 15)) * kLd + ko);
+            }
+#pragma unroll
+            for (int m = 0; m < 8; ++m) {
+                const int ro = (wc * 128 + m * 16 + (lane & 15)) * kLd + ko;
+                const v8bf_t a = *reinterpret_cast<const v8bf_t *>(Bs + ro);
+                if (SPLIT) {
+                    const v8bf_t al = *reinterpret_cast<const v8bf_t *>(Bs + 2 * kPlane + ro);
+#pragma unroll
+                    for (int nn = 0; nn < 4; ++nn) {
+                        acc[m][nn] = mfma_bf16(al, bl[nn], acc[m][nn]);
+                        acc[m][nn] = mfma_bf16(al, b[nn], acc[m][nn]);
+                        acc[m][nn] = mfma_bf16(a, bl[nn], acc[m][nn]);
+                    }
+                }
+#pragma unroll
+                for (int nn = 0; nn < 4; ++nn) acc[m][nn] = mfma_bf16(a, b[nn], acc[m][nn]);
+            }
+        }
+    };
+    int stg = 0;
+    Regs ra, rb;
+    gload(ra, rb, t0);
+    sstore(ra, rb, 0);
+    __syncthreads();
+    for (int kc = t0; kc < t1; kc += kKc) {
+        const bool more = kc + kKc < t1;
+        if (more) gload(ra, rb, kc + kKc);
+        if (active) mfma_stage(stg);
+        if (more) sstore(ra, rb, stg ^ 1);
+        __syncthreads();
+        stg ^= 1;
+    }
+}
+
 }  // namespace smfgpu

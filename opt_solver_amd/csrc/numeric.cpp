@@ -789,6 +789,11 @@ SMFGPU_API int smfgpu_plan_tune(smfgpu_plan_t P, int32_t key, int64_t value) {
         if (value != 0 && value != 1) return set_error(SMFGPU_ERR_INVALID_ARG, "tune: 0 or 1");
         P->ldlt_planes = static_cast<int32_t>(value);
         return SMFGPU_OK;
+    case SMFGPU_TUNE_LDLT_OUTER_WIDE:
+        if (value < 64 || value > kBigWork || value % 64 != 0)
+            return set_error(SMFGPU_ERR_INVALID_ARG, "tune: outer block must be a multiple of 64 in [64, 1024]");
+        P->ldlt_outer_wide = static_cast<int32_t>(value);
+        return SMFGPU_OK;
     case SMFGPU_TUNE_SYRK_AHEAD:
         if (value != 1 && value != 2) return set_error(SMFGPU_ERR_INVALID_ARG, "tune: chunks ahead must be 1 or 2");
         P->syrk_ahead = static_cast<int32_t>(value);

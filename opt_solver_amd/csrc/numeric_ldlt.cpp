@@ -262,7 +262,7 @@ static int ldlt_dev_levels(smfgpu_plan_s *P, int32_t lv0, double u, double eff_s
 // Large fronts of one level (multi-workgroup blocked LDL^T): the panel,
 // update and stall kernels in turn until every front is done (the state
 // read back every kBigCheck rounds), then the contribution blocks' update
-// in chunks of kBigWork pivots and the fronts' results.
+// in chunks of an outer block's pivots and the fronts' results.
 static int ldlt_big_fronts(smfgpu_plan_s *P, const LdltDev &d0, const std::vector<int32_t> &big,
                            const smfgpu_factor_options &opt, double eff_small, double small, int32_t nb,
                            std::vector<BigLd> &h, int32_t &reduced) {
@@ -322,13 +322,14 @@ static int ldlt_big_fronts(smfgpu_plan_s *P, const LdltDev &d0, const std::vecto
         B.s = big[b];
         B.live = F.ne;
         B.k_paired = -1;
-        B.oe = std::min(F.ne, kBigOuter);
         B.spec = group ? 1 : 0;
         // mixed precision: reduced-precision MFMA updates on the fronts of
         // at least mixed_min_front rows (the pivot logic stays FP64), the
         // requested one or the selector's choice
         if (autosel) B.prec = F.f >= opt.mixed_min_front ? P->ldlt_prec[big[b]] : 0;
         else B.prec = code > 0 && F.f >= opt.mixed_min_front ? code : 0;
+        B.outer = (P->ldlt_planes != 0 && B.prec >= 2) ? P->ldlt_outer_wide : kBigOuter;
+        B.oe = std::min(F.ne, B.outer);
         P->ldlt_prec[big[b]] = static_cast<int8_t>(B.prec);
         ++nmode[B.prec];
         cap += 4 * (static_cast<int64_t>(F.ne) / std::max(nb, 1) + 2) + 2 * static_cast<int64_t>(F.ne);
@@ -372,6 +373,16 @@ static int ldlt_big_fronts(smfgpu_plan_s *P, const LdltDev &d0, const std::vecto
     HIP_OK(hipGetLastError());
     int32_t pmax = 0;
     for (const BigLd &B : h) pmax = std::max(pmax, B.k);
+    // contribution blocks: chunks of kBigOuter pivots, of the planes' outer
+    // block for the BF16 modes on planes
+    auto cb_update = [&](const int32_t *nm) {
+        if (nm[0] + nm[1] > 0 || (d.xpl == nullptr && nm[2] + nm[3] > 0))
+            for (int32_t c0 = 0; c0 < pmax; c0 += kBigOuter)
+                launch_big_cb(st, d, P->d_big.p, nbig, c0, kBigOuter, per_front, nm, false);
+        if (d.xpl != nullptr && nm[2] + nm[3] > 0)
+            for (int32_t c0 = 0; c0 < pmax; c0 += P->ldlt_outer_wide)
+                launch_big_cb(st, d, P->d_big.p, nbig, c0, P->ldlt_outer_wide, per_front, nm, true);
+    };
     const bool labels = collect && !sf.empty() && maxq > 0;
     if (labels) {
         // training pairs: the candidates' contribution-block update run in
@@ -400,13 +411,13 @@ static int ldlt_big_fronts(smfgpu_plan_s *P, const LdltDev &d0, const std::vecto
                 if (is_cand[b]) ++nm[m];
             }
             HIP_OK(hipMemcpyAsync(P->d_big.p, h.data(), nbig * sizeof(BigLd), hipMemcpyHostToDevice, st));
-            for (int32_t c0 = 0; c0 < pmax; c0 += kBigWork) launch_big_cb(st, d, P->d_big.p, nbig, c0, kBigWork, per_front, nm);
+            cb_update(nm);
             if ((rc = copy_sq(true, kSelOutOfCode[m])) || (rc = copy_sq(false, 3))) return rc;
         }
         for (BigLd &B : h) B.prec = 0;
         HIP_OK(hipMemcpyAsync(P->d_big.p, h.data(), nbig * sizeof(BigLd), hipMemcpyHostToDevice, st));
     }
-    for (int32_t c0 = 0; c0 < pmax; c0 += kBigWork) launch_big_cb(st, d, P->d_big.p, nbig, c0, kBigWork, per_front, nmode);
+    cb_update(nmode);
     if (labels)
         launch_label_sq(st, P->d_arena.p, P->d_snap.p, snap_total, P->d_selfr.p, static_cast<int32_t>(sf.size()), maxq,
                         P->d_lab.p);

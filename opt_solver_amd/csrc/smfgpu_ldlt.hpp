@@ -64,8 +64,13 @@ struct LdltDev {
 // updates of the columns beyond it are held back (their unscaled columns
 // kept in W) and applied in one flush when the outer block ends, before a
 // stall's pulls or rotation, and before the second pass.
+// BF16 and split-BF16 fronts on bf16 planes take wider outer blocks (and
+// contribution-block chunks), 1024 by default: their 256 x 256 tiles then
+// read and write the FP64 entries once per rank-1024 update instead of
+// once per rank-512 one, which bounds them (the FP32 accumulators sum
+// twice as many products).
 constexpr int32_t kBigOuter = 512;
-constexpr int32_t kBigWork = 512;  // W columns per front (an outer block's pivots; L*D chunks of the final update)
+constexpr int32_t kBigWork = 1024;  // W columns per front (an outer block's pivots; L*D chunks of the final update)
 static_assert(kBigWork >= kBigOuter, "W holds the pivots of an outer block");
 struct BigLd {
     int32_t s;                  // supernode
@@ -75,6 +80,7 @@ struct BigLd {
     int32_t stalled;            // first pass: the last panel stalled
     int32_t ps, pl;             // the last panel
     int32_t oe;                 // outer block end
+    int32_t outer;              // outer block width (kBigOuter, or kBigOuterWide on planes)
     int32_t pend0;              // pivots [pend0, k) held back for the columns >= oe; W column 0
     int32_t u_c0, u_c1, u_p0, u_p1;  // update request: columns [u_c0, u_c1), rows [u_c0, f), pivots [u_p0, u_p1)
     int32_t x_c0, x_p0, x_p1;        // flush request: columns [x_c0, ne), rows [x_c0, f), pivots [x_p0, x_p1)
@@ -170,7 +176,7 @@ void launch_big_stall(hipStream_t st, const LdltDev &d, BigLd *states, int32_t n
 // final contribution-block update with pivots [c0, c0 + width) of every
 // front (L*D of the chunk into W first); then the fronts' results
 void launch_big_cb(hipStream_t st, const LdltDev &d, const BigLd *states, int32_t nbig, int32_t c0, int32_t width,
-                   int32_t per_front, const int32_t *nmode);
+                   int32_t per_front, const int32_t *nmode, bool wide);
 void launch_big_finish(hipStream_t st, const LdltDev &d, const BigLd *states, int32_t nbig);
 
 }  // namespace smfgpu

@@ -253,7 +253,8 @@ struct smfgpu_plan_s {
     int32_t cb_seg = 4096;                         // SPD: pivot columns per contribution-block update
     int32_t ldlt_big_min = 256;                    // LDL^T: fronts with at least this many rows take the large-front path
     DBuf<BigLd> d_big;                             // LDL^T: large-front states of the level
-    int32_t ldlt_planes = 1;                       // LDL^T: bf16 operand planes for the large fronts' rank-512 updates
+    int32_t ldlt_planes = 1;                       // LDL^T: bf16 operand planes for the large fronts' outer-block updates
+    int32_t ldlt_outer_wide = 1024;                // LDL^T: outer block of the fronts updated on planes
     int32_t ldlt_group = 1;                        // LDL^T: speculative panel groups on the large-front path
     int32_t ldlt_update_wg = 1024;                 // LDL^T: workgroups of the large-front update launches
     DBuf<double> d_gpart;                          // LDL^T: per-workgroup column maxima of a group

@@ -209,15 +209,17 @@ def test_large_front_default_threshold(oracle_lib):
 
 
 @pytest.mark.parametrize("prec", ["bf16x3", "bf16"])
-def test_large_front_planes_bitwise(prec):
-    """The large fronts' rank-512 updates in BF16 and split BF16 read bf16
-    copies of their operands written once per update (the held-back
-    pivots' flush past each 512-column outer block, the contribution
-    block's chunks): the same factor, bit for bit, as rounding the FP64
-    operands inside every tile (the same roundings, chunks and MFMA order),
-    on a grid KKT whose largest fronts (3016 rows) have several outer
-    blocks and contribution blocks."""
-    from opt_solver_amd._lib import TUNE_LDLT_PLANES
+def test_large_front_planes_bitwise(oracle_lib, prec):
+    """The large fronts' outer-block updates in BF16 and split BF16 read
+    bf16 copies of their operands written once per update (the held-back
+    pivots' flush past each outer block, the contribution block's chunks)
+    in 256 x 256 tiles: with 512-column outer blocks, the same factor, bit
+    for bit, as rounding the FP64 operands inside every 128 x 128 tile (the
+    same roundings, chunks and MFMA order per accumulator), on a grid KKT
+    whose largest fronts (3016 rows) have several outer blocks and
+    contribution blocks. With the default 1024-column outer blocks: the
+    FP64 factor's inertia, and refinement to eta <= 1e-12."""
+    from opt_solver_amd._lib import TUNE_LDLT_OUTER_WIDE, TUNE_LDLT_PLANES
     from opt_solver_amd.solver import PREC_BF16, PREC_BF16X3
     print("case planes", prec, flush=True)
     gsz = 120
@@ -227,6 +229,7 @@ def test_large_front_planes_bitwise(prec):
     plan = Plan(an, posdef=False)
     code = {"bf16x3": PREC_BF16X3, "bf16": PREC_BF16}[prec]
     ex = {}
+    plan.tune(TUNE_LDLT_OUTER_WIDE, 512)
     for planes in (1, 0):
         plan.tune(TUNE_LDLT_PLANES, planes)
         fac = plan.factor(A[3], u=0.01, precision=code, mixed_min_front=512)
@@ -236,6 +239,16 @@ def test_large_front_planes_bitwise(prec):
     for key in ("steps", "row_indices", "pivot_types", "factor_values"):
         assert np.array_equal(ex[0][0][key], ex[1][0][key]), key
     print(f"  reduced fronts {fac.info.fronts_reduced} inertia {ex[1][1]}", flush=True)
+    plan.tune(TUNE_LDLT_PLANES, 1)
+    plan.tune(TUNE_LDLT_OUTER_WIDE, 1024)
+    fac = plan.factor(A[3], u=0.01, precision=code, mixed_min_front=512)
+    assert fac.status == 0 and fac.inertia == (gsz * gsz, m, 0) == ex[1][1]
+    x = np.random.default_rng(9).uniform(-1, 1, A[0])
+    b = matvec(A, x)
+    xr, st = fac.refine(b, fac.solve(b), max_iters=30, tol=1e-12, max_steps=40)
+    eta, _ = oracle_lib.backward_error(A, b, xr)
+    print(f"  outer 1024: refinement steps {st['steps']} eta {eta:.1e}", flush=True)
+    assert eta <= 1e-12, (eta, st)
 
 
 @pytest.mark.parametrize("case", ["ocp_T200", "kkt_grid_60"])
