@@ -1204,7 +1204,7 @@ __device__ __forceinline__ void grow_pivots(double (&a)[kGroupCols], int ncols, 
         if (FULL || kk < ncols) {
             const double v = a[kk];
             if (STATS) red[q][threadIdx.x] = fabs(v);
-            else if (rowok) W[(wcol0 + kk) * f + i] = v;
+            if (rowok) W[(wcol0 + kk) * f + i] = v;
 #pragma unroll
             for (int jq = q + 1; jq < 16; ++jq) {
                 const int j = 16 * M + jq;
@@ -1259,6 +1259,54 @@ __device__ __forceinline__ void gpanel_max(double (*red)[kLT + 1], double (*red2
     }
 }
 
+// The four panels of the diagonal region in turn (FULL: 64 columns).
+template <bool FULL>
+__device__ __forceinline__ void gdiag_run(double (&a)[kGroupCols], int r, int ncols, double (*Wd)[kGroupCols + 1],
+                                          double (*Lm)[16], double *invd, double *dd) {
+    gdiag_pivots<0, FULL>(a, r, ncols, Wd, Lm, invd, dd);
+    __syncthreads();
+    if (FULL || 16 < ncols) {
+        gdeferred<0, FULL>(a, ncols, Wd, r, false);
+        gdiag_pivots<1, FULL>(a, r, ncols, Wd, Lm, invd, dd);
+    }
+    __syncthreads();
+    if (FULL || 32 < ncols) {
+        gdeferred<1, FULL>(a, ncols, Wd, r, false);
+        gdiag_pivots<2, FULL>(a, r, ncols, Wd, Lm, invd, dd);
+    }
+    __syncthreads();
+    if (FULL || 48 < ncols) {
+        gdeferred<2, FULL>(a, ncols, Wd, r, false);
+        gdiag_pivots<3, FULL>(a, r, ncols, Wd, Lm, invd, dd);
+    }
+    __syncthreads();
+}
+
+// The rows below the region (one per thread) through the four panels.
+template <bool STATS, bool FULL>
+__device__ __forceinline__ void grow_run(double (&a)[kGroupCols], int ncols, const double (*Wd)[kGroupCols + 1],
+                                         const double (*Lm)[16], const double *invd, double (*red)[kLT + 1],
+                                         double (*red2)[16], double *wgmax, double *W, int64_t wcol0, int f, int i,
+                                         bool rowok) {
+    grow_pivots<0, STATS, FULL>(a, ncols, Lm, invd, red, W, wcol0, f, i, rowok);
+    if (STATS) gpanel_max<0>(red, red2, wgmax);
+    if (FULL || 16 < ncols) {
+        gdeferred<0, FULL>(a, ncols, Wd, 0, true);
+        grow_pivots<1, STATS, FULL>(a, ncols, Lm, invd, red, W, wcol0, f, i, rowok);
+        if (STATS) gpanel_max<1>(red, red2, wgmax);
+    }
+    if (FULL || 32 < ncols) {
+        gdeferred<1, FULL>(a, ncols, Wd, 0, true);
+        grow_pivots<2, STATS, FULL>(a, ncols, Lm, invd, red, W, wcol0, f, i, rowok);
+        if (STATS) gpanel_max<2>(red, red2, wgmax);
+    }
+    if (FULL || 48 < ncols) {
+        gdeferred<2, FULL>(a, ncols, Wd, 0, true);
+        grow_pivots<3, STATS, FULL>(a, ncols, Lm, invd, red, W, wcol0, f, i, rowok);
+        if (STATS) gpanel_max<3>(red, red2, wgmax);
+    }
+}
+
 // The group's diagonal region [ps, ps + ncols), one wave per front: lane r
 // holds row ps + r. Results to the group buffer.
 __global__ __launch_bounds__(64) void k_big_gdiag(LdltDev d, const BigLd *__restrict__ states,
@@ -1280,23 +1328,8 @@ __global__ __launch_bounds__(64) void k_big_gdiag(LdltDev d, const BigLd *__rest
 #pragma unroll
     for (int c = 0; c < kGroupCols; ++c)
         a[c] = masked(G[static_cast<int64_t>(ps + min(c, ncols - 1)) * f + rr], mine && c <= r);
-    gdiag_pivots<0, false>(a, r, ncols, Wd, Lm, invd, dd);
-    __syncthreads();
-    if (16 < ncols) {
-        gdeferred<0, false>(a, ncols, Wd, r, false);
-        gdiag_pivots<1, false>(a, r, ncols, Wd, Lm, invd, dd);
-    }
-    __syncthreads();
-    if (32 < ncols) {
-        gdeferred<1, false>(a, ncols, Wd, r, false);
-        gdiag_pivots<2, false>(a, r, ncols, Wd, Lm, invd, dd);
-    }
-    __syncthreads();
-    if (48 < ncols) {
-        gdeferred<2, false>(a, ncols, Wd, r, false);
-        gdiag_pivots<3, false>(a, r, ncols, Wd, Lm, invd, dd);
-    }
-    __syncthreads();
+    if (ncols == kGroupCols) gdiag_run<true>(a, r, ncols, Wd, Lm, invd, dd);
+    else gdiag_run<false>(a, r, ncols, Wd, Lm, invd, dd);
 #pragma unroll
     for (int c = 0; c < kGroupCols; ++c) gp[kGdFin + c * kGroupCols + r] = a[c];
     for (int e = r; e < kGroupCols * kGroupCols; e += 64) gp[kGdWd + e] = Wd[e >> 6][e & 63];
@@ -1309,27 +1342,69 @@ __global__ __launch_bounds__(64) void k_big_gdiag(LdltDev d, const BigLd *__rest
 }
 
 // Rows of a group below its diagonal region, [ps + ncols, f), in chunks of
-// kGroupRows per workgroup, one row per thread, with the region's tables.
-// STATS: column maxima per pivot to the group buffer; else the rows' group
-// columns and W are written, and workgroup 0 also writes the region.
-template <bool STATS>
+// kGroupRows per workgroup, one row per thread, with the region's tables:
+// column maxima per pivot to the group buffer, each row's unscaled column
+// entries to W (the columns the panel kernel's pivots will have: W column
+// ps - pend0 + t, pend0 being the W base the panel kernel starts from) and
+// its final group entries to W's speculation columns [kGroupSpec,
+// kGroupSpec + 64). Nothing of the front square changes: a rejected panel
+// finds its columns as they were. W columns of rejected pivots are
+// rewritten (rows below the pivot) when those positions get their pivots.
 __global__ __launch_bounds__(kLT) void k_big_group(LdltDev d, const BigLd *__restrict__ states,
                                                    double *__restrict__ gpart, int32_t nch) {
     __shared__ double Wd[kGroupCols][kGroupCols + 1];
     __shared__ double Lm[kGroupCols][16];
     __shared__ double invd[kGroupCols], wgmax[kGroupCols];
-    __shared__ double red[STATS ? 16 : 1][kLT + 1];
+    __shared__ double red[16][kLT + 1];
     __shared__ double red2[16][16];
     const BigLd B = states[blockIdx.y];
-    int ps, ncols;
-    if (STATS) {
-        ps = B.k;
-        ncols = group_cols(B, 16);
-    } else {
-        ps = B.g_ps;
-        ncols = B.g_apply ? B.g_n : 0;
-    }
+    const int ps = B.k, ncols = group_cols(B, 16);
     if (ncols <= 0) return;
+    const FrontLd F = d.fronts[B.s];
+    const int f = F.f;
+    const int gend = ps + ncols;
+    const int bx = blockIdx.x;
+    if (bx >= max(1, (f - gend + kGroupRows - 1) / kGroupRows)) return;
+    const double *G = d.arena + F.fofs;
+    double *W = d.work + F.wofs;
+    const double *gp = gpart + blockIdx.y * gpart_stride(nch);
+    const int64_t wcol0 = ps - B.pend0;  // W column of the group's first pivot
+    const int tid = threadIdx.x;
+    for (int e = tid; e < kGroupCols * kGroupCols; e += kLT) Wd[e >> 6][e & 63] = gp[kGdWd + e];
+    for (int e = tid; e < kGroupCols * 16; e += kLT) Lm[e >> 4][e & 15] = gp[kGdLm + e];
+    if (tid < kGroupCols) invd[tid] = gp[kGdInv + tid];
+    __syncthreads();
+    const int i = gend + bx * kGroupRows + tid;
+    const bool rowok = i < f;
+    const int il = min(i, f - 1);
+    double a[kGroupCols];
+#pragma unroll
+    for (int c = 0; c < kGroupCols; ++c)
+        a[c] = masked(G[static_cast<int64_t>(ps + min(c, ncols - 1)) * f + il], rowok && c < ncols);
+    // (the unguarded form unrolls into more registers than a 256-thread
+    // workgroup has)
+    grow_run<true, false>(a, ncols, Wd, Lm, invd, red, red2, wgmax, W, wcol0, f, i, rowok);
+    __syncthreads();
+    if (tid < ncols) gpart[blockIdx.y * gpart_stride(nch) + kGMax + bx * kGroupCols + tid] = wgmax[tid];
+    if (rowok) {
+#pragma unroll
+        for (int c = 0; c < kGroupCols; ++c)
+            if (c < ncols) W[static_cast<int64_t>(kGroupSpec + c) * f + i] = a[c];
+    }
+}
+
+// The accepted panels of a group, columns [ps, ps + nacc): the region's
+// final entries and W rows (its rows [ps, ps + ncols), from the region
+// launch's tables) and the rows below it from W's speculation columns.
+// Per row, the region launch and k_big_group carried out the operations
+// of the one-panel path in its order; a row of the region at or past nacc
+// went through the same operations on the accepted columns as a row below
+// it would have.
+__global__ __launch_bounds__(kLT) void k_big_gapply(LdltDev d, const BigLd *__restrict__ states,
+                                                    const double *__restrict__ gpart, int32_t nch) {
+    const BigLd B = states[blockIdx.y];
+    if (!B.g_apply) return;
+    const int ps = B.g_ps, nacc = B.g_n, ncols = B.g_cols;
     const FrontLd F = d.fronts[B.s];
     const int f = F.f;
     const int gend = ps + ncols;
@@ -1338,54 +1413,21 @@ __global__ __launch_bounds__(kLT) void k_big_group(LdltDev d, const BigLd *__res
     double *G = d.arena + F.fofs;
     double *W = d.work + F.wofs;
     const double *gp = gpart + blockIdx.y * gpart_stride(nch);
-    const int64_t wcol0 = ps - B.w_base;  // W column of the group's first pivot
+    const int64_t wcol0 = ps - B.w_base;
     const int tid = threadIdx.x;
-    for (int e = tid; e < kGroupCols * kGroupCols; e += kLT) Wd[e >> 6][e & 63] = gp[kGdWd + e];
-    for (int e = tid; e < kGroupCols * 16; e += kLT) Lm[e >> 4][e & 15] = gp[kGdLm + e];
-    if (tid < kGroupCols) invd[tid] = gp[kGdInv + tid];
-    __syncthreads();
-    if (!STATS && bx == 0) {
-        // the region: final entries and W (rows below each pivot)
+    if (bx == 0) {
         for (int e = tid; e < kGroupCols * kGroupCols; e += kLT) {
             const int c = e >> 6, rw = e & 63;
-            if (c < ncols && rw < ncols && c <= rw) {
+            if (c < nacc && rw < ncols && c <= rw) {
                 G[static_cast<int64_t>(ps + c) * f + ps + rw] = gp[kGdFin + e];
-                if (c < rw) W[(wcol0 + c) * f + ps + rw] = Wd[c][rw];
+                if (c < rw) W[(wcol0 + c) * f + ps + rw] = gp[kGdWd + e];
             }
         }
     }
     const int i = gend + bx * kGroupRows + tid;
-    const bool rowok = i < f;
-    const int il = min(i, f - 1);
-    double a[kGroupCols];
-#pragma unroll
-    for (int c = 0; c < kGroupCols; ++c)
-        a[c] = masked(G[static_cast<int64_t>(ps + min(c, ncols - 1)) * f + il], rowok && c < ncols);
-    grow_pivots<0, STATS, false>(a, ncols, Lm, invd, red, W, wcol0, f, i, rowok);
-    if (STATS) gpanel_max<0>(red, red2, wgmax);
-    if (16 < ncols) {
-        gdeferred<0, false>(a, ncols, Wd, 0, true);
-        grow_pivots<1, STATS, false>(a, ncols, Lm, invd, red, W, wcol0, f, i, rowok);
-        if (STATS) gpanel_max<1>(red, red2, wgmax);
-    }
-    if (32 < ncols) {
-        gdeferred<1, false>(a, ncols, Wd, 0, true);
-        grow_pivots<2, STATS, false>(a, ncols, Lm, invd, red, W, wcol0, f, i, rowok);
-        if (STATS) gpanel_max<2>(red, red2, wgmax);
-    }
-    if (48 < ncols) {
-        gdeferred<2, false>(a, ncols, Wd, 0, true);
-        grow_pivots<3, STATS, false>(a, ncols, Lm, invd, red, W, wcol0, f, i, rowok);
-        if (STATS) gpanel_max<3>(red, red2, wgmax);
-    }
-    if (STATS) {
-        __syncthreads();
-        if (tid < ncols) gpart[blockIdx.y * gpart_stride(nch) + kGMax + bx * kGroupCols + tid] = wgmax[tid];
-    } else if (rowok) {
-#pragma unroll
-        for (int c = 0; c < kGroupCols; ++c)
-            if (c < ncols) G[static_cast<int64_t>(ps + c) * f + i] = a[c];
-    }
+    if (i < f)
+        for (int c = 0; c < nacc; ++c)
+            G[static_cast<int64_t>(ps + c) * f + i] = W[static_cast<int64_t>(kGroupSpec + c) * f + i];
 }
 
 __global__ __launch_bounds__(kLT) void k_big_panel(LdltDev d, BigLd *states, Prm prm, int32_t nb,
@@ -1418,7 +1460,7 @@ __global__ __launch_bounds__(kLT) void k_big_panel(LdltDev d, BigLd *states, Prm
     __syncthreads();
     int u_c0 = 0, u_c1 = 0, u_p0 = 0, u_p1 = 0, x_c0 = ne, x_p0 = 0, x_p1 = 0, w_base = pend0;
     int stalled = 0, npass = pass, ps = k, pl = k;
-    int spec = B.spec, g_apply = 0, g_ps = 0, g_n = 0, searched = 0;
+    int spec = B.spec, g_apply = 0, g_ps = 0, g_n = 0, g_cols = 0, searched = 0;
     bool group_done = false;
     const int gcols = gpart ? group_cols(B, nb) : 0;
     if (gcols > 0) {
@@ -1485,6 +1527,7 @@ __global__ __launch_bounds__(kLT) void k_big_panel(LdltDev d, BigLd *states, Prm
             g_apply = 1;
             g_ps = k;
             g_n = nacc;
+            g_cols = gcols;
             spec = nacc == gcols ? 1 : 0;  // a panel failed inside: the next panel goes the one-panel way
             ps = k + ((nacc - 1) / 16) * 16;
             pl = kn;
@@ -1610,6 +1653,7 @@ __global__ __launch_bounds__(kLT) void k_big_panel(LdltDev d, BigLd *states, Prm
         B.g_apply = g_apply;
         B.g_ps = g_ps;
         B.g_n = g_n;
+        B.g_cols = g_cols;
     }
 }
 
@@ -1898,12 +1942,12 @@ void launch_big_panel(hipStream_t st, const LdltDev &d, BigLd *states, int32_t n
 void launch_big_group_stats(hipStream_t st, const LdltDev &d, const BigLd *states, int32_t nbig, double *gpart,
                             int32_t nch) {
     k_big_gdiag<<<dim3(nbig), 64, 0, st>>>(d, states, gpart, nch);
-    k_big_group<true><<<dim3(nch, nbig), kLT, 0, st>>>(d, states, gpart, nch);
+    k_big_group<<<dim3(nch, nbig), kLT, 0, st>>>(d, states, gpart, nch);
 }
 
 void launch_big_group_apply(hipStream_t st, const LdltDev &d, const BigLd *states, int32_t nbig, double *gpart,
                             int32_t nch) {
-    k_big_group<false><<<dim3(nch, nbig), kLT, 0, st>>>(d, states, gpart, nch);
+    k_big_gapply<<<dim3(nch, nbig), kLT, 0, st>>>(d, states, gpart, nch);
 }
 
 // workgroups per front of the 256 x 256 tiles (one per CU; a multiple of 8)

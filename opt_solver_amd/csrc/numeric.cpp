@@ -790,7 +790,7 @@ SMFGPU_API int smfgpu_plan_tune(smfgpu_plan_t P, int32_t key, int64_t value) {
         P->ldlt_planes = static_cast<int32_t>(value);
         return SMFGPU_OK;
     case SMFGPU_TUNE_LDLT_OUTER_WIDE:
-        if (value < 64 || value > kBigWork || value % 64 != 0)
+        if (value < 64 || value > kBigOuterMax || value % 64 != 0)
             return set_error(SMFGPU_ERR_INVALID_ARG, "tune: outer block must be a multiple of 64 in [64, 1024]");
         P->ldlt_outer_wide = static_cast<int32_t>(value);
         return SMFGPU_OK;

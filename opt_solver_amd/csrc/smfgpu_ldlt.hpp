@@ -70,8 +70,12 @@ struct LdltDev {
 // once per rank-512 one, which bounds them (the FP32 accumulators sum
 // twice as many products).
 constexpr int32_t kBigOuter = 512;
-constexpr int32_t kBigWork = 1024;  // W columns per front (an outer block's pivots; L*D chunks of the final update)
-static_assert(kBigWork >= kBigOuter, "W holds the pivots of an outer block");
+constexpr int32_t kBigOuterMax = 1024;
+// W columns per front: an outer block's pivots (L*D chunks of the final
+// update), then a speculative group's final entries (kGroupSpec)
+constexpr int32_t kBigWork = kBigOuterMax + 64;
+constexpr int32_t kGroupSpec = kBigOuterMax;
+static_assert(kBigOuterMax >= kBigOuter, "W holds the pivots of an outer block");
 struct BigLd {
     int32_t s;                  // supernode
     int32_t k, live, k_paired;  // next pivot position, end of the live eligible columns, pulls' guard
@@ -89,11 +93,13 @@ struct BigLd {
     int32_t stalls, pulls, rotated, searches;  // as FrontLd's
     int32_t spec;               // 1: the next first-pass iteration tries a speculative group
     int32_t g_apply, g_ps, g_n; // accepted group to write: columns [g_ps, g_ps + g_n)
+    int32_t g_cols;             // the group's width (its diagonal region's rows)
     int32_t prec;               // precision code of the MFMA updates: 0 FP64, 1 FP32, 2 BF16, 3 split BF16
     double flops;
 };
 constexpr int32_t kGroupCols = 64;   // columns of a speculative group (four panels of 16)
 constexpr int32_t kGroupRows = 256;  // rows per workgroup below a group's diagonal region
+static_assert(kBigWork - kGroupSpec == kGroupCols, "W holds a group's speculative columns");
 constexpr int64_t kGroupBuf = 192 + 64 * 16 + 2 * 64 * 64;  // group buffer per front before the per-workgroup maxima
 
 // Static pivoting (control.hpp:333-364): sp = static_pivot_tol * max|A|
