@@ -419,14 +419,16 @@ __global__ void k_zero_i32(int32_t *p, int32_t n) {
 
 // ---------------------------------------------------------------------------
 // Sweeps of an LDL^T factor (unit L with D on the stored diagonal; the D^-1
-// step runs between them): the same step scheme with 64-pivot blocks and no
-// stored inverses. A triangle item takes the update of its own pivots by the
-// block solved in the step before, loads its unit triangle into LDS and
-// substitutes in one wave. Items: forward triangle {k, kb: previous block;
+// step runs between them): steps of kLS pivots (a large front's dense
+// triangle costs a launch per kLS pivots), each solved inside its triangle
+// item as up to four 64-pivot sub-blocks whose unit triangles go through
+// LDS. Items: forward triangle {k, kb: the block solved in the step before;
 // tk, tkb: this block}, forward chunk {rows [a0, a1), update by [k, k+kb)},
 // backward chunk {columns [a0, a1), contributions of rows [k, k+kb); flags
-// bit 0: atomic}, backward triangle {k, kb: block solved before; tk, tkb}.
-constexpr int kLB = 64;
+// bit 0: atomic}, backward triangle {k, kb: block solved before (or the
+// rows below the pivots); tk, tkb}.
+constexpr int kLB = 64;    // sub-block: a unit triangle in LDS, substituted by one wave
+constexpr int kLS = 256;   // pivots per step
 constexpr int kLThreads = 256;
 constexpr int kLSeg = 512;  // rows of a backward tail item
 constexpr int32_t kItemTri = 2;
@@ -438,16 +440,29 @@ __device__ __forceinline__ double lwave_sum(double v) {
     return v;
 }
 
-// sum_{c < kb} L(i, k + c) xs[c] for row i = r0 + (tid & 63) (< r1), the
-// four waves taking 16 columns each; the total lands in part[0][row].
+// sum_{c < kb} L(i, k + c) xs[c] (kb <= kLS) for row i = r0 + (tid & 63)
+// (< r1), wave w taking the columns [w q, (w + 1) q), q = ceil(kb / 4),
+// in order; the total ((w0 + w1) + (w2 + w3)) lands in part[0][row]. The
+// loads go out 16 columns at a time (clamped addresses, masked values), so
+// a workgroup waits on a column group's latency, not each column's.
 __device__ __forceinline__ void rows_times_block(const double *L, int f, int k, int kb, int r0, int r1,
                                                  const double *xs, double (*part)[kLB]) {
+    constexpr int kG = 16;
     const int row = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int i = r0 + row;
+    const int i = min(r0 + row, r1 - 1);
+    const int q = (kb + 3) >> 2;
+    const int cb = w * q, ce = min(kb, cb + q);
     double acc = 0.0;
-    if (i < r1) {
-        const int c1 = min(kb, w * 16 + 16);
-        for (int c = w * 16; c < c1; ++c) acc += L[static_cast<int64_t>(k + c) * f + i] * xs[c];
+    if (r0 + row < r1 && ce > cb) {
+        const double *Lc = L + static_cast<int64_t>(k) * f + i;
+        for (int c = cb; c < ce; c += kG) {
+            double lv[kG];
+#pragma unroll
+            for (int j = 0; j < kG; ++j) lv[j] = Lc[static_cast<int64_t>(min(c + j, ce - 1)) * f];
+#pragma unroll
+            for (int j = 0; j < kG; ++j)
+                if (c + j < ce) acc += lv[j] * xs[c + j];
+        }
     }
     part[w][row] = acc;
     __syncthreads();
@@ -455,11 +470,21 @@ __device__ __forceinline__ void rows_times_block(const double *L, int f, int k, 
     __syncthreads();
 }
 
+// the unit triangle of sub-block [c0, c0 + nc): T[c][l] = L(c0 + l, c0 + c), l > c
+__device__ __forceinline__ void load_unit_triangle(const double *L, int f, int c0, int nc, double (*T)[kLB + 1]) {
+    for (int e = threadIdx.x; e < kLB * kLB; e += kLThreads) {
+        const int c = e >> 6, l = e & 63;
+        T[c][l] = (c < nc && l < nc && l > c) ? L[static_cast<int64_t>(c0 + c) * f + c0 + l] : 0.0;
+    }
+}
+
 __global__ __launch_bounds__(kLThreads) void k_ldlt_fwd_step(LdltDev d, const SolveItem *__restrict__ items,
                                                             double *__restrict__ y, int32_t n, int32_t nrhs) {
-    __shared__ double xs[kLB];
+    __shared__ double xs[kLS];  // the block solved in the step before
+    __shared__ double xc[kLS];  // this item's block as it is solved
     __shared__ double part[4][kLB];
-    __shared__ double T[kLB][kLB + 1];  // T[c][l] = L(tk + l, tk + c)
+    __shared__ double part2[4][kLB];
+    __shared__ double T[kLB][kLB + 1];
     const SolveItem it = items[blockIdx.x];
     const FrontLd F = d.fronts[it.s];
     const int f = F.f, p = F.pstore;
@@ -467,107 +492,142 @@ __global__ __launch_bounds__(kLThreads) void k_ldlt_fwd_step(LdltDev d, const So
     const int32_t *rows = d.rows + F.rofs;
     const int tid = threadIdx.x;
     const bool tri = (it.flags & kItemTri) != 0;
-    if (tri) {
-        for (int e = tid; e < kLB * kLB; e += kLThreads) {
-            const int c = e >> 6, l = e & 63;
-            T[c][l] = (c < it.tkb && l < it.tkb && l > c) ? L[static_cast<int64_t>(it.tk + c) * f + it.tk + l] : 0.0;
-        }
-    }
-    const int r0 = tri ? it.tk : it.a0, r1 = tri ? it.tk + it.tkb : it.a1;
     for (int r = 0; r < nrhs; ++r) {
         double *yr = y + static_cast<int64_t>(r) * n;
-        if (tid < it.kb) xs[tid] = yr[rows[it.k + tid]];
+        for (int t = tid; t < it.kb; t += kLThreads) xs[t] = yr[rows[it.k + t]];
         __syncthreads();
-        if (it.kb > 0) rows_times_block(L, f, it.k, it.kb, r0, r1, xs, part);
-        else {
-            if (tid < kLB) part[0][tid] = 0.0;
-            __syncthreads();
-        }
         if (tri) {
-            if (tid < 64) {
-                const int l = tid;
-                double yv = l < it.tkb ? yr[rows[it.tk + l]] - part[0][l] : 0.0;
-                for (int c = 0; c < it.tkb; ++c) {
-                    const double xc = __shfl(yv, c, 64);
-                    if (l > c) yv -= T[c][l] * xc;
+            const int nsub = (it.tkb + kLB - 1) / kLB;
+            for (int sb = 0; sb < nsub; ++sb) {
+                const int c0 = it.tk + sb * kLB, nc = min(kLB, it.tkb - sb * kLB);
+                load_unit_triangle(L, f, c0, nc, T);
+                // the sub-block's rows: by the block before, then by this
+                // block's solved sub-blocks
+                rows_times_block(L, f, it.k, it.kb, c0, c0 + nc, xs, part);
+                rows_times_block(L, f, it.tk, sb * kLB, c0, c0 + nc, xc, part2);
+                if (tid < 64) {
+                    const int l = tid;
+                    double yv = l < nc ? (yr[rows[c0 + l]] - part[0][l]) - part2[0][l] : 0.0;
+                    for (int c = 0; c < nc; ++c) {
+                        const double xv = __shfl(yv, c, 64);
+                        if (l > c) yv -= T[c][l] * xv;
+                    }
+                    if (l < nc) {
+                        yr[rows[c0 + l]] = yv;
+                        xc[sb * kLB + l] = yv;
+                    }
                 }
-                if (l < it.tkb) {
-                    yr[rows[it.tk + l]] = yv;
-                    xs[l] = yv;
-                }
+                __syncthreads();
             }
             if (it.flags & kItemFused) {
                 // the rows below, with the block just solved (the same sums
                 // as the separate row items of the next step)
-                __syncthreads();
                 for (int rb = it.a0; rb < it.a1; rb += kLB) {
                     const int re = min(rb + kLB, it.a1);
-                    rows_times_block(L, f, it.tk, it.tkb, rb, re, xs, part);
+                    rows_times_block(L, f, it.tk, it.tkb, rb, re, xc, part);
                     if (tid < 64 && rb + tid < re) atomicAdd(&yr[rows[rb + tid]], -part[0][tid]);
                     __syncthreads();
                 }
             }
-        } else if (tid < 64) {
-            const int i = r0 + tid;
-            if (i < r1) {
-                if (i >= p) atomicAdd(&yr[rows[i]], -part[0][tid]);
-                else yr[rows[i]] -= part[0][tid];
+        } else {
+            rows_times_block(L, f, it.k, it.kb, it.a0, it.a1, xs, part);
+            if (tid < 64) {
+                const int i = it.a0 + tid;
+                if (i < it.a1) {
+                    if (i >= p) atomicAdd(&yr[rows[i]], -part[0][tid]);
+                    else yr[rows[i]] -= part[0][tid];
+                }
             }
         }
         __syncthreads();
     }
 }
 
+// Backward contributions of rows [k, k + kb) (values xs) to the columns
+// [c0, c1): each wave takes 16 columns at a time, its lanes the rows t =
+// lane + 64 m in order, the 16 columns' loads of one m in flight together;
+// out(c, sum) from lane 0.
+template <class Out>
+__device__ __forceinline__ void cols_times_rows(const double *L, int f, int k, int kb, int c0, int c1, const double *xs,
+                                                Out &&out) {
+    constexpr int kG = 16;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int cb = c0 + w * kG; cb < c1; cb += 4 * kG) {
+        double acc[kG];
+#pragma unroll
+        for (int j = 0; j < kG; ++j) acc[j] = 0.0;
+        for (int t0 = 0; t0 < kb; t0 += 64) {
+            const int t = t0 + lane;
+            const bool tin = t < kb;
+            const double xv = tin ? xs[t] : 0.0;
+            const int tc = min(t, kb - 1);
+            double lv[kG];
+#pragma unroll
+            for (int j = 0; j < kG; ++j) lv[j] = L[static_cast<int64_t>(min(cb + j, c1 - 1)) * f + k + tc];
+#pragma unroll
+            for (int j = 0; j < kG; ++j)
+                if (tin) acc[j] += lv[j] * xv;
+        }
+#pragma unroll
+        for (int j = 0; j < kG; ++j) {
+            const double v = lwave_sum(acc[j]);
+            if (lane == 0 && cb + j < c1) out(cb + j, v);
+        }
+    }
+}
+
 __global__ __launch_bounds__(kLThreads) void k_ldlt_bwd_step(LdltDev d, const SolveItem *__restrict__ items,
                                                             double *__restrict__ y, int32_t n, int32_t nrhs) {
     __shared__ double xs[kLSeg];
-    __shared__ double upd[kLB];
-    __shared__ double T[kLB][kLB + 1];  // T[c][l] = L(tk + l, tk + c)
+    __shared__ double xc[kLS];
+    __shared__ double upd[kLS];
+    __shared__ double upd2[kLB];
+    __shared__ double T[kLB][kLB + 1];
     const SolveItem it = items[blockIdx.x];
     const FrontLd F = d.fronts[it.s];
     const int f = F.f;
     const double *L = d.fac + F.lofs;
     const int32_t *rows = d.rows + F.rofs;
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int tid = threadIdx.x;
     const bool tri = (it.flags & kItemTri) != 0;
-    if (tri) {
-        for (int e = tid; e < kLB * kLB; e += kLThreads) {
-            const int c = e >> 6, l = e & 63;
-            T[c][l] = (c < it.tkb && l < it.tkb && l > c) ? L[static_cast<int64_t>(it.tk + c) * f + it.tk + l] : 0.0;
-        }
-    }
-    const int c0 = tri ? it.tk : it.a0, c1 = tri ? it.tk + it.tkb : it.a1;
     for (int r = 0; r < nrhs; ++r) {
         double *yr = y + static_cast<int64_t>(r) * n;
         for (int t = tid; t < it.kb; t += kLThreads) xs[t] = yr[rows[it.k + t]];
-        if (tid < kLB) upd[tid] = 0.0;
         __syncthreads();
-        // contributions of rows [k, k + kb) to the columns, one wave per
-        // 16 columns
-        for (int j = 0; j < 16; ++j) {
-            const int c = c0 + w * 16 + j;
-            if (c >= c1) break;
-            const double *Lc = L + static_cast<int64_t>(c) * f + it.k;
-            double acc = 0.0;
-            for (int t = lane; t < it.kb; t += 64) acc += Lc[t] * xs[t];
-            acc = lwave_sum(acc);
-            if (lane == 0) {
-                if (tri) upd[c - c0] = acc;
-                else if (it.flags & 1) atomicAdd(&yr[rows[c]], -acc);
+        if (!tri) {
+            cols_times_rows(L, f, it.k, it.kb, it.a0, it.a1, xs, [&](int c, double acc) {
+                if (it.flags & 1) atomicAdd(&yr[rows[c]], -acc);
                 else yr[rows[c]] -= acc;
-            }
+            });
+            __syncthreads();
+            continue;
         }
+        // the block's columns: contributions of the rows solved before
+        cols_times_rows(L, f, it.k, it.kb, it.tk, it.tk + it.tkb, xs,
+                        [&](int c, double acc) { upd[c - it.tk] = acc; });
         __syncthreads();
-        if (tri && tid < 64) {
-            const int l = tid;
-            double yv = l < it.tkb ? yr[rows[it.tk + l]] - upd[l] : 0.0;
-            for (int c = it.tkb - 1; c >= 0; --c) {
-                const double xc = __shfl(yv, c, 64);
-                if (l < c) yv -= T[l][c] * xc;
+        const int nsub = (it.tkb + kLB - 1) / kLB;
+        for (int sb = nsub - 1; sb >= 0; --sb) {
+            const int c0 = it.tk + sb * kLB, nc = min(kLB, it.tkb - sb * kLB);
+            const int e0 = c0 + nc, e1 = it.tk + it.tkb;  // this block's sub-blocks solved so far
+            load_unit_triangle(L, f, c0, nc, T);
+            cols_times_rows(L, f, e0, e1 - e0, c0, c0 + nc, xc + (e0 - it.tk),
+                            [&](int c, double acc) { upd2[c - c0] = acc; });
+            __syncthreads();
+            if (tid < 64) {
+                const int l = tid;
+                double yv = l < nc ? (yr[rows[c0 + l]] - upd[sb * kLB + l]) - (e1 > e0 ? upd2[l] : 0.0) : 0.0;
+                for (int c = nc - 1; c >= 0; --c) {
+                    const double xv = __shfl(yv, c, 64);
+                    if (l < c) yv -= T[l][c] * xv;
+                }
+                if (l < nc) {
+                    yr[rows[c0 + l]] = yv;
+                    xc[sb * kLB + l] = yv;
+                }
             }
-            if (l < it.tkb) yr[rows[it.tk + l]] = yv;
+            __syncthreads();
         }
-        __syncthreads();
     }
 }
 

@@ -438,7 +438,9 @@ static int ldlt_big_fronts(smfgpu_plan_s *P, const LdltDev &d0, const std::vecto
 // by step: emit(step, item).
 template <class Emit>
 static void front_solve_items(int32_t f, int32_t p, int32_t s, bool bwd, Emit &&emit) {
-    constexpr int32_t B = 64, Seg = 512, Tri = 2;
+    // B pivots per step (the kernels solve them as 64-pivot sub-blocks),
+    // chunks of C rows (forward) or columns (backward)
+    constexpr int32_t B = 256, C = 64, Seg = 512, Tri = 2;
     if (p <= 0) return;
     const int32_t nb = (p + B - 1) / B;
     if (!bwd) {
@@ -453,11 +455,11 @@ static void front_solve_items(int32_t f, int32_t p, int32_t s, bool bwd, Emit &&
             const int32_t k = t > 0 ? tk - B : 0, kb = t > 0 ? B : 0;
             emit(t, SolveItem{s, k, kb, 0, 0, tk, tkb, Tri});
             if (t > 0)
-                for (int32_t r = tk + tkb; r < f; r += B) emit(t, SolveItem{s, k, kb, r, std::min(r + B, f), 0, 0, 0});
+                for (int32_t r = tk + tkb; r < f; r += C) emit(t, SolveItem{s, k, kb, r, std::min(r + C, f), 0, 0, 0});
         }
         if (f > p) {
             const int32_t k = (nb - 1) * B;
-            for (int32_t r = p; r < f; r += B) emit(nb, SolveItem{s, k, p - k, r, std::min(r + B, f), 0, 0, 0});
+            for (int32_t r = p; r < f; r += C) emit(nb, SolveItem{s, k, p - k, r, std::min(r + C, f), 0, 0, 0});
         }
         return;
     }
@@ -470,7 +472,7 @@ static void front_solve_items(int32_t f, int32_t p, int32_t s, bool bwd, Emit &&
     int32_t t = 0;
     if (f > p) {
         for (int32_t r = p; r < f; r += Seg)
-            for (int32_t c = 0; c < p; c += B) emit(0, SolveItem{s, r, std::min(Seg, f - r), c, std::min(c + B, p), 0, 0, 1});
+            for (int32_t c = 0; c < p; c += C) emit(0, SolveItem{s, r, std::min(Seg, f - r), c, std::min(c + C, p), 0, 0, 1});
         t = 1;
     }
     const int32_t kL = (nb - 1) * B;
@@ -478,7 +480,7 @@ static void front_solve_items(int32_t f, int32_t p, int32_t s, bool bwd, Emit &&
     for (int32_t j = nb - 2; j >= 0; --j, ++t) {
         const int32_t tk = j * B, k = tk + B, kb = std::min(B, p - k);
         emit(t, SolveItem{s, k, kb, 0, 0, tk, B, Tri});
-        for (int32_t c = 0; c < tk; c += B) emit(t, SolveItem{s, k, kb, c, c + B, 0, 0, 0});
+        for (int32_t c = 0; c < tk; c += C) emit(t, SolveItem{s, k, kb, c, c + C, 0, 0, 0});
     }
 }
 
