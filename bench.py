@@ -494,8 +494,9 @@ def main():
         "roofline": {"bound": "mfma", "kernel": "k_syrk / k_syrk_red (Schur update), learned step",
                      "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": traffic,
-                     "traffic_source": "profiles/k_syrk_spd27_100cube_traffic.txt (FP32 Schur launches, "
-                                       "geometric-ND tree)"},
+                     "traffic_source": "profiles/k_syrk_spd27_100cube_traffic.txt: rocprofv3 FETCH_SIZE x 2.00 "
+                                       "+ WRITE_SIZE per Schur launch pair of the learned step (METIS tree), "
+                                       "calibrated on a 1 GiB device copy; not measured in this run"},
         "cpu_baseline": cpu,
         **extra,
     }
