@@ -134,6 +134,7 @@ __device__ void fwd_update(const double *Lb, const int32_t *frows, int f, int p,
 
 template <int NR>
 struct FwdLds {
+    static_assert(NR * 64 <= kSThreads, "one wave per right-hand side in the triangles");
     double Dg[kSD][kSD + 1];
     double zs[kSB][NR];
     double yb[kSB][NR];
@@ -657,10 +658,12 @@ void launch_sweep(hipStream_t st, const DevPtrs &d, bool forward, const SolveIte
     const dim3 grid(static_cast<unsigned>(nitems));  // one workgroup per item
     if (forward) {
         if (nrhs == 1) k_sweep<1, true><<<grid, kSThreads, 0, st>>>(d, items, step_of, step_cnt, ctr, err, y, n, nrhs);
-        else k_sweep<4, true><<<grid, kSThreads, 0, st>>>(d, items, step_of, step_cnt, ctr, err, y, n, nrhs);
+        else if (nrhs <= 4) k_sweep<4, true><<<grid, kSThreads, 0, st>>>(d, items, step_of, step_cnt, ctr, err, y, n, nrhs);
+        else k_sweep<8, true><<<grid, kSThreads, 0, st>>>(d, items, step_of, step_cnt, ctr, err, y, n, nrhs);
     } else {
         if (nrhs == 1) k_sweep<1, false><<<grid, kSThreads, 0, st>>>(d, items, step_of, step_cnt, ctr, err, y, n, nrhs);
-        else k_sweep<4, false><<<grid, kSThreads, 0, st>>>(d, items, step_of, step_cnt, ctr, err, y, n, nrhs);
+        else if (nrhs <= 4) k_sweep<4, false><<<grid, kSThreads, 0, st>>>(d, items, step_of, step_cnt, ctr, err, y, n, nrhs);
+        else k_sweep<8, false><<<grid, kSThreads, 0, st>>>(d, items, step_of, step_cnt, ctr, err, y, n, nrhs);
     }
 }
 
@@ -669,7 +672,8 @@ void launch_fwd_step(hipStream_t st, const DevPtrs &d, const SolveItem *items, i
     for (int64_t o = 0; o < count; o += kMaxLaunchGroups) {
         const dim3 grid(static_cast<unsigned>(std::min(kMaxLaunchGroups, count - o)));
         if (nrhs == 1) k_fwd_step<1><<<grid, kSThreads, 0, st>>>(d, items + o, y, n, nrhs);
-        else k_fwd_step<4><<<grid, kSThreads, 0, st>>>(d, items + o, y, n, nrhs);
+        else if (nrhs <= 4) k_fwd_step<4><<<grid, kSThreads, 0, st>>>(d, items + o, y, n, nrhs);
+        else k_fwd_step<8><<<grid, kSThreads, 0, st>>>(d, items + o, y, n, nrhs);
     }
 }
 
@@ -678,7 +682,8 @@ void launch_bwd_step(hipStream_t st, const DevPtrs &d, const SolveItem *items, i
     for (int64_t o = 0; o < count; o += kMaxLaunchGroups) {
         const dim3 grid(static_cast<unsigned>(std::min(kMaxLaunchGroups, count - o)));
         if (nrhs == 1) k_bwd_step<1><<<grid, kSThreads, 0, st>>>(d, items + o, y, n, nrhs);
-        else k_bwd_step<4><<<grid, kSThreads, 0, st>>>(d, items + o, y, n, nrhs);
+        else if (nrhs <= 4) k_bwd_step<4><<<grid, kSThreads, 0, st>>>(d, items + o, y, n, nrhs);
+        else k_bwd_step<8><<<grid, kSThreads, 0, st>>>(d, items + o, y, n, nrhs);
     }
 }
 

@@ -1219,14 +1219,16 @@ int solve_device(smfgpu_plan_s *P, const double *b_dev, int64_t ldb, double *out
     const int32_t n = P->n;
     DevPtrs d = dev_ptrs(P);
     const int32_t nl = static_cast<int32_t>(P->level_fronts.size());
-    // right-hand sides per pass through the sweeps: 4, or what a
-    // caller-given work block holds
-    const int32_t group = ybuf ? static_cast<int32_t>(std::min<int64_t>(4, ycap / std::max(n, 1))) : 4;
+    // right-hand sides per pass through the sweeps: 8 (a step's item keeps
+    // them in LDS, one wave each in the triangles), or what a caller-given
+    // work block holds
+    constexpr int32_t kGroup = 8;
+    const int32_t group = ybuf ? static_cast<int32_t>(std::min<int64_t>(kGroup, ycap / std::max(n, 1))) : kGroup;
     if (group < 1) return set_error(SMFGPU_ERR_INVALID_ARG, "solve: work block smaller than one column");
     for (int32_t r0 = 0; r0 < nrhs; r0 += group) {
         const int32_t nr = std::min(group, nrhs - r0);
         if (!ybuf && P->d_y.n < static_cast<size_t>(n) * nr) {
-            const int rc = P->d_y.alloc(static_cast<size_t>(n) * 4);
+            const int rc = P->d_y.alloc(static_cast<size_t>(n) * kGroup);
             if (rc) return rc;
         }
         double *y = ybuf ? ybuf : P->d_y.p;
