@@ -390,7 +390,8 @@ def main():
     ap.add_argument("--m", type=int, default=100, help="grid side of configuration 4 (n = m^3)")
     ap.add_argument("--ordering", choices=("metis", "geometric"), default="metis")
     ap.add_argument("--cpu-m", type=int, default=76)
-    ap.add_argument("--seq-grid", type=int, default=300, help="grid side of the refactor sequence")
+    ap.add_argument("--seq-grid", type=int, default=450,
+                    help="grid side of the refactor sequence (450: the largest that fits one GPU)")
     ap.add_argument("--seq-refactors", type=int, default=40)
     ap.add_argument("--seq-paths", default="fp64,fp32,learned", help="paths of the refactor sequence")
     ap.add_argument("--only-seq", action="store_true", help="run the refactor sequence only (no SPD legs)")
