@@ -588,7 +588,7 @@ int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgp
     std::vector<int32_t> cmap;
     std::vector<AsmTask> tasks;
     std::vector<ChildCol> cc;
-    std::vector<std::vector<ChildCol>> bucket;
+    std::vector<int32_t> colstart, colfill;
     std::vector<int32_t> ids, small_list, blocked_list, big_list, ep_list, store_list;
     std::vector<BigLd> big_h;
     std::vector<FrontLd> patch;
@@ -720,7 +720,7 @@ int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgp
             }
             // child maps (increasing: delayed rows come first in both) and
             // the contributions per parent column
-            bucket.assign(F.f, {});
+            const size_t cm0 = cmap.size();
             for (int32_t c = P->child_ptr[s]; c < P->child_ptr[s + 1]; ++c) {
                 const int32_t ch = P->child_idx[c];
                 FrontLd &C = P->fl[ch];
@@ -735,16 +735,24 @@ int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgp
                     cmap.push_back(l);
                     prev = l;
                 }
-                for (int32_t jj = 0; jj < q; ++jj) bucket[cmap[C.cmofs + jj]].push_back(ChildCol{ch, jj});
                 ids.push_back(ch);
                 patch.push_back(C);
             }
-            for (int32_t c = 0; c < F.f; ++c) {
-                AsmTask t{s, c, static_cast<int32_t>(cc.size()), 0};
-                cc.insert(cc.end(), bucket[c].begin(), bucket[c].end());
-                t.cc1 = static_cast<int32_t>(cc.size());
-                tasks.push_back(t);
+            // per parent column, its children's columns in child order (a
+            // counting sort over the maps just built)
+            colstart.assign(static_cast<size_t>(F.f) + 1, 0);
+            for (size_t e = cm0; e < cmap.size(); ++e) ++colstart[cmap[e] + 1];
+            for (int32_t c = 0; c < F.f; ++c) colstart[c + 1] += colstart[c];
+            const int32_t cc0 = static_cast<int32_t>(cc.size());
+            cc.resize(cc.size() + (cmap.size() - cm0));
+            colfill.assign(colstart.begin(), colstart.end() - 1);
+            for (int32_t c = P->child_ptr[s]; c < P->child_ptr[s + 1]; ++c) {
+                const int32_t ch = P->child_idx[c];
+                const FrontLd &C = P->fl[ch];
+                const int32_t q = C.f - C.p;
+                for (int32_t jj = 0; jj < q; ++jj) cc[cc0 + colfill[cmap[C.cmofs + jj]]++] = ChildCol{ch, jj};
             }
+            for (int32_t c = 0; c < F.f; ++c) tasks.push_back(AsmTask{s, c, cc0 + colstart[c], cc0 + colstart[c + 1]});
             for (size_t t = r0; t < lrows.size(); ++t) pos[lrows[t]] = -1;
             P->fl[s] = F;
             ids.push_back(s);
