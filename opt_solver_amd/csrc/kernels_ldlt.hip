@@ -1425,9 +1425,19 @@ __global__ __launch_bounds__(kLT) void k_big_gapply(LdltDev d, const BigLd *__re
         }
     }
     const int i = gend + bx * kGroupRows + tid;
-    if (i < f)
-        for (int c = 0; c < nacc; ++c)
-            G[static_cast<int64_t>(ps + c) * f + i] = W[static_cast<int64_t>(kGroupSpec + c) * f + i];
+    if (i < f) {
+        // eight columns' loads in flight at a time
+        const double *src = W + static_cast<int64_t>(kGroupSpec) * f + i;
+        double *dst = G + static_cast<int64_t>(ps) * f + i;
+        for (int c0 = 0; c0 < nacc; c0 += 8) {
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = src[static_cast<int64_t>(min(c0 + u, nacc - 1)) * f];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (c0 + u < nacc) dst[static_cast<int64_t>(c0 + u) * f] = v[u];
+        }
+    }
 }
 
 __global__ __launch_bounds__(kLT) void k_big_panel(LdltDev d, BigLd *states, Prm prm, int32_t nb,

@@ -502,13 +502,16 @@ __global__ __launch_bounds__(kLThreads) void k_ldlt_fwd_step(LdltDev d, const So
             for (int sb = 0; sb < nsub; ++sb) {
                 const int c0 = it.tk + sb * kLB, nc = min(kLB, it.tkb - sb * kLB);
                 load_unit_triangle(L, f, c0, nc, T);
-                // the sub-block's rows: by the block before, then by this
-                // block's solved sub-blocks
-                rows_times_block(L, f, it.k, it.kb, c0, c0 + nc, xs, part);
-                rows_times_block(L, f, it.tk, sb * kLB, c0, c0 + nc, xc, part2);
+                // the sub-block's rows: by the block before (when the item
+                // carries it), then by this block's solved sub-blocks
+                if (it.kb > 0) rows_times_block(L, f, it.k, it.kb, c0, c0 + nc, xs, part);
+                if (sb > 0) rows_times_block(L, f, it.tk, sb * kLB, c0, c0 + nc, xc, part2);
+                else __syncthreads();  // the triangle in LDS
                 if (tid < 64) {
                     const int l = tid;
-                    double yv = l < nc ? (yr[rows[c0 + l]] - part[0][l]) - part2[0][l] : 0.0;
+                    double yv = l < nc ? yr[rows[c0 + l]] : 0.0;
+                    if (it.kb > 0) yv -= part[0][l];
+                    if (sb > 0) yv -= part2[0][l];
                     for (int c = 0; c < nc; ++c) {
                         const double xv = __shfl(yv, c, 64);
                         if (l > c) yv -= T[c][l] * xv;
@@ -604,8 +607,12 @@ __global__ __launch_bounds__(kLThreads) void k_ldlt_bwd_step(LdltDev d, const So
             continue;
         }
         // the block's columns: contributions of the rows solved before
-        cols_times_rows(L, f, it.k, it.kb, it.tk, it.tk + it.tkb, xs,
-                        [&](int c, double acc) { upd[c - it.tk] = acc; });
+        // (when the item carries them)
+        if (it.kb > 0)
+            cols_times_rows(L, f, it.k, it.kb, it.tk, it.tk + it.tkb, xs,
+                            [&](int c, double acc) { upd[c - it.tk] = acc; });
+        else
+            for (int c = tid; c < it.tkb; c += kLThreads) upd[c] = 0.0;
         __syncthreads();
         const int nsub = (it.tkb + kLB - 1) / kLB;
         for (int sb = nsub - 1; sb >= 0; --sb) {

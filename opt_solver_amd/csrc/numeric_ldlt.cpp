@@ -450,16 +450,19 @@ static void front_solve_items(int32_t f, int32_t p, int32_t s, bool bwd, Emit &&
             emit(0, SolveItem{s, 0, 0, p, f, 0, p, Tri | Fused});
             return;
         }
-        for (int32_t t = 0; t < nb; ++t) {
-            const int32_t tk = t * B, tkb = std::min(B, p - tk);
-            const int32_t k = t > 0 ? tk - B : 0, kb = t > 0 ? B : 0;
-            emit(t, SolveItem{s, k, kb, 0, 0, tk, tkb, Tri});
-            if (t > 0)
-                for (int32_t r = tk + tkb; r < f; r += C) emit(t, SolveItem{s, k, kb, r, std::min(r + C, f), 0, 0, 0});
+        // block b: a step of row items applying block b - 1 to every row
+        // from block b down, then a step of its triangle item alone (so the
+        // triangle's one workgroup does no more than the block's own work)
+        int32_t t = 0;
+        emit(t++, SolveItem{s, 0, 0, 0, 0, 0, std::min(B, p), Tri});
+        for (int32_t b = 1; b < nb; ++b, ++t) {
+            const int32_t tk = b * B, tkb = std::min(B, p - tk), k = tk - B;
+            for (int32_t r = tk; r < f; r += C) emit(t, SolveItem{s, k, B, r, std::min(r + C, f), 0, 0, 0});
+            emit(++t, SolveItem{s, 0, 0, 0, 0, tk, tkb, Tri});
         }
         if (f > p) {
             const int32_t k = (nb - 1) * B;
-            for (int32_t r = p; r < f; r += C) emit(nb, SolveItem{s, k, p - k, r, std::min(r + C, f), 0, 0, 0});
+            for (int32_t r = p; r < f; r += C) emit(t, SolveItem{s, k, p - k, r, std::min(r + C, f), 0, 0, 0});
         }
         return;
     }
@@ -477,10 +480,12 @@ static void front_solve_items(int32_t f, int32_t p, int32_t s, bool bwd, Emit &&
     }
     const int32_t kL = (nb - 1) * B;
     emit(t++, SolveItem{s, 0, 0, 0, 0, kL, p - kL, Tri});
+    // block j: a step of column items applying block j + 1 to every column
+    // up to block j's last, then a step of its triangle item alone
     for (int32_t j = nb - 2; j >= 0; --j, ++t) {
         const int32_t tk = j * B, k = tk + B, kb = std::min(B, p - k);
-        emit(t, SolveItem{s, k, kb, 0, 0, tk, B, Tri});
-        for (int32_t c = 0; c < tk; c += C) emit(t, SolveItem{s, k, kb, c, c + C, 0, 0, 0});
+        for (int32_t c = 0; c < tk + B; c += C) emit(t, SolveItem{s, k, kb, c, c + C, 0, 0, 0});
+        emit(++t, SolveItem{s, 0, 0, 0, 0, tk, B, Tri});
     }
 }
 
