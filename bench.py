@@ -41,6 +41,18 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+TUNED = None  # the autotuner's choices for this device (opt_solver_amd/tuned), set in main
+
+
+def tuned(plan, posdef):
+    """Apply this device's tuned plan parameters, if the autotuner has
+    persisted any (`python -m opt_solver_amd.autotune`)."""
+    if TUNED:
+        from opt_solver_amd import autotune
+        autotune.apply_tuned(plan, TUNED, posdef)
+    return plan
+
+
 def metis_perm(name):
     """The reference's METIS ordering of a benchmark matrix (committed
     fixture written by tests/golden/make_golden.py), or None."""
@@ -147,7 +159,7 @@ def c4_measure(args, torch):
     if perm is None:
         perm = o.grid_nd((m, m, m), leaf=64)
     an = analyse(A, perm=perm, nemin=32)
-    plan = Plan(an, posdef=True)
+    plan = tuned(Plan(an, posdef=True), True)
     ps = plan.stats()
     log(f"c4 analysis+plan {time.perf_counter() - t0:.1f}s nsup={an.nsup} maxfront={an.max_front} "
         f"flops={ps['flops']:.3e} levels={int(ps['levels'])} launches={int(ps['launches'])}")
@@ -187,7 +199,7 @@ def kkt_measure(args, torch):
     b_d = torch.from_numpy(_matvec(A, x)).to(dev)
     torch.cuda.synchronize()
     an = analyse(A, perm=_kkt_perm(gsz, m, A), nemin=32)
-    plan = Plan(an, posdef=False)
+    plan = tuned(Plan(an, posdef=False), False)
     run = Runner(plan, vals_d, b_d, 1, an.flops, False)
     out = {}
     for name, prec in (("fp64", PREC_FP64), ("fp32", PREC_FP32), ("learned", PREC_AUTO)):
@@ -199,7 +211,7 @@ def kkt_measure(args, torch):
     if perm is not None and not args.no_kkt_metis:
         del plan, run
         anm = analyse(A, perm=perm, nemin=32)
-        planm = Plan(anm, posdef=False)
+        planm = tuned(Plan(anm, posdef=False), False)
         runm = Runner(planm, vals_d, b_d, 1, anm.flops, False)
         r = runm.measure("kkt metis fp64", PREC_FP64, 1, 0, timed_run=False)
         res["metis_fp64"] = {"workload": "the same KKT on the reference's METIS ordering (fixture)",
@@ -221,7 +233,7 @@ def seq_measure(args, torch):
     nx = G * G
     A0, s0 = g.kkt_grid(G, m, seed=5)
     an = analyse(A0, perm=_kkt_perm(G, m, A0), nemin=32)
-    plan = Plan(an, posdef=False)
+    plan = tuned(Plan(an, posdef=False), False)
     dev = torch.device("cuda:0")
     nref = args.seq_refactors
     vals = [torch.from_numpy(g.kkt_grid_refactor_values(A0, g.kkt_sigma_sequence(s0, k, seed=6), nx)).to(dev)
@@ -277,7 +289,7 @@ def batch_measure(steps, warmup, nsys=1024):
     t0 = time.perf_counter()
     mats, Ts = batch_systems(nsys)
     U, offs = batch.union_arrays([analyse(A, nemin=16).arrays() for A in mats])
-    plan = Plan(U, posdef=False)
+    plan = tuned(Plan(U, posdef=False), False)
     vals = np.concatenate([A[3] for A in mats])
     X = np.concatenate([np.random.default_rng(i).uniform(-1, 1, A[0]) for i, A in enumerate(mats)])
     starts = np.cumsum([0] + [A[0] for A in mats])
@@ -324,7 +336,7 @@ def c1_measure(steps, warmup):
     from opt_solver_amd.solver import PREC_FP64, Plan, analyse
     A = g.laplace7_3d(20)
     an = analyse(A, perm=o.grid_nd((20, 20, 20), leaf=32), nemin=32)
-    plan = Plan(an, posdef=True)
+    plan = tuned(Plan(an, posdef=True), True)
     dev = torch.device("cuda:0")
     X = np.random.default_rng(0).uniform(-1, 1, (A[0], 1))
     vals_d = torch.from_numpy(A[3]).to(dev)
@@ -382,6 +394,27 @@ def batch_cpu_baseline(sample, threads):
                       f"{fl:.3e} flops"}
 
 
+def autotune_workloads(args):
+    """The autotuner on this benchmark's workloads: configuration 4 on the
+    reference's METIS ordering (the timed one), the grid KKT on 300^2 with
+    the geometric ordering, the 1024-system batch."""
+    from opt_solver_amd import autotune, batch, generators as g, ordering as o
+    from opt_solver_amd.solver import analyse
+    wl = {}
+    A = g.spd27_3d(args.m, seed=0)
+    perm = metis_perm(f"spd27_{args.m}") if args.ordering == "metis" else None
+    if perm is None:
+        perm = o.grid_nd((args.m, args.m, args.m), leaf=64)
+    wl["spd"] = autotune.workload(A, analyse(A, perm=perm, nemin=32).arrays(), True)
+    gsz, m = 300, 30000
+    K, _ = g.kkt_grid(gsz, m, seed=1)
+    wl["kkt"] = autotune.workload(K, analyse(K, perm=_kkt_perm(gsz, m, K), nemin=32).arrays(), False)
+    mats, _ = batch_systems(1024)
+    U, _ = batch.union_arrays([analyse(Ai, nemin=16).arrays() for Ai in mats])
+    wl["batch"] = autotune.workload((int(U["n"]), None, None, np.concatenate([Ai[3] for Ai in mats])), U, False)
+    autotune.autotune(workloads=wl, log=log)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -402,6 +435,10 @@ def main():
     ap.add_argument("--no-seq", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-c1", action="store_true")
+    ap.add_argument("--autotune", action="store_true",
+                    help="tune the plan parameters on this benchmark's workloads (configuration 4 on the "
+                         "reference's METIS ordering, the grid KKT on 300^2, the 1024-system batch), write "
+                         "opt_solver_amd/tuned/<device>.json and exit")
     ap.add_argument("--copy-calib", action="store_true",
                     help="run one plain 1 GiB device copy first (scale for the HBM byte counters)")
     args = ap.parse_args()
@@ -414,6 +451,14 @@ def main():
         print("no GPU", file=sys.stderr)
         sys.exit(1)
     dev = torch.device("cuda:0")
+    global TUNED
+    from opt_solver_amd import autotune
+    if args.autotune:
+        autotune_workloads(args)
+        return
+    TUNED = autotune.load_tuned(autotune.device_name())
+    if TUNED:
+        log("tuned plan parameters:", json.dumps(TUNED["knobs"]))
     if args.copy_calib:
         src = torch.ones(1 << 27, dtype=torch.float64, device=dev)
         dst = torch.empty_like(src)
@@ -499,6 +544,8 @@ def main():
                                        "+ WRITE_SIZE per Schur launch pair of the learned step (METIS tree), "
                                        "calibrated on a 1 GiB device copy; not measured in this run"},
         "cpu_baseline": cpu,
+        "tuned": ({"file": os.path.relpath(autotune.tuned_path(TUNED["device"]), HERE), "knobs": TUNED["knobs"]}
+                  if TUNED else None),
         **extra,
     }
     print(json.dumps(out))
