@@ -1803,6 +1803,34 @@ __device__ __forceinline__ void big_tile_w(const LdltDev &d, const FrontLd &F, c
     tile_loop_bf16p_w<MODE == 3, kBigKcBf>(pl, pl + ps, pl + 2 * ps, pl + 3 * ps, f, i0, j0, f, c1, 0, np, active, smem,
                                            acc);
     if (!active) return;
+    const int ib0 = i0 + wr * 64, jb0 = j0 + wc * 128;
+    if (ib0 + 63 < f && jb0 + 127 < c1 && ib0 >= jb0 + 127) {
+        // the wave's 64 x 128 block lies inside the front's lower triangle:
+        // no masks, and two column blocks (32 entries per lane) at a time,
+        // their loads in flight together before their stores
+        double *base = G + static_cast<int64_t>(jb0 + 4 * (lane >> 4)) * f + ib0 + (lane & 15);
+#pragma unroll
+        for (int m0 = 0; m0 < 8; m0 += 2) {
+            double old[2][4][4];
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const double *col = base + static_cast<int64_t>((m0 + h) * 16 + v) * f;
+#pragma unroll
+                    for (int nn = 0; nn < 4; ++nn) old[h][nn][v] = col[nn * 16];
+                }
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    double *col = base + static_cast<int64_t>((m0 + h) * 16 + v) * f;
+#pragma unroll
+                    for (int nn = 0; nn < 4; ++nn) col[nn * 16] = old[h][nn][v] - acc[m0 + h][nn][v];
+                }
+        }
+        return;
+    }
     double *junk = d.junk + threadIdx.x;
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
@@ -1810,11 +1838,11 @@ __device__ __forceinline__ void big_tile_w(const LdltDev &d, const FrontLd &F, c
         double old[4][4];
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            const int j = j0 + wc * 128 + m * 16 + 4 * (lane >> 4) + v;
+            const int j = jb0 + m * 16 + 4 * (lane >> 4) + v;
             double *col = G + static_cast<int64_t>(min(j, c1 - 1)) * f;
 #pragma unroll
             for (int nn = 0; nn < 4; ++nn) {
-                const int i = i0 + wr * 64 + nn * 16 + (lane & 15);
+                const int i = ib0 + nn * 16 + (lane & 15);
                 dst[nn][v] = (i < f && j < c1 && i >= j) ? col + i : junk;
             }
         }
