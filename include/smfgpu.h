@@ -144,7 +144,20 @@ typedef struct {
     int32_t scaled;          /* 1: the values are scaled (the selector's scaling-state feature) */
     int32_t collect;         /* 1 (FP64 only): keep the features of the fronts of at least
                                 mixed_min_front rows for smfgpu_selector_pairs */
-    int32_t reserved[2];
+    int32_t scale_mem;       /* SMFGPU_MEM_* of scale */
+    int32_t reserved;
+    /* Symmetric scaling (ScalingMethod::UserSupplied, control.hpp:148; the
+     * S A S pass of factor_indef.cpp:508-559 and factor_posdef.cpp:405-421):
+     * null, or n finite positive doubles indexed by the original (analysed
+     * matrix's) row. The factorization then factors S A S, each entry
+     * a_rc * (s_r * s_c), with max|S A S| in place of max|A| for the pivot
+     * thresholds; its solves return x = S (S A S)^-1 S b (solve.cpp:1022-1067)
+     * and its refinement computes residuals with the unscaled A. An invalid
+     * entry (non-finite or <= 0) fails with SMFGPU_ERR_INVALID_ARG
+     * (validate_positive_scale, scaling.cpp:9-19). Sets the selector's
+     * scaling-state feature. smfgpu_equilibration_scale computes the
+     * reference's Equilibration scaling on the device. */
+    const double *scale;
 } smfgpu_factor_options;
 
 /* static_flags (control.hpp:349-364) */
@@ -197,6 +210,18 @@ SMFGPU_API void smfgpu_plan_free(smfgpu_plan_t *plan);
  * delay-free flops, factor entries, contribution-block entries, assembly
  * bytes (algorithmic), solve bytes per RHS (algorithmic). */
 SMFGPU_API int smfgpu_plan_stats(smfgpu_plan_t plan, double *out);
+
+/* Equilibration scaling on the device (ScalingMethod::Equilibration,
+ * compute_equilibration_scale, solver/src/scaling.cpp:57-101): from s = 1,
+ * iters passes (the reference runs 3) of s_i /= sqrt(max_j |a_ij| s_i s_j)
+ * over the symmetric matrix of the analysed pattern, every row's maximum
+ * taken with the previous pass's s; rows without a nonzero keep s_i. values:
+ * one per entry (memory kind value_mem, as for smfgpu_factor); scale
+ * receives the n factors in the original ordering (memory kind scale_mem),
+ * ready for smfgpu_factor_options.scale. The same s as the reference's to
+ * the bit. Non-finite values fail with SMFGPU_ERR_VALUE. */
+SMFGPU_API int smfgpu_equilibration_scale(smfgpu_plan_t plan, const double *values, int64_t nvalues,
+                                          int32_t value_mem, int32_t iters, double *scale, int32_t scale_mem);
 
 /* ---- numeric factorization. values: one per entry of the analysed
  * matrix (host or device memory per opt->value_mem). The returned factor

@@ -40,7 +40,7 @@ class FactorOptions(ctypes.Structure):
         ("mixed_min_front", _I), ("max_negative_pivots", _I), ("value_mem", _I),
         ("work_limit_on", _I), ("work_limit", _D), ("static_pivot_tol", _D),
         ("fallback_static_pivot_tol", _D), ("static_flags", _I), ("selector_mask", _I), ("selector_tol", _D),
-        ("scaled", _I), ("collect", _I), ("reserved", _I * 2),
+        ("scaled", _I), ("collect", _I), ("scale_mem", _I), ("reserved", _I), ("scale", _P),
     ]
 
 
@@ -79,6 +79,7 @@ EXPORTED = [
     "smfgpu_factor_front_stats", "smfgpu_launch_profile", "smfgpu_refine_default_options", "smfgpu_refine_ex",
     "smfgpu_solve_noalloc", "smfgpu_factor_precisions", "smfgpu_selector_set", "smfgpu_selector_pairs",
     "smfgpu_selector_train", "smfgpu_selector_predict", "smfgpu_factor_solve", "smfgpu_factor_stall_stats",
+    "smfgpu_equilibration_scale",
 ]
 SELECTOR_FEATURES, SELECTOR_OUTPUTS, SELECTOR_WEIGHTS = 10, 3, 1527
 STATIC_AS_ZERO = 1
@@ -138,6 +139,7 @@ def lib():
     L.smfgpu_selector_train.argtypes = [_P, _P, _P, _I, _I, ctypes.c_float, _P, _P]
     L.smfgpu_selector_predict.argtypes = [_P, _P, _I, _P]
     L.smfgpu_factor_solve.argtypes = [_P, _P, _L, _P, _P, _P, _I, _I, _P, _I, _P, _P, _P]
+    L.smfgpu_equilibration_scale.argtypes = [_P, _P, _L, _I, _I, _P, _I]
     L.smfgpu_refine.argtypes = [_P, _P, _P, _P, _I, _I, _D, _I, _P]
     L.smfgpu_refine_default_options.argtypes = [_P]
     L.smfgpu_refine_ex.argtypes = [_P, _P, _P, _I, _P, _I, _I, _P, _I, _P]

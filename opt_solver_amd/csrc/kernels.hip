@@ -90,6 +90,129 @@ void launch_map_values(hipStream_t st, const double *vals, int64_t nvals, const 
 }
 
 // ---------------------------------------------------------------------------
+// A1: symmetric scaling. The permuted lower slots are reached through the
+// full symmetric CSR (row i of it holds the slots (i, j <= i) once each);
+// an entry's original row and column are r = max(perm[i], perm[j]),
+// c = min(perm[i], perm[j]) (the cleaned matrix is lower in the original
+// numbering).
+
+__device__ __forceinline__ unsigned long long block_max_u64(unsigned long long m) {
+    __shared__ unsigned long long red[kThreads];
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w && red[threadIdx.x + w] > red[threadIdx.x]) red[threadIdx.x] = red[threadIdx.x + w];
+        __syncthreads();
+    }
+    return red[0];
+}
+
+// S A S (factor_indef.cpp:530-546, apply_scale_to_matrix scaling.cpp:93-101):
+// av = a * (s_r * s_c), the unscaled a kept in raw (the refinement's
+// residual matrix), max |S A S| as bits into mx.
+__global__ void k_scale_slots(int32_t n, const int64_t *__restrict__ fr_ptr, const int32_t *__restrict__ fr_col,
+                              const int64_t *__restrict__ fr_slot, const int32_t *__restrict__ perm,
+                              const double *__restrict__ s, double *__restrict__ av, double *__restrict__ raw,
+                              unsigned long long *mx) {
+    unsigned long long m = 0;
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int32_t pi = perm[i];
+        const double si = s[pi];
+        for (int64_t k = fr_ptr[i]; k < fr_ptr[i + 1]; ++k) {
+            const int32_t j = fr_col[k];
+            if (j > i) continue;
+            const int64_t sl = fr_slot[k];
+            const double a = av[sl];
+            raw[sl] = a;
+            const double v = a * (si * s[perm[j]]);
+            av[sl] = v;
+            const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(fabs(v)));
+            m = b > m ? b : m;
+        }
+    }
+    m = block_max_u64(m);
+    if (threadIdx.x == 0) atomicMax(mx, m);
+}
+
+// red[0] = max|S A S| unless it holds the non-finite mark of the input
+__global__ void k_take_scaled_max(unsigned long long *red) {
+    if (threadIdx.x == 0 && red[0] < 0x7ff0000000000000ull) red[0] = red[3];
+}
+
+// bad = 1 when a scale entry is not finite and positive
+// (validate_positive_scale, scaling.cpp:9-19)
+__global__ void k_check_scale(const double *__restrict__ s, int32_t n, unsigned long long *bad) {
+    unsigned long long m = 0;
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double v = s[i];
+        if (!(v > 0.0) || !(v < INFINITY)) m = 1;
+    }
+    m = block_max_u64(m);
+    if (threadIdx.x == 0 && m) atomicMax(bad, m);
+}
+
+// One equilibration pass (compute_equilibration_scale, scaling.cpp:64-89):
+// row_max_i = max over row i of |a| * s_r * s_c (that product order, the
+// previous pass's s), then s_i / sqrt(row_max_i) when row_max_i > 0.
+__global__ void k_equil_pass(int32_t n, const int64_t *__restrict__ fr_ptr, const int32_t *__restrict__ fr_col,
+                             const int64_t *__restrict__ fr_slot, const int32_t *__restrict__ perm,
+                             const double *__restrict__ av, const double *__restrict__ s_in,
+                             double *__restrict__ s_out) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t pi = perm[i];
+    double rm = 0.0;
+    for (int64_t k = fr_ptr[i]; k < fr_ptr[i + 1]; ++k) {
+        const int32_t pj = perm[fr_col[k]];
+        const int32_t r = max(pi, pj), c = min(pi, pj);
+        const double v = fabs(av[fr_slot[k]]) * s_in[r] * s_in[c];
+        rm = fmax(rm, v);
+    }
+    s_out[pi] = rm > 0.0 ? s_in[pi] / sqrt(rm) : s_in[pi];
+}
+
+__global__ void k_fill_f64(double *p, int64_t n, double v) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+void launch_scale_values(hipStream_t st, int32_t n, const int64_t *fr_ptr, const int32_t *fr_col,
+                         const int64_t *fr_slot, const int32_t *perm, const double *scale, double *av, double *raw,
+                         double *redbuf) {
+    auto *rb = reinterpret_cast<unsigned long long *>(redbuf);
+    k_fill_u64<<<1, 64, 0, st>>>(rb + 3, 1, 0ull);
+    if (n > 0) {
+        const int g = std::min((n + kThreads - 1) / kThreads, 4096);
+        k_scale_slots<<<dim3(g), kThreads, 0, st>>>(n, fr_ptr, fr_col, fr_slot, perm, scale, av, raw, rb + 3);
+    }
+    k_take_scaled_max<<<1, 64, 0, st>>>(rb);
+}
+
+void launch_check_scale(hipStream_t st, const double *scale, int32_t n, double *flag) {
+    auto *fb = reinterpret_cast<unsigned long long *>(flag);
+    k_fill_u64<<<1, 64, 0, st>>>(fb, 1, 0ull);
+    if (n > 0) {
+        const int g = std::min((n + kThreads - 1) / kThreads, 1024);
+        k_check_scale<<<dim3(g), kThreads, 0, st>>>(scale, n, fb);
+    }
+}
+
+void launch_equilibrate(hipStream_t st, int32_t n, const int64_t *fr_ptr, const int32_t *fr_col,
+                        const int64_t *fr_slot, const int32_t *perm, const double *av, int32_t iters, double *s0,
+                        double *s1, double **result) {
+    double *cur = s0, *nxt = s1;
+    if (n > 0) {
+        const unsigned g = static_cast<unsigned>((n + kThreads - 1) / kThreads);
+        k_fill_f64<<<dim3(g), kThreads, 0, st>>>(cur, n, 1.0);
+        for (int32_t it = 0; it < iters; ++it) {
+            k_equil_pass<<<dim3(g), kThreads, 0, st>>>(n, fr_ptr, fr_col, fr_slot, perm, av, cur, nxt);
+            std::swap(cur, nxt);
+        }
+    }
+    *result = cur;
+}
+
+// ---------------------------------------------------------------------------
 // A3: assembly. One workgroup per front column; the column (rows from the
 // diagonal down) is built in LDS in row chunks: zero, original entries (CSC
 // order), then each child's CB column in child order (the reference's
@@ -636,32 +759,47 @@ void launch_zero_upper(hipStream_t, const DevPtrs &, const PotrfItem *, int64_t)
 // ---------------------------------------------------------------------------
 // A15: solves. y is the permuted right-hand-side block (n x nrhs, ld n).
 
+// With a scale s (original numbering), y_i = b_perm[i] * s_perm[i] and
+// b_perm[i] = y_i * s_perm[i]: the scaled gather and scatter of solve_core's
+// full job (solve.cpp:1022-1047), which the forward and backward jobs equal
+// (apply_scale_to_rhs before the permuting forward sweep, apply_scale_to_
+// solution after the permuting backward sweep, scaling.cpp:105-119).
 __global__ void k_gather(const int32_t *__restrict__ perm, const double *__restrict__ b, int64_t ldb,
-                         double *__restrict__ y, int32_t n, int32_t nrhs) {
+                         double *__restrict__ y, int32_t n, int32_t nrhs, const double *__restrict__ sc) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int32_t o = perm[i];
-    for (int r = 0; r < nrhs; ++r) y[i + static_cast<int64_t>(r) * n] = b[o + r * ldb];
+    if (sc) {
+        const double so = sc[o];
+        for (int r = 0; r < nrhs; ++r) y[i + static_cast<int64_t>(r) * n] = b[o + r * ldb] * so;
+    } else {
+        for (int r = 0; r < nrhs; ++r) y[i + static_cast<int64_t>(r) * n] = b[o + r * ldb];
+    }
 }
 
 __global__ void k_scatter(const int32_t *__restrict__ perm, const double *__restrict__ y, double *__restrict__ b,
-                          int64_t ldb, int32_t n, int32_t nrhs) {
+                          int64_t ldb, int32_t n, int32_t nrhs, const double *__restrict__ sc) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int32_t o = perm[i];
-    for (int r = 0; r < nrhs; ++r) b[o + r * ldb] = y[i + static_cast<int64_t>(r) * n];
+    if (sc) {
+        const double so = sc[o];
+        for (int r = 0; r < nrhs; ++r) b[o + r * ldb] = y[i + static_cast<int64_t>(r) * n] * so;
+    } else {
+        for (int r = 0; r < nrhs; ++r) b[o + r * ldb] = y[i + static_cast<int64_t>(r) * n];
+    }
 }
 
 void launch_gather_rhs(hipStream_t st, const int32_t *perm, const double *b, int64_t ldb, double *y,
-                       int32_t n, int32_t nrhs) {
+                       int32_t n, int32_t nrhs, const double *scale) {
     if (n <= 0) return;
-    k_gather<<<dim3((n + kThreads - 1) / kThreads), kThreads, 0, st>>>(perm, b, ldb, y, n, nrhs);
+    k_gather<<<dim3((n + kThreads - 1) / kThreads), kThreads, 0, st>>>(perm, b, ldb, y, n, nrhs, scale);
 }
 
 void launch_scatter_rhs(hipStream_t st, const int32_t *perm, const double *y, double *b, int64_t ldb,
-                        int32_t n, int32_t nrhs) {
+                        int32_t n, int32_t nrhs, const double *scale) {
     if (n <= 0) return;
-    k_scatter<<<dim3((n + kThreads - 1) / kThreads), kThreads, 0, st>>>(perm, y, b, ldb, n, nrhs);
+    k_scatter<<<dim3((n + kThreads - 1) / kThreads), kThreads, 0, st>>>(perm, y, b, ldb, n, nrhs, scale);
 }
 
 // ---------------------------------------------------------------------------

@@ -864,6 +864,33 @@ SMFGPU_API int smfgpu_factor(smfgpu_plan_t P, const double *values, int64_t nval
                               st));
         dv = P->d_vals.p;
     }
+    // symmetric scaling: the scale into d_scale, checked, for the value
+    // preparation's S A S pass and the solves
+    P->scaled = false;
+    if (opt.scale) {
+        const int32_t n = P->n;
+        int rc;
+        if (P->d_scale.n < static_cast<size_t>(std::max(n, 1)) || P->d_av_raw.n < static_cast<size_t>(std::max<int64_t>(P->nnz_perm, 1))) {
+            if ((rc = P->d_scale.alloc(std::max(n, 1))) || (rc = P->d_av_raw.alloc(std::max<int64_t>(P->nnz_perm, 1))))
+                return rc;
+            release_graph(P);  // the replayed sequence reads both buffers
+        }
+        if (opt.scale_mem == SMFGPU_MEM_HOST) {
+            for (int32_t i = 0; i < n; ++i)
+                if (!(opt.scale[i] > 0.0) || !std::isfinite(opt.scale[i]))
+                    return set_error(SMFGPU_ERR_INVALID_ARG, "factor: scale entries must be finite and positive");
+            HIP_OK(hipMemcpyAsync(P->d_scale.p, opt.scale, n * sizeof(double), hipMemcpyHostToDevice, st));
+        } else {
+            HIP_OK(hipMemcpyAsync(P->d_scale.p, opt.scale, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+            launch_check_scale(st, P->d_scale.p, n, P->d_red.p + 4);
+            unsigned long long bad = 0;
+            HIP_OK(hipMemcpyAsync(&bad, P->d_red.p + 4, sizeof(bad), hipMemcpyDeviceToHost, st));
+            HIP_OK(hipStreamSynchronize(st));
+            if (bad) return set_error(SMFGPU_ERR_INVALID_ARG, "factor: scale entries must be finite and positive");
+        }
+        opt.scaled = 1;
+        P->scaled = true;
+    }
     if (!P->posdef) {
         auto F = std::make_unique<smfgpu_factor_s>();
         F->plan = P;
@@ -988,6 +1015,9 @@ SMFGPU_API int smfgpu_factor(smfgpu_plan_t P, const double *values, int64_t nval
         };
         if (rec_all) mark(2 * nl);
         launch_map_values(st, dv, nvalues, P->d_slot_ptr.p, P->d_slot_src.p, P->nnz_perm, P->d_av.p, P->d_red.p);
+        if (P->scaled)
+            launch_scale_values(st, P->n, P->d_fr_ptr.p, P->d_fr_col.p, P->d_fr_slot.p, P->d_perm.p, P->d_scale.p,
+                                P->d_av.p, P->d_av_raw.p, P->d_red.p);
         for (size_t li = 0; li < nl; ++li) {
             const Launch &l = LL[li];
             const bool rec = rec_all && (l.kind == K_SYRK || l.kind == K_ASSEMBLE);
@@ -1019,15 +1049,18 @@ SMFGPU_API int smfgpu_factor(smfgpu_plan_t P, const double *values, int64_t nval
     if (timing || nvalues == 0 || opt.collect) {
         HIP_OK(enqueue(timing));
     } else {
-        if (P->fexec[gi] && P->graph_red_modes[gi] != red_modes) {
-            // captured with another set of reduced-precision kernels
+        // the captured sequence: its reduced-precision kernels, and the
+        // scaling pass when the values are scaled
+        const int32_t gkey = red_modes | (P->scaled ? (1 << 20) : 0);
+        if (P->fexec[gi] && P->graph_red_modes[gi] != gkey) {
+            // captured with another set of kernels
             (void)hipGraphExecDestroy(P->fexec[gi]);
             (void)hipGraphDestroy(P->fgraph[gi]);
             P->fexec[gi] = nullptr;
             P->fgraph[gi] = nullptr;
         }
         if (!P->fexec[gi]) {
-            P->graph_red_modes[gi] = red_modes;
+            P->graph_red_modes[gi] = gkey;
             HIP_OK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             const hipError_t e1 = enqueue(false);
             hipGraph_t g = nullptr;
@@ -1112,6 +1145,46 @@ SMFGPU_API int smfgpu_factor(smfgpu_plan_t P, const double *values, int64_t nval
     P->max_abs = max_abs;
     if (info) *info = fi;
     *out = F.release();
+    return SMFGPU_OK;
+}
+
+SMFGPU_API int smfgpu_equilibration_scale(smfgpu_plan_t P, const double *values, int64_t nvalues, int32_t value_mem,
+                                          int32_t iters, double *scale, int32_t scale_mem) {
+    if (!P || !scale) return set_error(SMFGPU_ERR_INVALID_ARG, "null argument");
+    if (nvalues != P->n_orig) return set_error(SMFGPU_ERR_INVALID_ARG, "equilibration: wrong number of values");
+    if (nvalues > 0 && !values) return set_error(SMFGPU_ERR_INVALID_ARG, "equilibration: null values");
+    if (iters < 0) return set_error(SMFGPU_ERR_INVALID_ARG, "equilibration: iters must be >= 0");
+    hipStream_t st = P->stream;
+    const int32_t n = P->n;
+    int rc;
+    // the values mapped into the refinement's scratch slots (rewritten by
+    // every refinement against other values, so free between calls)
+    if (P->d_av_ref.n < static_cast<size_t>(std::max<int64_t>(P->nnz_perm, 1)) &&
+        (rc = P->d_av_ref.alloc(std::max<int64_t>(P->nnz_perm, 1))))
+        return rc;
+    if (P->d_eqs.n < 2 * static_cast<size_t>(std::max(n, 1)) && (rc = P->d_eqs.alloc(2 * static_cast<size_t>(std::max(n, 1)))))
+        return rc;
+    const double *vd = values;
+    if (value_mem == SMFGPU_MEM_HOST && nvalues > 0) {
+        if (P->d_vals.n < static_cast<size_t>(nvalues)) {
+            if ((rc = P->d_vals.alloc(nvalues))) return rc;
+            release_graph(P);
+        }
+        HIP_OK(hipMemcpyAsync(P->d_vals.p, values, nvalues * sizeof(double), hipMemcpyHostToDevice, st));
+        vd = P->d_vals.p;
+    }
+    launch_map_values(st, vd, nvalues, P->d_slot_ptr.p, P->d_slot_src.p, P->nnz_perm, P->d_av_ref.p, P->d_red.p + 5);
+    double *res = nullptr;
+    launch_equilibrate(st, n, P->d_fr_ptr.p, P->d_fr_col.p, P->d_fr_slot.p, P->d_perm.p, P->d_av_ref.p, iters,
+                       P->d_eqs.p, P->d_eqs.p + std::max(n, 1), &res);
+    HIP_OK(hipGetLastError());
+    unsigned long long maxbits = 0;
+    HIP_OK(hipMemcpyAsync(&maxbits, P->d_red.p + 5, sizeof(maxbits), hipMemcpyDeviceToHost, st));
+    if (n > 0)
+        HIP_OK(hipMemcpyAsync(scale, res, n * sizeof(double),
+                              scale_mem == SMFGPU_MEM_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice, st));
+    HIP_OK(hipStreamSynchronize(st));
+    if (!(maxbits < 0x7ff0000000000000ull)) return set_error(SMFGPU_ERR_VALUE, "equilibration: non-finite input value");
     return SMFGPU_OK;
 }
 
@@ -1233,7 +1306,8 @@ int solve_device(smfgpu_plan_s *P, const double *b_dev, int64_t ldb, double *out
         }
         double *y = ybuf ? ybuf : P->d_y.p;
         const bool fwd = job == 0 || job == 1;
-        if (fwd) launch_gather_rhs(st, P->d_perm.p, b_dev + r0 * ldb, ldb, y, n, nr);
+        const double *sc = P->scaled ? P->d_scale.p : nullptr;
+        if (fwd) launch_gather_rhs(st, P->d_perm.p, b_dev + r0 * ldb, ldb, y, n, nr, sc);
         else
             HIP_OK(hipMemcpy2DAsync(y, n * sizeof(double), b_dev + r0 * ldb, ldb * sizeof(double),
                                     n * sizeof(double), nr, hipMemcpyDeviceToDevice, st));
@@ -1284,7 +1358,7 @@ int solve_device(smfgpu_plan_s *P, const double *b_dev, int64_t ldb, double *out
             P->t_solve_phase[2] = static_cast<double>(P->fwd_steps.size());
             P->t_solve_phase[3] = static_cast<double>(P->bwd_steps.size());
         }
-        launch_scatter_rhs(st, P->d_perm.p, y, out_dev + r0 * ldo, ldo, n, nr);
+        launch_scatter_rhs(st, P->d_perm.p, y, out_dev + r0 * ldo, ldo, n, nr, sc);
     }
     HIP_OK(hipGetLastError());
     return SMFGPU_OK;

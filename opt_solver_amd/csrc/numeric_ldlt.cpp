@@ -566,6 +566,10 @@ int ldlt_factor(smfgpu_plan_s *P, const double *dv, int64_t nvalues, const smfgp
     const int32_t n = P->n;
     P->ldlt_ready = false;
     launch_map_values(st, dv, nvalues, P->d_slot_ptr.p, P->d_slot_src.p, P->nnz_perm, P->d_av.p, P->d_red.p);
+    // S A S, max|S A S| in place of max|A| (factor_indef.cpp:508-559, 608-613)
+    if (P->scaled)
+        launch_scale_values(st, n, P->d_fr_ptr.p, P->d_fr_col.p, P->d_fr_slot.p, P->d_perm.p, P->d_scale.p, P->d_av.p,
+                            P->d_av_raw.p, P->d_red.p);
     unsigned long long maxbits = 0;
     HIP_OK(hipMemcpyAsync(&maxbits, P->d_red.p, sizeof(maxbits), hipMemcpyDeviceToHost, st));
     HIP_OK(hipStreamSynchronize(st));
@@ -945,7 +949,8 @@ int ldlt_solve(smfgpu_plan_s *P, const double *b_dev, int64_t ldb, double *out_d
         }
         double *y = ybuf ? ybuf : P->d_y.p;
         const bool fwd = job == 0 || job == 1;
-        if (fwd) launch_gather_rhs(st, P->d_perm.p, b_dev + r0 * ldb, ldb, y, n, nr);
+        const double *sc = P->scaled ? P->d_scale.p : nullptr;
+        if (fwd) launch_gather_rhs(st, P->d_perm.p, b_dev + r0 * ldb, ldb, y, n, nr, sc);
         else
             HIP_OK(hipMemcpy2DAsync(y, n * sizeof(double), b_dev + r0 * ldb, ldb * sizeof(double),
                                     n * sizeof(double), nr, hipMemcpyDeviceToDevice, st));
@@ -980,7 +985,7 @@ int ldlt_solve(smfgpu_plan_s *P, const double *b_dev, int64_t ldb, double *out_d
             P->t_solve_phase[2] = static_cast<double>(P->fwd_steps.size());
             P->t_solve_phase[3] = static_cast<double>(P->bwd_steps.size());
         }
-        launch_scatter_rhs(st, P->d_perm.p, y, out_dev + r0 * ldo, ldo, n, nr);
+        launch_scatter_rhs(st, P->d_perm.p, y, out_dev + r0 * ldo, ldo, n, nr, sc);
     }
     HIP_OK(hipGetLastError());
     return SMFGPU_OK;

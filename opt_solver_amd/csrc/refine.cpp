@@ -284,8 +284,9 @@ int refine_run(smfgpu_factor_s *f, const double *values, int32_t value_mem, cons
     const size_t un = static_cast<size_t>(n), nn = un * nrhs;
     std::memset(&info, 0, sizeof(info));
     int rc;
-    // the residual's matrix: the factored values, or others mapped the same way
-    const double *av = P->d_av.p;
+    // the residual's matrix: the factored values (unscaled when the factor
+    // is of S A S), or others mapped the same way
+    const double *av = P->scaled ? P->d_av_raw.p : P->d_av.p;
     double norm_a = P->norm_a;
     bool have_norm = P->norm_a_valid;
     if (values) {

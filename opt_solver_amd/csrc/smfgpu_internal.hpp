@@ -125,6 +125,21 @@ struct DevPtrs {
 
 void launch_map_values(hipStream_t st, const double *vals, int64_t nvals, const int64_t *slot_ptr,
                        const int64_t *slot_src, int64_t nslots, double *av, double *redbuf);
+// S A S in place over the permuted slots (scale in the original numbering),
+// the unscaled values into raw; redbuf[0] becomes max|S A S| (bits), unless
+// it marks a non-finite input; uses redbuf[3].
+void launch_scale_values(hipStream_t st, int32_t n, const int64_t *fr_ptr, const int32_t *fr_col,
+                         const int64_t *fr_slot, const int32_t *perm, const double *scale, double *av, double *raw,
+                         double *redbuf);
+// *flag (as an unsigned 64-bit integer) becomes 1 when an entry of scale[0..n)
+// is not finite and positive, else 0
+void launch_check_scale(hipStream_t st, const double *scale, int32_t n, double *flag);
+// iters equilibration passes over the permuted slots' values av, from s = 1,
+// ping-ponging between s0 and s1 (n doubles each); *result: the final s
+// (original numbering)
+void launch_equilibrate(hipStream_t st, int32_t n, const int64_t *fr_ptr, const int32_t *fr_col,
+                        const int64_t *fr_slot, const int32_t *perm, const double *av, int32_t iters, double *s0,
+                        double *s1, double **result);
 // fmax: the longest front of the launch's level (sizes the LDS column chunk)
 void launch_assemble(hipStream_t st, const DevPtrs &d, const AsmTask *tasks, int64_t ntasks, const ChildCol *cc,
                      int32_t fmax);
@@ -137,10 +152,12 @@ void launch_syrk(hipStream_t st, const DevPtrs &d, const SyrkItem *it, int64_t n
 void launch_zero_upper(hipStream_t st, const DevPtrs &d, const PotrfItem *it, int64_t n);
 
 // Solves: y is the permuted right-hand side block, n x nrhs (ld n).
+// scale (original numbering, null: none) multiplies the gathered and the
+// scattered entries
 void launch_gather_rhs(hipStream_t st, const int32_t *perm, const double *b, int64_t ldb, double *y,
-                       int32_t n, int32_t nrhs);
+                       int32_t n, int32_t nrhs, const double *scale = nullptr);
 void launch_scatter_rhs(hipStream_t st, const int32_t *perm, const double *y, double *b, int64_t ldb,
-                        int32_t n, int32_t nrhs);
+                        int32_t n, int32_t nrhs, const double *scale = nullptr);
 // One step of a sweep (nrhs <= 4): a workgroup per item.
 constexpr int32_t kSolveBlock = 128;  // pivot columns per step
 constexpr int32_t kSolveSeg = 512;    // rows per backward update item

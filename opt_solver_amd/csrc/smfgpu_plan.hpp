@@ -273,6 +273,11 @@ struct smfgpu_plan_s {
     DBuf<double> d_rvec, d_stat, d_part, d_h, d_av_ref;
     double norm_a = 0.0;
     bool norm_a_valid = false;
+    // symmetric scaling of the live factor: the scale (original numbering),
+    // the unscaled permuted values (the refinement's residual matrix), and
+    // the equilibration's two scale vectors
+    DBuf<double> d_scale, d_av_raw, d_eqs;
+    bool scaled = false;
     std::vector<int64_t> syrk_flops_of_launch, syrk_flops_of_launch_red;
     // learned selector: weights, per-column statistics of the assembled
     // pivot columns, per-front features, candidate fronts per level (SPD:

@@ -152,7 +152,10 @@ class Plan:
     def factor(self, values=None, values_ptr=None, precision=PREC_FP64, mixed_min_front=1024, u=0.01,
                small_pivot=1e-20, panel=16, max_negative_pivots=-1, work_limit=None, static_pivot_tol=0.0,
                fallback_static_pivot_tol=0.0, static_flags=0, selector_tol=1e-4, selector_mask=0, scaled=False,
-               collect=False) -> "Factor":
+               collect=False, scale=None, scale_ptr=None) -> "Factor":
+        """scale (n host doubles) or scale_ptr (a device pointer to them): the
+        symmetric scaling S of the original rows; the factor is then of S A S
+        and its solves return A^-1 b (smf's ScalingMethod::UserSupplied)."""
         if precision == PREC_AUTO and not getattr(self, "selector_loaded", False):
             w = default_selector_weights()
             if w is not None:
@@ -160,6 +163,7 @@ class Plan:
         opt, vp, keep = self._options(values, values_ptr, precision, mixed_min_front, u, small_pivot, panel,
                                       max_negative_pivots, work_limit, static_pivot_tol, fallback_static_pivot_tol,
                                       static_flags, selector_tol, selector_mask, scaled, collect)
+        skeep = self._scale_option(opt, scale, scale_ptr)
         info = _lib.FactorInfo()
         h = ctypes.c_void_p()
         check(lib().smfgpu_factor(self.h, vp, self.n_orig, ctypes.byref(opt), ctypes.byref(h), ctypes.byref(info)),
@@ -180,6 +184,7 @@ class Plan:
                                       kw.get("panel", 16), -1, None, kw.get("static_pivot_tol", 0.0),
                                       kw.get("fallback_static_pivot_tol", 0.0), kw.get("static_flags", 0),
                                       selector_tol, selector_mask, kw.get("scaled", False), False)
+        skeep = self._scale_option(opt, kw.get("scale"), None)
         b = np.array(b, dtype=np.float64, order="F", copy=True)
         nrhs = 1 if b.ndim == 1 else b.shape[1]
         x = np.zeros_like(b)
@@ -193,6 +198,36 @@ class Plan:
         rd = Factor._refine_dict(rinf)
         rd["fallback"] = bool(rinf.fallback)
         return Factor(self, h, info), x, rd
+
+    def _scale_option(self, opt, scale, scale_ptr):
+        if scale_ptr is not None:
+            opt.scale = ctypes.c_void_p(scale_ptr)
+            opt.scale_mem = MEM_DEVICE
+            return None
+        if scale is None:
+            return None
+        s = np.ascontiguousarray(scale, np.float64)
+        if s.size != self.n:
+            raise ValueError(f"expected {self.n} scale factors, got {s.size}")
+        opt.scale = s.ctypes.data_as(ctypes.c_void_p)
+        opt.scale_mem = MEM_HOST
+        return s
+
+    def equilibration_scale(self, values=None, values_ptr=None, iters=3):
+        """smf's Equilibration scaling (compute_equilibration_scale,
+        solver/src/scaling.cpp:57-101) of the given values, computed on the
+        device: n factors in the original ordering."""
+        s = np.zeros(self.n)
+        if values_ptr is not None:
+            vp, mem = ctypes.c_void_p(values_ptr), MEM_DEVICE
+        else:
+            v = np.ascontiguousarray(values, np.float64)
+            if v.size != self.n_orig:
+                raise ValueError(f"expected {self.n_orig} values, got {v.size}")
+            vp, mem = _p(v), MEM_HOST
+        check(lib().smfgpu_equilibration_scale(self.h, vp, self.n_orig, mem, int(iters), _p(s), MEM_HOST),
+              "equilibration_scale")
+        return s
 
     def _options(self, values, values_ptr, precision, mixed_min_front, u, small_pivot, panel, max_negative_pivots,
                  work_limit, static_pivot_tol, fallback_static_pivot_tol, static_flags, selector_tol, selector_mask,

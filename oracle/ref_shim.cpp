@@ -235,6 +235,50 @@ void *ref_factor_limits(void *handle, double u, int max_negative_pivots, double 
     return h;
 }
 
+// ref_factor_ex's plain threshold factorization with a symmetric scaling:
+// method 1 ScalingMethod::UserSupplied (user_scale, n doubles), 2
+// ScalingMethod::Equilibration (scaling.cpp:57-101). scale_out (n doubles,
+// may be null) receives the scale the factor holds (FactorKeep::scale);
+// stats as ref_factor_ex.
+void *ref_factor_scaled(void *handle, const double *values, int64_t nvalues, double u, int method,
+                        const double *user_scale, double *scale_out, double *stats) {
+    auto *an = static_cast<RefAnalysis *>(handle);
+    auto *h = new RefFactor;
+    h->an = an;
+    h->ctrl = base_control(an->posdef, 1);
+    h->ctrl.pivot_u = u;
+    h->ctrl.scaling = method == 1 ? ScalingMethod::UserSupplied
+                                  : (method == 2 ? ScalingMethod::Equilibration : ScalingMethod::None);
+    h->ctrl.user_scale = user_scale;
+    h->ctrl.scale_on_delay_ratio = -1.0;
+    h->ctrl.matching_trigger = MatchingTrigger::Predicted;
+    Solver s;
+    const double t0 = now_s();
+    const FactorStatus st = s.factor(*an->ak, std::span<const double>(values, static_cast<std::size_t>(nvalues)),
+                                     h->ctrl, h->info, h->fk, h->ws);
+    const double t1 = now_s();
+    h->ctrl.user_scale = nullptr;  // copied into the factor; the caller's array may go
+    if (scale_out)
+        for (std::size_t i = 0; i < h->fk.scale.size(); ++i) scale_out[i] = h->fk.scale[i];
+    if (stats) {
+        stats[0] = static_cast<double>(st);
+        stats[1] = static_cast<double>(h->info.status);
+        stats[2] = h->info.num_positive;
+        stats[3] = h->info.num_negative;
+        stats[4] = h->info.num_zero;
+        stats[5] = h->info.delayed_pivots;
+        stats[6] = h->info.two_by_two_pivots;
+        stats[7] = h->info.actual_flops;
+        stats[8] = h->info.front_work;
+        stats[9] = t1 - t0;
+        stats[10] = h->info.static_pivots;
+        stats[11] = h->info.max_front_size;
+        stats[12] = h->info.pivot_growth;
+        stats[13] = static_cast<double>(h->fk.scale.size());
+    }
+    return h;
+}
+
 void *ref_factor(void *handle, const double *values, int64_t nvalues, double u, int num_threads, double *stats) {
     return ref_factor_ex(handle, values, nvalues, u, num_threads, 0.0, 0.0, 0, stats);
 }

@@ -42,6 +42,8 @@ def lib():
         L.ref_factor.argtypes = [_P, _P, _L, _D, _I, _P]
         L.ref_factor_ex.restype = _P
         L.ref_factor_ex.argtypes = [_P, _P, _L, _D, _I, _D, _D, _I, _P]
+        L.ref_factor_scaled.restype = _P
+        L.ref_factor_scaled.argtypes = [_P, _P, _L, _D, _I, _P, _P, _P]
         L.ref_factor_limits.restype = _P
         L.ref_factor_limits.argtypes = [_P, _D, _I, _D, _P]
         L.ref_factor_sizes.argtypes = [_P, _P]
@@ -122,13 +124,23 @@ class RefFactor:
     STATUS = {0: "Success", 1: "NotPositiveDefinite", 2: "Singular", 3: "MaxPivotDelays", 4: "InertiaExceeded"}
 
     def __init__(self, an: RefAnalysis, values, u=0.01, threads=1, static_pivot_tol=0.0,
-                 fallback_static_pivot_tol=0.0, static_flags=0, max_negative_pivots=None, work_limit=None):
+                 fallback_static_pivot_tol=0.0, static_flags=0, max_negative_pivots=None, work_limit=None,
+                 scale=None, equilibrate=False):
         """max_negative_pivots / work_limit: the early stops, on the values the
-        analysis was built from (values must be those)."""
+        analysis was built from (values must be those). scale (n doubles):
+        ScalingMethod::UserSupplied; equilibrate: ScalingMethod::Equilibration;
+        self.scale then holds the factor's scale."""
         self.an = an
         vals = np.ascontiguousarray(values, np.float64)
         st = np.zeros(16)
-        if max_negative_pivots is not None or work_limit is not None:
+        self.scale = None
+        if scale is not None or equilibrate:
+            us = None if scale is None else np.ascontiguousarray(scale, np.float64)
+            out = np.zeros(an.n)
+            self.h = lib().ref_factor_scaled(an.h, _ptr(vals), vals.size, u, 1 if us is not None else 2,
+                                             None if us is None else _ptr(us), _ptr(out), _ptr(st))
+            self.scale = out if int(st[13]) == an.n else None
+        elif max_negative_pivots is not None or work_limit is not None:
             self.h = lib().ref_factor_limits(an.h, u, -1 if max_negative_pivots is None else int(max_negative_pivots),
                                              -1.0 if work_limit is None else float(work_limit), _ptr(st))
         else:
