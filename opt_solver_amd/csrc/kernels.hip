@@ -393,23 +393,30 @@ void launch_assemble(hipStream_t st, const DevPtrs &d, const AsmTask *tasks, int
 // NotPositiveDefinite.
 constexpr int kNb = 64;
 
-// Two wavefronts per diagonal block: thread (r, h), r the lane and h the
-// wave, holds row r of the block's lower triangle in columns [32 h, 32 h +
-// 32) (a[i] = A[r][32 h + i]) and rows [32 h, 32 h + 32) of column r of W,
+// kPotrfWaves wavefronts per diagonal block, each owning kPotrfCols = 64 /
+// kPotrfWaves columns: thread (r, h), r the lane and h the wave, holds row r
+// of the block's lower triangle in columns [C h, C h + C) (a[i] =
+// A[r][C h + i], C = kPotrfCols) and rows [C h, C h + C) of column r of W,
 // the identity turned into the unit lower inverse by the same row
-// operations (w[i] = W[32 h + i][r]). Step j: the wave owning column j puts
+// operations (w[i] = W[C h + i][r]). Step j: the wave owning column j puts
 // the multipliers g_c = A[c][j] / d_jj, the column A[.][j] and the row
 // W[j][.] in LDS (two alternating buffers: one barrier per step); then each
 // thread updates A[r][c] -= A[r][j] g_c for c > j and W[i][r] -= g_i W[j][r]
-// for i > j in its half. inv(L_kk) = D^-1/2 W. Rows and columns past kb are
-// the identity.
-__device__ __forceinline__ void potrf_steps(double (&a)[32], double (&w)[32], int r, int h, int kb, double (*gl)[kNb],
-                                            double (*cola)[kNb], double (*colw)[kNb], double *rsv, bool &bad) {
-    for (int jb = 0; jb < 2; ++jb) {
+// for i > j in its columns. inv(L_kk) = D^-1/2 W. Rows and columns past kb
+// are the identity. The 64 steps are a dependent chain; more waves shorten
+// each step's update (2 C FMAs per thread).
+constexpr int kPotrfWaves = 8;
+constexpr int kPotrfCols = kNb / kPotrfWaves;
+
+__device__ __forceinline__ void potrf_steps(double (&a)[kPotrfCols], double (&w)[kPotrfCols], int r, int h, int kb,
+                                            double (*gl)[kNb], double (*cola)[kNb], double (*colw)[kNb], double *rsv,
+                                            bool &bad) {
+    constexpr int C = kPotrfCols;
+    for (int jb = 0; jb < kPotrfWaves; ++jb) {
 #pragma unroll
-        for (int jj = 0; jj < 32; ++jj) {
-            const int j = 32 * jb + jj;
-            const int bs = jj & 1;
+        for (int jj = 0; jj < C; ++jj) {
+            const int j = C * jb + jj;
+            const int bs = j & 1;
             if (h == jb) {
                 const double ajj = a[jj];
                 const double djj = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(ajj), j),
@@ -430,22 +437,23 @@ __device__ __forceinline__ void potrf_steps(double (&a)[32], double (&w)[32], in
             const double arj = cola[bs][r], wjr = colw[bs][r];
             if (h == jb) {
 #pragma unroll
-                for (int i = jj + 1; i < 32; ++i) {
-                    a[i] -= arj * gl[bs][32 * jb + i];
-                    w[i] -= gl[bs][32 * jb + i] * wjr;
+                for (int i = jj + 1; i < C; ++i) {
+                    a[i] -= arj * gl[bs][C * jb + i];
+                    w[i] -= gl[bs][C * jb + i] * wjr;
                 }
             } else if (h > jb) {
 #pragma unroll
-                for (int i = 0; i < 32; ++i) {
-                    a[i] -= arj * gl[bs][32 * h + i];
-                    w[i] -= gl[bs][32 * h + i] * wjr;
+                for (int i = 0; i < C; ++i) {
+                    a[i] -= arj * gl[bs][C * h + i];
+                    w[i] -= gl[bs][C * h + i] * wjr;
                 }
             }
         }
     }
 }
 
-__global__ __launch_bounds__(128) void k_potrf(DevPtrs d, const PotrfItem *__restrict__ items) {
+__global__ __launch_bounds__(kNb * kPotrfWaves) void k_potrf(DevPtrs d, const PotrfItem *__restrict__ items) {
+    constexpr int C = kPotrfCols;
     __shared__ double gl[2][kNb];
     __shared__ double cola[2][kNb];
     __shared__ double colw[2][kNb];
@@ -459,15 +467,15 @@ __global__ __launch_bounds__(128) void k_potrf(DevPtrs d, const PotrfItem *__res
     const int h = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (threadIdx.x == 0) flag_bad = 0;
     bool bad = false;
-    double a[32], w[32];
+    double a[C], w[C];
 #pragma unroll
-    for (int i = 0; i < 32; ++i) {
-        const int c = 32 * h + i;
+    for (int i = 0; i < C; ++i) {
+        const int c = C * h + i;
         a[i] = base[static_cast<int64_t>(k + min(c, kb - 1)) * f + k + min(r, kb - 1)];
     }
 #pragma unroll
-    for (int i = 0; i < 32; ++i) {
-        const int c = 32 * h + i;
+    for (int i = 0; i < C; ++i) {
+        const int c = C * h + i;
         const bool in = r < kb && c < kb && r >= c;
         a[i] = masked(a[i], in) + ((!in && r == c) ? 1.0 : 0.0);
         w[i] = (c == r) ? 1.0 : 0.0;
@@ -480,8 +488,8 @@ __global__ __launch_bounds__(128) void k_potrf(DevPtrs d, const PotrfItem *__res
     // = W[c][r] / L_cc
     if (r < kb) {
 #pragma unroll
-        for (int i = 0; i < 32; ++i) {
-            const int c = 32 * h + i;
+        for (int i = 0; i < C; ++i) {
+            const int c = C * h + i;
             if (c < kb) base[static_cast<int64_t>(k + c) * f + k + r] = c <= r ? a[i] : w[i] * rsv[c];
         }
     }
@@ -490,7 +498,7 @@ __global__ __launch_bounds__(128) void k_potrf(DevPtrs d, const PotrfItem *__res
 void launch_potrf(hipStream_t st, const DevPtrs &d, const PotrfItem *it, int64_t n) {
     if (n <= 0) return;
     for (int64_t o = 0; o < n; o += kMaxLaunchGroups)
-        k_potrf<<<dim3(static_cast<unsigned>(std::min(kMaxLaunchGroups, n - o))), 128, 0, st>>>(d, it + o);
+        k_potrf<<<dim3(static_cast<unsigned>(std::min(kMaxLaunchGroups, n - o))), kNb * kPotrfWaves, 0, st>>>(d, it + o);
 }
 
 // Panel TRSM: rows [r0, r0 + 64) of columns [k, k + kb): X = A M, with
