@@ -449,6 +449,12 @@ SMFGPU_API int smfgpu_factor_solve(smfgpu_plan_t plan, const double *values, int
  * reads and writes the FP64 entries once, and its FP32 accumulators sum w
  * products. */
 #define SMFGPU_TUNE_LDLT_OUTER_WIDE 14
+/* SMFGPU_TUNE_ASM_WAVE_ROWS: SPD assembly levels whose fronts all have at
+ * most this many rows (in [0, 4096], default 2048; 0: none) assemble one
+ * column per wavefront, four per workgroup, instead of one column per
+ * workgroup: the bottom levels' many short columns are otherwise bound by
+ * workgroup dispatch. The same sums in the same order either way. */
+#define SMFGPU_TUNE_ASM_WAVE_ROWS 15
 SMFGPU_API int smfgpu_plan_tune(smfgpu_plan_t plan, int32_t key, int64_t value);
 
 /* Device time (ms) of the last factorization's dominant kernel family:

@@ -104,6 +104,7 @@ TUNE_SOLVE_PERSISTENT = 11
 TUNE_PANEL_OUTER_REDUCED = 12
 TUNE_LDLT_PLANES = 13
 TUNE_LDLT_OUTER_WIDE = 14
+TUNE_ASM_WAVE_ROWS = 15
 
 _lib = None
 

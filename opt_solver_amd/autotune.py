@@ -53,7 +53,11 @@ KNOBS = [
      "SYRK_AHEAD", [2, 1], 2, ("spd",)),
     ("SPD assembly: largest LDS column chunk (rows)",
      "ASM_CHUNK", [2048, 1024, 4096], 2048, ("spd",)),
+    ("SPD assembly: levels of fronts up to this many rows take a wavefront per column",
+     "ASM_WAVE_ROWS", [2048, 1024, 3072], 2048, ("spd",)),
 ]
+# the parameters of SPD plans (the others are set on LDL^T plans)
+SPD_KNOBS = ("PANEL_OUTER", "PANEL_OUTER_REDUCED", "CB_SEG", "SYRK_AHEAD", "ASM_CHUNK", "ASM_WAVE_ROWS")
 
 
 def device_key(name):
@@ -85,7 +89,7 @@ def apply_tuned(plan, tuned, posdef):
     if not tuned:
         return done
     for name, value in tuned["knobs"].items():
-        spd_knob = name in ("PANEL_OUTER", "PANEL_OUTER_REDUCED", "CB_SEG", "SYRK_AHEAD", "ASM_CHUNK")
+        spd_knob = name in SPD_KNOBS
         if spd_knob != bool(posdef):
             continue
         plan.tune(getattr(_lib, "TUNE_" + name), int(value))
@@ -217,7 +221,7 @@ def autotune(quick=False, reps=3, log=print, out_path=None, workloads=None):
                 continue
             p = plans[wk]
             for kname, v in best.items():
-                spd_knob = kname in ("PANEL_OUTER", "PANEL_OUTER_REDUCED", "CB_SEG", "SYRK_AHEAD", "ASM_CHUNK")
+                spd_knob = kname in SPD_KNOBS
                 if spd_knob == wl[wk]["posdef"]:
                     p.tune(getattr(_lib, "TUNE_" + kname), int(v))
             tot += _time_one(p, wl[wk], prec, reps)

@@ -368,9 +368,97 @@ __global__ __launch_bounds__(kThreads) void k_assemble(DevPtrs d, const AsmTask 
     }
 }
 
+// Levels of small fronts (no front of more than wave_rows rows, at most
+// kAsmWaveRowsMax): one wavefront per column, four columns per workgroup,
+// each column in its own LDS stretch. The bottom levels hold over a million short columns, and one
+// workgroup each left their assembly bound by workgroup dispatch. Each entry
+// is summed in the order of k_assemble (zero, original entries, children in
+// order); a wavefront's LDS accesses take effect in program order, so no
+// barrier separates the children.
+constexpr int kAsmWaves = kThreads / 64;
+
+__global__ __launch_bounds__(kThreads) void k_assemble_w(DevPtrs d, const AsmTask *__restrict__ tasks,
+                                                         int64_t ntasks, const ChildCol *__restrict__ cc,
+                                                         int stride) {
+    extern __shared__ double bufw[];  // kAsmWaves stretches of stride doubles; the last of each is the junk slot
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t ti = static_cast<int64_t>(blockIdx.x) * kAsmWaves + w;
+    if (ti >= ntasks) return;
+    double *buf = bufw + w * stride;
+    const int junk = stride - 1;
+    const AsmTask t = tasks[ti];
+    const FrontDev F = d.fronts[t.s];
+    const int c = t.c, f = F.f, len = f - c;
+    double *colp = col_base(d, F, c);
+    for (int r = lane; r < len; r += 64) buf[r] = 0.0;
+    if (c < F.p) {
+        const int j = F.col0 + c;
+        const int k0 = d.pcol_ptr[j], k1 = d.pcol_ptr[j + 1];
+        for (int k = k0 + lane; k < k1; k += 64) {
+            const int lr = d.amap[k];
+            buf[(lr >= c && lr < f) ? lr - c : junk] += d.av[k];
+        }
+    }
+    for (int e = t.cc0; e < t.cc1; ++e) {
+        const ChildCol ch = cc[e];
+        const FrontDev C = d.fronts[ch.ch];
+        const int q = C.q;
+        const int32_t *cm = d.cmap + d.cmofs[ch.ch];
+        const double *cbcol = d.CB + C.cbofs + static_cast<int64_t>(ch.jj) * q;
+        for (int i0 = ch.jj; i0 < q; i0 += 4 * 64) {
+            int x[4];
+            double v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int ii = min(i0 + u * 64 + lane, q - 1);
+                x[u] = cm[ii];
+                v[u] = cbcol[ii];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) buf[i0 + u * 64 + lane < q ? x[u] - c : junk] += v[u];
+        }
+    }
+    double ssq = 0.0, dg = 0.0, offmax = 0.0;
+    for (int r = lane; r < len; r += 64) {
+        const double v = buf[r];
+        colp[c + r] = v;
+        ssq += v * v;
+        if (r == 0) dg = fabs(v);
+        else offmax = fmax(offmax, fabs(v));
+    }
+    if (d.colstat != nullptr && c < F.p) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            ssq += __shfl_xor(ssq, o);
+            dg = fmax(dg, __shfl_xor(dg, o));
+            offmax = fmax(offmax, __shfl_xor(offmax, o));
+        }
+        if (lane == 0)
+            d.colstat[F.col0 + c] = make_float4(static_cast<float>(sqrt(ssq)), static_cast<float>(dg),
+                                                static_cast<float>(offmax), 0.f);
+    }
+}
+
 void launch_assemble(hipStream_t st, const DevPtrs &d, const AsmTask *tasks, int64_t ntasks,
-                     const ChildCol *cc, int32_t fmax) {
+                     const ChildCol *cc, int32_t fmax, int32_t chunk_cap, int32_t wave_rows) {
     if (ntasks <= 0) return;
+    if (fmax <= std::min(wave_rows, kAsmWaveRowsMax)) {
+        static bool wattr = false;
+        if (!wattr) {
+            (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_assemble_w),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      kAsmWaves * (kAsmWaveRowsMax + 1) * sizeof(double));
+            wattr = true;
+        }
+        const int stride = std::max(fmax, 1) + 1;
+        const int64_t groups = (ntasks + kAsmWaves - 1) / kAsmWaves;
+        for (int64_t g0 = 0; g0 < groups; g0 += kMaxLaunchGroups)
+            k_assemble_w<<<dim3(static_cast<unsigned>(std::min(kMaxLaunchGroups, groups - g0))), kThreads,
+                           kAsmWaves * stride * sizeof(double), st>>>(d, tasks + g0 * kAsmWaves,
+                                                                      ntasks - g0 * kAsmWaves, cc, stride);
+        return;
+    }
+    fmax = std::min(fmax, chunk_cap);
     static bool attr = false;
     if (!attr) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_assemble),

@@ -776,6 +776,12 @@ SMFGPU_API int smfgpu_plan_tune(smfgpu_plan_t P, int32_t key, int64_t value) {
         P->asm_chunk = static_cast<int32_t>(value);
         release_graph(P);
         return SMFGPU_OK;
+    case SMFGPU_TUNE_ASM_WAVE_ROWS:
+        if (value < 0 || value > kAsmWaveRowsMax)
+            return set_error(SMFGPU_ERR_INVALID_ARG, "tune: wave assembly rows in [0, 4096]");
+        P->asm_wave_rows = static_cast<int32_t>(value);
+        release_graph(P);
+        return SMFGPU_OK;
     case SMFGPU_TUNE_PANEL_OUTER_REDUCED:
         if (value < kPanelInner || value % kPanelInner != 0 || value > 4096)
             return set_error(SMFGPU_ERR_INVALID_ARG, "tune: outer panel must be a multiple of 64 in [64, 4096]");
@@ -1026,7 +1032,7 @@ SMFGPU_API int smfgpu_factor(smfgpu_plan_t P, const double *values, int64_t nval
             case K_ASSEMBLE: {
                 int32_t fmax = 1;
                 for (int32_t s : P->level_fronts[l.level]) fmax = std::max(fmax, P->fronts[s].f);
-                launch_assemble(st, d, P->d_asm.p + l.first, l.count, P->d_cc.p, std::min(fmax, P->asm_chunk));
+                launch_assemble(st, d, P->d_asm.p + l.first, l.count, P->d_cc.p, fmax, P->asm_chunk, P->asm_wave_rows);
                 break;
             }
             case K_POTRF: launch_potrf(st, d, P->d_potrf.p + l.first, l.count); break;

@@ -140,9 +140,12 @@ void launch_check_scale(hipStream_t st, const double *scale, int32_t n, double *
 void launch_equilibrate(hipStream_t st, int32_t n, const int64_t *fr_ptr, const int32_t *fr_col,
                         const int64_t *fr_slot, const int32_t *perm, const double *av, int32_t iters, double *s0,
                         double *s1, double **result);
-// fmax: the longest front of the launch's level (sizes the LDS column chunk)
+// fmax: the longest front of the launch's level; levels with fmax <=
+// wave_rows take a wavefront per column, others a workgroup per column with
+// LDS chunks of at most chunk_cap rows
+constexpr int32_t kAsmWaveRowsMax = 4096;
 void launch_assemble(hipStream_t st, const DevPtrs &d, const AsmTask *tasks, int64_t ntasks, const ChildCol *cc,
-                     int32_t fmax);
+                     int32_t fmax, int32_t chunk_cap, int32_t wave_rows);
 void launch_potrf(hipStream_t st, const DevPtrs &d, const PotrfItem *it, int64_t n);
 void launch_trsm(hipStream_t st, const DevPtrs &d, const TrsmItem *it, int64_t n);
 // n items; the reduced-precision kernels run the last n_red of them, one

@@ -260,6 +260,7 @@ struct smfgpu_plan_s {
     DBuf<double> d_gpart;                          // LDL^T: per-workgroup column maxima of a group
     int32_t syrk_ahead = 2;                        // Schur update: chunks with global loads in flight (1 or 2)
     int32_t asm_chunk = 2048;                      // SPD assembly: largest LDS column chunk (rows)
+    int32_t asm_wave_rows = 2048;                  // SPD assembly: levels of fronts up to this many rows take a wavefront per column
     int32_t solve_fuse_rows = 512;                 // rows below a last block its own workgroup updates
     double t_solve_phase[4] = {0, 0, 0, 0};        // fwd ms, bwd ms, fwd launches, bwd launches
     // refinement CSR (full symmetric, permuted numbering)

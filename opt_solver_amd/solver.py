@@ -285,10 +285,12 @@ class Factor:
     def solve(self, b, job=SOLVE_FULL):
         """job: SOLVE_FULL, SOLVE_FORWARD (result in the permuted ordering),
         SOLVE_DIAG (permuted in and out), SOLVE_BACKWARD and SOLVE_DIAG_BACK
-        (permuted in, original out), as smf's SolveJob."""
+        (permuted in, original out), as smf's SolveJob. b is n x nrhs, or
+        ldb x nrhs with ldb > n (rows [n, ldb) are left as they are)."""
         b = np.array(b, dtype=np.float64, order="F", copy=True)
         nrhs = 1 if b.ndim == 1 else b.shape[1]
-        check(lib().smfgpu_solve(self.h, _p(b), nrhs, self.plan.n, int(job), MEM_HOST), "solve")
+        ldb = b.shape[0]
+        check(lib().smfgpu_solve(self.h, _p(b), nrhs, ldb, int(job), MEM_HOST), "solve")
         return b
 
     def precisions(self):

@@ -78,3 +78,43 @@ def test_solve_job_out_of_range():
     fac = plan.factor(A[3])
     with pytest.raises(SmfGpuError):
         fac.solve(np.ones(A[0]), 5)
+
+
+@pytest.mark.parametrize("case", ["ocp_T60", "lap3_12"])
+def test_leading_dimension(case):
+    """b with ldb > n (solve.cpp's multi-RHS layout): every job returns the
+    same columns as with ldb = n and leaves rows [n, ldb) untouched, for host
+    arrays, device arrays and the solve that allocates nothing."""
+    import ctypes
+
+    import torch
+    from opt_solver_amd._lib import lib
+    from opt_solver_amd.solver import analyse
+    print("case ldb", case, flush=True)
+    posdef = case.startswith("lap3")
+    A = g.laplace7_3d(12) if posdef else g.ocp_kkt(60, seed=3)
+    n, nrhs, ldb = A[0], 3, A[0] + 37
+    plan = Plan(analyse(A, nemin=16).arrays(), posdef=posdef)
+    fac = plan.factor(A[3], u=0.01)
+    assert fac.status == 0
+    b = np.random.default_rng(9).uniform(-1, 1, (n, nrhs))
+    pad = np.full((ldb, nrhs), 7.25)
+    pad[:n] = b
+    for job in (SOLVE_FULL, SOLVE_FORWARD, SOLVE_DIAG, SOLVE_BACKWARD, SOLVE_DIAG_BACK):
+        ref = fac.solve(b, job)
+        out = fac.solve(pad, job)
+        assert np.all(out[n:] == 7.25), job
+        assert _rel(out[:n], ref) <= 1e-13, job
+        bd = torch.from_numpy(pad.reshape(-1, order="F").copy()).cuda()
+        torch.cuda.synchronize()
+        fac.solve_device(bd.data_ptr(), nrhs, ldb=ldb, job=job)
+        od = bd.cpu().numpy().reshape((ldb, nrhs), order="F")
+        assert np.all(od[n:] == 7.25), job
+        assert _rel(od[:n], ref) <= 1e-13, job
+        bd = torch.from_numpy(pad.reshape(-1, order="F").copy()).cuda()
+        torch.cuda.synchronize()
+        rc = lib().smfgpu_solve_noalloc(fac.h, ctypes.c_void_p(bd.data_ptr()), nrhs, ldb, int(job))
+        assert rc == 0, lib().smfgpu_error_string()
+        od = bd.cpu().numpy().reshape((ldb, nrhs), order="F")
+        assert np.all(od[n:] == 7.25), job
+        assert _rel(od[:n], ref) <= 1e-13, job
