@@ -405,10 +405,11 @@ def autotune_workloads(args):
     perm = metis_perm(f"spd27_{args.m}") if args.ordering == "metis" else None
     if perm is None:
         perm = o.grid_nd((args.m, args.m, args.m), leaf=64)
-    wl["spd"] = autotune.workload(A, analyse(A, perm=perm, nemin=32).arrays(), True)
+    # the right-hand side of the timed legs (c4_measure: x from default_rng(0))
+    wl["spd"] = autotune.workload(A, analyse(A, perm=perm, nemin=32).arrays(), True, seed=0)
     gsz, m = 300, 30000
     K, _ = g.kkt_grid(gsz, m, seed=1)
-    wl["kkt"] = autotune.workload(K, analyse(K, perm=_kkt_perm(gsz, m, K), nemin=32).arrays(), False)
+    wl["kkt"] = autotune.workload(K, analyse(K, perm=_kkt_perm(gsz, m, K), nemin=32).arrays(), False, seed=0)
     mats, _ = batch_systems(1024)
     U, _ = batch.union_arrays([analyse(Ai, nemin=16).arrays() for Ai in mats])
     wl["batch"] = autotune.workload((int(U["n"]), None, None, np.concatenate([Ai[3] for Ai in mats])), U, False)
