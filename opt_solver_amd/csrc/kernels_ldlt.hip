@@ -2095,8 +2095,9 @@ __global__ __launch_bounds__(kLT) void k_ldlt_assemble(LdltDev d, const AsmTask 
             const int lo = (r0 == c || !tracked) ? lbound(cm, ch.jj, q, r0) : cursor[e - t.cc0];
             const int hi = lbound(cm, lo, min(q, lo + (r1 - r0)), r1);
             for (int ii = lo + tid; ii < hi; ii += kLT) buf[cm[ii] - r0] += cbcol[ii];
-            if (tracked && tid == 0) cursor[e - t.cc0] = hi;
+            // every thread has read this child's cursor before the barrier
             __syncthreads();
+            if (tracked && tid == 0) cursor[e - t.cc0] = hi;
         }
         for (int r = r0 + tid; r < r1; r += kLT) colp[r] = buf[r - r0];
         __syncthreads();
