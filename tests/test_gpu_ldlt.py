@@ -402,3 +402,34 @@ def test_work_limit(oracle_lib):
         assert st.info.front_work > lim
         with pytest.raises(SmfGpuError):
             st.solve(np.ones(A[0]))
+
+
+def test_multichunk_assembly_against_oracle(oracle_lib):
+    """Fronts of more than 2048 rows (the LDL^T assembly's LDS chunk): every
+    child column crosses several chunks, each starting at the previous
+    chunk's end. Against the oracle, and three refactorizations bit for bit
+    equal to each other (an assembly that lost a chunk's entries now and
+    then would show here)."""
+    print("case kkt_110 multi-chunk assembly", flush=True)
+    A, _ = g.kkt_grid(110, 2400, seed=110)
+    from oracle import refshim
+    arrays = _arrays_for(A, refshim.ORDER_METIS if refshim.available() else None)
+    assert int(np.diff(arrays["front_row_ptr"]).max()) > 2048
+    plan = Plan(arrays, posdef=False)
+    fac = plan.factor(A[3], u=0.01)
+    orc = oracle_lib.Factor(arrays, A[3], posdef=False)
+    assert fac.status == orc.status == 0
+    assert fac.inertia == orc.inertia == (110 * 110, 2400, 0)
+    assert fac.info.delayed_pivots == orc.delayed and fac.info.two_by_two_pivots == orc.two_by_two
+    e0 = fac.export()
+    err = compare_factor(e0, orc.export(), 1e-12)
+    for _ in range(3):
+        compare_factor(plan.factor(A[3], u=0.01).export(), e0, 0.0)
+    x = np.random.default_rng(2).uniform(-1, 1, A[0])
+    b = matvec(A, x)
+    f2 = plan.factor(A[3], u=0.01)
+    xs = f2.solve(b)
+    xr, _ = f2.refine(b, xs, max_iters=5, tol=1e-15)
+    eta, _ = oracle_lib.backward_error(A, b, xr)
+    print(f"  max front {fac.info.max_front} L err {err:.1e} refined eta {eta:.1e}")
+    assert eta <= 1e-14
