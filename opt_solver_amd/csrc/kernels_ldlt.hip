@@ -2045,7 +2045,6 @@ void launch_big_finish(hipStream_t st, const LdltDev &d, const BigLd *states, in
 // child's CB column in child order; the CB of a child is the trailing
 // (f_c - p_c) square of its own front.
 constexpr int kAsmChunkL = 2048;
-constexpr int kAsmCurL = 64;  // children per column whose chunk cursors are kept in LDS (others search)
 
 __device__ __forceinline__ int lbound(const int32_t *a, int lo, int hi, int32_t key) {
     while (lo < hi) {
@@ -2059,7 +2058,6 @@ __device__ __forceinline__ int lbound(const int32_t *a, int lo, int hi, int32_t 
 __global__ __launch_bounds__(kLT) void k_ldlt_assemble(LdltDev d, const AsmTask *__restrict__ tasks,
                                                       const ChildCol *__restrict__ cc) {
     __shared__ double buf[kAsmChunkL];
-    __shared__ int cursor[kAsmCurL];
     const AsmTask t = tasks[blockIdx.x];
     const FrontLd F = d.fronts[t.s];
     const int c = t.c;
@@ -2087,17 +2085,10 @@ __global__ __launch_bounds__(kLT) void k_ldlt_assemble(LdltDev d, const AsmTask 
             const int q = C.f - C.p;
             const int32_t *cm = d.cmap + C.cmofs;
             const double *cbcol = d.arena + C.fofs + static_cast<int64_t>(C.p + ch.jj) * C.f + C.p;
-            // the child column's rows map to increasing parent rows: this
-            // chunk's entries start where the previous chunk's ended (a
-            // cursor; a search on the first chunk) and number at most
-            // r1 - r0, which bounds the search for their end
-            const bool tracked = e - t.cc0 < kAsmCurL;
-            const int lo = (r0 == c || !tracked) ? lbound(cm, ch.jj, q, r0) : cursor[e - t.cc0];
-            const int hi = lbound(cm, lo, min(q, lo + (r1 - r0)), r1);
+            const int lo = lbound(cm, ch.jj, q, r0);
+            const int hi = lbound(cm, lo, q, r1);
             for (int ii = lo + tid; ii < hi; ii += kLT) buf[cm[ii] - r0] += cbcol[ii];
-            // every thread has read this child's cursor before the barrier
             __syncthreads();
-            if (tracked && tid == 0) cursor[e - t.cc0] = hi;
         }
         for (int r = r0 + tid; r < r1; r += kLT) colp[r] = buf[r - r0];
         __syncthreads();
