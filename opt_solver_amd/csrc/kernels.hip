@@ -596,7 +596,8 @@ void launch_potrf(hipStream_t st, const DevPtrs &d, const PotrfItem *it, int64_t
 // Wave w: rows 16 w .. 16 w + 15, four 16-column blocks; block cb needs
 // k-steps t < 16 cb + 16 only (M is upper triangular).
 __global__ __launch_bounds__(kThreads) void k_trsm(DevPtrs d, const TrsmItem *__restrict__ items) {
-    __shared__ double As[kNb][kNb + 2];
+    // M in LDS (every wave reads all of it); each wave's 16 rows of A, the
+    // MFMA B operand, in registers: 34 KB of LDS a workgroup, four per CU
     __shared__ double Ms[kNb][kNb + 2];
     const TrsmItem it = items[blockIdx.x];
     const FrontDev F = d.fronts[it.s];
@@ -604,52 +605,52 @@ __global__ __launch_bounds__(kThreads) void k_trsm(DevPtrs d, const TrsmItem *__
     double *base = d.L + F.lofs;
     const int tid = threadIdx.x;
     const int nr = min(kNb, f - r0);
+    const int lane = tid & 63, w = tid >> 6;
+    const int lr = lane & 15, lk = lane >> 4;
+    const int rr = 16 * w + lr;
+    double bv[16];
     {
-        const int rr = tid & 63, cg = tid >> 6;
-        double av[16], gv[16];
+        const int rc = r0 + min(rr, nr - 1);
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            const int t = min(cg + 4 * i, kb - 1);
-            av[i] = base[static_cast<int64_t>(k + t) * f + r0 + min(rr, nr - 1)];
-            gv[i] = base[static_cast<int64_t>(k + t) * f + k + min(rr, kb - 1)];
-        }
+        for (int kk = 0; kk < 16; ++kk) bv[kk] = base[static_cast<int64_t>(k + min(4 * kk + lk, kb - 1)) * f + rc];
+        const int mr = tid & 63, cg = tid >> 6;
+        double gv[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) gv[i] = base[static_cast<int64_t>(k + min(cg + 4 * i, kb - 1)) * f + k + min(mr, kb - 1)];
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
             const int t = cg + 4 * i;
-            As[rr][t] = masked(av[i], rr < nr && t < kb);
-            // Ms[t = rr][c]: strict upper M, diagonal 1 / L_cc
+            // Ms[t = mr][c]: strict upper M, diagonal 1 / L_cc
             const double g = gv[i];
-            const bool in = rr < kb && t < kb;
-            Ms[rr][t] = (in && rr < t) ? g : ((in && rr == t) ? 1.0 / g : 0.0);
+            const bool in = mr < kb && t < kb;
+            Ms[mr][t] = (in && mr < t) ? g : ((in && mr == t) ? 1.0 / g : 0.0);
         }
+#pragma unroll
+        for (int kk = 0; kk < 16; ++kk) bv[kk] = masked(bv[kk], rr < nr && 4 * kk + lk < kb);
     }
     __syncthreads();
-    const int lane = tid & 63, w = tid >> 6;
-    const int lr = lane & 15, lk = lane >> 4;
     v4d acc[4];
 #pragma unroll
     for (int cb = 0; cb < 4; ++cb) acc[cb] = v4d{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int kk = 0; kk < 16; ++kk) {
         const int t = 4 * kk + lk;
-        const double bop = As[16 * w + lr][t];
 #pragma unroll
         for (int cb = 0; cb < 4; ++cb)
-            if (kk < 4 * cb + 4) acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(Ms[t][16 * cb + lr], bop, acc[cb], 0, 0, 0);
+            if (kk < 4 * cb + 4) acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(Ms[t][16 * cb + lr], bv[kk], acc[cb], 0, 0, 0);
     }
-    const int rr = 16 * w + lr;
     // fronts with reduced-precision Schur updates also get the panel in
     // the precision their tiles read: FP32 (half the bytes), or bf16 in the
     // packed layout (high part, and low part for split BF16)
     const int pc = d.prec != nullptr ? d.prec[it.s] : 0;
     float *b32 = (d.L32 != nullptr && pc == 1) ? d.L32 + F.lofs : nullptr;
     if ((pc == 2 || pc == 3) && d.L16h != nullptr) {
-        __syncthreads();  // every wave is done with As
+        __syncthreads();  // every wave is done with M: its LDS stages the result
         if (rr < nr) {
 #pragma unroll
             for (int cb = 0; cb < 4; ++cb)
 #pragma unroll
-                for (int v = 0; v < 4; ++v) As[rr][16 * cb + 4 * v + lk] = acc[cb][v];
+                for (int v = 0; v < 4; ++v) Ms[rr][16 * cb + 4 * v + lk] = acc[cb][v];
         }
         __syncthreads();
         // 8-column groups of one row as 16-byte vectors; columns past kb
@@ -661,7 +662,7 @@ __global__ __launch_bounds__(kThreads) void k_trsm(DevPtrs d, const TrsmItem *__
             v8bf_t hi, lo;
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
-                const float x = 8 * g + u < kb ? static_cast<float>(As[row][8 * g + u]) : 0.f;
+                const float x = 8 * g + u < kb ? static_cast<float>(Ms[row][8 * g + u]) : 0.f;
                 hi[u] = static_cast<__bf16>(x);
                 lo[u] = static_cast<__bf16>(x - static_cast<float>(hi[u]));
             }
